@@ -1,0 +1,264 @@
+"""Benchmark: full particle-flow-map steps at 3D 256^3, 8 particles/cell.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--res 256] [--no-cpu-baseline]
+
+One "step" is one pass of the reference schedule (3D/run_3D.py:716-831): CFL
+dt, reinitialisations as scheduled (every 20 steps reseed + G2P, every step
+the short G2P), midpoint velocity (grid RK4, advection, projection), particle
+RK4, binning, P2G and the final projection (MGPCG to a 1e-6 relative
+residual).  The workload is the 3D 256^3 full step of the mixed-precision
+config (134,217,728 particles), synthetic: two coaxial vortex rings as the
+initial vorticity, projected.  All state is resident in HBM before timing.
+
+Prints ONE JSON line: particle-steps/s as `value`, plus MGPCG cells/s, a
+per-phase breakdown, the roofline of the dominant kernel (HIP events on the
+stream the kernel runs on) and a CPU baseline (the oracle, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "particle-flow-maps_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def rings_3d():
+    """Config: two coaxial rings, major radius 0.15 / 0.2 at z = 0.3 / 0.4,
+    core 0.02, unit circulation (add_vortex_ring, 3D/init_conditions.py:10-22)."""
+    return [
+        dict(w=1.0, rad=0.15, delta=0.02, c=[0.5, 0.5, 0.3], unit_x=[1.0, 0, 0], unit_y=[0, 1.0, 0]),
+        dict(w=1.0, rad=0.20, delta=0.02, c=[0.5, 0.5, 0.4], unit_x=[1.0, 0, 0], unit_y=[0, 1.0, 0]),
+    ]
+
+
+class EventTimer:
+    """Accumulates GPU time of named regions with HIP events recorded on the
+    stream the work is launched on (the current torch stream)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.pending = []
+        self.total = {}
+        self.count = {}
+
+    def region(self, name):
+        timer = self
+
+        class _R:
+            def __enter__(self_inner):
+                self_inner.a = timer.torch.cuda.Event(enable_timing=True)
+                self_inner.a.record(timer.torch.cuda.current_stream())
+
+            def __exit__(self_inner, *exc):
+                b = timer.torch.cuda.Event(enable_timing=True)
+                b.record(timer.torch.cuda.current_stream())
+                timer.pending.append((name, self_inner.a, b))
+
+        return _R()
+
+    def flush(self):
+        self.torch.cuda.synchronize()
+        for name, a, b in self.pending:
+            self.total[name] = self.total.get(name, 0.0) + a.elapsed_time(b)
+            self.count[name] = self.count.get(name, 0) + 1
+        self.pending = []
+
+
+def instrumented_step(sim, timer):
+    """sim.step() with event regions around each phase (same work, same order)."""
+    from pfm import core
+
+    sim.i += 1
+    i = sim.i
+    cfg = sim.cfg
+    with timer.region("max_speed"):
+        dt = sim.compute_dt()
+    sim.last_dt = dt
+    if i % cfg.reinit_every == 0:
+        with timer.region("reinit_long"):
+            if cfg.reinit_particle_pos or i == 0:
+                sim._seed()
+                sim.ik = i
+            core.g2p(sim.res, sim.dx, sim.pos, sim.u, sim.init_imp, None)
+            core.reset_T_identity(sim.T_init)
+            core.reset_T_identity(sim.T_grad_m)
+    if (i - sim.ik) % cfg.reinit_every_grad_m == 0:
+        with timer.region("g2p_grad_m"):
+            core.g2p(sim.res, sim.dx, sim.pos, sim.u, sim.imp_grad_m, sim.init_C)
+        with timer.region("update_T"):
+            core.update_T(sim.T_grad_m, sim.T_init, sim.T_init)
+            core.reset_T_identity(sim.T_grad_m)
+    with timer.region("midpoint_advect"):
+        for a in range(sim.dim):
+            core.reset_identity_grid(sim.res, sim.dx, sim.psi_grid[a], sim.T_grid[a])
+        for a in range(sim.dim):
+            core.rk4_grid(sim.res, sim.dx, sim.psi_grid[a], sim.T_grid[a], sim.u, 0.5 * dt)
+        for a in range(sim.dim):
+            sim.p2g_weight[a].copy_(sim.u[a])
+        core.advect_u(sim.res, sim.dx, sim.p2g_weight, sim.u, sim.T_grid, sim.psi_grid)
+    with timer.region("poisson"):
+        sim.project()
+    with timer.region("rk4_particles"):
+        core.rk4_particles(sim.res, sim.dx, sim.pos, sim.T_grad_m, sim.u, dt, True)
+    with timer.region("binning"):
+        sim.binning.bin(sim.pos, sim.dx, force=True)
+    with timer.region("p2g"):
+        sim.binning.p2g(sim.dx, sim.pos, sim.init_imp, sim.init_C, sim.u, W=sim.p2g_weight,
+                        Tgm=sim.T_grad_m, Tin=sim.T_init)
+    with timer.region("poisson"):
+        sim.project()
+    return dt
+
+
+def cpu_baseline(seconds_budget=15.0):
+    """The CPU oracle (C/OpenMP fp32 restatement) on a bounded sample of the
+    same workload: full 3D steps at 32^3 x 8 ppc with the same schedule."""
+    import numpy as np
+
+    sys.path.insert(0, ROOT)
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    import oracle
+
+    res = (32, 32, 32)
+    osim = oracle.OracleSim(res, reinit_every=20, reinit_every_grad_m=1, particles_per_cell=8, tol=1e-6,
+                            tol_mode=1, seed=1)
+    vc = np.zeros((3,) + res, np.float32)
+    for r in rings_3d():
+        oracle.vortex_ring(res, 1 / 32, vc, r["w"], r["rad"], r["delta"], r["c"], r["unit_x"], r["unit_y"], 2000)
+    osim.set_velocity_from_central(vc)
+    osim.step()  # warm
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        osim.step()
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds_budget or steps >= 20:
+            break
+    omp = int(os.environ.get("OMP_NUM_THREADS", cores))
+    return {"value": osim.n * steps / el, "unit": "particle-steps/s", "cores": omp, "kind": "port",
+            "sample": f"oracle (C/OpenMP fp32 restatement of the reference step) on 3D 32^3 x 8 ppc "
+                      f"({osim.n} particles), {steps} full steps after 1 warm-up step, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--res", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=2, help="instrumented steps after the timed ones")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the step shards by slab "
+              "decomposition but this build measures a single MI355X.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from pfm.simulation import Simulation, StepConfig
+
+    res = (args.res,) * 3
+    cfg = StepConfig(particles_per_cell=8, reinit_every=20, reinit_every_grad_m=1, CFL=0.5, tol=1e-6,
+                     tol_mode="rel", seed=1, particle_order=1, write_imp_grad_m=False, visualize_dt=1e9)
+    t_setup = time.perf_counter()
+    sim = Simulation(res, cfg)
+    sim.init_rings_3d(rings_3d())
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    print(f"# setup {t_setup:.1f}s, {sim.n} particles, {torch.cuda.memory_allocated() / 2**30:.1f} GiB",
+          file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        sim.step()
+    torch.cuda.synchronize()
+    iters_before = len(sim.poisson_iters)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sim.step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_per_step = el / args.steps * 1e3
+    value = sim.n * args.steps / el
+    its = sim.poisson_iters[iters_before:]
+    print(f"# timed {args.steps} steps: {ms_per_step:.1f} ms/step, poisson iters {its}", file=sys.stderr,
+          flush=True)
+
+    # per-phase breakdown (instrumented steps, not part of `value`)
+    timer = EventTimer(torch)
+    for _ in range(args.profile_steps):
+        instrumented_step(sim, timer)
+    timer.flush()
+    phases = {k: round(v / timer.count[k] * (2 if k == "poisson" else 1), 3) for k, v in timer.total.items()}
+
+    # standalone Poisson solve at the step's tolerance (MGPCG cells/s)
+    from pfm import core
+
+    u_save = [x.clone() for x in sim.u]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    reps = 3
+    pits = []
+    for _ in range(reps):
+        for a in range(3):
+            sim.u[a].copy_(u_save[a])
+        pits.append(sim.project())
+    torch.cuda.synchronize()
+    t_poisson = (time.perf_counter() - t1) / reps
+    cells = math.prod(res)
+
+    # roofline of the particle RK4 kernel: 96 B/particle compulsory traffic
+    ev_a = torch.cuda.Event(enable_timing=True)
+    ev_b = torch.cuda.Event(enable_timing=True)
+    nrep = 3
+    ev_a.record(torch.cuda.current_stream())
+    for _ in range(nrep):
+        core.rk4_particles(sim.res, sim.dx, sim.pos, sim.T_grad_m, sim.u, 1e-4, True)
+    ev_b.record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    rk4_ms = ev_a.elapsed_time(ev_b) / nrep
+    alg_bytes = 96.0 * sim.n
+    achieved = alg_bytes / (rk4_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: two coaxial vortex rings (Biot-Savart, projected), 8 particles/cell seed 1",
+        "config": {"workload": f"3D {args.res}^3 full particle-flow-map step, 8 particles/cell",
+                   "particles": sim.n, "cells": cells, "reinit_every": 20, "reinit_every_grad_m": 1,
+                   "CFL": 0.5, "mgpcg_tol": "1e-6 relative", "parallelism": "single GPU"},
+        "mgpcg_cells_per_s": cells / t_poisson,
+        "mgpcg_ms_per_solve": t_poisson * 1e3,
+        "mgpcg_iters": pits,
+        "phase_ms": phases,
+        "roofline": {"kernel": "k_rk4_particles<3>", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel_ms": rk4_ms, "alg_bytes_per_launch": alg_bytes},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # reported, never fatal for the GPU number
+            out["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,242 @@
+/*
+ * pfm.h -- C ABI of libpfm.so, the MI355X (gfx950) particle-flow-map step.
+ *
+ * Every entry point replaces one Taichi kernel (or solver method) of the
+ * reference zjw49246/particle-flow-maps; the reference file:line is given
+ * beside each declaration.  All entry points are asynchronous on the given
+ * HIP stream (pass NULL for the default stream) and return PFM_OK (0) or a
+ * negative error code; pfm_last_error() returns a thread-local description.
+ *
+ * Memory: every pointer argument is a device pointer owned by the caller
+ * (e.g. a torch tensor's data_ptr()).  Layouts:
+ *   - grid fields are row-major with the last axis fastest (the numpy view of
+ *     the reference's fields): a 3D x-face field has shape (rx+1, ry, rz) and
+ *     (i,j,k) lives at (i*ry + j)*rz + k; 2D fields are (nx, ny), j fastest;
+ *   - particle vector fields are SoA (d, n): component c of particle p at
+ *     c*n + p; one column/row vector per argument (T_x, T_y, T_z / C_x ...);
+ *   - grid-attached vector fields (psi_x_grid, T_x_grid ...) are SoA
+ *     (d, faces of that axis).
+ * The library allocates device memory only in pfm_mg_create (the multigrid
+ * hierarchy); step entry points take caller workspaces and never allocate.
+ */
+#ifndef PFM_H
+#define PFM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFM_ABI_VERSION 1
+
+#define PFM_OK 0
+#define PFM_ERR_ARG (-1)      /* bad shape, null pointer, unsupported option */
+#define PFM_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define PFM_ERR_OOM (-3)      /* device allocation failed (pfm_mg_create only) */
+#define PFM_WARN_NOT_CONVERGED 1 /* MGPCG hit max_iters (the reference prints and returns) */
+
+typedef void* pfm_stream_t; /* hipStream_t */
+
+/* Grid descriptor: the reference's module globals res_x/res_y/res_z and
+ * dx = 1/res_y (3D/run_3D.py:16, 2D/run_2D.py:14).  res[2] is ignored in 2D. */
+typedef struct {
+    int32_t dim;     /* 2 or 3 */
+    int32_t res[3];  /* cells per axis, each >= 4 */
+    float dx;        /* cell size */
+} pfm_grid;
+
+int pfm_abi_version(void);
+const char* pfm_last_error(void);
+
+/* ---- flow-map advection ------------------------------------------------ */
+
+/* RK4_T_forward: 3D/run_3D.py:175-219 (via stretch_T_and_advect_particles
+ * :336-337); 2D/run_2D.py:304-341 (advect_psi flag).  pos (d,n) and the d
+ * column vectors T_x..T_z (each (d,n)) are updated in place; u_* are the face
+ * velocities u_x0..u_z0.  T_z/u_z are ignored in 2D. */
+int pfm_rk4_particles(const pfm_grid* g, int64_t n, float* pos, float* T_x, float* T_y,
+                      float* T_z, const float* u_x, const float* u_y, const float* u_z,
+                      float dt, int advect_psi, pfm_stream_t stream);
+
+/* RK4_grid: 3D/run_3D.py:144-173, 2D/run_2D.py:233-261.  Backward RK4 of the
+ * faces of `axis`: psi (d, F_axis) and T (d, F_axis) in place. */
+int pfm_rk4_grid(const pfm_grid* g, int axis, float* psi, float* T, const float* u_x,
+                 const float* u_y, const float* u_z, float dt, pfm_stream_t stream);
+
+/* advect_u: 3D/run_3D.py:296-312, 2D/run_2D.py:429-441.  For each axis a:
+ * u1_a[f] = T_a[f] . u0(psi_a[f]).  u1_* must not alias u0_*. */
+int pfm_advect_u(const pfm_grid* g, const float* u0_x, const float* u0_y, const float* u0_z,
+                 float* u1_x, float* u1_y, float* u1_z, const float* T_x, const float* T_y,
+                 const float* T_z, const float* psi_x, const float* psi_y, const float* psi_z,
+                 pfm_stream_t stream);
+
+/* reset_to_identity_grid: 3D/run_3D.py:106-120 with the face coordinates of
+ * x/y/z_coords_func (3D/taichi_utils.py:33-46); 2D/run_2D.py:120-129. */
+int pfm_reset_identity_grid(const pfm_grid* g, int axis, float* psi, float* T,
+                            pfm_stream_t stream);
+
+/* reset_T_to_identity: 3D/run_3D.py:133-140, 2D/run_2D.py:141-146. */
+int pfm_reset_T_identity(int dim, int64_t n, float* T_x, float* T_y, float* T_z,
+                         pfm_stream_t stream);
+
+/* ---- particle <-> grid transfer ---------------------------------------- */
+
+/* init_particles_pos_uniform: 3D/init_conditions.py:238-252,
+ * 2D/init_conditions.py:139-166 (lattice, reference particle order).
+ * seed >= 0 adds seeded per-particle jitter inside each lattice sub-cell
+ * (extension; seed < 0 gives the reference lattice bit-exactly).  order 0 is
+ * the reference order (x fastest); order 1 numbers particles cell by cell
+ * (cells z fastest, ppc_axis^d per cell) for coalesced transfers. */
+int pfm_seed_particles(const pfm_grid* g, int ppc_axis, float dist, int64_t n, float* pos,
+                       int64_t seed, int order, pfm_stream_t stream);
+
+/* init_particles_imp (3D/init_conditions.py:254-263, 2D :183-191) when C_x is
+ * NULL; init_particles_imp_grad_m (3D :265-273, 2D :203-210) otherwise: imp
+ * (d,n) = u(pos) and rows C_a (d,n) = grad u_a(pos).  imp may be NULL when C_x
+ * is given (the 3D driver never reads particles_init_imp_grad_m). */
+int pfm_g2p(const pfm_grid* g, int64_t n, const float* pos, const float* u_x, const float* u_y,
+            const float* u_z, float* imp, float* C_x, float* C_y, float* C_z,
+            pfm_stream_t stream);
+
+/* update_T: 3D/run_3D.py:430-439 (out = in_init, in place) and
+ * 2D/run_2D.py:538-546 (out = T): out = cols(T_gm) @ cols(T_in). */
+int pfm_update_T(int dim, int64_t n, const float* Tgm_x, const float* Tgm_y, const float* Tgm_z,
+                 const float* Tin_x, const float* Tin_y, const float* Tin_z, float* Tout_x,
+                 float* Tout_y, float* Tout_z, pfm_stream_t stream);
+
+/* update_particles_imp: 2D/run_2D.py:517-524: imp = cols(T) @ init_imp. */
+int pfm_update_particles_imp(int dim, int64_t n, const float* T_x, const float* T_y,
+                             const float* T_z, const float* init_imp, float* imp,
+                             pfm_stream_t stream);
+
+/* update_particles_grad_m: 2D/run_2D.py:526-536: rows of
+ * cols(T_gm) @ (rows(init_C) @ rows(T_gm)) into C_x..C_z. */
+int pfm_update_particles_grad_m(int dim, int64_t n, float* C_x, float* C_y, float* C_z,
+                                const float* iC_x, const float* iC_y, const float* iC_z,
+                                const float* Tgm_x, const float* Tgm_y, const float* Tgm_z,
+                                pfm_stream_t stream);
+
+/* Workspace bytes needed by pfm_bin_particles and pfm_p2g for n particles. */
+size_t pfm_p2g_workspace_size(const pfm_grid* g, int64_t n);
+
+/* get_particles_id_in_every_cell: 3D/run_3D.py:342-351, as a deterministic
+ * stable counting sort.  cell_of (n, int32, nullable) = flat cell
+ * (i*ry + j)*rz + k of int(pos/dx) clamped to the grid; cell_start (C+1)
+ * exclusive offsets; perm (n) = particle ids sorted by cell, ascending id
+ * inside a cell (= numpy.argsort(cell_of, kind='stable')). */
+int pfm_bin_particles(const pfm_grid* g, int64_t n, const float* pos, int32_t* cell_of,
+                      int32_t* cell_start, int32_t* perm, void* workspace, size_t ws_bytes,
+                      pfm_stream_t stream);
+
+/* The reference's output layout of get_particles_id_in_every_cell: ids
+ * (C x cell_max, -1 filled, lowest ids first) and counts (C, full count). */
+int pfm_cell_lists(const pfm_grid* g, const int32_t* cell_start, const int32_t* perm,
+                   int32_t cell_max, int32_t* ids, int32_t* counts, pfm_stream_t stream);
+
+/* P2G: 3D/run_3D.py:441-552 (Tgm_x != NULL: imp = cols(T_gm)@cols(T_init)@
+ * init_imp and C = cols(T_gm)@rows(init_C)@rows(T_gm) are formed per particle,
+ * with `imp` = init_imp and C_* = init_C_*), 2D/run_2D.py:548-611 (Tgm_x ==
+ * NULL: `imp` and C_* are used as given).  Deterministic gather over the
+ * cell-sorted particles of pfm_bin_particles (cell_start, perm must describe
+ * pos).  Writes u_* and, when non-NULL, the weights W_* (p2g_weight_*). */
+int pfm_p2g(const pfm_grid* g, int64_t n, const float* pos, const float* imp, const float* C_x,
+            const float* C_y, const float* C_z, const float* Tgm_x, const float* Tgm_y,
+            const float* Tgm_z, const float* Tin_x, const float* Tin_y, const float* Tin_z,
+            const int32_t* cell_start, const int32_t* perm, float* u_x, float* u_y, float* u_z,
+            float* W_x, float* W_y, float* W_z, void* workspace, size_t ws_bytes,
+            pfm_stream_t stream);
+
+/* ---- grid utilities ---------------------------------------------------- */
+
+/* calc_max_speed: 3D/run_3D.py:95-103, 2D/run_2D.py:101-108.  Writes the max
+ * cell-centre speed (floored at 1e-3) to out[0] (device float). */
+int pfm_max_speed(const pfm_grid* g, const float* u_x, const float* u_y, const float* u_z,
+                  float* out, pfm_stream_t stream);
+
+/* split_central_vector: 3D/taichi_utils.py:87-100, 2D :56-65.  vc (d, cells). */
+int pfm_split_central(const pfm_grid* g, const float* vc, float* u_x, float* u_y, float* u_z,
+                      pfm_stream_t stream);
+
+/* get_central_vector: 3D/taichi_utils.py:80-85, 2D :50-54. */
+int pfm_get_central(const pfm_grid* g, const float* u_x, const float* u_y, const float* u_z,
+                    float* vc, pfm_stream_t stream);
+
+/* leapfrog_vel_func generalised to nv point vortices (2D/init_conditions.py:
+ * 5-9,29-60): vc (2, cells) = sum_q w_q * angular_vel_func(r, rad, strength)
+ * * (-y, x).  cx, cy, w are host arrays of nv floats (nv <= 4096). */
+int pfm_point_vortices(const pfm_grid* g, int nv, const float* cx, const float* cy,
+                       const float* w, float rad, float strength, float* vc,
+                       pfm_stream_t stream);
+
+/* add_vortex_ring: 3D/init_conditions.py:10-22.  Adds the smoothed
+ * Biot-Savart velocity of one ring to vc (3, cells).  c, ex, ey: host float[3]. */
+int pfm_vortex_ring(const pfm_grid* g, float w, float rad, float delta, const float* c,
+                    const float* ex, const float* ey, int num_samples, float* vc,
+                    pfm_stream_t stream);
+
+/* Kinetic energy (2D/run_2D.py:765-770, commented there): sum_a sum u_a^2 /
+ * |F_a| accumulated in double into out[0].  `out` is a device buffer of
+ * 1 + 3*1024 doubles; out[1..] holds the per-block partial sums. */
+int pfm_kinetic_energy(const pfm_grid* g, const float* u_x, const float* u_y, const float* u_z,
+                       double* out, pfm_stream_t stream);
+
+/* ---- MGPCG pressure solver --------------------------------------------- */
+
+/* MGPCG.__init__ + MGPCG_2/MGPCG_3.__init__: 3D/mgpcg.py:17-60,350-357,
+ * 2D/mgpcg.py.  bc[2*dim] = boundary types (1 Dirichlet, 2 Neumann) per axis
+ * (low, high).  precision 0: fp32 levels (the reference); 1: fp16-stored
+ * V-cycle levels (fp32 compute).  Allocates the hierarchy. */
+int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, int precision,
+                  void** handle);
+int pfm_mg_destroy(void* handle);
+int pfm_mg_n_levels(void* handle);
+
+/* Device pointer of an internal field (0 r, 1 z, 2 x, 3 p, 4 Ap, 5 u_div,
+ * 6 pressure) at `level` (r and z only), for tests and get_result. */
+void* pfm_mg_field(void* handle, int which, int level);
+
+/* MGPCG_3.apply_bc (3D/mgpcg.py:359-378), MGPCG_2.apply_bc (2D/mgpcg.py:
+ * 292-324). */
+int pfm_mg_apply_bc(void* handle, float* u_x, float* u_y, float* u_z, pfm_stream_t stream);
+
+/* divergence (3D/mgpcg.py:380-390) fused with init(u_div, k) (3D/mgpcg.py:
+ * 71-79): r = k * div u, z = Ap = p = x = 0. */
+int pfm_mg_divergence_init(void* handle, const float* u_x, const float* u_y, const float* u_z,
+                           float k, pfm_stream_t stream);
+
+/* init (3D/mgpcg.py:71-79) from an explicit right-hand side field. */
+int pfm_mg_init(void* handle, const float* rhs, float k, pfm_stream_t stream);
+
+/* solve (3D/mgpcg.py:202-280).  tol_mode 0: stop when rTr < tol (reference,
+ * absolute); 1: stop when sqrt(rTr/r0Tr0) < tol (relative extension).
+ * Synchronises the stream once per iteration to test convergence (the
+ * reference does three host reads per iteration).  Returns PFM_OK or
+ * PFM_WARN_NOT_CONVERGED; iters_out / rTr_out may be NULL. */
+int pfm_mg_solve(void* handle, int max_iters, double eps, double tol, int tol_mode,
+                 int* iters_out, double* rTr_out, pfm_stream_t stream);
+
+/* subtract_grad_p: 3D/mgpcg.py:392-418 using the solution x (get_result). */
+int pfm_mg_subtract_grad_p(void* handle, float* u_x, float* u_y, float* u_z,
+                           pfm_stream_t stream);
+
+/* Poisson: 3D/mgpcg.py:425-430 (solve_pressure_MGPCG :420-423): apply_bc,
+ * divergence, init(-1), solve, get_result, subtract_grad_p, apply_bc. */
+int pfm_mg_poisson(void* handle, float* u_x, float* u_y, float* u_z, int max_iters, double tol,
+                   int tol_mode, int* iters_out, double* rTr_out, pfm_stream_t stream);
+
+/* Building blocks exposed for parity tests: the V-cycle z0 = M^-1 r0
+ * (apply_preconditioner, 3D/mgpcg.py:182-200), one colour of RB-GS on a level
+ * (smooth, :165-172), restrict (:152-158), prolongate (:160-163) and the
+ * operator Ap = A p on level 0 (get_Ap, :124-129). */
+int pfm_mg_vcycle(void* handle, pfm_stream_t stream);
+int pfm_mg_smooth(void* handle, int level, int phase, pfm_stream_t stream);
+int pfm_mg_restrict(void* handle, int level, pfm_stream_t stream);
+int pfm_mg_prolongate(void* handle, int level, pfm_stream_t stream);
+int pfm_mg_apply_A(void* handle, const float* p, float* Ap, pfm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFM_H */

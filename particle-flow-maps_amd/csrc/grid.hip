@@ -1,0 +1,532 @@
+// grid.hip -- face-centred kernels of the flow-map step: backward RK4 of the
+// face flow map, semi-Lagrangian advection with the Jacobian, identity
+// resets, CFL max speed, centre<->face averaging, vortex initial conditions
+// and the kinetic-energy diagnostic.  One thread per face or cell.
+#include "pfm_host.h"
+
+namespace pfm {
+
+// Decode a flat face index of axis `a` into (i, j, k) (last axis fastest).
+template <int DIM>
+__device__ __forceinline__ void face_ijk(const GridDev& g, int a, int64_t f, int I[3]) {
+    int S[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) S[c] = g.res[c] + (c == a ? 1 : 0);
+    if (DIM == 3) {
+        I[2] = (int)(f % S[2]);
+        int64_t q = f / S[2];
+        I[1] = (int)(q % S[1]);
+        I[0] = (int)(q / S[1]);
+    } else {
+        I[2] = 0;
+        I[1] = (int)(f % S[1]);
+        I[0] = (int)(f / S[1]);
+    }
+}
+
+// RK4_grid: 3D/run_3D.py:144-173, 2D/run_2D.py:233-261.  dT = G @ T (rows form).
+template <int DIM>
+__global__ void __launch_bounds__(kBlock, 4) k_rk4_grid(GridDev g, int64_t nf,
+                                                     float* __restrict__ psi,
+                                                     float* __restrict__ T,
+                                                     const float* __restrict__ ux,
+                                                     const float* __restrict__ uy,
+                                                     const float* __restrict__ uz, float dt) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const float neg_dt = -1.0f * dt;
+    const float hs[3] = {0.5f * neg_dt, 0.5f * neg_dt, 1.0f * neg_dt};
+    const float ws[4] = {1.f, 2.f, 2.f, 1.f};
+    float p0[3] = {0.f, 0.f, 0.f}, T0[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        p0[c] = psi[c * nf + f];
+        T0[c] = T[c * nf + f];
+    }
+    float pk[3] = {p0[0], p0[1], p0[2]}, Tk[3] = {T0[0], T0[1], T0[2]};
+    float us[3] = {0.f, 0.f, 0.f}, dTs[3] = {0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int st = 0; st < 4; ++st) {
+        float u[3], G[3][3], dT[3];
+        interp_mac<DIM, true>(ux, uy, uz, g, pk, u, G);
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+            float acc = G[r][0] * Tk[0];
+#pragma unroll
+            for (int c = 1; c < DIM; ++c) acc += G[r][c] * Tk[c];
+            dT[r] = acc;
+        }
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            us[c] += ws[st] * u[c];
+            dTs[c] += ws[st] * dT[c];
+        }
+        if (st < 3) {
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) {
+                pk[c] = p0[c] + hs[st] * u[c];
+                Tk[c] = T0[c] + hs[st] * dT[c];
+            }
+        }
+    }
+    const float sixth = neg_dt * 1.0f / 6.0f;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        psi[c * nf + f] = p0[c] + sixth * us[c];
+        T[c * nf + f] = T0[c] + sixth * dTs[c];
+    }
+}
+
+// advect_u: 3D/run_3D.py:296-312 (one launch covers the faces of all axes).
+struct AdvectArgs {
+    const float* u0[3];
+    float* u1[3];
+    const float* T[3];
+    const float* psi[3];
+    int64_t nf[3];
+};
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock, 4) k_advect_u(GridDev g, AdvectArgs a) {
+    int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int ax = 0;
+    while (ax < DIM - 1 && f >= a.nf[ax]) {
+        f -= a.nf[ax];
+        ++ax;
+    }
+    if (f >= a.nf[ax]) return;
+    const int64_t nf = a.nf[ax];
+    float p[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) p[c] = a.psi[ax][c * nf + f];
+    float u[3], G[3][3];
+    interp_mac<DIM, false>(a.u0[0], a.u0[1], a.u0[2], g, p, u, G);
+    float acc = a.T[ax][f] * u[0];
+#pragma unroll
+    for (int c = 1; c < DIM; ++c) acc += a.T[ax][c * nf + f] * u[c];
+    a.u1[ax][f] = acc;
+}
+
+// reset_to_identity_grid with x/y/z_coords_func: 3D/run_3D.py:106-120,
+// 3D/taichi_utils.py:33-46: psi = (I + 0.5 off-axis) * dx, T = e_axis.
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_reset_identity_grid(GridDev g, int axis, int64_t nf,
+                                                                float* __restrict__ psi,
+                                                                float* __restrict__ T) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    int I[3];
+    face_ijk<DIM>(g, axis, f, I);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        float x = c == axis ? (float)I[c] : __fadd_rn((float)I[c], 0.5f);
+        psi[c * nf + f] = __fmul_rn(x, g.dx);
+        T[c * nf + f] = c == axis ? 1.0f : 0.0f;
+    }
+}
+
+// calc_max_speed: 3D/run_3D.py:95-103.  Max over non-negative floats is exact
+// and order independent, so unsigned atomicMax on the bit patterns is exact.
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_max_speed(GridDev g, const float* __restrict__ ux,
+                                                      const float* __restrict__ uy,
+                                                      const float* __restrict__ uz,
+                                                      unsigned int* __restrict__ out) {
+    const int64_t C = (int64_t)g.res[0] * g.res[1] * g.res[2];
+    float m = 0.0f;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < C;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        if (DIM == 3) {
+            k = (int)(q % g.res[2]);
+            j = (int)((q / g.res[2]) % g.res[1]);
+            i = (int)(q / ((int64_t)g.res[2] * g.res[1]));
+        } else {
+            k = 0;
+            j = (int)(q % g.res[1]);
+            i = (int)(q / g.res[1]);
+        }
+        float s2;
+        if (DIM == 3) {
+            const int rx = g.res[0], ry = g.res[1], rz = g.res[2];
+            float u = 0.5f * (ux[((int64_t)i * ry + j) * rz + k] + ux[((int64_t)(i + 1) * ry + j) * rz + k]);
+            float v = 0.5f * (uy[((int64_t)i * (ry + 1) + j) * rz + k] + uy[((int64_t)i * (ry + 1) + j + 1) * rz + k]);
+            float w = 0.5f * (uz[((int64_t)i * ry + j) * (rz + 1) + k] + uz[((int64_t)i * ry + j) * (rz + 1) + k + 1]);
+            (void)rx;
+            s2 = __fadd_rn(__fadd_rn(__fmul_rn(u, u), __fmul_rn(v, v)), __fmul_rn(w, w));
+        } else {
+            const int ry = g.res[1];
+            float u = 0.5f * (ux[(int64_t)i * ry + j] + ux[(int64_t)(i + 1) * ry + j]);
+            float v = 0.5f * (uy[(int64_t)i * (ry + 1) + j] + uy[(int64_t)i * (ry + 1) + j + 1]);
+            s2 = __fadd_rn(__fmul_rn(u, u), __fmul_rn(v, v));
+        }
+        m = fmaxf(m, __fsqrt_rn(s2));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+__global__ void k_set_u32(unsigned int* p, unsigned int v) { *p = v; }
+
+// split_central_vector: 3D/taichi_utils.py:87-100 (clamped samples).
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_split_central(GridDev g, int axis, int64_t nf,
+                                                          const float* __restrict__ vc,
+                                                          float* __restrict__ u) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    int I[3];
+    face_ijk<DIM>(g, axis, f, I);
+    const int64_t C = (int64_t)g.res[0] * g.res[1] * g.res[2];
+    int R[3] = {I[0], I[1], I[2]}, L[3] = {I[0], I[1], I[2]};
+    L[axis] -= 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        R[c] = max(0, min(R[c], g.res[c] - 1));
+        L[c] = max(0, min(L[c], g.res[c] - 1));
+    }
+    const float* v = vc + axis * C;
+    float r = v[((int64_t)R[0] * g.res[1] + R[1]) * g.res[2] + R[2]];
+    float l = v[((int64_t)L[0] * g.res[1] + L[1]) * g.res[2] + L[2]];
+    u[f] = 0.5f * __fadd_rn(r, l);
+}
+
+// get_central_vector: 3D/taichi_utils.py:80-85.
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_get_central(GridDev g, const float* __restrict__ ux,
+                                                        const float* __restrict__ uy,
+                                                        const float* __restrict__ uz,
+                                                        float* __restrict__ vc) {
+    const int64_t C = (int64_t)g.res[0] * g.res[1] * g.res[2];
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= C) return;
+    const int rx = g.res[0], ry = g.res[1], rz = g.res[2];
+    const int k = (int)(q % rz);
+    const int j = (int)((q / rz) % ry);
+    const int i = (int)(q / ((int64_t)rz * ry));
+    (void)rx;
+    vc[q] = 0.5f * __fadd_rn(ux[((int64_t)(i + 1) * ry + j) * rz + k], ux[((int64_t)i * ry + j) * rz + k]);
+    vc[C + q] = 0.5f * __fadd_rn(uy[((int64_t)i * (ry + 1) + j + 1) * rz + k], uy[((int64_t)i * (ry + 1) + j) * rz + k]);
+    if (DIM == 3)
+        vc[2 * C + q] = 0.5f * __fadd_rn(uz[((int64_t)i * ry + j) * (rz + 1) + k + 1], uz[((int64_t)i * ry + j) * (rz + 1) + k]);
+}
+
+// Point vortices: angular_vel_func + leapfrog_vel_func, 2D/init_conditions.py:5-9,29-60.
+constexpr int kMaxVort = 4096;
+__constant__ float c_vort[3 * kMaxVort];
+
+__global__ void __launch_bounds__(kBlock) k_point_vortices(GridDev g, int nv, float rad2,
+                                                           float strength, float* __restrict__ vc) {
+    const int64_t C = (int64_t)g.res[0] * g.res[1];
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= C) return;
+    const int j = (int)(q % g.res[1]);
+    const int i = (int)(q / g.res[1]);
+    const float X0 = __fmul_rn(__fadd_rn((float)i, 0.5f), g.dx);
+    const float X1 = __fmul_rn(__fadd_rn((float)j, 0.5f), g.dx);
+    float vx = 0.f, vy = 0.f;
+    for (int v = 0; v < nv; ++v) {
+        float px = __fsub_rn(X0, c_vort[v]);
+        float py = __fsub_rn(X1, c_vort[kMaxVort + v]);
+        float r = __fsqrt_rn(__fadd_rn(__fmul_rn(px, px), __fmul_rn(py, py)));
+        r = __fadd_rn(r, 1e-6f);
+        float lin = __fmul_rn(__fdiv_rn(__fmul_rn(strength, 1.0f), r),
+                              __fsub_rn(1.0f, expf(__fdiv_rn(-__fmul_rn(r, r), rad2))));
+        float ang = __fmul_rn(__fdiv_rn(1.0f, r), lin);
+        float s = __fmul_rn(ang, c_vort[2 * kMaxVort + v]);
+        vx = v == 0 ? __fmul_rn(s, -py) : __fadd_rn(vx, __fmul_rn(s, -py));
+        vy = v == 0 ? __fmul_rn(s, px) : __fadd_rn(vy, __fmul_rn(s, px));
+    }
+    vc[q] = vx;
+    vc[C + q] = vy;
+}
+
+// add_vortex_ring: 3D/init_conditions.py:10-22 (smoothed Biot-Savart of a
+// sampled ring, added into the cell-centred velocity).
+struct RingArgs {
+    float w, rad, delta, curve_length, four_pi, pi;
+    float c[3], ex[3], ey[3];
+    int num_samples;
+};
+
+__global__ void __launch_bounds__(kBlock) k_vortex_ring(GridDev g, RingArgs a, float* __restrict__ vc) {
+    const int64_t C = (int64_t)g.res[0] * g.res[1] * g.res[2];
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= C) return;
+    const int k = (int)(q % g.res[2]);
+    const int j = (int)((q / g.res[2]) % g.res[1]);
+    const int i = (int)(q / ((int64_t)g.res[2] * g.res[1]));
+    const float P[3] = {__fmul_rn(__fadd_rn((float)i, 0.5f), g.dx),
+                        __fmul_rn(__fadd_rn((float)j, 0.5f), g.dx),
+                        __fmul_rn(__fadd_rn((float)k, 0.5f), g.dx)};
+    float acc[3] = {vc[q], vc[C + q], vc[2 * C + q]};
+    for (int l = 0; l < a.num_samples; ++l) {
+        float theta = __fmul_rn(__fmul_rn(__fdiv_rn((float)l, (float)a.num_samples), 2.0f), a.pi);
+        float ct = cosf(theta), st = sinf(theta);
+        float pd[3], wv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float ps = __fadd_rn(__fmul_rn(a.rad, __fadd_rn(__fmul_rn(ct, a.ex[c]), __fmul_rn(st, a.ey[c]))), a.c[c]);
+            pd[c] = __fsub_rn(P[c], ps);
+            wv[c] = __fmul_rn(a.w, __fadd_rn(__fmul_rn(-st, a.ex[c]), __fmul_rn(ct, a.ey[c])));
+        }
+        float r = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(pd[0], pd[0]), __fmul_rn(pd[1], pd[1])), __fmul_rn(pd[2], pd[2])));
+        float rd = __fdiv_rn(r, a.delta);
+        float r3 = __fmul_rn(__fmul_rn(r, r), r);
+        float rd3 = __fmul_rn(__fmul_rn(rd, rd), rd);
+        float fac = __fmul_rn(a.curve_length,
+                              __fmul_rn(__fdiv_rn(-1.0f, __fmul_rn(a.four_pi, r3)), __fsub_rn(1.0f, expf(-rd3))));
+        float cr[3] = {__fsub_rn(__fmul_rn(pd[1], wv[2]), __fmul_rn(pd[2], wv[1])),
+                       __fsub_rn(__fmul_rn(pd[2], wv[0]), __fmul_rn(pd[0], wv[2])),
+                       __fsub_rn(__fmul_rn(pd[0], wv[1]), __fmul_rn(pd[1], wv[0]))};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(fac, cr[c]));
+    }
+    vc[q] = acc[0];
+    vc[C + q] = acc[1];
+    vc[2 * C + q] = acc[2];
+}
+
+// Kinetic energy (2D/run_2D.py:765-770): per-axis double sums of u^2,
+// reduced per block; partials summed in a fixed order by k_ke_finish.
+constexpr int kRedBlocks = 1024;
+
+__global__ void __launch_bounds__(kBlock) k_sumsq(const float* __restrict__ u, int64_t n,
+                                                  double* __restrict__ partial) {
+    __shared__ double sh[kBlock / 64];
+    double s = 0.0;
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < n;
+         f += (int64_t)gridDim.x * blockDim.x) {
+        double v = (double)u[f];
+        s += v * v;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_ke_finish(const double* __restrict__ partial, int dim,
+                                                      double n0, double n1, double n2, double* out) {
+    __shared__ double sh[kBlock / 64];
+    double E = 0.0;
+    const double nf[3] = {n0, n1, n2};
+    for (int a = 0; a < dim; ++a) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) s += partial[a * kRedBlocks + b];
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) E += s / nf[a];
+    }
+    if (threadIdx.x == 0) *out = E;
+}
+
+}  // namespace pfm
+
+using namespace pfm;
+
+extern "C" {
+
+int pfm_rk4_grid(const pfm_grid* gp, int axis, float* psi, float* T, const float* u_x,
+                 const float* u_y, const float* u_z, float dt, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(axis >= 0 && axis < g.dim, "axis out of range");
+    PFM_CHECK_ARG(psi && T && u_x && u_y && (g.dim == 2 || u_z), "null pointer");
+    const int64_t nf = face_count_h(g, axis);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_rk4_grid<3>, dim3(grid_for(nf)), dim3(kBlock), 0, as_stream(stream), g, nf,
+                           psi, T, u_x, u_y, u_z, dt);
+    else
+        hipLaunchKernelGGL(k_rk4_grid<2>, dim3(grid_for(nf)), dim3(kBlock), 0, as_stream(stream), g, nf,
+                           psi, T, u_x, u_y, u_z, dt);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_advect_u(const pfm_grid* gp, const float* u0_x, const float* u0_y, const float* u0_z,
+                 float* u1_x, float* u1_y, float* u1_z, const float* T_x, const float* T_y,
+                 const float* T_z, const float* psi_x, const float* psi_y, const float* psi_z,
+                 pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    AdvectArgs a;
+    const float* u0[3] = {u0_x, u0_y, u0_z};
+    float* u1[3] = {u1_x, u1_y, u1_z};
+    const float* T[3] = {T_x, T_y, T_z};
+    const float* psi[3] = {psi_x, psi_y, psi_z};
+    int64_t total = 0;
+    for (int c = 0; c < 3; ++c) {
+        a.u0[c] = u0[c];
+        a.u1[c] = u1[c];
+        a.T[c] = T[c];
+        a.psi[c] = psi[c];
+        a.nf[c] = c < g.dim ? face_count_h(g, c) : 0;
+        if (c < g.dim) {
+            PFM_CHECK_ARG(u0[c] && u1[c] && T[c] && psi[c], "null pointer for axis %d", c);
+            for (int d = 0; d < g.dim; ++d)
+                PFM_CHECK_ARG(u1[c] != u0[d], "u1 must not alias u0");
+        }
+        total += a.nf[c];
+    }
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_advect_u<3>, dim3(grid_for(total)), dim3(kBlock), 0, as_stream(stream), g, a);
+    else
+        hipLaunchKernelGGL(k_advect_u<2>, dim3(grid_for(total)), dim3(kBlock), 0, as_stream(stream), g, a);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_reset_identity_grid(const pfm_grid* gp, int axis, float* psi, float* T,
+                            pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(axis >= 0 && axis < g.dim, "axis out of range");
+    PFM_CHECK_ARG(psi && T, "null pointer");
+    const int64_t nf = face_count_h(g, axis);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_reset_identity_grid<3>, dim3(grid_for(nf)), dim3(kBlock), 0, as_stream(stream), g,
+                           axis, nf, psi, T);
+    else
+        hipLaunchKernelGGL(k_reset_identity_grid<2>, dim3(grid_for(nf)), dim3(kBlock), 0, as_stream(stream), g,
+                           axis, nf, psi, T);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_max_speed(const pfm_grid* gp, const float* u_x, const float* u_y, const float* u_z,
+                  float* out, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(u_x && u_y && (g.dim == 2 || u_z) && out, "null pointer");
+    hipStream_t s = as_stream(stream);
+    unsigned int init_bits;
+    float floor_v = 1.e-3f;  // max_speed[None] = 1.e-3 (3D/run_3D.py:97)
+    memcpy(&init_bits, &floor_v, 4);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, reinterpret_cast<unsigned int*>(out), init_bits);
+    PFM_LAUNCH_CHECK();
+    const int64_t C = cell_count(g);
+    unsigned blocks = grid_for(C);
+    if (blocks > 2048) blocks = 2048;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_max_speed<3>, dim3(blocks), dim3(kBlock), 0, s, g, u_x, u_y, u_z,
+                           reinterpret_cast<unsigned int*>(out));
+    else
+        hipLaunchKernelGGL(k_max_speed<2>, dim3(blocks), dim3(kBlock), 0, s, g, u_x, u_y, u_z,
+                           reinterpret_cast<unsigned int*>(out));
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_split_central(const pfm_grid* gp, const float* vc, float* u_x, float* u_y, float* u_z,
+                      pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    float* u[3] = {u_x, u_y, u_z};
+    PFM_CHECK_ARG(vc && u_x && u_y && (g.dim == 2 || u_z), "null pointer");
+    for (int a = 0; a < g.dim; ++a) {
+        const int64_t nf = face_count_h(g, a);
+        if (g.dim == 3)
+            hipLaunchKernelGGL(k_split_central<3>, dim3(grid_for(nf)), dim3(kBlock), 0, as_stream(stream), g, a,
+                               nf, vc, u[a]);
+        else
+            hipLaunchKernelGGL(k_split_central<2>, dim3(grid_for(nf)), dim3(kBlock), 0, as_stream(stream), g, a,
+                               nf, vc, u[a]);
+        PFM_LAUNCH_CHECK();
+    }
+    return PFM_OK;
+}
+
+int pfm_get_central(const pfm_grid* gp, const float* u_x, const float* u_y, const float* u_z,
+                    float* vc, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(vc && u_x && u_y && (g.dim == 2 || u_z), "null pointer");
+    const int64_t C = cell_count(g);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_get_central<3>, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), g, u_x, u_y,
+                           u_z, vc);
+    else
+        hipLaunchKernelGGL(k_get_central<2>, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), g, u_x, u_y,
+                           u_z, vc);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_point_vortices(const pfm_grid* gp, int nv, const float* cx, const float* cy,
+                       const float* w, float rad, float strength, float* vc,
+                       pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(g.dim == 2, "point vortices are 2D");
+    PFM_CHECK_ARG(nv >= 1 && nv <= kMaxVort, "nv must be in [1, %d]", kMaxVort);
+    PFM_CHECK_ARG(cx && cy && w && vc, "null pointer");
+    static float host[3 * kMaxVort];
+    memcpy(host, cx, sizeof(float) * nv);
+    memcpy(host + kMaxVort, cy, sizeof(float) * nv);
+    memcpy(host + 2 * kMaxVort, w, sizeof(float) * nv);
+    hipStream_t s = as_stream(stream);
+    PFM_HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_vort), host, sizeof(host), 0, hipMemcpyHostToDevice, s));
+    PFM_HIP_TRY(hipStreamSynchronize(s));
+    const int64_t C = cell_count(g);
+    const float rad2 = rad * rad;
+    hipLaunchKernelGGL(k_point_vortices, dim3(grid_for(C)), dim3(kBlock), 0, s, g, nv, rad2, strength, vc);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_vortex_ring(const pfm_grid* gp, float w, float rad, float delta, const float* c,
+                    const float* ex, const float* ey, int num_samples, float* vc,
+                    pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(g.dim == 3, "vortex rings are 3D");
+    PFM_CHECK_ARG(num_samples >= 1 && c && ex && ey && vc, "bad arguments");
+    RingArgs a;
+    a.w = w;
+    a.rad = rad;
+    a.delta = delta;
+    a.curve_length = (float)((2.0 * M_PI * (double)rad) / num_samples);
+    a.four_pi = (float)(4.0 * M_PI);
+    a.pi = (float)M_PI;
+    for (int q = 0; q < 3; ++q) {
+        a.c[q] = c[q];
+        a.ex[q] = ex[q];
+        a.ey[q] = ey[q];
+    }
+    a.num_samples = num_samples;
+    const int64_t C = cell_count(g);
+    hipLaunchKernelGGL(k_vortex_ring, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), g, a, vc);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_kinetic_energy(const pfm_grid* gp, const float* u_x, const float* u_y, const float* u_z,
+                       double* out, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(u_x && u_y && (g.dim == 2 || u_z) && out, "null pointer");
+    // The partial sums live in the output's trailing scratch: the caller's
+    // `out` must hold 1 + 3*1024 doubles.
+    double* partial = out + 1;
+    const float* u[3] = {u_x, u_y, u_z};
+    double nf[3] = {1, 1, 1};
+    hipStream_t s = as_stream(stream);
+    for (int a = 0; a < g.dim; ++a) {
+        nf[a] = (double)face_count_h(g, a);
+        hipLaunchKernelGGL(k_sumsq, dim3(kRedBlocks), dim3(kBlock), 0, s, u[a], face_count_h(g, a),
+                           partial + a * kRedBlocks);
+        PFM_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_ke_finish, dim3(1), dim3(kBlock), 0, s, partial, g.dim, nf[0], nf[1], nf[2], out);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+}  // extern "C"

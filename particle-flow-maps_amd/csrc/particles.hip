@@ -1,0 +1,444 @@
+// particles.hip -- per-particle kernels of the flow-map step: forward RK4 of
+// position and backward-map Jacobian, G2P at reinitialisation, Jacobian
+// composition, impulse/affine updates, identity resets and lattice seeding.
+// One thread per particle over SoA arrays (coalesced); the 4^d B-spline
+// stencil is evaluated separably from 16-byte row loads (see pfm_device.h).
+#include "pfm_host.h"
+
+namespace pfm {
+
+// RK4_T_forward: 3D/run_3D.py:175-219, 2D/run_2D.py:304-341.
+// dT = M @ T with M = cols(grad u_x, grad u_y, grad u_z): dT[r] = sum_c G[c][r] T[c].
+template <int DIM>
+__global__ void __launch_bounds__(kBlock, 4) k_rk4_particles(GridDev g, int64_t n,
+                                                          float* __restrict__ pos,
+                                                          float* __restrict__ Tx,
+                                                          float* __restrict__ Ty,
+                                                          float* __restrict__ Tz,
+                                                          const float* __restrict__ ux,
+                                                          const float* __restrict__ uy,
+                                                          const float* __restrict__ uz, float dt,
+                                                          int advect_psi) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float* Ts[3] = {Tx, Ty, Tz};
+    float p0[3] = {0.f, 0.f, 0.f};
+    float T0[3][3];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) p0[c] = pos[c * n + i];
+#pragma unroll
+    for (int m = 0; m < DIM; ++m)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) T0[m][c] = Ts[m][c * n + i];
+
+    const float half_dt = 0.5f * dt;
+    const float hstage[3] = {half_dt, half_dt, 1.0f * dt};
+    const float wstage[4] = {1.f, 2.f, 2.f, 1.f};
+    float pk[3] = {p0[0], p0[1], p0[2]};
+    float Tk[3][3];
+    float us[3] = {0.f, 0.f, 0.f};
+    float dTs[3][3];
+#pragma unroll
+    for (int m = 0; m < DIM; ++m)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            Tk[m][c] = T0[m][c];
+            dTs[m][c] = 0.f;
+        }
+
+#pragma unroll 1
+    for (int st = 0; st < 4; ++st) {
+        float u[3], G[3][3];
+        interp_mac<DIM, true>(ux, uy, uz, g, pk, u, G);
+        float dT[3][3];
+#pragma unroll
+        for (int m = 0; m < DIM; ++m)
+#pragma unroll
+            for (int r = 0; r < DIM; ++r) {
+                float acc = G[0][r] * Tk[m][0];
+#pragma unroll
+                for (int c = 1; c < DIM; ++c) acc += G[c][r] * Tk[m][c];
+                dT[m][r] = acc;
+            }
+        const float ws = wstage[st];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) us[c] += ws * u[c];
+#pragma unroll
+        for (int m = 0; m < DIM; ++m)
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) dTs[m][c] += ws * dT[m][c];
+        if (st < 3) {
+            const float h = hstage[st];
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) pk[c] = p0[c] + h * u[c];
+#pragma unroll
+            for (int m = 0; m < DIM; ++m)
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) Tk[m][c] = T0[m][c] - h * dT[m][c];
+        }
+    }
+    const float sixth_dt = dt * 1.0f / 6.0f;
+    if (advect_psi) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) pos[c * n + i] = p0[c] + sixth_dt * us[c];
+    }
+#pragma unroll
+    for (int m = 0; m < DIM; ++m)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) Ts[m][c * n + i] = T0[m][c] - sixth_dt * dTs[m][c];
+}
+
+// init_particles_imp / init_particles_imp_grad_m: 3D/init_conditions.py:254-273,
+// 2D/init_conditions.py:183-210 (interp_u_MAC_grad_imp, 3D/taichi_utils.py:326-376:
+// new_C rows equal the velocity gradient rows).
+template <int DIM, bool GRAD>
+__global__ void __launch_bounds__(kBlock, 4) k_g2p(GridDev g, int64_t n, const float* __restrict__ pos,
+                                                const float* __restrict__ ux,
+                                                const float* __restrict__ uy,
+                                                const float* __restrict__ uz,
+                                                float* __restrict__ imp, float* __restrict__ Cx,
+                                                float* __restrict__ Cy, float* __restrict__ Cz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) p[c] = pos[c * n + i];
+    float u[3], G[3][3];
+    interp_mac<DIM, GRAD>(ux, uy, uz, g, p, u, G);
+    if (imp != nullptr) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) imp[c * n + i] = u[c];
+    }
+    if (GRAD) {
+        float* Cs[3] = {Cx, Cy, Cz};
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) Cs[a][c * n + i] = G[a][c];
+    }
+}
+
+// update_T: out = cols(T_gm) @ cols(T_in) (3D/run_3D.py:430-439, 2D/run_2D.py:538-546).
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_update_T(int64_t n, const float* __restrict__ gx,
+                                                     const float* __restrict__ gy,
+                                                     const float* __restrict__ gz,
+                                                     const float* inx, const float* iny,
+                                                     const float* inz, float* ox, float* oy,
+                                                     float* oz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* G[3] = {gx, gy, gz};
+    const float* In[3] = {inx, iny, inz};
+    float* O[3] = {ox, oy, oz};
+    float A[3][3], B[3][3];
+#pragma unroll
+    for (int m = 0; m < DIM; ++m)
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+            A[r][m] = G[m][r * n + i];
+            B[r][m] = In[m][r * n + i];
+        }
+#pragma unroll
+    for (int m = 0; m < DIM; ++m)
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+            float acc = A[r][0] * B[0][m];
+#pragma unroll
+            for (int k = 1; k < DIM; ++k) acc += A[r][k] * B[k][m];
+            O[m][r * n + i] = acc;
+        }
+}
+
+// update_particles_imp: imp = cols(T) @ init_imp (2D/run_2D.py:517-524).
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_update_imp(int64_t n, const float* __restrict__ tx,
+                                                       const float* __restrict__ ty,
+                                                       const float* __restrict__ tz,
+                                                       const float* __restrict__ init_imp,
+                                                       float* __restrict__ imp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* T[3] = {tx, ty, tz};
+    float m0[3];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) m0[c] = init_imp[c * n + i];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+        float acc = T[0][r * n + i] * m0[0];
+#pragma unroll
+        for (int c = 1; c < DIM; ++c) acc += T[c][r * n + i] * m0[c];
+        imp[r * n + i] = acc;
+    }
+}
+
+// T_init_C_T = cols(T_gm) @ (rows(init_C) @ rows(T_gm)) (2D/run_2D.py:526-536,
+// 3D/run_3D.py:462-468).  A[r][c] = T_r[c] (rows), C[r][c] = init_C_r[c].
+template <int DIM>
+__device__ __forceinline__ void compose_C(const float A[3][3], const float C[3][3], float Q[3][3]) {
+    float P[3][3];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            float acc = C[r][0] * A[0][c];
+#pragma unroll
+            for (int k = 1; k < DIM; ++k) acc += C[r][k] * A[k][c];
+            P[r][c] = acc;
+        }
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            float acc = A[0][r] * P[0][c];
+#pragma unroll
+            for (int k = 1; k < DIM; ++k) acc += A[k][r] * P[k][c];
+            Q[r][c] = acc;
+        }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_update_grad_m(
+    int64_t n, float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz,
+    const float* __restrict__ icx, const float* __restrict__ icy, const float* __restrict__ icz,
+    const float* __restrict__ gx, const float* __restrict__ gy, const float* __restrict__ gz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* Gs[3] = {gx, gy, gz};
+    const float* Ic[3] = {icx, icy, icz};
+    float* Co[3] = {cx, cy, cz};
+    float A[3][3], C[3][3], Q[3][3];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            A[r][c] = Gs[r][c * n + i];
+            C[r][c] = Ic[r][c * n + i];
+        }
+    compose_C<DIM>(A, C, Q);
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) Co[r][c * n + i] = Q[r][c];
+}
+
+// reset_T_to_identity: 3D/run_3D.py:133-140 (T_m = unit vector m).
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_reset_T(int64_t n, float* tx, float* ty, float* tz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float* T[3] = {tx, ty, tz};
+#pragma unroll
+    for (int m = 0; m < DIM; ++m)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) T[m][c * n + i] = (m == c) ? 1.0f : 0.0f;
+}
+
+// Seeded jitter (extension): counter-based hash of (seed, particle id, component),
+// identical to the oracle's so CPU and GPU seed the same positions bit-exactly.
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ float jitter_u(uint32_t seed, uint64_t id, uint32_t comp) {
+    uint32_t h = hash32((uint32_t)id ^ hash32(seed * 0x9e3779b9u + comp));
+    h = hash32(h ^ (uint32_t)(id >> 32) * 0x27d4eb2fu);
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+// init_particles_pos_uniform: 3D/init_conditions.py:238-252, 2D/init_conditions.py:139-166.
+// order 0: the reference numbering (lattice x fastest).  order 1: cell-major
+// (cells with the last axis fastest, the ppc_axis^d sub-lattice inside).
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_seed(GridDev g, int ppc_axis, float dist, int64_t n,
+                                                 float* __restrict__ pos, int64_t seed,
+                                                 int order) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t id[3] = {0, 0, 0};
+    if (order == 0) {
+        const int64_t nx = (int64_t)ppc_axis * g.res[0];
+        const int64_t ny = (int64_t)ppc_axis * g.res[1];
+        id[0] = i % nx;
+        const int64_t iyz = i / nx;
+        if (DIM == 3) {
+            id[1] = iyz % ny;
+            id[2] = iyz / ny;
+        } else {
+            id[1] = iyz;
+        }
+    } else {
+        int64_t per = ppc_axis;
+        for (int c = 1; c < DIM; ++c) per *= ppc_axis;
+        const int64_t cell = i / per;
+        int64_t sub = i % per;
+        int64_t ci[3] = {0, 0, 0};
+        if (DIM == 3) {
+            ci[2] = cell % g.res[2];
+            ci[1] = (cell / g.res[2]) % g.res[1];
+            ci[0] = cell / ((int64_t)g.res[2] * g.res[1]);
+        } else {
+            ci[1] = cell % g.res[1];
+            ci[0] = cell / g.res[1];
+        }
+        for (int c = DIM - 1; c >= 0; --c) {
+            id[c] = ci[c] * ppc_axis + sub % ppc_axis;
+            sub /= ppc_axis;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        float off = seed >= 0 ? jitter_u((uint32_t)seed, (uint64_t)i, (uint32_t)c) : 0.5f;
+        pos[c * n + i] = __fmul_rn(__fadd_rn((float)id[c], off), dist);
+    }
+}
+
+}  // namespace pfm
+
+using namespace pfm;
+
+extern "C" {
+
+int pfm_rk4_particles(const pfm_grid* gp, int64_t n, float* pos, float* T_x, float* T_y,
+                      float* T_z, const float* u_x, const float* u_y, const float* u_z, float dt,
+                      int advect_psi, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return PFM_OK;
+    PFM_CHECK_ARG(pos && T_x && T_y && u_x && u_y, "null pointer");
+    PFM_CHECK_ARG(g.dim == 2 || (T_z && u_z), "3D needs T_z and u_z");
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_rk4_particles<3>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
+                           g, n, pos, T_x, T_y, T_z, u_x, u_y, u_z, dt, advect_psi);
+    else
+        hipLaunchKernelGGL(k_rk4_particles<2>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
+                           g, n, pos, T_x, T_y, T_z, u_x, u_y, u_z, dt, advect_psi);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_g2p(const pfm_grid* gp, int64_t n, const float* pos, const float* u_x, const float* u_y,
+            const float* u_z, float* imp, float* C_x, float* C_y, float* C_z,
+            pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return PFM_OK;
+    PFM_CHECK_ARG(pos && u_x && u_y, "null pointer");
+    PFM_CHECK_ARG(g.dim == 2 || u_z, "3D needs u_z");
+    const bool grad = C_x != nullptr;
+    PFM_CHECK_ARG(!grad || (C_y && (g.dim == 2 || C_z)), "C_x given without C_y/C_z");
+    PFM_CHECK_ARG(imp || grad, "nothing to write: imp and C_x are both NULL");
+    dim3 gr(grid_for(n)), bl(kBlock);
+    hipStream_t s = as_stream(stream);
+    if (g.dim == 3) {
+        if (grad)
+            hipLaunchKernelGGL((k_g2p<3, true>), gr, bl, 0, s, g, n, pos, u_x, u_y, u_z, imp, C_x, C_y, C_z);
+        else
+            hipLaunchKernelGGL((k_g2p<3, false>), gr, bl, 0, s, g, n, pos, u_x, u_y, u_z, imp, C_x, C_y, C_z);
+    } else {
+        if (grad)
+            hipLaunchKernelGGL((k_g2p<2, true>), gr, bl, 0, s, g, n, pos, u_x, u_y, u_z, imp, C_x, C_y, C_z);
+        else
+            hipLaunchKernelGGL((k_g2p<2, false>), gr, bl, 0, s, g, n, pos, u_x, u_y, u_z, imp, C_x, C_y, C_z);
+    }
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_update_T(int dim, int64_t n, const float* Tgm_x, const float* Tgm_y, const float* Tgm_z,
+                 const float* Tin_x, const float* Tin_y, const float* Tin_z, float* Tout_x,
+                 float* Tout_y, float* Tout_z, pfm_stream_t stream) {
+    PFM_CHECK_ARG(dim == 2 || dim == 3, "dim must be 2 or 3");
+    PFM_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return PFM_OK;
+    PFM_CHECK_ARG(Tgm_x && Tgm_y && Tin_x && Tin_y && Tout_x && Tout_y, "null pointer");
+    PFM_CHECK_ARG(dim == 2 || (Tgm_z && Tin_z && Tout_z), "3D needs z columns");
+    if (dim == 3)
+        hipLaunchKernelGGL(k_update_T<3>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n,
+                           Tgm_x, Tgm_y, Tgm_z, Tin_x, Tin_y, Tin_z, Tout_x, Tout_y, Tout_z);
+    else
+        hipLaunchKernelGGL(k_update_T<2>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n,
+                           Tgm_x, Tgm_y, Tgm_z, Tin_x, Tin_y, Tin_z, Tout_x, Tout_y, Tout_z);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_update_particles_imp(int dim, int64_t n, const float* T_x, const float* T_y,
+                             const float* T_z, const float* init_imp, float* imp,
+                             pfm_stream_t stream) {
+    PFM_CHECK_ARG(dim == 2 || dim == 3, "dim must be 2 or 3");
+    PFM_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return PFM_OK;
+    PFM_CHECK_ARG(T_x && T_y && init_imp && imp && (dim == 2 || T_z), "null pointer");
+    if (dim == 3)
+        hipLaunchKernelGGL(k_update_imp<3>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n,
+                           T_x, T_y, T_z, init_imp, imp);
+    else
+        hipLaunchKernelGGL(k_update_imp<2>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n,
+                           T_x, T_y, T_z, init_imp, imp);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_update_particles_grad_m(int dim, int64_t n, float* C_x, float* C_y, float* C_z,
+                                const float* iC_x, const float* iC_y, const float* iC_z,
+                                const float* Tgm_x, const float* Tgm_y, const float* Tgm_z,
+                                pfm_stream_t stream) {
+    PFM_CHECK_ARG(dim == 2 || dim == 3, "dim must be 2 or 3");
+    PFM_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return PFM_OK;
+    PFM_CHECK_ARG(C_x && C_y && iC_x && iC_y && Tgm_x && Tgm_y, "null pointer");
+    PFM_CHECK_ARG(dim == 2 || (C_z && iC_z && Tgm_z), "3D needs z rows");
+    if (dim == 3)
+        hipLaunchKernelGGL(k_update_grad_m<3>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n,
+                           C_x, C_y, C_z, iC_x, iC_y, iC_z, Tgm_x, Tgm_y, Tgm_z);
+    else
+        hipLaunchKernelGGL(k_update_grad_m<2>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n,
+                           C_x, C_y, C_z, iC_x, iC_y, iC_z, Tgm_x, Tgm_y, Tgm_z);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_reset_T_identity(int dim, int64_t n, float* T_x, float* T_y, float* T_z,
+                         pfm_stream_t stream) {
+    PFM_CHECK_ARG(dim == 2 || dim == 3, "dim must be 2 or 3");
+    PFM_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return PFM_OK;
+    PFM_CHECK_ARG(T_x && T_y && (dim == 2 || T_z), "null pointer");
+    if (dim == 3)
+        hipLaunchKernelGGL(k_reset_T<3>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n, T_x, T_y, T_z);
+    else
+        hipLaunchKernelGGL(k_reset_T<2>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n, T_x, T_y, T_z);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_seed_particles(const pfm_grid* gp, int ppc_axis, float dist, int64_t n, float* pos,
+                       int64_t seed, int order, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(ppc_axis >= 1, "ppc_axis must be >= 1");
+    PFM_CHECK_ARG(order == 0 || order == 1, "order must be 0 or 1");
+    PFM_CHECK_ARG(n >= 0 && pos, "bad particle buffer");
+    if (n == 0) return PFM_OK;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_seed<3>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), g, ppc_axis,
+                           dist, n, pos, seed, order);
+    else
+        hipLaunchKernelGGL(k_seed<2>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), g, ppc_axis,
+                           dist, n, pos, seed, order);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+}  // extern "C"

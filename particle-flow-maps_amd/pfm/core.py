@@ -1,0 +1,321 @@
+"""Dimension-generic host wrappers over the libpfm C ABI.
+
+Field conventions (all float32, contiguous, on the GPU):
+  * face field of axis a: shape ``res`` with ``res[a] + 1`` (the reference's
+    ``u_x`` is ``(rx+1, ry, rz)``);
+  * particle vector field: ``(d, n)`` SoA; a per-particle matrix is held as d
+    vector fields (``T_x, T_y, T_z`` columns or ``C_x, C_y, C_z`` rows);
+  * grid-attached vector field (``psi_x_grid``, ``T_x_grid``): ``(d, *face_shape)``.
+
+The reference-named modules (``pfm.d3.run_3D``, ``pfm.d2.run_2D`` ...) are thin
+argument-order adapters over these functions.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Optional, Sequence
+
+import torch
+
+from . import _lib as L
+
+
+def face_shape(res: Sequence[int], axis: int):
+    s = list(res)
+    s[axis] += 1
+    return tuple(s)
+
+
+def res_from_faces(u_x: torch.Tensor, dim: Optional[int] = None):
+    """Cell resolution of a grid from its x-face field (rx+1, ry[, rz])."""
+    s = list(u_x.shape)
+    s[0] -= 1
+    return tuple(s)
+
+
+def default_dx(res) -> float:
+    """dx = 1/res_y (3D/run_3D.py:16, 2D/run_2D.py:14)."""
+    return 1.0 / res[1]
+
+
+def _stream(stream):
+    return L.stream_ptr(stream)
+
+
+# Writes made by the library through raw pointers do not bump torch's version
+# counter, so positions written here are tracked separately for the cached
+# binning in ``Binning.bin``.
+_WRITES = {}
+
+
+def _mark_written(t):
+    k = t.data_ptr()
+    _WRITES[k] = _WRITES.get(k, 0) + 1
+
+
+def write_count(t) -> int:
+    return _WRITES.get(t.data_ptr(), 0)
+
+
+def _u_ptrs(u, res):
+    d = len(res)
+    out = []
+    for a in range(3):
+        if a < d:
+            t = u[a]
+            if tuple(t.shape) != face_shape(res, a):
+                raise L.PfmError(f"face field {a} has shape {tuple(t.shape)}, expected {face_shape(res, a)}")
+            out.append(L.ptr(t, name=f"u[{a}]"))
+        else:
+            out.append(None)
+    return out
+
+
+def _pvec(t, d, n, name):
+    if tuple(t.shape) != (d, n):
+        raise L.PfmError(f"{name} must have shape ({d}, {n}), got {tuple(t.shape)}")
+    return L.ptr(t, name=name)
+
+
+def _cols(Ts, d, n, name):
+    out = []
+    for m in range(3):
+        out.append(_pvec(Ts[m], d, n, f"{name}[{m}]") if m < d else None)
+    return out
+
+
+# --------------------------------------------------------------------------
+# flow-map advection
+# --------------------------------------------------------------------------
+def rk4_particles(res, dx, pos, T, u, dt, advect_psi=True, stream=None):
+    d, n = pos.shape
+    g = L.make_grid(res, dx)
+    L.call("pfm_rk4_particles", ctypes.byref(g), n, _pvec(pos, d, n, "pos"), *_cols(T, d, n, "T"),
+           *_u_ptrs(u, res), float(dt), int(bool(advect_psi)), _stream(stream))
+    if advect_psi:
+        _mark_written(pos)
+
+
+def face_axis(res, t, lead=1):
+    """Which face axis a grid-attached field (lead dims + face shape) belongs to."""
+    sp = tuple(t.shape[lead:])
+    for a in range(len(res)):
+        if sp == face_shape(res, a):
+            return a
+    raise L.PfmError(f"shape {tuple(t.shape)} is not a face-attached field of grid {tuple(res)}")
+
+
+def rk4_grid(res, dx, psi, T, u, dt, stream=None):
+    d = len(res)
+    a = face_axis(res, psi)
+    if tuple(T.shape) != tuple(psi.shape) or psi.shape[0] != d:
+        raise L.PfmError("psi and T must both be (d, *face_shape)")
+    g = L.make_grid(res, dx)
+    L.call("pfm_rk4_grid", ctypes.byref(g), a, L.ptr(psi, name="psi"), L.ptr(T, name="T"),
+           *_u_ptrs(u, res), float(dt), _stream(stream))
+
+
+def advect_u(res, dx, u0, u1, T, psi, stream=None):
+    d = len(res)
+    g = L.make_grid(res, dx)
+    Tp, Pp = [], []
+    for a in range(3):
+        if a < d:
+            if face_axis(res, T[a]) != a or face_axis(res, psi[a]) != a:
+                raise L.PfmError(f"T/psi of axis {a} have the wrong face shape")
+            Tp.append(L.ptr(T[a], name=f"T[{a}]"))
+            Pp.append(L.ptr(psi[a], name=f"psi[{a}]"))
+        else:
+            Tp.append(None)
+            Pp.append(None)
+    L.call("pfm_advect_u", ctypes.byref(g), *_u_ptrs(u0, res), *_u_ptrs(u1, res), *Tp, *Pp,
+           _stream(stream))
+
+
+def reset_identity_grid(res, dx, psi, T, stream=None):
+    g = L.make_grid(res, dx)
+    a = face_axis(res, psi)
+    L.call("pfm_reset_identity_grid", ctypes.byref(g), a, L.ptr(psi, name="psi"), L.ptr(T, name="T"),
+           _stream(stream))
+
+
+def reset_T_identity(T, stream=None):
+    d, n = T[0].shape
+    L.call("pfm_reset_T_identity", d, n, *_cols(T, d, n, "T"), _stream(stream))
+
+
+# --------------------------------------------------------------------------
+# particle <-> grid
+# --------------------------------------------------------------------------
+def seed_particles(res, ppc_axis, dist, pos, seed=-1, order=0, stream=None):
+    d, n = pos.shape
+    g = L.make_grid(res, 1.0 / res[1])
+    L.call("pfm_seed_particles", ctypes.byref(g), int(ppc_axis), float(dist), n, _pvec(pos, d, n, "pos"),
+           int(seed), int(order), _stream(stream))
+    _mark_written(pos)
+
+
+def g2p(res, dx, pos, u, imp, C=None, stream=None):
+    d, n = pos.shape
+    g = L.make_grid(res, dx)
+    Cp = _cols(C, d, n, "C") if C is not None else [None, None, None]
+    ip = _pvec(imp, d, n, "imp") if imp is not None else None
+    L.call("pfm_g2p", ctypes.byref(g), n, _pvec(pos, d, n, "pos"), *_u_ptrs(u, res), ip, *Cp,
+           _stream(stream))
+
+
+def update_T(Tgm, Tin, Tout, stream=None):
+    d, n = Tgm[0].shape
+    L.call("pfm_update_T", d, n, *_cols(Tgm, d, n, "T_grad_m"), *_cols(Tin, d, n, "T_in"),
+           *_cols(Tout, d, n, "T_out"), _stream(stream))
+
+
+def update_particles_imp(T, init_imp, imp, stream=None):
+    d, n = init_imp.shape
+    L.call("pfm_update_particles_imp", d, n, *_cols(T, d, n, "T"), _pvec(init_imp, d, n, "init_imp"),
+           _pvec(imp, d, n, "imp"), _stream(stream))
+
+
+def update_particles_grad_m(C, init_C, Tgm, stream=None):
+    d, n = C[0].shape
+    L.call("pfm_update_particles_grad_m", d, n, *_cols(C, d, n, "C"), *_cols(init_C, d, n, "init_C"),
+           *_cols(Tgm, d, n, "T_grad_m"), _stream(stream))
+
+
+class Binning:
+    """Cell sort of one particle set (pfm_bin_particles) plus the P2G workspace.
+
+    Keeps the last sort and re-bins only when the position tensor changed
+    (tracked by data pointer and torch's in-place version counter)."""
+
+    def __init__(self, res, n, device=None):
+        self.res = tuple(res)
+        self.n = int(n)
+        dev = device or torch.device("cuda")
+        C = math.prod(self.res)
+        g = L.make_grid(self.res, 1.0 / self.res[1])
+        ws = L.load().pfm_p2g_workspace_size(ctypes.byref(g), self.n)
+        if ws == 0:
+            raise L.PfmError(f"pfm_p2g_workspace_size failed: {L.last_error()}")
+        self.workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
+        self.cell_start = torch.empty(C + 1, dtype=torch.int32, device=dev)
+        self.perm = torch.empty(max(self.n, 1), dtype=torch.int32, device=dev)
+        self.cell_of = torch.empty(max(self.n, 1), dtype=torch.int32, device=dev)
+        self._key = None
+
+    def key(self, pos):
+        return (pos.data_ptr(), pos._version, write_count(pos), tuple(pos.shape))
+
+    def bin(self, pos, dx, stream=None, force=False):
+        if not force and self._key == self.key(pos):
+            return
+        d, n = pos.shape
+        if n != self.n:
+            raise L.PfmError(f"Binning built for {self.n} particles, got {n}")
+        g = L.make_grid(self.res, dx)
+        L.call("pfm_bin_particles", ctypes.byref(g), n, _pvec(pos, d, n, "pos"),
+               L.ptr(self.cell_of, dtype=torch.int32), L.ptr(self.cell_start, dtype=torch.int32),
+               L.ptr(self.perm, dtype=torch.int32), self.workspace.data_ptr(), self.workspace.numel(),
+               _stream(stream))
+        self._key = self.key(pos)
+
+    def cell_lists(self, dx, ids, counts, stream=None):
+        g = L.make_grid(self.res, dx)
+        cell_max = ids.shape[-1]
+        L.call("pfm_cell_lists", ctypes.byref(g), L.ptr(self.cell_start, dtype=torch.int32),
+               L.ptr(self.perm, dtype=torch.int32), int(cell_max), L.ptr(ids, dtype=torch.int32),
+               L.ptr(counts, dtype=torch.int32), _stream(stream))
+
+    def p2g(self, dx, pos, imp, C, u, W=None, Tgm=None, Tin=None, stream=None):
+        d, n = pos.shape
+        self.bin(pos, dx, stream)
+        g = L.make_grid(self.res, dx)
+        Wp = [L.ptr(W[a], name=f"W[{a}]") if (W is not None and a < d) else None for a in range(3)]
+        if W is not None:
+            for a in range(d):
+                if tuple(W[a].shape) != face_shape(self.res, a):
+                    raise L.PfmError(f"weight field {a} has the wrong shape")
+        Gp = _cols(Tgm, d, n, "T_grad_m") if Tgm is not None else [None] * 3
+        Ip = _cols(Tin, d, n, "T_init") if Tin is not None else [None] * 3
+        L.call("pfm_p2g", ctypes.byref(g), n, _pvec(pos, d, n, "pos"), _pvec(imp, d, n, "imp"),
+               *_cols(C, d, n, "C"), *Gp, *Ip, L.ptr(self.cell_start, dtype=torch.int32),
+               L.ptr(self.perm, dtype=torch.int32), *_u_ptrs(u, self.res), *Wp,
+               self.workspace.data_ptr(), self.workspace.numel(), _stream(stream))
+
+
+_BINNING_CACHE = {}
+
+
+def binning_for(res, n, device=None) -> Binning:
+    key = (tuple(res), int(n), str(device))
+    b = _BINNING_CACHE.get(key)
+    if b is None:
+        b = Binning(res, n, device)
+        _BINNING_CACHE[key] = b
+    return b
+
+
+# --------------------------------------------------------------------------
+# grid utilities
+# --------------------------------------------------------------------------
+def max_speed(res, u, out=None, stream=None):
+    """Device float32 tensor [1] with calc_max_speed's value."""
+    if out is None:
+        out = torch.empty(1, dtype=torch.float32, device=u[0].device)
+    g = L.make_grid(res, default_dx(res))
+    L.call("pfm_max_speed", ctypes.byref(g), *_u_ptrs(u, res), L.ptr(out, name="out"), _stream(stream))
+    return out
+
+
+def split_central(res, vc, u, stream=None):
+    g = L.make_grid(res, default_dx(res))
+    if tuple(vc.shape) != (len(res),) + tuple(res):
+        raise L.PfmError(f"central field must be (d, *res), got {tuple(vc.shape)}")
+    L.call("pfm_split_central", ctypes.byref(g), L.ptr(vc, name="vc"), *_u_ptrs(u, res), _stream(stream))
+
+
+def get_central(res, u, vc, stream=None):
+    g = L.make_grid(res, default_dx(res))
+    if tuple(vc.shape) != (len(res),) + tuple(res):
+        raise L.PfmError(f"central field must be (d, *res), got {tuple(vc.shape)}")
+    L.call("pfm_get_central", ctypes.byref(g), *_u_ptrs(u, res), L.ptr(vc, name="vc"), _stream(stream))
+
+
+def point_vortices(res, dx, centers, weights, vc, rad=0.02, strength=-0.01, stream=None):
+    import numpy as np
+
+    c = np.ascontiguousarray(np.asarray(centers, np.float32).reshape(-1, 2))
+    cx = np.ascontiguousarray(c[:, 0])
+    cy = np.ascontiguousarray(c[:, 1])
+    w = np.ascontiguousarray(np.asarray(weights, np.float32))
+    g = L.make_grid(res, dx)
+    L.call("pfm_point_vortices", ctypes.byref(g), len(c), cx.ctypes.data, cy.ctypes.data, w.ctypes.data,
+           float(rad), float(strength), L.ptr(vc, name="vc"), _stream(stream))
+
+
+def vortex_ring(res, dx, vc, w, rad, delta, c, ex, ey, num_samples=2000, stream=None):
+    import numpy as np
+
+    cc = np.asarray(c, np.float32)
+    xx = np.asarray(ex, np.float32)
+    yy = np.asarray(ey, np.float32)
+    g = L.make_grid(res, dx)
+    L.call("pfm_vortex_ring", ctypes.byref(g), float(w), float(rad), float(delta), cc.ctypes.data,
+           xx.ctypes.data, yy.ctypes.data, int(num_samples), L.ptr(vc, name="vc"), _stream(stream))
+
+
+_KE_SCRATCH = {}
+
+
+def kinetic_energy(res, u, stream=None) -> torch.Tensor:
+    """Device float64 tensor [1]: sum_a sum u_a^2 / |F_a|."""
+    dev = u[0].device
+    buf = _KE_SCRATCH.get(str(dev))
+    if buf is None:
+        buf = torch.empty(1 + 3 * 1024, dtype=torch.float64, device=dev)
+        _KE_SCRATCH[str(dev)] = buf
+    g = L.make_grid(res, default_dx(res))
+    L.call("pfm_kinetic_energy", ctypes.byref(g), *_u_ptrs(u, res), L.ptr(buf, dtype=torch.float64),
+           _stream(stream))
+    return buf[:1].clone()

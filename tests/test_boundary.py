@@ -1,0 +1,83 @@
+"""The C-ABI boundary: libpfm.so loads, exports every entry point that
+include/pfm.h declares, and the Python surface fails loudly without a GPU.
+No compute calls (CPU only)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pfm.h")
+LIB = os.path.join(ROOT, "particle-flow-maps_amd", "pfm", "libpfm.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pfm_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "pfm_rk4_particles" in names and "pfm_mg_poisson" in names and "pfm_p2g" in names
+    assert len(names) >= 30
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "build libpfm.so first (particle-flow-maps_amd/build.py)"
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, f"missing exports: {missing}"
+
+
+def test_python_binding_covers_header():
+    from pfm import _lib
+
+    assert set(_lib.exported_symbols()) == set(declared())
+
+
+def test_abi_version_and_error_text():
+    lib = ctypes.CDLL(LIB)
+    assert lib.pfm_abi_version() == 1
+    lib.pfm_last_error.restype = ctypes.c_char_p
+    # argument validation runs on the host and needs no device
+    from pfm import _lib
+
+    g = _lib.make_grid((8, 8, 2), 0.125)  # res 2 < 4 -> rejected
+    rc = lib.pfm_reset_identity_grid(ctypes.byref(g), 0, None, None, None)
+    assert rc == -1
+    assert b"res[2]" in lib.pfm_last_error()
+
+
+def test_workspace_size_query():
+    from pfm import _lib
+
+    L = _lib.load(require_gpu=False)
+    g = _lib.make_grid((16, 16, 16), 1 / 16)
+    ws = L.pfm_p2g_workspace_size(ctypes.byref(g), 16 ** 3 * 8)
+    assert ws >= 16 ** 3 * 8 * (4 + 4 + 15 * 4)
+
+
+def test_product_path_fails_loudly_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    import pfm
+
+    with pytest.raises(pfm._lib.PfmError):
+        pfm.load_library(require_gpu=True)
+    with pytest.raises(Exception):
+        pfm.MGPCG_3([[2, 2]] * 3, [16, 16, 16])
+
+
+def test_product_does_not_import_oracle():
+    """Nothing in the package may reach the CPU oracle."""
+    pkg = os.path.join(ROOT, "particle-flow-maps_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "from oracle" not in text, f
+                assert "libpfm_oracle" not in text, f

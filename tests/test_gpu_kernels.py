@@ -1,0 +1,301 @@
+"""GPU parity of the libpfm kernels against the CPU oracle on the same seeded
+inputs.  Bit-exact for integer work (seeding lattice, binning, permutation,
+cell lists, max speed) and for the fp32 passes written without contraction
+(identity resets, centre/face averaging, MG operator, smoothing, restriction,
+prolongation, V-cycle, divergence, pressure gradient); within stated
+tolerances for the B-spline kernels that use FMA and separable sums."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import affine_mac_field, rel_l2, smooth_mac_field
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def G(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+def _particles(oracle, res, ppa=2, seed=3, order=0):
+    d = len(res)
+    dx = 1.0 / res[1]
+    n = int(np.prod(res)) * ppa ** d
+    return oracle.seed_particles(res, ppa, dx / ppa, n, seed=seed, order=order), dx, n
+
+
+# ---------------------------------------------------------------- seeding
+@pytest.mark.parametrize("res", [(32, 32), (16, 12, 8), (10, 7)])
+@pytest.mark.parametrize("seed", [-1, 0, 7])
+@pytest.mark.parametrize("order", [0, 1])
+def test_seed_bit_exact(oracle, pfm, res, seed, order):
+    from pfm import core
+
+    d = len(res)
+    ppa = 2 if d == 3 else 2
+    dx = 1.0 / res[1]
+    n = int(np.prod(res)) * ppa ** d
+    want = oracle.seed_particles(res, ppa, dx / ppa, n, seed=seed, order=order)
+    pos = torch.empty((d, n), dtype=torch.float32, device=DEV)
+    core.seed_particles(res, ppa, np.float32(dx / ppa), pos, seed=seed, order=order)
+    np.testing.assert_array_equal(N(pos), want)
+
+
+# ---------------------------------------------------------------- binning
+@pytest.mark.parametrize("res", [(32, 32), (16, 12, 8)])
+def test_binning_bit_exact(oracle, pfm, res):
+    from pfm import core
+
+    d = len(res)
+    dx = 1.0 / res[1]
+    rng = np.random.default_rng(1)
+    n = int(np.prod(res)) * 6
+    ext = np.array(res) * dx
+    pos = (rng.uniform(-0.2 * dx, 1.0, size=(d, n)) * ext[:, None]).astype(np.float32)
+    cell, counts, start, perm = oracle.bin_particles(res, dx, pos)
+    b = core.Binning(res, n)
+    b.bin(G(pos), dx)
+    np.testing.assert_array_equal(N(b.cell_of), cell)
+    np.testing.assert_array_equal(N(b.cell_start), start)
+    np.testing.assert_array_equal(N(b.perm), perm)
+    cm = 12
+    ids = torch.empty(tuple(res) + (cm,), dtype=torch.int32, device=DEV)
+    cnt = torch.empty(tuple(res), dtype=torch.int32, device=DEV)
+    b.cell_lists(dx, ids, cnt)
+    np.testing.assert_array_equal(N(ids), oracle.cell_lists(res, counts, perm, start, cm))
+    np.testing.assert_array_equal(N(cnt).ravel(), counts)
+
+
+def test_binning_empty_and_crowded_cells(oracle, pfm):
+    """All particles in one cell (> cell_max and > the register sort size)."""
+    from pfm import core
+
+    res = (8, 8, 8)
+    dx = 1 / 8
+    n = 100
+    pos = np.full((3, n), 0.3, np.float32)
+    pos[0] += np.linspace(0, 0.01, n, dtype=np.float32)
+    cell, counts, start, perm = oracle.bin_particles(res, dx, pos)
+    b = core.Binning(res, n)
+    b.bin(G(pos), dx)
+    np.testing.assert_array_equal(N(b.perm), perm)
+    np.testing.assert_array_equal(N(b.cell_start), start)
+
+
+# ---------------------------------------------------------------- RK4
+@pytest.mark.parametrize("res", [(32, 32), (16, 16, 16), (12, 16, 10)])
+def test_rk4_particles(oracle, pfm, res):
+    from pfm import core
+
+    d = len(res)
+    pos, dx, n = _particles(oracle, res)
+    u = smooth_mac_field(res, seed=2)
+    rng = np.random.default_rng(3)
+    T = (np.eye(d)[:, :, None] + 0.1 * rng.normal(size=(d, d, n))).astype(np.float32)
+    dt = 0.5 * dx / 3.0
+    pg, Tg = G(pos), [G(T[m]) for m in range(d)]
+    core.rk4_particles(res, dx, pg, Tg, [G(x) for x in u], dt)
+    oracle.rk4_particles(res, dx, pos, T, u, dt)
+    assert np.max(np.abs(N(pg) - pos)) < 2e-6
+    for m in range(d):
+        assert rel_l2(N(Tg[m]), T[m]) < 1e-5
+
+
+def test_rk4_particles_2d_no_psi(oracle, pfm):
+    from pfm import core
+
+    res = (32, 32)
+    pos, dx, n = _particles(oracle, res)
+    u = smooth_mac_field(res, seed=4)
+    T = np.tile(np.eye(2, dtype=np.float32)[:, :, None], (1, 1, n)).copy()
+    pg, Tg = G(pos), [G(T[m]) for m in range(2)]
+    core.rk4_particles(res, dx, pg, Tg, [G(x) for x in u], 0.01, advect_psi=False)
+    oracle.rk4_particles(res, dx, pos, T, u, 0.01, advect_psi=False)
+    np.testing.assert_array_equal(N(pg), pos)
+    for m in range(2):
+        assert rel_l2(N(Tg[m]), T[m]) < 1e-5
+
+
+@pytest.mark.parametrize("res", [(32, 32), (16, 16, 16)])
+def test_rk4_grid_and_advect(oracle, pfm, res):
+    from pfm import core
+
+    d = len(res)
+    dx = 1.0 / res[1]
+    u = smooth_mac_field(res, seed=5)
+    ug = [G(x) for x in u]
+    dt = 0.02
+    psis, Ts = [], []
+    for a in range(d):
+        psi, T = oracle.reset_identity_grid(res, dx, a)
+        pg = torch.empty_like(G(psi))
+        tg = torch.empty_like(pg)
+        core.reset_identity_grid(res, dx, pg, tg)
+        np.testing.assert_array_equal(N(pg), psi)
+        np.testing.assert_array_equal(N(tg), T)
+        oracle.rk4_grid(res, dx, a, psi, T, u, 0.5 * dt)
+        core.rk4_grid(res, dx, pg, tg, ug, 0.5 * dt)
+        assert np.max(np.abs(N(pg) - psi)) < 2e-6
+        assert rel_l2(N(tg), T) < 1e-5
+        psis.append((psi, pg))
+        Ts.append((T, tg))
+    u1g = [torch.empty_like(x) for x in ug]
+    core.advect_u(res, dx, ug, u1g, [t[1] for t in Ts], [p[1] for p in psis])
+    for a in range(d):
+        u1 = np.empty_like(u[a])
+        oracle.advect_u(res, dx, a, u, u1, Ts[a][0], psis[a][0])
+        assert rel_l2(N(u1g[a]), u1) < 1e-5
+
+
+# ---------------------------------------------------------------- G2P & T
+@pytest.mark.parametrize("res", [(32, 32), (16, 16, 16)])
+def test_g2p(oracle, pfm, res):
+    from pfm import core
+
+    d = len(res)
+    pos, dx, n = _particles(oracle, res, seed=9)
+    u = smooth_mac_field(res, seed=6)
+    imp = np.empty((d, n), np.float32)
+    C = np.empty((d, d, n), np.float32)
+    oracle.g2p(res, dx, pos, u, imp, C)
+    ig = torch.empty((d, n), device=DEV)
+    Cg = [torch.empty((d, n), device=DEV) for _ in range(d)]
+    core.g2p(res, dx, G(pos), [G(x) for x in u], ig, Cg)
+    assert rel_l2(N(ig), imp) < 1e-6
+    for a in range(d):
+        assert rel_l2(N(Cg[a]), C[a]) < 1e-5
+    ig2 = torch.empty((d, n), device=DEV)
+    core.g2p(res, dx, G(pos), [G(x) for x in u], ig2, None)
+    np.testing.assert_array_equal(N(ig2), N(ig))
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_update_T_imp_gradm(oracle, pfm, d):
+    from pfm import core
+
+    rng = np.random.default_rng(10)
+    n = 5000
+    Tgm = rng.normal(size=(d, d, n)).astype(np.float32)
+    Tin = rng.normal(size=(d, d, n)).astype(np.float32)
+    C = rng.normal(size=(d, d, n)).astype(np.float32)
+    m0 = rng.normal(size=(d, n)).astype(np.float32)
+    want_T = oracle.update_T(Tgm, Tin)
+    out = [torch.empty((d, n), device=DEV) for _ in range(d)]
+    core.update_T([G(Tgm[m]) for m in range(d)], [G(Tin[m]) for m in range(d)], out)
+    for m in range(d):
+        assert np.max(np.abs(N(out[m]) - want_T[m])) < 1e-5
+    want_C, _ = oracle.p2g_prep(Tgm, None, C, None)
+    Cg = [torch.empty((d, n), device=DEV) for _ in range(d)]
+    core.update_particles_grad_m(Cg, [G(C[m]) for m in range(d)], [G(Tgm[m]) for m in range(d)])
+    for m in range(d):
+        assert np.max(np.abs(N(Cg[m]) - want_C[m])) < 1e-4
+    want_imp = oracle.update_imp(Tin, m0)
+    ig = torch.empty((d, n), device=DEV)
+    core.update_particles_imp([G(Tin[m]) for m in range(d)], G(m0), ig)
+    assert np.max(np.abs(N(ig) - want_imp)) < 1e-5
+    Tg = [G(Tin[m]) for m in range(d)]
+    core.reset_T_identity(Tg)
+    for m in range(d):
+        e = np.zeros((d, n), np.float32)
+        e[m] = 1
+        np.testing.assert_array_equal(N(Tg[m]), e)
+
+
+# ---------------------------------------------------------------- P2G
+@pytest.mark.parametrize("res", [(32, 32), (16, 16, 16), (12, 16, 10)])
+@pytest.mark.parametrize("fused", [False, True])
+def test_p2g(oracle, pfm, res, fused):
+    from pfm import core
+
+    d = len(res)
+    pos, dx, n = _particles(oracle, res, seed=12)
+    rng = np.random.default_rng(13)
+    pos = (pos + rng.uniform(-0.4, 0.4, size=pos.shape).astype(np.float32) * np.float32(dx)).astype(np.float32)
+    pos = np.clip(pos, 0, (np.array(res) * dx)[:, None] - 1e-6).astype(np.float32)
+    imp0 = rng.normal(size=(d, n)).astype(np.float32)
+    C0 = rng.normal(size=(d, d, n)).astype(np.float32)
+    if fused:
+        Tgm = (np.eye(d)[:, :, None] + 0.05 * rng.normal(size=(d, d, n))).astype(np.float32)
+        Tin = (np.eye(d)[:, :, None] + 0.05 * rng.normal(size=(d, d, n))).astype(np.float32)
+        C, imp = oracle.p2g_prep(Tgm, Tin, C0, imp0)
+    else:
+        C, imp = C0, imp0
+    u_ref, W_ref = oracle.p2g(res, dx, pos, imp, C, with_weights=True)
+    b = core.Binning(res, n)
+    ug = [torch.empty(core.face_shape(res, a), device=DEV) for a in range(d)]
+    Wg = [torch.empty(core.face_shape(res, a), device=DEV) for a in range(d)]
+    kw = {}
+    if fused:
+        kw = dict(Tgm=[G(Tgm[m]) for m in range(d)], Tin=[G(Tin[m]) for m in range(d)])
+    b.p2g(dx, G(pos), G(imp0 if fused else imp), [G(C0[m] if fused else C[m]) for m in range(d)], ug, W=Wg, **kw)
+    for a in range(d):
+        assert rel_l2(N(Wg[a]), W_ref[a]) < 1e-6
+        assert rel_l2(N(ug[a]), u_ref[a]) < 1e-5, (a, rel_l2(N(ug[a]), u_ref[a]))
+        # faces without particles stay zero, like the reference's filled field
+        np.testing.assert_array_equal(N(ug[a])[W_ref[a] == 0], 0)
+
+
+def test_p2g_deterministic(oracle, pfm):
+    from pfm import core
+
+    res = (16, 16, 16)
+    pos, dx, n = _particles(oracle, res, seed=14)
+    rng = np.random.default_rng(15)
+    imp = rng.normal(size=(3, n)).astype(np.float32)
+    C = rng.normal(size=(3, 3, n)).astype(np.float32)
+    outs = []
+    for _ in range(3):
+        b = core.Binning(res, n)
+        ug = [torch.empty(core.face_shape(res, a), device=DEV) for a in range(3)]
+        b.p2g(dx, G(pos), G(imp), [G(C[m]) for m in range(3)], ug)
+        outs.append([N(x) for x in ug])
+    for o in outs[1:]:
+        for a in range(3):
+            np.testing.assert_array_equal(o[a], outs[0][a])
+
+
+# ---------------------------------------------------------------- grid utils
+@pytest.mark.parametrize("res", [(32, 24), (16, 12, 8)])
+def test_grid_utils_bit_exact(oracle, pfm, res):
+    from pfm import core
+
+    d = len(res)
+    rng = np.random.default_rng(16)
+    vc = rng.normal(size=(d,) + res).astype(np.float32)
+    u = oracle.split_central(res, vc)
+    ug = [torch.empty(core.face_shape(res, a), device=DEV) for a in range(d)]
+    core.split_central(res, G(vc), ug)
+    for a in range(d):
+        np.testing.assert_array_equal(N(ug[a]), u[a])
+    back = torch.empty((d,) + res, device=DEV)
+    core.get_central(res, ug, back)
+    np.testing.assert_array_equal(N(back), oracle.get_central(res, u))
+    ms = core.max_speed(res, ug)
+    assert N(ms)[0] == np.float32(oracle.max_speed(res, u))
+    ke = core.kinetic_energy(res, ug)
+    assert abs(float(N(ke)[0]) - oracle.kinetic_energy(res, u)) < 1e-12 * oracle.kinetic_energy(res, u)
+
+
+def test_point_vortices_and_ring(oracle, pfm):
+    from pfm import core
+
+    res = (64, 64)
+    dx = 1 / 64
+    cen = [(0.25, 0.45), (0.25, 0.55), (0.35, 0.45), (0.35, 0.55)]
+    w = [1.0, -1.0, 1.0, -1.0]
+    want = oracle.point_vortices(res, dx, cen, w)
+    vc = torch.empty((2,) + res, device=DEV)
+    core.point_vortices(res, dx, cen, w, vc)
+    assert rel_l2(N(vc), want) < 1e-6
+    res3 = (16, 16, 16)
+    want3 = np.zeros((3,) + res3, np.float32)
+    oracle.vortex_ring(res3, 1 / 16, want3, 1.0, 0.15, 0.02, [0.5, 0.5, 0.3], [1, 0, 0], [0, 1, 0], 200)
+    vc3 = torch.zeros((3,) + res3, device=DEV)
+    core.vortex_ring(res3, 1 / 16, vc3, 1.0, 0.15, 0.02, [0.5, 0.5, 0.3], [1, 0, 0], [0, 1, 0], 200)
+    assert rel_l2(N(vc3), want3) < 1e-5

@@ -1,0 +1,121 @@
+"""GPU parity of the MGPCG solver: the stencil passes bit-exact against the
+fp32 oracle, the solve within tolerance of the oracle and of the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import rel_l2, smooth_mac_field
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def G(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("N_", [(32, 32), (16, 16, 16), (32, 16, 16), (64, 64)])
+def test_mg_passes_bit_exact(oracle, pfm, N_):
+    d = len(N_)
+    mg_o = oracle.MG(N_, base_level=3)
+    mg_g = pfm.MGPCG([[2, 2]] * d, list(N_), dim=d, base_level=3)
+    assert mg_g.n_mg_levels == mg_o.n_levels
+    rng = np.random.default_rng(20)
+    p = rng.normal(size=N_).astype(np.float32)
+    Ap = torch.empty(N_, device=DEV)
+    mg_g.get_Ap(G(p), Ap)
+    np.testing.assert_array_equal(N(Ap), mg_o.apply_A(0, p))
+    # one colour of RB-GS, restriction and prolongation on level 0
+    r = rng.normal(size=N_).astype(np.float32)
+    z = rng.normal(size=N_).astype(np.float32)
+    mg_o.field("r", 0)[...] = r
+    mg_o.field("z", 0)[...] = z
+    mg_g.r[0].copy_(G(r))
+    mg_g.z[0].copy_(G(z))
+    for phase in (0, 1, 1, 0):
+        mg_o.smooth(0, phase)
+        mg_g.smooth(0, phase)
+    np.testing.assert_array_equal(N(mg_g.z[0]), mg_o.field("z", 0))
+    mg_o.restrict(0)
+    mg_g.restrict(0)
+    np.testing.assert_array_equal(N(mg_g.r[1]), mg_o.field("r", 1))
+    zc = rng.normal(size=mg_o.level_shape(1)).astype(np.float32)
+    mg_o.field("z", 1)[...] = zc
+    mg_g.z[1].copy_(G(zc))
+    mg_o.prolongate(0)
+    mg_g.prolongate(0)
+    np.testing.assert_array_equal(N(mg_g.z[0]), mg_o.field("z", 0))
+    # the whole V-cycle (bottom solve in LDS)
+    r = rng.normal(size=N_).astype(np.float32)
+    mg_g.r[0].copy_(G(r))
+    mg_g.apply_preconditioner()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(N(mg_g.z[0]), mg_o.vcycle(r))
+
+
+@pytest.mark.parametrize("N_,tol_mode,tol", [((64, 64), "abs", 1e-12), ((32, 32, 32), "rel", 1e-6),
+                                             ((16, 16, 16), "abs", 1e-12), ((64, 32, 32), "rel", 1e-7)])
+def test_poisson_matches_oracle(oracle, pfm, N_, tol_mode, tol):
+    d = len(N_)
+    u = smooth_mac_field(N_, seed=21)
+    ug = [G(x) for x in u]
+    mg_g = pfm.MGPCG([[2, 2]] * d, list(N_), dim=d, base_level=3)
+    it_g = mg_g.Poisson(*ug, tol=tol, tol_mode=tol_mode)
+    mg_o = oracle.MG(N_, base_level=3)
+    uo = [x.copy() for x in u]
+    it_o, rtr, conv = mg_o.poisson(uo, 400, tol, 0 if tol_mode == "abs" else 1)
+    assert conv and mg_g.converged
+    assert abs(it_g - it_o) <= 1
+    for a in range(d):
+        assert rel_l2(N(ug[a]), uo[a]) < 1e-5
+    p_g = N(mg_g.p).astype(np.float64)
+    p_o = mg_o.field("x").astype(np.float64)
+    assert rel_l2(p_g - p_g.mean(), p_o - p_o.mean()) < 1e-5
+
+
+def test_poisson_vs_fp64_oracle(oracle, pfm):
+    N_ = (32, 32, 32)
+    u = smooth_mac_field(N_, seed=22)
+    mg_o = oracle.MG(N_, base_level=3)
+    mg_o.apply_bc(u)
+    mg_o.divergence(u)
+    div = mg_o.field("u_div").copy()
+    x64, it, conv = oracle.poisson_fp64_solve(div.astype(np.float64))
+    assert conv
+    ug = [G(x) for x in u]
+    mg_g = pfm.MGPCG([[2, 2]] * 3, list(N_), dim=3, base_level=3)
+    mg_g.Poisson(*ug, tol=1e-6, tol_mode="rel")
+    p = N(mg_g.p).astype(np.float64)
+    assert rel_l2(p - p.mean(), x64 - x64.mean()) < 1e-5
+
+
+def test_poisson_not_converged_is_reported(oracle, pfm):
+    N_ = (32, 32, 32)
+    u = [G(x) for x in smooth_mac_field(N_, seed=23)]
+    mg = pfm.MGPCG([[2, 2]] * 3, list(N_), dim=3, base_level=3)
+    it = mg.Poisson(*u, max_iters=1, tol=1e-30)
+    assert it == 1 and not mg.converged
+
+
+def test_solver_api_pieces(oracle, pfm):
+    """divergence / init / solve / get_result / subtract_grad_p in the
+    reference order equal Poisson."""
+    N_ = (32, 32)
+    u = smooth_mac_field(N_, seed=24)
+    a = [G(x) for x in u]
+    b = [G(x) for x in u]
+    m1 = pfm.MGPCG_2([[2, 2], [2, 2]], list(N_))
+    m1.Poisson(*a)
+    m2 = pfm.MGPCG_2([[2, 2], [2, 2]], list(N_))
+    m2.apply_bc(*b)
+    m2.divergence(*b)
+    m2.solve_pressure_MGPCG()
+    m2.subtract_grad_p(*b)
+    m2.apply_bc(*b)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(N(x), N(y))
