@@ -148,14 +148,39 @@ def cpu_baseline(seconds_budget=15.0):
                       f"({osim.n} particles), {steps} full steps after 1 warm-up step, {el:.1f} s"}
 
 
+def traffic_probe(sim, torch):
+    """Kernels for HBM-traffic counters: a 1 GiB device copy (known bytes:
+    1 GiB read + 1 GiB written) and one dispatch of each step kernel."""
+    from pfm import core
+
+    for _ in range(2):
+        sim.step()
+    src = torch.empty(2 ** 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = 0.5 * sim.last_dt
+    core.rk4_particles(sim.res, sim.dx, sim.pos, sim.T_grad_m, sim.u, dt, True)
+    sim.binning.bin(sim.pos, sim.dx, force=True)
+    sim.binning.p2g(sim.dx, sim.pos, sim.init_imp, sim.init_C, sim.u, W=sim.p2g_weight, Tgm=sim.T_grad_m,
+                    Tin=sim.T_init)
+    core.g2p(sim.res, sim.dx, sim.pos, sim.u, None, sim.init_C)
+    torch.cuda.synchronize()
+    print(f"# traffic probe done: copy {src.numel() * 4} B, {sim.n} particles", file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2, help="instrumented steps after the timed ones")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: one plain 1 GiB copy (the counter scale reference) "
+                         "then one launch of each step kernel after two steps; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the step shards by slab "
@@ -176,6 +201,10 @@ def main():
     t_setup = time.perf_counter() - t_setup
     print(f"# setup {t_setup:.1f}s, {sim.n} particles, {torch.cuda.memory_allocated() / 2**30:.1f} GiB",
           file=sys.stderr, flush=True)
+
+    if args.traffic_probe:
+        traffic_probe(sim, torch)
+        return
 
     for _ in range(args.warmup):
         sim.step()
@@ -199,21 +228,29 @@ def main():
     timer.flush()
     phases = {k: round(v / timer.count[k] * (2 if k == "poisson" else 1), 3) for k, v in timer.total.items()}
 
-    # standalone Poisson solve at the step's tolerance (MGPCG cells/s)
+    # standalone Poisson solve (MGPCG cells/s): divergence of a seeded
+    # 64-mode Fourier velocity, fp32 MGPCG to a 1e-6 relative residual
     from pfm import core
+    from pfm.workloads import fourier_velocity
 
-    u_save = [x.clone() for x in sim.u]
+    u_f = fourier_velocity(res, seed=0)
+    work = [x.clone() for x in u_f]
+    sim.solver.Poisson(*work, tol=1e-6, tol_mode="rel")  # warm
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
     reps = 3
     pits = []
+    t_poisson = 0.0
     for _ in range(reps):
         for a in range(3):
-            sim.u[a].copy_(u_save[a])
-        pits.append(sim.project())
-    torch.cuda.synchronize()
-    t_poisson = (time.perf_counter() - t1) / reps
+            work[a].copy_(u_f[a])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        pits.append(sim.solver.Poisson(*work, tol=1e-6, tol_mode="rel"))
+        torch.cuda.synchronize()
+        t_poisson += time.perf_counter() - t1
+    t_poisson /= reps
     cells = math.prod(res)
+    del u_f, work
 
     # roofline of the particle RK4 kernel: 96 B/particle compulsory traffic
     ev_a = torch.cuda.Event(enable_timing=True)

@@ -351,6 +351,14 @@ class MG:
         self._f("get_result")(self.h, _p(out))
         return out
 
+    def apply_bc(self, u):
+        """MGPCG_3/MGPCG_2.apply_bc on float32 face arrays in place."""
+        self._f("apply_bc")(self.h, *_u3(u))
+
+    def divergence(self, u):
+        """divergence -> the u_div field (3D/mgpcg.py:380-390)."""
+        self._f("divergence")(self.h, *_u3(u))
+
     def poisson(self, u, max_iters=400, tol=1e-12, tol_mode=0):
         """Poisson on float32 face arrays in place; returns (iters, rTr, converged)."""
         rtr = ctypes.c_double()

@@ -39,10 +39,17 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+# Sources whose fp32 passes must equal a sequential restatement bit for bit:
+# HIP's __fmul_rn/__fadd_rn are plain operators, so only a file-wide
+# -ffp-contract=off keeps the compiler from fusing them into FMAs.
+NO_CONTRACT = {"mgpcg.hip"}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD_DIR, os.path.basename(src) + ".o")
     lang = ["-x", "hip"] if src.endswith((".hip", ".cpp")) else []
-    cmd = [hipcc(), *lang, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    extra = ["-ffp-contract=off"] if src in NO_CONTRACT else []
+    cmd = [hipcc(), *lang, *CFLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

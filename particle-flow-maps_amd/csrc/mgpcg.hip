@@ -144,13 +144,16 @@ __global__ void __launch_bounds__(kBlock) k_restrict(LvlDev Lf, LvlDev Lc, int64
     int C[3];
     cell_ijk(Lc, qc, C);
     const float scale = (float)Lf.dim - 1.0f;
-    const int kk = Lf.dim == 3 ? 2 : 1;
+    // children 2C, 2C+1 per axis; the last coarse cell of an odd fine extent
+    // also owns the trailing fine cell (fine I maps to min(I/2, coarse-1))
+    int hi[3];
+    for (int a = 0; a < 3; ++a) hi[a] = (C[a] == Lc.d[a] - 1) ? Lf.d[a] - 1 - 2 * C[a] : 1;
+    if (Lf.dim == 2) hi[2] = 0;
     float acc = 0.0f;
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-            for (int c = 0; c < kk; ++c) {
+    for (int a = 0; a <= hi[0]; ++a)
+        for (int b = 0; b <= hi[1]; ++b)
+            for (int c = 0; c <= hi[2]; ++c) {
                 int I[3] = {2 * C[0] + a, 2 * C[1] + b, Lf.dim == 3 ? 2 * C[2] + c : 0};
-                if (I[0] >= Lf.d[0] || I[1] >= Lf.d[1] || I[2] >= Lf.d[2]) continue;
                 const int64_t q = ((int64_t)I[0] * Lf.d[1] + I[1]) * Lf.d[2] + I[2];
                 const float res = __fsub_rn(r[q], __fsub_rn(__fmul_rn(nfn(Lf, I), z[q]), nsum(Lf, z, q, I)));
                 acc = __fadd_rn(acc, __fdiv_rn(__fmul_rn(res, 1.0f), scale));
@@ -166,7 +169,9 @@ __global__ void __launch_bounds__(kBlock) k_prolongate(LvlDev Lf, LvlDev Lc, int
     if (q >= n) return;
     int I[3];
     cell_ijk(Lf, q, I);
-    const int64_t qc = ((int64_t)(I[0] / 2) * Lc.d[1] + I[1] / 2) * Lc.d[2] + (Lf.dim == 3 ? I[2] / 2 : 0);
+    const int c0 = min(I[0] / 2, Lc.d[0] - 1), c1 = min(I[1] / 2, Lc.d[1] - 1);
+    const int c2 = Lf.dim == 3 ? min(I[2] / 2, Lc.d[2] - 1) : 0;
+    const int64_t qc = ((int64_t)c0 * Lc.d[1] + c1) * Lc.d[2] + c2;
     z[q] = __fadd_rn(z[q], zc[qc]);
 }
 
