@@ -158,7 +158,7 @@ def traffic_probe(sim, torch):
     src = torch.empty(2 ** 28, dtype=torch.float32, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
-    dst.copy_(src)
+    torch.neg(src, out=dst)  # the reference stream: reads 1 GiB, writes 1 GiB
     torch.cuda.synchronize()
     dt = 0.5 * sim.last_dt
     core.rk4_particles(sim.res, sim.dx, sim.pos, sim.T_grad_m, sim.u, dt, True)
@@ -168,6 +168,30 @@ def traffic_probe(sim, torch):
     core.g2p(sim.res, sim.dx, sim.pos, sim.u, None, sim.init_C)
     torch.cuda.synchronize()
     print(f"# traffic probe done: copy {src.numel() * 4} B, {sim.n} particles", file=sys.stderr)
+
+
+def summarize(db_path, top=25):
+    """Text summary of a rocprofv3 results database (kernel trace or PMC)."""
+    import sqlite3
+
+    db = sqlite3.connect(db_path)
+    cur = db.cursor()
+    lines = []
+    rows = cur.execute("select name, count(*), sum(duration), avg(duration) from kernels "
+                       "group by name order by sum(duration) desc").fetchall()
+    total = sum(r[2] for r in rows) or 1
+    pmc = cur.execute("select counter_name, name, count(*), avg(counter_value), max(counter_value) "
+                      "from pmc_events group by counter_name, name").fetchall()
+    if pmc:
+        lines.append(f"{'counter':<12} {'dispatches':>10} {'avg KiB/disp':>14} {'max KiB/disp':>14}  kernel")
+        for cname, kname, cnt, avg, mx in sorted(pmc, key=lambda r: -r[4])[:top]:
+            lines.append(f"{cname:<12} {cnt:>10} {avg:>14.1f} {mx:>14.1f}  {kname[:100]}")
+        return "\n".join(lines)
+    lines.append(f"{'calls':>7} {'total ms':>10} {'avg us':>10} {'%':>6}  kernel")
+    for name, cnt, tot, avg in rows[:top]:
+        lines.append(f"{cnt:>7} {tot / 1e6:>10.3f} {avg / 1e3:>10.2f} {100 * tot / total:>6.2f}  {name[:110]}")
+    lines.append(f"total GPU kernel time {total / 1e6:.3f} ms over {sum(r[1] for r in rows)} dispatches")
+    return "\n".join(lines)
 
 
 def main():
@@ -181,7 +205,11 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: one plain 1 GiB copy (the counter scale reference) "
                          "then one launch of each step kernel after two steps; prints no JSON")
+    ap.add_argument("--summarize", metavar="DB", help="print a text summary of a rocprofv3 results db and exit")
     args = ap.parse_args()
+    if args.summarize:
+        print(summarize(args.summarize))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the step shards by slab "
               "decomposition but this build measures a single MI355X.", file=sys.stderr)

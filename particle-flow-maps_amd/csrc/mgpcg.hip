@@ -11,6 +11,7 @@
 //     recentring mean are formed on the device, so the only host round trip
 //     per iteration is one 8-byte read of r.r for the convergence test;
 //   - the coarsest level's 50 red/black sweeps run in one workgroup out of LDS.
+#include <algorithm>
 #include <new>
 
 #include "pfm_host.h"
@@ -49,7 +50,7 @@ struct MGHandle {
     float* x;
     float* p;
     float* Ap;
-    double* partial;     // 2 * kMgRedBlocks
+    double* partial;     // max(2 * kMgRedBlocks, blocks of a level-0 stencil launch)
     MGScalars* sc;       // device
     double* host_rtr;    // pinned
 };
@@ -65,6 +66,40 @@ __device__ __forceinline__ void cell_ijk(const LvlDev& L, int64_t q, int I[3]) {
     I[2] = (int)(q % L.d[2]);
     I[1] = (int)((q / L.d[2]) % L.d[1]);
     I[0] = (int)(q / ((int64_t)L.d[2] * L.d[1]));
+}
+
+// Thread -> cell mapping for the stencil kernels: a 3D launch (64 x 4
+// threads per block) with the fastest array axis on threadIdx.x, so no
+// integer division is needed.  3D: (i, j, k) = (block.z, y, x); 2D: (i, j) =
+// (y, x).  With `pairs`, x counts pairs of cells along the fast axis and the
+// thread takes the member whose colour (I.sum() & 1) equals `phase`.
+constexpr int kTx = 64, kTy = 4;
+
+__device__ __forceinline__ bool cell_of_thread(const LvlDev& L, int phase, bool pairs, int I[3]) {
+    const int fx = blockIdx.x * kTx + threadIdx.x;
+    const int fy = blockIdx.y * kTy + threadIdx.y;
+    if (L.dim == 3) {
+        I[0] = blockIdx.z;
+        I[1] = fy;
+        I[2] = pairs ? 2 * fx + ((I[0] + I[1] + phase) & 1) : fx;
+    } else {
+        I[0] = fy;
+        I[1] = pairs ? 2 * fx + ((I[0] + phase) & 1) : fx;
+        I[2] = 0;
+    }
+    return I[0] < L.d[0] && I[1] < L.d[1] && I[2] < L.d[2];
+}
+
+static inline dim3 cell_grid(const int d[3], int dim, bool pairs) {
+    const int fast = dim == 3 ? d[2] : d[1];
+    const int mid = dim == 3 ? d[1] : d[0];
+    const int nx = pairs ? (fast + 1) / 2 : fast;
+    return dim3((unsigned)((nx + kTx - 1) / kTx), (unsigned)((mid + kTy - 1) / kTy),
+                (unsigned)(dim == 3 ? d[0] : 1));
+}
+
+__device__ __forceinline__ int64_t cell_index(const LvlDev& L, const int I[3]) {
+    return ((int64_t)I[0] * L.d[1] + I[1]) * L.d[2] + I[2];
 }
 
 // num_fluid_neighbors: 3D/mgpcg.py:106-115.
@@ -95,54 +130,58 @@ __global__ void __launch_bounds__(kBlock) k_fill_f32(float* p, int64_t n, float 
         p[i] = v;
 }
 
-// smooth: 3D/mgpcg.py:165-172 (one colour).
-__global__ void __launch_bounds__(kBlock) k_smooth(LvlDev L, int64_t n, float* __restrict__ z,
+// smooth: 3D/mgpcg.py:165-172 (one colour; one thread per cell of that colour).
+__global__ void __launch_bounds__(kBlock) k_smooth(LvlDev L, float* __restrict__ z,
                                                    const float* __restrict__ r, int phase) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
     int I[3];
-    cell_ijk(L, q, I);
-    if (((I[0] + I[1] + I[2]) & 1) != phase) return;
+    if (!cell_of_thread(L, phase, true, I)) return;
+    const int64_t q = cell_index(L, I);
     z[q] = __fdiv_rn(__fadd_rn(r[q], nsum(L, z, q, I)), nfn(L, I));
 }
 
-// compute_Ap (3D/mgpcg.py:117-122) fused with the p.Ap reduction.
-__global__ void __launch_bounds__(kBlock) k_ap_dot(LvlDev L, int64_t n, const float* __restrict__ p,
+// compute_Ap (3D/mgpcg.py:117-122) fused with the p.Ap reduction; one
+// partial per block, blocks numbered linearly.
+__global__ void __launch_bounds__(kBlock) k_ap_dot(LvlDev L, const float* __restrict__ p,
                                                    float* __restrict__ Ap, double* __restrict__ partial) {
     __shared__ double sh[kBlock / 64];
     double s = 0.0;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        int I[3];
-        cell_ijk(L, q, I);
+    int I[3];
+    if (cell_of_thread(L, 0, false, I)) {
+        const int64_t q = cell_index(L, I);
         const float pq = p[q];
         const float a = __fsub_rn(__fmul_rn(nfn(L, I), pq), nsum(L, p, q, I));
         Ap[q] = a;
-        s += (double)pq * (double)a;
+        s = (double)pq * (double)a;
     }
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    const int tid = threadIdx.y * kTx + threadIdx.x;
+    // block_sum expects a 1D block index
+    s = wave_sum(s);
+    if ((tid & 63) == 0) sh[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += sh[w];
+        partial[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
+    }
 }
 
 // get_Ap for tests (no reduction).
-__global__ void __launch_bounds__(kBlock) k_apply_A(LvlDev L, int64_t n, const float* __restrict__ p,
+__global__ void __launch_bounds__(kBlock) k_apply_A(LvlDev L, const float* __restrict__ p,
                                                     float* __restrict__ Ap) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
     int I[3];
-    cell_ijk(L, q, I);
+    if (!cell_of_thread(L, 0, false, I)) return;
+    const int64_t q = cell_index(L, I);
     Ap[q] = __fsub_rn(__fmul_rn(nfn(L, I), p[q]), nsum(L, p, q, I));
 }
 
 // restrict (3D/mgpcg.py:152-158) as a gather over the 2^d children.
-__global__ void __launch_bounds__(kBlock) k_restrict(LvlDev Lf, LvlDev Lc, int64_t nc,
+__global__ void __launch_bounds__(kBlock) k_restrict(LvlDev Lf, LvlDev Lc,
                                                      const float* __restrict__ z,
                                                      const float* __restrict__ r,
                                                      float* __restrict__ rc) {
-    const int64_t qc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (qc >= nc) return;
     int C[3];
-    cell_ijk(Lc, qc, C);
+    if (!cell_of_thread(Lc, 0, false, C)) return;
+    const int64_t qc = cell_index(Lc, C);
     const float scale = (float)Lf.dim - 1.0f;
     // children 2C, 2C+1 per axis; the last coarse cell of an odd fine extent
     // also owns the trailing fine cell (fine I maps to min(I/2, coarse-1))
@@ -162,13 +201,11 @@ __global__ void __launch_bounds__(kBlock) k_restrict(LvlDev Lf, LvlDev Lc, int64
 }
 
 // prolongate: 3D/mgpcg.py:160-163.
-__global__ void __launch_bounds__(kBlock) k_prolongate(LvlDev Lf, LvlDev Lc, int64_t n,
-                                                       float* __restrict__ z,
+__global__ void __launch_bounds__(kBlock) k_prolongate(LvlDev Lf, LvlDev Lc, float* __restrict__ z,
                                                        const float* __restrict__ zc) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
     int I[3];
-    cell_ijk(Lf, q, I);
+    if (!cell_of_thread(Lf, 0, false, I)) return;
+    const int64_t q = cell_index(Lf, I);
     const int c0 = min(I[0] / 2, Lc.d[0] - 1), c1 = min(I[1] / 2, Lc.d[1] - 1);
     const int c2 = Lf.dim == 3 ? min(I[2] / 2, Lc.d[2] - 1) : 0;
     const int64_t qc = ((int64_t)c0 * Lc.d[1] + c1) * Lc.d[2] + c2;
@@ -256,11 +293,11 @@ enum FinalizeMode { kFinMean = 0, kFinR0 = 1, kFinZTr = 2, kFinAlpha = 3, kFinRT
 
 // Fixed-order reduction of the per-block partials and the scalar recurrences
 // of solve() (3D/mgpcg.py:229-275), all in double.
-__global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ partial, int mode,
+__global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ partial, int nparts, int mode,
                                                      double eps, double ncell, MGScalars* sc) {
     __shared__ double sh[kBlock / 64];
     double s = 0.0;
-    for (int b = threadIdx.x; b < kMgRedBlocks; b += blockDim.x) s += partial[b];
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += partial[b];
     s = block_sum(s, sh);
     double s1 = 0.0;
     if (mode == kFinRTr) {
@@ -306,16 +343,15 @@ __global__ void k_reset_scalars(MGScalars* sc) {
 
 // divergence (3D/mgpcg.py:380-390) fused with init(r, k) (:71-79).
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_div_init(LvlDev L, int64_t n, const float* __restrict__ ux,
+__global__ void __launch_bounds__(kBlock) k_div_init(LvlDev L, const float* __restrict__ ux,
                                                      const float* __restrict__ uy,
                                                      const float* __restrict__ uz, float k,
                                                      float* __restrict__ r, float* __restrict__ z,
                                                      float* __restrict__ Ap, float* __restrict__ p,
                                                      float* __restrict__ x) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
     int I[3];
-    cell_ijk(L, q, I);
+    if (!cell_of_thread(L, 0, false, I)) return;
+    const int64_t q = cell_index(L, I);
     const int rx = L.d[0], ry = L.d[1], rz = L.d[2];
     (void)rx;
     float div;
@@ -356,17 +392,13 @@ __global__ void __launch_bounds__(kBlock) k_init_rhs(int64_t n, const float* __r
 
 // subtract_grad_p: 3D/mgpcg.py:392-418 (pressure outside the domain = 0).
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_subtract_grad_p(LvlDev L, int axis, int64_t nf,
+__global__ void __launch_bounds__(kBlock) k_subtract_grad_p(LvlDev L, LvlDev F, int axis,
                                                             const float* __restrict__ pr,
                                                             float* __restrict__ u) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
-    int S[3] = {L.d[0], L.d[1], L.d[2]};
-    S[axis] += 1;
+    // F describes the face array of `axis` (extent d[axis] + 1)
     int I[3];
-    I[2] = (int)(f % S[2]);
-    I[1] = (int)((f / S[2]) % S[1]);
-    I[0] = (int)(f / ((int64_t)S[2] * S[1]));
+    if (!cell_of_thread(F, 0, false, I)) return;
+    const int64_t f = cell_index(F, I);
     int R[3] = {I[0], I[1], I[2]}, Lf[3] = {I[0], I[1], I[2]};
     Lf[axis] -= 1;
     const float prv = R[axis] >= L.d[axis]
@@ -418,26 +450,27 @@ static int fill(float* p, int64_t n, float v, hipStream_t s) {
     return PFM_OK;
 }
 
+static const dim3 kCellBlock(kTx, kTy, 1);
+
 static int smooth(MGHandle* h, int l, int phase, hipStream_t s) {
     LvlDev L = lvl_dev(h, l);
-    hipLaunchKernelGGL(k_smooth, dim3(grid_for(h->lv[l].n)), dim3(kBlock), 0, s, L, h->lv[l].n, h->z[l],
-                       h->r[l], phase);
+    hipLaunchKernelGGL(k_smooth, cell_grid(L.d, h->dim, true), kCellBlock, 0, s, L, h->z[l], h->r[l], phase);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
 
 static int restrict_level(MGHandle* h, int l, hipStream_t s) {
     LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
-    hipLaunchKernelGGL(k_restrict, dim3(grid_for(h->lv[l + 1].n)), dim3(kBlock), 0, s, Lf, Lc,
-                       h->lv[l + 1].n, h->z[l], h->r[l], h->r[l + 1]);
+    hipLaunchKernelGGL(k_restrict, cell_grid(Lc.d, h->dim, false), kCellBlock, 0, s, Lf, Lc, h->z[l],
+                       h->r[l], h->r[l + 1]);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
 
 static int prolongate_level(MGHandle* h, int l, hipStream_t s) {
     LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
-    hipLaunchKernelGGL(k_prolongate, dim3(grid_for(h->lv[l].n)), dim3(kBlock), 0, s, Lf, Lc, h->lv[l].n,
-                       h->z[l], h->z[l + 1]);
+    hipLaunchKernelGGL(k_prolongate, cell_grid(Lf.d, h->dim, false), kCellBlock, 0, s, Lf, Lc, h->z[l],
+                       h->z[l + 1]);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -481,11 +514,16 @@ static int vcycle(MGHandle* h, hipStream_t s) {
     return PFM_OK;
 }
 
-static int finalize(MGHandle* h, int mode, double eps, hipStream_t s) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, h->partial, mode, eps,
+static int finalize(MGHandle* h, int mode, double eps, hipStream_t s, int nparts = kMgRedBlocks) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, h->partial, nparts, mode, eps,
                        (double)h->lv[0].n, h->sc);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
+}
+
+static int ap_dot_parts(const MGHandle* h) {
+    const dim3 gd = cell_grid(h->lv[0].d, h->dim, false);
+    return (int)(gd.x * gd.y * gd.z);
 }
 
 static int dot(MGHandle* h, const float* a, const float* b, hipStream_t s) {
@@ -532,9 +570,10 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
     double rtr = 0.0;
     bool converged = false;
     while (max_iters == -1 || it < max_iters) {
-        hipLaunchKernelGGL(k_ap_dot, dim3(rb), dim3(kBlock), 0, s, L0, n, h->p, h->Ap, h->partial);
+        hipLaunchKernelGGL(k_ap_dot, cell_grid(L0.d, h->dim, false), kCellBlock, 0, s, L0, h->p, h->Ap,
+                           h->partial);
         PFM_LAUNCH_CHECK();
-        PFM_TRY(finalize(h, kFinAlpha, eps, s));
+        PFM_TRY(finalize(h, kFinAlpha, eps, s, ap_dot_parts(h)));
         hipLaunchKernelGGL(k_update_xr, dim3(rb), dim3(kBlock), 0, s, n, h->x, h->r[0], h->p, h->Ap, h->sc,
                            h->partial);
         PFM_LAUNCH_CHECK();
@@ -598,13 +637,13 @@ static int apply_bc(MGHandle* h, float* const* u, hipStream_t s) {
 
 static int div_init(MGHandle* h, const float* const* u, float k, hipStream_t s) {
     LvlDev L0 = lvl_dev(h, 0);
-    const int64_t n = h->lv[0].n;
+    const dim3 gd = cell_grid(L0.d, h->dim, false);
     if (h->dim == 3)
-        hipLaunchKernelGGL(k_div_init<3>, dim3(grid_for(n)), dim3(kBlock), 0, s, L0, n, u[0], u[1], u[2], k,
-                           h->r[0], h->z[0], h->Ap, h->p, h->x);
+        hipLaunchKernelGGL(k_div_init<3>, gd, kCellBlock, 0, s, L0, u[0], u[1], u[2], k, h->r[0], h->z[0],
+                           h->Ap, h->p, h->x);
     else
-        hipLaunchKernelGGL(k_div_init<2>, dim3(grid_for(n)), dim3(kBlock), 0, s, L0, n, u[0], u[1], u[2], k,
-                           h->r[0], h->z[0], h->Ap, h->p, h->x);
+        hipLaunchKernelGGL(k_div_init<2>, gd, kCellBlock, 0, s, L0, u[0], u[1], u[2], k, h->r[0], h->z[0],
+                           h->Ap, h->p, h->x);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -612,12 +651,13 @@ static int div_init(MGHandle* h, const float* const* u, float k, hipStream_t s) 
 static int subtract_grad_p(MGHandle* h, float* const* u, hipStream_t s) {
     LvlDev L0 = lvl_dev(h, 0);
     for (int a = 0; a < h->dim; ++a) {
-        int64_t nf = 1;
-        for (int c = 0; c < h->dim; ++c) nf *= (int64_t)(h->N[c] + (c == a ? 1 : 0));
+        LvlDev F = L0;
+        F.d[a] += 1;
+        const dim3 gd = cell_grid(F.d, h->dim, false);
         if (h->dim == 3)
-            hipLaunchKernelGGL(k_subtract_grad_p<3>, dim3(grid_for(nf)), dim3(kBlock), 0, s, L0, a, nf, h->x, u[a]);
+            hipLaunchKernelGGL(k_subtract_grad_p<3>, gd, kCellBlock, 0, s, L0, F, a, h->x, u[a]);
         else
-            hipLaunchKernelGGL(k_subtract_grad_p<2>, dim3(grid_for(nf)), dim3(kBlock), 0, s, L0, a, nf, h->x, u[a]);
+            hipLaunchKernelGGL(k_subtract_grad_p<2>, gd, kCellBlock, 0, s, L0, F, a, h->x, u[a]);
         PFM_LAUNCH_CHECK();
     }
     return PFM_OK;
@@ -689,7 +729,8 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     if (hipMalloc(&h->x, b0) != hipSuccess) return fail("x");
     if (hipMalloc(&h->p, b0) != hipSuccess) return fail("p");
     if (hipMalloc(&h->Ap, b0) != hipSuccess) return fail("Ap");
-    if (hipMalloc(&h->partial, sizeof(double) * 2 * kMgRedBlocks) != hipSuccess) return fail("partial");
+    const size_t nparts = std::max<size_t>(2 * kMgRedBlocks, (size_t)ap_dot_parts(h));
+    if (hipMalloc(&h->partial, sizeof(double) * nparts) != hipSuccess) return fail("partial");
     if (hipMalloc(&h->sc, sizeof(MGScalars)) != hipSuccess) return fail("scalars");
     if (hipHostMalloc(&h->host_rtr, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess)
         return fail("pinned scalars");
@@ -824,8 +865,7 @@ int pfm_mg_apply_A(void* hv, const float* p, float* Ap, pfm_stream_t stream) {
     MGHandle* h = (MGHandle*)hv;
     PFM_CHECK_ARG(h && p && Ap, "null argument");
     LvlDev L0 = lvl_dev(h, 0);
-    hipLaunchKernelGGL(k_apply_A, dim3(grid_for(h->lv[0].n)), dim3(kBlock), 0, as_stream(stream), L0,
-                       h->lv[0].n, p, Ap);
+    hipLaunchKernelGGL(k_apply_A, cell_grid(L0.d, h->dim, false), kCellBlock, 0, as_stream(stream), L0, p, Ap);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

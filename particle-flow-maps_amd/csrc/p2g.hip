@@ -8,11 +8,12 @@
 // cell on every run.
 //
 // P2G: instead of the reference's float atomics (81 face splats per particle),
-// each face gathers from the particles of its 4x3x3 neighbourhood of cells in
-// a fixed order (cells lexicographic, then sorted particle order), so the
-// result is bit-reproducible.  A prep pass composes each particle's impulse
-// and affine matrix once and writes them, in sorted order, into a SoA payload
-// that the gather reads contiguously per cell.
+// a prep pass composes each particle's impulse and affine matrix once and
+// writes them, in sorted order, as a 64-byte record; a tiled pass then
+// scatters each cell's particles into per-thread face windows and sums the
+// windows of a tile in LDS in a fixed order; a combine pass adds the <= 2^d
+// tile partials of every face in tile order.  No float atomics anywhere, so
+// the result is bit-reproducible run to run.
 #include "pfm_host.h"
 
 namespace pfm {
@@ -26,8 +27,10 @@ __global__ void __launch_bounds__(kBlock) k_fill_i32(int32_t* p, int64_t n, int3
         p[i] = v;
 }
 
-// get_particles_id_in_every_cell (3D/run_3D.py:342-351): cell index and an
-// arbitrary slot from the count atomic (reordered later).
+// get_particles_id_in_every_cell (3D/run_3D.py:342-351): cell index and a
+// slot from the count atomic (reordered later).  Runs of equal cells among
+// consecutive lanes -- the common case, particles are numbered cell by cell
+// and move < 1 cell per step -- are merged into one atomic per run.
 template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_bin_count(GridDev g, int64_t n,
                                                       const float* __restrict__ pos,
@@ -35,16 +38,34 @@ __global__ void __launch_bounds__(kBlock) k_bin_count(GridDev g, int64_t n,
                                                       int32_t* __restrict__ counts,
                                                       int32_t* __restrict__ slot) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int ci[3] = {0, 0, 0};
+    const bool valid = i < n;
+    int32_t cell = -1;
+    if (valid) {
+        int ci[3] = {0, 0, 0};
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-        int v = (int)(pos[c * n + i] * g.inv_dx);
-        ci[c] = max(0, min(v, g.res[c] - 1));
+        for (int c = 0; c < DIM; ++c) {
+            int v = (int)(pos[c * n + i] * g.inv_dx);
+            ci[c] = max(0, min(v, g.res[c] - 1));
+        }
+        cell = (int32_t)(((int64_t)ci[0] * g.res[1] + ci[1]) * g.res[2] + ci[2]);
+        cell_of[i] = cell;
     }
-    const int32_t cell = (int32_t)(((int64_t)ci[0] * g.res[1] + ci[1]) * g.res[2] + ci[2]);
-    cell_of[i] = cell;
-    slot[i] = atomicAdd(&counts[cell], 1);
+    const int lane = threadIdx.x & 63;
+    const int32_t prev = __shfl_up(cell, 1, 64);
+    const bool head = valid && (lane == 0 || prev != cell);
+    const uint64_t heads = __ballot(head);
+    // run start: highest head at or below this lane; run end: next head
+    const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+    const int first = 63 - __clzll(upto | 1ull);
+    const uint64_t above = lane == 63 ? 0ull : (heads & ~((2ull << lane) - 1));
+    const uint64_t validm = __ballot(valid);
+    int next = above ? __ffsll((long long)above) - 1 : 64;
+    const int last_valid = 63 - __clzll(validm | 1ull);
+    if (next > last_valid + 1) next = last_valid + 1;
+    int32_t base = 0;
+    if (head) base = atomicAdd(&counts[cell], next - lane);
+    base = __shfl(base, first, 64);
+    if (valid) slot[i] = base + (lane - first);
 }
 
 // Exclusive scan, phase 1: per-tile totals.
@@ -188,13 +209,15 @@ __global__ void __launch_bounds__(kBlock) k_cell_lists(int64_t C, const int32_t*
     for (int q = 0; q < cell_max; ++q) ids[c * cell_max + q] = q < m ? perm[s + q] : -1;
 }
 
-// Payload (SoA, sorted order): pos/dx (DIM), imp (DIM), C rows (DIM*DIM).
+// Payload record (AoS, sorted order), kRec floats per particle:
+//   3D: [0..2] pos/dx, [3..5] imp, [6..14] C rows (row r at 6 + 3r), [15] pad
+//   2D: [0..1] pos/dx, [2..3] imp, [4..7] C rows (row r at 4 + 2r)
 template <int DIM>
 struct Pay {
     static constexpr int kPos = 0;
     static constexpr int kImp = DIM;
     static constexpr int kC = 2 * DIM;
-    static constexpr int kCount = 2 * DIM + DIM * DIM;
+    static constexpr int kRec = DIM == 3 ? 16 : 8;
 };
 
 struct PrepArgs {
@@ -207,6 +230,7 @@ struct PrepArgs {
 
 // P2G per-particle composition (3D/run_3D.py:460-479): T_init_C_T = cols(T_gm)
 // @ (rows(init_C) @ rows(T_gm)), imp = cols(T_gm) @ cols(T_init) @ init_imp.
+// One thread per sorted slot q; reads particle perm[q], writes record q.
 template <int DIM, bool FUSED>
 __global__ void __launch_bounds__(kBlock) k_p2g_prep(GridDev g, int64_t n, PrepArgs a,
                                                      const int32_t* __restrict__ perm,
@@ -215,8 +239,11 @@ __global__ void __launch_bounds__(kBlock) k_p2g_prep(GridDev g, int64_t n, PrepA
     if (q >= n) return;
     const int64_t i = perm[q];
     using P = Pay<DIM>;
+    float rec[P::kRec];
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) pay[(P::kPos + c) * n + q] = a.pos[c * n + i] * g.inv_dx;
+    for (int k = 0; k < P::kRec; ++k) rec[k] = 0.f;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) rec[P::kPos + c] = a.pos[c * n + i] * g.inv_dx;
     float C[3][3], imp[3];
 #pragma unroll
     for (int r = 0; r < DIM; ++r) {
@@ -278,25 +305,229 @@ __global__ void __launch_bounds__(kBlock) k_p2g_prep(GridDev g, int64_t n, PrepA
     }
 #pragma unroll
     for (int r = 0; r < DIM; ++r) {
-        pay[(P::kImp + r) * n + q] = imp[r];
+        rec[P::kImp + r] = imp[r];
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) pay[(P::kC + r * DIM + c) * n + q] = C[r][c];
+        for (int c = 0; c < DIM; ++c) rec[P::kC + r * DIM + c] = C[r][c];
     }
+    float4* dst = reinterpret_cast<float4*>(pay + q * P::kRec);
+#pragma unroll
+    for (int k = 0; k < P::kRec / 4; ++k)
+        dst[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
 }
 
-struct GatherArgs {
+// ---- tiled P2G --------------------------------------------------------------
+// A block owns a tile of cells (one thread per cell).  For each face axis A,
+// every thread scatters its cell's particles (sorted, ascending id) into a
+// private window of the faces they reach -- 4 along A, 3 along the other
+// axes (|x_p - x_f| < 3/2) -- with separable B-spline weights.  The windows
+// are then summed into an LDS box in 4*3*3 fixed "offset phases": in phase
+// (i,j,k) every thread adds its window entry (i,j,k) to the face at its cell
+// + (i,j,k) - 1, so no two threads touch the same face within a phase and the
+// order of additions is fixed.  Each tile writes its whole box (faces of its
+// cells plus the halo its particles reach) as a partial; k_p2g_combine sums
+// the partials of the <= 2^d tiles that reach a face in tile order and
+// normalises by the weight.  Every step is deterministic.
+template <int DIM>
+struct TileCfg;
+template <>
+struct TileCfg<3> {
+    static constexpr int T0 = 4, T1 = 8, T2 = 8;
+};
+template <>
+struct TileCfg<2> {
+    static constexpr int T0 = 16, T1 = 16, T2 = 1;
+};
+
+template <int DIM, int A, int C>
+struct Win {
+    static constexpr int value = C >= DIM ? 1 : (C == A ? 4 : 3);
+};
+
+template <int DIM, int A>
+struct Box {
+    using T = TileCfg<DIM>;
+    static constexpr int B0 = T::T0 + Win<DIM, A, 0>::value - 1;
+    static constexpr int B1 = T::T1 + Win<DIM, A, 1>::value - 1;
+    static constexpr int B2 = DIM == 3 ? T::T2 + Win<DIM, A, 2>::value - 1 : 1;
+    static constexpr int size = B0 * B1 * B2;
+};
+
+template <int DIM, int A>
+struct BoxOff {
+    static constexpr int value = A == 0 ? 0 : BoxOff<DIM, A - 1>::value + 2 * Box<DIM, A - 1>::size;
+};
+template <int DIM>
+struct BoxOff<DIM, 0> {
+    static constexpr int value = 0;
+};
+
+template <int DIM>
+struct TileStride {
+    static constexpr int value =
+        DIM == 3 ? BoxOff<3, 2>::value + 2 * Box<3, 2>::size : BoxOff<2, 1>::value + 2 * Box<2, 1>::size;
+};
+
+constexpr int kTileLds = 704;  // >= the largest box (3D x-faces: 7*10*10)
+
+template <int DIM, int A>
+__device__ __forceinline__ void tile_axis(const GridDev& g, const float* __restrict__ pay, int32_t s,
+                                          int32_t e, const int cc[3], const int tl[3], float* lu,
+                                          float* lw, float* __restrict__ part) {
+    using P = Pay<DIM>;
+    constexpr int W0 = Win<DIM, A, 0>::value, W1 = Win<DIM, A, 1>::value, W2 = Win<DIM, A, 2>::value;
+    using BX = Box<DIM, A>;
+    constexpr int Wn[3] = {W0, W1, W2};
+    float au[W0][W1][W2], aw[W0][W1][W2];
+#pragma unroll
+    for (int i = 0; i < W0; ++i)
+#pragma unroll
+        for (int j = 0; j < W1; ++j)
+#pragma unroll
+            for (int k = 0; k < W2; ++k) {
+                au[i][j][k] = 0.f;
+                aw[i][j][k] = 0.f;
+            }
+    for (int32_t q = s; q < e; ++q) {
+        const float* rec = pay + (int64_t)q * P::kRec;
+        float p[3], crow[3];
+        const float impA = rec[P::kImp + A];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            p[c] = rec[P::kPos + c];
+            crow[c] = rec[P::kC + A * DIM + c];
+        }
+        float wt[3][4], dd[3][4];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+#pragma unroll
+            for (int w = 0; w < Wn[c]; ++w) {
+                const float f = (float)(cc[c] - 1 + w);
+                // reference: N_2(pos - face) along A, N_2(pos - face - 0.5)
+                // elsewhere; dpos = (face (+0.5) - pos) * dx (3D/run_3D.py:486-489)
+                const float d = c == A ? p[c] - f : (p[c] - f) - 0.5f;
+                const float o = c == A ? f - p[c] : (f + 0.5f) - p[c];
+                wt[c][w] = N2(d);
+                dd[c][w] = crow[c] * (o * g.dx);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < W0; ++i)
+#pragma unroll
+            for (int j = 0; j < W1; ++j) {
+                const float wij = wt[0][i] * wt[1][j];
+                const float dij = dd[0][i] + dd[1][j];
+#pragma unroll
+                for (int k = 0; k < W2; ++k) {
+                    const float w = DIM == 3 ? wij * wt[2][k] : wij;
+                    const float delta = DIM == 3 ? dij + dd[2][k] : dij;
+                    au[i][j][k] += (impA + delta) * w;
+                    aw[i][j][k] += w;
+                }
+            }
+    }
+    // zero the box, then the offset phases
+    for (int t = threadIdx.x; t < BX::size; t += blockDim.x) {
+        lu[t] = 0.f;
+        lw[t] = 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < W0; ++i)
+#pragma unroll
+        for (int j = 0; j < W1; ++j)
+#pragma unroll
+            for (int k = 0; k < W2; ++k) {
+                const int slot = ((tl[0] + i) * BX::B1 + (tl[1] + j)) * BX::B2 + (DIM == 3 ? tl[2] + k : 0);
+                lu[slot] += au[i][j][k];
+                lw[slot] += aw[i][j][k];
+                __syncthreads();
+            }
+    float* dst = part + BoxOff<DIM, A>::value;
+    for (int t = threadIdx.x; t < BX::size; t += blockDim.x) {
+        dst[t] = lu[t];
+        dst[BX::size + t] = lw[t];
+    }
+    __syncthreads();
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_p2g_tile(GridDev g, const int32_t* __restrict__ start,
+                                                     const float* __restrict__ pay, int nt1, int nt2,
+                                                     float* __restrict__ partial) {
+    using TC = TileCfg<DIM>;
+    __shared__ float lu[kTileLds], lw[kTileLds];
+    const int tile = blockIdx.x;
+    const int tb[3] = {tile / (nt1 * nt2), (tile / nt2) % nt1, tile % nt2};
+    int tl[3];
+    if (DIM == 3) {
+        tl[0] = threadIdx.x / (TC::T1 * TC::T2);
+        tl[1] = (threadIdx.x / TC::T2) % TC::T1;
+        tl[2] = threadIdx.x % TC::T2;
+    } else {
+        tl[0] = threadIdx.x / TC::T1;
+        tl[1] = threadIdx.x % TC::T1;
+        tl[2] = 0;
+    }
+    const int T[3] = {TC::T0, TC::T1, TC::T2};
+    int cc[3];
+    bool valid = true;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        cc[c] = tb[c] * T[c] + tl[c];
+        if (c < DIM && cc[c] >= g.res[c]) valid = false;
+    }
+    int32_t s = 0, e = 0;
+    if (valid) {
+        const int64_t cell = ((int64_t)cc[0] * g.res[1] + cc[1]) * g.res[2] + (DIM == 3 ? cc[2] : 0);
+        s = start[cell];
+        e = start[cell + 1];
+    }
+    float* part = partial + (int64_t)tile * TileStride<DIM>::value;
+    tile_axis<DIM, 0>(g, pay, s, e, cc, tl, lu, lw, part);
+    tile_axis<DIM, 1>(g, pay, s, e, cc, tl, lu, lw, part);
+    if (DIM == 3) tile_axis<DIM, DIM == 3 ? 2 : 1>(g, pay, s, e, cc, tl, lu, lw, part);
+}
+
+struct CombineArgs {
     float* u[3];
     float* W[3];
     int64_t nf[3];
+    int nt[3];
 };
 
-// Gather P2G for one face (3D/run_3D.py:477-552 restated as a gather): sum
-// over the particles of cells [I_A-2, I_A+1] x [I_c-1, I_c+1] of
-// (imp_A + C_A . dpos) * w and w, then u = sum / W where W > 0.
+template <int DIM, int A>
+__device__ __forceinline__ void combine_face(const GridDev& g, const CombineArgs& a, const int I[3],
+                                             const float* __restrict__ partial, float& su, float& sw) {
+    using TC = TileCfg<DIM>;
+    using BX = Box<DIM, A>;
+    const int T[3] = {TC::T0, TC::T1, TC::T2};
+    int tlo[3] = {0, 0, 0}, thi[3] = {0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        const int lo = max(0, I[c] - (c == A ? 2 : 1));
+        const int hi = min(g.res[c] - 1, I[c] + 1);
+        tlo[c] = lo / T[c];
+        thi[c] = hi / T[c];
+    }
+    su = 0.f;
+    sw = 0.f;
+    for (int t0 = tlo[0]; t0 <= thi[0]; ++t0)
+        for (int t1 = tlo[1]; t1 <= thi[1]; ++t1)
+            for (int t2 = tlo[2]; t2 <= thi[2]; ++t2) {
+                const int64_t tile = ((int64_t)t0 * a.nt[1] + t1) * a.nt[2] + t2;
+                const int l0 = I[0] - (t0 * T[0] - 1);
+                const int l1 = I[1] - (t1 * T[1] - 1);
+                const int l2 = DIM == 3 ? I[2] - (t2 * T[2] - 1) : 0;
+                const float* box = partial + tile * TileStride<DIM>::value + BoxOff<DIM, A>::value;
+                const int idx = (l0 * BX::B1 + l1) * BX::B2 + l2;
+                su += box[idx];
+                sw += box[BX::size + idx];
+            }
+}
+
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_p2g_gather(GridDev g, int64_t n, GatherArgs a,
-                                                       const int32_t* __restrict__ start,
-                                                       const float* __restrict__ pay) {
+__global__ void __launch_bounds__(kBlock) k_p2g_combine(GridDev g, CombineArgs a,
+                                                        const float* __restrict__ partial) {
     int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int A = 0;
     while (A < DIM - 1 && f >= a.nf[A]) {
@@ -304,7 +535,6 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(GridDev g, int64_t n, Gat
         ++A;
     }
     if (f >= a.nf[A]) return;
-    using P = Pay<DIM>;
     int S[3], I[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) S[c] = g.res[c] + (c == A ? 1 : 0);
@@ -317,44 +547,14 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(GridDev g, int64_t n, Gat
         I[1] = (int)(f % S[1]);
         I[0] = (int)(f / S[1]);
     }
-    // face position in grid units and dpos offsets (reference: face_id[c] for
-    // c == A, face_id[c] + 0.5 otherwise)
-    float fc[3];
-    int lo[3], hi[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        fc[c] = (float)I[c];
-        if (c < DIM) {
-            lo[c] = max(0, I[c] - (c == A ? 2 : 1));
-            hi[c] = min(g.res[c] - 1, I[c] + 1);
-        } else {
-            lo[c] = 0;
-            hi[c] = 0;
-        }
-    }
-    const float* pimp = pay + (int64_t)(P::kImp + A) * n;
-    const float* pC = pay + (int64_t)(P::kC + A * DIM) * n;
-    float su = 0.f, sw = 0.f;
-    for (int ci = lo[0]; ci <= hi[0]; ++ci)
-        for (int cj = lo[1]; cj <= hi[1]; ++cj) {
-            const int64_t rowc = ((int64_t)ci * g.res[1] + cj) * g.res[2];
-            const int32_t s = start[rowc + lo[2]];
-            const int32_t e = start[rowc + hi[2] + 1];
-            for (int32_t q = s; q < e; ++q) {
-                float w = 1.f, delta = 0.f;
-#pragma unroll
-                for (int c = 0; c < DIM; ++c) {
-                    const float pc = pay[(int64_t)(P::kPos + c) * n + q];
-                    const float d = c == A ? pc - fc[c] : (pc - fc[c]) - 0.5f;
-                    const float e2 = c == A ? fc[c] - pc : (fc[c] + 0.5f) - pc;
-                    w = c == 0 ? N2(d) : w * N2(d);
-                    const float dp = e2 * g.dx;
-                    delta = c == 0 ? pC[q] * dp : delta + pC[(int64_t)c * n + q] * dp;
-                }
-                su += (pimp[q] + delta) * w;
-                sw += w;
-            }
-        }
+    float su, sw;
+    if (A == 0)
+        combine_face<DIM, 0>(g, a, I, partial, su, sw);
+    else if (A == 1 || DIM == 2)
+        combine_face<DIM, 1>(g, a, I, partial, su, sw);
+    else
+        combine_face<DIM, DIM == 3 ? 2 : 1>(g, a, I, partial, su, sw);
+    // u /= W where W > 0 (3D/run_3D.py:530-552)
     if (sw > 0.f) su = su / sw;
     a.u[A][f] = su;
     if (a.W[A]) a.W[A][f] = sw;
@@ -362,18 +562,32 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(GridDev g, int64_t n, Gat
 
 struct Workspace {
     int32_t* counts;     // C
-    int32_t* tiles;      // ntiles
+    int32_t* tiles;      // scan tiles
     int32_t* slot;       // n
     int32_t* cell_tmp;   // n
-    float* pay;          // kCount * n
+    float* pay;          // kRec * n
+    float* partial;      // P2G tile partials
 };
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+static void tile_counts(const GridDev& g, int nt[3]) {
+    const int T3[3] = {TileCfg<3>::T0, TileCfg<3>::T1, TileCfg<3>::T2};
+    const int T2[3] = {TileCfg<2>::T0, TileCfg<2>::T1, 1};
+    for (int c = 0; c < 3; ++c) {
+        const int T = g.dim == 3 ? T3[c] : T2[c];
+        nt[c] = c < g.dim ? (g.res[c] + T - 1) / T : 1;
+    }
+}
+
 static size_t ws_layout(const GridDev& g, int64_t n, char* base, Workspace* w) {
     const int64_t C = cell_count(g);
     const int64_t ntiles = (C + kScanTile - 1) / kScanTile;
-    const int npay = g.dim == 3 ? Pay<3>::kCount : Pay<2>::kCount;
+    const int rec = g.dim == 3 ? Pay<3>::kRec : Pay<2>::kRec;
+    int nt[3];
+    tile_counts(g, nt);
+    const int64_t ptiles = (int64_t)nt[0] * nt[1] * nt[2];
+    const int64_t stride = g.dim == 3 ? TileStride<3>::value : TileStride<2>::value;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         char* p = base ? base + off : nullptr;
@@ -385,7 +599,8 @@ static size_t ws_layout(const GridDev& g, int64_t n, char* base, Workspace* w) {
     t.tiles = (int32_t*)take(sizeof(int32_t) * (size_t)(ntiles + 1));
     t.slot = (int32_t*)take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
     t.cell_tmp = (int32_t*)take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
-    t.pay = (float*)take(sizeof(float) * (size_t)npay * (size_t)(n > 0 ? n : 1));
+    t.pay = (float*)take(sizeof(float) * (size_t)rec * (size_t)(n > 0 ? n : 1));
+    t.partial = (float*)take(sizeof(float) * (size_t)(ptiles * stride));
     if (w) *w = t;
     return off;
 }
@@ -508,22 +723,31 @@ int pfm_p2g(const pfm_grid* gp, int64_t n, const float* pos, const float* imp, c
         }
         PFM_LAUNCH_CHECK();
     }
-    GatherArgs ga;
+    int nt[3];
+    tile_counts(g, nt);
+    const unsigned ptiles = (unsigned)((int64_t)nt[0] * nt[1] * nt[2]);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_p2g_tile<3>, dim3(ptiles), dim3(kBlock), 0, s, g, cell_start, w.pay, nt[1], nt[2],
+                           w.partial);
+    else
+        hipLaunchKernelGGL(k_p2g_tile<2>, dim3(ptiles), dim3(kBlock), 0, s, g, cell_start, w.pay, nt[1], nt[2],
+                           w.partial);
+    PFM_LAUNCH_CHECK();
+    CombineArgs ca;
     float* us[3] = {u_x, u_y, u_z};
     float* Ws[3] = {W_x, W_y, W_z};
     int64_t total = 0;
     for (int c = 0; c < 3; ++c) {
-        ga.u[c] = us[c];
-        ga.W[c] = Ws[c];
-        ga.nf[c] = c < g.dim ? face_count_h(g, c) : 0;
-        total += ga.nf[c];
+        ca.u[c] = us[c];
+        ca.W[c] = Ws[c];
+        ca.nf[c] = c < g.dim ? face_count_h(g, c) : 0;
+        ca.nt[c] = nt[c];
+        total += ca.nf[c];
     }
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_p2g_gather<3>, dim3(grid_for(total)), dim3(kBlock), 0, s, g, n, ga, cell_start,
-                           w.pay);
+        hipLaunchKernelGGL(k_p2g_combine<3>, dim3(grid_for(total)), dim3(kBlock), 0, s, g, ca, w.partial);
     else
-        hipLaunchKernelGGL(k_p2g_gather<2>, dim3(grid_for(total)), dim3(kBlock), 0, s, g, n, ga, cell_start,
-                           w.pay);
+        hipLaunchKernelGGL(k_p2g_combine<2>, dim3(grid_for(total)), dim3(kBlock), 0, s, g, ca, w.partial);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

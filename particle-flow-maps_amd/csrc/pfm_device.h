@@ -80,85 +80,92 @@ __device__ __forceinline__ float N2(float x) {
 // only need dword alignment).
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
 
-__device__ __forceinline__ void load_row4(const float* __restrict__ row, int i0, int ext,
-                                          float v[4]) {
-    // elements i0-1 .. i0+2 clamped to ext-1 (sample(), 3D/taichi_utils.py:48-54);
-    // i0 >= 1 always (s >= 1) and i0 <= ext-2, ext >= 4.
-    const int shift = (i0 + 2 > ext - 1) ? 1 : 0;
-    const f4u q = *reinterpret_cast<const f4u*>(row + (i0 - 1 - shift));
-    if (shift) {
-        v[0] = q.y; v[1] = q.z; v[2] = q.w; v[3] = q.w;
-    } else {
-        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+// Per-axis stencil of interp_grad_2 (3D/taichi_utils.py:278-307):
+// u = p/dx - BL, s = clamp(u, 1, hi), iu = floor(s), f = s - iu, and the
+// weights N_2 / dN_2 at the distances f+1, f, f-1, f-2.
+struct AxisStencil {
+    int i0;
+    float w[4];
+    float dw[4];
+};
+
+template <bool GRAD>
+__device__ __forceinline__ void axis_stencil(float p, float inv_dx, float BL, float hi, AxisStencil& a) {
+    const float u = p * inv_dx - BL;
+    const float s = fmaxf(1.0f, fminf(u, hi));
+    const float fl = floorf(s);
+    a.i0 = (int)fl;
+    const float f = s - fl;
+    if (GRAD)
+        bspline4(f, a.w, a.dw);
+    else
+        bspline4_val(f, a.w);
+}
+
+// The stencils of all MAC components at one point: along axis c, component c
+// uses the face stencil (BL = 0, extent res+1) and the others the cell
+// stencil (BL = 0.5, extent res), so 2 per axis serve all components.
+template <int DIM, bool GRAD>
+struct MacStencil {
+    AxisStencil F[3];  // BL = 0, extent res[c] + 1
+    AxisStencil C[3];  // BL = 0.5, extent res[c]
+    __device__ __forceinline__ void build(const GridDev& g, const float p[3]) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            axis_stencil<GRAD>(p[c], g.inv_dx, 0.0f, g.hi_f[c], F[c]);
+            axis_stencil<GRAD>(p[c], g.inv_dx, 0.5f, g.hi_c[c], C[c]);
+        }
     }
-}
+};
 
-// Per-axis stencil set-up of interp_grad_2 (3D/taichi_utils.py:278-287):
-// u = p/dx - BL, s = clamp(u, 1, hi), iu = floor(s), f = s - iu.
-__device__ __forceinline__ void axis_setup(float p, float inv_dx, float BL, float hi, int& i0,
-                                           float& f) {
-    float u = p * inv_dx - BL;
-    float s = fmaxf(1.0f, fminf(u, hi));
-    float fl = floorf(s);
-    i0 = (int)fl;
-    f = s - fl;
-}
-
-// Interpolate MAC component A (a face field of extents S) at p:
-// value and (if GRAD) gradient, separable form of interp_grad_2
-// (3D/taichi_utils.py:274-307; 2D :214-248).
+// Interpolate MAC component A from its stencils: value and (if GRAD) the
+// gradient, the separable form of interp_grad_2 (3D/taichi_utils.py:274-307;
+// 2D :214-248).  Offsets are 32-bit (a face field has < 2^31 elements).
 template <int DIM, int A, bool GRAD>
 __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const GridDev& g,
-                                            const float p[3], float& val, float grad[3]) {
-    int S[3];
-    float hi[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        S[c] = g.res[c] + (c == A ? 1 : 0);
-        hi[c] = c == A ? g.hi_f[c] : g.hi_c[c];
-    }
-    int i0[3];
-    float f[3];
-    float w[3][4], dw[3][4];
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-        axis_setup(p[c], g.inv_dx, c == A ? 0.0f : 0.5f, hi[c], i0[c], f[c]);
-        if (GRAD)
-            bspline4(f[c], w[c], dw[c]);
-        else
-            bspline4_val(f[c], w[c]);
-    }
+                                            const MacStencil<DIM, GRAD>& st, float& val, float grad[3]) {
+    const AxisStencil& s0 = A == 0 ? st.F[0] : st.C[0];
+    const AxisStencil& s1 = A == 1 ? st.F[1] : st.C[1];
+    const int S0 = g.res[0] + (A == 0 ? 1 : 0);
+    const int S1 = g.res[1] + (A == 1 ? 1 : 0);
     int ix[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) ix[m] = min(i0[0] + m - 1, S[0] - 1);
+    for (int m = 0; m < 4; ++m) ix[m] = min(s0.i0 + m - 1, S0 - 1);
     if (DIM == 3) {
+        const AxisStencil& s2 = A == 2 ? st.F[2] : st.C[2];
+        const int S2 = g.res[2] + (A == 2 ? 1 : 0);
         int iy[4];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) iy[m] = min(i0[1] + m - 1, S[1] - 1);
+        for (int m = 0; m < 4; ++m) iy[m] = min(s1.i0 + m - 1, S1 - 1);
+        // elements i0-1 .. i0+2 along z, clamped to S2-1 (sample(),
+        // 3D/taichi_utils.py:48-54): i0 is in [1, S2-2], so only i0 = S2-2
+        // clamps; then load from i0-2 and shift the lanes by one.
+        const bool shift = s2.i0 + 2 > S2 - 1;
+        const uint32_t z0 = (uint32_t)(s2.i0 - 1 - (shift ? 1 : 0));
         float v0 = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-        // One x-slab (4 rows of 4) per iteration keeps 16 loaded values live
-        // instead of 64, which is what bounds the register count here.
+        // one x-slab (4 rows of 4) per iteration keeps 16 loaded values live
 #pragma unroll 1
         for (int a = 0; a < 4; ++a) {
+            const uint32_t xb = (uint32_t)ix[a] * (uint32_t)S1;
             float c_v = 0.f, c_dy = 0.f, c_dz = 0.f;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const float* row = fld + ((int64_t)ix[a] * S[1] + iy[b]) * S[2];
-                float v[4];
-                load_row4(row, i0[2], S[2], v);
-                float rz = w[2][0] * v[0] + w[2][1] * v[1] + w[2][2] * v[2] + w[2][3] * v[3];
-                c_v += w[1][b] * rz;
+                const uint32_t off = (xb + (uint32_t)iy[b]) * (uint32_t)S2 + z0;
+                const f4u q = *reinterpret_cast<const f4u*>(fld + off);
+                const float v[4] = {shift ? q.y : q.x, shift ? q.z : q.y, shift ? q.w : q.z, q.w};
+                const float rz = s2.w[0] * v[0] + s2.w[1] * v[1] + s2.w[2] * v[2] + s2.w[3] * v[3];
+                c_v += s1.w[b] * rz;
                 if (GRAD) {
-                    float rdz = dw[2][0] * v[0] + dw[2][1] * v[1] + dw[2][2] * v[2] + dw[2][3] * v[3];
-                    c_dy += dw[1][b] * rz;
-                    c_dz += w[1][b] * rdz;
+                    const float rdz = s2.dw[0] * v[0] + s2.dw[1] * v[1] + s2.dw[2] * v[2] + s2.dw[3] * v[3];
+                    c_dy += s1.dw[b] * rz;
+                    c_dz += s1.w[b] * rdz;
                 }
             }
-            v0 += w[0][a] * c_v;
+            v0 += s0.w[a] * c_v;
             if (GRAD) {
-                gx += dw[0][a] * c_v;
-                gy += w[0][a] * c_dy;
-                gz += w[0][a] * c_dz;
+                gx += s0.dw[a] * c_v;
+                gy += s0.w[a] * c_dy;
+                gz += s0.w[a] * c_dz;
             }
         }
         val = v0;
@@ -168,18 +175,20 @@ __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const
             grad[2] = gz * g.inv_dx;
         }
     } else {
+        const bool shift = s1.i0 + 2 > S1 - 1;
+        const uint32_t y0 = (uint32_t)(s1.i0 - 1 - (shift ? 1 : 0));
         float v0 = 0.f, gx = 0.f, gy = 0.f;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const float* row = fld + (int64_t)ix[a] * S[1];
-            float v[4];
-            load_row4(row, i0[1], S[1], v);
-            float ry = w[1][0] * v[0] + w[1][1] * v[1] + w[1][2] * v[2] + w[1][3] * v[3];
-            v0 += w[0][a] * ry;
+            const uint32_t off = (uint32_t)ix[a] * (uint32_t)S1 + y0;
+            const f4u q = *reinterpret_cast<const f4u*>(fld + off);
+            const float v[4] = {shift ? q.y : q.x, shift ? q.z : q.y, shift ? q.w : q.z, q.w};
+            const float ry = s1.w[0] * v[0] + s1.w[1] * v[1] + s1.w[2] * v[2] + s1.w[3] * v[3];
+            v0 += s0.w[a] * ry;
             if (GRAD) {
-                float rdy = dw[1][0] * v[0] + dw[1][1] * v[1] + dw[1][2] * v[2] + dw[1][3] * v[3];
-                gx += dw[0][a] * ry;
-                gy += w[0][a] * rdy;
+                const float rdy = s1.dw[0] * v[0] + s1.dw[1] * v[1] + s1.dw[2] * v[2] + s1.dw[3] * v[3];
+                gx += s0.dw[a] * ry;
+                gy += s0.w[a] * rdy;
             }
         }
         val = v0;
@@ -197,10 +206,12 @@ __device__ __forceinline__ void interp_mac(const float* __restrict__ ux,
                                            const float* __restrict__ uy,
                                            const float* __restrict__ uz, const GridDev& g,
                                            const float p[3], float u[3], float G[3][3]) {
-    interp_comp<DIM, 0, GRAD>(ux, g, p, u[0], G[0]);
-    interp_comp<DIM, 1, GRAD>(uy, g, p, u[1], G[1]);
+    MacStencil<DIM, GRAD> st;
+    st.build(g, p);
+    interp_comp<DIM, 0, GRAD>(ux, g, st, u[0], G[0]);
+    interp_comp<DIM, 1, GRAD>(uy, g, st, u[1], G[1]);
     if (DIM == 3) {
-        interp_comp<DIM, 2, GRAD>(uz, g, p, u[2], G[2]);
+        interp_comp<DIM, 2, GRAD>(uz, g, st, u[2], G[2]);
     } else {
         u[2] = 0.f;
         G[2][0] = G[2][1] = G[2][2] = 0.f;
