@@ -145,13 +145,24 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot(LvlDev L, const float* __rest
                                                    float* __restrict__ Ap, double* __restrict__ partial) {
     __shared__ double sh[kBlock / 64];
     double s = 0.0;
-    int I[3];
-    if (cell_of_thread(L, 0, false, I)) {
-        const int64_t q = cell_index(L, I);
-        const float pq = p[q];
-        const float a = __fsub_rn(__fmul_rn(nfn(L, I), pq), nsum(L, p, q, I));
-        Ap[q] = a;
-        s = (double)pq * (double)a;
+    // 3D: blockIdx.z strides over the x-planes so the grid stays <= kMgRedBlocks
+    const int planes = L.dim == 3 ? L.d[0] : 1;
+    for (int pl = blockIdx.z; pl < planes; pl += gridDim.z) {
+        int I[3];
+        const int fx = blockIdx.x * kTx + threadIdx.x;
+        const int fy = blockIdx.y * kTy + threadIdx.y;
+        if (L.dim == 3) {
+            I[0] = pl; I[1] = fy; I[2] = fx;
+        } else {
+            I[0] = fy; I[1] = fx; I[2] = 0;
+        }
+        if (I[0] < L.d[0] && I[1] < L.d[1] && I[2] < L.d[2]) {
+            const int64_t q = cell_index(L, I);
+            const float pq = p[q];
+            const float a = __fsub_rn(__fmul_rn(nfn(L, I), pq), nsum(L, p, q, I));
+            Ap[q] = a;
+            s += (double)pq * (double)a;
+        }
     }
     const int tid = threadIdx.y * kTx + threadIdx.x;
     // block_sum expects a 1D block index
@@ -212,25 +223,58 @@ __global__ void __launch_bounds__(kBlock) k_prolongate(LvlDev Lf, LvlDev Lc, flo
     z[q] = __fadd_rn(z[q], zc[qc]);
 }
 
-// Bottom solve: `sweeps` red/black iterations on a level small enough for LDS.
+// Bottom solve: `sweeps` red/black iterations on a level small enough for LDS
+// (apply_preconditioner's bottom loop, 3D/mgpcg.py:192-194).  Each thread owns
+// up to kBottomPer cells whose indices, colours, diagonals and right-hand
+// sides are decoded once; the sweeps then only touch LDS.
 constexpr int kBottomMax = 8192;
-__global__ void __launch_bounds__(1024) k_bottom(LvlDev L, int n, float* __restrict__ z,
-                                                 const float* __restrict__ r, int sweeps) {
+constexpr int kBottomThreads = 1024;
+constexpr int kBottomPer = kBottomMax / kBottomThreads;
+__global__ void __launch_bounds__(kBottomThreads) k_bottom(LvlDev L, int n, float* __restrict__ z,
+                                                           const float* __restrict__ r, int sweeps) {
     __shared__ float zs[kBottomMax];
-    for (int q = threadIdx.x; q < n; q += blockDim.x) zs[q] = z[q];
+    int qi[kBottomPer], col[kBottomPer];
+    int nb[kBottomPer][6];
+    float rv[kBottomPer], mv[kBottomPer];
+#pragma unroll
+    for (int k = 0; k < kBottomPer; ++k) {
+        const int q = threadIdx.x + k * kBottomThreads;
+        qi[k] = q;
+        col[k] = -1;
+        if (q < n) {
+            int I[3];
+            cell_ijk(L, q, I);
+            col[k] = (I[0] + I[1] + I[2]) & 1;
+            rv[k] = r[q];
+            mv[k] = nfn(L, I);
+            const int st[3] = {L.d[1] * L.d[2], L.d[2], 1};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                // neighbour order of neighbor_sum: +e then -e per axis; -1 = absent
+                nb[k][2 * a] = (a < L.dim && I[a] < L.d[a] - 1) ? q + st[a] : -1;
+                nb[k][2 * a + 1] = (a < L.dim && I[a] > 0) ? q - st[a] : -1;
+            }
+            zs[q] = z[q];
+        }
+    }
     __syncthreads();
     for (int it = 0; it < sweeps; ++it) {
         for (int phase = 0; phase < 2; ++phase) {
-            for (int q = threadIdx.x; q < n; q += blockDim.x) {
-                int I[3];
-                cell_ijk(L, q, I);
-                if (((I[0] + I[1] + I[2]) & 1) != phase) continue;
-                zs[q] = __fdiv_rn(__fadd_rn(r[q], nsum(L, zs, q, I)), nfn(L, I));
+#pragma unroll
+            for (int k = 0; k < kBottomPer; ++k) {
+                if (col[k] != phase) continue;
+                float sum = 0.0f;
+#pragma unroll
+                for (int e = 0; e < 6; ++e)
+                    if (nb[k][e] >= 0) sum = __fadd_rn(sum, zs[nb[k][e]]);
+                zs[qi[k]] = __fdiv_rn(__fadd_rn(rv[k], sum), mv[k]);
             }
             __syncthreads();
         }
     }
-    for (int q = threadIdx.x; q < n; q += blockDim.x) z[q] = zs[q];
+#pragma unroll
+    for (int k = 0; k < kBottomPer; ++k)
+        if (qi[k] < n) z[qi[k]] = zs[qi[k]];
 }
 
 // Partial sums of a.b (and, if b == nullptr, of a) in double.
@@ -496,7 +540,7 @@ static int vcycle(MGHandle* h, hipStream_t s) {
     const int b = L - 1;
     if (h->lv[b].n <= kBottomMax) {
         LvlDev Lb = lvl_dev(h, b);
-        hipLaunchKernelGGL(k_bottom, dim3(1), dim3(1024), 0, s, Lb, (int)h->lv[b].n, h->z[b], h->r[b], 50);
+        hipLaunchKernelGGL(k_bottom, dim3(1), dim3(kBottomThreads), 0, s, Lb, (int)h->lv[b].n, h->z[b], h->r[b], 50);
         PFM_LAUNCH_CHECK();
     } else {
         for (int i = 0; i < 50; ++i) {
@@ -521,8 +565,19 @@ static int finalize(MGHandle* h, int mode, double eps, hipStream_t s, int nparts
     return PFM_OK;
 }
 
+static dim3 ap_dot_grid(const MGHandle* h) {
+    dim3 gd = cell_grid(h->lv[0].d, h->dim, false);
+    if (h->dim == 3) {
+        const unsigned per_plane = gd.x * gd.y;
+        unsigned z = per_plane >= (unsigned)kMgRedBlocks ? 1u : (unsigned)kMgRedBlocks / per_plane;
+        if (z > gd.z) z = gd.z;
+        gd.z = z;
+    }
+    return gd;
+}
+
 static int ap_dot_parts(const MGHandle* h) {
-    const dim3 gd = cell_grid(h->lv[0].d, h->dim, false);
+    const dim3 gd = ap_dot_grid(h);
     return (int)(gd.x * gd.y * gd.z);
 }
 
@@ -570,8 +625,7 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
     double rtr = 0.0;
     bool converged = false;
     while (max_iters == -1 || it < max_iters) {
-        hipLaunchKernelGGL(k_ap_dot, cell_grid(L0.d, h->dim, false), kCellBlock, 0, s, L0, h->p, h->Ap,
-                           h->partial);
+        hipLaunchKernelGGL(k_ap_dot, ap_dot_grid(h), kCellBlock, 0, s, L0, h->p, h->Ap, h->partial);
         PFM_LAUNCH_CHECK();
         PFM_TRY(finalize(h, kFinAlpha, eps, s, ap_dot_parts(h)));
         hipLaunchKernelGGL(k_update_xr, dim3(rb), dim3(kBlock), 0, s, n, h->x, h->r[0], h->p, h->Ap, h->sc,

@@ -10,7 +10,7 @@ namespace pfm {
 // RK4_T_forward: 3D/run_3D.py:175-219, 2D/run_2D.py:304-341.
 // dT = M @ T with M = cols(grad u_x, grad u_y, grad u_z): dT[r] = sum_c G[c][r] T[c].
 template <int DIM>
-__global__ void __launch_bounds__(kBlock, 4) k_rk4_particles(GridDev g, int64_t n,
+__global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t n,
                                                           float* __restrict__ pos,
                                                           float* __restrict__ Tx,
                                                           float* __restrict__ Ty,

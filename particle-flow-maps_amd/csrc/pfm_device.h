@@ -121,6 +121,23 @@ struct MacStencil {
 // Interpolate MAC component A from its stencils: value and (if GRAD) the
 // gradient, the separable form of interp_grad_2 (3D/taichi_utils.py:274-307;
 // 2D :214-248).  Offsets are 32-bit (a face field has < 2^31 elements).
+//
+// Along the contiguous axis the 4 samples i0-1 .. i0+2 are clamped to the
+// last element (sample(), 3D/taichi_utils.py:48-54).  i0 lies in [1, S-2], so
+// only i0 = S-2 clamps; then the row is loaded from i0-2 and the weights are
+// shifted instead of the data: (0, w0, w1, w2 + w3) applied to the loaded
+// (v[i0-2], v[i0-1], v[i0], v[i0+1]).
+__device__ __forceinline__ void edge_weights(bool shift, const float w[4], float o[4]) {
+    o[0] = shift ? 0.f : w[0];
+    o[1] = shift ? w[0] : w[1];
+    o[2] = shift ? w[1] : w[2];
+    o[3] = shift ? w[2] + w[3] : w[3];
+}
+
+__device__ __forceinline__ float pick4(int a, const float v[4]) {
+    return a == 0 ? v[0] : (a == 1 ? v[1] : (a == 2 ? v[2] : v[3]));
+}
+
 template <int DIM, int A, bool GRAD>
 __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const GridDev& g,
                                             const MacStencil<DIM, GRAD>& st, float& val, float grad[3]) {
@@ -128,44 +145,41 @@ __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const
     const AxisStencil& s1 = A == 1 ? st.F[1] : st.C[1];
     const int S0 = g.res[0] + (A == 0 ? 1 : 0);
     const int S1 = g.res[1] + (A == 1 ? 1 : 0);
-    int ix[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) ix[m] = min(s0.i0 + m - 1, S0 - 1);
     if (DIM == 3) {
         const AxisStencil& s2 = A == 2 ? st.F[2] : st.C[2];
         const int S2 = g.res[2] + (A == 2 ? 1 : 0);
         int iy[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) iy[m] = min(s1.i0 + m - 1, S1 - 1);
-        // elements i0-1 .. i0+2 along z, clamped to S2-1 (sample(),
-        // 3D/taichi_utils.py:48-54): i0 is in [1, S2-2], so only i0 = S2-2
-        // clamps; then load from i0-2 and shift the lanes by one.
         const bool shift = s2.i0 + 2 > S2 - 1;
         const uint32_t z0 = (uint32_t)(s2.i0 - 1 - (shift ? 1 : 0));
+        float wz[4], dwz[4];
+        edge_weights(shift, s2.w, wz);
+        if (GRAD) edge_weights(shift, s2.dw, dwz);
         float v0 = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
         // one x-slab (4 rows of 4) per iteration keeps 16 loaded values live
 #pragma unroll 1
         for (int a = 0; a < 4; ++a) {
-            const uint32_t xb = (uint32_t)ix[a] * (uint32_t)S1;
+            const uint32_t xb = (uint32_t)min(s0.i0 + a - 1, S0 - 1) * (uint32_t)S1;
             float c_v = 0.f, c_dy = 0.f, c_dz = 0.f;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const uint32_t off = (xb + (uint32_t)iy[b]) * (uint32_t)S2 + z0;
                 const f4u q = *reinterpret_cast<const f4u*>(fld + off);
-                const float v[4] = {shift ? q.y : q.x, shift ? q.z : q.y, shift ? q.w : q.z, q.w};
-                const float rz = s2.w[0] * v[0] + s2.w[1] * v[1] + s2.w[2] * v[2] + s2.w[3] * v[3];
+                const float rz = wz[0] * q.x + wz[1] * q.y + wz[2] * q.z + wz[3] * q.w;
                 c_v += s1.w[b] * rz;
                 if (GRAD) {
-                    const float rdz = s2.dw[0] * v[0] + s2.dw[1] * v[1] + s2.dw[2] * v[2] + s2.dw[3] * v[3];
+                    const float rdz = dwz[0] * q.x + dwz[1] * q.y + dwz[2] * q.z + dwz[3] * q.w;
                     c_dy += s1.dw[b] * rz;
                     c_dz += s1.w[b] * rdz;
                 }
             }
-            v0 += s0.w[a] * c_v;
+            const float wa = pick4(a, s0.w);
+            v0 += wa * c_v;
             if (GRAD) {
-                gx += s0.dw[a] * c_v;
-                gy += s0.w[a] * c_dy;
-                gz += s0.w[a] * c_dz;
+                gx += pick4(a, s0.dw) * c_v;
+                gy += wa * c_dy;
+                gz += wa * c_dz;
             }
         }
         val = v0;
@@ -177,16 +191,18 @@ __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const
     } else {
         const bool shift = s1.i0 + 2 > S1 - 1;
         const uint32_t y0 = (uint32_t)(s1.i0 - 1 - (shift ? 1 : 0));
+        float wy[4], dwy[4];
+        edge_weights(shift, s1.w, wy);
+        if (GRAD) edge_weights(shift, s1.dw, dwy);
         float v0 = 0.f, gx = 0.f, gy = 0.f;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const uint32_t off = (uint32_t)ix[a] * (uint32_t)S1 + y0;
+            const uint32_t off = (uint32_t)min(s0.i0 + a - 1, S0 - 1) * (uint32_t)S1 + y0;
             const f4u q = *reinterpret_cast<const f4u*>(fld + off);
-            const float v[4] = {shift ? q.y : q.x, shift ? q.z : q.y, shift ? q.w : q.z, q.w};
-            const float ry = s1.w[0] * v[0] + s1.w[1] * v[1] + s1.w[2] * v[2] + s1.w[3] * v[3];
+            const float ry = wy[0] * q.x + wy[1] * q.y + wy[2] * q.z + wy[3] * q.w;
             v0 += s0.w[a] * ry;
             if (GRAD) {
-                const float rdy = s1.dw[0] * v[0] + s1.dw[1] * v[1] + s1.dw[2] * v[2] + s1.dw[3] * v[3];
+                const float rdy = dwy[0] * q.x + dwy[1] * q.y + dwy[2] * q.z + dwy[3] * q.w;
                 gx += s0.dw[a] * ry;
                 gy += s0.w[a] * rdy;
             }
