@@ -148,6 +148,26 @@ int pfm_p2g(const pfm_grid* g, int64_t n, const float* pos, const float* imp, co
             float* W_x, float* W_y, float* W_z, void* workspace, size_t ws_bytes,
             pfm_stream_t stream);
 
+/* Fused forms used by the step driver (no separate counterpart in the
+ * reference; together they compute exactly RK4_T_forward followed by P2G).
+ * pfm_rk4_particles_p2g_record = pfm_rk4_particles plus the per-particle P2G
+ * composition of 3D/run_3D.py:460-479 with the advected T_grad_m, written as
+ * 64-byte records in particle order into `records` (2D: the composition of
+ * update_particles_grad_m / update_T / update_particles_imp is not fused; use
+ * pfm_p2g).  pfm_p2g_records = the P2G tile and combine passes over those
+ * records, read through `perm` (after pfm_bin_particles on the new
+ * positions).  pfm_p2g_record_buffer returns the record area inside a P2G
+ * workspace (NULL on bad arguments). */
+int pfm_rk4_particles_p2g_record(const pfm_grid* g, int64_t n, float* pos, float* T_x, float* T_y,
+                                 float* T_z, const float* u_x, const float* u_y, const float* u_z,
+                                 float dt, const float* Tin_x, const float* Tin_y, const float* Tin_z,
+                                 const float* iC_x, const float* iC_y, const float* iC_z,
+                                 const float* init_imp, float* records, pfm_stream_t stream);
+int pfm_p2g_records(const pfm_grid* g, int64_t n, const int32_t* cell_start, const int32_t* perm,
+                    float* u_x, float* u_y, float* u_z, float* W_x, float* W_y, float* W_z,
+                    void* workspace, size_t ws_bytes, pfm_stream_t stream);
+void* pfm_p2g_record_buffer(const pfm_grid* g, int64_t n, void* workspace);
+
 /* ---- grid utilities ---------------------------------------------------- */
 
 /* calc_max_speed: 3D/run_3D.py:95-103, 2D/run_2D.py:101-108.  Writes the max

@@ -35,6 +35,11 @@ int make_grid(const pfm_grid* g, GridDev* out) {
     }
     int64_t cells = (int64_t)out->res[0] * out->res[1] * out->res[2];
     PFM_CHECK_ARG(cells < (int64_t)INT32_MAX - 16, "grid too large for int32 cell indices");
+    // face fields are addressed with 32-bit byte offsets
+    for (int a = 0; a < out->dim; ++a) {
+        int64_t faces = cells / out->res[a] * (out->res[a] + 1);
+        PFM_CHECK_ARG(faces * 4 < (int64_t)INT32_MAX, "face field %d exceeds 2 GiB", a);
+    }
     return PFM_OK;
 }
 

@@ -209,17 +209,6 @@ __global__ void __launch_bounds__(kBlock) k_cell_lists(int64_t C, const int32_t*
     for (int q = 0; q < cell_max; ++q) ids[c * cell_max + q] = q < m ? perm[s + q] : -1;
 }
 
-// Payload record (AoS, sorted order), kRec floats per particle:
-//   3D: [0..2] pos/dx, [3..5] imp, [6..14] C rows (row r at 6 + 3r), [15] pad
-//   2D: [0..1] pos/dx, [2..3] imp, [4..7] C rows (row r at 4 + 2r)
-template <int DIM>
-struct Pay {
-    static constexpr int kPos = 0;
-    static constexpr int kImp = DIM;
-    static constexpr int kC = 2 * DIM;
-    static constexpr int kRec = DIM == 3 ? 16 : 8;
-};
-
 struct PrepArgs {
     const float* pos;
     const float* imp;
@@ -228,9 +217,8 @@ struct PrepArgs {
     const float* Tin[3];
 };
 
-// P2G per-particle composition (3D/run_3D.py:460-479): T_init_C_T = cols(T_gm)
-// @ (rows(init_C) @ rows(T_gm)), imp = cols(T_gm) @ cols(T_init) @ init_imp.
-// One thread per sorted slot q; reads particle perm[q], writes record q.
+// Record pass of P2G: one thread per sorted slot q reads particle perm[q] and
+// writes record q (write_p2g_record).
 template <int DIM, bool FUSED>
 __global__ void __launch_bounds__(kBlock) k_p2g_prep(GridDev g, int64_t n, PrepArgs a,
                                                      const int32_t* __restrict__ perm,
@@ -238,81 +226,21 @@ __global__ void __launch_bounds__(kBlock) k_p2g_prep(GridDev g, int64_t n, PrepA
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int64_t i = perm[q];
-    using P = Pay<DIM>;
-    float rec[P::kRec];
-#pragma unroll
-    for (int k = 0; k < P::kRec; ++k) rec[k] = 0.f;
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) rec[P::kPos + c] = a.pos[c * n + i] * g.inv_dx;
-    float C[3][3], imp[3];
+    float pos[3] = {0.f, 0.f, 0.f}, A[3][3], B[3][3], C[3][3], imp[3];
 #pragma unroll
     for (int r = 0; r < DIM; ++r) {
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) C[r][c] = a.C[r][c * n + i];
+        pos[r] = a.pos[r * n + i];
         imp[r] = a.imp[r * n + i];
-    }
-    if (FUSED) {
-        float A[3][3], B[3][3], P2[3][3], Q[3][3];
 #pragma unroll
-        for (int r = 0; r < DIM; ++r)
-#pragma unroll
-            for (int c = 0; c < DIM; ++c) {
+        for (int c = 0; c < DIM; ++c) {
+            C[r][c] = a.C[r][c * n + i];
+            if (FUSED) {
                 A[r][c] = a.Tgm[r][c * n + i];
                 B[r][c] = a.Tin[c][r * n + i];
             }
-#pragma unroll
-        for (int r = 0; r < DIM; ++r)
-#pragma unroll
-            for (int c = 0; c < DIM; ++c) {
-                float acc = C[r][0] * A[0][c];
-#pragma unroll
-                for (int k = 1; k < DIM; ++k) acc += C[r][k] * A[k][c];
-                P2[r][c] = acc;
-            }
-#pragma unroll
-        for (int r = 0; r < DIM; ++r)
-#pragma unroll
-            for (int c = 0; c < DIM; ++c) {
-                float acc = A[0][r] * P2[0][c];
-#pragma unroll
-                for (int k = 1; k < DIM; ++k) acc += A[k][r] * P2[k][c];
-                Q[r][c] = acc;
-            }
-        float Tt[3][3];
-#pragma unroll
-        for (int r = 0; r < DIM; ++r)
-#pragma unroll
-            for (int c = 0; c < DIM; ++c) {
-                float acc = A[0][r] * B[0][c];
-#pragma unroll
-                for (int k = 1; k < DIM; ++k) acc += A[k][r] * B[k][c];
-                Tt[r][c] = acc;
-            }
-        float m2[3];
-#pragma unroll
-        for (int r = 0; r < DIM; ++r) {
-            float acc = Tt[r][0] * imp[0];
-#pragma unroll
-            for (int c = 1; c < DIM; ++c) acc += Tt[r][c] * imp[c];
-            m2[r] = acc;
-        }
-#pragma unroll
-        for (int r = 0; r < DIM; ++r) {
-            imp[r] = m2[r];
-#pragma unroll
-            for (int c = 0; c < DIM; ++c) C[r][c] = Q[r][c];
         }
     }
-#pragma unroll
-    for (int r = 0; r < DIM; ++r) {
-        rec[P::kImp + r] = imp[r];
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) rec[P::kC + r * DIM + c] = C[r][c];
-    }
-    float4* dst = reinterpret_cast<float4*>(pay + q * P::kRec);
-#pragma unroll
-    for (int k = 0; k < P::kRec / 4; ++k)
-        dst[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+    write_p2g_record<DIM>(pay + q * Pay<DIM>::kRec, pos, g.inv_dx, FUSED, A, B, C, imp);
 }
 
 // ---- tiled P2G --------------------------------------------------------------
@@ -370,9 +298,10 @@ struct TileStride {
 constexpr int kTileLds = 704;  // >= the largest box (3D x-faces: 7*10*10)
 
 template <int DIM, int A>
-__device__ __forceinline__ void tile_axis(const GridDev& g, const float* __restrict__ pay, int32_t s,
-                                          int32_t e, const int cc[3], const int tl[3], float* lu,
-                                          float* lw, float* __restrict__ part) {
+__device__ __forceinline__ void tile_axis(const GridDev& g, const float* __restrict__ pay,
+                                          const int32_t* __restrict__ perm, int32_t s, int32_t e,
+                                          const int cc[3], const int tl[3], float* lu, float* lw,
+                                          float* __restrict__ part) {
     using P = Pay<DIM>;
     constexpr int W0 = Win<DIM, A, 0>::value, W1 = Win<DIM, A, 1>::value, W2 = Win<DIM, A, 2>::value;
     using BX = Box<DIM, A>;
@@ -388,7 +317,8 @@ __device__ __forceinline__ void tile_axis(const GridDev& g, const float* __restr
                 aw[i][j][k] = 0.f;
             }
     for (int32_t q = s; q < e; ++q) {
-        const float* rec = pay + (int64_t)q * P::kRec;
+        // records in sorted order, or in particle order through the permutation
+        const float* rec = pay + (int64_t)(perm ? perm[q] : q) * P::kRec;
         float p[3], crow[3];
         const float impA = rec[P::kImp + A];
 #pragma unroll
@@ -452,7 +382,8 @@ __device__ __forceinline__ void tile_axis(const GridDev& g, const float* __restr
 
 template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_p2g_tile(GridDev g, const int32_t* __restrict__ start,
-                                                     const float* __restrict__ pay, int nt1, int nt2,
+                                                     const float* __restrict__ pay,
+                                                     const int32_t* __restrict__ perm, int nt1, int nt2,
                                                      float* __restrict__ partial) {
     using TC = TileCfg<DIM>;
     __shared__ float lu[kTileLds], lw[kTileLds];
@@ -483,9 +414,9 @@ __global__ void __launch_bounds__(kBlock) k_p2g_tile(GridDev g, const int32_t* _
         e = start[cell + 1];
     }
     float* part = partial + (int64_t)tile * TileStride<DIM>::value;
-    tile_axis<DIM, 0>(g, pay, s, e, cc, tl, lu, lw, part);
-    tile_axis<DIM, 1>(g, pay, s, e, cc, tl, lu, lw, part);
-    if (DIM == 3) tile_axis<DIM, DIM == 3 ? 2 : 1>(g, pay, s, e, cc, tl, lu, lw, part);
+    tile_axis<DIM, 0>(g, pay, perm, s, e, cc, tl, lu, lw, part);
+    tile_axis<DIM, 1>(g, pay, perm, s, e, cc, tl, lu, lw, part);
+    if (DIM == 3) tile_axis<DIM, DIM == 3 ? 2 : 1>(g, pay, perm, s, e, cc, tl, lu, lw, part);
 }
 
 struct CombineArgs {
@@ -605,6 +536,10 @@ static size_t ws_layout(const GridDev& g, int64_t n, char* base, Workspace* w) {
     return off;
 }
 
+static int tile_and_combine(const GridDev& g, const int32_t* cell_start, const int32_t* perm,
+                            const Workspace& w, float* u_x, float* u_y, float* u_z, float* W_x, float* W_y,
+                            float* W_z, hipStream_t s);
+
 }  // namespace pfm
 
 using namespace pfm;
@@ -723,15 +658,48 @@ int pfm_p2g(const pfm_grid* gp, int64_t n, const float* pos, const float* imp, c
         }
         PFM_LAUNCH_CHECK();
     }
+    return tile_and_combine(g, cell_start, nullptr, w, u_x, u_y, u_z, W_x, W_y, W_z, s);
+}
+
+int pfm_p2g_records(const pfm_grid* gp, int64_t n, const int32_t* cell_start, const int32_t* perm,
+                    float* u_x, float* u_y, float* u_z, float* W_x, float* W_y, float* W_z, void* workspace,
+                    size_t ws_bytes, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(n >= 0 && n < (int64_t)INT32_MAX, "n out of range");
+    PFM_CHECK_ARG(cell_start && perm && workspace && u_x && u_y && (g.dim == 2 || u_z), "null pointer");
+    const size_t need = ws_layout(g, n, nullptr, nullptr);
+    PFM_CHECK_ARG(ws_bytes >= need, "workspace too small: %zu < %zu bytes", ws_bytes, need);
+    Workspace w;
+    ws_layout(g, n, (char*)workspace, &w);
+    return tile_and_combine(g, cell_start, perm, w, u_x, u_y, u_z, W_x, W_y, W_z, as_stream(stream));
+}
+
+void* pfm_p2g_record_buffer(const pfm_grid* gp, int64_t n, void* workspace) {
+    GridDev g;
+    if (make_grid(gp, &g) != PFM_OK || n < 0 || !workspace) return nullptr;
+    Workspace w;
+    ws_layout(g, n, (char*)workspace, &w);
+    return w.pay;
+}
+
+}  // extern "C"
+
+namespace pfm {
+
+static int tile_and_combine(const GridDev& g, const int32_t* cell_start, const int32_t* perm,
+                            const Workspace& w, float* u_x, float* u_y, float* u_z, float* W_x, float* W_y,
+                            float* W_z, hipStream_t s) {
     int nt[3];
     tile_counts(g, nt);
     const unsigned ptiles = (unsigned)((int64_t)nt[0] * nt[1] * nt[2]);
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_p2g_tile<3>, dim3(ptiles), dim3(kBlock), 0, s, g, cell_start, w.pay, nt[1], nt[2],
-                           w.partial);
+        hipLaunchKernelGGL(k_p2g_tile<3>, dim3(ptiles), dim3(kBlock), 0, s, g, cell_start, w.pay, perm, nt[1],
+                           nt[2], w.partial);
     else
-        hipLaunchKernelGGL(k_p2g_tile<2>, dim3(ptiles), dim3(kBlock), 0, s, g, cell_start, w.pay, nt[1], nt[2],
-                           w.partial);
+        hipLaunchKernelGGL(k_p2g_tile<2>, dim3(ptiles), dim3(kBlock), 0, s, g, cell_start, w.pay, perm, nt[1],
+                           nt[2], w.partial);
     PFM_LAUNCH_CHECK();
     CombineArgs ca;
     float* us[3] = {u_x, u_y, u_z};
@@ -752,4 +720,4 @@ int pfm_p2g(const pfm_grid* gp, int64_t n, const float* pos, const float* imp, c
     return PFM_OK;
 }
 
-}  // extern "C"
+}  // namespace pfm

@@ -9,7 +9,16 @@ namespace pfm {
 
 // RK4_T_forward: 3D/run_3D.py:175-219, 2D/run_2D.py:304-341.
 // dT = M @ T with M = cols(grad u_x, grad u_y, grad u_z): dT[r] = sum_c G[c][r] T[c].
-template <int DIM>
+// Optional epilogue: the P2G record of the advected particle (the composition
+// of 3D/run_3D.py:460-479 with the new T_grad_m), written in particle order.
+struct RecArgs {
+    const float* Tin[3];
+    const float* C[3];
+    const float* imp;
+    float* rec;
+};
+
+template <int DIM, bool REC = false>
 __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t n,
                                                           float* __restrict__ pos,
                                                           float* __restrict__ Tx,
@@ -18,7 +27,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t 
                                                           const float* __restrict__ ux,
                                                           const float* __restrict__ uy,
                                                           const float* __restrict__ uz, float dt,
-                                                          int advect_psi) {
+                                                          int advect_psi, RecArgs ra = RecArgs{}) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float* Ts[3] = {Tx, Ty, Tz};
@@ -78,14 +87,36 @@ __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t 
         }
     }
     const float sixth_dt = dt * 1.0f / 6.0f;
+    float p1[3] = {p0[0], p0[1], p0[2]};
     if (advect_psi) {
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) pos[c * n + i] = p0[c] + sixth_dt * us[c];
+        for (int c = 0; c < DIM; ++c) {
+            p1[c] = p0[c] + sixth_dt * us[c];
+            pos[c * n + i] = p1[c];
+        }
     }
+    float T1[3][3];
 #pragma unroll
     for (int m = 0; m < DIM; ++m)
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) Ts[m][c * n + i] = T0[m][c] - sixth_dt * dTs[m][c];
+        for (int c = 0; c < DIM; ++c) {
+            T1[m][c] = T0[m][c] - sixth_dt * dTs[m][c];
+            Ts[m][c * n + i] = T1[m][c];
+        }
+    if (REC) {
+        float A[3][3], B[3][3], C[3][3], m0[3];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+            m0[r] = ra.imp[r * n + i];
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) {
+                A[r][c] = T1[r][c];               // rows(T_grad_m): A[r][c] = T_r[c]
+                B[r][c] = ra.Tin[c][r * n + i];   // cols(T_init)
+                C[r][c] = ra.C[r][c * n + i];     // rows(init_C)
+            }
+        }
+        write_p2g_record<DIM>(ra.rec + i * Pay<DIM>::kRec, p1, g.inv_dx, true, A, B, C, m0);
+    }
 }
 
 // init_particles_imp / init_particles_imp_grad_m: 3D/init_conditions.py:254-273,
@@ -315,11 +346,40 @@ int pfm_rk4_particles(const pfm_grid* gp, int64_t n, float* pos, float* T_x, flo
     PFM_CHECK_ARG(pos && T_x && T_y && u_x && u_y, "null pointer");
     PFM_CHECK_ARG(g.dim == 2 || (T_z && u_z), "3D needs T_z and u_z");
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_rk4_particles<3>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
+        hipLaunchKernelGGL((k_rk4_particles<3, false>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
                            g, n, pos, T_x, T_y, T_z, u_x, u_y, u_z, dt, advect_psi);
     else
-        hipLaunchKernelGGL(k_rk4_particles<2>, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
+        hipLaunchKernelGGL((k_rk4_particles<2, false>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
                            g, n, pos, T_x, T_y, T_z, u_x, u_y, u_z, dt, advect_psi);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_rk4_particles_p2g_record(const pfm_grid* gp, int64_t n, float* pos, float* T_x, float* T_y,
+                                 float* T_z, const float* u_x, const float* u_y, const float* u_z, float dt,
+                                 const float* Tin_x, const float* Tin_y, const float* Tin_z, const float* iC_x,
+                                 const float* iC_y, const float* iC_z, const float* init_imp, float* records,
+                                 pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return PFM_OK;
+    PFM_CHECK_ARG(pos && T_x && T_y && u_x && u_y && Tin_x && Tin_y && iC_x && iC_y && init_imp && records,
+                  "null pointer");
+    PFM_CHECK_ARG(g.dim == 2 || (T_z && u_z && Tin_z && iC_z), "3D needs the z columns/rows");
+    PFM_CHECK_ARG(((uintptr_t)records & 15) == 0, "records must be 16-byte aligned");
+    RecArgs ra;
+    ra.Tin[0] = Tin_x; ra.Tin[1] = Tin_y; ra.Tin[2] = Tin_z;
+    ra.C[0] = iC_x; ra.C[1] = iC_y; ra.C[2] = iC_z;
+    ra.imp = init_imp;
+    ra.rec = records;
+    if (g.dim == 3)
+        hipLaunchKernelGGL((k_rk4_particles<3, true>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), g,
+                           n, pos, T_x, T_y, T_z, u_x, u_y, u_z, dt, 1, ra);
+    else
+        hipLaunchKernelGGL((k_rk4_particles<2, true>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), g,
+                           n, pos, T_x, T_y, T_z, u_x, u_y, u_z, dt, 1, ra);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

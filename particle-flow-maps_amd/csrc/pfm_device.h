@@ -134,8 +134,12 @@ __device__ __forceinline__ void edge_weights(bool shift, const float w[4], float
     o[3] = shift ? w[2] + w[3] : w[3];
 }
 
-__device__ __forceinline__ float pick4(int a, const float v[4]) {
-    return a == 0 ? v[0] : (a == 1 ? v[1] : (a == 2 ? v[2] : v[3]));
+// Raw buffer resource over a face field (word 3 = 0x00020000 on gfx9-family
+// parts: raw 32-bit dwords).  Loads through it take a 32-bit byte offset, so a
+// row address costs one VALU add; offsets past num_records read zero.
+typedef float v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t face_rsrc(const float* fld, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(fld), (short)0, (int)bytes, 0x00020000);
 }
 
 template <int DIM, int A, bool GRAD>
@@ -156,16 +160,23 @@ __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const
         float wz[4], dwz[4];
         edge_weights(shift, s2.w, wz);
         if (GRAD) edge_weights(shift, s2.dw, dwz);
+        const uint32_t plane = (uint32_t)S1 * (uint32_t)S2 * 4u;
+        const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, plane * (uint32_t)S0);
+        uint32_t rowoff[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) rowoff[b] = ((uint32_t)iy[b] * (uint32_t)S2 + z0) * 4u;
+        // x-slab weights, rotated one slot per iteration
+        float wa[4] = {s0.w[0], s0.w[1], s0.w[2], s0.w[3]};
+        float dwa[4] = {s0.dw[0], s0.dw[1], s0.dw[2], s0.dw[3]};
         float v0 = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
         // one x-slab (4 rows of 4) per iteration keeps 16 loaded values live
 #pragma unroll 1
         for (int a = 0; a < 4; ++a) {
-            const uint32_t xb = (uint32_t)min(s0.i0 + a - 1, S0 - 1) * (uint32_t)S1;
+            const uint32_t xoff = (uint32_t)min(s0.i0 + a - 1, S0 - 1) * plane;
             float c_v = 0.f, c_dy = 0.f, c_dz = 0.f;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const uint32_t off = (xb + (uint32_t)iy[b]) * (uint32_t)S2 + z0;
-                const f4u q = *reinterpret_cast<const f4u*>(fld + off);
+                const v4f q = __builtin_amdgcn_raw_buffer_load_b128(rs, xoff + rowoff[b], 0, 0);
                 const float rz = wz[0] * q.x + wz[1] * q.y + wz[2] * q.z + wz[3] * q.w;
                 c_v += s1.w[b] * rz;
                 if (GRAD) {
@@ -174,13 +185,14 @@ __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const
                     c_dz += s1.w[b] * rdz;
                 }
             }
-            const float wa = pick4(a, s0.w);
-            v0 += wa * c_v;
+            v0 += wa[0] * c_v;
             if (GRAD) {
-                gx += pick4(a, s0.dw) * c_v;
-                gy += wa * c_dy;
-                gz += wa * c_dz;
+                gx += dwa[0] * c_v;
+                gy += wa[0] * c_dy;
+                gz += wa[0] * c_dz;
             }
+            wa[0] = wa[1]; wa[1] = wa[2]; wa[2] = wa[3];
+            if (GRAD) { dwa[0] = dwa[1]; dwa[1] = dwa[2]; dwa[2] = dwa[3]; }
         }
         val = v0;
         if (GRAD) {
@@ -232,6 +244,85 @@ __device__ __forceinline__ void interp_mac(const float* __restrict__ ux,
         u[2] = 0.f;
         G[2][0] = G[2][1] = G[2][2] = 0.f;
     }
+}
+
+// ---- P2G particle record ---------------------------------------------------
+// Record of one particle for the tiled P2G, kRec floats:
+//   3D: [0..2] pos/dx, [3..5] imp, [6..14] C rows (row r at 6 + 3r), [15] pad
+//   2D: [0..1] pos/dx, [2..3] imp, [4..7] C rows (row r at 4 + 2r)
+template <int DIM>
+struct Pay {
+    static constexpr int kPos = 0;
+    static constexpr int kImp = DIM;
+    static constexpr int kC = 2 * DIM;
+    static constexpr int kRec = DIM == 3 ? 16 : 8;
+};
+
+// P2G per-particle composition (3D/run_3D.py:460-479) and record write:
+// with A = rows(T_grad_m) (A[r][c] = T_r[c]) and B = cols(T_init)
+// (B[r][c] = T_c_init[r]), C <- A^T (init_C A) and imp <- (A^T B) init_imp.
+// Without `compose`, C and imp are written as given (the 2D driver forms them
+// in update_particles_grad_m / update_particles_imp).
+template <int DIM>
+__device__ __forceinline__ void write_p2g_record(float* __restrict__ dst, const float pos[3], float inv_dx,
+                                                 bool compose, const float A[3][3], const float B[3][3],
+                                                 const float C[3][3], const float imp[3]) {
+    using P = Pay<DIM>;
+    float rec[P::kRec];
+#pragma unroll
+    for (int k = 0; k < P::kRec; ++k) rec[k] = 0.f;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) rec[P::kPos + c] = pos[c] * inv_dx;
+    float Co[3][3], mo[3];
+    if (compose) {
+        float P2[3][3], Tt[3][3];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r)
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) {
+                float acc = C[r][0] * A[0][c];
+#pragma unroll
+                for (int k = 1; k < DIM; ++k) acc += C[r][k] * A[k][c];
+                P2[r][c] = acc;
+            }
+#pragma unroll
+        for (int r = 0; r < DIM; ++r)
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) {
+                float acc = A[0][r] * P2[0][c];
+#pragma unroll
+                for (int k = 1; k < DIM; ++k) acc += A[k][r] * P2[k][c];
+                Co[r][c] = acc;
+                float t = A[0][r] * B[0][c];
+#pragma unroll
+                for (int k = 1; k < DIM; ++k) t += A[k][r] * B[k][c];
+                Tt[r][c] = t;
+            }
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+            float acc = Tt[r][0] * imp[0];
+#pragma unroll
+            for (int c = 1; c < DIM; ++c) acc += Tt[r][c] * imp[c];
+            mo[r] = acc;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+            mo[r] = imp[r];
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) Co[r][c] = C[r][c];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+        rec[P::kImp + r] = mo[r];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) rec[P::kC + r * DIM + c] = Co[r][c];
+    }
+    float4* d4 = reinterpret_cast<float4*>(dst);
+#pragma unroll
+    for (int k = 0; k < P::kRec / 4; ++k)
+        d4[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
 }
 
 // ---- reductions ----------------------------------------------------------

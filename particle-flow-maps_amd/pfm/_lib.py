@@ -50,6 +50,10 @@ _SIGS = {
     "pfm_bin_particles": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp],
     "pfm_cell_lists": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp],
     "pfm_p2g": [c_vp, c_i64] + [c_vp] * 11 + [c_vp, c_vp] + [c_vp] * 6 + [c_vp, c_sz, c_vp],
+    "pfm_rk4_particles_p2g_record": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_f] + [c_vp] * 8
+                                    + [c_vp],
+    "pfm_p2g_records": [c_vp, c_i64, c_vp, c_vp] + [c_vp] * 6 + [c_vp, c_sz, c_vp],
+    "pfm_p2g_record_buffer": [c_vp, c_i64, c_vp],
     "pfm_max_speed": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "pfm_split_central": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "pfm_get_central": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
@@ -77,6 +81,7 @@ _RESTYPES = {
     "pfm_last_error": ctypes.c_char_p,
     "pfm_p2g_workspace_size": c_sz,
     "pfm_mg_field": c_vp,
+    "pfm_p2g_record_buffer": c_vp,
 }
 
 _lib = None

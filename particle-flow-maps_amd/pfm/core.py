@@ -244,6 +244,28 @@ class Binning:
                self.workspace.data_ptr(), self.workspace.numel(), _stream(stream))
 
 
+    def rk4_then_p2g(self, dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, u_out, W=None, stream=None):
+        """RK4_T_forward followed by P2G (3D/run_3D.py:795-812) with the P2G
+        composition fused into the RK4 kernel: advect pos/T_grad_m in place,
+        bin the new positions, then scatter into u_out (may alias u_vel only
+        after the RK4 reads are done -- the launches are ordered on one stream)."""
+        d, n = pos.shape
+        g = L.make_grid(self.res, dx)
+        lib = L.load()
+        rec = lib.pfm_p2g_record_buffer(ctypes.byref(g), n, self.workspace.data_ptr())
+        if not rec:
+            raise L.PfmError("pfm_p2g_record_buffer failed")
+        L.call("pfm_rk4_particles_p2g_record", ctypes.byref(g), n, _pvec(pos, d, n, "pos"),
+               *_cols(Tgm, d, n, "T_grad_m"), *_u_ptrs(u_vel, self.res), float(dt), *_cols(Tin, d, n, "T_init"),
+               *_cols(init_C, d, n, "init_C"), _pvec(init_imp, d, n, "init_imp"), rec, _stream(stream))
+        _mark_written(pos)
+        self.bin(pos, dx, stream, force=True)
+        Wp = [L.ptr(W[a], name=f"W[{a}]") if (W is not None and a < d) else None for a in range(3)]
+        L.call("pfm_p2g_records", ctypes.byref(g), n, L.ptr(self.cell_start, dtype=torch.int32),
+               L.ptr(self.perm, dtype=torch.int32), *_u_ptrs(u_out, self.res), *Wp, self.workspace.data_ptr(),
+               self.workspace.numel(), _stream(stream))
+
+
 _BINNING_CACHE = {}
 
 

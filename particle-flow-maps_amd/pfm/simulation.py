@@ -49,6 +49,21 @@ class StepConfig:
     write_imp_grad_m: bool = True
 
 
+def reinit_decision(i: int, ik: int, cfg: StepConfig, dim: int):
+    """Reinitialisation decisions of step i: (long, reseed, short, new ik).
+
+    Long reinit when i % reinit_every == 0 (3D/run_3D.py:748, 2D/run_2D.py:804);
+    the 3D driver moves ik only when it reseeds (:751-755), the 2D driver at
+    every long reinit (2D/run_2D.py:819); short reinit when (i - ik) %
+    reinit_every_grad_m == 0 (3D/run_3D.py:767-768)."""
+    long_r = i % cfg.reinit_every == 0
+    reseed = long_r and (cfg.reinit_particle_pos or i == 0)
+    if long_r and (reseed or dim == 2):
+        ik = i
+    short = (i - ik) % cfg.reinit_every_grad_m == 0
+    return long_r, reseed, short, ik
+
+
 class Simulation:
     """One scene of the particle-flow-map solver, all state in HBM."""
 
@@ -91,6 +106,8 @@ class Simulation:
         self.solver = MGPCG(bt, list(self.res), dim=d, base_level=self.cfg.base_level)
         self.binning = core.Binning(self.res, n, dev)
         self.max_speed_buf = torch.empty(1, **f32)
+        # 3D: fuse the P2G particle composition into the RK4 kernel
+        self.fused_p2g = d == 3
         # schedule state (3D/run_3D.py:708-713)
         self.i = -1
         self.ik = 0
@@ -168,24 +185,28 @@ class Simulation:
 
     def _step3(self, i, dt):
         cfg = self.cfg
-        if i % cfg.reinit_every == 0:
-            if cfg.reinit_particle_pos or i == 0:
+        long_r, reseed, short, self.ik = reinit_decision(i, self.ik, cfg, 3)
+        if long_r:
+            if reseed:
                 self._seed()
-                self.ik = i
             core.g2p(self.res, self.dx, self.pos, self.u, self.init_imp, None)
             core.reset_T_identity(self.T_init)
             core.reset_T_identity(self.T_grad_m)
-        if (i - self.ik) % cfg.reinit_every_grad_m == 0:
+        if short:
             core.g2p(self.res, self.dx, self.pos, self.u,
                      self.imp_grad_m if self.imp_grad_m is not None else None, self.init_C)
             core.update_T(self.T_grad_m, self.T_init, self.T_init)
             core.reset_T_identity(self.T_grad_m)
         if cfg.use_midpoint_vel:
             self._midpoint(dt)
-        core.rk4_particles(self.res, self.dx, self.pos, self.T_grad_m, self.u, dt, True)
-        self.binning.bin(self.pos, self.dx, force=True)
-        self.binning.p2g(self.dx, self.pos, self.init_imp, self.init_C, self.u, W=self.p2g_weight,
-                         Tgm=self.T_grad_m, Tin=self.T_init)
+        if self.fused_p2g:
+            self.binning.rk4_then_p2g(self.dx, self.pos, self.T_grad_m, self.u, dt, self.T_init, self.init_C,
+                                      self.init_imp, self.u, W=self.p2g_weight)
+        else:
+            core.rk4_particles(self.res, self.dx, self.pos, self.T_grad_m, self.u, dt, True)
+            self.binning.bin(self.pos, self.dx, force=True)
+            self.binning.p2g(self.dx, self.pos, self.init_imp, self.init_C, self.u, W=self.p2g_weight,
+                             Tgm=self.T_grad_m, Tin=self.T_init)
         self.project()
 
     def initialize_2d_particles(self):
@@ -198,15 +219,15 @@ class Simulation:
 
     def _step2(self, i, dt):
         cfg = self.cfg
-        if i % cfg.reinit_every == 0:
-            if cfg.reinit_particle_pos or i == 0:
+        long_r, reseed, short, self.ik = reinit_decision(i, self.ik, cfg, 2)
+        if long_r:
+            if reseed:
                 self._seed()
-            self.ik = i
             core.g2p(self.res, self.dx, self.pos, self.u, self.imp, None)
             self.init_imp.copy_(self.imp)
             core.reset_T_identity(self.T)
             core.reset_T_identity(self.T_init)
-        if (i - self.ik) % cfg.reinit_every_grad_m == 0:
+        if short:
             core.g2p(self.res, self.dx, self.pos, self.u, self.imp_grad_m, self.init_C)
             core.reset_T_identity(self.T_grad_m)
             for a in range(2):

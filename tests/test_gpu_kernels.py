@@ -299,3 +299,36 @@ def test_point_vortices_and_ring(oracle, pfm):
     vc3 = torch.zeros((3,) + res3, device=DEV)
     core.vortex_ring(res3, 1 / 16, vc3, 1.0, 0.15, 0.02, [0.5, 0.5, 0.3], [1, 0, 0], [0, 1, 0], 200)
     assert rel_l2(N(vc3), want3) < 1e-5
+
+
+def test_fused_rk4_p2g_matches_separate(oracle, pfm):
+    """pfm_rk4_particles_p2g_record + pfm_p2g_records == RK4_T_forward then P2G."""
+    from pfm import core
+
+    res = (16, 16, 16)
+    pos, dx, n = _particles(oracle, res, seed=17, order=1)
+    u = smooth_mac_field(res, seed=18)
+    rng = np.random.default_rng(19)
+    Tgm = (np.eye(3)[:, :, None] + 0.05 * rng.normal(size=(3, 3, n))).astype(np.float32)
+    Tin = (np.eye(3)[:, :, None] + 0.05 * rng.normal(size=(3, 3, n))).astype(np.float32)
+    C0 = rng.normal(size=(3, 3, n)).astype(np.float32)
+    m0 = rng.normal(size=(3, n)).astype(np.float32)
+    dt = 0.01
+    # separate
+    p1, T1 = G(pos), [G(Tgm[m]) for m in range(3)]
+    ug = [G(x) for x in u]
+    core.rk4_particles(res, dx, p1, T1, ug, dt)
+    b1 = core.Binning(res, n)
+    u1 = [torch.empty(core.face_shape(res, a), device=DEV) for a in range(3)]
+    b1.p2g(dx, p1, G(m0), [G(C0[m]) for m in range(3)], u1, Tgm=T1, Tin=[G(Tin[m]) for m in range(3)])
+    # fused
+    p2, T2 = G(pos), [G(Tgm[m]) for m in range(3)]
+    b2 = core.Binning(res, n)
+    u2 = [torch.empty(core.face_shape(res, a), device=DEV) for a in range(3)]
+    b2.rk4_then_p2g(dx, p2, T2, ug, dt, [G(Tin[m]) for m in range(3)], [G(C0[m]) for m in range(3)], G(m0), u2)
+    np.testing.assert_array_equal(N(p1), N(p2))
+    for m in range(3):
+        np.testing.assert_array_equal(N(T1[m]), N(T2[m]))
+    np.testing.assert_array_equal(N(b1.perm), N(b2.perm))
+    for a in range(3):
+        assert rel_l2(N(u2[a]), N(u1[a])) < 1e-6
