@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t 
 // 2D/init_conditions.py:183-210 (interp_u_MAC_grad_imp, 3D/taichi_utils.py:326-376:
 // new_C rows equal the velocity gradient rows).
 template <int DIM, bool GRAD>
-__global__ void __launch_bounds__(kBlock, 4) k_g2p(GridDev g, int64_t n, const float* __restrict__ pos,
+__global__ void __launch_bounds__(kBlock, 2) k_g2p(GridDev g, int64_t n, const float* __restrict__ pos,
                                                 const float* __restrict__ ux,
                                                 const float* __restrict__ uy,
                                                 const float* __restrict__ uz,

@@ -76,17 +76,16 @@ __device__ __forceinline__ float N2(float x) {
     return ax < 0.5f ? 0.75f - ax * ax : r;
 }
 
-// Unaligned 16-byte row load of 4 consecutive floats (gfx950 global loads
-// only need dword alignment).
-typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
-
-// Per-axis stencil of interp_grad_2 (3D/taichi_utils.py:278-307):
-// u = p/dx - BL, s = clamp(u, 1, hi), iu = floor(s), f = s - iu, and the
-// weights N_2 / dN_2 at the distances f+1, f, f-1, f-2.
+// Per-axis stencil of interp_grad_2 (3D/taichi_utils.py:278-307): u = p/dx -
+// BL, s = clamp(u, 1, hi), iu = floor(s), f = s - iu.  The reference sums the
+// 4 nodes iu-1 .. iu+2, but a quadratic B-spline is zero at one of them:
+// N_2(f+1) = 0 when f >= 1/2 and N_2(f-2) = 0 when f <= 1/2.  The stencil
+// therefore keeps the 3 live nodes j0 .. j0+2 with j0 = iu-1 (f < 1/2) or iu
+// (f >= 1/2); the dropped term adds an exact zero in the reference.
 struct AxisStencil {
-    int i0;
-    float w[4];
-    float dw[4];
+    int j0;
+    float w[3];
+    float dw[3];
 };
 
 template <bool GRAD>
@@ -94,12 +93,19 @@ __device__ __forceinline__ void axis_stencil(float p, float inv_dx, float BL, fl
     const float u = p * inv_dx - BL;
     const float s = fmaxf(1.0f, fminf(u, hi));
     const float fl = floorf(s);
-    a.i0 = (int)fl;
     const float f = s - fl;
+    const bool up = f >= 0.5f;
+    a.j0 = (int)fl - 1 + (up ? 1 : 0);
+    float w4[4], dw4[4];
     if (GRAD)
-        bspline4(f, a.w, a.dw);
+        bspline4(f, w4, dw4);
     else
-        bspline4_val(f, a.w);
+        bspline4_val(f, w4);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        a.w[m] = up ? w4[m + 1] : w4[m];
+        if (GRAD) a.dw[m] = up ? dw4[m + 1] : dw4[m];
+    }
 }
 
 // The stencils of all MAC components at one point: along axis c, component c
@@ -118,22 +124,6 @@ struct MacStencil {
     }
 };
 
-// Interpolate MAC component A from its stencils: value and (if GRAD) the
-// gradient, the separable form of interp_grad_2 (3D/taichi_utils.py:274-307;
-// 2D :214-248).  Offsets are 32-bit (a face field has < 2^31 elements).
-//
-// Along the contiguous axis the 4 samples i0-1 .. i0+2 are clamped to the
-// last element (sample(), 3D/taichi_utils.py:48-54).  i0 lies in [1, S-2], so
-// only i0 = S-2 clamps; then the row is loaded from i0-2 and the weights are
-// shifted instead of the data: (0, w0, w1, w2 + w3) applied to the loaded
-// (v[i0-2], v[i0-1], v[i0], v[i0+1]).
-__device__ __forceinline__ void edge_weights(bool shift, const float w[4], float o[4]) {
-    o[0] = shift ? 0.f : w[0];
-    o[1] = shift ? w[0] : w[1];
-    o[2] = shift ? w[1] : w[2];
-    o[3] = shift ? w[2] + w[3] : w[3];
-}
-
 // Raw buffer resource over a face field (word 3 = 0x00020000 on gfx9-family
 // parts: raw 32-bit dwords).  Loads through it take a 32-bit byte offset, so a
 // row address costs one VALU add; offsets past num_records read zero.
@@ -142,6 +132,21 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t face_rsrc(const float* fld, ui
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(fld), (short)0, (int)bytes, 0x00020000);
 }
 
+// Weights of the 16-byte row load along the contiguous axis.  The row is
+// loaded from ls = min(j0, S-4) (so all 4 floats are inside the row); the 3
+// live nodes j0 .. j0+2 sit at lanes o = j0-ls .. o+2, and a node past the
+// end (j0+2 = S, clamped to S-1 by sample(), 3D/taichi_utils.py:48-54) folds
+// its weight onto the last lane.
+__device__ __forceinline__ void row_weights(int o, const float w[3], float q[4]) {
+    q[0] = o == 0 ? w[0] : 0.f;
+    q[1] = o == 0 ? w[1] : (o == 1 ? w[0] : 0.f);
+    q[2] = o == 0 ? w[2] : (o == 1 ? w[1] : w[0]);
+    q[3] = o == 0 ? 0.f : (o == 1 ? w[2] : w[1] + w[2]);
+}
+
+// Interpolate MAC component A from its stencils: value and (if GRAD) the
+// gradient, the separable form of interp_grad_2 (3D/taichi_utils.py:274-307;
+// 2D :214-248) over the 3^d live nodes.
 template <int DIM, int A, bool GRAD>
 __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const GridDev& g,
                                             const MacStencil<DIM, GRAD>& st, float& val, float grad[3]) {
@@ -152,30 +157,22 @@ __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const
     if (DIM == 3) {
         const AxisStencil& s2 = A == 2 ? st.F[2] : st.C[2];
         const int S2 = g.res[2] + (A == 2 ? 1 : 0);
-        int iy[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) iy[m] = min(s1.i0 + m - 1, S1 - 1);
-        const bool shift = s2.i0 + 2 > S2 - 1;
-        const uint32_t z0 = (uint32_t)(s2.i0 - 1 - (shift ? 1 : 0));
+        const int ls = min(s2.j0, S2 - 4);
         float wz[4], dwz[4];
-        edge_weights(shift, s2.w, wz);
-        if (GRAD) edge_weights(shift, s2.dw, dwz);
+        row_weights(s2.j0 - ls, s2.w, wz);
+        if (GRAD) row_weights(s2.j0 - ls, s2.dw, dwz);
         const uint32_t plane = (uint32_t)S1 * (uint32_t)S2 * 4u;
         const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, plane * (uint32_t)S0);
-        uint32_t rowoff[4];
+        uint32_t rowoff[3];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) rowoff[b] = ((uint32_t)iy[b] * (uint32_t)S2 + z0) * 4u;
-        // x-slab weights, rotated one slot per iteration
-        float wa[4] = {s0.w[0], s0.w[1], s0.w[2], s0.w[3]};
-        float dwa[4] = {s0.dw[0], s0.dw[1], s0.dw[2], s0.dw[3]};
+        for (int b = 0; b < 3; ++b) rowoff[b] = ((uint32_t)min(s1.j0 + b, S1 - 1) * (uint32_t)S2 + (uint32_t)ls) * 4u;
         float v0 = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-        // one x-slab (4 rows of 4) per iteration keeps 16 loaded values live
-#pragma unroll 1
-        for (int a = 0; a < 4; ++a) {
-            const uint32_t xoff = (uint32_t)min(s0.i0 + a - 1, S0 - 1) * plane;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t xoff = (uint32_t)min(s0.j0 + a, S0 - 1) * plane;
             float c_v = 0.f, c_dy = 0.f, c_dz = 0.f;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
+            for (int b = 0; b < 3; ++b) {
                 const v4f q = __builtin_amdgcn_raw_buffer_load_b128(rs, xoff + rowoff[b], 0, 0);
                 const float rz = wz[0] * q.x + wz[1] * q.y + wz[2] * q.z + wz[3] * q.w;
                 c_v += s1.w[b] * rz;
@@ -185,14 +182,12 @@ __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const
                     c_dz += s1.w[b] * rdz;
                 }
             }
-            v0 += wa[0] * c_v;
+            v0 += s0.w[a] * c_v;
             if (GRAD) {
-                gx += dwa[0] * c_v;
-                gy += wa[0] * c_dy;
-                gz += wa[0] * c_dz;
+                gx += s0.dw[a] * c_v;
+                gy += s0.w[a] * c_dy;
+                gz += s0.w[a] * c_dz;
             }
-            wa[0] = wa[1]; wa[1] = wa[2]; wa[2] = wa[3];
-            if (GRAD) { dwa[0] = dwa[1]; dwa[1] = dwa[2]; dwa[2] = dwa[3]; }
         }
         val = v0;
         if (GRAD) {
@@ -201,16 +196,17 @@ __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const
             grad[2] = gz * g.inv_dx;
         }
     } else {
-        const bool shift = s1.i0 + 2 > S1 - 1;
-        const uint32_t y0 = (uint32_t)(s1.i0 - 1 - (shift ? 1 : 0));
+        const int ls = min(s1.j0, S1 - 4);
         float wy[4], dwy[4];
-        edge_weights(shift, s1.w, wy);
-        if (GRAD) edge_weights(shift, s1.dw, dwy);
+        row_weights(s1.j0 - ls, s1.w, wy);
+        if (GRAD) row_weights(s1.j0 - ls, s1.dw, dwy);
+        const uint32_t rowb = (uint32_t)S1 * 4u;
+        const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, rowb * (uint32_t)S0);
         float v0 = 0.f, gx = 0.f, gy = 0.f;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const uint32_t off = (uint32_t)min(s0.i0 + a - 1, S0 - 1) * (uint32_t)S1 + y0;
-            const f4u q = *reinterpret_cast<const f4u*>(fld + off);
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t off = (uint32_t)min(s0.j0 + a, S0 - 1) * rowb + (uint32_t)ls * 4u;
+            const v4f q = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
             const float ry = wy[0] * q.x + wy[1] * q.y + wy[2] * q.z + wy[3] * q.w;
             v0 += s0.w[a] * ry;
             if (GRAD) {
