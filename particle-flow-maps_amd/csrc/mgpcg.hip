@@ -1,11 +1,21 @@
 // mgpcg.hip -- multigrid-preconditioned CG for the MAC pressure Poisson
 // problem (3D/mgpcg.py, 2D/mgpcg.py), restated for gfx950:
 //   - every level is a dense array (the reference's pointer SNodes are fully
-//     activated by the first solve, so they behave as dense);
+//     activated by the first solve, so they behave as dense); a 2D level of
+//     n0 x n1 cells is addressed as the 3D level (1, n0, n1), so one set of
+//     stencil kernels serves both dimensions;
 //   - red/black Gauss-Seidel, Ap, restriction and prolongation use the
 //     reference's operation order in fp32 without contraction, so each pass
 //     is bit-identical to a sequential restatement; restriction is a gather of
 //     the 2^d children in fixed order instead of float atomics;
+//   - inside the V-cycle two colour sweeps run per launch on an LDS tile with
+//     a two-cell halo (the first sweep also updates the inner halo ring, so
+//     the second needs nothing from other tiles); the zero fill before the
+//     pre-smoother, the prolongation before the post-smoother, the residual
+//     recentring and the z.r product are folded into those launches, and the
+//     residual + restriction runs on an LDS tile as well;
+//   - the search-direction update p = z + beta p is folded into the Ap launch
+//     (p ping-pongs between two buffers so neighbouring tiles read old p);
 //   - dot products accumulate in FP64 (wave shuffles, then a block tree, then
 //     a fixed-order sum of the per-block partials); alpha, beta and the
 //     recentring mean are formed on the device, so the only host round trip
@@ -15,6 +25,7 @@
 #include <new>
 
 #include "pfm_host.h"
+#include "pfm_div.h"
 
 namespace pfm {
 
@@ -33,7 +44,7 @@ struct MGScalars {
 };
 
 struct Level {
-    int d[3];
+    int d[3];      // (1, n0, n1) for a 2D level
     int64_t n;
 };
 
@@ -41,23 +52,27 @@ struct MGHandle {
     int dim;
     int N[3];
     int n_levels;
+    int coarse_level;       // first level run by k_coarse, -1 if none
     int bc[3][2];
     int dirichlet_counts;
     int precision;
     Level lv[kMaxLevels];
     float* r[kMaxLevels];
     float* z[kMaxLevels];
+    float* z2[kMaxLevels];  // ping-pong partner of z inside the fused sweeps
     float* x;
-    float* p;
+    float* p;               // current search direction
+    float* p2;              // its ping-pong partner
     float* Ap;
-    double* partial;     // max(2 * kMgRedBlocks, blocks of a level-0 stencil launch)
-    MGScalars* sc;       // device
-    double* host_rtr;    // pinned
+    double* partial;        // max(2 * kMgRedBlocks, tiles of level 0)
+    int n_partial;
+    MGScalars* sc;          // device
+    double* host_rtr;       // pinned
 };
 
-// Stencil coefficients shared by all kernels of one level.
+// Stencil description of one level, always in 3D index form.
 struct LvlDev {
-    int dim;
+    int nb2;                 // 2 * dim: neighbour count of an interior cell
     int d[3];
     int bc_lo[3], bc_hi[3];  // 1 when that wall decrements the diagonal
 };
@@ -68,60 +83,63 @@ __device__ __forceinline__ void cell_ijk(const LvlDev& L, int64_t q, int I[3]) {
     I[0] = (int)(q / ((int64_t)L.d[2] * L.d[1]));
 }
 
-// Thread -> cell mapping for the stencil kernels: a 3D launch (64 x 4
-// threads per block) with the fastest array axis on threadIdx.x, so no
-// integer division is needed.  3D: (i, j, k) = (block.z, y, x); 2D: (i, j) =
-// (y, x).  With `pairs`, x counts pairs of cells along the fast axis and the
-// thread takes the member whose colour (I.sum() & 1) equals `phase`.
+// Thread -> cell mapping of the per-cell kernels: 64 x 4 threads per block,
+// the contiguous axis on threadIdx.x, axis 0 on blockIdx.z.  With `pairs`, x
+// counts pairs of cells along the contiguous axis and the thread takes the
+// member whose colour (I.sum() & 1) equals `phase`.
 constexpr int kTx = 64, kTy = 4;
 
 __device__ __forceinline__ bool cell_of_thread(const LvlDev& L, int phase, bool pairs, int I[3]) {
     const int fx = blockIdx.x * kTx + threadIdx.x;
-    const int fy = blockIdx.y * kTy + threadIdx.y;
-    if (L.dim == 3) {
-        I[0] = blockIdx.z;
-        I[1] = fy;
-        I[2] = pairs ? 2 * fx + ((I[0] + I[1] + phase) & 1) : fx;
-    } else {
-        I[0] = fy;
-        I[1] = pairs ? 2 * fx + ((I[0] + phase) & 1) : fx;
-        I[2] = 0;
-    }
+    I[0] = blockIdx.z;
+    I[1] = blockIdx.y * kTy + threadIdx.y;
+    I[2] = pairs ? 2 * fx + ((I[0] + I[1] + phase) & 1) : fx;
     return I[0] < L.d[0] && I[1] < L.d[1] && I[2] < L.d[2];
 }
 
-static inline dim3 cell_grid(const int d[3], int dim, bool pairs) {
-    const int fast = dim == 3 ? d[2] : d[1];
-    const int mid = dim == 3 ? d[1] : d[0];
-    const int nx = pairs ? (fast + 1) / 2 : fast;
-    return dim3((unsigned)((nx + kTx - 1) / kTx), (unsigned)((mid + kTy - 1) / kTy),
-                (unsigned)(dim == 3 ? d[0] : 1));
+static inline dim3 cell_grid(const int d[3], bool pairs) {
+    const int nx = pairs ? (d[2] + 1) / 2 : d[2];
+    return dim3((unsigned)((nx + kTx - 1) / kTx), (unsigned)((d[1] + kTy - 1) / kTy), (unsigned)d[0]);
 }
 
 __device__ __forceinline__ int64_t cell_index(const LvlDev& L, const int I[3]) {
     return ((int64_t)I[0] * L.d[1] + I[1]) * L.d[2] + I[2];
 }
 
+__device__ __forceinline__ bool inside(const LvlDev& L, int g0, int g1, int g2) {
+    return (unsigned)g0 < (unsigned)L.d[0] && (unsigned)g1 < (unsigned)L.d[1] && (unsigned)g2 < (unsigned)L.d[2];
+}
+
 // num_fluid_neighbors: 3D/mgpcg.py:106-115.
-__device__ __forceinline__ float nfn(const LvlDev& L, const int I[3]) {
-    float num = 2.0f * (float)L.dim;
-    for (int a = 0; a < L.dim; ++a) {
+__device__ __forceinline__ float nfn(const LvlDev& L, int i0, int i1, int i2) {
+    float num = (float)L.nb2;
+    const int I[3] = {i0, i1, i2};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
         if (I[a] <= 0 && L.bc_lo[a]) num -= 1.0f;
         if (I[a] >= L.d[a] - 1 && L.bc_hi[a]) num -= 1.0f;
     }
     return num;
 }
 
-// neighbor_sum: 3D/mgpcg.py:92-104 (per axis: +e then -e).
-__device__ __forceinline__ float nsum(const LvlDev& L, const float* __restrict__ x, int64_t q,
-                                      const int I[3]) {
+__device__ __forceinline__ float nfn(const LvlDev& L, const int I[3]) { return nfn(L, I[0], I[1], I[2]); }
+
+// neighbor_sum: 3D/mgpcg.py:92-104 (per axis: +e then -e).  `x` points at
+// the cell; s0 / s1 are the strides of axes 0 / 1 (axis 2 has stride 1).
+__device__ __forceinline__ float nsum_at(const LvlDev& L, const float* x, int64_t s0, int64_t s1, int i0, int i1,
+                                         int i2) {
     float ret = 0.0f;
-    int64_t stride[3] = {(int64_t)L.d[1] * L.d[2], (int64_t)L.d[2], 1};
-    for (int a = 0; a < L.dim; ++a) {
-        if (I[a] < L.d[a] - 1) ret = __fadd_rn(ret, x[q + stride[a]]);
-        if (I[a] > 0) ret = __fadd_rn(ret, x[q - stride[a]]);
-    }
+    if (i0 < L.d[0] - 1) ret = __fadd_rn(ret, x[s0]);
+    if (i0 > 0) ret = __fadd_rn(ret, x[-s0]);
+    if (i1 < L.d[1] - 1) ret = __fadd_rn(ret, x[s1]);
+    if (i1 > 0) ret = __fadd_rn(ret, x[-s1]);
+    if (i2 < L.d[2] - 1) ret = __fadd_rn(ret, x[1]);
+    if (i2 > 0) ret = __fadd_rn(ret, x[-1]);
     return ret;
+}
+
+__device__ __forceinline__ float nsum(const LvlDev& L, const float* __restrict__ x, int64_t q, const int I[3]) {
+    return nsum_at(L, x + q, (int64_t)L.d[1] * L.d[2], L.d[2], I[0], I[1], I[2]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_fill_f32(float* p, int64_t n, float v) {
@@ -129,6 +147,8 @@ __global__ void __launch_bounds__(kBlock) k_fill_f32(float* p, int64_t n, float 
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+
+// ---- single-pass kernels (the reference's kernels one by one) -------------
 
 // smooth: 3D/mgpcg.py:165-172 (one colour; one thread per cell of that colour).
 __global__ void __launch_bounds__(kBlock) k_smooth(LvlDev L, float* __restrict__ z,
@@ -139,44 +159,7 @@ __global__ void __launch_bounds__(kBlock) k_smooth(LvlDev L, float* __restrict__
     z[q] = __fdiv_rn(__fadd_rn(r[q], nsum(L, z, q, I)), nfn(L, I));
 }
 
-// compute_Ap (3D/mgpcg.py:117-122) fused with the p.Ap reduction; one
-// partial per block, blocks numbered linearly.
-__global__ void __launch_bounds__(kBlock) k_ap_dot(LvlDev L, const float* __restrict__ p,
-                                                   float* __restrict__ Ap, double* __restrict__ partial) {
-    __shared__ double sh[kBlock / 64];
-    double s = 0.0;
-    // 3D: blockIdx.z strides over the x-planes so the grid stays <= kMgRedBlocks
-    const int planes = L.dim == 3 ? L.d[0] : 1;
-    for (int pl = blockIdx.z; pl < planes; pl += gridDim.z) {
-        int I[3];
-        const int fx = blockIdx.x * kTx + threadIdx.x;
-        const int fy = blockIdx.y * kTy + threadIdx.y;
-        if (L.dim == 3) {
-            I[0] = pl; I[1] = fy; I[2] = fx;
-        } else {
-            I[0] = fy; I[1] = fx; I[2] = 0;
-        }
-        if (I[0] < L.d[0] && I[1] < L.d[1] && I[2] < L.d[2]) {
-            const int64_t q = cell_index(L, I);
-            const float pq = p[q];
-            const float a = __fsub_rn(__fmul_rn(nfn(L, I), pq), nsum(L, p, q, I));
-            Ap[q] = a;
-            s += (double)pq * (double)a;
-        }
-    }
-    const int tid = threadIdx.y * kTx + threadIdx.x;
-    // block_sum expects a 1D block index
-    s = wave_sum(s);
-    if ((tid & 63) == 0) sh[tid >> 6] = s;
-    __syncthreads();
-    if (tid == 0) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) t += sh[w];
-        partial[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
-    }
-}
-
-// get_Ap for tests (no reduction).
+// compute_Ap (3D/mgpcg.py:117-122) without the reduction (get_Ap).
 __global__ void __launch_bounds__(kBlock) k_apply_A(LvlDev L, const float* __restrict__ p,
                                                     float* __restrict__ Ap) {
     int I[3];
@@ -185,30 +168,33 @@ __global__ void __launch_bounds__(kBlock) k_apply_A(LvlDev L, const float* __res
     Ap[q] = __fsub_rn(__fmul_rn(nfn(L, I), p[q]), nsum(L, p, q, I));
 }
 
-// restrict (3D/mgpcg.py:152-158) as a gather over the 2^d children.
-__global__ void __launch_bounds__(kBlock) k_restrict(LvlDev Lf, LvlDev Lc,
+// restrict (3D/mgpcg.py:152-158) as a gather over the 2^d children.  Any
+// extents: the last coarse cell of an odd fine extent also owns the trailing
+// fine cell (fine I maps to min(I/2, coarse-1)).
+__global__ void __launch_bounds__(kBlock) k_restrict(LvlDev Lf, LvlDev Lc, float scale,
                                                      const float* __restrict__ z,
                                                      const float* __restrict__ r,
                                                      float* __restrict__ rc) {
     int C[3];
     if (!cell_of_thread(Lc, 0, false, C)) return;
     const int64_t qc = cell_index(Lc, C);
-    const float scale = (float)Lf.dim - 1.0f;
-    // children 2C, 2C+1 per axis; the last coarse cell of an odd fine extent
-    // also owns the trailing fine cell (fine I maps to min(I/2, coarse-1))
     int hi[3];
     for (int a = 0; a < 3; ++a) hi[a] = (C[a] == Lc.d[a] - 1) ? Lf.d[a] - 1 - 2 * C[a] : 1;
-    if (Lf.dim == 2) hi[2] = 0;
     float acc = 0.0f;
     for (int a = 0; a <= hi[0]; ++a)
         for (int b = 0; b <= hi[1]; ++b)
             for (int c = 0; c <= hi[2]; ++c) {
-                int I[3] = {2 * C[0] + a, 2 * C[1] + b, Lf.dim == 3 ? 2 * C[2] + c : 0};
-                const int64_t q = ((int64_t)I[0] * Lf.d[1] + I[1]) * Lf.d[2] + I[2];
+                int I[3] = {2 * C[0] + a, 2 * C[1] + b, 2 * C[2] + c};
+                const int64_t q = cell_index(Lf, I);
                 const float res = __fsub_rn(r[q], __fsub_rn(__fmul_rn(nfn(Lf, I), z[q]), nsum(Lf, z, q, I)));
                 acc = __fadd_rn(acc, __fdiv_rn(__fmul_rn(res, 1.0f), scale));
             }
     rc[qc] = acc;
+}
+
+__device__ __forceinline__ int64_t parent_index(const LvlDev& Lc, int i0, int i1, int i2) {
+    const int c0 = min(i0 >> 1, Lc.d[0] - 1), c1 = min(i1 >> 1, Lc.d[1] - 1), c2 = min(i2 >> 1, Lc.d[2] - 1);
+    return ((int64_t)c0 * Lc.d[1] + c1) * Lc.d[2] + c2;
 }
 
 // prolongate: 3D/mgpcg.py:160-163.
@@ -217,16 +203,608 @@ __global__ void __launch_bounds__(kBlock) k_prolongate(LvlDev Lf, LvlDev Lc, flo
     int I[3];
     if (!cell_of_thread(Lf, 0, false, I)) return;
     const int64_t q = cell_index(Lf, I);
-    const int c0 = min(I[0] / 2, Lc.d[0] - 1), c1 = min(I[1] / 2, Lc.d[1] - 1);
-    const int c2 = Lf.dim == 3 ? min(I[2] / 2, Lc.d[2] - 1) : 0;
-    const int64_t qc = ((int64_t)c0 * Lc.d[1] + c1) * Lc.d[2] + c2;
-    z[q] = __fadd_rn(z[q], zc[qc]);
+    z[q] = __fadd_rn(z[q], zc[parent_index(Lc, I[0], I[1], I[2])]);
+}
+
+// ---- LDS-tiled V-cycle kernels --------------------------------------------
+
+// Interior tile of the sweep and Ap kernels along (axis 0, axis 1, axis 2).
+template <int DIM> struct MgTile;
+template <> struct MgTile<3> { static constexpr int I = 8, J = 16, K = 32; };
+template <> struct MgTile<2> { static constexpr int I = 1, J = 32, K = 64; };
+// Coarse tile of the restriction (the fine tile is twice as large per axis,
+// except axis 0 of a 2D level, which has extent 1 on every level).
+template <int DIM> struct RsTile;
+template <> struct RsTile<3> { static constexpr int I = 4, J = 8, K = 16; };
+template <> struct RsTile<2> { static constexpr int I = 1, J = 16, K = 32; };
+
+struct TileMap {
+    int n0, n1, n2;  // tiles along each axis
+    int total;       // tiles in the launch (= blocks)
+};
+
+// Tile of this block.  Workgroups are dealt to the 8 XCDs round robin, so the
+// block index is permuted to give each XCD a contiguous run of tiles, and
+// tiles are numbered with axis 0 fastest: the tiles that share halo cells
+// with a tile ran just before it on the same XCD, so the halo is still in
+// that XCD's L2.
+__device__ __forceinline__ void tile_of_block(const TileMap& tm, int t[3]) {
+    int b = blockIdx.x;
+    if ((tm.total & 7) == 0) b = (b & 7) * (tm.total >> 3) + (b >> 3);
+    t[0] = b % tm.n0;
+    b /= tm.n0;
+    t[2] = b % tm.n2;
+    t[1] = b / tm.n2;
+}
+
+// Buffer view of a level array: the tiled kernels load through 32-bit byte
+// offsets, and an offset past the end (kOOB) reads 0, which is the value the
+// LDS tiles hold for cells outside the level.  Loads are issued without
+// branches, so all of a thread's loads can be in flight at once.
+constexpr uint32_t kOOB = 0xFFFFFFF0u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t lvl_rsrc(const float* p, const LvlDev& L) {
+    return face_rsrc(p, (uint32_t)L.d[0] * (uint32_t)L.d[1] * (uint32_t)L.d[2] * 4u);
+}
+
+__device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t rs, uint32_t boff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, boff, 0, 0));
+}
+
+// Byte offset of cell g, or kOOB outside the level.
+__device__ __forceinline__ uint32_t cell_off(const LvlDev& L, int g0, int g1, int g2) {
+    return inside(L, g0, g1, g2) ? (uint32_t)(((g0 * L.d[1] + g1) * L.d[2] + g2) * 4) : kOOB;
+}
+
+// Walls next to cell g that decrement its diagonal (num_fluid_neighbors).
+__device__ __forceinline__ int walls(const LvlDev& L, int g0, int g1, int g2) {
+    return (g0 <= 0 ? L.bc_lo[0] : 0) + (g0 >= L.d[0] - 1 ? L.bc_hi[0] : 0) + (g1 <= 0 ? L.bc_lo[1] : 0) +
+           (g1 >= L.d[1] - 1 ? L.bc_hi[1] : 0) + (g2 <= 0 ? L.bc_lo[2] : 0) + (g2 >= L.d[2] - 1 ? L.bc_hi[2] : 0);
+}
+
+// Does every cell of the box [lo, hi) lie strictly inside the level (no
+// wall and all six neighbours present)?
+__device__ __forceinline__ bool box_interior(const LvlDev& L, int lo0, int lo1, int lo2, int hi0, int hi1, int hi2,
+                                             bool axis0) {
+    return (!axis0 || (lo0 >= 1 && hi0 <= L.d[0] - 1)) && lo1 >= 1 && hi1 <= L.d[1] - 1 && lo2 >= 1 &&
+           hi2 <= L.d[2] - 1;
+}
+
+// neighbor_sum (3D/mgpcg.py:92-104) on an LDS tile.  A neighbour outside the
+// level is an LDS cell holding +0, and adding +0 never changes the running
+// sum (it starts at +0 and so can never be -0), so the reference's skipped
+// terms need no test.  2D levels have no neighbours along axis 0.
+template <int DIM>
+__device__ __forceinline__ float nsum_lds(const float* x, int sp, int sk) {
+    float ret = 0.0f;
+    if (DIM == 3) {
+        ret = __fadd_rn(ret, x[sp]);
+        ret = __fadd_rn(ret, x[-sp]);
+    }
+    ret = __fadd_rn(ret, x[sk]);
+    ret = __fadd_rn(ret, x[-sk]);
+    ret = __fadd_rn(ret, x[1]);
+    ret = __fadd_rn(ret, x[-1]);
+    return ret;
+}
+
+// Diagonal n (as float) and RN(1/n) of cell g; `interior` (block uniform)
+// skips the wall tests.
+struct Diag {
+    float n, y;
+};
+
+__device__ __forceinline__ Diag diag(const LvlDev& L, bool interior, float n_in, float y_in, int g0, int g1,
+                                     int g2) {
+    Diag dg{n_in, y_in};
+    if (!interior) {
+        const int n = L.nb2 - walls(L, g0, g1, g2);
+        dg.n = (float)n;
+        dg.y = pfm_rcp_small(n);
+    }
+    return dg;
+}
+
+// One slot of a red/black pass over a region of RI x RJ x (2 RK2) cells whose
+// first cell is the LDS cell (l0, l1, l2): slot e is a pair of cells along
+// axis 2, of which the member of colour `col` is taken.
+template <int RJ, int RK2>
+__device__ __forceinline__ void rb_slot(int e, int l0, int l1, int l2, int b0, int b1, int b2, int col, int& li,
+                                        int& lj, int& lk, int& g0, int& g1, int& g2) {
+    const int pk = e % RK2, rj = (e / RK2) % RJ, ri = e / (RK2 * RJ);
+    li = l0 + ri;
+    lj = l1 + rj;
+    g0 = b0 + li;
+    g1 = b1 + lj;
+    const int lk0 = l2 + 2 * pk;
+    lk = lk0 + ((col + g0 + g1 + b2 + lk0) & 1);
+    g2 = b2 + lk;
+}
+
+// Two red/black passes (colours c0, then c0 ^ 1) over one tile: the
+// reference's smooth(l, c0); smooth(l, c0 ^ 1) (3D/mgpcg.py:165-172,
+// 186-188/197-199).  The first pass updates its colour on the tile plus a
+// one-cell ring, reading the two-cell halo of z_in; the second updates the
+// other colour on the tile, so every cell sees exactly the values a
+// sequential sweep would.  Options:
+//   ZERO_IN  z_in is all zero (the pre-smoother after z.fill(0), :185,190);
+//   PROLONG  z_in += zc[parent] on load (prolongate, :160-163,196);
+//   SUB_MEAN use r - mean (the recentring of solve, 3D/mgpcg.py:174-180);
+//   DOT      write per-block partials of z_out . r.
+// All global loads (z tile, r of both passes) are issued before the first
+// barrier; the passes are branch free apart from the block-uniform choice
+// between interior tiles (no walls) and tiles that touch the boundary.
+template <int DIM, bool ZERO_IN, bool PROLONG, bool SUB_MEAN, bool DOT>
+__global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap tm,
+                                                 const float* __restrict__ zin, const float* __restrict__ zc,
+                                                 const float* __restrict__ r, float* __restrict__ zout, int c0,
+                                                 const MGScalars* __restrict__ sc, double* __restrict__ partial) {
+    using T = MgTile<DIM>;
+    constexpr int H = 2, HI = DIM == 3 ? H : 0;
+    constexpr int SI = T::I + 2 * HI, SJ = T::J + 2 * H, SK = T::K + 2 * H, SP = SJ * SK, NS = SI * SP;
+    constexpr int NL = (NS + kBlock - 1) / kBlock;
+    // pass 0: tile + one-cell ring; pass 1: tile
+    constexpr int A0 = DIM == 3 ? 1 : 0;
+    constexpr int R0I = T::I + 2 * A0, R0J = T::J + 2, R0K2 = (T::K + 2) / 2, N0 = R0I * R0J * R0K2;
+    constexpr int R1J = T::J, R1K2 = T::K / 2, N1 = T::I * R1J * R1K2;
+    constexpr int NP0 = (N0 + kBlock - 1) / kBlock, NP1 = (N1 + kBlock - 1) / kBlock;
+    static_assert(T::K % 2 == 0, "pairs along axis 2");
+    __shared__ float zs[NS];
+    int t[3];
+    tile_of_block(tm, t);
+    const int o0 = t[0] * T::I, o1 = t[1] * T::J, o2 = t[2] * T::K;
+    const int b0 = o0 - HI, b1 = o1 - H, b2 = o2 - H;  // global cell of zs[0]
+    const int tid = threadIdx.x;
+    const int c1 = c0 ^ 1;
+    const __amdgpu_buffer_rsrc_t rr = lvl_rsrc(r, L);
+    float zv[NL], rv0[NP0], rv1[NP1];
+    if (!ZERO_IN) {
+        const __amdgpu_buffer_rsrc_t rz = lvl_rsrc(zin, L);
+        const __amdgpu_buffer_rsrc_t rc = lvl_rsrc(zc, Lc);
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int e = tid + k * kBlock;
+            const int lk = e % SK, lj = (e / SK) % SJ, li = e / SP;
+            const int g0 = b0 + li, g1 = b1 + lj, g2 = b2 + lk;
+            const bool in = e < NS && inside(L, g0, g1, g2);
+            float v = ld(rz, in ? cell_off(L, g0, g1, g2) : kOOB);
+            // outside the level both loads read 0 and 0 + 0 stays +0
+            if (PROLONG) v = __fadd_rn(v, ld(rc, in ? (uint32_t)parent_index(Lc, g0, g1, g2) * 4u : kOOB));
+            zv[k] = v;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NP0; ++k) {
+        int li, lj, lk, g0, g1, g2;
+        rb_slot<R0J, R0K2>(min(tid + k * kBlock, N0 - 1), HI - A0, H - 1, H - 1, b0, b1, b2, c0, li, lj, lk, g0,
+                           g1, g2);
+        rv0[k] = ld(rr, cell_off(L, g0, g1, g2));
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int e = tid + k * kBlock;
+        if (e < NS) zs[e] = ZERO_IN ? 0.f : zv[k];
+    }
+    const float mean = SUB_MEAN ? sc->mean : 0.f;
+    // the pass-0 region [o - A0, o + T + A0) x ... decides the interior case
+    const bool interior =
+        box_interior(L, o0 - A0, o1 - 1, o2 - 1, o0 + T::I + A0, o1 + T::J + 1, o2 + T::K + 1, DIM == 3);
+    const float n_int = (float)L.nb2, y_int = pfm_rcp_small(L.nb2);
+    double s = 0.0;
+    __syncthreads();
+    // the thread index is hidden from the optimiser past the barrier so the
+    // slot coordinates are recomputed rather than held in registers
+    int tid2 = tid;
+    asm volatile("" : "+v"(tid2));
+    // r of the second pass arrives while the first one runs
+#pragma unroll
+    for (int k = 0; k < NP1; ++k) {
+        int li, lj, lk, g0, g1, g2;
+        rb_slot<R1J, R1K2>(min(tid2 + k * kBlock, N1 - 1), HI, H, H, b0, b1, b2, c1, li, lj, lk, g0, g1, g2);
+        rv1[k] = ld(rr, cell_off(L, g0, g1, g2));
+    }
+#pragma unroll
+    for (int k = 0; k < NP0; ++k) {
+        const int e = tid2 + k * kBlock;
+        int li, lj, lk, g0, g1, g2;
+        rb_slot<R0J, R0K2>(min(e, N0 - 1), HI - A0, H - 1, H - 1, b0, b1, b2, c0, li, lj, lk, g0, g1, g2);
+        const int le = (li * SJ + lj) * SK + lk;
+        const float rv = SUB_MEAN ? __fsub_rn(rv0[k], mean) : rv0[k];
+        const Diag dg = diag(L, interior, n_int, y_int, g0, g1, g2);
+        const float v = pfm_div_small(__fadd_rn(rv, nsum_lds<DIM>(zs + le, SP, SK)), dg.n, dg.y);
+        if (e < N0 && inside(L, g0, g1, g2)) zs[le] = v;
+        // cells of colour c0 inside the tile are final after this pass
+        if (DOT && e < N0 && g0 >= o0 && g0 < o0 + T::I && g1 >= o1 && g1 < o1 + T::J && g2 >= o2 &&
+            g2 < o2 + T::K && inside(L, g0, g1, g2))
+            s += (double)v * (double)rv;
+    }
+    __syncthreads();
+    const bool interior1 = box_interior(L, o0, o1, o2, o0 + T::I, o1 + T::J, o2 + T::K, DIM == 3);
+    int tid3 = tid;
+    asm volatile("" : "+v"(tid3));
+#pragma unroll
+    for (int k = 0; k < NP1; ++k) {
+        const int e = tid3 + k * kBlock;
+        int li, lj, lk, g0, g1, g2;
+        rb_slot<R1J, R1K2>(min(e, N1 - 1), HI, H, H, b0, b1, b2, c1, li, lj, lk, g0, g1, g2);
+        const int le = (li * SJ + lj) * SK + lk;
+        const float rv = SUB_MEAN ? __fsub_rn(rv1[k], mean) : rv1[k];
+        const Diag dg = diag(L, interior1, n_int, y_int, g0, g1, g2);
+        const float v = pfm_div_small(__fadd_rn(rv, nsum_lds<DIM>(zs + le, SP, SK)), dg.n, dg.y);
+        const bool in = e < N1 && inside(L, g0, g1, g2);
+        if (in) zs[le] = v;
+        if (DOT && in) s += (double)v * (double)rv;
+    }
+    __syncthreads();
+    constexpr int NT = T::I * T::J * T::K;
+#pragma unroll 4
+    for (int e = tid; e < NT; e += kBlock) {
+        const int tk = e % T::K, tj = (e / T::K) % T::J, ti = e / (T::K * T::J);
+        const int g0 = o0 + ti, g1 = o1 + tj, g2 = o2 + tk;
+        if (!inside(L, g0, g1, g2)) continue;
+        zout[(g0 * L.d[1] + g1) * L.d[2] + g2] = zs[((ti + HI) * SJ + tj + H) * SK + tk + H];
+    }
+    if (DOT) {
+        __shared__ double sh[kBlock / 64];
+        s = block_sum(s, sh);
+        if (tid == 0) partial[blockIdx.x] = s;
+    }
+}
+
+// Residual r - A z on an LDS tile and its restriction (3D/mgpcg.py:152-158):
+// each coarse cell sums its 2^d children in fixed order.  Requires every fine
+// extent to be twice the coarse one (axis 0 of 2D levels: both 1).  With
+// SUB_MEAN the recentred r (3D/mgpcg.py:174-180) is used and written back;
+// each fine r is touched by exactly one block, so this is race free.
+// `inv_scale` is 1 / (dim - 1) (0.5 or 1), so the product equals the
+// reference's division exactly.
+template <int DIM, bool SUB_MEAN>
+__global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc, TileMap tm, float inv_scale,
+                                                           const float* __restrict__ z, float* __restrict__ r,
+                                                           float* __restrict__ rc,
+                                                           const MGScalars* __restrict__ sc) {
+    using C = RsTile<DIM>;
+    constexpr int FI = DIM == 3 ? 2 * C::I : 1, FJ = 2 * C::J, FK = 2 * C::K, NF = FI * FJ * FK;
+    constexpr int HI = DIM == 3 ? 1 : 0;
+    constexpr int SI = FI + 2 * HI, SJ = FJ + 2, SK = FK + 2, SP = SJ * SK, NS = SI * SP;
+    constexpr int NL = (NS + kBlock - 1) / kBlock, NR = (NF + kBlock - 1) / kBlock;
+    static_assert(NF % kBlock == 0, "whole fine tile per thread sweep");
+    __shared__ float zs[NS];
+    __shared__ float ts[NF];
+    int t[3];
+    tile_of_block(tm, t);
+    const int c0 = t[0] * C::I, c1 = t[1] * C::J, c2 = t[2] * C::K;
+    const int o0 = DIM == 3 ? 2 * c0 : 0, o1 = 2 * c1, o2 = 2 * c2;
+    const int b0 = o0 - HI, b1 = o1 - 1, b2 = o2 - 1;
+    const int tid = threadIdx.x;
+    const __amdgpu_buffer_rsrc_t rz = lvl_rsrc(z, Lf), rr = lvl_rsrc(r, Lf);
+    float zv[NL], rv[NR];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int e = tid + k * kBlock;
+        const int lk = e % SK, lj = (e / SK) % SJ, li = e / SP;
+        zv[k] = ld(rz, e < NS ? cell_off(Lf, b0 + li, b1 + lj, b2 + lk) : kOOB);
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int e = tid + k * kBlock;
+        const int fk = e % FK, fj = (e / FK) % FJ, fi = e / (FK * FJ);
+        rv[k] = ld(rr, cell_off(Lf, o0 + fi, o1 + fj, o2 + fk));
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int e = tid + k * kBlock;
+        if (e < NS) zs[e] = zv[k];
+    }
+    const float mean = SUB_MEAN ? sc->mean : 0.f;
+    const bool interior = box_interior(Lf, o0, o1, o2, o0 + FI, o1 + FJ, o2 + FK, DIM == 3);
+    const float n_int = (float)Lf.nb2;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int e = tid + k * kBlock;
+        const int fk = e % FK, fj = (e / FK) % FJ, fi = e / (FK * FJ);
+        const int g0 = o0 + fi, g1 = o1 + fj, g2 = o2 + fk;
+        const bool in = inside(Lf, g0, g1, g2);
+        float rq = rv[k];
+        if (SUB_MEAN) {
+            rq = __fsub_rn(rq, mean);
+            if (in) r[(g0 * Lf.d[1] + g1) * Lf.d[2] + g2] = rq;
+        }
+        const int le = ((fi + HI) * SJ + fj + 1) * SK + fk + 1;
+        const float nf = interior ? n_int : (float)(Lf.nb2 - walls(Lf, g0, g1, g2));
+        const float res = __fsub_rn(rq, __fsub_rn(__fmul_rn(nf, zs[le]), nsum_lds<DIM>(zs + le, SP, SK)));
+        ts[e] = __fmul_rn(__fmul_rn(res, 1.0f), inv_scale);
+    }
+    __syncthreads();
+    constexpr int NA = DIM == 3 ? 2 : 1;
+    for (int e = tid; e < C::I * C::J * C::K; e += kBlock) {
+        const int ck = e % C::K, cj = (e / C::K) % C::J, ci = e / (C::K * C::J);
+        const int g0 = c0 + ci, g1 = c1 + cj, g2 = c2 + ck;
+        if (!inside(Lc, g0, g1, g2)) continue;
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                    acc = __fadd_rn(acc, ts[((NA * ci + a) * FJ + 2 * cj + b) * FK + 2 * ck + c]);
+        rc[(g0 * Lc.d[1] + g1) * Lc.d[2] + g2] = acc;
+    }
+}
+
+// update_p (3D/mgpcg.py:147-150) fused with compute_Ap (:117-122) and the
+// p.Ap reduction: p_new = z + beta p_old on the tile and a one-cell halo,
+// then Ap from LDS.  p_new goes to a separate buffer because neighbouring
+// tiles still read p_old.
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const float* __restrict__ z,
+                                                     const float* __restrict__ p_old, float* __restrict__ p_new,
+                                                     float* __restrict__ Ap, const MGScalars* __restrict__ sc,
+                                                     double* __restrict__ partial) {
+    using T = MgTile<DIM>;
+    constexpr int HI = DIM == 3 ? 1 : 0;
+    constexpr int SI = T::I + 2 * HI, SJ = T::J + 2, SK = T::K + 2, SP = SJ * SK, NS = SI * SP;
+    constexpr int NL = (NS + kBlock - 1) / kBlock;
+    constexpr int NT = T::I * T::J * T::K;
+    static_assert(NT % kBlock == 0, "whole tile per thread sweep");
+    __shared__ float ps[NS];
+    __shared__ double sh[kBlock / 64];
+    int t[3];
+    tile_of_block(tm, t);
+    const int o0 = t[0] * T::I, o1 = t[1] * T::J, o2 = t[2] * T::K;
+    const int b0 = o0 - HI, b1 = o1 - 1, b2 = o2 - 1;
+    const int tid = threadIdx.x;
+    const __amdgpu_buffer_rsrc_t rz = lvl_rsrc(z, L), rp = lvl_rsrc(p_old, L);
+    float zv[NL], pv[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int e = tid + k * kBlock;
+        const int lk = e % SK, lj = (e / SK) % SJ, li = e / SP;
+        const uint32_t off = e < NS ? cell_off(L, b0 + li, b1 + lj, b2 + lk) : kOOB;
+        zv[k] = ld(rz, off);
+        pv[k] = ld(rp, off);
+    }
+    const float beta = sc->beta;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int e = tid + k * kBlock;
+        const int lk = e % SK, lj = (e / SK) % SJ, li = e / SP;
+        if (e < NS) ps[e] = inside(L, b0 + li, b1 + lj, b2 + lk) ? __fadd_rn(zv[k], __fmul_rn(beta, pv[k])) : 0.f;
+    }
+    const bool interior = box_interior(L, o0, o1, o2, o0 + T::I, o1 + T::J, o2 + T::K, DIM == 3);
+    const float n_int = (float)L.nb2;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll 4
+    for (int e = tid; e < NT; e += kBlock) {
+        const int tk = e % T::K, tj = (e / T::K) % T::J, ti = e / (T::K * T::J);
+        const int g0 = o0 + ti, g1 = o1 + tj, g2 = o2 + tk;
+        const int le = ((ti + HI) * SJ + tj + 1) * SK + tk + 1;
+        const float pq = ps[le];
+        const float nf = interior ? n_int : (float)(L.nb2 - walls(L, g0, g1, g2));
+        const float a = __fsub_rn(__fmul_rn(nf, pq), nsum_lds<DIM>(ps + le, SP, SK));
+        if (inside(L, g0, g1, g2)) {
+            const int q = (g0 * L.d[1] + g1) * L.d[2] + g2;
+            p_new[q] = pq;
+            Ap[q] = a;
+            s += (double)pq * (double)a;
+        }
+    }
+    s = block_sum(s, sh);
+    if (tid == 0) partial[blockIdx.x] = s;
+}
+
+// ---- coarse levels in one workgroup --------------------------------------
+// The levels from `coarse_level` down to the bottom are small; launching their
+// sweeps one by one costs a kernel's fixed latency each.  k_coarse runs that
+// whole tail of the V-cycle (pre-smoothing, restriction, the bottom solve,
+// prolongation, post-smoothing) in one workgroup out of LDS, with the same
+// per-cell arithmetic and colour order as the level kernels.  z is kept with
+// a one-cell zero border (axes 1 and 2 only for 2D levels) so the neighbour
+// sums need no tests.
+constexpr int kCoarseMaxLevels = 4;
+constexpr int kCoarseLds = 14336;  // floats (56 KiB)
+constexpr int kCoarseSlots = 4;    // bottom-level cells per colour per thread
+
+struct CoarseArgs {
+    int nl;         // levels handled; the last one is the bottom
+    int sweeps;     // bottom red/black iterations
+    float inv_scale;
+    int zoff[kCoarseMaxLevels], roff[kCoarseMaxLevels];
+    LvlDev L[kCoarseMaxLevels];
+    float* z[kCoarseMaxLevels];
+    float* r[kCoarseMaxLevels];
+};
+
+template <int DIM>
+struct Padded {
+    int s1, s0;  // LDS strides of axes 1 and 0 (axis 2 has stride 1)
+    __device__ __forceinline__ explicit Padded(const LvlDev& L) {
+        s1 = L.d[2] + 2;
+        s0 = (L.d[1] + 2) * s1;
+    }
+    __device__ __forceinline__ int at(int i, int j, int k) const {
+        return (DIM == 3 ? (i + 1) * s0 : 0) + (j + 1) * s1 + k + 1;
+    }
+};
+
+template <int DIM>
+__host__ __device__ __forceinline__ int padded_size(const int d[3]) {
+    return (DIM == 3 ? d[0] + 2 : 1) * (d[1] + 2) * (d[2] + 2);
+}
+
+// smooth(l, c) (3D/mgpcg.py:165-172) on one padded LDS level; one row of the
+// level per thread.
+template <int DIM>
+__device__ void coarse_pass(const LvlDev& L, float* zs, const float* rs, int c) {
+    const Padded<DIM> P(L);
+    const int h2 = (L.d[2] + 1) / 2;
+    for (int row = threadIdx.x; row < L.d[0] * L.d[1]; row += blockDim.x) {
+        const int i = row / L.d[1], j = row - i * L.d[1];
+        const int zb = P.at(i, j, 0), rb = row * L.d[2];
+        const int off = (c + i + j) & 1;
+        for (int pk = 0; pk < h2; ++pk) {
+            const int k = 2 * pk + off;
+            if (k >= L.d[2]) break;
+            const int n = L.nb2 - walls(L, i, j, k);
+            zs[zb + k] = pfm_div_small(__fadd_rn(rs[rb + k], nsum_lds<DIM>(zs + zb + k, P.s0, P.s1)), (float)n,
+                                       pfm_rcp_small(n));
+        }
+    }
+}
+
+// Residual and restriction (3D/mgpcg.py:152-158) from LDS level f to LDS
+// level c, children summed in fixed order, odd extents as in k_restrict.
+template <int DIM>
+__device__ void coarse_restrict(const LvlDev& Lf, const LvlDev& Lc, const float* zs, const float* rs, float* rcs,
+                                float inv_scale) {
+    const Padded<DIM> P(Lf);
+    for (int row = threadIdx.x; row < Lc.d[0] * Lc.d[1]; row += blockDim.x) {
+        const int C0 = row / Lc.d[1], C1 = row - C0 * Lc.d[1];
+        const int h0 = C0 == Lc.d[0] - 1 ? Lf.d[0] - 1 - 2 * C0 : 1;
+        const int h1 = C1 == Lc.d[1] - 1 ? Lf.d[1] - 1 - 2 * C1 : 1;
+        for (int C2 = 0; C2 < Lc.d[2]; ++C2) {
+            const int h2 = C2 == Lc.d[2] - 1 ? Lf.d[2] - 1 - 2 * C2 : 1;
+            float acc = 0.0f;
+            for (int a = 0; a <= h0; ++a)
+                for (int b = 0; b <= h1; ++b)
+                    for (int c = 0; c <= h2; ++c) {
+                        const int i = 2 * C0 + a, j = 2 * C1 + b, k = 2 * C2 + c;
+                        const int zq = P.at(i, j, k);
+                        const float nf = (float)(Lf.nb2 - walls(Lf, i, j, k));
+                        const float res = __fsub_rn(rs[(i * Lf.d[1] + j) * Lf.d[2] + k],
+                                                    __fsub_rn(__fmul_rn(nf, zs[zq]), nsum_lds<DIM>(zs + zq, P.s0, P.s1)));
+                        acc = __fadd_rn(acc, __fmul_rn(__fmul_rn(res, 1.0f), inv_scale));
+                    }
+            rcs[row * Lc.d[2] + C2] = acc;
+        }
+    }
+}
+
+// prolongate (3D/mgpcg.py:160-163) between padded LDS levels.
+template <int DIM>
+__device__ void coarse_prolong(const LvlDev& Lf, const LvlDev& Lc, float* zs, const float* zcs) {
+    const Padded<DIM> P(Lf), Pc(Lc);
+    for (int row = threadIdx.x; row < Lf.d[0] * Lf.d[1]; row += blockDim.x) {
+        const int i = row / Lf.d[1], j = row - i * Lf.d[1];
+        const int c0 = min(i >> 1, Lc.d[0] - 1), c1 = min(j >> 1, Lc.d[1] - 1);
+        for (int k = 0; k < Lf.d[2]; ++k) {
+            const int q = P.at(i, j, k);
+            zs[q] = __fadd_rn(zs[q], zcs[Pc.at(c0, c1, min(k >> 1, Lc.d[2] - 1))]);
+        }
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, int dot,
+                                                   const MGScalars* __restrict__ sc, double* __restrict__ partial) {
+    __shared__ float sm[kCoarseLds];
+    const int tid = threadIdx.x;
+    for (int l = 0; l < a.nl; ++l)
+        for (int e = tid; e < padded_size<DIM>(a.L[l].d); e += kBlock) sm[a.zoff[l] + e] = 0.f;
+    {
+        const LvlDev& L0 = a.L[0];
+        const int n0 = L0.d[0] * L0.d[1] * L0.d[2];
+        const float mean = sub_mean ? sc->mean : 0.f;
+        for (int q = tid; q < n0; q += kBlock) {
+            float v = a.r[0][q];
+            if (sub_mean) {
+                v = __fsub_rn(v, mean);
+                a.r[0][q] = v;
+            }
+            sm[a.roff[0] + q] = v;
+        }
+    }
+    __syncthreads();
+    for (int l = 0; l < a.nl - 1; ++l) {
+        for (int it = 0; it < 2; ++it)
+            for (int c = 0; c < 2; ++c) {
+                coarse_pass<DIM>(a.L[l], sm + a.zoff[l], sm + a.roff[l], c);
+                __syncthreads();
+            }
+        coarse_restrict<DIM>(a.L[l], a.L[l + 1], sm + a.zoff[l], sm + a.roff[l], sm + a.roff[l + 1], a.inv_scale);
+        __syncthreads();
+    }
+    // bottom: each thread keeps its cells of each colour decoded in registers
+    {
+        const int b = a.nl - 1;
+        const LvlDev& L = a.L[b];
+        const Padded<DIM> P(L);
+        float* zs = sm + a.zoff[b];
+        const float* rs = sm + a.roff[b];
+        const int h2 = (L.d[2] + 1) / 2, nslot = L.d[0] * L.d[1] * h2;
+        int bl[2][kCoarseSlots];
+        float br[2][kCoarseSlots], bn[2][kCoarseSlots], by[2][kCoarseSlots];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int m = 0; m < kCoarseSlots; ++m) {
+                const int s = tid + m * kBlock;
+                bl[c][m] = -1;
+                br[c][m] = bn[c][m] = by[c][m] = 0.f;
+                if (s < nslot) {
+                    const int row = s / h2, pk = s - row * h2;
+                    const int i = row / L.d[1], j = row - i * L.d[1];
+                    const int k = 2 * pk + ((c + i + j) & 1);
+                    if (k < L.d[2]) {
+                        const int n = L.nb2 - walls(L, i, j, k);
+                        bl[c][m] = P.at(i, j, k);
+                        br[c][m] = rs[row * L.d[2] + k];
+                        bn[c][m] = (float)n;
+                        by[c][m] = pfm_rcp_small(n);
+                    }
+                }
+            }
+        for (int it = 0; it < a.sweeps; ++it)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                for (int m = 0; m < kCoarseSlots; ++m)
+                    if (bl[c][m] >= 0)
+                        zs[bl[c][m]] = pfm_div_small(__fadd_rn(br[c][m], nsum_lds<DIM>(zs + bl[c][m], P.s0, P.s1)),
+                                                     bn[c][m], by[c][m]);
+                __syncthreads();
+            }
+    }
+    for (int l = a.nl - 2; l >= 0; --l) {
+        coarse_prolong<DIM>(a.L[l], a.L[l + 1], sm + a.zoff[l], sm + a.zoff[l + 1]);
+        __syncthreads();
+        for (int it = 0; it < 2; ++it)
+            for (int c = 1; c >= 0; --c) {
+                coarse_pass<DIM>(a.L[l], sm + a.zoff[l], sm + a.roff[l], c);
+                __syncthreads();
+            }
+    }
+    // write every level back, as the level-by-level V-cycle leaves them
+    double s = 0.0;
+    for (int l = 0; l < a.nl; ++l) {
+        const LvlDev& L = a.L[l];
+        const Padded<DIM> P(L);
+        for (int row = tid; row < L.d[0] * L.d[1]; row += kBlock) {
+            const int i = row / L.d[1], j = row - i * L.d[1];
+            for (int k = 0; k < L.d[2]; ++k) {
+                const float zq = sm[a.zoff[l] + P.at(i, j, k)];
+                const int q = row * L.d[2] + k;
+                a.z[l][q] = zq;
+                if (l > 0) a.r[l][q] = sm[a.roff[l] + q];
+                if (dot && l == 0) s += (double)zq * (double)sm[a.roff[0] + q];
+            }
+        }
+    }
+    if (dot) {
+        __shared__ double sh[kBlock / 64];
+        s = block_sum(s, sh);
+        if (tid == 0) partial[0] = s;
+    }
 }
 
 // Bottom solve: `sweeps` red/black iterations on a level small enough for LDS
-// (apply_preconditioner's bottom loop, 3D/mgpcg.py:192-194).  Each thread owns
-// up to kBottomPer cells whose indices, colours, diagonals and right-hand
-// sides are decoded once; the sweeps then only touch LDS.
+// (apply_preconditioner's bottom loop, 3D/mgpcg.py:192-194), starting from
+// z = 0 (the fill at :190).  Each thread owns up to kBottomPer cells whose
+// indices, colours, diagonals and right-hand sides are decoded once; the
+// sweeps then only touch LDS.
 constexpr int kBottomMax = 8192;
 constexpr int kBottomThreads = 1024;
 constexpr int kBottomPer = kBottomMax / kBottomThreads;
@@ -251,10 +829,10 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom(LvlDev L, int n, floa
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 // neighbour order of neighbor_sum: +e then -e per axis; -1 = absent
-                nb[k][2 * a] = (a < L.dim && I[a] < L.d[a] - 1) ? q + st[a] : -1;
-                nb[k][2 * a + 1] = (a < L.dim && I[a] > 0) ? q - st[a] : -1;
+                nb[k][2 * a] = I[a] < L.d[a] - 1 ? q + st[a] : -1;
+                nb[k][2 * a + 1] = I[a] > 0 ? q - st[a] : -1;
             }
-            zs[q] = z[q];
+            zs[q] = 0.f;
         }
     }
     __syncthreads();
@@ -276,6 +854,8 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom(LvlDev L, int n, floa
     for (int k = 0; k < kBottomPer; ++k)
         if (qi[k] < n) z[qi[k]] = zs[qi[k]];
 }
+
+// ---- vector kernels of the CG recurrence ----------------------------------
 
 // Partial sums of a.b (and, if b == nullptr, of a) in double.
 __global__ void __launch_bounds__(kBlock) k_dot(int64_t n, const float* __restrict__ a,
@@ -315,7 +895,7 @@ __global__ void __launch_bounds__(kBlock) k_update_xr(int64_t n, float* __restri
     }
 }
 
-// update_p: 3D/mgpcg.py:147-150.
+// update_p: 3D/mgpcg.py:147-150 (in place; used where no Ap follows).
 __global__ void __launch_bounds__(kBlock) k_update_p(int64_t n, float* __restrict__ p,
                                                      const float* __restrict__ z,
                                                      const MGScalars* __restrict__ sc) {
@@ -385,6 +965,8 @@ __global__ void k_reset_scalars(MGScalars* sc) {
     sc->beta = beta;
 }
 
+// ---- field kernels (velocity <-> level 0) ---------------------------------
+
 // divergence (3D/mgpcg.py:380-390) fused with init(r, k) (:71-79).
 template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_div_init(LvlDev L, const float* __restrict__ ux,
@@ -396,10 +978,9 @@ __global__ void __launch_bounds__(kBlock) k_div_init(LvlDev L, const float* __re
     int I[3];
     if (!cell_of_thread(L, 0, false, I)) return;
     const int64_t q = cell_index(L, I);
-    const int rx = L.d[0], ry = L.d[1], rz = L.d[2];
-    (void)rx;
     float div;
     if (DIM == 3) {
+        const int ry = L.d[1], rz = L.d[2];
         const float vl = ux[((int64_t)I[0] * ry + I[1]) * rz + I[2]];
         const float vr = ux[((int64_t)(I[0] + 1) * ry + I[1]) * rz + I[2]];
         const float vb = uy[((int64_t)I[0] * (ry + 1) + I[1]) * rz + I[2]];
@@ -408,10 +989,12 @@ __global__ void __launch_bounds__(kBlock) k_div_init(LvlDev L, const float* __re
         const float vc = uz[((int64_t)I[0] * ry + I[1]) * (rz + 1) + I[2] + 1];
         div = __fsub_rn(__fadd_rn(__fsub_rn(__fadd_rn(__fsub_rn(vr, vl), vt), vb), vc), va);
     } else {
-        const float vl = ux[(int64_t)I[0] * ry + I[1]];
-        const float vr = ux[(int64_t)(I[0] + 1) * ry + I[1]];
-        const float vb = uy[(int64_t)I[0] * (ry + 1) + I[1]];
-        const float vt = uy[(int64_t)I[0] * (ry + 1) + I[1] + 1];
+        // 2D cell (i, j) sits at (0, i, j); u_x is (n0 + 1, n1), u_y (n0, n1 + 1)
+        const int i = I[1], j = I[2], n1 = L.d[2];
+        const float vl = ux[(int64_t)i * n1 + j];
+        const float vr = ux[(int64_t)(i + 1) * n1 + j];
+        const float vb = uy[(int64_t)i * (n1 + 1) + j];
+        const float vt = uy[(int64_t)i * (n1 + 1) + j + 1];
         div = __fsub_rn(__fadd_rn(__fsub_rn(vr, vl), vt), vb);
     }
     r[q] = __fmul_rn(div, k);
@@ -435,20 +1018,17 @@ __global__ void __launch_bounds__(kBlock) k_init_rhs(int64_t n, const float* __r
 }
 
 // subtract_grad_p: 3D/mgpcg.py:392-418 (pressure outside the domain = 0).
-template <int DIM>
+// F describes the face array of `axis` (3D index form, extent d[axis] + 1).
 __global__ void __launch_bounds__(kBlock) k_subtract_grad_p(LvlDev L, LvlDev F, int axis,
                                                             const float* __restrict__ pr,
                                                             float* __restrict__ u) {
-    // F describes the face array of `axis` (extent d[axis] + 1)
     int I[3];
     if (!cell_of_thread(F, 0, false, I)) return;
     const int64_t f = cell_index(F, I);
     int R[3] = {I[0], I[1], I[2]}, Lf[3] = {I[0], I[1], I[2]};
     Lf[axis] -= 1;
-    const float prv = R[axis] >= L.d[axis]
-                          ? 0.0f
-                          : pr[((int64_t)R[0] * L.d[1] + R[1]) * L.d[2] + R[2]];
-    const float plv = Lf[axis] < 0 ? 0.0f : pr[((int64_t)Lf[0] * L.d[1] + Lf[1]) * L.d[2] + Lf[2]];
+    const float prv = R[axis] >= L.d[axis] ? 0.0f : pr[cell_index(L, R)];
+    const float plv = Lf[axis] < 0 ? 0.0f : pr[cell_index(L, Lf)];
     u[f] = __fsub_rn(u[f], __fsub_rn(prv, plv));
 }
 
@@ -469,21 +1049,57 @@ __global__ void __launch_bounds__(kBlock) k_zero_plane(int S0, int S1, int S2, i
     u[((int64_t)I[0] * S[1] + I[1]) * S[2] + I[2]] = 0.0f;
 }
 
+// ---- host side ------------------------------------------------------------
+
+// Axis of the 3D index form that holds axis a of the problem.
+static inline int map_axis(const MGHandle* h, int a) { return a + (3 - h->dim); }
+
 static LvlDev lvl_dev(const MGHandle* h, int l) {
     LvlDev L;
-    L.dim = h->dim;
-    for (int a = 0; a < 3; ++a) {
-        L.d[a] = h->lv[l].d[a];
-        int lo = a < h->dim ? h->bc[a][0] : 0, hi = a < h->dim ? h->bc[a][1] : 0;
-        L.bc_lo[a] = lo == 2 || (h->dirichlet_counts && lo == 1);
-        L.bc_hi[a] = hi == 2 || (h->dirichlet_counts && hi == 1);
+    L.nb2 = 2 * h->dim;
+    for (int m = 0; m < 3; ++m) {
+        L.d[m] = h->lv[l].d[m];
+        L.bc_lo[m] = L.bc_hi[m] = 0;
+    }
+    for (int a = 0; a < h->dim; ++a) {
+        const int m = map_axis(h, a), lo = h->bc[a][0], hi = h->bc[a][1];
+        L.bc_lo[m] = lo == 2 || (h->dirichlet_counts && lo == 1);
+        L.bc_hi[m] = hi == 2 || (h->dirichlet_counts && hi == 1);
     }
     return L;
 }
 
-static unsigned red_blocks(int64_t n) {
-    (void)n;
-    return kMgRedBlocks;
+static inline float restrict_scale(const MGHandle* h) { return (float)h->dim - 1.0f; }
+
+// 1 / (dim - 1): 0.5 or 1, exact, so x * inv equals x / (dim - 1) bit for bit.
+static inline float restrict_inv_scale(const MGHandle* h) { return h->dim == 3 ? 0.5f : 1.0f; }
+
+static TileMap tile_map(const int d[3], int ti, int tj, int tk) {
+    TileMap tm;
+    tm.n0 = (d[0] + ti - 1) / ti;
+    tm.n1 = (d[1] + tj - 1) / tj;
+    tm.n2 = (d[2] + tk - 1) / tk;
+    tm.total = tm.n0 * tm.n1 * tm.n2;
+    return tm;
+}
+
+static TileMap sweep_tiles(const MGHandle* h, int l) {
+    return h->dim == 3 ? tile_map(h->lv[l].d, MgTile<3>::I, MgTile<3>::J, MgTile<3>::K)
+                       : tile_map(h->lv[l].d, MgTile<2>::I, MgTile<2>::J, MgTile<2>::K);
+}
+
+static TileMap restrict_tiles(const MGHandle* h, int l) {
+    return h->dim == 3 ? tile_map(h->lv[l + 1].d, RsTile<3>::I, RsTile<3>::J, RsTile<3>::K)
+                       : tile_map(h->lv[l + 1].d, RsTile<2>::I, RsTile<2>::J, RsTile<2>::K);
+}
+
+// The tiled restriction needs fine extents of exactly twice the coarse ones.
+static bool restrict_tiled_ok(const MGHandle* h, int l) {
+    for (int m = 0; m < 3; ++m) {
+        const int f = h->lv[l].d[m], c = h->lv[l + 1].d[m];
+        if (!(f == 2 * c || (f == 1 && c == 1 && m < 3 - h->dim))) return false;
+    }
+    return true;
 }
 
 static int fill(float* p, int64_t n, float v, hipStream_t s) {
@@ -498,14 +1114,14 @@ static const dim3 kCellBlock(kTx, kTy, 1);
 
 static int smooth(MGHandle* h, int l, int phase, hipStream_t s) {
     LvlDev L = lvl_dev(h, l);
-    hipLaunchKernelGGL(k_smooth, cell_grid(L.d, h->dim, true), kCellBlock, 0, s, L, h->z[l], h->r[l], phase);
+    hipLaunchKernelGGL(k_smooth, cell_grid(L.d, true), kCellBlock, 0, s, L, h->z[l], h->r[l], phase);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
 
 static int restrict_level(MGHandle* h, int l, hipStream_t s) {
     LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
-    hipLaunchKernelGGL(k_restrict, cell_grid(Lc.d, h->dim, false), kCellBlock, 0, s, Lf, Lc, h->z[l],
+    hipLaunchKernelGGL(k_restrict, cell_grid(Lc.d, false), kCellBlock, 0, s, Lf, Lc, restrict_scale(h), h->z[l],
                        h->r[l], h->r[l + 1]);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
@@ -513,8 +1129,7 @@ static int restrict_level(MGHandle* h, int l, hipStream_t s) {
 
 static int prolongate_level(MGHandle* h, int l, hipStream_t s) {
     LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
-    hipLaunchKernelGGL(k_prolongate, cell_grid(Lf.d, h->dim, false), kCellBlock, 0, s, Lf, Lc, h->z[l],
-                       h->z[l + 1]);
+    hipLaunchKernelGGL(k_prolongate, cell_grid(Lf.d, false), kCellBlock, 0, s, Lf, Lc, h->z[l], h->z[l + 1]);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -525,35 +1140,172 @@ static int prolongate_level(MGHandle* h, int l, hipStream_t s) {
         if (_rc) return _rc;     \
     } while (0)
 
-// apply_preconditioner: 3D/mgpcg.py:182-200.
-static int vcycle(MGHandle* h, hipStream_t s) {
-    const int L = h->n_levels;
-    PFM_TRY(fill(h->z[0], h->lv[0].n, 0.f, s));
-    for (int l = 0; l < L - 1; ++l) {
-        for (int i = 0; i < 2; ++i) {
-            PFM_TRY(smooth(h, l, 0, s));
-            PFM_TRY(smooth(h, l, 1, s));
-        }
-        PFM_TRY(fill(h->z[l + 1], h->lv[l + 1].n, 0.f, s));
-        PFM_TRY(restrict_level(h, l, s));
+enum SweepKind { kSweepPlain = 0, kSweepZero = 1, kSweepProlong = 2 };
+
+template <int DIM>
+static void launch_rbgs(int kind, bool sub_mean, bool dot, dim3 g, hipStream_t s, LvlDev L, LvlDev Lc, TileMap tm,
+                        const float* zin, const float* zc, const float* r, float* zout, int c0,
+                        const MGScalars* sc, double* partial) {
+#define PFM_RBGS(Z, P, M, D) \
+    hipLaunchKernelGGL((k_rbgs<DIM, Z, P, M, D>), g, dim3(kBlock), 0, s, L, Lc, tm, zin, zc, r, zout, c0, sc, partial)
+    if (kind == kSweepZero) {
+        if (sub_mean) PFM_RBGS(true, false, true, false);
+        else PFM_RBGS(true, false, false, false);
+    } else if (kind == kSweepProlong) {
+        PFM_RBGS(false, true, false, false);
+    } else if (sub_mean) {
+        PFM_RBGS(false, false, true, false);
+    } else if (dot) {
+        PFM_RBGS(false, false, false, true);
+    } else {
+        PFM_RBGS(false, false, false, false);
     }
-    const int b = L - 1;
+#undef PFM_RBGS
+}
+
+// Two colour passes (c0, c0 ^ 1) on level l, z_in -> z_out.
+static int rbgs(MGHandle* h, int l, int kind, int c0, const float* zin, float* zout, bool sub_mean, bool dot,
+                hipStream_t s) {
+    const LvlDev L = lvl_dev(h, l);
+    const LvlDev Lc = kind == kSweepProlong ? lvl_dev(h, l + 1) : L;
+    const float* zc = kind == kSweepProlong ? h->z[l + 1] : nullptr;
+    const TileMap tm = sweep_tiles(h, l);
+    const dim3 g((unsigned)tm.total);
+    if (h->dim == 3)
+        launch_rbgs<3>(kind, sub_mean, dot, g, s, L, Lc, tm, zin, zc, h->r[l], zout, c0, h->sc, h->partial);
+    else
+        launch_rbgs<2>(kind, sub_mean, dot, g, s, L, Lc, tm, zin, zc, h->r[l], zout, c0, h->sc, h->partial);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+// Residual + restriction of level l into r[l + 1] (tiled when the extents
+// halve exactly).  With sub_mean, r[l] is recentred first (written back).
+static int restrict_fused(MGHandle* h, int l, bool sub_mean, hipStream_t s) {
+    if (!restrict_tiled_ok(h, l)) {
+        if (sub_mean) {
+            hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[l].n, h->r[l], h->sc);
+            PFM_LAUNCH_CHECK();
+        }
+        return restrict_level(h, l, s);
+    }
+    const LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
+    const TileMap tm = restrict_tiles(h, l);
+    const dim3 g((unsigned)tm.total);
+    const float sc = restrict_inv_scale(h);
+    if (h->dim == 3) {
+        if (sub_mean)
+            hipLaunchKernelGGL((k_restrict_tiled<3, true>), g, dim3(kBlock), 0, s, Lf, Lc, tm, sc, h->z[l], h->r[l],
+                               h->r[l + 1], h->sc);
+        else
+            hipLaunchKernelGGL((k_restrict_tiled<3, false>), g, dim3(kBlock), 0, s, Lf, Lc, tm, sc, h->z[l],
+                               h->r[l], h->r[l + 1], h->sc);
+    } else {
+        if (sub_mean)
+            hipLaunchKernelGGL((k_restrict_tiled<2, true>), g, dim3(kBlock), 0, s, Lf, Lc, tm, sc, h->z[l], h->r[l],
+                               h->r[l + 1], h->sc);
+        else
+            hipLaunchKernelGGL((k_restrict_tiled<2, false>), g, dim3(kBlock), 0, s, Lf, Lc, tm, sc, h->z[l],
+                               h->r[l], h->r[l + 1], h->sc);
+    }
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+// 50 red/black iterations from z = 0 on the coarsest level (3D/mgpcg.py:190-194).
+static int bottom(MGHandle* h, int b, hipStream_t s) {
     if (h->lv[b].n <= kBottomMax) {
-        LvlDev Lb = lvl_dev(h, b);
+        const LvlDev Lb = lvl_dev(h, b);
         hipLaunchKernelGGL(k_bottom, dim3(1), dim3(kBottomThreads), 0, s, Lb, (int)h->lv[b].n, h->z[b], h->r[b], 50);
         PFM_LAUNCH_CHECK();
-    } else {
-        for (int i = 0; i < 50; ++i) {
-            PFM_TRY(smooth(h, b, 0, s));
-            PFM_TRY(smooth(h, b, 1, s));
-        }
+        return PFM_OK;
     }
-    for (int l = L - 2; l >= 0; --l) {
-        PFM_TRY(prolongate_level(h, l, s));
-        for (int i = 0; i < 2; ++i) {
-            PFM_TRY(smooth(h, l, 1, s));
-            PFM_TRY(smooth(h, l, 0, s));
-        }
+    // one launch per red/black iteration, alternating buffers so the last lands in z[b]
+    const float* zin = nullptr;
+    for (int j = 0; j < 50; ++j) {
+        float* zout = ((49 - j) & 1) ? h->z2[b] : h->z[b];
+        PFM_TRY(rbgs(h, b, j == 0 ? kSweepZero : kSweepPlain, 0, zin, zout, false, false, s));
+        zin = zout;
+    }
+    return PFM_OK;
+}
+
+// Level from which k_coarse can run the rest of the V-cycle (all those
+// levels fit its LDS and the bottom fits its register slots), or -1.
+static int pick_coarse_level(const MGHandle* h) {
+    const int L = h->n_levels;
+    const int* db = h->lv[L - 1].d;
+    if ((int64_t)db[0] * db[1] * ((db[2] + 1) / 2) > (int64_t)kCoarseSlots * kBlock) return -1;
+    int best = -1;
+    int64_t used = 0;
+    for (int l = L - 1; l >= 0 && L - l <= kCoarseMaxLevels; --l) {
+        used += (h->dim == 3 ? padded_size<3>(h->lv[l].d) : padded_size<2>(h->lv[l].d)) + h->lv[l].n;
+        if (used > kCoarseLds) break;
+        best = l;
+    }
+    return best;
+}
+
+static int coarse(MGHandle* h, bool sub_mean, bool dot, hipStream_t s) {
+    CoarseArgs a{};
+    const int lc = h->coarse_level;
+    a.nl = h->n_levels - lc;
+    a.sweeps = 50;
+    a.inv_scale = restrict_inv_scale(h);
+    int off = 0;
+    for (int i = 0; i < a.nl; ++i) {
+        const int l = lc + i;
+        a.L[i] = lvl_dev(h, l);
+        a.zoff[i] = off;
+        off += h->dim == 3 ? padded_size<3>(h->lv[l].d) : padded_size<2>(h->lv[l].d);
+        a.roff[i] = off;
+        off += (int)h->lv[l].n;
+        a.z[i] = h->z[l];
+        a.r[i] = h->r[l];
+    }
+    if (h->dim == 3)
+        hipLaunchKernelGGL(k_coarse<3>, dim3(1), dim3(kBlock), 0, s, a, (int)sub_mean, (int)dot, h->sc, h->partial);
+    else
+        hipLaunchKernelGGL(k_coarse<2>, dim3(1), dim3(kBlock), 0, s, a, (int)sub_mean, (int)dot, h->sc, h->partial);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+// apply_preconditioner: 3D/mgpcg.py:182-200.  z[l] holds each level's final
+// z afterwards (z2[l] is scratch).  Levels above coarse_level run as tiled
+// launches, the rest in one k_coarse launch.  With sub_mean, r[0] is
+// recentred by the V-cycle's own launches (the recentring of solve,
+// :174-180); with dot_parts, the last level-0 launch leaves partials of z.r
+// in `partial` and their count in *dot_parts (0 when no launch did).
+static int vcycle(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
+    const int L = h->n_levels;
+    const int lc = h->coarse_level;
+    const int top = lc >= 0 ? lc : L - 1;  // levels [0, top) run as tiled launches
+    const bool want_dot = dot_parts != nullptr;
+    if (dot_parts) *dot_parts = 0;
+    const bool tiled_mean = sub_mean && top > 0 && restrict_tiled_ok(h, 0);
+    const bool coarse_mean = sub_mean && lc == 0;
+    if (sub_mean && !tiled_mean && !coarse_mean) {
+        hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[0].n, h->r[0], h->sc);
+        PFM_LAUNCH_CHECK();
+    }
+    for (int l = 0; l < top; ++l) {
+        const bool sm = tiled_mean && l == 0;
+        PFM_TRY(rbgs(h, l, kSweepZero, 0, nullptr, h->z2[l], sm, false, s));
+        PFM_TRY(rbgs(h, l, kSweepPlain, 0, h->z2[l], h->z[l], sm, false, s));
+        PFM_TRY(restrict_fused(h, l, sm, s));
+    }
+    if (lc >= 0) {
+        PFM_TRY(coarse(h, coarse_mean, want_dot && lc == 0, s));
+        if (want_dot && lc == 0) *dot_parts = 1;
+    } else {
+        PFM_TRY(bottom(h, L - 1, s));
+    }
+    for (int l = top - 1; l >= 0; --l) {
+        const bool dot = want_dot && l == 0;
+        PFM_TRY(rbgs(h, l, kSweepProlong, 1, h->z[l], h->z2[l], false, false, s));
+        PFM_TRY(rbgs(h, l, kSweepPlain, 1, h->z2[l], h->z[l], false, dot, s));
+        if (dot) *dot_parts = sweep_tiles(h, 0).total;
     }
     return PFM_OK;
 }
@@ -565,25 +1317,27 @@ static int finalize(MGHandle* h, int mode, double eps, hipStream_t s, int nparts
     return PFM_OK;
 }
 
-static dim3 ap_dot_grid(const MGHandle* h) {
-    dim3 gd = cell_grid(h->lv[0].d, h->dim, false);
-    if (h->dim == 3) {
-        const unsigned per_plane = gd.x * gd.y;
-        unsigned z = per_plane >= (unsigned)kMgRedBlocks ? 1u : (unsigned)kMgRedBlocks / per_plane;
-        if (z > gd.z) z = gd.z;
-        gd.z = z;
-    }
-    return gd;
-}
-
-static int ap_dot_parts(const MGHandle* h) {
-    const dim3 gd = ap_dot_grid(h);
-    return (int)(gd.x * gd.y * gd.z);
-}
-
 static int dot(MGHandle* h, const float* a, const float* b, hipStream_t s) {
-    hipLaunchKernelGGL(k_dot, dim3(red_blocks(h->lv[0].n)), dim3(kBlock), 0, s, h->lv[0].n, a, b, h->partial);
+    hipLaunchKernelGGL(k_dot, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[0].n, a, b, h->partial);
     PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+// p <- z + beta p (into the partner buffer), Ap and p.Ap partials; returns
+// the partial count through *parts.
+static int ap_dot_p(MGHandle* h, hipStream_t s, int* parts) {
+    const LvlDev L0 = lvl_dev(h, 0);
+    const TileMap tm = sweep_tiles(h, 0);
+    const dim3 g((unsigned)tm.total);
+    if (h->dim == 3)
+        hipLaunchKernelGGL(k_ap_dot_p<3>, g, dim3(kBlock), 0, s, L0, tm, h->z[0], h->p, h->p2, h->Ap, h->sc,
+                           h->partial);
+    else
+        hipLaunchKernelGGL(k_ap_dot_p<2>, g, dim3(kBlock), 0, s, L0, tm, h->z[0], h->p, h->p2, h->Ap, h->sc,
+                           h->partial);
+    PFM_LAUNCH_CHECK();
+    std::swap(h->p, h->p2);
+    *parts = tm.total;
     return PFM_OK;
 }
 
@@ -593,18 +1347,30 @@ static bool all_neumann(const MGHandle* h) {
     return true;
 }
 
-// solve: 3D/mgpcg.py:202-280.
+// V-cycle followed by z.r into the partials; returns the partial count.
+static int precondition_dot(MGHandle* h, bool sub_mean, hipStream_t s, int* parts) {
+    PFM_TRY(vcycle(h, s, sub_mean, parts));
+    if (*parts == 0) {
+        PFM_TRY(dot(h, h->z[0], h->r[0], s));
+        *parts = kMgRedBlocks;
+    }
+    return PFM_OK;
+}
+
+// solve: 3D/mgpcg.py:202-280.  The reference's update_p after each
+// preconditioner application is carried out at the start of the next Ap
+// launch (same arithmetic, same beta), or explicitly when the loop ends
+// without converging.
 static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mode, int* iters_out,
                  double* rtr_out, hipStream_t s) {
     const int64_t n = h->lv[0].n;
-    const unsigned rb = red_blocks(n);
     const bool neumann = all_neumann(h);
     hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(1), 0, s, h->sc);
     PFM_LAUNCH_CHECK();
     if (neumann) {
         PFM_TRY(dot(h, h->r[0], nullptr, s));
         PFM_TRY(finalize(h, kFinMean, eps, s));
-        hipLaunchKernelGGL(k_sub_mean, dim3(rb), dim3(kBlock), 0, s, n, h->r[0], h->sc);
+        hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->r[0], h->sc);
         PFM_LAUNCH_CHECK();
     }
     double thresh = tol;
@@ -615,21 +1381,17 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
         PFM_HIP_TRY(hipStreamSynchronize(s));
         thresh = tol * tol * h->host_rtr[0];
     }
-    PFM_TRY(vcycle(h, s));
-    hipLaunchKernelGGL(k_update_p, dim3(rb), dim3(kBlock), 0, s, n, h->p, h->z[0], h->sc);
-    PFM_LAUNCH_CHECK();
-    PFM_TRY(dot(h, h->z[0], h->r[0], s));
-    PFM_TRY(finalize(h, kFinZTr, eps, s));
-    LvlDev L0 = lvl_dev(h, 0);
+    int parts = 0;
+    PFM_TRY(precondition_dot(h, false, s, &parts));
+    PFM_TRY(finalize(h, kFinZTr, eps, s, parts));
     int it = 0;
     double rtr = 0.0;
     bool converged = false;
     while (max_iters == -1 || it < max_iters) {
-        hipLaunchKernelGGL(k_ap_dot, ap_dot_grid(h), kCellBlock, 0, s, L0, h->p, h->Ap, h->partial);
-        PFM_LAUNCH_CHECK();
-        PFM_TRY(finalize(h, kFinAlpha, eps, s, ap_dot_parts(h)));
-        hipLaunchKernelGGL(k_update_xr, dim3(rb), dim3(kBlock), 0, s, n, h->x, h->r[0], h->p, h->Ap, h->sc,
-                           h->partial);
+        PFM_TRY(ap_dot_p(h, s, &parts));
+        PFM_TRY(finalize(h, kFinAlpha, eps, s, parts));
+        hipLaunchKernelGGL(k_update_xr, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, h->r[0], h->p, h->Ap,
+                           h->sc, h->partial);
         PFM_LAUNCH_CHECK();
         PFM_TRY(finalize(h, kFinRTr, eps, s));
         PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr, &h->sc->rTr, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -639,16 +1401,13 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
             converged = true;
             break;
         }
-        if (neumann) {
-            hipLaunchKernelGGL(k_sub_mean, dim3(rb), dim3(kBlock), 0, s, n, h->r[0], h->sc);
-            PFM_LAUNCH_CHECK();
-        }
-        PFM_TRY(vcycle(h, s));
-        PFM_TRY(dot(h, h->z[0], h->r[0], s));
-        PFM_TRY(finalize(h, kFinBeta, eps, s));
-        hipLaunchKernelGGL(k_update_p, dim3(rb), dim3(kBlock), 0, s, n, h->p, h->z[0], h->sc);
-        PFM_LAUNCH_CHECK();
+        PFM_TRY(precondition_dot(h, neumann, s, &parts));
+        PFM_TRY(finalize(h, kFinBeta, eps, s, parts));
         ++it;
+    }
+    if (!converged) {
+        hipLaunchKernelGGL(k_update_p, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->p, h->z[0], h->sc);
+        PFM_LAUNCH_CHECK();
     }
     if (iters_out) *iters_out = it;
     if (rtr_out) *rtr_out = rtr;
@@ -691,7 +1450,7 @@ static int apply_bc(MGHandle* h, float* const* u, hipStream_t s) {
 
 static int div_init(MGHandle* h, const float* const* u, float k, hipStream_t s) {
     LvlDev L0 = lvl_dev(h, 0);
-    const dim3 gd = cell_grid(L0.d, h->dim, false);
+    const dim3 gd = cell_grid(L0.d, false);
     if (h->dim == 3)
         hipLaunchKernelGGL(k_div_init<3>, gd, kCellBlock, 0, s, L0, u[0], u[1], u[2], k, h->r[0], h->z[0],
                            h->Ap, h->p, h->x);
@@ -705,13 +1464,10 @@ static int div_init(MGHandle* h, const float* const* u, float k, hipStream_t s) 
 static int subtract_grad_p(MGHandle* h, float* const* u, hipStream_t s) {
     LvlDev L0 = lvl_dev(h, 0);
     for (int a = 0; a < h->dim; ++a) {
+        const int m = map_axis(h, a);
         LvlDev F = L0;
-        F.d[a] += 1;
-        const dim3 gd = cell_grid(F.d, h->dim, false);
-        if (h->dim == 3)
-            hipLaunchKernelGGL(k_subtract_grad_p<3>, gd, kCellBlock, 0, s, L0, F, a, h->x, u[a]);
-        else
-            hipLaunchKernelGGL(k_subtract_grad_p<2>, gd, kCellBlock, 0, s, L0, F, a, h->x, u[a]);
+        F.d[m] += 1;
+        hipLaunchKernelGGL(k_subtract_grad_p, cell_grid(F.d, false), kCellBlock, 0, s, L0, F, m, h->x, u[a]);
         PFM_LAUNCH_CHECK();
     }
     return PFM_OK;
@@ -738,9 +1494,14 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     PFM_CHECK_ARG(precision == 0, "precision %d not supported by this build", precision);
     int mn = N[0];
     for (int a = 0; a < dim; ++a) {
-        PFM_CHECK_ARG(N[a] >= 2, "N[%d] must be >= 2", a);
+        PFM_CHECK_ARG(N[a] >= 2 && N[a] <= 65535, "N[%d]=%d out of range [2, 65535]", a, N[a]);
         if (N[a] < mn) mn = N[a];
     }
+    int64_t cells = 1;
+    for (int a = 0; a < dim; ++a) cells *= N[a];
+    // level arrays are addressed with 32-bit byte offsets
+    PFM_CHECK_ARG(cells < (int64_t)1 << 30, "grid too large for one solver (%lld cells, limit 2^30)",
+                  (long long)cells);
     MGHandle* h = new (std::nothrow) MGHandle();
     if (!h) {
         set_error("host allocation failed");
@@ -763,38 +1524,47 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     }
     for (int l = 0; l < L; ++l) {
         int64_t n = 1;
-        for (int a = 0; a < 3; ++a) {
-            h->lv[l].d[a] = a < dim ? h->N[a] >> l : 1;
-            if (h->lv[l].d[a] < 1) h->lv[l].d[a] = 1;
-            n *= h->lv[l].d[a];
+        for (int m = 0; m < 3; ++m) {
+            const int a = m - (3 - dim);  // problem axis held by index axis m
+            int e = a >= 0 ? h->N[a] >> l : 1;
+            if (e < 1) e = 1;
+            h->lv[l].d[m] = e;
+            n *= e;
         }
         h->lv[l].n = n;
     }
+    h->coarse_level = pick_coarse_level(h);
     auto fail = [&](const char* what) {
         set_error("hipMalloc failed for %s", what);
         pfm_mg_destroy(h);
         return PFM_ERR_OOM;
     };
     for (int l = 0; l < L; ++l) {
-        if (hipMalloc(&h->r[l], sizeof(float) * h->lv[l].n) != hipSuccess) return fail("r");
-        if (hipMalloc(&h->z[l], sizeof(float) * h->lv[l].n) != hipSuccess) return fail("z");
+        const size_t b = sizeof(float) * h->lv[l].n;
+        if (hipMalloc(&h->r[l], b) != hipSuccess) return fail("r");
+        if (hipMalloc(&h->z[l], b) != hipSuccess) return fail("z");
+        if (hipMalloc(&h->z2[l], b) != hipSuccess) return fail("z scratch");
     }
     const size_t b0 = sizeof(float) * h->lv[0].n;
     if (hipMalloc(&h->x, b0) != hipSuccess) return fail("x");
     if (hipMalloc(&h->p, b0) != hipSuccess) return fail("p");
+    if (hipMalloc(&h->p2, b0) != hipSuccess) return fail("p scratch");
     if (hipMalloc(&h->Ap, b0) != hipSuccess) return fail("Ap");
-    const size_t nparts = std::max<size_t>(2 * kMgRedBlocks, (size_t)ap_dot_parts(h));
-    if (hipMalloc(&h->partial, sizeof(double) * nparts) != hipSuccess) return fail("partial");
+    h->n_partial = std::max(2 * kMgRedBlocks, sweep_tiles(h, 0).total);
+    if (hipMalloc(&h->partial, sizeof(double) * h->n_partial) != hipSuccess) return fail("partial");
     if (hipMalloc(&h->sc, sizeof(MGScalars)) != hipSuccess) return fail("scalars");
     if (hipHostMalloc(&h->host_rtr, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess)
         return fail("pinned scalars");
     PFM_HIP_TRY(hipMemset(h->sc, 0, sizeof(MGScalars)));
     for (int l = 0; l < L; ++l) {
-        PFM_HIP_TRY(hipMemset(h->r[l], 0, sizeof(float) * h->lv[l].n));
-        PFM_HIP_TRY(hipMemset(h->z[l], 0, sizeof(float) * h->lv[l].n));
+        const size_t b = sizeof(float) * h->lv[l].n;
+        PFM_HIP_TRY(hipMemset(h->r[l], 0, b));
+        PFM_HIP_TRY(hipMemset(h->z[l], 0, b));
+        PFM_HIP_TRY(hipMemset(h->z2[l], 0, b));
     }
     PFM_HIP_TRY(hipMemset(h->x, 0, b0));
     PFM_HIP_TRY(hipMemset(h->p, 0, b0));
+    PFM_HIP_TRY(hipMemset(h->p2, 0, b0));
     PFM_HIP_TRY(hipMemset(h->Ap, 0, b0));
     PFM_HIP_TRY(hipDeviceSynchronize());
     *handle = h;
@@ -807,9 +1577,11 @@ int pfm_mg_destroy(void* hv) {
     for (int l = 0; l < kMaxLevels; ++l) {
         if (h->r[l]) (void)hipFree(h->r[l]);
         if (h->z[l]) (void)hipFree(h->z[l]);
+        if (h->z2[l]) (void)hipFree(h->z2[l]);
     }
     if (h->x) (void)hipFree(h->x);
     if (h->p) (void)hipFree(h->p);
+    if (h->p2) (void)hipFree(h->p2);
     if (h->Ap) (void)hipFree(h->Ap);
     if (h->partial) (void)hipFree(h->partial);
     if (h->sc) (void)hipFree(h->sc);
@@ -894,7 +1666,7 @@ int pfm_mg_poisson(void* hv, float* u_x, float* u_y, float* u_z, int max_iters, 
 int pfm_mg_vcycle(void* hv, pfm_stream_t stream) {
     MGHandle* h = (MGHandle*)hv;
     PFM_CHECK_ARG(h, "null solver handle");
-    return vcycle(h, as_stream(stream));
+    return vcycle(h, as_stream(stream), false, nullptr);
 }
 
 int pfm_mg_smooth(void* hv, int level, int phase, pfm_stream_t stream) {
@@ -919,7 +1691,7 @@ int pfm_mg_apply_A(void* hv, const float* p, float* Ap, pfm_stream_t stream) {
     MGHandle* h = (MGHandle*)hv;
     PFM_CHECK_ARG(h && p && Ap, "null argument");
     LvlDev L0 = lvl_dev(h, 0);
-    hipLaunchKernelGGL(k_apply_A, cell_grid(L0.d, h->dim, false), kCellBlock, 0, as_stream(stream), L0, p, Ap);
+    hipLaunchKernelGGL(k_apply_A, cell_grid(L0.d, false), kCellBlock, 0, as_stream(stream), L0, p, Ap);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

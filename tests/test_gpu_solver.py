@@ -19,7 +19,8 @@ def N(t):
     return t.detach().cpu().numpy()
 
 
-@pytest.mark.parametrize("N_", [(32, 32), (16, 16, 16), (32, 16, 16), (64, 64)])
+@pytest.mark.parametrize("N_", [(32, 32), (16, 16, 16), (32, 16, 16), (64, 64), (50, 34, 34), (100, 68),
+                                (40, 72, 24)])
 def test_mg_passes_bit_exact(oracle, pfm, N_):
     d = len(N_)
     mg_o = oracle.MG(N_, base_level=3)
