@@ -72,46 +72,48 @@ class EventTimer:
 
 
 def instrumented_step(sim, timer):
-    """sim.step() with event regions around each phase (same work, same order)."""
+    """One 3D step with event regions around each phase: the launches and
+    their order are those of Simulation.step (the fused RK4 + P2G record path)."""
     from pfm import core
+    from pfm.simulation import reinit_decision
 
     sim.i += 1
     i = sim.i
     cfg = sim.cfg
-    with timer.region("max_speed"):
+    with timer.region("max_speed_dt"):
         dt = sim.compute_dt()
     sim.last_dt = dt
-    if i % cfg.reinit_every == 0:
+    long_r, reseed, short, sim.ik = reinit_decision(i, sim.ik, cfg, 3)
+    if long_r:
         with timer.region("reinit_long"):
-            if cfg.reinit_particle_pos or i == 0:
+            if reseed:
                 sim._seed()
-                sim.ik = i
             core.g2p(sim.res, sim.dx, sim.pos, sim.u, sim.init_imp, None)
             core.reset_T_identity(sim.T_init)
             core.reset_T_identity(sim.T_grad_m)
-    if (i - sim.ik) % cfg.reinit_every_grad_m == 0:
-        with timer.region("g2p_grad_m"):
+    if short:
+        with timer.region("reinit_short_g2p"):
             core.g2p(sim.res, sim.dx, sim.pos, sim.u, sim.imp_grad_m, sim.init_C)
-        with timer.region("update_T"):
+        with timer.region("reinit_short_T"):
             core.update_T(sim.T_grad_m, sim.T_init, sim.T_init)
             core.reset_T_identity(sim.T_grad_m)
-    with timer.region("midpoint_advect"):
-        for a in range(sim.dim):
-            core.reset_identity_grid(sim.res, sim.dx, sim.psi_grid[a], sim.T_grid[a])
-        for a in range(sim.dim):
-            core.rk4_grid(sim.res, sim.dx, sim.psi_grid[a], sim.T_grid[a], sim.u, 0.5 * dt)
-        for a in range(sim.dim):
-            sim.p2g_weight[a].copy_(sim.u[a])
-        core.advect_u(sim.res, sim.dx, sim.p2g_weight, sim.u, sim.T_grid, sim.psi_grid)
-    with timer.region("poisson"):
-        sim.project()
-    with timer.region("rk4_particles"):
-        core.rk4_particles(sim.res, sim.dx, sim.pos, sim.T_grad_m, sim.u, dt, True)
+    if cfg.use_midpoint_vel:
+        with timer.region("midpoint_advect"):
+            for a in range(sim.dim):
+                core.reset_identity_grid(sim.res, sim.dx, sim.psi_grid[a], sim.T_grid[a])
+            for a in range(sim.dim):
+                core.rk4_grid(sim.res, sim.dx, sim.psi_grid[a], sim.T_grid[a], sim.u, 0.5 * dt)
+            for a in range(sim.dim):
+                sim.p2g_weight[a].copy_(sim.u[a])
+            core.advect_u(sim.res, sim.dx, sim.p2g_weight, sim.u, sim.T_grid, sim.psi_grid)
+        with timer.region("poisson"):
+            sim.project()
+    with timer.region("rk4_record"):
+        sim.binning.rk4_record(sim.dx, sim.pos, sim.T_grad_m, sim.u, dt, sim.T_init, sim.init_C, sim.init_imp)
     with timer.region("binning"):
         sim.binning.bin(sim.pos, sim.dx, force=True)
     with timer.region("p2g"):
-        sim.binning.p2g(sim.dx, sim.pos, sim.init_imp, sim.init_C, sim.u, W=sim.p2g_weight,
-                        Tgm=sim.T_grad_m, Tin=sim.T_init)
+        sim.binning.p2g_from_records(sim.dx, sim.n, sim.u, W=sim.p2g_weight)
     with timer.region("poisson"):
         sim.project()
     return dt
@@ -149,10 +151,9 @@ def cpu_baseline(seconds_budget=15.0):
 
 
 def traffic_probe(sim, torch):
-    """Kernels for HBM-traffic counters: a 1 GiB device copy (known bytes:
-    1 GiB read + 1 GiB written) and one dispatch of each step kernel."""
-    from pfm import core
-
+    """Kernels for HBM-traffic counters: two steps, a 1 GiB device copy
+    (known bytes: 1 GiB read + 1 GiB written), then one more full step, so
+    every step kernel is dispatched after the reference copy."""
     for _ in range(2):
         sim.step()
     src = torch.empty(2 ** 28, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -160,23 +161,83 @@ def traffic_probe(sim, torch):
     torch.cuda.synchronize()
     torch.neg(src, out=dst)  # the reference stream: reads 1 GiB, writes 1 GiB
     torch.cuda.synchronize()
-    dt = 0.5 * sim.last_dt
-    core.rk4_particles(sim.res, sim.dx, sim.pos, sim.T_grad_m, sim.u, dt, True)
-    sim.binning.bin(sim.pos, sim.dx, force=True)
-    sim.binning.p2g(sim.dx, sim.pos, sim.init_imp, sim.init_C, sim.u, W=sim.p2g_weight, Tgm=sim.T_grad_m,
-                    Tin=sim.T_init)
-    core.g2p(sim.res, sim.dx, sim.pos, sim.u, None, sim.init_C)
+    del src, dst
+    sim.step()
     torch.cuda.synchronize()
-    print(f"# traffic probe done: copy {src.numel() * 4} B, {sim.n} particles", file=sys.stderr)
+    print(f"# traffic probe done: copy {2 ** 30} B, {sim.n} particles", file=sys.stderr)
 
 
-def summarize(db_path, top=25):
-    """Text summary of a rocprofv3 results database (kernel trace or PMC)."""
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_step_256.md")
+
+
+def pmc_traffic(fetch_db, write_db):
+    """Markdown table of HBM bytes per dispatch per kernel from a FETCH_SIZE
+    run and a WRITE_SIZE run of `bench.py --traffic-probe`; each counter is
+    scaled by what it reads for the 1 GiB copy (torch.neg) of the same run."""
+    import sqlite3
+
+    def per_kernel(path, counter):
+        db = sqlite3.connect(path)
+        rows = db.execute("select name, avg(counter_value), count(*) from pmc_events where counter_name = ? "
+                          "group by name", (counter,)).fetchall()
+        return {name: (avg * 1024.0, cnt) for name, avg, cnt in rows}
+
+    f = per_kernel(fetch_db, "FETCH_SIZE")
+    w = per_kernel(write_db, "WRITE_SIZE")
+    copy = max((k for k in f if "neg_kernel" in k), key=lambda k: f[k][0])
+    fs, ws = 2 ** 30 / f[copy][0], 2 ** 30 / w[copy][0]
+    lines = [f"copy reference: FETCH_SIZE reads {f[copy][0] / 2**20:.1f} MiB and WRITE_SIZE "
+             f"{w[copy][0] / 2**20:.1f} MiB for 1024 MiB each way; scale factors fetch x{fs:.3f}, "
+             f"write x{ws:.3f}", "",
+             "| kernel | dispatches | fetch B/dispatch | write B/dispatch | traffic B/dispatch |",
+             "|---|---|---|---|---|"]
+    rows = []
+    for k in f:
+        if k not in w:
+            continue
+        fb, wb = f[k][0] * fs, w[k][0] * ws
+        rows.append((fb + wb, k, f[k][1], fb, wb))
+    for tot, k, cnt, fb, wb in sorted(rows, reverse=True)[:40]:
+        name = k.split("(")[0].replace("void ", "")
+        lines.append(f"| {name} | {cnt} | {fb:.4g} | {wb:.4g} | {tot:.4g} |")
+    return "\n".join(lines)
+
+
+def committed_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed PMC summary."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 5 and cells[0] == kernel:
+                    return float(cells[4])
+    except OSError:
+        pass
+    return None
+
+
+def summarize(db_path, top=25, by_grid=False, match=None):
+    """Text summary of a rocprofv3 results database (kernel trace or PMC).
+    With by_grid, kernel-trace rows are split by launch size (workgroups),
+    which separates the multigrid levels; `match` keeps names containing it."""
     import sqlite3
 
     db = sqlite3.connect(db_path)
     cur = db.cursor()
     lines = []
+    if by_grid:
+        rows = cur.execute("select name, grid_x*grid_y*grid_z/(workgroup_x*workgroup_y*workgroup_z) as g, "
+                           "count(*), sum(duration), avg(duration) from kernels group by name, g "
+                           "order by sum(duration) desc").fetchall()
+        lines.append(f"{'calls':>7} {'total ms':>10} {'avg us':>10} {'blocks':>8}  kernel")
+        for name, g, cnt, tot, avg in rows:
+            if match and match not in name:
+                continue
+            short = name.split("(")[0].replace("void ", "")
+            lines.append(f"{cnt:>7} {tot / 1e6:>10.3f} {avg / 1e3:>10.2f} {g:>8}  {short[:90]}")
+            if len(lines) > top:
+                break
+        return "\n".join(lines)
     rows = cur.execute("select name, count(*), sum(duration), avg(duration) from kernels "
                        "group by name order by sum(duration) desc").fetchall()
     total = sum(r[2] for r in rows) or 1
@@ -206,9 +267,16 @@ def main():
                     help="for rocprofv3 --pmc runs: one plain 1 GiB copy (the counter scale reference) "
                          "then one launch of each step kernel after two steps; prints no JSON")
     ap.add_argument("--summarize", metavar="DB", help="print a text summary of a rocprofv3 results db and exit")
+    ap.add_argument("--by-grid", action="store_true", help="with --summarize: split rows by launch size")
+    ap.add_argument("--match", default=None, help="with --summarize --by-grid: keep kernels containing this")
+    ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="print the per-kernel HBM traffic table of two --traffic-probe PMC runs and exit")
     args = ap.parse_args()
     if args.summarize:
-        print(summarize(args.summarize))
+        print(summarize(args.summarize, top=40 if args.by_grid else 25, by_grid=args.by_grid, match=args.match))
+        return
+    if args.pmc_traffic:
+        print(pmc_traffic(*args.pmc_traffic))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the step shards by slab "
@@ -280,18 +348,23 @@ def main():
     cells = math.prod(res)
     del u_f, work
 
-    # roofline of the particle RK4 kernel: 96 B/particle compulsory traffic
+    # roofline of the step's dominant kernel, RK4 + P2G record
+    # (k_rk4_particles<3, true>): per particle it reads pos 12 B, T_grad_m
+    # 36 B, T_init 36 B, init_C 36 B, init_imp 12 B and writes pos 12 B,
+    # T_grad_m 36 B and a 64 B record: 244 B/particle
     ev_a = torch.cuda.Event(enable_timing=True)
     ev_b = torch.cuda.Event(enable_timing=True)
     nrep = 3
     ev_a.record(torch.cuda.current_stream())
     for _ in range(nrep):
-        core.rk4_particles(sim.res, sim.dx, sim.pos, sim.T_grad_m, sim.u, 1e-4, True)
+        sim.binning.rk4_record(sim.dx, sim.pos, sim.T_grad_m, sim.u, 1e-4, sim.T_init, sim.init_C, sim.init_imp)
     ev_b.record(torch.cuda.current_stream())
     torch.cuda.synchronize()
     rk4_ms = ev_a.elapsed_time(ev_b) / nrep
-    alg_bytes = 96.0 * sim.n
+    alg_bytes = 244.0 * sim.n
     achieved = alg_bytes / (rk4_ms * 1e-3) / 1e9
+    roof_kernel = "pfm::k_rk4_particles<3, true>"
+    traffic = committed_traffic(roof_kernel) if args.res == 256 else None
 
     out = {
         "metric": "particle-steps/s",
@@ -313,8 +386,8 @@ def main():
         "mgpcg_ms_per_solve": t_poisson * 1e3,
         "mgpcg_iters": pits,
         "phase_ms": phases,
-        "roofline": {"kernel": "k_rk4_particles<3>", "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"kernel": roof_kernel, "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms": rk4_ms, "alg_bytes_per_launch": alg_bytes},
     }
     if not args.no_cpu_baseline:

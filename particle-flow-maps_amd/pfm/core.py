@@ -244,11 +244,10 @@ class Binning:
                self.workspace.data_ptr(), self.workspace.numel(), _stream(stream))
 
 
-    def rk4_then_p2g(self, dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, u_out, W=None, stream=None):
-        """RK4_T_forward followed by P2G (3D/run_3D.py:795-812) with the P2G
-        composition fused into the RK4 kernel: advect pos/T_grad_m in place,
-        bin the new positions, then scatter into u_out (may alias u_vel only
-        after the RK4 reads are done -- the launches are ordered on one stream)."""
+    def rk4_record(self, dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, stream=None):
+        """RK4_T_forward (3D/run_3D.py:795-796) with the P2G particle
+        composition (3D/run_3D.py:460-479) fused in: advects pos/T_grad_m in
+        place and writes one P2G record per particle into the workspace."""
         d, n = pos.shape
         g = L.make_grid(self.res, dx)
         lib = L.load()
@@ -259,11 +258,25 @@ class Binning:
                *_cols(Tgm, d, n, "T_grad_m"), *_u_ptrs(u_vel, self.res), float(dt), *_cols(Tin, d, n, "T_init"),
                *_cols(init_C, d, n, "init_C"), _pvec(init_imp, d, n, "init_imp"), rec, _stream(stream))
         _mark_written(pos)
-        self.bin(pos, dx, stream, force=True)
+
+    def p2g_from_records(self, dx, n, u_out, W=None, stream=None):
+        """P2G (3D/run_3D.py:812, 3D/taichi_utils.py P2G) from the records of
+        rk4_record, using the current binning of the advected positions."""
+        d = len(self.res)
+        g = L.make_grid(self.res, dx)
         Wp = [L.ptr(W[a], name=f"W[{a}]") if (W is not None and a < d) else None for a in range(3)]
         L.call("pfm_p2g_records", ctypes.byref(g), n, L.ptr(self.cell_start, dtype=torch.int32),
                L.ptr(self.perm, dtype=torch.int32), *_u_ptrs(u_out, self.res), *Wp, self.workspace.data_ptr(),
                self.workspace.numel(), _stream(stream))
+
+    def rk4_then_p2g(self, dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, u_out, W=None, stream=None):
+        """RK4_T_forward followed by P2G (3D/run_3D.py:795-812) with the P2G
+        composition fused into the RK4 kernel: advect pos/T_grad_m in place,
+        bin the new positions, then scatter into u_out (may alias u_vel only
+        after the RK4 reads are done -- the launches are ordered on one stream)."""
+        self.rk4_record(dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, stream)
+        self.bin(pos, dx, stream, force=True)
+        self.p2g_from_records(dx, pos.shape[1], u_out, W, stream)
 
 
 _BINNING_CACHE = {}
