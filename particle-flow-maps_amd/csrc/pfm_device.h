@@ -1,6 +1,6 @@
 // pfm_device.h -- device-side building blocks shared by the libpfm kernels
 // (gfx950, wave64).  Quadratic B-spline weights in closed form, the separable
-// 4^d-point MAC interpolation, and block reductions.
+// MAC interpolation over the 3^d live nodes, and block reductions.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,45 +27,6 @@ __device__ __forceinline__ int64_t face_count(const GridDev& g, int a) {
     int64_t n = 1;
     for (int c = 0; c < g.dim; ++c) n *= (int64_t)(g.res[c] + (c == a ? 1 : 0));
     return n;
-}
-
-// Quadratic B-spline values and derivatives at the four stencil distances
-// f+1, f, f-1, f-2 (f in [0,1)), i.e. N_2/dN_2 of 3D/taichi_utils.py:252-272
-// evaluated at x_p - x_i for i = iu-1 .. iu+2.  Every intermediate is exact
-// in fp32 (all operands are multiples of 2^-23 below 4), so these closed forms
-// give the same bits as evaluating the reference's branchy N_2/dN_2 on the
-// four distances.
-__device__ __forceinline__ void bspline4(float f, float w[4], float dw[4]) {
-    const bool lo = f < 0.5f;
-    const bool hi = f > 0.5f;
-    const float a = 0.5f - f;
-    const float b = 1.5f - f;
-    const float c = 0.5f + f;
-    const float d = f - 0.5f;
-    const float e = 1.0f - f;
-    w[0] = lo ? 0.5f * (a * a) : 0.0f;
-    w[1] = lo ? 0.75f - f * f : 0.5f * (b * b);
-    w[2] = hi ? 0.75f - e * e : 0.5f * (c * c);
-    w[3] = hi ? 0.5f * (d * d) : 0.0f;
-    const float t = 2.0f * f;
-    dw[0] = lo ? 0.5f * (t - 1.0f) : 0.0f;
-    dw[1] = lo ? -t : 0.5f * (t - 3.0f);
-    dw[2] = hi ? 2.0f * e : 0.5f * (1.0f + t);
-    dw[3] = hi ? 0.5f * (t - 1.0f) : 0.0f;
-}
-
-__device__ __forceinline__ void bspline4_val(float f, float w[4]) {
-    const bool lo = f < 0.5f;
-    const bool hi = f > 0.5f;
-    const float a = 0.5f - f;
-    const float b = 1.5f - f;
-    const float c = 0.5f + f;
-    const float d = f - 0.5f;
-    const float e = 1.0f - f;
-    w[0] = lo ? 0.5f * (a * a) : 0.0f;
-    w[1] = lo ? 0.75f - f * f : 0.5f * (b * b);
-    w[2] = hi ? 0.75f - e * e : 0.5f * (c * c);
-    w[3] = hi ? 0.5f * (d * d) : 0.0f;
 }
 
 // N_2 on an arbitrary argument (3D/taichi_utils.py:252-260), used by P2G.
@@ -96,15 +57,18 @@ __device__ __forceinline__ void axis_stencil(float p, float inv_dx, float BL, fl
     const float f = s - fl;
     const bool up = f >= 0.5f;
     a.j0 = (int)fl - 1 + (up ? 1 : 0);
-    float w4[4], dw4[4];
-    if (GRAD)
-        bspline4(f, w4, dw4);
-    else
-        bspline4_val(f, w4);
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        a.w[m] = up ? w4[m + 1] : w4[m];
-        if (GRAD) a.dw[m] = up ? dw4[m + 1] : dw4[m];
+    // d = s - (j0 + 1) in [-1/2, 1/2): the distances to the live nodes are
+    // d + 1, d, d - 1.  With f in [0, 1) every operation below is exact, so
+    // the weights are bit-equal to N_2/dN_2 at those distances.
+    const float d = up ? f - 1.0f : f;
+    const float a0 = 0.5f - d, a2 = 0.5f + d;
+    a.w[0] = 0.5f * (a0 * a0);
+    a.w[1] = 0.75f - d * d;
+    a.w[2] = 0.5f * (a2 * a2);
+    if (GRAD) {
+        a.dw[0] = d - 0.5f;
+        a.dw[1] = -2.0f * d;
+        a.dw[2] = d + 0.5f;
     }
 }
 
@@ -127,29 +91,25 @@ struct MacStencil {
 // Raw buffer resource over a face field (word 3 = 0x00020000 on gfx9-family
 // parts: raw 32-bit dwords).  Loads through it take a 32-bit byte offset, so a
 // row address costs one VALU add; offsets past num_records read zero.
-typedef float v4f __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t face_rsrc(const float* fld, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(fld), (short)0, (int)bytes, 0x00020000);
 }
 
-// Weights of the 16-byte row load along the contiguous axis.  The row is
-// loaded from ls = min(j0, S-4) (so all 4 floats are inside the row); the 3
-// live nodes j0 .. j0+2 sit at lanes o = j0-ls .. o+2, and a node past the
-// end (j0+2 = S, clamped to S-1 by sample(), 3D/taichi_utils.py:48-54) folds
-// its weight onto the last lane.
-__device__ __forceinline__ void row_weights(int o, const float w[3], float q[4]) {
-    q[0] = o == 0 ? w[0] : 0.f;
-    q[1] = o == 0 ? w[1] : (o == 1 ? w[0] : 0.f);
-    q[2] = o == 0 ? w[2] : (o == 1 ? w[1] : w[0]);
-    q[3] = o == 0 ? 0.f : (o == 1 ? w[2] : w[1] + w[2]);
-}
+typedef float v3f __attribute__((ext_vector_type(3)));
 
-// Interpolate MAC component A from its stencils: value and (if GRAD) the
-// gradient, the separable form of interp_grad_2 (3D/taichi_utils.py:274-307;
-// 2D :214-248) over the 3^d live nodes.
+// Rows of MAC component A under its stencil: the 3^(d-1) rows of 3 nodes
+// along the contiguous axis, one 12-byte load each.  The clamp
+// s <= extent - 2 - eps keeps the live nodes j0 .. j0+2 inside
+// [0, extent-1] (the reference's sample() clamps only the fourth, zero-weight
+// node of interp_grad_2), so no row needs folding.
+template <int DIM>
+struct Rows {
+    v3f q[DIM == 3 ? 9 : 3];
+};
+
 template <int DIM, int A, bool GRAD>
-__device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const GridDev& g,
-                                            const MacStencil<DIM, GRAD>& st, float& val, float grad[3]) {
+__device__ __forceinline__ void load_rows(const float* __restrict__ fld, const GridDev& g,
+                                          const MacStencil<DIM, GRAD>& st, Rows<DIM>& r) {
     const AxisStencil& s0 = A == 0 ? st.F[0] : st.C[0];
     const AxisStencil& s1 = A == 1 ? st.F[1] : st.C[1];
     const int S0 = g.res[0] + (A == 0 ? 1 : 0);
@@ -157,27 +117,44 @@ __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const
     if (DIM == 3) {
         const AxisStencil& s2 = A == 2 ? st.F[2] : st.C[2];
         const int S2 = g.res[2] + (A == 2 ? 1 : 0);
-        const int ls = min(s2.j0, S2 - 4);
-        float wz[4], dwz[4];
-        row_weights(s2.j0 - ls, s2.w, wz);
-        if (GRAD) row_weights(s2.j0 - ls, s2.dw, dwz);
-        const uint32_t plane = (uint32_t)S1 * (uint32_t)S2 * 4u;
+        const uint32_t rowb = (uint32_t)S2 * 4u;
+        const uint32_t plane = (uint32_t)S1 * rowb;
         const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, plane * (uint32_t)S0);
-        uint32_t rowoff[3];
+        const uint32_t base = (uint32_t)s0.j0 * plane + (uint32_t)s1.j0 * rowb + (uint32_t)s2.j0 * 4u;
 #pragma unroll
-        for (int b = 0; b < 3; ++b) rowoff[b] = ((uint32_t)min(s1.j0 + b, S1 - 1) * (uint32_t)S2 + (uint32_t)ls) * 4u;
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                r.q[3 * a + b] = __builtin_amdgcn_raw_buffer_load_b96(rs, base + a * plane + b * rowb, 0, 0);
+    } else {
+        const uint32_t rowb = (uint32_t)S1 * 4u;
+        const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, rowb * (uint32_t)S0);
+        const uint32_t base = (uint32_t)s0.j0 * rowb + (uint32_t)s1.j0 * 4u;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r.q[a] = __builtin_amdgcn_raw_buffer_load_b96(rs, base + a * rowb, 0, 0);
+    }
+}
+
+// Value and (if GRAD) gradient of component A from its rows: the separable
+// form of interp_grad_2 (3D/taichi_utils.py:274-307; 2D :214-248).
+template <int DIM, int A, bool GRAD>
+__device__ __forceinline__ void combine_rows(const GridDev& g, const MacStencil<DIM, GRAD>& st, const Rows<DIM>& r,
+                                             float& val, float grad[3]) {
+    const AxisStencil& s0 = A == 0 ? st.F[0] : st.C[0];
+    const AxisStencil& s1 = A == 1 ? st.F[1] : st.C[1];
+    if (DIM == 3) {
+        const AxisStencil& s2 = A == 2 ? st.F[2] : st.C[2];
         float v0 = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const uint32_t xoff = (uint32_t)min(s0.j0 + a, S0 - 1) * plane;
             float c_v = 0.f, c_dy = 0.f, c_dz = 0.f;
 #pragma unroll
             for (int b = 0; b < 3; ++b) {
-                const v4f q = __builtin_amdgcn_raw_buffer_load_b128(rs, xoff + rowoff[b], 0, 0);
-                const float rz = wz[0] * q.x + wz[1] * q.y + wz[2] * q.z + wz[3] * q.w;
+                const v3f q = r.q[3 * a + b];
+                const float rz = s2.w[0] * q.x + s2.w[1] * q.y + s2.w[2] * q.z;
                 c_v += s1.w[b] * rz;
                 if (GRAD) {
-                    const float rdz = dwz[0] * q.x + dwz[1] * q.y + dwz[2] * q.z + dwz[3] * q.w;
+                    const float rdz = s2.dw[0] * q.x + s2.dw[1] * q.y + s2.dw[2] * q.z;
                     c_dy += s1.dw[b] * rz;
                     c_dz += s1.w[b] * rdz;
                 }
@@ -196,21 +173,14 @@ __device__ __forceinline__ void interp_comp(const float* __restrict__ fld, const
             grad[2] = gz * g.inv_dx;
         }
     } else {
-        const int ls = min(s1.j0, S1 - 4);
-        float wy[4], dwy[4];
-        row_weights(s1.j0 - ls, s1.w, wy);
-        if (GRAD) row_weights(s1.j0 - ls, s1.dw, dwy);
-        const uint32_t rowb = (uint32_t)S1 * 4u;
-        const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, rowb * (uint32_t)S0);
         float v0 = 0.f, gx = 0.f, gy = 0.f;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const uint32_t off = (uint32_t)min(s0.j0 + a, S0 - 1) * rowb + (uint32_t)ls * 4u;
-            const v4f q = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-            const float ry = wy[0] * q.x + wy[1] * q.y + wy[2] * q.z + wy[3] * q.w;
+            const v3f q = r.q[a];
+            const float ry = s1.w[0] * q.x + s1.w[1] * q.y + s1.w[2] * q.z;
             v0 += s0.w[a] * ry;
             if (GRAD) {
-                const float rdy = dwy[0] * q.x + dwy[1] * q.y + dwy[2] * q.z + dwy[3] * q.w;
+                const float rdy = s1.dw[0] * q.x + s1.dw[1] * q.y + s1.dw[2] * q.z;
                 gx += s0.dw[a] * ry;
                 gy += s0.w[a] * rdy;
             }
@@ -232,10 +202,15 @@ __device__ __forceinline__ void interp_mac(const float* __restrict__ ux,
                                            const float p[3], float u[3], float G[3][3]) {
     MacStencil<DIM, GRAD> st;
     st.build(g, p);
-    interp_comp<DIM, 0, GRAD>(ux, g, st, u[0], G[0]);
-    interp_comp<DIM, 1, GRAD>(uy, g, st, u[1], G[1]);
+    Rows<DIM> r0, r1;
+    load_rows<DIM, 0, GRAD>(ux, g, st, r0);
+    load_rows<DIM, 1, GRAD>(uy, g, st, r1);
+    combine_rows<DIM, 0, GRAD>(g, st, r0, u[0], G[0]);
+    combine_rows<DIM, 1, GRAD>(g, st, r1, u[1], G[1]);
     if (DIM == 3) {
-        interp_comp<DIM, 2, GRAD>(uz, g, st, u[2], G[2]);
+        Rows<DIM> r2;
+        load_rows<DIM, DIM == 3 ? 2 : 1, GRAD>(uz, g, st, r2);
+        combine_rows<DIM, DIM == 3 ? 2 : 1, GRAD>(g, st, r2, u[2], G[2]);
     } else {
         u[2] = 0.f;
         G[2][0] = G[2][1] = G[2][2] = 0.f;
