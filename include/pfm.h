@@ -183,6 +183,11 @@ int pfm_split_central(const pfm_grid* g, const float* vc, float* u_x, float* u_y
 int pfm_get_central(const pfm_grid* g, const float* u_x, const float* u_y, const float* u_z,
                     float* vc, pfm_stream_t stream);
 
+/* curl: 3D/taichi_utils.py:56-78 (vc (3, cells) -> out (3, cells)) and
+ * 2D/taichi_utils.py:97-103 (vc (2, cells) -> out (cells)); central
+ * differences with sample() clamping at the walls. */
+int pfm_curl(const pfm_grid* g, const float* vc, float* out, pfm_stream_t stream);
+
 /* leapfrog_vel_func generalised to nv point vortices (2D/init_conditions.py:
  * 5-9,29-60): vc (2, cells) = sum_q w_q * angular_vel_func(r, rad, strength)
  * * (-y, x).  cx, cy, w are host arrays of nv floats (nv <= 4096). */

@@ -73,6 +73,7 @@ def lib():
         L.oracle_max_speed.argtypes = [ci, I32P, vp, vp, vp]
         L.oracle_split_central.argtypes = [ci, I32P, vp, vp, vp, vp]
         L.oracle_get_central.argtypes = [ci, I32P, vp, vp, vp, vp]
+        L.oracle_curl.argtypes = [ci, I32P, fl, vp, vp]
         L.oracle_point_vortices.argtypes = [I32P, fl, ci, F32P, F32P, F32P, fl, fl, vp]
         L.oracle_vortex_ring.argtypes = [I32P, fl, fl, fl, fl, F32P, F32P, F32P, ci, vp]
         L.oracle_kinetic_energy.restype = ctypes.c_double
@@ -254,6 +255,14 @@ def get_central(res, u):
     vc = np.empty((d,) + tuple(res), np.float32)
     lib().oracle_get_central(d, _res(res), *_u3(u), _p(vc))
     return vc
+
+
+def curl(res, dx, vc):
+    """curl of the cell-centred vector vc (d, *res): (3, *res) in 3D, res in 2D."""
+    d = len(res)
+    out = np.empty((3,) + tuple(res) if d == 3 else tuple(res), np.float32)
+    lib().oracle_curl(d, _res(res), dx, _p(np.ascontiguousarray(vc, np.float32)), _p(out))
+    return out
 
 
 def point_vortices(res, dx, centers, weights, rad=0.02, strength=-0.01):

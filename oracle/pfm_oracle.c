@@ -646,6 +646,44 @@ void oracle_get_central(int dim, const int* res, const float* ux, const float* u
     }
 }
 
+/* ---- curl: 3D/taichi_utils.py:56-78 (sample() clamps the neighbour indices,
+ * 3D/taichi_utils.py:48-54), 2D/taichi_utils.py:97-103.  vc is SoA (d x C);
+ * out is 3 x C (3D) or C (2D). */
+static float vc_at(const grid_t* g, const float* vc, int c, int i, int j, int k) {
+    int64_t C = (int64_t)g->res[0] * g->res[1] * g->res[2];
+    i = i < 0 ? 0 : (i > g->res[0] - 1 ? g->res[0] - 1 : i);
+    j = j < 0 ? 0 : (j > g->res[1] - 1 ? g->res[1] - 1 : j);
+    k = k < 0 ? 0 : (k > g->res[2] - 1 ? g->res[2] - 1 : k);
+    return vc[c * C + idx3(g->res, i, j, k)];
+}
+
+void oracle_curl(int dim, const int* res, float dx, const float* vc, float* out) {
+    grid_t g;
+    make_grid(&g, dim, res, dx);
+    int64_t C = (int64_t)g.res[0] * g.res[1] * g.res[2];
+    const float inv_dist = 1.0f / (2.0f * dx); /* 1./(2*dx) in the kernel's f32 */
+    for (int i = 0; i < g.res[0]; ++i)
+        for (int j = 0; j < g.res[1]; ++j)
+            for (int k = 0; k < g.res[2]; ++k) {
+                int64_t q = idx3(g.res, i, j, k);
+                if (dim == 3) {
+                    float d_vx_dz = inv_dist * (vc_at(&g, vc, 0, i, j, k + 1) - vc_at(&g, vc, 0, i, j, k - 1));
+                    float d_vx_dy = inv_dist * (vc_at(&g, vc, 0, i, j + 1, k) - vc_at(&g, vc, 0, i, j - 1, k));
+                    float d_vy_dx = inv_dist * (vc_at(&g, vc, 1, i + 1, j, k) - vc_at(&g, vc, 1, i - 1, j, k));
+                    float d_vy_dz = inv_dist * (vc_at(&g, vc, 1, i, j, k + 1) - vc_at(&g, vc, 1, i, j, k - 1));
+                    float d_vz_dx = inv_dist * (vc_at(&g, vc, 2, i + 1, j, k) - vc_at(&g, vc, 2, i - 1, j, k));
+                    float d_vz_dy = inv_dist * (vc_at(&g, vc, 2, i, j + 1, k) - vc_at(&g, vc, 2, i, j - 1, k));
+                    out[q] = d_vz_dy - d_vy_dz;
+                    out[C + q] = d_vx_dz - d_vz_dx;
+                    out[2 * C + q] = d_vy_dx - d_vx_dy;
+                } else {
+                    float vr_y = vc_at(&g, vc, 1, i + 1, j, 0), vl_y = vc_at(&g, vc, 1, i - 1, j, 0);
+                    float vt_x = vc_at(&g, vc, 0, i, j + 1, 0), vb_x = vc_at(&g, vc, 0, i, j - 1, 0);
+                    out[q] = (vr_y - vl_y - vt_x + vb_x) / (2.0f * dx);
+                }
+            }
+}
+
 /* ---- 31. point vortices: angular_vel_func + leapfrog_vel_func,
  * 2D/init_conditions.py:5-9,29-60, generalised to a list of vortices.
  * Writes the cell-centred velocity vc (SoA 2 x C) at centres (i+0.5)dx. */

@@ -211,6 +211,45 @@ __global__ void __launch_bounds__(kBlock) k_get_central(GridDev g, const float* 
         vc[2 * C + q] = 0.5f * __fadd_rn(uz[((int64_t)i * ry + j) * (rz + 1) + k + 1], uz[((int64_t)i * ry + j) * (rz + 1) + k]);
 }
 
+// curl: 3D/taichi_utils.py:56-78 -- central differences of the cell-centred
+// vector vc (component-major, (3, cells)) with sample() clamping and
+// inv = float(1 / (2 dx)); 2D/taichi_utils.py:97-103 -- the scalar
+// (vr.y - vl.y - vt.x + vb.x) / (2 dx).  No contraction: every product and
+// difference rounds as in the reference.
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_curl(GridDev g, const float* __restrict__ vc, float inv, float two_dx,
+                                                 float* __restrict__ cf) {
+#pragma clang fp contract(off)
+    const int64_t C = (int64_t)g.res[0] * g.res[1] * g.res[2];
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= C) return;
+    const int ry = g.res[1], rz = g.res[2];
+    const int k = (int)(q % rz);
+    const int j = (int)((q / rz) % ry);
+    const int i = (int)(q / ((int64_t)rz * ry));
+    auto at = [&](int c, int ii, int jj, int kk) {
+        ii = min(max(ii, 0), g.res[0] - 1);
+        jj = min(max(jj, 0), g.res[1] - 1);
+        kk = min(max(kk, 0), g.res[2] - 1);
+        return vc[c * C + ((int64_t)ii * ry + jj) * rz + kk];
+    };
+    if (DIM == 3) {
+        const float d_vx_dz = inv * (at(0, i, j, k + 1) - at(0, i, j, k - 1));
+        const float d_vx_dy = inv * (at(0, i, j + 1, k) - at(0, i, j - 1, k));
+        const float d_vy_dx = inv * (at(1, i + 1, j, k) - at(1, i - 1, j, k));
+        const float d_vy_dz = inv * (at(1, i, j, k + 1) - at(1, i, j, k - 1));
+        const float d_vz_dx = inv * (at(2, i + 1, j, k) - at(2, i - 1, j, k));
+        const float d_vz_dy = inv * (at(2, i, j + 1, k) - at(2, i, j - 1, k));
+        cf[q] = d_vz_dy - d_vy_dz;
+        cf[C + q] = d_vx_dz - d_vz_dx;
+        cf[2 * C + q] = d_vy_dx - d_vx_dy;
+    } else {
+        const float vr_y = at(1, i + 1, j, 0), vl_y = at(1, i - 1, j, 0);
+        const float vt_x = at(0, i, j + 1, 0), vb_x = at(0, i, j - 1, 0);
+        cf[q] = (((vr_y - vl_y) - vt_x) + vb_x) / two_dx;
+    }
+}
+
 // Point vortices: angular_vel_func + leapfrog_vel_func, 2D/init_conditions.py:5-9,29-60.
 constexpr int kMaxVort = 4096;
 __constant__ float c_vort[3 * kMaxVort];
@@ -452,6 +491,22 @@ int pfm_get_central(const pfm_grid* gp, const float* u_x, const float* u_y, cons
     else
         hipLaunchKernelGGL(k_get_central<2>, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), g, u_x, u_y,
                            u_z, vc);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_curl(const pfm_grid* gp, const float* vc, float* out, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(vc && out, "null pointer");
+    const int64_t C = cell_count(g);
+    const float two_dx = 2.0f * g.dx;
+    const float inv = 1.0f / two_dx;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_curl<3>, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), g, vc, inv, two_dx, out);
+    else
+        hipLaunchKernelGGL(k_curl<2>, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), g, vc, inv, two_dx, out);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

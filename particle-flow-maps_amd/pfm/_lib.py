@@ -57,6 +57,7 @@ _SIGS = {
     "pfm_max_speed": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "pfm_split_central": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "pfm_get_central": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "pfm_curl": [c_vp, c_vp, c_vp, c_vp],
     "pfm_point_vortices": [c_vp, c_int, c_vp, c_vp, c_vp, c_f, c_f, c_vp, c_vp],
     "pfm_vortex_ring": [c_vp, c_f, c_f, c_f, c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
     "pfm_kinetic_energy": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],

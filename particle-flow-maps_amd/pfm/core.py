@@ -317,6 +317,19 @@ def get_central(res, u, vc, stream=None):
     L.call("pfm_get_central", ctypes.byref(g), *_u_ptrs(u, res), L.ptr(vc, name="vc"), _stream(stream))
 
 
+def curl(res, dx, vc, out, stream=None):
+    """curl (3D/taichi_utils.py:56-78; 2D/taichi_utils.py:97-103) of the
+    cell-centred vector vc (d, *res): out is (3, *res) in 3D, res in 2D."""
+    d = len(res)
+    g = L.make_grid(res, dx)
+    if tuple(vc.shape) != (d,) + tuple(res):
+        raise L.PfmError(f"central field must be (d, *res), got {tuple(vc.shape)}")
+    want = (3,) + tuple(res) if d == 3 else tuple(res)
+    if tuple(out.shape) != want:
+        raise L.PfmError(f"curl output must be {want}, got {tuple(out.shape)}")
+    L.call("pfm_curl", ctypes.byref(g), L.ptr(vc, name="vc"), L.ptr(out, name="out"), _stream(stream))
+
+
 def point_vortices(res, dx, centers, weights, vc, rad=0.02, strength=-0.01, stream=None):
     import numpy as np
 

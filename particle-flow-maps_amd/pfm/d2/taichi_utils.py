@@ -17,3 +17,8 @@ def split_central_vector(central, horizontal, vertical):
 def get_central_vector(horizontal, vertical, central):
     """2D/taichi_utils.py:50-54."""
     core.get_central(tuple(central.shape[1:]), (horizontal, vertical), central)
+
+
+def curl(vf, cf, dx):
+    """2D/taichi_utils.py:97-103: vf is cell-centred (2, rx, ry), cf (rx, ry)."""
+    core.curl(tuple(vf.shape[1:]), dx, vf, cf)

@@ -17,3 +17,8 @@ def split_central_vector(vc, vx, vy, vz):
 def get_central_vector(vx, vy, vz, vc):
     """3D/taichi_utils.py:80-85: faces -> cell-centred (3, rx, ry, rz)."""
     core.get_central(tuple(vc.shape[1:]), (vx, vy, vz), vc)
+
+
+def curl(vf, cf, dx):
+    """3D/taichi_utils.py:56-78: vf, cf are cell-centred (3, rx, ry, rz)."""
+    core.curl(tuple(vf.shape[1:]), dx, vf, cf)

@@ -431,3 +431,40 @@ def test_poisson_projection_divergence_free(oracle):
     u0 = smooth_mac_field(res, seed=14)
     div0 = (u0[0][1:] - u0[0][:-1]) + (u0[1][:, 1:] - u0[1][:, :-1]) + (u0[2][:, :, 1:] - u0[2][:, :, :-1])
     assert np.abs(div).max() < 1e-4 * np.abs(div0).max()
+
+
+# ---- diagnostics: curl ------------------------------------------------------
+def test_curl_linear_field_and_walls(oracle):
+    """Interior: the exact curl of a linear field.  Walls: sample() clamps the
+    neighbour, so the difference is one-sided over 2 dx (numpy edge padding)."""
+    dx = 0.125
+    res = (6, 7, 8)
+    x, y, z = [(np.arange(n) + 0.5) * dx for n in res]
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    vc = np.stack([2 * Y + 3 * Z, -X + 0.5 * Z, 4 * X - Y]).astype(np.float32)
+    c = oracle.curl(res, dx, vc)
+    # curl (2y+3z, -x+z/2, 4x-y) = (-1 - 1/2, 3 - 4, -1 - 2)
+    inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+    np.testing.assert_allclose(c[inner][0], -1.5, atol=1e-5)
+    np.testing.assert_allclose(c[inner][1], -1.0, atol=1e-5)
+    np.testing.assert_allclose(c[inner][2], -3.0, atol=1e-5)
+    # every cell against an edge-padded central difference in float64
+    p = np.pad(vc.astype(np.float64), [(0, 0), (1, 1), (1, 1), (1, 1)], mode="edge")
+
+    def dd(comp, axis):
+        hi = [slice(1, -1)] * 3
+        lo = [slice(1, -1)] * 3
+        hi[axis] = slice(2, None)
+        lo[axis] = slice(0, -2)
+        return (p[comp][tuple(hi)] - p[comp][tuple(lo)]) / (2 * dx)
+
+    want = np.stack([dd(2, 1) - dd(1, 2), dd(0, 2) - dd(2, 0), dd(1, 0) - dd(0, 1)])
+    np.testing.assert_allclose(c, want, atol=1e-4)
+    # 2D: scalar vorticity of (-y, x) is 2 inside
+    res2 = (9, 6)
+    x2, y2 = [(np.arange(n) + 0.5) * dx for n in res2]
+    X2, Y2 = np.meshgrid(x2, y2, indexing="ij")
+    v2 = np.stack([-Y2, X2]).astype(np.float32)
+    c2 = oracle.curl(res2, dx, v2)
+    np.testing.assert_allclose(c2[1:-1, 1:-1], 2.0, atol=1e-5)
+    np.testing.assert_allclose(c2[0, 3], 1.5, atol=1e-5)  # one-sided in x at the wall
