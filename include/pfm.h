@@ -34,7 +34,7 @@ extern "C" {
 #define PFM_OK 0
 #define PFM_ERR_ARG (-1)      /* bad shape, null pointer, unsupported option */
 #define PFM_ERR_HIP (-2)      /* a HIP runtime call failed */
-#define PFM_ERR_OOM (-3)      /* device allocation failed (pfm_mg_create only) */
+#define PFM_ERR_OOM (-3)      /* device allocation failed (pfm_mg_create, pfm_mg_set_solid) */
 #define PFM_WARN_NOT_CONVERGED 1 /* MGPCG hit max_iters (the reference prints and returns) */
 
 typedef void* pfm_stream_t; /* hipStream_t */
@@ -221,6 +221,14 @@ int pfm_mg_n_levels(void* handle);
 /* Device pointer of an internal field (0 r, 1 z, 2 x, 3 p, 4 Ap, 5 u_div,
  * 6 pressure) at `level` (r and z only), for tests and get_result. */
 void* pfm_mg_field(void* handle, int which, int level);
+
+/* Solid obstacle (an extension: the reference solver has none).  `solid`
+ * marks solid cells of the finest level (device bytes, nonzero = solid, the
+ * cell array's shape); NULL removes the obstacle.  Solid cells carry no
+ * pressure; their fluid neighbours see them as walls; the faces touching them
+ * are zeroed by apply_bc / poisson (free slip); the recentring mean is over
+ * the fluid cells.  A coarse cell is solid when all the cells it owns are. */
+int pfm_mg_set_solid(void* h, const uint8_t* solid, pfm_stream_t stream);
 
 /* MGPCG_3.apply_bc (3D/mgpcg.py:359-378), MGPCG_2.apply_bc (2D/mgpcg.py:
  * 292-324). */

@@ -99,6 +99,8 @@ def lib():
             getattr(L, pre + "apply_bc").argtypes = [vp, vp, vp, vp]
             getattr(L, pre + "divergence").argtypes = [vp, vp, vp, vp]
             getattr(L, pre + "subtract_grad_p").argtypes = [vp, vp, vp, vp]
+            getattr(L, pre + "set_solid").argtypes = [vp, vp]
+            getattr(L, pre + "apply_solid_bc").argtypes = [vp, vp, vp, vp]
             getattr(L, pre + "poisson").restype = ci
             getattr(L, pre + "poisson").argtypes = [vp, vp, vp, vp, ci, ctypes.c_double, ci,
                                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ci)]
@@ -364,6 +366,16 @@ class MG:
         """MGPCG_3/MGPCG_2.apply_bc on float32 face arrays in place."""
         self._f("apply_bc")(self.h, *_u3(u))
 
+    def set_solid(self, solid):
+        """Solid obstacle cells (bool/uint8 array of the finest level's shape)."""
+        m = np.ascontiguousarray(np.asarray(solid, bool).reshape(self.N), np.uint8)
+        self._solid = m
+        self._f("set_solid")(self.h, _p(m))
+
+    def apply_solid_bc(self, u):
+        """Zero the faces that touch a solid cell (free slip at the obstacle)."""
+        self._f("apply_solid_bc")(self.h, *_u3(u))
+
     def divergence(self, u):
         """divergence -> the u_div field (3D/mgpcg.py:380-390)."""
         self._f("divergence")(self.h, *_u3(u))
@@ -377,12 +389,14 @@ class MG:
         return it, rtr.value, bool(conv.value)
 
 
-def poisson_fp64_solve(rhs, base_level=3, boundary_types=None, tol=1e-8):
+def poisson_fp64_solve(rhs, base_level=3, boundary_types=None, tol=1e-8, solid=None):
     """FP64 oracle: solve A x = -rhs (the reference's init(k=-1)) in double to a
     relative residual `tol` (PCG with the reference V-cycle stalls near 1e-9,
     giving solutions within ~1e-8 relative of a direct solve)."""
     N = rhs.shape
     mg = MG(N, base_level=base_level, boundary_types=boundary_types, fp64=True)
+    if solid is not None:
+        mg.set_solid(solid)
     mg.init(rhs.astype(np.float64), -1.0)
     it, rtr, conv = mg.solve(max_iters=200, tol=tol, tol_mode=1)
     return mg.get_result(), it, conv

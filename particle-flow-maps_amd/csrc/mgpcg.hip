@@ -68,6 +68,11 @@ struct MGHandle {
     int n_partial;
     MGScalars* sc;          // device
     double* host_rtr;       // pinned
+    // solid obstacle (extension; nullptr without one): per level, 1 = solid,
+    // and the diagonal (0 for a solid cell)
+    uint8_t* solid[kMaxLevels];
+    uint8_t* dg[kMaxLevels];
+    int64_t n_fluid;        // cells that carry pressure (the recentring mean)
 };
 
 // Stencil description of one level, always in 3D index form.
@@ -75,6 +80,10 @@ struct LvlDev {
     int nb2;                 // 2 * dim: neighbour count of an interior cell
     int d[3];
     int bc_lo[3], bc_hi[3];  // 1 when that wall decrements the diagonal
+    // With a solid obstacle: the diagonal of every cell (open walls and solid
+    // neighbours removed), 0 for a solid cell; nullptr without one.  Solid
+    // cells hold z = r = p = x = 0, so as neighbours they add +0.
+    const uint8_t* dg;
 };
 
 __device__ __forceinline__ void cell_ijk(const LvlDev& L, int64_t q, int I[3]) {
@@ -110,8 +119,9 @@ __device__ __forceinline__ bool inside(const LvlDev& L, int g0, int g1, int g2) 
     return (unsigned)g0 < (unsigned)L.d[0] && (unsigned)g1 < (unsigned)L.d[1] && (unsigned)g2 < (unsigned)L.d[2];
 }
 
-// num_fluid_neighbors: 3D/mgpcg.py:106-115.
+// num_fluid_neighbors: 3D/mgpcg.py:106-115 (with an obstacle, the stored diagonal).
 __device__ __forceinline__ float nfn(const LvlDev& L, int i0, int i1, int i2) {
+    if (L.dg) return (float)L.dg[((int64_t)i0 * L.d[1] + i1) * L.d[2] + i2];
     float num = (float)L.nb2;
     const int I[3] = {i0, i1, i2};
 #pragma unroll
@@ -123,6 +133,8 @@ __device__ __forceinline__ float nfn(const LvlDev& L, int i0, int i1, int i2) {
 }
 
 __device__ __forceinline__ float nfn(const LvlDev& L, const int I[3]) { return nfn(L, I[0], I[1], I[2]); }
+
+__device__ __forceinline__ bool solid_at(const LvlDev& L, int64_t q) { return L.dg && L.dg[q] == 0; }
 
 // neighbor_sum: 3D/mgpcg.py:92-104 (per axis: +e then -e).  `x` points at
 // the cell; s0 / s1 are the strides of axes 0 / 1 (axis 2 has stride 1).
@@ -156,6 +168,7 @@ __global__ void __launch_bounds__(kBlock) k_smooth(LvlDev L, float* __restrict__
     int I[3];
     if (!cell_of_thread(L, phase, true, I)) return;
     const int64_t q = cell_index(L, I);
+    if (solid_at(L, q)) return;
     z[q] = __fdiv_rn(__fadd_rn(r[q], nsum(L, z, q, I)), nfn(L, I));
 }
 
@@ -165,7 +178,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_A(LvlDev L, const float* __res
     int I[3];
     if (!cell_of_thread(L, 0, false, I)) return;
     const int64_t q = cell_index(L, I);
-    Ap[q] = __fsub_rn(__fmul_rn(nfn(L, I), p[q]), nsum(L, p, q, I));
+    Ap[q] = solid_at(L, q) ? 0.0f : __fsub_rn(__fmul_rn(nfn(L, I), p[q]), nsum(L, p, q, I));
 }
 
 // restrict (3D/mgpcg.py:152-158) as a gather over the 2^d children.  Any
@@ -186,6 +199,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(LvlDev Lf, LvlDev Lc, float
             for (int c = 0; c <= hi[2]; ++c) {
                 int I[3] = {2 * C[0] + a, 2 * C[1] + b, 2 * C[2] + c};
                 const int64_t q = cell_index(Lf, I);
+                if (solid_at(Lf, q)) continue;
                 const float res = __fsub_rn(r[q], __fsub_rn(__fmul_rn(nfn(Lf, I), z[q]), nsum(Lf, z, q, I)));
                 acc = __fadd_rn(acc, __fdiv_rn(__fmul_rn(res, 1.0f), scale));
             }
@@ -203,6 +217,7 @@ __global__ void __launch_bounds__(kBlock) k_prolongate(LvlDev Lf, LvlDev Lc, flo
     int I[3];
     if (!cell_of_thread(Lf, 0, false, I)) return;
     const int64_t q = cell_index(Lf, I);
+    if (solid_at(Lf, q)) return;  // solid cells carry no pressure
     z[q] = __fadd_rn(z[q], zc[parent_index(Lc, I[0], I[1], I[2])]);
 }
 
@@ -254,6 +269,18 @@ __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t rs, uint32_t boff) {
 // Byte offset of cell g, or kOOB outside the level.
 __device__ __forceinline__ uint32_t cell_off(const LvlDev& L, int g0, int g1, int g2) {
     return inside(L, g0, g1, g2) ? (uint32_t)(((g0 * L.d[1] + g1) * L.d[2] + g2) * 4) : kOOB;
+}
+
+// The obstacle's diagonal byte of cell g (0 = solid; 0 outside the level).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t dg_rsrc(const LvlDev& L) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(L.dg), (short)0,
+                                             (int)((uint32_t)L.d[0] * (uint32_t)L.d[1] * (uint32_t)L.d[2]),
+                                             0x00020000);
+}
+
+__device__ __forceinline__ int ld_dg(__amdgpu_buffer_rsrc_t rs, const LvlDev& L, int g0, int g1, int g2) {
+    const uint32_t off = inside(L, g0, g1, g2) ? (uint32_t)((g0 * L.d[1] + g1) * L.d[2] + g2) : kOOB;
+    return (int)__builtin_amdgcn_raw_buffer_load_b8(rs, off, 0, 0);
 }
 
 // Walls next to cell g that decrement its diagonal (num_fluid_neighbors).
@@ -369,7 +396,11 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
             const bool in = e < NS && inside(L, g0, g1, g2);
             float v = ld(rz, in ? cell_off(L, g0, g1, g2) : kOOB);
             // outside the level both loads read 0 and 0 + 0 stays +0
-            if (PROLONG) v = __fadd_rn(v, ld(rc, in ? (uint32_t)parent_index(Lc, g0, g1, g2) * 4u : kOOB));
+            if (PROLONG) {
+                v = __fadd_rn(v, ld(rc, in ? (uint32_t)parent_index(Lc, g0, g1, g2) * 4u : kOOB));
+                // solid cells take no prolongation (they carry no pressure)
+                if (L.dg && ld_dg(dg_rsrc(L), L, g0, g1, g2) == 0) v = 0.f;
+            }
             zv[k] = v;
         }
     }
@@ -410,12 +441,18 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
         rb_slot<R0J, R0K2>(min(e, N0 - 1), HI - A0, H - 1, H - 1, b0, b1, b2, c0, li, lj, lk, g0, g1, g2);
         const int le = (li * SJ + lj) * SK + lk;
         const float rv = SUB_MEAN ? __fsub_rn(rv0[k], mean) : rv0[k];
-        const Diag dg = diag(L, interior, n_int, y_int, g0, g1, g2);
+        Diag dg = diag(L, interior && !L.dg, n_int, y_int, g0, g1, g2);
+        bool fluid = true;
+        if (L.dg) {
+            const int n = ld_dg(dg_rsrc(L), L, g0, g1, g2);
+            dg = Diag{(float)n, pfm_rcp_small(n)};
+            fluid = n != 0;
+        }
         const float v = pfm_div_small(__fadd_rn(rv, nsum_lds<DIM>(zs + le, SP, SK)), dg.n, dg.y);
-        if (e < N0 && inside(L, g0, g1, g2)) zs[le] = v;
+        const bool upd = e < N0 && inside(L, g0, g1, g2) && fluid;
+        if (upd) zs[le] = v;
         // cells of colour c0 inside the tile are final after this pass
-        if (DOT && e < N0 && g0 >= o0 && g0 < o0 + T::I && g1 >= o1 && g1 < o1 + T::J && g2 >= o2 &&
-            g2 < o2 + T::K && inside(L, g0, g1, g2))
+        if (DOT && upd && g0 >= o0 && g0 < o0 + T::I && g1 >= o1 && g1 < o1 + T::J && g2 >= o2 && g2 < o2 + T::K)
             s += (double)v * (double)rv;
     }
     __syncthreads();
@@ -429,9 +466,15 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
         rb_slot<R1J, R1K2>(min(e, N1 - 1), HI, H, H, b0, b1, b2, c1, li, lj, lk, g0, g1, g2);
         const int le = (li * SJ + lj) * SK + lk;
         const float rv = SUB_MEAN ? __fsub_rn(rv1[k], mean) : rv1[k];
-        const Diag dg = diag(L, interior1, n_int, y_int, g0, g1, g2);
+        Diag dg = diag(L, interior1 && !L.dg, n_int, y_int, g0, g1, g2);
+        bool fluid = true;
+        if (L.dg) {
+            const int n = ld_dg(dg_rsrc(L), L, g0, g1, g2);
+            dg = Diag{(float)n, pfm_rcp_small(n)};
+            fluid = n != 0;
+        }
         const float v = pfm_div_small(__fadd_rn(rv, nsum_lds<DIM>(zs + le, SP, SK)), dg.n, dg.y);
-        const bool in = e < N1 && inside(L, g0, g1, g2);
+        const bool in = e < N1 && inside(L, g0, g1, g2) && fluid;
         if (in) zs[le] = v;
         if (DOT && in) s += (double)v * (double)rv;
     }
@@ -505,16 +548,23 @@ __global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc,
         const int e = tid + k * kBlock;
         const int fk = e % FK, fj = (e / FK) % FJ, fi = e / (FK * FJ);
         const int g0 = o0 + fi, g1 = o1 + fj, g2 = o2 + fk;
-        const bool in = inside(Lf, g0, g1, g2);
+        float nf = interior ? n_int : (float)(Lf.nb2 - walls(Lf, g0, g1, g2));
+        bool fluid = true;
+        if (Lf.dg) {
+            const int n = ld_dg(dg_rsrc(Lf), Lf, g0, g1, g2);
+            nf = (float)n;
+            fluid = n != 0;
+        }
+        const bool in = inside(Lf, g0, g1, g2) && fluid;
         float rq = rv[k];
         if (SUB_MEAN) {
             rq = __fsub_rn(rq, mean);
             if (in) r[(g0 * Lf.d[1] + g1) * Lf.d[2] + g2] = rq;
         }
         const int le = ((fi + HI) * SJ + fj + 1) * SK + fk + 1;
-        const float nf = interior ? n_int : (float)(Lf.nb2 - walls(Lf, g0, g1, g2));
         const float res = __fsub_rn(rq, __fsub_rn(__fmul_rn(nf, zs[le]), nsum_lds<DIM>(zs + le, SP, SK)));
-        ts[e] = __fmul_rn(__fmul_rn(res, 1.0f), inv_scale);
+        // solid children add nothing (the oracle skips them; acc starts at +0)
+        ts[e] = in ? __fmul_rn(__fmul_rn(res, 1.0f), inv_scale) : 0.f;
     }
     __syncthreads();
     constexpr int NA = DIM == 3 ? 2 : 1;
@@ -583,8 +633,14 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
         const int g0 = o0 + ti, g1 = o1 + tj, g2 = o2 + tk;
         const int le = ((ti + HI) * SJ + tj + 1) * SK + tk + 1;
         const float pq = ps[le];
-        const float nf = interior ? n_int : (float)(L.nb2 - walls(L, g0, g1, g2));
-        const float a = __fsub_rn(__fmul_rn(nf, pq), nsum_lds<DIM>(ps + le, SP, SK));
+        float nf = interior ? n_int : (float)(L.nb2 - walls(L, g0, g1, g2));
+        bool fluid = true;
+        if (L.dg) {
+            const int n = ld_dg(dg_rsrc(L), L, g0, g1, g2);
+            nf = (float)n;
+            fluid = n != 0;
+        }
+        const float a = fluid ? __fsub_rn(__fmul_rn(nf, pq), nsum_lds<DIM>(ps + le, SP, SK)) : 0.f;
         if (inside(L, g0, g1, g2)) {
             const int q = (g0 * L.d[1] + g1) * L.d[2] + g2;
             p_new[q] = pq;
@@ -648,7 +704,8 @@ __device__ void coarse_pass(const LvlDev& L, float* zs, const float* rs, int c) 
         for (int pk = 0; pk < h2; ++pk) {
             const int k = 2 * pk + off;
             if (k >= L.d[2]) break;
-            const int n = L.nb2 - walls(L, i, j, k);
+            const int n = L.dg ? (int)L.dg[rb + k] : L.nb2 - walls(L, i, j, k);
+            if (n == 0) continue;  // solid cell
             zs[zb + k] = pfm_div_small(__fadd_rn(rs[rb + k], nsum_lds<DIM>(zs + zb + k, P.s0, P.s1)), (float)n,
                                        pfm_rcp_small(n));
         }
@@ -673,7 +730,9 @@ __device__ void coarse_restrict(const LvlDev& Lf, const LvlDev& Lc, const float*
                     for (int c = 0; c <= h2; ++c) {
                         const int i = 2 * C0 + a, j = 2 * C1 + b, k = 2 * C2 + c;
                         const int zq = P.at(i, j, k);
-                        const float nf = (float)(Lf.nb2 - walls(Lf, i, j, k));
+                        const int n = Lf.dg ? (int)Lf.dg[(i * Lf.d[1] + j) * Lf.d[2] + k] : Lf.nb2 - walls(Lf, i, j, k);
+                        if (n == 0) continue;  // solid children add nothing
+                        const float nf = (float)n;
                         const float res = __fsub_rn(rs[(i * Lf.d[1] + j) * Lf.d[2] + k],
                                                     __fsub_rn(__fmul_rn(nf, zs[zq]), nsum_lds<DIM>(zs + zq, P.s0, P.s1)));
                         acc = __fadd_rn(acc, __fmul_rn(__fmul_rn(res, 1.0f), inv_scale));
@@ -691,6 +750,7 @@ __device__ void coarse_prolong(const LvlDev& Lf, const LvlDev& Lc, float* zs, co
         const int i = row / Lf.d[1], j = row - i * Lf.d[1];
         const int c0 = min(i >> 1, Lc.d[0] - 1), c1 = min(j >> 1, Lc.d[1] - 1);
         for (int k = 0; k < Lf.d[2]; ++k) {
+            if (Lf.dg && Lf.dg[row * Lf.d[2] + k] == 0) continue;  // solid cells carry no pressure
             const int q = P.at(i, j, k);
             zs[q] = __fadd_rn(zs[q], zcs[Pc.at(c0, c1, min(k >> 1, Lc.d[2] - 1))]);
         }
@@ -710,7 +770,7 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
         const float mean = sub_mean ? sc->mean : 0.f;
         for (int q = tid; q < n0; q += kBlock) {
             float v = a.r[0][q];
-            if (sub_mean) {
+            if (sub_mean && !(L0.dg && L0.dg[q] == 0)) {
                 v = __fsub_rn(v, mean);
                 a.r[0][q] = v;
             }
@@ -748,8 +808,8 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
                     const int row = s / h2, pk = s - row * h2;
                     const int i = row / L.d[1], j = row - i * L.d[1];
                     const int k = 2 * pk + ((c + i + j) & 1);
-                    if (k < L.d[2]) {
-                        const int n = L.nb2 - walls(L, i, j, k);
+                    const int n = k >= L.d[2] ? 0 : (L.dg ? (int)L.dg[row * L.d[2] + k] : L.nb2 - walls(L, i, j, k));
+                    if (n != 0) {
                         bl[c][m] = P.at(i, j, k);
                         br[c][m] = rs[row * L.d[2] + k];
                         bn[c][m] = (float)n;
@@ -822,7 +882,7 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom(LvlDev L, int n, floa
         if (q < n) {
             int I[3];
             cell_ijk(L, q, I);
-            col[k] = (I[0] + I[1] + I[2]) & 1;
+            col[k] = solid_at(L, q) ? -1 : (I[0] + I[1] + I[2]) & 1;
             rv[k] = r[q];
             mv[k] = nfn(L, I);
             const int st[3] = {L.d[1] * L.d[2], L.d[2], 1};
@@ -906,11 +966,12 @@ __global__ void __launch_bounds__(kBlock) k_update_p(int64_t n, float* __restric
 }
 
 __global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t n, float* __restrict__ r,
-                                                     const MGScalars* __restrict__ sc) {
+                                                     const MGScalars* __restrict__ sc,
+                                                     const uint8_t* __restrict__ dg) {
     const float m = sc->mean;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x)
-        r[q] = __fsub_rn(r[q], m);
+        if (!dg || dg[q] != 0) r[q] = __fsub_rn(r[q], m);  // solid cells stay 0
 }
 
 enum FinalizeMode { kFinMean = 0, kFinR0 = 1, kFinZTr = 2, kFinAlpha = 3, kFinRTr = 4, kFinBeta = 5 };
@@ -997,7 +1058,7 @@ __global__ void __launch_bounds__(kBlock) k_div_init(LvlDev L, const float* __re
         const float vt = uy[(int64_t)i * (n1 + 1) + j + 1];
         div = __fsub_rn(__fadd_rn(__fsub_rn(vr, vl), vt), vb);
     }
-    r[q] = __fmul_rn(div, k);
+    r[q] = solid_at(L, q) ? 0.f : __fmul_rn(div, k);
     z[q] = 0.f;
     Ap[q] = 0.f;
     p[q] = 0.f;
@@ -1007,10 +1068,10 @@ __global__ void __launch_bounds__(kBlock) k_div_init(LvlDev L, const float* __re
 __global__ void __launch_bounds__(kBlock) k_init_rhs(int64_t n, const float* __restrict__ rhs, float k,
                                                      float* __restrict__ r, float* __restrict__ z,
                                                      float* __restrict__ Ap, float* __restrict__ p,
-                                                     float* __restrict__ x) {
+                                                     float* __restrict__ x, const uint8_t* __restrict__ dg) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    r[q] = __fmul_rn(rhs[q], k);
+    r[q] = (dg && dg[q] == 0) ? 0.f : __fmul_rn(rhs[q], k);
     z[q] = 0.f;
     Ap[q] = 0.f;
     p[q] = 0.f;
@@ -1049,6 +1110,73 @@ __global__ void __launch_bounds__(kBlock) k_zero_plane(int S0, int S1, int S2, i
     u[((int64_t)I[0] * S[1] + I[1]) * S[2] + I[2]] = 0.0f;
 }
 
+// ---- solid obstacle (extension of the reference solver) ----------------------
+
+// Free slip at the obstacle: zero the faces of `axis` (F = that face array in
+// 3D index form) that have a solid cell on either side.
+__global__ void __launch_bounds__(kBlock) k_zero_solid_faces(LvlDev L, LvlDev F, int axis, float* __restrict__ u) {
+    int I[3];
+    if (!cell_of_thread(F, 0, false, I)) return;
+    int R[3] = {I[0], I[1], I[2]}, Lc[3] = {I[0], I[1], I[2]};
+    Lc[axis] -= 1;
+    const bool sr = R[axis] < L.d[axis] && solid_at(L, cell_index(L, R));
+    const bool sl = Lc[axis] >= 0 && solid_at(L, cell_index(L, Lc));
+    if (sr || sl) u[cell_index(F, I)] = 0.0f;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mask01(int64_t n, const uint8_t* __restrict__ in, uint8_t* __restrict__ out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = in[q] != 0;
+}
+
+// A coarse cell is solid when every fine cell it owns is (the last coarse
+// cell of an odd fine extent also owns the trailing fine cell).
+__global__ void __launch_bounds__(kBlock) k_solid_coarsen(LvlDev Lf, LvlDev Lc, const uint8_t* __restrict__ sf,
+                                                          uint8_t* __restrict__ sc) {
+    int C[3];
+    if (!cell_of_thread(Lc, 0, false, C)) return;
+    int hi[3];
+    for (int a = 0; a < 3; ++a) hi[a] = (C[a] == Lc.d[a] - 1) ? Lf.d[a] - 1 - 2 * C[a] : 1;
+    uint8_t all = 1;
+    for (int a = 0; a <= hi[0]; ++a)
+        for (int b = 0; b <= hi[1]; ++b)
+            for (int c = 0; c <= hi[2]; ++c) {
+                int I[3] = {2 * C[0] + a, 2 * C[1] + b, 2 * C[2] + c};
+                all &= sf[cell_index(Lf, I)];
+            }
+    sc[cell_index(Lc, C)] = all;
+}
+
+// Diagonal of every cell: 2 dim minus the counting walls and the solid
+// neighbours; 0 for a solid cell.  L carries no dg yet (walls only).
+__global__ void __launch_bounds__(kBlock) k_solid_diag(LvlDev L, const uint8_t* __restrict__ sol,
+                                                       uint8_t* __restrict__ dg) {
+    int I[3];
+    if (!cell_of_thread(L, 0, false, I)) return;
+    const int64_t q = cell_index(L, I);
+    if (sol[q]) {
+        dg[q] = 0;
+        return;
+    }
+    int n = L.nb2;
+    for (int a = 0; a < 3; ++a) {
+        if (L.d[a] == 1 && !L.bc_lo[a] && !L.bc_hi[a]) continue;  // the unused axis of a 2D level
+        int J[3] = {I[0], I[1], I[2]};
+        if (I[a] <= 0) n -= L.bc_lo[a];
+        else {
+            J[a] = I[a] - 1;
+            n -= sol[cell_index(L, J)];
+        }
+        J[a] = I[a];
+        if (I[a] >= L.d[a] - 1) n -= L.bc_hi[a];
+        else {
+            J[a] = I[a] + 1;
+            n -= sol[cell_index(L, J)];
+        }
+    }
+    dg[q] = (uint8_t)n;
+}
+
 // ---- host side ------------------------------------------------------------
 
 // Axis of the 3D index form that holds axis a of the problem.
@@ -1057,6 +1185,7 @@ static inline int map_axis(const MGHandle* h, int a) { return a + (3 - h->dim); 
 static LvlDev lvl_dev(const MGHandle* h, int l) {
     LvlDev L;
     L.nb2 = 2 * h->dim;
+    L.dg = h->dg[l];
     for (int m = 0; m < 3; ++m) {
         L.d[m] = h->lv[l].d[m];
         L.bc_lo[m] = L.bc_hi[m] = 0;
@@ -1184,7 +1313,7 @@ static int rbgs(MGHandle* h, int l, int kind, int c0, const float* zin, float* z
 static int restrict_fused(MGHandle* h, int l, bool sub_mean, hipStream_t s) {
     if (!restrict_tiled_ok(h, l)) {
         if (sub_mean) {
-            hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[l].n, h->r[l], h->sc);
+            hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[l].n, h->r[l], h->sc, h->dg[l]);
             PFM_LAUNCH_CHECK();
         }
         return restrict_level(h, l, s);
@@ -1286,7 +1415,7 @@ static int vcycle(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
     const bool tiled_mean = sub_mean && top > 0 && restrict_tiled_ok(h, 0);
     const bool coarse_mean = sub_mean && lc == 0;
     if (sub_mean && !tiled_mean && !coarse_mean) {
-        hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[0].n, h->r[0], h->sc);
+        hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[0].n, h->r[0], h->sc, h->dg[0]);
         PFM_LAUNCH_CHECK();
     }
     for (int l = 0; l < top; ++l) {
@@ -1312,7 +1441,7 @@ static int vcycle(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
 
 static int finalize(MGHandle* h, int mode, double eps, hipStream_t s, int nparts = kMgRedBlocks) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, h->partial, nparts, mode, eps,
-                       (double)h->lv[0].n, h->sc);
+                       (double)h->n_fluid, h->sc);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -1370,7 +1499,7 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
     if (neumann) {
         PFM_TRY(dot(h, h->r[0], nullptr, s));
         PFM_TRY(finalize(h, kFinMean, eps, s));
-        hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->r[0], h->sc);
+        hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->r[0], h->sc, h->dg[0]);
         PFM_LAUNCH_CHECK();
     }
     double thresh = tol;
@@ -1443,6 +1572,17 @@ static int apply_bc(MGHandle* h, float* const* u, hipStream_t s) {
                     PFM_LAUNCH_CHECK();
                 }
             }
+        }
+    }
+    // free slip at the obstacle (an extension: the reference has no solids)
+    if (h->dg[0]) {
+        const LvlDev L0 = lvl_dev(h, 0);
+        for (int a = 0; a < h->dim; ++a) {
+            const int m = map_axis(h, a);
+            LvlDev F = L0;
+            F.d[m] += 1;
+            hipLaunchKernelGGL(k_zero_solid_faces, cell_grid(F.d, false), kCellBlock, 0, s, L0, F, m, u[a]);
+            PFM_LAUNCH_CHECK();
         }
     }
     return PFM_OK;
@@ -1534,6 +1674,7 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
         h->lv[l].n = n;
     }
     h->coarse_level = pick_coarse_level(h);
+    h->n_fluid = h->lv[0].n;
     auto fail = [&](const char* what) {
         set_error("hipMalloc failed for %s", what);
         pfm_mg_destroy(h);
@@ -1578,6 +1719,8 @@ int pfm_mg_destroy(void* hv) {
         if (h->r[l]) (void)hipFree(h->r[l]);
         if (h->z[l]) (void)hipFree(h->z[l]);
         if (h->z2[l]) (void)hipFree(h->z2[l]);
+        if (h->solid[l]) (void)hipFree(h->solid[l]);
+        if (h->dg[l]) (void)hipFree(h->dg[l]);
     }
     if (h->x) (void)hipFree(h->x);
     if (h->p) (void)hipFree(h->p);
@@ -1591,6 +1734,67 @@ int pfm_mg_destroy(void* hv) {
 }
 
 int pfm_mg_n_levels(void* hv) { return hv ? ((MGHandle*)hv)->n_levels : 0; }
+
+int pfm_mg_set_solid(void* hv, const uint8_t* solid, pfm_stream_t stream) {
+    MGHandle* h = (MGHandle*)hv;
+    PFM_CHECK_ARG(h, "null solver handle");
+    hipStream_t s = as_stream(stream);
+    if (!solid) {
+        PFM_HIP_TRY(hipStreamSynchronize(s));
+        for (int l = 0; l < kMaxLevels; ++l) {
+            if (h->solid[l]) (void)hipFree(h->solid[l]);
+            if (h->dg[l]) (void)hipFree(h->dg[l]);
+            h->solid[l] = h->dg[l] = nullptr;
+        }
+        h->n_fluid = h->lv[0].n;
+        return PFM_OK;
+    }
+    for (int l = 0; l < h->n_levels; ++l) {
+        const size_t b = (size_t)h->lv[l].n;
+        if (!h->solid[l] && hipMalloc(&h->solid[l], b) != hipSuccess) {
+            set_error("hipMalloc failed for the solid mask");
+            return PFM_ERR_OOM;
+        }
+        if (!h->dg[l] && hipMalloc(&h->dg[l], b) != hipSuccess) {
+            set_error("hipMalloc failed for the solid diagonal");
+            return PFM_ERR_OOM;
+        }
+    }
+    const int64_t n0 = h->lv[0].n;
+    hipLaunchKernelGGL(k_mask01, dim3(grid_for(n0)), dim3(kBlock), 0, s, n0, solid, h->solid[0]);
+    PFM_LAUNCH_CHECK();
+    for (int l = 0; l + 1 < h->n_levels; ++l) {
+        LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
+        hipLaunchKernelGGL(k_solid_coarsen, cell_grid(Lc.d, false), kCellBlock, 0, s, Lf, Lc, h->solid[l],
+                           h->solid[l + 1]);
+        PFM_LAUNCH_CHECK();
+    }
+    for (int l = 0; l < h->n_levels; ++l) {
+        LvlDev L = lvl_dev(h, l);
+        L.dg = nullptr;
+        hipLaunchKernelGGL(k_solid_diag, cell_grid(L.d, false), kCellBlock, 0, s, L, h->solid[l], h->dg[l]);
+        PFM_LAUNCH_CHECK();
+    }
+    // fluid cells of the finest level (the recentring mean divides by them)
+    uint8_t* host = (uint8_t*)malloc((size_t)n0);
+    if (!host) {
+        set_error("host allocation failed");
+        return PFM_ERR_OOM;
+    }
+    hipError_t e = hipMemcpyAsync(host, h->solid[0], (size_t)n0, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        free(host);
+        set_error("HIP error %s reading the solid mask", hipGetErrorString(e));
+        return PFM_ERR_HIP;
+    }
+    int64_t nf = 0;
+    for (int64_t q = 0; q < n0; ++q) nf += host[q] == 0;
+    free(host);
+    PFM_CHECK_ARG(nf > 0, "the obstacle fills the whole domain");
+    h->n_fluid = nf;
+    return PFM_OK;
+}
 
 void* pfm_mg_field(void* hv, int which, int level) {
     MGHandle* h = (MGHandle*)hv;
@@ -1626,7 +1830,7 @@ int pfm_mg_init(void* hv, const float* rhs, float k, pfm_stream_t stream) {
     PFM_CHECK_ARG(h && rhs, "null argument");
     const int64_t n = h->lv[0].n;
     hipLaunchKernelGGL(k_init_rhs, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n, rhs, k, h->r[0],
-                       h->z[0], h->Ap, h->p, h->x);
+                       h->z[0], h->Ap, h->p, h->x, h->dg[0]);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

@@ -66,6 +66,7 @@ _SIGS = {
     "pfm_mg_n_levels": [c_vp],
     "pfm_mg_field": [c_vp, c_int, c_int],
     "pfm_mg_apply_bc": [c_vp, c_vp, c_vp, c_vp, c_vp],
+    "pfm_mg_set_solid": [c_vp, c_vp, c_vp],
     "pfm_mg_divergence_init": [c_vp, c_vp, c_vp, c_vp, c_f, c_vp],
     "pfm_mg_init": [c_vp, c_vp, c_f, c_vp],
     "pfm_mg_solve": [c_vp, c_int, c_d, c_d, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_d), c_vp],

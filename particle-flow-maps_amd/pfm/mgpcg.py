@@ -82,6 +82,24 @@ class MGPCG:
         except Exception:
             pass
 
+    # ---- solid obstacle (extension) ----
+    def set_solid(self, solid, stream=None):
+        """Mark solid cells of the finest level (bool/uint8 CUDA tensor of shape N,
+        nonzero = solid; None removes the obstacle).  Solid cells carry no
+        pressure, their fluid neighbours treat them as walls, and apply_bc /
+        Poisson zero the faces that touch them (free slip)."""
+        lib = L.load()
+        if solid is None:
+            self._solid = None
+            L.check(lib.pfm_mg_set_solid(self._h, None, L.stream_ptr(stream)), "pfm_mg_set_solid")
+            return
+        if tuple(solid.shape) != tuple(self.N):
+            raise L.PfmError(f"solid mask must have shape {tuple(self.N)}, got {tuple(solid.shape)}")
+        m = solid.to(torch.uint8).contiguous() if solid.dtype != torch.uint8 else solid.contiguous()
+        self._solid = m  # keep alive until the kernels that read it have run
+        L.check(lib.pfm_mg_set_solid(self._h, L.ptr(m, dtype=torch.uint8, name="solid"), L.stream_ptr(stream)),
+                "pfm_mg_set_solid")
+
     # ---- field views (the reference's self.r[l], self.z[l], self.x, ...) ----
     def _level_shape(self, l):
         return tuple(max(1, n >> l) for n in self.N)

@@ -80,3 +80,41 @@ def polyP(Z):
     Z2 = Z @ Z
     Z3 = Z2 @ Z
     return I + Z + Z2 / 2 + Z3 / 6 + Z2 @ Z2 / 24
+
+
+def fluid_laplacian(solid):
+    """Graph Laplacian of the fluid cells of a box with all-Neumann walls and
+    solid cells (scipy sparse, on all cells; solid rows and columns are empty):
+    every pair of face-adjacent fluid cells is an edge."""
+    import scipy.sparse as sp
+
+    shape = solid.shape
+    n = solid.size
+    idx = np.arange(n).reshape(shape)
+    rows, cols = [], []
+    for a in range(len(shape)):
+        lo = [slice(None)] * len(shape)
+        hi = [slice(None)] * len(shape)
+        lo[a] = slice(0, -1)
+        hi[a] = slice(1, None)
+        ok = ~solid[tuple(lo)] & ~solid[tuple(hi)]
+        rows.append(idx[tuple(lo)][ok])
+        cols.append(idx[tuple(hi)][ok])
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    W = sp.coo_matrix((np.ones(r.size), (r, c)), shape=(n, n))
+    W = (W + W.T).tocsr()
+    deg = np.asarray(W.sum(axis=1)).ravel()
+    return (sp.diags(deg) - W).tocsr()
+
+
+def box_solid(shape, lo=0.4, hi=0.6):
+    """Cells whose centre lies in the box [lo, hi]^d of the unit-height domain
+    (dx = 1/shape[1])."""
+    dx = 1.0 / shape[1]
+    axes = [(np.arange(n) + 0.5) * dx for n in shape]
+    grids = np.meshgrid(*axes, indexing="ij")
+    m = np.ones(shape, bool)
+    for g in grids:
+        m &= (g >= lo) & (g <= hi)
+    return m

@@ -120,3 +120,82 @@ def test_solver_api_pieces(oracle, pfm):
     m2.apply_bc(*b)
     for x, y in zip(a, b):
         np.testing.assert_array_equal(N(x), N(y))
+
+
+# ---- solid obstacle (extension) ---------------------------------------------
+@pytest.mark.parametrize("N_", [(16, 16, 16), (32, 32, 32), (64, 64), (50, 34, 34), (40, 72, 24)])
+def test_mg_solid_passes_bit_exact(oracle, pfm, N_):
+    from helpers import box_solid
+
+    d = len(N_)
+    solid = box_solid(N_, 0.3, 0.55)
+    mg_o = oracle.MG(N_, base_level=3)
+    mg_o.set_solid(solid)
+    mg_g = pfm.MGPCG([[2, 2]] * d, list(N_), dim=d, base_level=3)
+    mg_g.set_solid(G(solid))
+    rng = np.random.default_rng(50)
+    p = rng.normal(size=N_).astype(np.float32)
+    p[solid] = 0
+    Ap = torch.empty(N_, device=DEV)
+    mg_g.get_Ap(G(p), Ap)
+    np.testing.assert_array_equal(N(Ap), mg_o.apply_A(0, p))
+    r = rng.normal(size=N_).astype(np.float32)
+    r[solid] = 0
+    z = rng.normal(size=N_).astype(np.float32)
+    z[solid] = 0
+    mg_o.field("r", 0)[...] = r
+    mg_o.field("z", 0)[...] = z
+    mg_g.r[0].copy_(G(r))
+    mg_g.z[0].copy_(G(z))
+    for phase in (0, 1):
+        mg_o.smooth(0, phase)
+        mg_g.smooth(0, phase)
+    np.testing.assert_array_equal(N(mg_g.z[0]), mg_o.field("z", 0))
+    mg_o.restrict(0)
+    mg_g.restrict(0)
+    np.testing.assert_array_equal(N(mg_g.r[1]), mg_o.field("r", 1))
+    # the whole V-cycle
+    r = rng.normal(size=N_).astype(np.float32)
+    r[solid] = 0
+    mg_g.r[0].copy_(G(r))
+    mg_g.apply_preconditioner()
+    torch.cuda.synchronize()
+    zo = mg_o.vcycle(r)
+    np.testing.assert_array_equal(N(mg_g.z[0]), zo)
+    assert np.all(zo[solid] == 0)
+
+
+@pytest.mark.parametrize("N_", [(32, 32, 32), (64, 48)])
+def test_poisson_solid_vs_oracles(oracle, pfm, N_):
+    from helpers import box_solid
+
+    d = len(N_)
+    solid = box_solid(N_, 0.4, 0.6)
+    u = smooth_mac_field(N_, seed=51)
+    ug = [G(x) for x in u]
+    mg_g = pfm.MGPCG([[2, 2]] * d, list(N_), dim=d, base_level=3)
+    mg_g.set_solid(G(solid))
+    mg_g.Poisson(*ug, tol=1e-7, tol_mode="rel")
+    assert mg_g.converged
+    mg_o = oracle.MG(N_, base_level=3)
+    mg_o.set_solid(solid)
+    uo = [x.copy() for x in u]
+    it_o, rtr, conv = mg_o.poisson(uo, 400, 1e-7, 1)
+    assert conv
+    for a in range(d):
+        assert rel_l2(N(ug[a]), uo[a]) < 1e-5
+        np.testing.assert_array_equal(N(ug[a]) == 0, uo[a] == 0)
+    # pressure against the FP64 oracle on the fluid cells (zero mean there)
+    mg_ref = oracle.MG(N_, base_level=3)
+    mg_ref.set_solid(solid)
+    ub = [x.copy() for x in u]
+    mg_ref.apply_bc(ub)
+    mg_ref.apply_solid_bc(ub)
+    mg_ref.divergence(ub)
+    rhs = mg_ref.field("u_div", 0).astype(np.float64).copy()
+    x64, it64, conv64 = oracle.poisson_fp64_solve(rhs, base_level=3, tol=1e-9, solid=solid)
+    pg = N(mg_g.p).astype(np.float64)
+    fl = ~solid
+    assert np.all(pg[solid] == 0)
+    e = rel_l2(pg[fl] - pg[fl].mean(), x64[fl] - x64[fl].mean())
+    assert e < 1e-5, e

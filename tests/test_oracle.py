@@ -468,3 +468,99 @@ def test_curl_linear_field_and_walls(oracle):
     c2 = oracle.curl(res2, dx, v2)
     np.testing.assert_allclose(c2[1:-1, 1:-1], 2.0, atol=1e-5)
     np.testing.assert_allclose(c2[0, 3], 1.5, atol=1e-5)  # one-sided in x at the wall
+
+
+# ---- MGPCG with a solid obstacle (extension) -----------------------------
+def _solid_case(N):
+    from helpers import box_solid
+
+    return box_solid(N, 0.3, 0.55)
+
+
+@pytest.mark.parametrize("N", [(16, 16, 16), (24, 16)])
+def test_mg_solid_operator_and_coarsening(oracle, N):
+    from helpers import fluid_laplacian
+
+    solid = _solid_case(N)
+    assert solid.any() and not solid.all()
+    mg = oracle.MG(N, base_level=2)
+    mg.set_solid(solid)
+    p = np.random.default_rng(40).normal(size=N).astype(np.float32)
+    p[solid] = 0
+    Ap = mg.apply_A(0, p)
+    want = (fluid_laplacian(solid) @ p.astype(np.float64).ravel()).reshape(N)
+    assert np.max(np.abs(Ap - want)) < 1e-5
+    assert np.all(Ap[solid] == 0)
+    # one colour of RB-GS zeroes the residual of that colour on fluid cells, solid z stays 0
+    r = np.random.default_rng(41).normal(size=N).astype(np.float32)
+    r[solid] = 0
+    mg.field("r", 0)[...] = r
+    mg.field("z", 0)[...] = 0
+    mg.smooth(0, 0)
+    z = mg.field("z", 0).astype(np.float64)
+    res = r - (fluid_laplacian(solid) @ z.ravel()).reshape(N)
+    col = np.indices(N).sum(0) & 1
+    assert np.max(np.abs(res[(col == 0) & ~solid])) < 1e-5
+    assert np.all(z[solid] == 0)
+    # restriction: coarse value = sum of the fluid children's residuals / (d-1)
+    mg.restrict(0)
+    rc = mg.field("r", 1)
+    d = len(N)
+    children = res.reshape([n // 2 if i < d else n for i, n in enumerate(N)][:0] + sum(([n // 2, 2] for n in N), []))
+    want_c = children.sum(axis=tuple(range(1, 2 * d, 2))) / (d - 1)
+    assert np.max(np.abs(rc - want_c)) < 1e-5
+
+
+def test_mg_solid_solve_vs_scipy(oracle):
+    import scipy.sparse.linalg as spla
+    from helpers import fluid_laplacian
+
+    N = (16, 16, 16)
+    solid = _solid_case(N)
+    A = fluid_laplacian(solid)
+    fl = ~solid.ravel()
+    rng = np.random.default_rng(42)
+    rhs = rng.normal(size=N)
+    rhs[solid] = 0
+    rhs[~solid] -= rhs[~solid].mean()
+    # direct solve of the fluid block with the zero-mean constraint (bordered system)
+    import scipy.sparse as sp
+
+    Af = A[fl][:, fl]
+    nf = Af.shape[0]
+    K = sp.bmat([[Af, np.ones((nf, 1))], [np.ones((1, nf)), None]]).tocsc()
+    sol = spla.spsolve(K, np.concatenate([-rhs.ravel()[fl], [0.0]]))[:nf]
+    for fp64, tol in ((False, 1e-5), (True, 1e-7)):
+        mg = oracle.MG(N, base_level=3, fp64=fp64)
+        mg.set_solid(solid)
+        mg.init(rhs.astype(np.float64 if fp64 else np.float32), -1.0)
+        it, rtr, conv = mg.solve(max_iters=200, tol=1e-9 if fp64 else 1e-6, tol_mode=1)
+        x = mg.get_result().astype(np.float64).ravel()
+        assert np.all(x[~fl] == 0)
+        xf = x[fl] - x[fl].mean()
+        assert rel_l2(xf, sol - sol.mean()) < tol, (fp64, rel_l2(xf, sol - sol.mean()))
+
+
+def test_mg_solid_projection(oracle):
+    from helpers import box_solid
+
+    res = (16, 16, 16)
+    solid = box_solid(res, 0.3, 0.55)
+    u = smooth_mac_field(res, seed=43)
+    mg = oracle.MG(res, base_level=3)
+    mg.set_solid(solid)
+    it, rtr, conv = mg.poisson(u, max_iters=400, tol=1e-7, tol_mode=1)
+    assert conv
+    # faces touching the obstacle are zero; the divergence vanishes on fluid cells
+    sp = np.pad(solid, 1)
+    for a in range(3):
+        lo = [slice(1, -1)] * 3
+        hi = [slice(1, -1)] * 3
+        lo[a] = slice(0, -1)
+        hi[a] = slice(1, None)
+        touch = sp[tuple(lo)] | sp[tuple(hi)]
+        assert np.all(u[a][touch] == 0)
+    div = (u[0][1:] - u[0][:-1]) + (u[1][:, 1:] - u[1][:, :-1]) + (u[2][:, :, 1:] - u[2][:, :, :-1])
+    u0 = smooth_mac_field(res, seed=43)
+    div0 = (u0[0][1:] - u0[0][:-1]) + (u0[1][:, 1:] - u0[1][:, :-1]) + (u0[2][:, :, 1:] - u0[2][:, :, :-1])
+    assert np.abs(div[~solid]).max() < 1e-4 * np.abs(div0).max()
