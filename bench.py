@@ -324,29 +324,50 @@ def main():
     timer.flush()
     phases = {k: round(v / timer.count[k] * (2 if k == "poisson" else 1), 3) for k, v in timer.total.items()}
 
-    # standalone Poisson solve (MGPCG cells/s): divergence of a seeded
-    # 64-mode Fourier velocity, fp32 MGPCG to a 1e-6 relative residual
-    from pfm import core
-    from pfm.workloads import fourier_velocity
+    # standalone Poisson solves (MGPCG cells/s): divergence of a seeded
+    # 64-mode Fourier velocity with free-slip walls and a solid box obstacle
+    # [0.4, 0.6]^3, to a 1e-6 relative residual; plain fp32 levels and the
+    # fp16 / bf16 V-cycle storage (FP32 PCG, FP64 dots and bottom solve)
+    from pfm.mgpcg import MGPCG
+    from pfm.workloads import box_mask, fourier_velocity
 
     u_f = fourier_velocity(res, seed=0)
-    work = [x.clone() for x in u_f]
-    sim.solver.Poisson(*work, tol=1e-6, tol_mode="rel")  # warm
-    torch.cuda.synchronize()
-    reps = 3
-    pits = []
-    t_poisson = 0.0
-    for _ in range(reps):
-        for a in range(3):
-            work[a].copy_(u_f[a])
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        pits.append(sim.solver.Poisson(*work, tol=1e-6, tol_mode="rel"))
-        torch.cuda.synchronize()
-        t_poisson += time.perf_counter() - t1
-    t_poisson /= reps
+    solid = box_mask(res, 0.4, 0.6)
     cells = math.prod(res)
-    del u_f, work
+    mg_runs = {}
+    p_ref = None
+    for name, prec in (("fp32", 0), ("fp16", 1), ("bf16", 2)):
+        solver = MGPCG([[2, 2]] * 3, list(res), dim=3, base_level=3, precision=prec)
+        solver.set_solid(solid)
+        work = [x.clone() for x in u_f]
+        solver.Poisson(*work, tol=1e-6, tol_mode="rel")  # warm
+        torch.cuda.synchronize()
+        reps, pits, t_p = 3, [], 0.0
+        for _ in range(reps):
+            for a in range(3):
+                work[a].copy_(u_f[a])
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            pits.append(solver.Poisson(*work, tol=1e-6, tol_mode="rel"))
+            torch.cuda.synchronize()
+            t_p += time.perf_counter() - t1
+        t_p /= reps
+        pr = solver.p.double()
+        fl = solid == 0
+        pr = pr - pr[fl].mean()
+        run = {"ms_per_solve": t_p * 1e3, "cells_per_s": cells / t_p, "iters": pits,
+               "converged": bool(solver.converged)}
+        if p_ref is None:
+            p_ref = pr
+        else:
+            run["pressure_rel_l2_vs_fp32"] = float(torch.linalg.vector_norm((pr - p_ref)[fl]) /
+                                                   torch.linalg.vector_norm(p_ref[fl]))
+            run["speedup_vs_fp32"] = mg_runs["fp32"]["ms_per_solve"] / run["ms_per_solve"]
+        mg_runs[name] = run
+        del solver, work
+    t_poisson = mg_runs["fp32"]["ms_per_solve"] * 1e-3
+    pits = mg_runs["fp32"]["iters"]
+    del u_f
 
     # roofline of the step's dominant kernel, RK4 + P2G record
     # (k_rk4_particles<3, true>): per particle it reads pos 12 B, T_grad_m
@@ -385,6 +406,9 @@ def main():
         "mgpcg_cells_per_s": cells / t_poisson,
         "mgpcg_ms_per_solve": t_poisson * 1e3,
         "mgpcg_iters": pits,
+        "mgpcg": {"workload": f"{args.res}^3 Poisson, free-slip walls + solid box [0.4,0.6]^3, "
+                              "RHS = div of 64 Fourier modes (seed 0), 1e-6 relative residual",
+                  **mg_runs},
         "phase_ms": phases,
         "roofline": {"kernel": roof_kernel, "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -23,6 +23,10 @@
 //   - the coarsest level's 50 red/black sweeps run in one workgroup out of LDS.
 #include <algorithm>
 #include <new>
+#include <type_traits>
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 
 #include "pfm_host.h"
 #include "pfm_div.h"
@@ -40,7 +44,10 @@ struct MGScalars {
     double sum_r;
     double new_zTr;
     float alpha, beta, mean;
-    int pad;
+    // 16-bit V-cycle storage: the preconditioner is linear, so it runs on
+    // vs * r (vs a power of two near 1/rms(r), exact to apply and undo) and
+    // its output is read back times vinv = 1/vs; keeps values in fp16 range
+    float vs, vinv;
 };
 
 struct Level {
@@ -57,9 +64,12 @@ struct MGHandle {
     int dirichlet_counts;
     int precision;
     Level lv[kMaxLevels];
-    float* r[kMaxLevels];
-    float* z[kMaxLevels];
-    float* z2[kMaxLevels];  // ping-pong partner of z inside the fused sweeps
+    // level arrays: r[0] is fp32 (the PCG residual); z, z2 and r[l >= 1] are
+    // fp32 at precision 0, __half (1) or __hip_bfloat16 (2) otherwise
+    void* r[kMaxLevels];
+    void* z[kMaxLevels];
+    void* z2[kMaxLevels];   // ping-pong partner of z inside the fused sweeps
+    float* b;               // right-hand side kept for true-residual replacement (precision 1/2)
     float* x;
     float* p;               // current search direction
     float* p2;              // its ping-pong partner
@@ -252,23 +262,44 @@ __device__ __forceinline__ void tile_of_block(const TileMap& tm, int t[3]) {
     t[1] = b / tm.n2;
 }
 
+__device__ __forceinline__ float to_f(float v) { return v; }
+__device__ __forceinline__ float to_f(__half v) { return __half2float(v); }
+__device__ __forceinline__ float to_f(__hip_bfloat16 v) { return __bfloat162float(v); }
+template <typename V> __device__ __forceinline__ V from_f(float v);
+template <> __device__ __forceinline__ float from_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ __half from_f<__half>(float v) { return __float2half(v); }
+template <> __device__ __forceinline__ __hip_bfloat16 from_f<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+
 // Buffer view of a level array: the tiled kernels load through 32-bit byte
 // offsets, and an offset past the end (kOOB) reads 0, which is the value the
 // LDS tiles hold for cells outside the level.  Loads are issued without
 // branches, so all of a thread's loads can be in flight at once.
 constexpr uint32_t kOOB = 0xFFFFFFF0u;
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t lvl_rsrc(const float* p, const LvlDev& L) {
-    return face_rsrc(p, (uint32_t)L.d[0] * (uint32_t)L.d[1] * (uint32_t)L.d[2] * 4u);
+// Level arrays are stored as V = float, or (mixed precision) __half or
+// __hip_bfloat16; every kernel computes in fp32 and rounds to nearest on store.
+template <typename V>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t lvl_rsrc(const V* p, const LvlDev& L) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<V*>(p), (short)0,
+                                             (int)((uint32_t)L.d[0] * (uint32_t)L.d[1] * (uint32_t)L.d[2] *
+                                                   (uint32_t)sizeof(V)),
+                                             0x00020000);
 }
 
-__device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t rs, uint32_t boff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, boff, 0, 0));
+template <typename V>
+__device__ __forceinline__ float ldv(__amdgpu_buffer_rsrc_t rs, uint32_t boff) {
+    if constexpr (sizeof(V) == 4) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, boff, 0, 0));
+    } else {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(rs, boff, 0, 0);
+        return to_f(__builtin_bit_cast(V, b));
+    }
 }
 
-// Byte offset of cell g, or kOOB outside the level.
+// Byte offset of cell g in an array of V, or kOOB outside the level.
+template <typename V>
 __device__ __forceinline__ uint32_t cell_off(const LvlDev& L, int g0, int g1, int g2) {
-    return inside(L, g0, g1, g2) ? (uint32_t)(((g0 * L.d[1] + g1) * L.d[2] + g2) * 4) : kOOB;
+    return inside(L, g0, g1, g2) ? (uint32_t)((g0 * L.d[1] + g1) * L.d[2] + g2) * (uint32_t)sizeof(V) : kOOB;
 }
 
 // The obstacle's diagonal byte of cell g (0 = solid; 0 outside the level).
@@ -361,10 +392,12 @@ __device__ __forceinline__ void rb_slot(int e, int l0, int l1, int l2, int b0, i
 // All global loads (z tile, r of both passes) are issued before the first
 // barrier; the passes are branch free apart from the block-uniform choice
 // between interior tiles (no walls) and tiles that touch the boundary.
-template <int DIM, bool ZERO_IN, bool PROLONG, bool SUB_MEAN, bool DOT>
+// ZT: storage of z (float, or 16-bit in mixed precision); RT: storage of r
+// (float on level 0, ZT on coarser levels).
+template <int DIM, typename ZT, typename RT, bool ZERO_IN, bool PROLONG, bool SUB_MEAN, bool DOT>
 __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap tm,
-                                                 const float* __restrict__ zin, const float* __restrict__ zc,
-                                                 const float* __restrict__ r, float* __restrict__ zout, int c0,
+                                                 const ZT* __restrict__ zin, const ZT* __restrict__ zc,
+                                                 const RT* __restrict__ r, ZT* __restrict__ zout, int c0,
                                                  const MGScalars* __restrict__ sc, double* __restrict__ partial) {
     using T = MgTile<DIM>;
     constexpr int H = 2, HI = DIM == 3 ? H : 0;
@@ -394,10 +427,11 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
             const int lk = e % SK, lj = (e / SK) % SJ, li = e / SP;
             const int g0 = b0 + li, g1 = b1 + lj, g2 = b2 + lk;
             const bool in = e < NS && inside(L, g0, g1, g2);
-            float v = ld(rz, in ? cell_off(L, g0, g1, g2) : kOOB);
+            float v = ldv<ZT>(rz, in ? cell_off<ZT>(L, g0, g1, g2) : kOOB);
             // outside the level both loads read 0 and 0 + 0 stays +0
             if (PROLONG) {
-                v = __fadd_rn(v, ld(rc, in ? (uint32_t)parent_index(Lc, g0, g1, g2) * 4u : kOOB));
+                v = __fadd_rn(v, ldv<ZT>(rc, in ? (uint32_t)parent_index(Lc, g0, g1, g2) * (uint32_t)sizeof(ZT)
+                                              : kOOB));
                 // solid cells take no prolongation (they carry no pressure)
                 if (L.dg && ld_dg(dg_rsrc(L), L, g0, g1, g2) == 0) v = 0.f;
             }
@@ -409,7 +443,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
         int li, lj, lk, g0, g1, g2;
         rb_slot<R0J, R0K2>(min(tid + k * kBlock, N0 - 1), HI - A0, H - 1, H - 1, b0, b1, b2, c0, li, lj, lk, g0,
                            g1, g2);
-        rv0[k] = ld(rr, cell_off(L, g0, g1, g2));
+        rv0[k] = ldv<RT>(rr, cell_off<RT>(L, g0, g1, g2));
     }
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
@@ -417,6 +451,9 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
         if (e < NS) zs[e] = ZERO_IN ? 0.f : zv[k];
     }
     const float mean = SUB_MEAN ? sc->mean : 0.f;
+    // 16-bit z: level 0 works on vs * r (coarser r arrive scaled)
+    constexpr bool kScale = sizeof(ZT) == 2 && sizeof(RT) == 4;
+    const float vs = kScale ? sc->vs : 1.0f;
     // the pass-0 region [o - A0, o + T + A0) x ... decides the interior case
     const bool interior =
         box_interior(L, o0 - A0, o1 - 1, o2 - 1, o0 + T::I + A0, o1 + T::J + 1, o2 + T::K + 1, DIM == 3);
@@ -432,7 +469,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
     for (int k = 0; k < NP1; ++k) {
         int li, lj, lk, g0, g1, g2;
         rb_slot<R1J, R1K2>(min(tid2 + k * kBlock, N1 - 1), HI, H, H, b0, b1, b2, c1, li, lj, lk, g0, g1, g2);
-        rv1[k] = ld(rr, cell_off(L, g0, g1, g2));
+        rv1[k] = ldv<RT>(rr, cell_off<RT>(L, g0, g1, g2));
     }
 #pragma unroll
     for (int k = 0; k < NP0; ++k) {
@@ -440,7 +477,8 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
         int li, lj, lk, g0, g1, g2;
         rb_slot<R0J, R0K2>(min(e, N0 - 1), HI - A0, H - 1, H - 1, b0, b1, b2, c0, li, lj, lk, g0, g1, g2);
         const int le = (li * SJ + lj) * SK + lk;
-        const float rv = SUB_MEAN ? __fsub_rn(rv0[k], mean) : rv0[k];
+        const float rraw = SUB_MEAN ? __fsub_rn(rv0[k], mean) : rv0[k];
+        const float rv = kScale ? rraw * vs : rraw;
         Diag dg = diag(L, interior && !L.dg, n_int, y_int, g0, g1, g2);
         bool fluid = true;
         if (L.dg) {
@@ -451,9 +489,10 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
         const float v = pfm_div_small(__fadd_rn(rv, nsum_lds<DIM>(zs + le, SP, SK)), dg.n, dg.y);
         const bool upd = e < N0 && inside(L, g0, g1, g2) && fluid;
         if (upd) zs[le] = v;
-        // cells of colour c0 inside the tile are final after this pass
+        // cells of colour c0 inside the tile are final after this pass (the
+        // product uses z as stored)
         if (DOT && upd && g0 >= o0 && g0 < o0 + T::I && g1 >= o1 && g1 < o1 + T::J && g2 >= o2 && g2 < o2 + T::K)
-            s += (double)v * (double)rv;
+            s += (double)to_f(from_f<ZT>(v)) * (double)rraw;
     }
     __syncthreads();
     const bool interior1 = box_interior(L, o0, o1, o2, o0 + T::I, o1 + T::J, o2 + T::K, DIM == 3);
@@ -465,7 +504,8 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
         int li, lj, lk, g0, g1, g2;
         rb_slot<R1J, R1K2>(min(e, N1 - 1), HI, H, H, b0, b1, b2, c1, li, lj, lk, g0, g1, g2);
         const int le = (li * SJ + lj) * SK + lk;
-        const float rv = SUB_MEAN ? __fsub_rn(rv1[k], mean) : rv1[k];
+        const float rraw = SUB_MEAN ? __fsub_rn(rv1[k], mean) : rv1[k];
+        const float rv = kScale ? rraw * vs : rraw;
         Diag dg = diag(L, interior1 && !L.dg, n_int, y_int, g0, g1, g2);
         bool fluid = true;
         if (L.dg) {
@@ -476,7 +516,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
         const float v = pfm_div_small(__fadd_rn(rv, nsum_lds<DIM>(zs + le, SP, SK)), dg.n, dg.y);
         const bool in = e < N1 && inside(L, g0, g1, g2) && fluid;
         if (in) zs[le] = v;
-        if (DOT && in) s += (double)v * (double)rv;
+        if (DOT && in) s += (double)to_f(from_f<ZT>(v)) * (double)rraw;
     }
     __syncthreads();
     constexpr int NT = T::I * T::J * T::K;
@@ -485,12 +525,12 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
         const int tk = e % T::K, tj = (e / T::K) % T::J, ti = e / (T::K * T::J);
         const int g0 = o0 + ti, g1 = o1 + tj, g2 = o2 + tk;
         if (!inside(L, g0, g1, g2)) continue;
-        zout[(g0 * L.d[1] + g1) * L.d[2] + g2] = zs[((ti + HI) * SJ + tj + H) * SK + tk + H];
+        zout[(g0 * L.d[1] + g1) * L.d[2] + g2] = from_f<ZT>(zs[((ti + HI) * SJ + tj + H) * SK + tk + H]);
     }
     if (DOT) {
         __shared__ double sh[kBlock / 64];
         s = block_sum(s, sh);
-        if (tid == 0) partial[blockIdx.x] = s;
+        if (tid == 0) partial[blockIdx.x] = kScale ? s * (double)sc->vinv : s;  // z was stored scaled
     }
 }
 
@@ -501,10 +541,11 @@ __global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap
 // each fine r is touched by exactly one block, so this is race free.
 // `inv_scale` is 1 / (dim - 1) (0.5 or 1), so the product equals the
 // reference's division exactly.
-template <int DIM, bool SUB_MEAN>
+// ZT: storage of z and of the coarse r; RT: storage of the fine r.
+template <int DIM, typename ZT, typename RT, bool SUB_MEAN>
 __global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc, TileMap tm, float inv_scale,
-                                                           const float* __restrict__ z, float* __restrict__ r,
-                                                           float* __restrict__ rc,
+                                                           const ZT* __restrict__ z, RT* __restrict__ r,
+                                                           ZT* __restrict__ rc,
                                                            const MGScalars* __restrict__ sc) {
     using C = RsTile<DIM>;
     constexpr int FI = DIM == 3 ? 2 * C::I : 1, FJ = 2 * C::J, FK = 2 * C::K, NF = FI * FJ * FK;
@@ -526,13 +567,13 @@ __global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc,
     for (int k = 0; k < NL; ++k) {
         const int e = tid + k * kBlock;
         const int lk = e % SK, lj = (e / SK) % SJ, li = e / SP;
-        zv[k] = ld(rz, e < NS ? cell_off(Lf, b0 + li, b1 + lj, b2 + lk) : kOOB);
+        zv[k] = ldv<ZT>(rz, e < NS ? cell_off<ZT>(Lf, b0 + li, b1 + lj, b2 + lk) : kOOB);
     }
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         const int e = tid + k * kBlock;
         const int fk = e % FK, fj = (e / FK) % FJ, fi = e / (FK * FJ);
-        rv[k] = ld(rr, cell_off(Lf, o0 + fi, o1 + fj, o2 + fk));
+        rv[k] = ldv<RT>(rr, cell_off<RT>(Lf, o0 + fi, o1 + fj, o2 + fk));
     }
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
@@ -559,9 +600,11 @@ __global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc,
         float rq = rv[k];
         if (SUB_MEAN) {
             rq = __fsub_rn(rq, mean);
-            if (in) r[(g0 * Lf.d[1] + g1) * Lf.d[2] + g2] = rq;
+            if (in) r[(g0 * Lf.d[1] + g1) * Lf.d[2] + g2] = from_f<RT>(rq);
         }
         const int le = ((fi + HI) * SJ + fj + 1) * SK + fk + 1;
+        // 16-bit z: level 0 works on vs * r (coarser r arrive scaled)
+        if constexpr (sizeof(ZT) == 2 && sizeof(RT) == 4) rq *= sc->vs;
         const float res = __fsub_rn(rq, __fsub_rn(__fmul_rn(nf, zs[le]), nsum_lds<DIM>(zs + le, SP, SK)));
         // solid children add nothing (the oracle skips them; acc starts at +0)
         ts[e] = in ? __fmul_rn(__fmul_rn(res, 1.0f), inv_scale) : 0.f;
@@ -580,7 +623,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc,
 #pragma unroll
                 for (int c = 0; c < 2; ++c)
                     acc = __fadd_rn(acc, ts[((NA * ci + a) * FJ + 2 * cj + b) * FK + 2 * ck + c]);
-        rc[(g0 * Lc.d[1] + g1) * Lc.d[2] + g2] = acc;
+        rc[(g0 * Lc.d[1] + g1) * Lc.d[2] + g2] = from_f<ZT>(acc);
     }
 }
 
@@ -588,8 +631,8 @@ __global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc,
 // p.Ap reduction: p_new = z + beta p_old on the tile and a one-cell halo,
 // then Ap from LDS.  p_new goes to a separate buffer because neighbouring
 // tiles still read p_old.
-template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const float* __restrict__ z,
+template <int DIM, typename ZT>
+__global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const ZT* __restrict__ z,
                                                      const float* __restrict__ p_old, float* __restrict__ p_new,
                                                      float* __restrict__ Ap, const MGScalars* __restrict__ sc,
                                                      double* __restrict__ partial) {
@@ -612,16 +655,19 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
     for (int k = 0; k < NL; ++k) {
         const int e = tid + k * kBlock;
         const int lk = e % SK, lj = (e / SK) % SJ, li = e / SP;
-        const uint32_t off = e < NS ? cell_off(L, b0 + li, b1 + lj, b2 + lk) : kOOB;
-        zv[k] = ld(rz, off);
-        pv[k] = ld(rp, off);
+        const bool in = e < NS && inside(L, b0 + li, b1 + lj, b2 + lk);
+        const uint32_t q = (uint32_t)(((b0 + li) * L.d[1] + b1 + lj) * L.d[2] + b2 + lk);
+        zv[k] = ldv<ZT>(rz, in ? q * (uint32_t)sizeof(ZT) : kOOB);
+        pv[k] = ldv<float>(rp, in ? q * 4u : kOOB);
     }
     const float beta = sc->beta;
+    const float zi = sizeof(ZT) == 2 ? sc->vinv : 1.0f;  // 16-bit z is stored scaled
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
         const int e = tid + k * kBlock;
         const int lk = e % SK, lj = (e / SK) % SJ, li = e / SP;
-        if (e < NS) ps[e] = inside(L, b0 + li, b1 + lj, b2 + lk) ? __fadd_rn(zv[k], __fmul_rn(beta, pv[k])) : 0.f;
+        const float zk = sizeof(ZT) == 2 ? zv[k] * zi : zv[k];
+        if (e < NS) ps[e] = inside(L, b0 + li, b1 + lj, b2 + lk) ? __fadd_rn(zk, __fmul_rn(beta, pv[k])) : 0.f;
     }
     const bool interior = box_interior(L, o0, o1, o2, o0 + T::I, o1 + T::J, o2 + T::K, DIM == 3);
     const float n_int = (float)L.nb2;
@@ -664,14 +710,16 @@ constexpr int kCoarseMaxLevels = 4;
 constexpr int kCoarseLds = 14336;  // floats (56 KiB)
 constexpr int kCoarseSlots = 4;    // bottom-level cells per colour per thread
 
+constexpr int kCoarseBottom64 = 4096;  // padded bottom cells of the FP64 bottom solve
+
 struct CoarseArgs {
     int nl;         // levels handled; the last one is the bottom
     int sweeps;     // bottom red/black iterations
     float inv_scale;
     int zoff[kCoarseMaxLevels], roff[kCoarseMaxLevels];
     LvlDev L[kCoarseMaxLevels];
-    float* z[kCoarseMaxLevels];
-    float* r[kCoarseMaxLevels];
+    void* z[kCoarseMaxLevels];  // ZT arrays
+    void* r[kCoarseMaxLevels];  // R0T on the first level, ZT below
 };
 
 template <int DIM>
@@ -757,7 +805,10 @@ __device__ void coarse_prolong(const LvlDev& Lf, const LvlDev& Lc, float* zs, co
     }
 }
 
-template <int DIM>
+// ZT: storage of z and of the coarse r in HBM; R0T: storage of r on the
+// first level handled; BT: arithmetic of the bottom solve (float, or double
+// for the mixed-precision solver).
+template <int DIM, typename ZT, typename R0T, typename BT>
 __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, int dot,
                                                    const MGScalars* __restrict__ sc, double* __restrict__ partial) {
     __shared__ float sm[kCoarseLds];
@@ -768,13 +819,16 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
         const LvlDev& L0 = a.L[0];
         const int n0 = L0.d[0] * L0.d[1] * L0.d[2];
         const float mean = sub_mean ? sc->mean : 0.f;
+        R0T* r0 = (R0T*)a.r[0];
+        // 16-bit z with an fp32 first level (level 0): work on vs * r
+        const float vs = (sizeof(ZT) == 2 && sizeof(R0T) == 4) ? sc->vs : 1.0f;
         for (int q = tid; q < n0; q += kBlock) {
-            float v = a.r[0][q];
+            float v = to_f(r0[q]);
             if (sub_mean && !(L0.dg && L0.dg[q] == 0)) {
                 v = __fsub_rn(v, mean);
-                a.r[0][q] = v;
+                r0[q] = from_f<R0T>(v);
             }
-            sm[a.roff[0] + q] = v;
+            sm[a.roff[0] + q] = (sizeof(ZT) == 2 && sizeof(R0T) == 4) ? v * vs : v;
         }
     }
     __syncthreads();
@@ -817,16 +871,40 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
                     }
                 }
             }
-        for (int it = 0; it < a.sweeps; ++it)
+        if constexpr (sizeof(BT) == 4) {
+            for (int it = 0; it < a.sweeps; ++it)
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
+                for (int c = 0; c < 2; ++c) {
 #pragma unroll
-                for (int m = 0; m < kCoarseSlots; ++m)
-                    if (bl[c][m] >= 0)
-                        zs[bl[c][m]] = pfm_div_small(__fadd_rn(br[c][m], nsum_lds<DIM>(zs + bl[c][m], P.s0, P.s1)),
-                                                     bn[c][m], by[c][m]);
-                __syncthreads();
-            }
+                    for (int m = 0; m < kCoarseSlots; ++m)
+                        if (bl[c][m] >= 0)
+                            zs[bl[c][m]] = pfm_div_small(
+                                __fadd_rn(br[c][m], nsum_lds<DIM>(zs + bl[c][m], P.s0, P.s1)), bn[c][m], by[c][m]);
+                    __syncthreads();
+                }
+        } else {
+            // the coarsest-level solve in double, rounded to fp32 once
+            __shared__ double bz[kCoarseBottom64];
+            const int np = padded_size<DIM>(L.d);
+            for (int e = tid; e < np; e += kBlock) bz[e] = 0.0;
+            __syncthreads();
+            for (int it = 0; it < a.sweeps; ++it)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                    for (int m = 0; m < kCoarseSlots; ++m)
+                        if (bl[c][m] >= 0) {
+                            const double* x = bz + bl[c][m];
+                            double ns = 0.0;
+                            if (DIM == 3) ns = (ns + x[P.s0]) + x[-P.s0];
+                            ns = (((ns + x[P.s1]) + x[-P.s1]) + x[1]) + x[-1];
+                            bz[bl[c][m]] = ((double)br[c][m] + ns) / (double)bn[c][m];
+                        }
+                    __syncthreads();
+                }
+            for (int e = tid; e < np; e += kBlock) zs[e] = (float)bz[e];
+            __syncthreads();
+        }
     }
     for (int l = a.nl - 2; l >= 0; --l) {
         coarse_prolong<DIM>(a.L[l], a.L[l + 1], sm + a.zoff[l], sm + a.zoff[l + 1]);
@@ -845,18 +923,19 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
         for (int row = tid; row < L.d[0] * L.d[1]; row += kBlock) {
             const int i = row / L.d[1], j = row - i * L.d[1];
             for (int k = 0; k < L.d[2]; ++k) {
-                const float zq = sm[a.zoff[l] + P.at(i, j, k)];
+                const ZT zq = from_f<ZT>(sm[a.zoff[l] + P.at(i, j, k)]);
                 const int q = row * L.d[2] + k;
-                a.z[l][q] = zq;
-                if (l > 0) a.r[l][q] = sm[a.roff[l] + q];
-                if (dot && l == 0) s += (double)zq * (double)sm[a.roff[0] + q];
+                ((ZT*)a.z[l])[q] = zq;
+                if (l > 0) ((ZT*)a.r[l])[q] = from_f<ZT>(sm[a.roff[l] + q]);
+                if (dot && l == 0) s += (double)to_f(zq) * (double)sm[a.roff[0] + q];
             }
         }
     }
     if (dot) {
         __shared__ double sh[kBlock / 64];
         s = block_sum(s, sh);
-        if (tid == 0) partial[0] = s;
+        // both factors carry vs when the first level is scaled
+        if (tid == 0) partial[0] = (sizeof(ZT) == 2 && sizeof(R0T) == 4) ? s * (double)sc->vinv * sc->vinv : s;
     }
 }
 
@@ -868,8 +947,9 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
 constexpr int kBottomMax = 8192;
 constexpr int kBottomThreads = 1024;
 constexpr int kBottomPer = kBottomMax / kBottomThreads;
-__global__ void __launch_bounds__(kBottomThreads) k_bottom(LvlDev L, int n, float* __restrict__ z,
-                                                           const float* __restrict__ r, int sweeps) {
+template <typename ZT>
+__global__ void __launch_bounds__(kBottomThreads) k_bottom(LvlDev L, int n, ZT* __restrict__ z,
+                                                           const ZT* __restrict__ r, int sweeps) {
     __shared__ float zs[kBottomMax];
     int qi[kBottomPer], col[kBottomPer];
     int nb[kBottomPer][6];
@@ -883,7 +963,7 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom(LvlDev L, int n, floa
             int I[3];
             cell_ijk(L, q, I);
             col[k] = solid_at(L, q) ? -1 : (I[0] + I[1] + I[2]) & 1;
-            rv[k] = r[q];
+            rv[k] = to_f(r[q]);
             mv[k] = nfn(L, I);
             const int st[3] = {L.d[1] * L.d[2], L.d[2], 1};
 #pragma unroll
@@ -912,21 +992,23 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom(LvlDev L, int n, floa
     }
 #pragma unroll
     for (int k = 0; k < kBottomPer; ++k)
-        if (qi[k] < n) z[qi[k]] = zs[qi[k]];
+        if (qi[k] < n) z[qi[k]] = from_f<ZT>(zs[qi[k]]);
 }
 
 // ---- vector kernels of the CG recurrence ----------------------------------
 
 // Partial sums of a.b (and, if b == nullptr, of a) in double.
-__global__ void __launch_bounds__(kBlock) k_dot(int64_t n, const float* __restrict__ a,
-                                                const float* __restrict__ b, double* __restrict__ partial) {
+template <typename AT>
+__global__ void __launch_bounds__(kBlock) k_dot(int64_t n, const AT* __restrict__ a,
+                                                const float* __restrict__ b, double* __restrict__ partial,
+                                                const float* __restrict__ ascale) {
     __shared__ double sh[kBlock / 64];
     double s = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x)
-        s += b ? (double)a[q] * (double)b[q] : (double)a[q];
+        s += b ? (double)to_f(a[q]) * (double)b[q] : (double)to_f(a[q]);
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (threadIdx.x == 0) partial[blockIdx.x] = ascale ? s * (double)*ascale : s;
 }
 
 // update_x + update_r (3D/mgpcg.py:137-145) fused with the r.r and sum(r)
@@ -956,13 +1038,14 @@ __global__ void __launch_bounds__(kBlock) k_update_xr(int64_t n, float* __restri
 }
 
 // update_p: 3D/mgpcg.py:147-150 (in place; used where no Ap follows).
+template <typename ZT>
 __global__ void __launch_bounds__(kBlock) k_update_p(int64_t n, float* __restrict__ p,
-                                                     const float* __restrict__ z,
+                                                     const ZT* __restrict__ z,
                                                      const MGScalars* __restrict__ sc) {
     const float beta = sc->beta;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x)
-        p[q] = __fadd_rn(z[q], __fmul_rn(beta, p[q]));
+        p[q] = __fadd_rn(sizeof(ZT) == 2 ? to_f(z[q]) * sc->vinv : to_f(z[q]), __fmul_rn(beta, p[q]));
 }
 
 __global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t n, float* __restrict__ r,
@@ -972,6 +1055,18 @@ __global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t n, float* __restric
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x)
         if (!dg || dg[q] != 0) r[q] = __fsub_rn(r[q], m);  // solid cells stay 0
+}
+
+// vs = 2^round(-log2(rms r)) from r.r over n cells (1 when r vanishes).
+__device__ __forceinline__ void set_vscale(MGScalars* sc, double rtr, double n) {
+    if (rtr > 0.0 && n > 0.0) {
+        const double e = rint(-0.5 * log2(rtr / n));
+        const double c = fmin(fmax(e, -100.0), 100.0);
+        sc->vs = (float)exp2(c);
+        sc->vinv = (float)exp2(-c);
+    } else {
+        sc->vs = sc->vinv = 1.0f;
+    }
 }
 
 enum FinalizeMode { kFinMean = 0, kFinR0 = 1, kFinZTr = 2, kFinAlpha = 3, kFinRTr = 4, kFinBeta = 5 };
@@ -998,6 +1093,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ 
         case kFinR0:
             sc->r0 = s;
             sc->rTr = s;
+            set_vscale(sc, s, ncell);
             break;
         case kFinZTr:
             sc->zTr = s;
@@ -1010,6 +1106,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ 
             sc->rTr = s;
             sc->sum_r = s1;
             sc->mean = (float)(s1 / ncell);
+            set_vscale(sc, s, ncell);
             break;
         case kFinBeta: {
             sc->new_zTr = s;
@@ -1024,7 +1121,9 @@ __global__ void k_reset_scalars(MGScalars* sc) {
     float beta = sc->beta;  // the reference's beta field persists across solves
     *sc = MGScalars{};
     sc->beta = beta;
+    sc->vs = sc->vinv = 1.0f;
 }
+
 
 // ---- field kernels (velocity <-> level 0) ---------------------------------
 
@@ -1059,7 +1158,7 @@ __global__ void __launch_bounds__(kBlock) k_div_init(LvlDev L, const float* __re
         div = __fsub_rn(__fadd_rn(__fsub_rn(vr, vl), vt), vb);
     }
     r[q] = solid_at(L, q) ? 0.f : __fmul_rn(div, k);
-    z[q] = 0.f;
+    if (z) z[q] = 0.f;
     Ap[q] = 0.f;
     p[q] = 0.f;
     x[q] = 0.f;
@@ -1072,7 +1171,7 @@ __global__ void __launch_bounds__(kBlock) k_init_rhs(int64_t n, const float* __r
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     r[q] = (dg && dg[q] == 0) ? 0.f : __fmul_rn(rhs[q], k);
-    z[q] = 0.f;
+    if (z) z[q] = 0.f;
     Ap[q] = 0.f;
     p[q] = 0.f;
     x[q] = 0.f;
@@ -1108,6 +1207,16 @@ __global__ void __launch_bounds__(kBlock) k_zero_plane(int S0, int S1, int S2, i
     I[a1] = (int)(t / S[a2]);
     I[a2] = (int)(t % S[a2]);
     u[((int64_t)I[0] * S[1] + I[1]) * S[2] + I[2]] = 0.0f;
+}
+
+// r = b - A x: the true-residual replacement of the mixed-precision PCG.
+__global__ void __launch_bounds__(kBlock) k_true_residual(LvlDev L, const float* __restrict__ b,
+                                                          const float* __restrict__ x, float* __restrict__ r) {
+    int I[3];
+    if (!cell_of_thread(L, 0, false, I)) return;
+    const int64_t q = cell_index(L, I);
+    r[q] = solid_at(L, q) ? 0.0f
+                          : __fsub_rn(b[q], __fsub_rn(__fmul_rn(nfn(L, I), x[q]), nsum(L, x, q, I)));
 }
 
 // ---- solid obstacle (extension of the reference solver) ----------------------
@@ -1241,24 +1350,28 @@ static int fill(float* p, int64_t n, float v, hipStream_t s) {
 
 static const dim3 kCellBlock(kTx, kTy, 1);
 
+// The per-pass entry points (smooth, restrict, prolongate one level) work on
+// fp32 level arrays, i.e. precision 0.
 static int smooth(MGHandle* h, int l, int phase, hipStream_t s) {
     LvlDev L = lvl_dev(h, l);
-    hipLaunchKernelGGL(k_smooth, cell_grid(L.d, true), kCellBlock, 0, s, L, h->z[l], h->r[l], phase);
+    hipLaunchKernelGGL(k_smooth, cell_grid(L.d, true), kCellBlock, 0, s, L, (float*)h->z[l], (const float*)h->r[l],
+                       phase);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
 
 static int restrict_level(MGHandle* h, int l, hipStream_t s) {
     LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
-    hipLaunchKernelGGL(k_restrict, cell_grid(Lc.d, false), kCellBlock, 0, s, Lf, Lc, restrict_scale(h), h->z[l],
-                       h->r[l], h->r[l + 1]);
+    hipLaunchKernelGGL(k_restrict, cell_grid(Lc.d, false), kCellBlock, 0, s, Lf, Lc, restrict_scale(h),
+                       (const float*)h->z[l], (const float*)h->r[l], (float*)h->r[l + 1]);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
 
 static int prolongate_level(MGHandle* h, int l, hipStream_t s) {
     LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
-    hipLaunchKernelGGL(k_prolongate, cell_grid(Lf.d, false), kCellBlock, 0, s, Lf, Lc, h->z[l], h->z[l + 1]);
+    hipLaunchKernelGGL(k_prolongate, cell_grid(Lf.d, false), kCellBlock, 0, s, Lf, Lc, (float*)h->z[l],
+                       (const float*)h->z[l + 1]);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -1271,12 +1384,13 @@ static int prolongate_level(MGHandle* h, int l, hipStream_t s) {
 
 enum SweepKind { kSweepPlain = 0, kSweepZero = 1, kSweepProlong = 2 };
 
-template <int DIM>
+template <int DIM, typename ZT, typename RT>
 static void launch_rbgs(int kind, bool sub_mean, bool dot, dim3 g, hipStream_t s, LvlDev L, LvlDev Lc, TileMap tm,
-                        const float* zin, const float* zc, const float* r, float* zout, int c0,
-                        const MGScalars* sc, double* partial) {
-#define PFM_RBGS(Z, P, M, D) \
-    hipLaunchKernelGGL((k_rbgs<DIM, Z, P, M, D>), g, dim3(kBlock), 0, s, L, Lc, tm, zin, zc, r, zout, c0, sc, partial)
+                        const ZT* zin, const ZT* zc, const RT* r, ZT* zout, int c0, const MGScalars* sc,
+                        double* partial) {
+#define PFM_RBGS(Z, P, M, D)                                                                                     \
+    hipLaunchKernelGGL((k_rbgs<DIM, ZT, RT, Z, P, M, D>), g, dim3(kBlock), 0, s, L, Lc, tm, zin, zc, r, zout, c0, \
+                       sc, partial)
     if (kind == kSweepZero) {
         if (sub_mean) PFM_RBGS(true, false, true, false);
         else PFM_RBGS(true, false, false, false);
@@ -1292,28 +1406,42 @@ static void launch_rbgs(int kind, bool sub_mean, bool dot, dim3 g, hipStream_t s
 #undef PFM_RBGS
 }
 
-// Two colour passes (c0, c0 ^ 1) on level l, z_in -> z_out.
-static int rbgs(MGHandle* h, int l, int kind, int c0, const float* zin, float* zout, bool sub_mean, bool dot,
+// Two colour passes (c0, c0 ^ 1) on level l, z_in -> z_out (ZT arrays).
+template <typename ZT>
+static int rbgs(MGHandle* h, int l, int kind, int c0, const ZT* zin, ZT* zout, bool sub_mean, bool dot,
                 hipStream_t s) {
     const LvlDev L = lvl_dev(h, l);
     const LvlDev Lc = kind == kSweepProlong ? lvl_dev(h, l + 1) : L;
-    const float* zc = kind == kSweepProlong ? h->z[l + 1] : nullptr;
+    const ZT* zc = kind == kSweepProlong ? (const ZT*)h->z[l + 1] : nullptr;
     const TileMap tm = sweep_tiles(h, l);
     const dim3 g((unsigned)tm.total);
-    if (h->dim == 3)
-        launch_rbgs<3>(kind, sub_mean, dot, g, s, L, Lc, tm, zin, zc, h->r[l], zout, c0, h->sc, h->partial);
-    else
-        launch_rbgs<2>(kind, sub_mean, dot, g, s, L, Lc, tm, zin, zc, h->r[l], zout, c0, h->sc, h->partial);
+    // r is fp32 on level 0 (the PCG residual) and ZT below
+    if (l == 0 || std::is_same<ZT, float>::value) {
+        const float* r = (const float*)h->r[l];
+        if (h->dim == 3)
+            launch_rbgs<3, ZT, float>(kind, sub_mean, dot, g, s, L, Lc, tm, zin, zc, r, zout, c0, h->sc, h->partial);
+        else
+            launch_rbgs<2, ZT, float>(kind, sub_mean, dot, g, s, L, Lc, tm, zin, zc, r, zout, c0, h->sc, h->partial);
+    } else {
+        const ZT* r = (const ZT*)h->r[l];
+        if (h->dim == 3)
+            launch_rbgs<3, ZT, ZT>(kind, false, false, g, s, L, Lc, tm, zin, zc, r, zout, c0, h->sc, h->partial);
+        else
+            launch_rbgs<2, ZT, ZT>(kind, false, false, g, s, L, Lc, tm, zin, zc, r, zout, c0, h->sc, h->partial);
+    }
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
 
 // Residual + restriction of level l into r[l + 1] (tiled when the extents
-// halve exactly).  With sub_mean, r[l] is recentred first (written back).
+// halve exactly; otherwise, at precision 0 only, the per-cell kernel).  With
+// sub_mean, r[l] is recentred first (written back).
+template <typename ZT>
 static int restrict_fused(MGHandle* h, int l, bool sub_mean, hipStream_t s) {
     if (!restrict_tiled_ok(h, l)) {
         if (sub_mean) {
-            hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[l].n, h->r[l], h->sc, h->dg[l]);
+            hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[l].n, (float*)h->r[l],
+                               h->sc, h->dg[l]);
             PFM_LAUNCH_CHECK();
         }
         return restrict_level(h, l, s);
@@ -1321,39 +1449,43 @@ static int restrict_fused(MGHandle* h, int l, bool sub_mean, hipStream_t s) {
     const LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
     const TileMap tm = restrict_tiles(h, l);
     const dim3 g((unsigned)tm.total);
-    const float sc = restrict_inv_scale(h);
-    if (h->dim == 3) {
-        if (sub_mean)
-            hipLaunchKernelGGL((k_restrict_tiled<3, true>), g, dim3(kBlock), 0, s, Lf, Lc, tm, sc, h->z[l], h->r[l],
-                               h->r[l + 1], h->sc);
-        else
-            hipLaunchKernelGGL((k_restrict_tiled<3, false>), g, dim3(kBlock), 0, s, Lf, Lc, tm, sc, h->z[l],
-                               h->r[l], h->r[l + 1], h->sc);
+    const float isc = restrict_inv_scale(h);
+    const ZT* z = (const ZT*)h->z[l];
+    ZT* rc = (ZT*)h->r[l + 1];
+#define PFM_RS(D, RT, M) \
+    hipLaunchKernelGGL((k_restrict_tiled<D, ZT, RT, M>), g, dim3(kBlock), 0, s, Lf, Lc, tm, isc, z, (RT*)h->r[l], rc, h->sc)
+    if (l == 0 || std::is_same<ZT, float>::value) {
+        if (h->dim == 3) {
+            if (sub_mean) PFM_RS(3, float, true);
+            else PFM_RS(3, float, false);
+        } else {
+            if (sub_mean) PFM_RS(2, float, true);
+            else PFM_RS(2, float, false);
+        }
     } else {
-        if (sub_mean)
-            hipLaunchKernelGGL((k_restrict_tiled<2, true>), g, dim3(kBlock), 0, s, Lf, Lc, tm, sc, h->z[l], h->r[l],
-                               h->r[l + 1], h->sc);
-        else
-            hipLaunchKernelGGL((k_restrict_tiled<2, false>), g, dim3(kBlock), 0, s, Lf, Lc, tm, sc, h->z[l],
-                               h->r[l], h->r[l + 1], h->sc);
+        if (h->dim == 3) PFM_RS(3, ZT, false);
+        else PFM_RS(2, ZT, false);
     }
+#undef PFM_RS
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
 
 // 50 red/black iterations from z = 0 on the coarsest level (3D/mgpcg.py:190-194).
+template <typename ZT>
 static int bottom(MGHandle* h, int b, hipStream_t s) {
-    if (h->lv[b].n <= kBottomMax) {
+    if (h->lv[b].n <= kBottomMax && (b > 0 || std::is_same<ZT, float>::value)) {
         const LvlDev Lb = lvl_dev(h, b);
-        hipLaunchKernelGGL(k_bottom, dim3(1), dim3(kBottomThreads), 0, s, Lb, (int)h->lv[b].n, h->z[b], h->r[b], 50);
+        hipLaunchKernelGGL(k_bottom<ZT>, dim3(1), dim3(kBottomThreads), 0, s, Lb, (int)h->lv[b].n, (ZT*)h->z[b],
+                           (const ZT*)h->r[b], 50);
         PFM_LAUNCH_CHECK();
         return PFM_OK;
     }
     // one launch per red/black iteration, alternating buffers so the last lands in z[b]
-    const float* zin = nullptr;
+    const ZT* zin = nullptr;
     for (int j = 0; j < 50; ++j) {
-        float* zout = ((49 - j) & 1) ? h->z2[b] : h->z[b];
-        PFM_TRY(rbgs(h, b, j == 0 ? kSweepZero : kSweepPlain, 0, zin, zout, false, false, s));
+        ZT* zout = (ZT*)(((49 - j) & 1) ? h->z2[b] : h->z[b]);
+        PFM_TRY(rbgs<ZT>(h, b, j == 0 ? kSweepZero : kSweepPlain, 0, zin, zout, false, false, s));
         zin = zout;
     }
     return PFM_OK;
@@ -1365,6 +1497,8 @@ static int pick_coarse_level(const MGHandle* h) {
     const int L = h->n_levels;
     const int* db = h->lv[L - 1].d;
     if ((int64_t)db[0] * db[1] * ((db[2] + 1) / 2) > (int64_t)kCoarseSlots * kBlock) return -1;
+    const int pb = h->dim == 3 ? padded_size<3>(db) : padded_size<2>(db);
+    if (h->precision != 0 && pb > kCoarseBottom64) return -1;
     int best = -1;
     int64_t used = 0;
     for (int l = L - 1; l >= 0 && L - l <= kCoarseMaxLevels; --l) {
@@ -1375,6 +1509,7 @@ static int pick_coarse_level(const MGHandle* h) {
     return best;
 }
 
+template <typename ZT>
 static int coarse(MGHandle* h, bool sub_mean, bool dot, hipStream_t s) {
     CoarseArgs a{};
     const int lc = h->coarse_level;
@@ -1392,10 +1527,22 @@ static int coarse(MGHandle* h, bool sub_mean, bool dot, hipStream_t s) {
         a.z[i] = h->z[l];
         a.r[i] = h->r[l];
     }
-    if (h->dim == 3)
-        hipLaunchKernelGGL(k_coarse<3>, dim3(1), dim3(kBlock), 0, s, a, (int)sub_mean, (int)dot, h->sc, h->partial);
-    else
-        hipLaunchKernelGGL(k_coarse<2>, dim3(1), dim3(kBlock), 0, s, a, (int)sub_mean, (int)dot, h->sc, h->partial);
+#define PFM_CO(D, R0T, BT) \
+    hipLaunchKernelGGL((k_coarse<D, ZT, R0T, BT>), dim3(1), dim3(kBlock), 0, s, a, (int)sub_mean, (int)dot, h->sc, h->partial)
+    if constexpr (std::is_same<ZT, float>::value) {
+        if (h->dim == 3) PFM_CO(3, float, float);
+        else PFM_CO(2, float, float);
+    } else {
+        // 16-bit levels: the bottom solve runs in double
+        if (lc == 0) {
+            if (h->dim == 3) PFM_CO(3, float, double);
+            else PFM_CO(2, float, double);
+        } else {
+            if (h->dim == 3) PFM_CO(3, ZT, double);
+            else PFM_CO(2, ZT, double);
+        }
+    }
+#undef PFM_CO
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -1406,7 +1553,8 @@ static int coarse(MGHandle* h, bool sub_mean, bool dot, hipStream_t s) {
 // recentred by the V-cycle's own launches (the recentring of solve,
 // :174-180); with dot_parts, the last level-0 launch leaves partials of z.r
 // in `partial` and their count in *dot_parts (0 when no launch did).
-static int vcycle(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
+template <typename ZT>
+static int vcycle_t(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
     const int L = h->n_levels;
     const int lc = h->coarse_level;
     const int top = lc >= 0 ? lc : L - 1;  // levels [0, top) run as tiled launches
@@ -1415,28 +1563,37 @@ static int vcycle(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
     const bool tiled_mean = sub_mean && top > 0 && restrict_tiled_ok(h, 0);
     const bool coarse_mean = sub_mean && lc == 0;
     if (sub_mean && !tiled_mean && !coarse_mean) {
-        hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[0].n, h->r[0], h->sc, h->dg[0]);
+        hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[0].n, (float*)h->r[0], h->sc,
+                           h->dg[0]);
         PFM_LAUNCH_CHECK();
     }
     for (int l = 0; l < top; ++l) {
         const bool sm = tiled_mean && l == 0;
-        PFM_TRY(rbgs(h, l, kSweepZero, 0, nullptr, h->z2[l], sm, false, s));
-        PFM_TRY(rbgs(h, l, kSweepPlain, 0, h->z2[l], h->z[l], sm, false, s));
-        PFM_TRY(restrict_fused(h, l, sm, s));
+        PFM_TRY(rbgs<ZT>(h, l, kSweepZero, 0, nullptr, (ZT*)h->z2[l], sm, false, s));
+        PFM_TRY(rbgs<ZT>(h, l, kSweepPlain, 0, (const ZT*)h->z2[l], (ZT*)h->z[l], sm, false, s));
+        PFM_TRY(restrict_fused<ZT>(h, l, sm, s));
     }
     if (lc >= 0) {
-        PFM_TRY(coarse(h, coarse_mean, want_dot && lc == 0, s));
+        PFM_TRY(coarse<ZT>(h, coarse_mean, want_dot && lc == 0, s));
         if (want_dot && lc == 0) *dot_parts = 1;
     } else {
-        PFM_TRY(bottom(h, L - 1, s));
+        PFM_TRY(bottom<ZT>(h, L - 1, s));
     }
     for (int l = top - 1; l >= 0; --l) {
         const bool dot = want_dot && l == 0;
-        PFM_TRY(rbgs(h, l, kSweepProlong, 1, h->z[l], h->z2[l], false, false, s));
-        PFM_TRY(rbgs(h, l, kSweepPlain, 1, h->z2[l], h->z[l], false, dot, s));
+        PFM_TRY(rbgs<ZT>(h, l, kSweepProlong, 1, (const ZT*)h->z[l], (ZT*)h->z2[l], false, false, s));
+        PFM_TRY(rbgs<ZT>(h, l, kSweepPlain, 1, (const ZT*)h->z2[l], (ZT*)h->z[l], false, dot, s));
         if (dot) *dot_parts = sweep_tiles(h, 0).total;
     }
     return PFM_OK;
+}
+
+static int vcycle(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
+    switch (h->precision) {
+        case 1: return vcycle_t<__half>(h, s, sub_mean, dot_parts);
+        case 2: return vcycle_t<__hip_bfloat16>(h, s, sub_mean, dot_parts);
+        default: return vcycle_t<float>(h, s, sub_mean, dot_parts);
+    }
 }
 
 static int finalize(MGHandle* h, int mode, double eps, hipStream_t s, int nparts = kMgRedBlocks) {
@@ -1446,10 +1603,23 @@ static int finalize(MGHandle* h, int mode, double eps, hipStream_t s, int nparts
     return PFM_OK;
 }
 
-static int dot(MGHandle* h, const float* a, const float* b, hipStream_t s) {
-    hipLaunchKernelGGL(k_dot, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[0].n, a, b, h->partial);
+template <typename AT>
+static int dot_t(MGHandle* h, const AT* a, const float* b, hipStream_t s, double* out,
+                 const float* ascale = nullptr) {
+    hipLaunchKernelGGL(k_dot<AT>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[0].n, a, b, out, ascale);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
+}
+
+static int dot(MGHandle* h, const float* a, const float* b, hipStream_t s) { return dot_t(h, a, b, s, h->partial); }
+
+// z[0] . r[0] for whatever z[0] stores
+static int dot_zr(MGHandle* h, hipStream_t s) {
+    switch (h->precision) {
+        case 1: return dot_t(h, (const __half*)h->z[0], (const float*)h->r[0], s, h->partial, &h->sc->vinv);
+        case 2: return dot_t(h, (const __hip_bfloat16*)h->z[0], (const float*)h->r[0], s, h->partial, &h->sc->vinv);
+        default: return dot_t(h, (const float*)h->z[0], (const float*)h->r[0], s, h->partial);
+    }
 }
 
 // p <- z + beta p (into the partner buffer), Ap and p.Ap partials; returns
@@ -1458,15 +1628,44 @@ static int ap_dot_p(MGHandle* h, hipStream_t s, int* parts) {
     const LvlDev L0 = lvl_dev(h, 0);
     const TileMap tm = sweep_tiles(h, 0);
     const dim3 g((unsigned)tm.total);
-    if (h->dim == 3)
-        hipLaunchKernelGGL(k_ap_dot_p<3>, g, dim3(kBlock), 0, s, L0, tm, h->z[0], h->p, h->p2, h->Ap, h->sc,
-                           h->partial);
-    else
-        hipLaunchKernelGGL(k_ap_dot_p<2>, g, dim3(kBlock), 0, s, L0, tm, h->z[0], h->p, h->p2, h->Ap, h->sc,
-                           h->partial);
+#define PFM_AP(D, ZT) \
+    hipLaunchKernelGGL((k_ap_dot_p<D, ZT>), g, dim3(kBlock), 0, s, L0, tm, (const ZT*)h->z[0], h->p, h->p2, h->Ap, h->sc, h->partial)
+    switch (h->precision) {
+        case 1:
+            if (h->dim == 3) PFM_AP(3, __half);
+            else PFM_AP(2, __half);
+            break;
+        case 2:
+            if (h->dim == 3) PFM_AP(3, __hip_bfloat16);
+            else PFM_AP(2, __hip_bfloat16);
+            break;
+        default:
+            if (h->dim == 3) PFM_AP(3, float);
+            else PFM_AP(2, float);
+    }
+#undef PFM_AP
     PFM_LAUNCH_CHECK();
     std::swap(h->p, h->p2);
     *parts = tm.total;
+    return PFM_OK;
+}
+
+static int update_p(MGHandle* h, hipStream_t s) {
+    const int64_t n = h->lv[0].n;
+    switch (h->precision) {
+        case 1:
+            hipLaunchKernelGGL(k_update_p<__half>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->p,
+                               (const __half*)h->z[0], h->sc);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_update_p<__hip_bfloat16>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->p,
+                               (const __hip_bfloat16*)h->z[0], h->sc);
+            break;
+        default:
+            hipLaunchKernelGGL(k_update_p<float>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->p,
+                               (const float*)h->z[0], h->sc);
+    }
+    PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
 
@@ -1480,36 +1679,48 @@ static bool all_neumann(const MGHandle* h) {
 static int precondition_dot(MGHandle* h, bool sub_mean, hipStream_t s, int* parts) {
     PFM_TRY(vcycle(h, s, sub_mean, parts));
     if (*parts == 0) {
-        PFM_TRY(dot(h, h->z[0], h->r[0], s));
+        PFM_TRY(dot_zr(h, s));
         *parts = kMgRedBlocks;
     }
     return PFM_OK;
 }
 
+// Iterations between true-residual replacements (r = b - A x) in the
+// mixed-precision solver.
+constexpr int kReplaceEvery = 8;
+
 // solve: 3D/mgpcg.py:202-280.  The reference's update_p after each
 // preconditioner application is carried out at the start of the next Ap
 // launch (same arithmetic, same beta), or explicitly when the loop ends
-// without converging.
+// without converging.  At precision 1/2 (16-bit V-cycle storage) the PCG
+// residual is replaced by the true residual b - A x every kReplaceEvery
+// iterations.
 static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mode, int* iters_out,
                  double* rtr_out, hipStream_t s) {
     const int64_t n = h->lv[0].n;
+    float* r0 = (float*)h->r[0];
     const bool neumann = all_neumann(h);
     hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(1), 0, s, h->sc);
     PFM_LAUNCH_CHECK();
     if (neumann) {
-        PFM_TRY(dot(h, h->r[0], nullptr, s));
+        PFM_TRY(dot(h, r0, nullptr, s));
         PFM_TRY(finalize(h, kFinMean, eps, s));
-        hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->r[0], h->sc, h->dg[0]);
+        hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, r0, h->sc, h->dg[0]);
         PFM_LAUNCH_CHECK();
     }
+    const bool replace = h->precision != 0;
+    if (replace) PFM_HIP_TRY(hipMemcpyAsync(h->b, r0, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     double thresh = tol;
-    if (tol_mode == 1) {
-        PFM_TRY(dot(h, h->r[0], h->r[0], s));
+    if (tol_mode == 1 || replace) {  // r0 (and the V-cycle scale of 16-bit storage)
+        PFM_TRY(dot(h, r0, r0, s));
         PFM_TRY(finalize(h, kFinR0, eps, s));
+    }
+    if (tol_mode == 1) {
         PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr, &h->sc->r0, sizeof(double), hipMemcpyDeviceToHost, s));
         PFM_HIP_TRY(hipStreamSynchronize(s));
         thresh = tol * tol * h->host_rtr[0];
     }
+    const LvlDev L0 = lvl_dev(h, 0);
     int parts = 0;
     PFM_TRY(precondition_dot(h, false, s, &parts));
     PFM_TRY(finalize(h, kFinZTr, eps, s, parts));
@@ -1519,9 +1730,15 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
     while (max_iters == -1 || it < max_iters) {
         PFM_TRY(ap_dot_p(h, s, &parts));
         PFM_TRY(finalize(h, kFinAlpha, eps, s, parts));
-        hipLaunchKernelGGL(k_update_xr, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, h->r[0], h->p, h->Ap,
-                           h->sc, h->partial);
+        hipLaunchKernelGGL(k_update_xr, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, r0, h->p, h->Ap, h->sc,
+                           h->partial);
         PFM_LAUNCH_CHECK();
+        if (replace && (it + 1) % kReplaceEvery == 0) {
+            hipLaunchKernelGGL(k_true_residual, cell_grid(L0.d, false), kCellBlock, 0, s, L0, h->b, h->x, r0);
+            PFM_LAUNCH_CHECK();
+            PFM_TRY(dot_t(h, r0, r0, s, h->partial));
+            PFM_TRY(dot_t(h, r0, (const float*)nullptr, s, h->partial + kMgRedBlocks));
+        }
         PFM_TRY(finalize(h, kFinRTr, eps, s));
         PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr, &h->sc->rTr, sizeof(double), hipMemcpyDeviceToHost, s));
         PFM_HIP_TRY(hipStreamSynchronize(s));
@@ -1534,10 +1751,7 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
         PFM_TRY(finalize(h, kFinBeta, eps, s, parts));
         ++it;
     }
-    if (!converged) {
-        hipLaunchKernelGGL(k_update_p, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->p, h->z[0], h->sc);
-        PFM_LAUNCH_CHECK();
-    }
+    if (!converged) PFM_TRY(update_p(h, s));
     if (iters_out) *iters_out = it;
     if (rtr_out) *rtr_out = rtr;
     return converged ? PFM_OK : PFM_WARN_NOT_CONVERGED;
@@ -1592,10 +1806,10 @@ static int div_init(MGHandle* h, const float* const* u, float k, hipStream_t s) 
     LvlDev L0 = lvl_dev(h, 0);
     const dim3 gd = cell_grid(L0.d, false);
     if (h->dim == 3)
-        hipLaunchKernelGGL(k_div_init<3>, gd, kCellBlock, 0, s, L0, u[0], u[1], u[2], k, h->r[0], h->z[0],
+        hipLaunchKernelGGL(k_div_init<3>, gd, kCellBlock, 0, s, L0, u[0], u[1], u[2], k, (float*)h->r[0], h->precision == 0 ? (float*)h->z[0] : nullptr,
                            h->Ap, h->p, h->x);
     else
-        hipLaunchKernelGGL(k_div_init<2>, gd, kCellBlock, 0, s, L0, u[0], u[1], u[2], k, h->r[0], h->z[0],
+        hipLaunchKernelGGL(k_div_init<2>, gd, kCellBlock, 0, s, L0, u[0], u[1], u[2], k, (float*)h->r[0], h->precision == 0 ? (float*)h->z[0] : nullptr,
                            h->Ap, h->p, h->x);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
@@ -1631,7 +1845,8 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     *handle = nullptr;
     PFM_CHECK_ARG(dim == 2 || dim == 3, "dim must be 2 or 3");
     PFM_CHECK_ARG(N && bc, "null N or bc");
-    PFM_CHECK_ARG(precision == 0, "precision %d not supported by this build", precision);
+    PFM_CHECK_ARG(precision >= 0 && precision <= 2, "precision %d: 0 = fp32, 1 = fp16, 2 = bf16 level storage",
+                  precision);
     int mn = N[0];
     for (int a = 0; a < dim; ++a) {
         PFM_CHECK_ARG(N[a] >= 2 && N[a] <= 65535, "N[%d]=%d out of range [2, 65535]", a, N[a]);
@@ -1680,12 +1895,14 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
         pfm_mg_destroy(h);
         return PFM_ERR_OOM;
     };
+    const size_t esz = precision == 0 ? sizeof(float) : 2;  // z, z2, r[l >= 1]
     for (int l = 0; l < L; ++l) {
-        const size_t b = sizeof(float) * h->lv[l].n;
-        if (hipMalloc(&h->r[l], b) != hipSuccess) return fail("r");
+        const size_t b = esz * h->lv[l].n;
+        if (hipMalloc(&h->r[l], l == 0 ? sizeof(float) * h->lv[l].n : b) != hipSuccess) return fail("r");
         if (hipMalloc(&h->z[l], b) != hipSuccess) return fail("z");
         if (hipMalloc(&h->z2[l], b) != hipSuccess) return fail("z scratch");
     }
+    if (precision != 0 && hipMalloc(&h->b, sizeof(float) * h->lv[0].n) != hipSuccess) return fail("rhs copy");
     const size_t b0 = sizeof(float) * h->lv[0].n;
     if (hipMalloc(&h->x, b0) != hipSuccess) return fail("x");
     if (hipMalloc(&h->p, b0) != hipSuccess) return fail("p");
@@ -1698,8 +1915,8 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
         return fail("pinned scalars");
     PFM_HIP_TRY(hipMemset(h->sc, 0, sizeof(MGScalars)));
     for (int l = 0; l < L; ++l) {
-        const size_t b = sizeof(float) * h->lv[l].n;
-        PFM_HIP_TRY(hipMemset(h->r[l], 0, b));
+        const size_t b = esz * h->lv[l].n;
+        PFM_HIP_TRY(hipMemset(h->r[l], 0, l == 0 ? sizeof(float) * h->lv[l].n : b));
         PFM_HIP_TRY(hipMemset(h->z[l], 0, b));
         PFM_HIP_TRY(hipMemset(h->z2[l], 0, b));
     }
@@ -1722,6 +1939,7 @@ int pfm_mg_destroy(void* hv) {
         if (h->solid[l]) (void)hipFree(h->solid[l]);
         if (h->dg[l]) (void)hipFree(h->dg[l]);
     }
+    if (h->b) (void)hipFree(h->b);
     if (h->x) (void)hipFree(h->x);
     if (h->p) (void)hipFree(h->p);
     if (h->p2) (void)hipFree(h->p2);
@@ -1799,6 +2017,8 @@ int pfm_mg_set_solid(void* hv, const uint8_t* solid, pfm_stream_t stream) {
 void* pfm_mg_field(void* hv, int which, int level) {
     MGHandle* h = (MGHandle*)hv;
     if (!h || level < 0 || level >= h->n_levels) return nullptr;
+    // 16-bit level arrays have no fp32 view
+    if (h->precision != 0 && (which == 1 || (which == 0 && level > 0))) return nullptr;
     switch (which) {
         case 0: return h->r[level];
         case 1: return h->z[level];
@@ -1829,8 +2049,8 @@ int pfm_mg_init(void* hv, const float* rhs, float k, pfm_stream_t stream) {
     MGHandle* h = (MGHandle*)hv;
     PFM_CHECK_ARG(h && rhs, "null argument");
     const int64_t n = h->lv[0].n;
-    hipLaunchKernelGGL(k_init_rhs, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n, rhs, k, h->r[0],
-                       h->z[0], h->Ap, h->p, h->x, h->dg[0]);
+    hipLaunchKernelGGL(k_init_rhs, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n, rhs, k,
+                       (float*)h->r[0], h->precision == 0 ? (float*)h->z[0] : nullptr, h->Ap, h->p, h->x, h->dg[0]);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -1876,18 +2096,21 @@ int pfm_mg_vcycle(void* hv, pfm_stream_t stream) {
 int pfm_mg_smooth(void* hv, int level, int phase, pfm_stream_t stream) {
     MGHandle* h = (MGHandle*)hv;
     PFM_CHECK_ARG(h && level >= 0 && level < h->n_levels && (phase == 0 || phase == 1), "bad arguments");
+    PFM_CHECK_ARG(h->precision == 0, "per-pass entry points need precision 0 (fp32 levels)");
     return smooth(h, level, phase, as_stream(stream));
 }
 
 int pfm_mg_restrict(void* hv, int level, pfm_stream_t stream) {
     MGHandle* h = (MGHandle*)hv;
     PFM_CHECK_ARG(h && level >= 0 && level < h->n_levels - 1, "bad level");
+    PFM_CHECK_ARG(h->precision == 0, "per-pass entry points need precision 0 (fp32 levels)");
     return restrict_level(h, level, as_stream(stream));
 }
 
 int pfm_mg_prolongate(void* hv, int level, pfm_stream_t stream) {
     MGHandle* h = (MGHandle*)hv;
     PFM_CHECK_ARG(h && level >= 0 && level < h->n_levels - 1, "bad level");
+    PFM_CHECK_ARG(h->precision == 0, "per-pass entry points need precision 0 (fp32 levels)");
     return prolongate_level(h, level, as_stream(stream));
 }
 

@@ -6,10 +6,11 @@ Reference-compatible call surfaces live in ``pfm.d3`` (modules ``run_3D``,
 the step drivers that follow the reference main loops.
 """
 from . import _lib
+from ._lib import PfmError
 from .core import face_shape, default_dx
 from .mgpcg import MGPCG, MGPCG_2, MGPCG_3
 
-__all__ = ["MGPCG", "MGPCG_2", "MGPCG_3", "face_shape", "default_dx", "load_library"]
+__all__ = ["MGPCG", "MGPCG_2", "MGPCG_3", "PfmError", "face_shape", "default_dx", "load_library"]
 
 
 def load_library(require_gpu: bool = True):

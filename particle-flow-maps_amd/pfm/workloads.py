@@ -39,3 +39,19 @@ def fourier_velocity(res, n_modes=64, kmin=1.0, kmax=8.0, seed=0, device="cuda")
             u += amp[m, a].item() * torch.sin(ph + arg)
         out.append(u.to(torch.float32).contiguous())
     return out
+
+
+def box_mask(res, lo=0.4, hi=0.6, device="cuda"):
+    """Solid cells of an axis-aligned box [lo, hi]^d in the unit-height domain
+    (dx = 1/res[1]): a cell is solid when its centre lies in the box
+    (uint8 tensor of shape res)."""
+    dx = 1.0 / res[1]
+    m = None
+    for a, n in enumerate(res):
+        c = (torch.arange(n, dtype=torch.float64, device=device) + 0.5) * dx
+        inside = (c >= lo) & (c <= hi)
+        shape = [1] * len(res)
+        shape[a] = n
+        inside = inside.reshape(shape)
+        m = inside if m is None else m & inside
+    return m.to(torch.uint8).contiguous()

@@ -199,3 +199,70 @@ def test_poisson_solid_vs_oracles(oracle, pfm, N_):
     assert np.all(pg[solid] == 0)
     e = rel_l2(pg[fl] - pg[fl].mean(), x64[fl] - x64[fl].mean())
     assert e < 1e-5, e
+
+
+# ---- mixed precision (16-bit V-cycle storage, FP32 PCG, FP64 dots) --------
+def _fp64_pressure(oracle, N_, u, solid):
+    mg_ref = oracle.MG(N_, base_level=3)
+    ub = [x.copy() for x in u]
+    mg_ref.apply_bc(ub)
+    if solid is not None:
+        mg_ref.set_solid(solid)
+        mg_ref.apply_solid_bc(ub)
+    mg_ref.divergence(ub)
+    rhs = mg_ref.field("u_div", 0).astype(np.float64).copy()
+    x64, it, conv = oracle.poisson_fp64_solve(rhs, base_level=3, tol=1e-8, solid=solid)
+    assert conv
+    return x64
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+@pytest.mark.parametrize("N_,with_solid", [((32, 32, 32), False), ((48, 32, 32), True), ((64, 64), False),
+                                           ((64, 48), True)])
+def test_mixed_precision_poisson_vs_fp64(oracle, pfm, precision, N_, with_solid):
+    from helpers import box_solid
+
+    d = len(N_)
+    solid = box_solid(N_, 0.4, 0.6) if with_solid else None
+    u = smooth_mac_field(N_, seed=52)
+    x64 = _fp64_pressure(oracle, N_, u, solid)
+    fl = np.ones(N_, bool) if solid is None else ~solid
+    out = {}
+    for prec in (0, precision):
+        mg = pfm.MGPCG([[2, 2]] * d, list(N_), dim=d, base_level=3, precision=prec)
+        if solid is not None:
+            mg.set_solid(G(solid))
+        ug = [G(x) for x in u]
+        it = mg.Poisson(*ug, tol=1e-6, tol_mode="rel")
+        assert mg.converged
+        p = N(mg.p).astype(np.float64)
+        err = rel_l2(p[fl] - p[fl].mean(), x64[fl] - x64[fl].mean())
+        assert err < 1e-5, (prec, err)
+        out[prec] = (it, [N(t) for t in ug])
+    # the projected velocities agree with the fp32 solver's
+    for a in range(d):
+        assert rel_l2(out[precision][1][a], out[0][1][a]) < 1e-5
+    # 16-bit storage costs at most a few extra iterations
+    assert out[precision][0] <= out[0][0] + 3
+
+
+def test_mixed_precision_true_residual_replacement(oracle, pfm):
+    """A tight tolerance runs past the replacement interval; the solution
+    still matches the FP64 oracle."""
+    N_ = (64, 64, 64)
+    u = smooth_mac_field(N_, seed=53)
+    x64 = _fp64_pressure(oracle, N_, u, None)
+    mg = pfm.MGPCG([[2, 2]] * 3, list(N_), dim=3, base_level=3, precision=1)
+    ug = [G(x) for x in u]
+    it = mg.Poisson(*ug, tol=1e-9, tol_mode="rel", max_iters=40)
+    assert it >= 8
+    p = N(mg.p).astype(np.float64)
+    assert rel_l2(p - p.mean(), x64 - x64.mean()) < 1e-5
+
+
+def test_mixed_precision_hides_fp32_views(pfm):
+    mg = pfm.MGPCG([[2, 2]] * 3, [16, 16, 16], dim=3, base_level=3, precision=2)
+    with pytest.raises(pfm.PfmError):
+        mg.z
+    with pytest.raises(pfm.PfmError):
+        mg.smooth(0, 0)
