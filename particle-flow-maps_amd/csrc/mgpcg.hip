@@ -363,21 +363,9 @@ __device__ __forceinline__ Diag diag(const LvlDev& L, bool interior, float n_in,
     return dg;
 }
 
-// One slot of a red/black pass over a region of RI x RJ x (2 RK2) cells whose
-// first cell is the LDS cell (l0, l1, l2): slot e is a pair of cells along
-// axis 2, of which the member of colour `col` is taken.
-template <int RJ, int RK2>
-__device__ __forceinline__ void rb_slot(int e, int l0, int l1, int l2, int b0, int b1, int b2, int col, int& li,
-                                        int& lj, int& lk, int& g0, int& g1, int& g2) {
-    const int pk = e % RK2, rj = (e / RK2) % RJ, ri = e / (RK2 * RJ);
-    li = l0 + ri;
-    lj = l1 + rj;
-    g0 = b0 + li;
-    g1 = b1 + lj;
-    const int lk0 = l2 + 2 * pk;
-    lk = lk0 + ((col + g0 + g1 + b2 + lk0) & 1);
-    g2 = b2 + lk;
-}
+// Threads of a sweep block: 5 waves, so the 306 (j, k-pair) columns of the
+// first pass of a 3D tile take one round.
+constexpr int kSweepThreads = 320;
 
 // Two red/black passes (colours c0, then c0 ^ 1) over one tile: the
 // reference's smooth(l, c0); smooth(l, c0 ^ 1) (3D/mgpcg.py:165-172,
@@ -389,146 +377,252 @@ __device__ __forceinline__ void rb_slot(int e, int l0, int l1, int l2, int b0, i
 //   PROLONG  z_in += zc[parent] on load (prolongate, :160-163,196);
 //   SUB_MEAN use r - mean (the recentring of solve, 3D/mgpcg.py:174-180);
 //   DOT      write per-block partials of z_out . r.
-// All global loads (z tile, r of both passes) are issued before the first
-// barrier; the passes are branch free apart from the block-uniform choice
-// between interior tiles (no walls) and tiles that touch the boundary.
 // ZT: storage of z (float, or 16-bit in mixed precision); RT: storage of r
 // (float on level 0, ZT on coarser levels).
+//
+// Work layout: every thread owns fixed columns along axis 0 -- a (j, k)
+// position of the tile face for the loads and stores, a (j, pair of k)
+// position for the passes -- and walks down the column with incremental
+// addresses, so no per-cell index decoding is needed.  The colour member of
+// a pair alternates from one axis-0 plane to the next.  All global loads
+// (z tile, r of the first pass) are issued before the first barrier, the
+// second pass's r right after it.
 template <int DIM, typename ZT, typename RT, bool ZERO_IN, bool PROLONG, bool SUB_MEAN, bool DOT>
-__global__ void __launch_bounds__(kBlock, 4) k_rbgs(LvlDev L, LvlDev Lc, TileMap tm,
-                                                 const ZT* __restrict__ zin, const ZT* __restrict__ zc,
-                                                 const RT* __restrict__ r, ZT* __restrict__ zout, int c0,
-                                                 const MGScalars* __restrict__ sc, double* __restrict__ partial) {
+__global__ void __launch_bounds__(kSweepThreads, 3) k_rbgs(LvlDev L, LvlDev Lc, TileMap tm,
+                                                        const ZT* __restrict__ zin, const ZT* __restrict__ zc,
+                                                        const RT* __restrict__ r, ZT* __restrict__ zout, int c0,
+                                                        const MGScalars* __restrict__ sc,
+                                                        double* __restrict__ partial) {
     using T = MgTile<DIM>;
-    constexpr int H = 2, HI = DIM == 3 ? H : 0;
+    constexpr int NT = kSweepThreads;
+    constexpr int H = 2, HI = DIM == 3 ? H : 0, A0 = DIM == 3 ? 1 : 0;
     constexpr int SI = T::I + 2 * HI, SJ = T::J + 2 * H, SK = T::K + 2 * H, SP = SJ * SK, NS = SI * SP;
-    constexpr int NL = (NS + kBlock - 1) / kBlock;
-    // pass 0: tile + one-cell ring; pass 1: tile
-    constexpr int A0 = DIM == 3 ? 1 : 0;
-    constexpr int R0I = T::I + 2 * A0, R0J = T::J + 2, R0K2 = (T::K + 2) / 2, N0 = R0I * R0J * R0K2;
-    constexpr int R1J = T::J, R1K2 = T::K / 2, N1 = T::I * R1J * R1K2;
-    constexpr int NP0 = (N0 + kBlock - 1) / kBlock, NP1 = (N1 + kBlock - 1) / kBlock;
+    constexpr int ZC = SJ * SK, ZR = (ZC + NT - 1) / NT;                      // z tile columns, rounds
+    constexpr int P0I = T::I + 2 * A0, P0P = (T::K + 2) / 2, P0C = (T::J + 2) * P0P;
+    constexpr int P0R = (P0C + NT - 1) / NT;                                  // pass 0: tile + ring
+    constexpr int P1I = T::I, P1P = T::K / 2, P1C = T::J * P1P, P1R = (P1C + NT - 1) / NT;  // pass 1: tile
+    constexpr int WC = T::J * T::K, WR = (WC + NT - 1) / NT;                  // interior columns
     static_assert(T::K % 2 == 0, "pairs along axis 2");
     __shared__ float zs[NS];
     int t[3];
     tile_of_block(tm, t);
     const int o0 = t[0] * T::I, o1 = t[1] * T::J, o2 = t[2] * T::K;
     const int b0 = o0 - HI, b1 = o1 - H, b2 = o2 - H;  // global cell of zs[0]
+    const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+    const int plane = d1 * d2;
     const int tid = threadIdx.x;
     const int c1 = c0 ^ 1;
     const __amdgpu_buffer_rsrc_t rr = lvl_rsrc(r, L);
-    float zv[NL], rv0[NP0], rv1[NP1];
+    // ---- z tile into registers
+    float zv[ZR][SI];
     if (!ZERO_IN) {
         const __amdgpu_buffer_rsrc_t rz = lvl_rsrc(zin, L);
         const __amdgpu_buffer_rsrc_t rc = lvl_rsrc(zc, Lc);
 #pragma unroll
-        for (int k = 0; k < NL; ++k) {
-            const int e = tid + k * kBlock;
-            const int lk = e % SK, lj = (e / SK) % SJ, li = e / SP;
-            const int g0 = b0 + li, g1 = b1 + lj, g2 = b2 + lk;
-            const bool in = e < NS && inside(L, g0, g1, g2);
-            float v = ldv<ZT>(rz, in ? cell_off<ZT>(L, g0, g1, g2) : kOOB);
-            // outside the level both loads read 0 and 0 + 0 stays +0
-            if (PROLONG) {
-                v = __fadd_rn(v, ldv<ZT>(rc, in ? (uint32_t)parent_index(Lc, g0, g1, g2) * (uint32_t)sizeof(ZT)
-                                              : kOOB));
-                // solid cells take no prolongation (they carry no pressure)
-                if (L.dg && ld_dg(dg_rsrc(L), L, g0, g1, g2) == 0) v = 0.f;
+        for (int m = 0; m < ZR; ++m) {
+            const int p = tid + m * NT;
+            const int lj = p / SK, lk = p - lj * SK;
+            const int g1 = b1 + lj, g2 = b2 + lk;
+            const bool col = p < ZC && (unsigned)g1 < (unsigned)d1 && (unsigned)g2 < (unsigned)d2;
+            const int base = g1 * d2 + g2;
+#pragma unroll
+            for (int li = 0; li < SI; ++li) {
+                const int g0 = b0 + li;
+                const bool in = col && (unsigned)g0 < (unsigned)d0;
+                zv[m][li] = ldv<ZT>(rz, in ? (uint32_t)(g0 * plane + base) * (uint32_t)sizeof(ZT) : kOOB);
             }
-            zv[k] = v;
+            if (PROLONG) {
+                // b0 is even, so cells li = 2h and 2h + 1 share the coarse
+                // parent along axis 0: one coarse load per pair
+                static_assert(DIM == 2 || (MgTile<3>::I % 2 == 0 && HI % 2 == 0), "even axis-0 tile origin");
+                const int e1 = Lc.d[1], e2 = Lc.d[2];
+                const int pcol = min(g1 >> 1, e1 - 1) * e2 + min(g2 >> 1, e2 - 1);
+                const int pplane = e1 * e2;
+                constexpr int SH = (SI + 1) / 2;
+                float zcv[SH];
+#pragma unroll
+                for (int h = 0; h < SH; ++h) {
+                    const int g0 = b0 + 2 * h;
+                    const bool in = col && (unsigned)g0 < (unsigned)d0;
+                    const int pc = min(g0 >> 1, Lc.d[0] - 1) * pplane + pcol;
+                    zcv[h] = ldv<ZT>(rc, in ? (uint32_t)pc * (uint32_t)sizeof(ZT) : kOOB);
+                }
+                // outside the level the z load reads 0 and 0 + 0 stays +0
+#pragma unroll
+                for (int li = 0; li < SI; ++li) {
+                    const bool in = col && (unsigned)(b0 + li) < (unsigned)d0;
+                    zv[m][li] = __fadd_rn(zv[m][li], in ? zcv[li >> 1] : 0.f);
+                }
+                if (L.dg) {
+                    // solid cells take no prolongation (they carry no pressure)
+                    const __amdgpu_buffer_rsrc_t rd = dg_rsrc(L);
+                    int sv[SI];
+#pragma unroll
+                    for (int li = 0; li < SI; ++li) {
+                        const int g0 = b0 + li;
+                        const bool in = col && (unsigned)g0 < (unsigned)d0;
+                        sv[li] = (int)__builtin_amdgcn_raw_buffer_load_b8(rd, in ? (uint32_t)(g0 * plane + base)
+                                                                                 : kOOB, 0, 0);
+                    }
+#pragma unroll
+                    for (int li = 0; li < SI; ++li) zv[m][li] = sv[li] == 0 ? 0.f : zv[m][li];
+                }
+            }
         }
     }
+    // ---- r of the first pass
+    float rv0[P0R][P0I];
 #pragma unroll
-    for (int k = 0; k < NP0; ++k) {
-        int li, lj, lk, g0, g1, g2;
-        rb_slot<R0J, R0K2>(min(tid + k * kBlock, N0 - 1), HI - A0, H - 1, H - 1, b0, b1, b2, c0, li, lj, lk, g0,
-                           g1, g2);
-        rv0[k] = ldv<RT>(rr, cell_off<RT>(L, g0, g1, g2));
+    for (int m = 0; m < P0R; ++m) {
+        const int p = tid + m * NT;
+        const int rj = p / P0P, pk = p - rj * P0P;
+        const int g1 = b1 + 1 + rj, gk0 = b2 + 1 + 2 * pk;
+        const bool col = p < P0C && (unsigned)g1 < (unsigned)d1;
+#pragma unroll
+        for (int ri = 0; ri < P0I; ++ri) {
+            const int g0 = b0 + HI - A0 + ri;
+            const int g2 = gk0 + ((c0 + g0 + g1 + gk0) & 1);
+            const bool in = col && (unsigned)g0 < (unsigned)d0 && (unsigned)g2 < (unsigned)d2;
+            rv0[m][ri] = ldv<RT>(rr, in ? (uint32_t)(g0 * plane + g1 * d2 + g2) * (uint32_t)sizeof(RT) : kOOB);
+        }
     }
+    // ---- z tile into LDS (LDS index = li * SP + p with p = lj * SK + lk)
 #pragma unroll
-    for (int k = 0; k < NL; ++k) {
-        const int e = tid + k * kBlock;
-        if (e < NS) zs[e] = ZERO_IN ? 0.f : zv[k];
+    for (int m = 0; m < ZR; ++m) {
+        const int p = tid + m * NT;
+        if (p < ZC) {
+#pragma unroll
+            for (int li = 0; li < SI; ++li) zs[li * SP + p] = ZERO_IN ? 0.f : zv[m][li];
+        }
     }
     const float mean = SUB_MEAN ? sc->mean : 0.f;
     // 16-bit z: level 0 works on vs * r (coarser r arrive scaled)
     constexpr bool kScale = sizeof(ZT) == 2 && sizeof(RT) == 4;
     const float vs = kScale ? sc->vs : 1.0f;
     // the pass-0 region [o - A0, o + T + A0) x ... decides the interior case
-    const bool interior =
-        box_interior(L, o0 - A0, o1 - 1, o2 - 1, o0 + T::I + A0, o1 + T::J + 1, o2 + T::K + 1, DIM == 3);
+    const bool interior = !L.dg && box_interior(L, o0 - A0, o1 - 1, o2 - 1, o0 + T::I + A0, o1 + T::J + 1,
+                                                o2 + T::K + 1, DIM == 3);
+    const bool interior1 = !L.dg && box_interior(L, o0, o1, o2, o0 + T::I, o1 + T::J, o2 + T::K, DIM == 3);
     const float n_int = (float)L.nb2, y_int = pfm_rcp_small(L.nb2);
     double s = 0.0;
     __syncthreads();
     // the thread index is hidden from the optimiser past the barrier so the
-    // slot coordinates are recomputed rather than held in registers
+    // column coordinates are recomputed rather than held in registers
     int tid2 = tid;
     asm volatile("" : "+v"(tid2));
-    // r of the second pass arrives while the first one runs
+    // ---- r of the second pass arrives while the first one runs
+    float rv1[P1R][P1I];
 #pragma unroll
-    for (int k = 0; k < NP1; ++k) {
-        int li, lj, lk, g0, g1, g2;
-        rb_slot<R1J, R1K2>(min(tid2 + k * kBlock, N1 - 1), HI, H, H, b0, b1, b2, c1, li, lj, lk, g0, g1, g2);
-        rv1[k] = ldv<RT>(rr, cell_off<RT>(L, g0, g1, g2));
+    for (int m = 0; m < P1R; ++m) {
+        const int p = tid2 + m * NT;
+        const int rj = p / P1P, pk = p - rj * P1P;
+        const int g1 = b1 + 2 + rj, gk0 = b2 + 2 + 2 * pk;
+        const bool col = p < P1C && (unsigned)g1 < (unsigned)d1;
+#pragma unroll
+        for (int ri = 0; ri < P1I; ++ri) {
+            const int g0 = b0 + HI + ri;
+            const int g2 = gk0 + ((c1 + g0 + g1 + gk0) & 1);
+            const bool in = col && (unsigned)g0 < (unsigned)d0 && (unsigned)g2 < (unsigned)d2;
+            rv1[m][ri] = ldv<RT>(rr, in ? (uint32_t)(g0 * plane + g1 * d2 + g2) * (uint32_t)sizeof(RT) : kOOB);
+        }
     }
-#pragma unroll
-    for (int k = 0; k < NP0; ++k) {
-        const int e = tid2 + k * kBlock;
-        int li, lj, lk, g0, g1, g2;
-        rb_slot<R0J, R0K2>(min(e, N0 - 1), HI - A0, H - 1, H - 1, b0, b1, b2, c0, li, lj, lk, g0, g1, g2);
-        const int le = (li * SJ + lj) * SK + lk;
-        const float rraw = SUB_MEAN ? __fsub_rn(rv0[k], mean) : rv0[k];
+    // one cell update of a pass, given its neighbour sum; returns whether the
+    // cell was updated.  A pass writes only cells of its colour and reads only
+    // the other colour, so a column's neighbour sums are all gathered first
+    // (their LDS reads overlap) and the stores follow.
+    auto update = [&](int le, int g0, int g1, int g2, bool in, bool intr, float rin, float ns, float& zval,
+                      float& rraw) -> bool {
+        rraw = SUB_MEAN ? __fsub_rn(rin, mean) : rin;
         const float rv = kScale ? rraw * vs : rraw;
-        Diag dg = diag(L, interior && !L.dg, n_int, y_int, g0, g1, g2);
+        Diag dg = diag(L, intr, n_int, y_int, g0, g1, g2);
         bool fluid = true;
         if (L.dg) {
             const int n = ld_dg(dg_rsrc(L), L, g0, g1, g2);
             dg = Diag{(float)n, pfm_rcp_small(n)};
             fluid = n != 0;
         }
-        const float v = pfm_div_small(__fadd_rn(rv, nsum_lds<DIM>(zs + le, SP, SK)), dg.n, dg.y);
-        const bool upd = e < N0 && inside(L, g0, g1, g2) && fluid;
-        if (upd) zs[le] = v;
-        // cells of colour c0 inside the tile are final after this pass (the
-        // product uses z as stored)
-        if (DOT && upd && g0 >= o0 && g0 < o0 + T::I && g1 >= o1 && g1 < o1 + T::J && g2 >= o2 && g2 < o2 + T::K)
-            s += (double)to_f(from_f<ZT>(v)) * (double)rraw;
-    }
-    __syncthreads();
-    const bool interior1 = box_interior(L, o0, o1, o2, o0 + T::I, o1 + T::J, o2 + T::K, DIM == 3);
-    int tid3 = tid;
-    asm volatile("" : "+v"(tid3));
+        zval = pfm_div_small(__fadd_rn(rv, ns), dg.n, dg.y);
+        const bool upd = in && fluid;
+        if (upd) zs[le] = zval;
+        return upd;
+    };
+    // ---- pass 0 (colour c0) on the tile and its one-cell ring
 #pragma unroll
-    for (int k = 0; k < NP1; ++k) {
-        const int e = tid3 + k * kBlock;
-        int li, lj, lk, g0, g1, g2;
-        rb_slot<R1J, R1K2>(min(e, N1 - 1), HI, H, H, b0, b1, b2, c1, li, lj, lk, g0, g1, g2);
-        const int le = (li * SJ + lj) * SK + lk;
-        const float rraw = SUB_MEAN ? __fsub_rn(rv1[k], mean) : rv1[k];
-        const float rv = kScale ? rraw * vs : rraw;
-        Diag dg = diag(L, interior1 && !L.dg, n_int, y_int, g0, g1, g2);
-        bool fluid = true;
-        if (L.dg) {
-            const int n = ld_dg(dg_rsrc(L), L, g0, g1, g2);
-            dg = Diag{(float)n, pfm_rcp_small(n)};
-            fluid = n != 0;
+    for (int m = 0; m < P0R; ++m) {
+        const int p = tid2 + m * NT;
+        const int rj = p / P0P, pk = p - rj * P0P;
+        const int lj = 1 + rj, g1 = b1 + lj;
+        const int lk0 = 1 + 2 * pk, gk0 = b2 + lk0;
+        const bool col = p < P0C && (unsigned)g1 < (unsigned)d1;
+        float ns[P0I];
+#pragma unroll
+        for (int ri = 0; ri < P0I; ++ri) {
+            const int li = HI - A0 + ri, g0 = b0 + li;
+            const int off = (c0 + g0 + g1 + gk0) & 1;
+            ns[ri] = nsum_lds<DIM>(zs + (li * SJ + lj) * SK + lk0 + off, SP, SK);
         }
-        const float v = pfm_div_small(__fadd_rn(rv, nsum_lds<DIM>(zs + le, SP, SK)), dg.n, dg.y);
-        const bool in = e < N1 && inside(L, g0, g1, g2) && fluid;
-        if (in) zs[le] = v;
-        if (DOT && in) s += (double)to_f(from_f<ZT>(v)) * (double)rraw;
+#pragma unroll
+        for (int ri = 0; ri < P0I; ++ri) {
+            const int li = HI - A0 + ri, g0 = b0 + li;
+            const int off = (c0 + g0 + g1 + gk0) & 1;
+            const int g2 = gk0 + off;
+            const bool in = col && (unsigned)g0 < (unsigned)d0 && (unsigned)g2 < (unsigned)d2;
+            float v, rraw;
+            const bool upd = update((li * SJ + lj) * SK + lk0 + off, g0, g1, g2, in, interior, rv0[m][ri], ns[ri], v,
+                                    rraw);
+            // cells of colour c0 inside the tile are final after this pass (the
+            // product uses z as stored)
+            if (DOT && upd && g0 >= o0 && g0 < o0 + T::I && g1 >= o1 && g1 < o1 + T::J && g2 >= o2 &&
+                g2 < o2 + T::K)
+                s += (double)to_f(from_f<ZT>(v)) * (double)rraw;
+        }
     }
     __syncthreads();
-    constexpr int NT = T::I * T::J * T::K;
-#pragma unroll 4
-    for (int e = tid; e < NT; e += kBlock) {
-        const int tk = e % T::K, tj = (e / T::K) % T::J, ti = e / (T::K * T::J);
-        const int g0 = o0 + ti, g1 = o1 + tj, g2 = o2 + tk;
-        if (!inside(L, g0, g1, g2)) continue;
-        zout[(g0 * L.d[1] + g1) * L.d[2] + g2] = from_f<ZT>(zs[((ti + HI) * SJ + tj + H) * SK + tk + H]);
+    // ---- pass 1 (colour c1) on the tile
+#pragma unroll
+    for (int m = 0; m < P1R; ++m) {
+        const int p = tid2 + m * NT;
+        const int rj = p / P1P, pk = p - rj * P1P;
+        const int lj = 2 + rj, g1 = b1 + lj;
+        const int lk0 = 2 + 2 * pk, gk0 = b2 + lk0;
+        const bool col = p < P1C && (unsigned)g1 < (unsigned)d1;
+        float ns[P1I];
+#pragma unroll
+        for (int ri = 0; ri < P1I; ++ri) {
+            const int li = HI + ri, g0 = b0 + li;
+            const int off = (c1 + g0 + g1 + gk0) & 1;
+            ns[ri] = nsum_lds<DIM>(zs + (li * SJ + lj) * SK + lk0 + off, SP, SK);
+        }
+#pragma unroll
+        for (int ri = 0; ri < P1I; ++ri) {
+            const int li = HI + ri, g0 = b0 + li;
+            const int off = (c1 + g0 + g1 + gk0) & 1;
+            const int g2 = gk0 + off;
+            const bool in = col && (unsigned)g0 < (unsigned)d0 && (unsigned)g2 < (unsigned)d2;
+            float v, rraw;
+            const bool upd = update((li * SJ + lj) * SK + lk0 + off, g0, g1, g2, in, interior1, rv1[m][ri], ns[ri],
+                                    v, rraw);
+            if (DOT && upd) s += (double)to_f(from_f<ZT>(v)) * (double)rraw;
+        }
+    }
+    __syncthreads();
+    // ---- interior to z_out
+#pragma unroll
+    for (int m = 0; m < WR; ++m) {
+        const int p = tid + m * NT;
+        const int tj = p / T::K, tk = p - tj * T::K;
+        const int g1 = o1 + tj, g2 = o2 + tk;
+        if (p < WC && g1 < d1 && g2 < d2) {
+            const int base = g1 * d2 + g2;
+#pragma unroll
+            for (int ti = 0; ti < T::I; ++ti) {
+                const int g0 = o0 + ti;
+                if (g0 < d0) zout[g0 * plane + base] = from_f<ZT>(zs[((ti + HI) * SJ + tj + H) * SK + tk + H]);
+            }
+        }
     }
     if (DOT) {
-        __shared__ double sh[kBlock / 64];
+        __shared__ double sh[NT / 64];
         s = block_sum(s, sh);
         if (tid == 0) partial[blockIdx.x] = kScale ? s * (double)sc->vinv : s;  // z was stored scaled
     }
@@ -1389,7 +1483,7 @@ static void launch_rbgs(int kind, bool sub_mean, bool dot, dim3 g, hipStream_t s
                         const ZT* zin, const ZT* zc, const RT* r, ZT* zout, int c0, const MGScalars* sc,
                         double* partial) {
 #define PFM_RBGS(Z, P, M, D)                                                                                     \
-    hipLaunchKernelGGL((k_rbgs<DIM, ZT, RT, Z, P, M, D>), g, dim3(kBlock), 0, s, L, Lc, tm, zin, zc, r, zout, c0, \
+    hipLaunchKernelGGL((k_rbgs<DIM, ZT, RT, Z, P, M, D>), g, dim3(kSweepThreads), 0, s, L, Lc, tm, zin, zc, r, zout, c0, \
                        sc, partial)
     if (kind == kSweepZero) {
         if (sub_mean) PFM_RBGS(true, false, true, false);
