@@ -251,10 +251,16 @@ __global__ void __launch_bounds__(kBlock) k_curl(GridDev g, const float* __restr
 }
 
 // Point vortices: angular_vel_func + leapfrog_vel_func, 2D/init_conditions.py:5-9,29-60.
+// The vortices travel by value in the kernel arguments, kVortChunk per
+// launch; a launch after the first continues the running sums stored in vc,
+// so the summation order (vortex 0, 1, ...) is that of a single loop.
 constexpr int kMaxVort = 4096;
-__constant__ float c_vort[3 * kMaxVort];
+constexpr int kVortChunk = 128;
+struct VortChunk {
+    float x[kVortChunk], y[kVortChunk], w[kVortChunk];
+};
 
-__global__ void __launch_bounds__(kBlock) k_point_vortices(GridDev g, int nv, float rad2,
+__global__ void __launch_bounds__(kBlock) k_point_vortices(GridDev g, VortChunk vt, int nv, bool first, float rad2,
                                                            float strength, float* __restrict__ vc) {
     const int64_t C = (int64_t)g.res[0] * g.res[1];
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -263,18 +269,19 @@ __global__ void __launch_bounds__(kBlock) k_point_vortices(GridDev g, int nv, fl
     const int i = (int)(q / g.res[1]);
     const float X0 = __fmul_rn(__fadd_rn((float)i, 0.5f), g.dx);
     const float X1 = __fmul_rn(__fadd_rn((float)j, 0.5f), g.dx);
-    float vx = 0.f, vy = 0.f;
+    float vx = first ? 0.f : vc[q], vy = first ? 0.f : vc[C + q];
     for (int v = 0; v < nv; ++v) {
-        float px = __fsub_rn(X0, c_vort[v]);
-        float py = __fsub_rn(X1, c_vort[kMaxVort + v]);
+        float px = __fsub_rn(X0, vt.x[v]);
+        float py = __fsub_rn(X1, vt.y[v]);
         float r = __fsqrt_rn(__fadd_rn(__fmul_rn(px, px), __fmul_rn(py, py)));
         r = __fadd_rn(r, 1e-6f);
         float lin = __fmul_rn(__fdiv_rn(__fmul_rn(strength, 1.0f), r),
                               __fsub_rn(1.0f, expf(__fdiv_rn(-__fmul_rn(r, r), rad2))));
         float ang = __fmul_rn(__fdiv_rn(1.0f, r), lin);
-        float s = __fmul_rn(ang, c_vort[2 * kMaxVort + v]);
-        vx = v == 0 ? __fmul_rn(s, -py) : __fadd_rn(vx, __fmul_rn(s, -py));
-        vy = v == 0 ? __fmul_rn(s, px) : __fadd_rn(vy, __fmul_rn(s, px));
+        float s = __fmul_rn(ang, vt.w[v]);
+        const bool init = first && v == 0;
+        vx = init ? __fmul_rn(s, -py) : __fadd_rn(vx, __fmul_rn(s, -py));
+        vy = init ? __fmul_rn(s, px) : __fadd_rn(vy, __fmul_rn(s, px));
     }
     vc[q] = vx;
     vc[C + q] = vy;
@@ -520,17 +527,19 @@ int pfm_point_vortices(const pfm_grid* gp, int nv, const float* cx, const float*
     PFM_CHECK_ARG(g.dim == 2, "point vortices are 2D");
     PFM_CHECK_ARG(nv >= 1 && nv <= kMaxVort, "nv must be in [1, %d]", kMaxVort);
     PFM_CHECK_ARG(cx && cy && w && vc, "null pointer");
-    static float host[3 * kMaxVort];
-    memcpy(host, cx, sizeof(float) * nv);
-    memcpy(host + kMaxVort, cy, sizeof(float) * nv);
-    memcpy(host + 2 * kMaxVort, w, sizeof(float) * nv);
     hipStream_t s = as_stream(stream);
-    PFM_HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_vort), host, sizeof(host), 0, hipMemcpyHostToDevice, s));
-    PFM_HIP_TRY(hipStreamSynchronize(s));
     const int64_t C = cell_count(g);
     const float rad2 = rad * rad;
-    hipLaunchKernelGGL(k_point_vortices, dim3(grid_for(C)), dim3(kBlock), 0, s, g, nv, rad2, strength, vc);
-    PFM_LAUNCH_CHECK();
+    for (int v0 = 0; v0 < nv; v0 += kVortChunk) {
+        const int m = nv - v0 < kVortChunk ? nv - v0 : kVortChunk;
+        VortChunk vt = {};
+        memcpy(vt.x, cx + v0, sizeof(float) * m);
+        memcpy(vt.y, cy + v0, sizeof(float) * m);
+        memcpy(vt.w, w + v0, sizeof(float) * m);
+        hipLaunchKernelGGL(k_point_vortices, dim3(grid_for(C)), dim3(kBlock), 0, s, g, vt, m, v0 == 0, rad2,
+                           strength, vc);
+        PFM_LAUNCH_CHECK();
+    }
     return PFM_OK;
 }
 

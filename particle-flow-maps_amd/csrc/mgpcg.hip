@@ -40,6 +40,7 @@ struct MGScalars {
     double zTr;     // old z.r
     double pAp;
     double rTr;
+    double xTx;     // x.x (read with rTr: the fp32 floor of the true residual)
     double r0;      // r.r of the recentred initial residual
     double sum_r;
     double new_zTr;
@@ -63,6 +64,7 @@ struct MGHandle {
     int bc[3][2];
     int dirichlet_counts;
     int precision;
+    int vshift;  // extra binades of V-cycle input down-scaling (fp16 storage)
     Level lv[kMaxLevels];
     // level arrays: r[0] is fp32 (the PCG residual); z, z2 and r[l >= 1] are
     // fp32 at precision 0, __half (1) or __hip_bfloat16 (2) otherwise
@@ -267,8 +269,50 @@ __device__ __forceinline__ float to_f(__half v) { return __half2float(v); }
 __device__ __forceinline__ float to_f(__hip_bfloat16 v) { return __bfloat162float(v); }
 template <typename V> __device__ __forceinline__ V from_f(float v);
 template <> __device__ __forceinline__ float from_f<float>(float v) { return v; }
-template <> __device__ __forceinline__ __half from_f<__half>(float v) { return __float2half(v); }
+// fp16 stores saturate at the largest finite half, so an outlier can never
+// turn the preconditioner's output into inf / NaN.
+template <> __device__ __forceinline__ __half from_f<__half>(float v) {
+    return __float2half(fminf(fmaxf(v, -65504.0f), 65504.0f));
+}
 template <> __device__ __forceinline__ __hip_bfloat16 from_f<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+
+// restrict (3D/mgpcg.py:152-158) for any extents with 16-bit z and coarse r
+// (the mixed-precision solver; the tiled kernel needs exactly halving
+// extents): the gather of k_restrict over typed storage.  On level 0 (fp32 r,
+// 16-bit z) r enters as vs * r, as in the tiled kernels.
+template <typename ZT, typename RT>
+__global__ void __launch_bounds__(kBlock) k_restrict_any(LvlDev Lf, LvlDev Lc, float inv_scale,
+                                                         const ZT* __restrict__ z, const RT* __restrict__ r,
+                                                         ZT* __restrict__ rc, const MGScalars* __restrict__ sc) {
+    int C[3];
+    if (!cell_of_thread(Lc, 0, false, C)) return;
+    const int64_t qc = cell_index(Lc, C);
+    const float vs = (sizeof(ZT) == 2 && sizeof(RT) == 4) ? sc->vs : 1.0f;
+    const int64_t s0 = (int64_t)Lf.d[1] * Lf.d[2], s1 = Lf.d[2];
+    int hi[3];
+    for (int a = 0; a < 3; ++a) hi[a] = (C[a] == Lc.d[a] - 1) ? Lf.d[a] - 1 - 2 * C[a] : 1;
+    float acc = 0.0f;
+    for (int a = 0; a <= hi[0]; ++a)
+        for (int b = 0; b <= hi[1]; ++b)
+            for (int c = 0; c <= hi[2]; ++c) {
+                int I[3] = {2 * C[0] + a, 2 * C[1] + b, 2 * C[2] + c};
+                const int64_t q = cell_index(Lf, I);
+                if (solid_at(Lf, q)) continue;
+                const ZT* x = z + q;
+                float ns = 0.0f;
+                if (I[0] < Lf.d[0] - 1) ns = __fadd_rn(ns, to_f(x[s0]));
+                if (I[0] > 0) ns = __fadd_rn(ns, to_f(x[-s0]));
+                if (I[1] < Lf.d[1] - 1) ns = __fadd_rn(ns, to_f(x[s1]));
+                if (I[1] > 0) ns = __fadd_rn(ns, to_f(x[-s1]));
+                if (I[2] < Lf.d[2] - 1) ns = __fadd_rn(ns, to_f(x[1]));
+                if (I[2] > 0) ns = __fadd_rn(ns, to_f(x[-1]));
+                float rq = to_f(r[q]);
+                if (sizeof(ZT) == 2 && sizeof(RT) == 4) rq *= vs;
+                const float res = __fsub_rn(rq, __fsub_rn(__fmul_rn(nfn(Lf, I), to_f(x[0])), ns));
+                acc = __fadd_rn(acc, __fmul_rn(__fmul_rn(res, 1.0f), inv_scale));
+            }
+    rc[qc] = from_f<ZT>(acc);
+}
 
 // Buffer view of a level array: the tiled kernels load through 32-bit byte
 // offsets, and an offset past the end (kOOB) reads 0, which is the value the
@@ -1107,6 +1151,8 @@ __global__ void __launch_bounds__(kBlock) k_dot(int64_t n, const AT* __restrict_
 
 // update_x + update_r (3D/mgpcg.py:137-145) fused with the r.r and sum(r)
 // reductions (convergence test :249-255 and recenter :174-180).
+// XX: also the partials of x.x (third block of kMgRedBlocks partials).
+template <bool XX>
 __global__ void __launch_bounds__(kBlock) k_update_xr(int64_t n, float* __restrict__ x,
                                                       float* __restrict__ r, const float* __restrict__ p,
                                                       const float* __restrict__ Ap,
@@ -1114,20 +1160,24 @@ __global__ void __launch_bounds__(kBlock) k_update_xr(int64_t n, float* __restri
                                                       double* __restrict__ partial) {
     __shared__ double sh[kBlock / 64];
     const float alpha = sc->alpha;
-    double s2 = 0.0, s1 = 0.0;
+    double s2 = 0.0, s1 = 0.0, s3 = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x) {
-        x[q] = __fadd_rn(x[q], __fmul_rn(alpha, p[q]));
+        const float xq = __fadd_rn(x[q], __fmul_rn(alpha, p[q]));
+        x[q] = xq;
         const float rq = __fsub_rn(r[q], __fmul_rn(alpha, Ap[q]));
         r[q] = rq;
         s2 += (double)rq * (double)rq;
         s1 += (double)rq;
+        if (XX) s3 += (double)xq * (double)xq;
     }
     s2 = block_sum(s2, sh);
     s1 = block_sum(s1, sh);
+    if (XX) s3 = block_sum(s3, sh);
     if (threadIdx.x == 0) {
         partial[blockIdx.x] = s2;
         partial[kMgRedBlocks + blockIdx.x] = s1;
+        if (XX) partial[2 * kMgRedBlocks + blockIdx.x] = s3;
     }
 }
 
@@ -1151,10 +1201,11 @@ __global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t n, float* __restric
         if (!dg || dg[q] != 0) r[q] = __fsub_rn(r[q], m);  // solid cells stay 0
 }
 
-// vs = 2^round(-log2(rms r)) from r.r over n cells (1 when r vanishes).
-__device__ __forceinline__ void set_vscale(MGScalars* sc, double rtr, double n) {
+// vs = 2^(round(-log2(rms r)) - vshift) from r.r over n cells (1 when r
+// vanishes).
+__device__ __forceinline__ void set_vscale(MGScalars* sc, double rtr, double n, int vshift) {
     if (rtr > 0.0 && n > 0.0) {
-        const double e = rint(-0.5 * log2(rtr / n));
+        const double e = rint(-0.5 * log2(rtr / n)) - vshift;
         const double c = fmin(fmax(e, -100.0), 100.0);
         sc->vs = (float)exp2(c);
         sc->vinv = (float)exp2(-c);
@@ -1163,20 +1214,24 @@ __device__ __forceinline__ void set_vscale(MGScalars* sc, double rtr, double n) 
     }
 }
 
-enum FinalizeMode { kFinMean = 0, kFinR0 = 1, kFinZTr = 2, kFinAlpha = 3, kFinRTr = 4, kFinBeta = 5 };
+enum FinalizeMode { kFinMean = 0, kFinR0 = 1, kFinZTr = 2, kFinAlpha = 3, kFinRTr = 4, kFinBeta = 5, kFinRTrX = 6 };
 
 // Fixed-order reduction of the per-block partials and the scalar recurrences
 // of solve() (3D/mgpcg.py:229-275), all in double.
 __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ partial, int nparts, int mode,
-                                                     double eps, double ncell, MGScalars* sc) {
+                                                     double eps, double ncell, int vshift, MGScalars* sc) {
     __shared__ double sh[kBlock / 64];
     double s = 0.0;
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += partial[b];
     s = block_sum(s, sh);
-    double s1 = 0.0;
-    if (mode == kFinRTr) {
+    double s1 = 0.0, s3 = 0.0;
+    if (mode == kFinRTr || mode == kFinRTrX) {
         for (int b = threadIdx.x; b < kMgRedBlocks; b += blockDim.x) s1 += partial[kMgRedBlocks + b];
         s1 = block_sum(s1, sh);
+    }
+    if (mode == kFinRTrX) {
+        for (int b = threadIdx.x; b < kMgRedBlocks; b += blockDim.x) s3 += partial[2 * kMgRedBlocks + b];
+        s3 = block_sum(s3, sh);
     }
     if (threadIdx.x != 0) return;
     switch (mode) {
@@ -1187,7 +1242,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ 
         case kFinR0:
             sc->r0 = s;
             sc->rTr = s;
-            set_vscale(sc, s, ncell);
+            set_vscale(sc, s, ncell, vshift);
             break;
         case kFinZTr:
             sc->zTr = s;
@@ -1196,11 +1251,14 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ 
             sc->pAp = s;
             sc->alpha = (float)(sc->zTr / (s + eps));
             break;
+        case kFinRTrX:
+            sc->xTx = s3;
+            [[fallthrough]];
         case kFinRTr:
             sc->rTr = s;
             sc->sum_r = s1;
             sc->mean = (float)(s1 / ncell);
-            set_vscale(sc, s, ncell);
+            set_vscale(sc, s, ncell, vshift);
             break;
         case kFinBeta: {
             sc->new_zTr = s;
@@ -1309,8 +1367,23 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(LvlDev L, const float*
     int I[3];
     if (!cell_of_thread(L, 0, false, I)) return;
     const int64_t q = cell_index(L, I);
-    r[q] = solid_at(L, q) ? 0.0f
-                          : __fsub_rn(b[q], __fsub_rn(__fmul_rn(nfn(L, I), x[q]), nsum(L, x, q, I)));
+    if (solid_at(L, q)) {
+        r[q] = 0.0f;
+        return;
+    }
+    // FP64: the stencil terms are large and nearly cancel, so in fp32 the
+    // rounding of A x would swamp the residual; in double the result is the
+    // residual of the stored x rounded once.
+    const int64_t s0 = (int64_t)L.d[1] * L.d[2], s1 = L.d[2];
+    const float* xq = x + q;
+    double nb = 0.0;
+    if (I[0] < L.d[0] - 1) nb += xq[s0];
+    if (I[0] > 0) nb += xq[-s0];
+    if (I[1] < L.d[1] - 1) nb += xq[s1];
+    if (I[1] > 0) nb += xq[-s1];
+    if (I[2] < L.d[2] - 1) nb += xq[1];
+    if (I[2] > 0) nb += xq[-1];
+    r[q] = (float)((double)b[q] - ((double)nfn(L, I) * (double)xq[0] - nb));
 }
 
 // ---- solid obstacle (extension of the reference solver) ----------------------
@@ -1528,7 +1601,7 @@ static int rbgs(MGHandle* h, int l, int kind, int c0, const ZT* zin, ZT* zout, b
 }
 
 // Residual + restriction of level l into r[l + 1] (tiled when the extents
-// halve exactly; otherwise, at precision 0 only, the per-cell kernel).  With
+// halve exactly; otherwise the per-cell kernel, typed for 16-bit storage).  With
 // sub_mean, r[l] is recentred first (written back).
 template <typename ZT>
 static int restrict_fused(MGHandle* h, int l, bool sub_mean, hipStream_t s) {
@@ -1538,7 +1611,18 @@ static int restrict_fused(MGHandle* h, int l, bool sub_mean, hipStream_t s) {
                                h->sc, h->dg[l]);
             PFM_LAUNCH_CHECK();
         }
-        return restrict_level(h, l, s);
+        if (std::is_same<ZT, float>::value) return restrict_level(h, l, s);
+        // 16-bit storage: the typed per-cell gather
+        const LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
+        const float isc = restrict_inv_scale(h);
+        if (l == 0)
+            hipLaunchKernelGGL((k_restrict_any<ZT, float>), cell_grid(Lc.d, false), kCellBlock, 0, s, Lf, Lc, isc,
+                               (const ZT*)h->z[l], (const float*)h->r[l], (ZT*)h->r[l + 1], h->sc);
+        else
+            hipLaunchKernelGGL((k_restrict_any<ZT, ZT>), cell_grid(Lc.d, false), kCellBlock, 0, s, Lf, Lc, isc,
+                               (const ZT*)h->z[l], (const ZT*)h->r[l], (ZT*)h->r[l + 1], h->sc);
+        PFM_LAUNCH_CHECK();
+        return PFM_OK;
     }
     const LvlDev Lf = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
     const TileMap tm = restrict_tiles(h, l);
@@ -1692,7 +1776,7 @@ static int vcycle(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
 
 static int finalize(MGHandle* h, int mode, double eps, hipStream_t s, int nparts = kMgRedBlocks) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, h->partial, nparts, mode, eps,
-                       (double)h->n_fluid, h->sc);
+                       (double)h->n_fluid, h->vshift, h->sc);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -1782,6 +1866,14 @@ static int precondition_dot(MGHandle* h, bool sub_mean, hipStream_t s, int* part
 // Iterations between true-residual replacements (r = b - A x) in the
 // mixed-precision solver.
 constexpr int kReplaceEvery = 8;
+// A replacement is made only while |r| exceeds kFloorMargin times the
+// residual that the fp32 representation of x alone produces,
+// 2 dim 2^-24 |x| (each stencil term of A x carries a rounding of x).  Below
+// that the true residual is that representation noise, which no fp32
+// correction of x can remove, and replacing r with it would stall the
+// iteration; the recursive residual, which the fp32 solver also relies on,
+// is then kept.
+constexpr double kFloorMargin = 16.0;
 
 // solve: 3D/mgpcg.py:202-280.  The reference's update_p after each
 // preconditioner application is carried out at the start of the next Ap
@@ -1789,6 +1881,12 @@ constexpr int kReplaceEvery = 8;
 // without converging.  At precision 1/2 (16-bit V-cycle storage) the PCG
 // residual is replaced by the true residual b - A x every kReplaceEvery
 // iterations.
+static int read_rtr(MGHandle* h, hipStream_t s) {
+    PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr, &h->sc->rTr, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    PFM_HIP_TRY(hipStreamSynchronize(s));
+    return PFM_OK;
+}
+
 static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mode, int* iters_out,
                  double* rtr_out, hipStream_t s) {
     const int64_t n = h->lv[0].n;
@@ -1821,22 +1919,30 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
     int it = 0;
     double rtr = 0.0;
     bool converged = false;
+    const double floor_c = kFloorMargin * 2.0 * h->dim * 0x1p-24;
     while (max_iters == -1 || it < max_iters) {
         PFM_TRY(ap_dot_p(h, s, &parts));
         PFM_TRY(finalize(h, kFinAlpha, eps, s, parts));
-        hipLaunchKernelGGL(k_update_xr, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, r0, h->p, h->Ap, h->sc,
-                           h->partial);
+        if (replace)
+            hipLaunchKernelGGL(k_update_xr<true>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, r0, h->p, h->Ap,
+                               h->sc, h->partial);
+        else
+            hipLaunchKernelGGL(k_update_xr<false>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, r0, h->p, h->Ap,
+                               h->sc, h->partial);
         PFM_LAUNCH_CHECK();
-        if (replace && (it + 1) % kReplaceEvery == 0) {
+        PFM_TRY(finalize(h, replace ? kFinRTrX : kFinRTr, eps, s));
+        PFM_TRY(read_rtr(h, s));
+        rtr = h->host_rtr[0];
+        const double xtx = h->host_rtr[1];
+        if (replace && (it + 1) % kReplaceEvery == 0 && rtr > floor_c * floor_c * xtx) {
             hipLaunchKernelGGL(k_true_residual, cell_grid(L0.d, false), kCellBlock, 0, s, L0, h->b, h->x, r0);
             PFM_LAUNCH_CHECK();
             PFM_TRY(dot_t(h, r0, r0, s, h->partial));
             PFM_TRY(dot_t(h, r0, (const float*)nullptr, s, h->partial + kMgRedBlocks));
+            PFM_TRY(finalize(h, kFinRTr, eps, s));
+            PFM_TRY(read_rtr(h, s));
+            rtr = h->host_rtr[0];
         }
-        PFM_TRY(finalize(h, kFinRTr, eps, s));
-        PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr, &h->sc->rTr, sizeof(double), hipMemcpyDeviceToHost, s));
-        PFM_HIP_TRY(hipStreamSynchronize(s));
-        rtr = h->host_rtr[0];
         if (rtr < thresh) {
             converged = true;
             break;
@@ -1967,6 +2073,16 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     h->n_levels = L;
     h->dirichlet_counts = dim == 2;  // 2D/mgpcg.py:111-113
     h->precision = precision;
+    {
+        // The V-cycle output z grows like r / lambda_min with lambda_min ~
+        // dim pi^2 / N^2 for the smoothest mode of the undivided Laplacian:
+        // with fp16 storage, scale the input down so that rms(r) * N^2 /
+        // (dim pi^2) stays below 2^10, keeping 2^5 of headroom for peaks.
+        int nmax = 1;
+        for (int a = 0; a < dim; ++a) nmax = N[a] > nmax ? N[a] : nmax;
+        const double growth = (double)nmax * nmax / (dim * M_PI * M_PI);
+        h->vshift = precision == 1 ? (int)fmax(0.0, ceil(log2(growth)) - 10.0) : 0;
+    }
     for (int a = 0; a < 3; ++a) {
         h->bc[a][0] = a < dim ? bc[2 * a] : 0;
         h->bc[a][1] = a < dim ? bc[2 * a + 1] : 0;
@@ -2002,7 +2118,7 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     if (hipMalloc(&h->p, b0) != hipSuccess) return fail("p");
     if (hipMalloc(&h->p2, b0) != hipSuccess) return fail("p scratch");
     if (hipMalloc(&h->Ap, b0) != hipSuccess) return fail("Ap");
-    h->n_partial = std::max(2 * kMgRedBlocks, sweep_tiles(h, 0).total);
+    h->n_partial = std::max(3 * kMgRedBlocks, sweep_tiles(h, 0).total);
     if (hipMalloc(&h->partial, sizeof(double) * h->n_partial) != hipSuccess) return fail("partial");
     if (hipMalloc(&h->sc, sizeof(MGScalars)) != hipSuccess) return fail("scalars");
     if (hipHostMalloc(&h->host_rtr, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess)

@@ -359,10 +359,12 @@ _KE_SCRATCH = {}
 def kinetic_energy(res, u, stream=None) -> torch.Tensor:
     """Device float64 tensor [1]: sum_a sum u_a^2 / |F_a|."""
     dev = u[0].device
-    buf = _KE_SCRATCH.get(str(dev))
+    # one scratch per (device, stream): calls on different streams may overlap
+    key = (str(dev), _stream(stream))
+    buf = _KE_SCRATCH.get(key)
     if buf is None:
         buf = torch.empty(1 + 3 * 1024, dtype=torch.float64, device=dev)
-        _KE_SCRATCH[str(dev)] = buf
+        _KE_SCRATCH[key] = buf
     g = L.make_grid(res, default_dx(res))
     L.call("pfm_kinetic_energy", ctypes.byref(g), *_u_ptrs(u, res), L.ptr(buf, dtype=torch.float64),
            _stream(stream))

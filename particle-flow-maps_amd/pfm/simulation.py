@@ -47,6 +47,8 @@ class StepConfig:
     seed: int = -1
     particle_order: int = 0
     write_imp_grad_m: bool = True
+    # MGPCG level storage: 0 fp32 (the reference), 1 fp16, 2 bf16
+    mg_precision: int = 0
 
 
 def reinit_decision(i: int, ik: int, cfg: StepConfig, dim: int):
@@ -103,7 +105,8 @@ class Simulation:
             self.T = [pv() for _ in range(d)]
             self.C = [pv() for _ in range(d)]
         bt = boundary_types if boundary_types is not None else [[2, 2]] * d
-        self.solver = MGPCG(bt, list(self.res), dim=d, base_level=self.cfg.base_level)
+        self.solver = MGPCG(bt, list(self.res), dim=d, base_level=self.cfg.base_level,
+                            precision=self.cfg.mg_precision)
         self.binning = core.Binning(self.res, n, dev)
         self.max_speed_buf = torch.empty(1, **f32)
         # 3D: fuse the P2G particle composition into the RK4 kernel

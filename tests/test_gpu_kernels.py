@@ -293,6 +293,13 @@ def test_point_vortices_and_ring(oracle, pfm):
     vc = torch.empty((2,) + res, device=DEV)
     core.point_vortices(res, dx, cen, w, vc)
     assert rel_l2(N(vc), want) < 1e-6
+    # more vortices than one launch carries (128): the running sums continue
+    rng = np.random.default_rng(5)
+    cen = rng.uniform(0.2, 0.8, size=(300, 2))
+    w = rng.normal(size=300)
+    want = oracle.point_vortices(res, dx, cen, w, 1.5 * dx)
+    core.point_vortices(res, dx, cen, w, vc, 1.5 * dx)
+    assert rel_l2(N(vc), want) < 1e-6
     res3 = (16, 16, 16)
     want3 = np.zeros((3,) + res3, np.float32)
     oracle.vortex_ring(res3, 1 / 16, want3, 1.0, 0.15, 0.02, [0.5, 0.5, 0.3], [1, 0, 0], [0, 1, 0], 200)

@@ -202,7 +202,7 @@ def test_poisson_solid_vs_oracles(oracle, pfm, N_):
 
 
 # ---- mixed precision (16-bit V-cycle storage, FP32 PCG, FP64 dots) --------
-def _fp64_pressure(oracle, N_, u, solid):
+def _fp64_pressure(oracle, N_, u, solid, require_converged=True):
     mg_ref = oracle.MG(N_, base_level=3)
     ub = [x.copy() for x in u]
     mg_ref.apply_bc(ub)
@@ -212,14 +212,20 @@ def _fp64_pressure(oracle, N_, u, solid):
     mg_ref.divergence(ub)
     rhs = mg_ref.field("u_div", 0).astype(np.float64).copy()
     x64, it, conv = oracle.poisson_fp64_solve(rhs, base_level=3, tol=1e-8, solid=solid)
-    assert conv
+    # the FP64 PCG may stall just short of 1e-8 (its V-cycle is only nearly
+    # symmetric); its iterate is then still far more accurate than the checks
+    assert conv or not require_converged
     return x64
 
 
 @pytest.mark.parametrize("precision", [1, 2])
 @pytest.mark.parametrize("N_,with_solid", [((32, 32, 32), False), ((48, 32, 32), True), ((64, 64), False),
-                                           ((64, 48), True)])
+                                           ((64, 48), True), ((51, 34, 34), False), ((170, 170), False),
+                                           ((90, 34, 34), True)])
 def test_mixed_precision_poisson_vs_fp64(oracle, pfm, precision, N_, with_solid):
+    """(51,34,34), (170,170) and (90,34,34) have odd extents on levels that
+    run as separate launches (51: on the finest level), so their restriction
+    is the typed per-cell gather."""
     from helpers import box_solid
 
     d = len(N_)
@@ -243,6 +249,44 @@ def test_mixed_precision_poisson_vs_fp64(oracle, pfm, precision, N_, with_solid)
     for a in range(d):
         assert rel_l2(out[precision][1][a], out[0][1][a]) < 1e-5
     # 16-bit storage costs at most a few extra iterations
+    assert out[precision][0] <= out[0][0] + 3
+
+
+@pytest.mark.parametrize("precision,n", [(1, 512), (1, 1024)])
+def test_mixed_precision_large_2d_point_vortices(oracle, pfm, precision, n):
+    """Large 2D grids with localized sources: the V-cycle output grows like
+    N^2 times the residual, past the fp16 range without the grid-size
+    down-scaling.  (bf16 keeps the range, but its 8-bit significand lets the
+    quantisation noise of the fine-level z, amplified by the stencil, swamp
+    the residual: on these inputs it does not converge from 256^2 up, so
+    bf16 is not used for 2D.)  At a 1e-6 relative residual the mixed solve converges
+    within 3 iterations of the fp32 one; its pressure is within 2e-5 of the
+    FP64 oracle and no further from it than 1.5x the fp32 solver's, and its
+    projected velocity is within 1e-5 of the fp32 solver's."""
+    from pfm import core
+
+    res = (n, n)
+    rng = np.random.default_rng(n)
+    nv = 24
+    vc = torch.zeros((2,) + res, device=DEV)
+    core.point_vortices(res, 1.0 / n, rng.uniform(0.2, 0.8, size=(nv, 2)), rng.normal(size=nv), vc, 1.5 / n)
+    u0 = [torch.zeros(core.face_shape(res, a), device=DEV) for a in range(2)]
+    core.split_central(res, vc, u0)
+    x64 = _fp64_pressure(oracle, res, [N(t) for t in u0], None, require_converged=False)
+    x64 = x64 - x64.mean()
+    out = {}
+    for prec in (0, precision):
+        u = [t.clone() for t in u0]
+        mg = pfm.MGPCG([[2, 2]] * 2, list(res), dim=2, base_level=3, precision=prec)
+        it = mg.Poisson(*u, tol=1e-6, tol_mode="rel")
+        assert mg.converged, (prec, it)
+        p = N(mg.p).astype(np.float64)
+        out[prec] = (it, rel_l2(p - p.mean(), x64), [N(t) for t in u])
+    eu = max(rel_l2(out[precision][2][a], out[0][2][a]) for a in range(2))
+    print(f"n={n} prec={precision} iters {out[0][0]} / {out[precision][0]} pressure vs fp64 "
+          f"{out[0][1]:.2e} / {out[precision][1]:.2e} velocity vs fp32 {eu:.2e}")
+    assert out[precision][1] < 2e-5 and out[precision][1] <= 1.5 * out[0][1] + 1e-6
+    assert eu < 1e-5
     assert out[precision][0] <= out[0][0] + 3
 
 
