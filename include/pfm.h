@@ -269,6 +269,73 @@ int pfm_mg_restrict(void* handle, int level, pfm_stream_t stream);
 int pfm_mg_prolongate(void* handle, int level, pfm_stream_t stream);
 int pfm_mg_apply_A(void* handle, const float* p, float* Ap, pfm_stream_t stream);
 
+/* ---- native step driver ------------------------------------------------
+ * One scene of the solver stepped by the library: the per-step phase
+ * sequence of the reference main loops (3D/run_3D.py:716-831,
+ * 2D/run_2D.py:772-909 -- reinit schedule, CFL dt with the frame clip,
+ * midpoint velocity, particle RK4, T/imp/C updates, binning, P2G,
+ * projections) issued from C++ with exactly the arguments of the per-kernel
+ * entry points above.  Every field stays owned by the caller. */
+typedef struct {
+    int32_t dim;                  /* 2 or 3 */
+    int32_t res[3];               /* cells per axis; dx = 1 / res[1] */
+    /* hyperparameters.py knobs */
+    int32_t reinit_every, reinit_every_grad_m;
+    int32_t use_midpoint_vel, reinit_particle_pos;
+    double CFL, visualize_dt;
+    int32_t ppc_axis;             /* particles per cell = ppc_axis^dim */
+    int32_t particle_order;       /* pfm_seed_particles order */
+    int64_t seed;                 /* pfm_seed_particles seed (< 0: the reference lattice) */
+    /* projection (pfm_mg_poisson) */
+    int32_t max_iters, tol_mode;
+    double tol;
+    int32_t fused_p2g;            /* 3D: pfm_rk4_particles_p2g_record + pfm_p2g_records */
+    int64_t n;                    /* particles */
+    /* grid fields (face arrays; unused axes NULL) */
+    float* u[3];
+    float* p2g_weight[3];         /* also the copy of u for advect_u (3D/run_3D.py:783-787) */
+    float* psi_grid[3];           /* (dim, face shape) per axis */
+    float* T_grid[3];
+    /* particle fields, (dim, n) each; T and C are 2D only; imp_grad_m may be NULL */
+    float* pos;
+    float* init_imp;
+    float* imp_grad_m;
+    float* T_init[3];
+    float* T_grad_m[3];
+    float* init_C[3];
+    float* imp;
+    float* T[3];
+    float* C[3];
+    /* binning (pfm_bin_particles) and P2G workspace */
+    int32_t* cell_of;
+    int32_t* cell_start;
+    int32_t* perm;
+    void* workspace;
+    size_t ws_bytes;
+    float* max_speed;             /* device float[1] */
+    void* mg;                     /* pfm_mg_create handle of the scene's grid */
+} pfm_scene_desc;
+
+/* Schedule state (3D/run_3D.py:708-713), read and advanced by each step. */
+typedef struct {
+    int64_t i;                    /* step index, -1 before the first step */
+    int64_t ik;                   /* index of the last reinit that moved ik */
+    int64_t frame_idx;
+    double sub_t;
+    double last_dt;
+    int32_t n_solves;             /* projections of the last step (2 with the midpoint) */
+    int32_t iters[2];             /* their PCG iteration counts */
+    int32_t converged[2];
+} pfm_scene_state;
+
+/* Validates the descriptor and keeps a copy of it. */
+int pfm_scene_create(const pfm_scene_desc* desc, void** scene);
+/* One step.  dt < 0: calc_max_speed + CFL + frame clip (3D/run_3D.py:725-743);
+ * else the given dt (sub_t unchanged).  Synchronises `stream` for the max
+ * speed and for each projection's convergence tests. */
+int pfm_scene_step(void* scene, pfm_scene_state* state, double dt, pfm_stream_t stream);
+int pfm_scene_destroy(void* scene);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libpfm.so")
 ARCH = os.environ.get("PFM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["abi.cpp", "particles.hip", "grid.hip", "p2g.hip", "mgpcg.hip"]
+SOURCES = ["abi.cpp", "particles.hip", "grid.hip", "p2g.hip", "mgpcg.hip", "scene.cpp"]
 
 CFLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",

@@ -28,6 +28,36 @@ class PfmGrid(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("res", ctypes.c_int32 * 3), ("dx", ctypes.c_float)]
 
 
+class PfmSceneDesc(ctypes.Structure):
+    """pfm_scene_desc of include/pfm.h."""
+
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("res", ctypes.c_int32 * 3),
+        ("reinit_every", ctypes.c_int32), ("reinit_every_grad_m", ctypes.c_int32),
+        ("use_midpoint_vel", ctypes.c_int32), ("reinit_particle_pos", ctypes.c_int32),
+        ("CFL", ctypes.c_double), ("visualize_dt", ctypes.c_double),
+        ("ppc_axis", ctypes.c_int32), ("particle_order", ctypes.c_int32), ("seed", ctypes.c_int64),
+        ("max_iters", ctypes.c_int32), ("tol_mode", ctypes.c_int32), ("tol", ctypes.c_double),
+        ("fused_p2g", ctypes.c_int32), ("n", ctypes.c_int64),
+        ("u", c_vp * 3), ("p2g_weight", c_vp * 3), ("psi_grid", c_vp * 3), ("T_grid", c_vp * 3),
+        ("pos", c_vp), ("init_imp", c_vp), ("imp_grad_m", c_vp),
+        ("T_init", c_vp * 3), ("T_grad_m", c_vp * 3), ("init_C", c_vp * 3),
+        ("imp", c_vp), ("T", c_vp * 3), ("C", c_vp * 3),
+        ("cell_of", c_vp), ("cell_start", c_vp), ("perm", c_vp), ("workspace", c_vp), ("ws_bytes", c_sz),
+        ("max_speed", c_vp), ("mg", c_vp),
+    ]
+
+
+class PfmSceneState(ctypes.Structure):
+    """pfm_scene_state of include/pfm.h."""
+
+    _fields_ = [
+        ("i", ctypes.c_int64), ("ik", ctypes.c_int64), ("frame_idx", ctypes.c_int64),
+        ("sub_t", ctypes.c_double), ("last_dt", ctypes.c_double),
+        ("n_solves", ctypes.c_int32), ("iters", ctypes.c_int32 * 2), ("converged", ctypes.c_int32 * 2),
+    ]
+
+
 class PfmError(RuntimeError):
     pass
 
@@ -78,6 +108,9 @@ _SIGS = {
     "pfm_mg_restrict": [c_vp, c_int, c_vp],
     "pfm_mg_prolongate": [c_vp, c_int, c_vp],
     "pfm_mg_apply_A": [c_vp, c_vp, c_vp, c_vp],
+    "pfm_scene_create": [ctypes.POINTER(PfmSceneDesc), ctypes.POINTER(c_vp)],
+    "pfm_scene_step": [c_vp, ctypes.POINTER(PfmSceneState), c_d, c_vp],
+    "pfm_scene_destroy": [c_vp],
 }
 _RESTYPES = {
     "pfm_last_error": ctypes.c_char_p,

@@ -16,12 +16,14 @@ Python double exactly as the reference does it.
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import math
 from typing import List, Optional, Sequence
 
 import torch
 
+from . import _lib as L
 from . import core
 from .mgpcg import MGPCG
 
@@ -70,7 +72,7 @@ class Simulation:
     """One scene of the particle-flow-map solver, all state in HBM."""
 
     def __init__(self, res: Sequence[int], cfg: Optional[StepConfig] = None, device="cuda",
-                 boundary_types=None):
+                 boundary_types=None, native: bool = True):
         self.res = tuple(int(r) for r in res)
         self.dim = len(self.res)
         self.cfg = cfg or StepConfig()
@@ -118,6 +120,10 @@ class Simulation:
         self.frame_idx = 0
         self.last_dt = 0.0
         self.poisson_iters: List[int] = []
+        # step() runs the whole step in the library (pfm_scene_step) unless
+        # native is False, when it issues the same entry points from Python
+        self.native = native
+        self._scene = None
 
     # ------------------------------------------------------------------ IC
     def set_velocity_from_central(self, vc: torch.Tensor, project: bool = True):
@@ -175,9 +181,10 @@ class Simulation:
 
     def step(self, dt: Optional[float] = None):
         """One reference step; returns the dt used."""
+        if self.native:
+            return self._native_step(dt)
         self.i += 1
         i = self.i
-        cfg = self.cfg
         curr_dt = self.compute_dt() if dt is None else dt
         self.last_dt = curr_dt
         if self.dim == 3:
@@ -185,6 +192,11 @@ class Simulation:
         else:
             self._step2(i, curr_dt)
         return curr_dt
+
+    def _native_step(self, dt):
+        if self._scene is None:
+            self._scene = _NativeScene(self)
+        return self._scene.step(self, dt)
 
     def _step3(self, i, dt):
         cfg = self.cfg
@@ -248,3 +260,74 @@ class Simulation:
     # ----------------------------------------------------------- diagnostics
     def kinetic_energy(self) -> float:
         return float(core.kinetic_energy(self.res, self.u).item())
+
+
+class _NativeScene:
+    """pfm_scene_create over a Simulation's fields (captured once: every
+    Simulation method updates its fields in place)."""
+
+    def __init__(self, sim: "Simulation"):
+        lib = L.load()
+        cfg, d = sim.cfg, sim.dim
+        D = L.PfmSceneDesc()
+        D.dim = d
+        for a in range(3):
+            D.res[a] = sim.res[a] if a < d else 1
+        D.reinit_every, D.reinit_every_grad_m = cfg.reinit_every, cfg.reinit_every_grad_m
+        D.use_midpoint_vel, D.reinit_particle_pos = int(cfg.use_midpoint_vel), int(cfg.reinit_particle_pos)
+        D.CFL, D.visualize_dt = float(cfg.CFL), float(cfg.visualize_dt)
+        D.ppc_axis, D.particle_order, D.seed = sim.ppc_axis, int(cfg.particle_order), int(cfg.seed)
+        D.max_iters, D.tol = int(cfg.max_iters), float(cfg.tol)
+        D.tol_mode = 0 if cfg.tol_mode == "abs" else 1
+        D.fused_p2g = int(sim.fused_p2g)
+        D.n = sim.n
+
+        def p(t):
+            return None if t is None else L.ptr(t)
+
+        for a in range(d):
+            D.u[a], D.p2g_weight[a] = p(sim.u[a]), p(sim.p2g_weight[a])
+            D.psi_grid[a], D.T_grid[a] = p(sim.psi_grid[a]), p(sim.T_grid[a])
+            D.T_init[a], D.T_grad_m[a], D.init_C[a] = p(sim.T_init[a]), p(sim.T_grad_m[a]), p(sim.init_C[a])
+            if d == 2:
+                D.T[a], D.C[a] = p(sim.T[a]), p(sim.C[a])
+        D.pos, D.init_imp, D.imp_grad_m = p(sim.pos), p(sim.init_imp), p(sim.imp_grad_m)
+        if d == 2:
+            D.imp = p(sim.imp)
+        b = sim.binning
+        D.cell_of, D.cell_start, D.perm = (L.ptr(b.cell_of, dtype=torch.int32), L.ptr(b.cell_start, dtype=torch.int32),
+                                           L.ptr(b.perm, dtype=torch.int32))
+        D.workspace, D.ws_bytes = b.workspace.data_ptr(), b.workspace.numel()
+        D.max_speed = p(sim.max_speed_buf)
+        D.mg = sim.solver._h.value
+        h = ctypes.c_void_p()
+        L.check(lib.pfm_scene_create(ctypes.byref(D), ctypes.byref(h)), "pfm_scene_create")
+        self._lib = lib
+        self._h = h
+        self.desc = D
+        self.state = L.PfmSceneState()
+
+    def step(self, sim: "Simulation", dt):
+        st = self.state
+        st.i, st.ik, st.frame_idx = sim.i, sim.ik, sim.frame_idx
+        st.sub_t, st.last_dt = sim.sub_t, sim.last_dt
+        rc = self._lib.pfm_scene_step(self._h, ctypes.byref(st), -1.0 if dt is None else float(dt),
+                                      L.stream_ptr())
+        L.check(rc, "pfm_scene_step")
+        sim.i, sim.ik, sim.frame_idx = st.i, st.ik, st.frame_idx
+        sim.sub_t, sim.last_dt = st.sub_t, st.last_dt
+        k = min(int(st.n_solves), 2)
+        sim.poisson_iters.extend(st.iters[:k])
+        sv = sim.solver
+        if k:
+            sv.iterations, sv.converged = st.iters[k - 1], all(st.converged[:k])
+        core._mark_written(sim.pos)
+        return st.last_dt
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self._lib.pfm_scene_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

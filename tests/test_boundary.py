@@ -81,3 +81,34 @@ def test_product_does_not_import_oracle():
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "from oracle" not in text, f
                 assert "libpfm_oracle" not in text, f
+
+
+def test_scene_structs_match_header(tmp_path):
+    """The ctypes mirrors of pfm_scene_desc / pfm_scene_state have the C
+    layout (sizes and every field offset, from a gcc build of the header)."""
+    import ctypes
+    import shutil
+    import subprocess
+
+    from pfm import _lib
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    structs = {"pfm_scene_desc": _lib.PfmSceneDesc, "pfm_scene_state": _lib.PfmSceneState}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _t in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py)
+        for f, _t in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
