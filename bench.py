@@ -255,6 +255,99 @@ def summarize(db_path, top=25, by_grid=False, match=None):
     return "\n".join(lines)
 
 
+def _shift(t, a, s):
+    """t[i + s] along axis a (s = +-1), zero outside the domain."""
+    n = t.shape[a]
+    out = t.new_zeros(t.shape)
+    if s > 0:
+        out.narrow(a, 0, n - 1).copy_(t.narrow(a, 1, n - 1))
+    else:
+        out.narrow(a, 1, n - 1).copy_(t.narrow(a, 0, n - 1))
+    return out
+
+
+def fp64_reference(solver, u_bc, div, solid, torch, max_refine=6):
+    """FP64 solution of the obstacle Poisson problem the solvers see (free-slip
+    walls and solid cells; b = -div of the wall- and obstacle-corrected
+    velocity), by iterative refinement: the residual b - A x is formed in
+    float64 (torch, the 7-point operator over fluid neighbours) and each
+    correction comes from the fp32 MGPCG `solver` at a 1e-6 relative
+    residual; stops at a 1e-12 relative FP64 residual.  Returns the pressure
+    (zero mean over fluid cells), the projected FP64 face velocities and the
+    final relative residual."""
+    f64 = torch.float64
+    fl = (solid == 0).to(f64)
+    nb = sum(_shift(fl, a, s) for a in range(3) for s in (1, -1)) * fl
+
+    def A(x):
+        xf = x * fl
+        return fl * (nb * xf - sum(_shift(xf, a, s) for a in range(3) for s in (1, -1)))
+
+    def center(r):
+        return fl * (r - (r * fl).sum() / fl.sum())
+
+    b = center(-div.to(f64))
+    nb_ = float(torch.linalg.vector_norm(b))
+    x = torch.zeros_like(b)
+    rel = 1.0
+    for _ in range(max_refine):
+        r = center(b - A(x))
+        rel = float(torch.linalg.vector_norm(r)) / nb_
+        if rel < 1e-12:
+            break
+        solver.init(r.float().contiguous(), 1.0)
+        solver.solve(max_iters=400, tol=1e-6, tol_mode="rel")
+        x = x + solver.x.to(f64) * fl
+    x = center(x)
+    u64 = []
+    for a in range(3):
+        ua = u_bc[a].to(f64).clone()
+        n = x.shape[a]
+        # interior faces between two fluid cells: u -= p[f] - p[f - 1]; wall and
+        # obstacle faces stay zero (apply_bc after subtract_grad_p)
+        inner = ua.narrow(a, 1, n - 1)
+        both = fl.narrow(a, 1, n - 1) * fl.narrow(a, 0, n - 1)
+        inner.copy_((inner - (x.narrow(a, 1, n - 1) - x.narrow(a, 0, n - 1))) * both)
+        ua.narrow(a, 0, 1).zero_()
+        ua.narrow(a, n, 1).zero_()
+        u64.append(ua)
+    return x, u64, rel
+
+
+def serving_bench(n_jobs, rates, slots, torch):
+    """Served job stream (Poisson arrivals at each rate, 70% 2D 128^2-1024^2,
+    30% 3D 64^3-192^3, 20-200 steps, seed 3) on `slots` concurrently resident
+    jobs with the fp16-level MGPCG, against the same jobs run one at a time
+    with the plain fp32 solver."""
+    from pfm import serving
+
+    warm = [serving.Job(0, (64, 64), 3, 1), serving.Job(1, (32, 32, 32), 3, 2)]
+    serving.Server(slots=2, mg_precision=1).run(warm, realtime=False)
+    serving.Server(slots=1, mg_precision=0).run(warm, realtime=False)
+    top = max(rates)
+    jobs = serving.job_stream(n_jobs, top, seed=3)
+    base = serving.Server(slots=1, mg_precision=0).run(jobs, realtime=True).summary()
+    print(f"# serving one-at-a-time fp32: {base['jobs_per_s']:.2f} jobs/s, p50 {base['step_ms_p50']:.2f} ms "
+          f"p99 {base['step_ms_p99']:.2f} ms", file=sys.stderr, flush=True)
+    served = {}
+    for r in rates:
+        s = serving.Server(slots=slots, mg_precision=1).run(serving.job_stream(n_jobs, r, seed=3),
+                                                            realtime=True).summary()
+        s["speedup_jobs_per_s_vs_one_at_a_time"] = s["jobs_per_s"] / base["jobs_per_s"]
+        served[f"rate_{r:g}_jobs_per_s"] = s
+        print(f"# served at {r:g} jobs/s: {s['jobs_per_s']:.2f} jobs/s, p50 {s['step_ms_p50']:.2f} ms "
+              f"p99 {s['step_ms_p99']:.2f} ms", file=sys.stderr, flush=True)
+    sizes = [j.res for j in jobs]
+    return {
+        "workload": f"{n_jobs} jobs, Poisson arrivals, 70% 2D 128^2-1024^2 (4 ppc) / 30% 3D 64^3-192^3 (8 ppc), "
+                    "log-uniform sizes, 20-200 steps, vortex-blob / vortex-ring ICs, seed 3, MGPCG 1e-6 relative",
+        "jobs_2d": sum(len(r) == 2 for r in sizes), "jobs_3d": sum(len(r) == 3 for r in sizes),
+        "total_steps": sum(j.steps for j in jobs),
+        "one_at_a_time_fp32": base,
+        "served_fp16": served,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +355,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served job stream")
+    ap.add_argument("--serve-jobs", type=int, default=24)
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--profile-steps", type=int, default=2, help="instrumented steps after the timed ones")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: one plain 1 GiB copy (the counter scale reference) "
@@ -336,6 +432,18 @@ def main():
     cells = math.prod(res)
     mg_runs = {}
     p_ref = None
+    # the FP64 reference of this problem (pressure and projected velocity)
+    ref_solver = MGPCG([[2, 2]] * 3, list(res), dim=3, base_level=3, precision=0)
+    ref_solver.set_solid(solid)
+    u_bc = [x.clone() for x in u_f]
+    ref_solver.apply_bc(*u_bc)
+    ref_solver.divergence(*u_bc)
+    div = ref_solver.u_div.clone()
+    p64, u64, ref_rel = fp64_reference(ref_solver, u_bc, div, solid, torch)
+    del ref_solver, div
+    fl = solid == 0
+    n_u64 = math.sqrt(sum(float((t * t).sum()) for t in u64))
+    n_p64 = float(torch.linalg.vector_norm(p64[fl]))
     for name, prec in (("fp32", 0), ("fp16", 1), ("bf16", 2)):
         solver = MGPCG([[2, 2]] * 3, list(res), dim=3, base_level=3, precision=prec)
         solver.set_solid(solid)
@@ -353,10 +461,12 @@ def main():
             t_p += time.perf_counter() - t1
         t_p /= reps
         pr = solver.p.double()
-        fl = solid == 0
         pr = pr - pr[fl].mean()
         run = {"ms_per_solve": t_p * 1e3, "cells_per_s": cells / t_p, "iters": pits,
-               "converged": bool(solver.converged)}
+               "converged": bool(solver.converged),
+               "pressure_rel_l2_vs_fp64": float(torch.linalg.vector_norm((pr - p64)[fl])) / n_p64,
+               "velocity_rel_l2_vs_fp64": math.sqrt(sum(float(((w.double() - v) ** 2).sum())
+                                                        for w, v in zip(work, u64))) / n_u64}
         if p_ref is None:
             p_ref = pr
         else:
@@ -367,7 +477,7 @@ def main():
         del solver, work
     t_poisson = mg_runs["fp32"]["ms_per_solve"] * 1e-3
     pits = mg_runs["fp32"]["iters"]
-    del u_f
+    del u_f, u_bc, u64, p64
 
     # roofline of the step's dominant kernel, RK4 + P2G record
     # (k_rk4_particles<3, true>): per particle it reads pos 12 B, T_grad_m
@@ -408,12 +518,18 @@ def main():
         "mgpcg_iters": pits,
         "mgpcg": {"workload": f"{args.res}^3 Poisson, free-slip walls + solid box [0.4,0.6]^3, "
                               "RHS = div of 64 Fourier modes (seed 0), 1e-6 relative residual",
+                  "fp64_reference": "iterative refinement, float64 residuals, to a "
+                                    f"{ref_rel:.1e} relative residual",
                   **mg_runs},
         "phase_ms": phases,
         "roofline": {"kernel": roof_kernel, "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms": rk4_ms, "alg_bytes_per_launch": alg_bytes},
     }
+    del sim
+    torch.cuda.empty_cache()
+    if not args.no_serve:
+        out["serving"] = serving_bench(args.serve_jobs, (10.0, 50.0, 200.0), args.serve_slots, torch)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline()

@@ -847,6 +847,7 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
 constexpr int kCoarseMaxLevels = 4;
 constexpr int kCoarseLds = 14336;  // floats (56 KiB)
 constexpr int kCoarseSlots = 4;    // bottom-level cells per colour per thread
+constexpr int kCoarseThreads = 512;
 
 constexpr int kCoarseBottom64 = 4096;  // padded bottom cells of the FP64 bottom solve
 
@@ -877,69 +878,72 @@ __host__ __device__ __forceinline__ int padded_size(const int d[3]) {
     return (DIM == 3 ? d[0] + 2 : 1) * (d[1] + 2) * (d[2] + 2);
 }
 
-// smooth(l, c) (3D/mgpcg.py:165-172) on one padded LDS level; one row of the
-// level per thread.
+// smooth(l, c) (3D/mgpcg.py:165-172) on one padded LDS level; the cells of
+// colour c are spread over all threads (a 2D level has a single axis-0 row
+// plane, so per-row work would leave most threads idle).
 template <int DIM>
 __device__ void coarse_pass(const LvlDev& L, float* zs, const float* rs, int c) {
     const Padded<DIM> P(L);
     const int h2 = (L.d[2] + 1) / 2;
-    for (int row = threadIdx.x; row < L.d[0] * L.d[1]; row += blockDim.x) {
+    const int ne = L.d[0] * L.d[1] * h2;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        const int row = e / h2, pk = e - row * h2;
         const int i = row / L.d[1], j = row - i * L.d[1];
-        const int zb = P.at(i, j, 0), rb = row * L.d[2];
-        const int off = (c + i + j) & 1;
-        for (int pk = 0; pk < h2; ++pk) {
-            const int k = 2 * pk + off;
-            if (k >= L.d[2]) break;
-            const int n = L.dg ? (int)L.dg[rb + k] : L.nb2 - walls(L, i, j, k);
-            if (n == 0) continue;  // solid cell
-            zs[zb + k] = pfm_div_small(__fadd_rn(rs[rb + k], nsum_lds<DIM>(zs + zb + k, P.s0, P.s1)), (float)n,
-                                       pfm_rcp_small(n));
-        }
+        const int k = 2 * pk + ((c + i + j) & 1);
+        if (k >= L.d[2]) continue;
+        const int q = row * L.d[2] + k;
+        const int n = L.dg ? (int)L.dg[q] : L.nb2 - walls(L, i, j, k);
+        if (n == 0) continue;  // solid cell
+        const int zq = P.at(i, j, k);
+        zs[zq] = pfm_div_small(__fadd_rn(rs[q], nsum_lds<DIM>(zs + zq, P.s0, P.s1)), (float)n, pfm_rcp_small(n));
     }
 }
 
 // Residual and restriction (3D/mgpcg.py:152-158) from LDS level f to LDS
-// level c, children summed in fixed order, odd extents as in k_restrict.
+// level c, one coarse cell per thread iteration, children summed in fixed
+// order, odd extents as in k_restrict.
 template <int DIM>
 __device__ void coarse_restrict(const LvlDev& Lf, const LvlDev& Lc, const float* zs, const float* rs, float* rcs,
                                 float inv_scale) {
     const Padded<DIM> P(Lf);
-    for (int row = threadIdx.x; row < Lc.d[0] * Lc.d[1]; row += blockDim.x) {
+    const int nc = Lc.d[0] * Lc.d[1] * Lc.d[2];
+    for (int e = threadIdx.x; e < nc; e += blockDim.x) {
+        const int row = e / Lc.d[2], C2 = e - row * Lc.d[2];
         const int C0 = row / Lc.d[1], C1 = row - C0 * Lc.d[1];
         const int h0 = C0 == Lc.d[0] - 1 ? Lf.d[0] - 1 - 2 * C0 : 1;
         const int h1 = C1 == Lc.d[1] - 1 ? Lf.d[1] - 1 - 2 * C1 : 1;
-        for (int C2 = 0; C2 < Lc.d[2]; ++C2) {
-            const int h2 = C2 == Lc.d[2] - 1 ? Lf.d[2] - 1 - 2 * C2 : 1;
-            float acc = 0.0f;
-            for (int a = 0; a <= h0; ++a)
-                for (int b = 0; b <= h1; ++b)
-                    for (int c = 0; c <= h2; ++c) {
-                        const int i = 2 * C0 + a, j = 2 * C1 + b, k = 2 * C2 + c;
-                        const int zq = P.at(i, j, k);
-                        const int n = Lf.dg ? (int)Lf.dg[(i * Lf.d[1] + j) * Lf.d[2] + k] : Lf.nb2 - walls(Lf, i, j, k);
-                        if (n == 0) continue;  // solid children add nothing
-                        const float nf = (float)n;
-                        const float res = __fsub_rn(rs[(i * Lf.d[1] + j) * Lf.d[2] + k],
-                                                    __fsub_rn(__fmul_rn(nf, zs[zq]), nsum_lds<DIM>(zs + zq, P.s0, P.s1)));
-                        acc = __fadd_rn(acc, __fmul_rn(__fmul_rn(res, 1.0f), inv_scale));
-                    }
-            rcs[row * Lc.d[2] + C2] = acc;
-        }
+        const int h2 = C2 == Lc.d[2] - 1 ? Lf.d[2] - 1 - 2 * C2 : 1;
+        float acc = 0.0f;
+        for (int a = 0; a <= h0; ++a)
+            for (int b = 0; b <= h1; ++b)
+                for (int c = 0; c <= h2; ++c) {
+                    const int i = 2 * C0 + a, j = 2 * C1 + b, k = 2 * C2 + c;
+                    const int zq = P.at(i, j, k);
+                    const int fq = (i * Lf.d[1] + j) * Lf.d[2] + k;
+                    const int n = Lf.dg ? (int)Lf.dg[fq] : Lf.nb2 - walls(Lf, i, j, k);
+                    if (n == 0) continue;  // solid children add nothing
+                    const float nf = (float)n;
+                    const float res =
+                        __fsub_rn(rs[fq], __fsub_rn(__fmul_rn(nf, zs[zq]), nsum_lds<DIM>(zs + zq, P.s0, P.s1)));
+                    acc = __fadd_rn(acc, __fmul_rn(__fmul_rn(res, 1.0f), inv_scale));
+                }
+        rcs[e] = acc;
     }
 }
 
-// prolongate (3D/mgpcg.py:160-163) between padded LDS levels.
+// prolongate (3D/mgpcg.py:160-163) between padded LDS levels, one fine cell
+// per thread iteration.
 template <int DIM>
 __device__ void coarse_prolong(const LvlDev& Lf, const LvlDev& Lc, float* zs, const float* zcs) {
     const Padded<DIM> P(Lf), Pc(Lc);
-    for (int row = threadIdx.x; row < Lf.d[0] * Lf.d[1]; row += blockDim.x) {
+    const int nf = Lf.d[0] * Lf.d[1] * Lf.d[2];
+    for (int e = threadIdx.x; e < nf; e += blockDim.x) {
+        if (Lf.dg && Lf.dg[e] == 0) continue;  // solid cells carry no pressure
+        const int row = e / Lf.d[2], k = e - row * Lf.d[2];
         const int i = row / Lf.d[1], j = row - i * Lf.d[1];
-        const int c0 = min(i >> 1, Lc.d[0] - 1), c1 = min(j >> 1, Lc.d[1] - 1);
-        for (int k = 0; k < Lf.d[2]; ++k) {
-            if (Lf.dg && Lf.dg[row * Lf.d[2] + k] == 0) continue;  // solid cells carry no pressure
-            const int q = P.at(i, j, k);
-            zs[q] = __fadd_rn(zs[q], zcs[Pc.at(c0, c1, min(k >> 1, Lc.d[2] - 1))]);
-        }
+        const int c0 = min(i >> 1, Lc.d[0] - 1), c1 = min(j >> 1, Lc.d[1] - 1), c2 = min(k >> 1, Lc.d[2] - 1);
+        const int q = P.at(i, j, k);
+        zs[q] = __fadd_rn(zs[q], zcs[Pc.at(c0, c1, c2)]);
     }
 }
 
@@ -947,12 +951,12 @@ __device__ void coarse_prolong(const LvlDev& Lf, const LvlDev& Lc, float* zs, co
 // first level handled; BT: arithmetic of the bottom solve (float, or double
 // for the mixed-precision solver).
 template <int DIM, typename ZT, typename R0T, typename BT>
-__global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, int dot,
+__global__ void __launch_bounds__(kCoarseThreads) k_coarse(CoarseArgs a, int sub_mean, int dot,
                                                    const MGScalars* __restrict__ sc, double* __restrict__ partial) {
     __shared__ float sm[kCoarseLds];
     const int tid = threadIdx.x;
     for (int l = 0; l < a.nl; ++l)
-        for (int e = tid; e < padded_size<DIM>(a.L[l].d); e += kBlock) sm[a.zoff[l] + e] = 0.f;
+        for (int e = tid; e < padded_size<DIM>(a.L[l].d); e += kCoarseThreads) sm[a.zoff[l] + e] = 0.f;
     {
         const LvlDev& L0 = a.L[0];
         const int n0 = L0.d[0] * L0.d[1] * L0.d[2];
@@ -960,7 +964,7 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
         R0T* r0 = (R0T*)a.r[0];
         // 16-bit z with an fp32 first level (level 0): work on vs * r
         const float vs = (sizeof(ZT) == 2 && sizeof(R0T) == 4) ? sc->vs : 1.0f;
-        for (int q = tid; q < n0; q += kBlock) {
+        for (int q = tid; q < n0; q += kCoarseThreads) {
             float v = to_f(r0[q]);
             if (sub_mean && !(L0.dg && L0.dg[q] == 0)) {
                 v = __fsub_rn(v, mean);
@@ -993,7 +997,7 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int m = 0; m < kCoarseSlots; ++m) {
-                const int s = tid + m * kBlock;
+                const int s = tid + m * kCoarseThreads;
                 bl[c][m] = -1;
                 br[c][m] = bn[c][m] = by[c][m] = 0.f;
                 if (s < nslot) {
@@ -1024,7 +1028,7 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
             // the coarsest-level solve in double, rounded to fp32 once
             __shared__ double bz[kCoarseBottom64];
             const int np = padded_size<DIM>(L.d);
-            for (int e = tid; e < np; e += kBlock) bz[e] = 0.0;
+            for (int e = tid; e < np; e += kCoarseThreads) bz[e] = 0.0;
             __syncthreads();
             for (int it = 0; it < a.sweeps; ++it)
 #pragma unroll
@@ -1040,7 +1044,7 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
                         }
                     __syncthreads();
                 }
-            for (int e = tid; e < np; e += kBlock) zs[e] = (float)bz[e];
+            for (int e = tid; e < np; e += kCoarseThreads) zs[e] = (float)bz[e];
             __syncthreads();
         }
     }
@@ -1058,19 +1062,18 @@ __global__ void __launch_bounds__(kBlock) k_coarse(CoarseArgs a, int sub_mean, i
     for (int l = 0; l < a.nl; ++l) {
         const LvlDev& L = a.L[l];
         const Padded<DIM> P(L);
-        for (int row = tid; row < L.d[0] * L.d[1]; row += kBlock) {
+        const int n = L.d[0] * L.d[1] * L.d[2];
+        for (int q = tid; q < n; q += kCoarseThreads) {
+            const int row = q / L.d[2], k = q - row * L.d[2];
             const int i = row / L.d[1], j = row - i * L.d[1];
-            for (int k = 0; k < L.d[2]; ++k) {
-                const ZT zq = from_f<ZT>(sm[a.zoff[l] + P.at(i, j, k)]);
-                const int q = row * L.d[2] + k;
-                ((ZT*)a.z[l])[q] = zq;
-                if (l > 0) ((ZT*)a.r[l])[q] = from_f<ZT>(sm[a.roff[l] + q]);
-                if (dot && l == 0) s += (double)to_f(zq) * (double)sm[a.roff[0] + q];
-            }
+            const ZT zq = from_f<ZT>(sm[a.zoff[l] + P.at(i, j, k)]);
+            ((ZT*)a.z[l])[q] = zq;
+            if (l > 0) ((ZT*)a.r[l])[q] = from_f<ZT>(sm[a.roff[l] + q]);
+            if (dot && l == 0) s += (double)to_f(zq) * (double)sm[a.roff[0] + q];
         }
     }
     if (dot) {
-        __shared__ double sh[kBlock / 64];
+        __shared__ double sh[kCoarseThreads / 64];
         s = block_sum(s, sh);
         // both factors carry vs when the first level is scaled
         if (tid == 0) partial[0] = (sizeof(ZT) == 2 && sizeof(R0T) == 4) ? s * (double)sc->vinv * sc->vinv : s;
@@ -1674,7 +1677,7 @@ static int bottom(MGHandle* h, int b, hipStream_t s) {
 static int pick_coarse_level(const MGHandle* h) {
     const int L = h->n_levels;
     const int* db = h->lv[L - 1].d;
-    if ((int64_t)db[0] * db[1] * ((db[2] + 1) / 2) > (int64_t)kCoarseSlots * kBlock) return -1;
+    if ((int64_t)db[0] * db[1] * ((db[2] + 1) / 2) > (int64_t)kCoarseSlots * kCoarseThreads) return -1;
     const int pb = h->dim == 3 ? padded_size<3>(db) : padded_size<2>(db);
     if (h->precision != 0 && pb > kCoarseBottom64) return -1;
     int best = -1;
@@ -1706,7 +1709,7 @@ static int coarse(MGHandle* h, bool sub_mean, bool dot, hipStream_t s) {
         a.r[i] = h->r[l];
     }
 #define PFM_CO(D, R0T, BT) \
-    hipLaunchKernelGGL((k_coarse<D, ZT, R0T, BT>), dim3(1), dim3(kBlock), 0, s, a, (int)sub_mean, (int)dot, h->sc, h->partial)
+    hipLaunchKernelGGL((k_coarse<D, ZT, R0T, BT>), dim3(1), dim3(kCoarseThreads), 0, s, a, (int)sub_mean, (int)dot, h->sc, h->partial)
     if constexpr (std::is_same<ZT, float>::value) {
         if (h->dim == 3) PFM_CO(3, float, float);
         else PFM_CO(2, float, float);
