@@ -34,7 +34,7 @@ extern "C" {
 #define PFM_OK 0
 #define PFM_ERR_ARG (-1)      /* bad shape, null pointer, unsupported option */
 #define PFM_ERR_HIP (-2)      /* a HIP runtime call failed */
-#define PFM_ERR_OOM (-3)      /* device allocation failed (pfm_mg_create, pfm_mg_set_solid) */
+#define PFM_ERR_OOM (-3)      /* allocation failed (pfm_mg_create, pfm_mg_set_solid, pfm_scene_create) */
 #define PFM_WARN_NOT_CONVERGED 1 /* MGPCG hit max_iters (the reference prints and returns) */
 
 typedef void* pfm_stream_t; /* hipStream_t */
