@@ -204,6 +204,22 @@ int pfm_vortex_ring(const pfm_grid* g, float w, float rad, float delta, const fl
 /* Kinetic energy (2D/run_2D.py:765-770, commented there): sum_a sum u_a^2 /
  * |F_a| accumulated in double into out[0].  `out` is a device buffer of
  * 1 + 3*1024 doubles; out[1..] holds the per-block partial sums. */
+/* advect_smoke (3D/run_3D.py:315-323, 2D/run_2D.py:444-450) and clamp_smoke
+ * (3D/run_3D.py:325-334): cell-centred smoke (channels x cells, channel-major)
+ * sampled with interp_1 at the mean backward map of each cell's faces
+ * (psi_x/y/z as pfm_rk4_grid's psi, (dim, face shape) each).  mode 1 writes
+ * smoke1 = interp_1(smoke0, psi_c); 2 clamps smoke1 in place to the range of
+ * smoke0's 2^d samples around psi_c (3D only); 3 does both. */
+int pfm_advect_smoke(const pfm_grid* g, int channels, const float* smoke0, float* smoke1,
+                     const float* psi_x, const float* psi_y, const float* psi_z, int mode,
+                     pfm_stream_t stream);
+/* The smoke part of add_vortex_ring_and_smoke (3D/init_conditions.py:31-56):
+ * adds the ring's density into channel 3 of smoke (4 x cells), then sets every
+ * cell to (color, 1) where channel 3 > 0.002 and to 0 elsewhere.  The velocity
+ * part is pfm_vortex_ring. */
+int pfm_vortex_ring_smoke(const pfm_grid* g, float rad, float delta, const float* c, const float* ex,
+                          const float* ey, int num_samples, const float* color, float* smoke,
+                          pfm_stream_t stream);
 int pfm_kinetic_energy(const pfm_grid* g, const float* u_x, const float* u_y, const float* u_z,
                        double* out, pfm_stream_t stream);
 

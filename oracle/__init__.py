@@ -76,6 +76,8 @@ def lib():
         L.oracle_curl.argtypes = [ci, I32P, fl, vp, vp]
         L.oracle_point_vortices.argtypes = [I32P, fl, ci, F32P, F32P, F32P, fl, fl, vp]
         L.oracle_vortex_ring.argtypes = [I32P, fl, fl, fl, fl, F32P, F32P, F32P, ci, vp]
+        L.oracle_advect_smoke.argtypes = [ci, I32P, fl, ci, vp, vp, vp, vp, vp, ci]
+        L.oracle_vortex_ring_smoke.argtypes = [I32P, fl, fl, fl, F32P, F32P, F32P, ci, F32P, vp]
         L.oracle_kinetic_energy.restype = ctypes.c_double
         L.oracle_kinetic_energy.argtypes = [ci, I32P, vp, vp, vp]
         for pre in ("oracle_mg_", "oracle_mg64_"):
@@ -280,6 +282,24 @@ def vortex_ring(res, dx, vc, w, rad, delta, c, ex, ey, num_samples=2000):
     lib().oracle_vortex_ring(_res(res), dx, w, rad, delta, np.asarray(c, np.float32),
                              np.asarray(ex, np.float32), np.asarray(ey, np.float32),
                              num_samples, _p(vc))
+
+
+def advect_smoke(res, dx, smoke0, psi, smoke1=None, mode=1):
+    """advect_smoke / clamp_smoke (3D/run_3D.py:315-334, 2D/run_2D.py:444-450).
+    smoke0: (nch, *res); psi: list of d arrays (d, *face_shape(a)); mode 1
+    advects into a new array (or smoke1), 2 clamps smoke1 in place, 3 both."""
+    d = len(res)
+    s0 = np.ascontiguousarray(smoke0, np.float32)
+    s1 = np.empty_like(s0) if smoke1 is None else smoke1
+    ps = [np.ascontiguousarray(p, np.float32) for p in psi] + [None] * (3 - d)
+    lib().oracle_advect_smoke(d, _res(res), dx, s0.shape[0], _p(s0), _p(s1), *[_p(p) for p in ps], mode)
+    return s1
+
+
+def vortex_ring_smoke(res, dx, smoke, rad, delta, c, ex, ey, color, num_samples=2000):
+    lib().oracle_vortex_ring_smoke(_res(res), dx, rad, delta, np.asarray(c, np.float32),
+                                   np.asarray(ex, np.float32), np.asarray(ey, np.float32), num_samples,
+                                   np.asarray(color, np.float32), _p(smoke))
 
 
 def kinetic_energy(res, u):

@@ -748,6 +748,143 @@ void oracle_vortex_ring(const int* res, float dx, float w, float rad, float delt
     }
 }
 
+/* ---- (f)4 smoke transport on the grid.
+ * advect_smoke: 3D/run_3D.py:315-323 (psi_c = 1./6 * (sum of the 6 face maps,
+ * left to right)), 2D/run_2D.py:444-450 (0.25 * sum of 4); interp_1:
+ * 3D/taichi_utils.py:193-220 (s = max(1, min(u, dim-1-eps)), 8 samples in
+ * (i, j, k) order, value * N_1 * N_1 * N_1 accumulated from 0 * sample) and
+ * 2D/taichi_utils.py:119-137 (s = max(0, ...), a N N + b N N + c N N + d N N);
+ * clamp_smoke: 3D/run_3D.py:325-334 with sample_min_max_1
+ * (3D/taichi_utils.py:222-247).  p / dx is evaluated as p * (1 / dx), as
+ * everywhere in this restatement.  mode bit 0 advects, bit 1 clamps. */
+static inline float hat1(float x) { return 1.0f - fabsf(x); }
+
+void oracle_advect_smoke(int dim, const int* res, float dx, int nch, const float* s0, float* s1,
+                         const float* px, const float* py, const float* pz, int mode) {
+    grid_t g;
+    make_grid(&g, dim, res, dx);
+    const int rx = g.res[0], ry = g.res[1], rz = g.res[2];
+    const int64_t C = (int64_t)rx * ry * rz;
+    const int shp[3] = {rx, ry, rz};
+    float hi[3];
+    for (int a = 0; a < 3; ++a) hi[a] = (float)((double)shp[a] - 1.0 - ORACLE_EPS);
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < C; ++q) {
+        float pc[3] = {0.f, 0.f, 0.f};
+        if (dim == 3) {
+            int k = (int)(q % rz), j = (int)((q / rz) % ry), i = (int)(q / ((int64_t)rz * ry));
+            int64_t Fx = (int64_t)(rx + 1) * ry * rz, Fy = (int64_t)rx * (ry + 1) * rz, Fz = (int64_t)rx * ry * (rz + 1);
+            for (int c = 0; c < 3; ++c) {
+                float sum = px[c * Fx + ((int64_t)i * ry + j) * rz + k];
+                sum = sum + px[c * Fx + ((int64_t)(i + 1) * ry + j) * rz + k];
+                sum = sum + py[c * Fy + ((int64_t)i * (ry + 1) + j) * rz + k];
+                sum = sum + py[c * Fy + ((int64_t)i * (ry + 1) + j + 1) * rz + k];
+                sum = sum + pz[c * Fz + ((int64_t)i * ry + j) * (rz + 1) + k];
+                sum = sum + pz[c * Fz + ((int64_t)i * ry + j) * (rz + 1) + k + 1];
+                pc[c] = (float)(1.0 / 6.0) * sum;
+            }
+        } else {
+            int j = (int)(q % ry), i = (int)(q / ry);
+            int64_t Fx = (int64_t)(rx + 1) * ry, Fy = (int64_t)rx * (ry + 1);
+            for (int c = 0; c < 2; ++c) {
+                float sum = px[c * Fx + (int64_t)i * ry + j];
+                sum = sum + px[c * Fx + (int64_t)(i + 1) * ry + j];
+                sum = sum + py[c * Fy + (int64_t)i * (ry + 1) + j];
+                sum = sum + py[c * Fy + (int64_t)i * (ry + 1) + j + 1];
+                pc[c] = 0.25f * sum;
+            }
+        }
+        float sv[3] = {0.f, 0.f, 0.f}, fl[3] = {0.f, 0.f, 0.f};
+        for (int c = 0; c < dim; ++c) {
+            float u = pc[c] * g.inv_dx - 0.5f;
+            float lo = dim == 3 ? 1.0f : 0.0f;
+            float m = u < hi[c] ? u : hi[c];
+            sv[c] = m > lo ? m : lo;
+            fl[c] = floorf(sv[c]);
+        }
+        for (int ch = 0; ch < nch; ++ch) {
+            const float* f = s0 + (int64_t)ch * C;
+            if (dim == 3) {
+                const int i0 = (int)fl[0], j0 = (int)fl[1], k0 = (int)fl[2];
+                float base = sample3(f, shp, i0, j0, k0);
+                if (mode & 1) {
+                    float acc = 0.0f * base;
+                    for (int i = 0; i < 2; ++i)
+                        for (int j = 0; j < 2; ++j)
+                            for (int k = 0; k < 2; ++k) {
+                                float xi = sv[0] - (fl[0] + (float)i);
+                                float yj = sv[1] - (fl[1] + (float)j);
+                                float zk = sv[2] - (fl[2] + (float)k);
+                                float v = sample3(f, shp, (int)(fl[0] + (float)i), (int)(fl[1] + (float)j),
+                                                  (int)(fl[2] + (float)k));
+                                acc = acc + v * hat1(xi) * hat1(yj) * hat1(zk);
+                            }
+                    s1[(int64_t)ch * C + q] = acc;
+                }
+                if (mode & 2) {
+                    float mn = base, mx = base;
+                    for (int i = 0; i < 2; ++i)
+                        for (int j = 0; j < 2; ++j)
+                            for (int k = 0; k < 2; ++k) {
+                                float v = sample3(f, shp, (int)(fl[0] + (float)i), (int)(fl[1] + (float)j),
+                                                  (int)(fl[2] + (float)k));
+                                mn = v < mn ? v : mn;
+                                mx = v > mx ? v : mx;
+                            }
+                    float cur = s1[(int64_t)ch * C + q];
+                    cur = cur > mn ? cur : mn;
+                    s1[(int64_t)ch * C + q] = cur < mx ? cur : mx;
+                }
+            } else {
+                int shp2[3] = {rx, ry, 1};
+                float fu = sv[0] - fl[0], fv = sv[1] - fl[1];
+                float a = sample3(f, shp2, (int)fl[0], (int)fl[1], 0);
+                float b = sample3(f, shp2, (int)(fl[0] + 1.0f), (int)fl[1], 0);
+                float c = sample3(f, shp2, (int)fl[0], (int)(fl[1] + 1.0f), 0);
+                float d = sample3(f, shp2, (int)(fl[0] + 1.0f), (int)(fl[1] + 1.0f), 0);
+                s1[(int64_t)ch * C + q] = a * hat1(fu) * hat1(fv) + b * hat1(fu - 1.0f) * hat1(fv) +
+                                          c * hat1(fu) * hat1(fv - 1.0f) + d * hat1(fu - 1.0f) * hat1(fv - 1.0f);
+            }
+        }
+    }
+}
+
+/* ---- the smoke part of add_vortex_ring_and_smoke (3D/init_conditions.py:31-56):
+ * channel 3 += curve_length * exp(-(r/delta)^3) per ring sample, then every
+ * cell becomes (color, 1) if channel 3 > 0.002, else 0.  smoke is 4 x C. */
+void oracle_vortex_ring_smoke(const int* res, float dx, float rad, float delta, const float* c,
+                              const float* ex, const float* ey, int num_samples, const float* color,
+                              float* smoke) {
+    int64_t C = (int64_t)res[0] * res[1] * res[2];
+    float curve_length = (float)((2.0 * M_PI * (double)rad) / num_samples);
+    float pi_f = (float)M_PI;
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < C; ++q) {
+        int k = (int)(q % res[2]);
+        int j = (int)((q / res[2]) % res[1]);
+        int i = (int)(q / ((int64_t)res[1] * res[2]));
+        float P[3] = {((float)i + 0.5f) * dx, ((float)j + 0.5f) * dx, ((float)k + 0.5f) * dx};
+        float acc = smoke[3 * C + q];
+        for (int l = 0; l < num_samples; ++l) {
+            float theta = (float)l / (float)num_samples * 2.0f * pi_f;
+            float ct = cosf(theta), st = sinf(theta);
+            float pd[3];
+            for (int a = 0; a < 3; ++a) pd[a] = P[a] - (rad * (ct * ex[a] + st * ey[a]) + c[a]);
+            float r = sqrtf(pd[0] * pd[0] + pd[1] * pd[1] + pd[2] * pd[2]);
+            float rd = r / delta;
+            acc += curve_length * expf(-(rd * rd * rd));
+        }
+        smoke[3 * C + q] = acc;
+    }
+    for (int64_t q = 0; q < C; ++q) {
+        int on = smoke[3 * C + q] > 0.002f;
+        smoke[q] = on ? color[0] : 0.f;
+        smoke[C + q] = on ? color[1] : 0.f;
+        smoke[2 * C + q] = on ? color[2] : 0.f;
+        smoke[3 * C + q] = on ? 1.0f : 0.f;
+    }
+}
+
 /* ---- 32. kinetic energy (2D/run_2D.py:765-770): sum u_a^2 / |F_a|, in double */
 double oracle_kinetic_energy(int dim, const int* res, const float* ux, const float* uy,
                              const float* uz) {

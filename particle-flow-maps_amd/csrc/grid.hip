@@ -250,6 +250,101 @@ __global__ void __launch_bounds__(kBlock) k_curl(GridDev g, const float* __restr
     }
 }
 
+// Smoke transport on the grid: advect_smoke (3D/run_3D.py:315-323,
+// 2D/run_2D.py:444-450) and clamp_smoke (3D/run_3D.py:325-334).  The cell's
+// backward-map point is the mean of the map on its 2d faces (psi_c), the
+// smoke is interpolated there with interp_1 (3D/taichi_utils.py:193-220,
+// 2D/taichi_utils.py:119-137: linear hat weights, the clamp s = max(lo,
+// min(u, dim-1-eps)) with lo = 1 in 3D and 0 in 2D) and, for clamp_smoke,
+// clamped to the range of the 2^d samples around that point
+// (sample_min_max_1, 3D/taichi_utils.py:222-247).  Smoke fields are
+// channel-major (nch, cells).  mode bit 0: advect into s1; bit 1: clamp s1.
+// No contraction: every operation rounds as in the reference.
+struct SmokeArgs {
+    int nch, mode;
+    float hi[3];     // float(extent - 1 - eps) per axis
+    float lo;        // 1 (3D) or 0 (2D)
+    float third;     // the psi_c weight: float(1/6) in 3D, 0.25 in 2D
+};
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_advect_smoke(GridDev g, SmokeArgs a, const float* __restrict__ s0,
+                                                         float* __restrict__ s1, const float* __restrict__ px,
+                                                         const float* __restrict__ py,
+                                                         const float* __restrict__ pz) {
+#pragma clang fp contract(off)
+    const int64_t C = (int64_t)g.res[0] * g.res[1] * g.res[2];
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= C) return;
+    const int rx = g.res[0], ry = g.res[1], rz = g.res[2];
+    float pc[3] = {0.f, 0.f, 0.f};
+    if (DIM == 3) {
+        const int k = (int)(q % rz), j = (int)((q / rz) % ry), i = (int)(q / ((int64_t)rz * ry));
+        const int64_t Fx = (int64_t)(rx + 1) * ry * rz, Fy = (int64_t)rx * (ry + 1) * rz, Fz = (int64_t)rx * ry * (rz + 1);
+        const int64_t x0 = ((int64_t)i * ry + j) * rz + k, x1 = x0 + (int64_t)ry * rz;
+        const int64_t y0 = ((int64_t)i * (ry + 1) + j) * rz + k, y1 = y0 + rz;
+        const int64_t z0 = ((int64_t)i * ry + j) * (rz + 1) + k, z1 = z0 + 1;
+        for (int c = 0; c < 3; ++c)
+            pc[c] = a.third * (((((px[c * Fx + x0] + px[c * Fx + x1]) + py[c * Fy + y0]) + py[c * Fy + y1]) +
+                                pz[c * Fz + z0]) + pz[c * Fz + z1]);
+    } else {
+        // 2D cell (i, j); u_x faces (rx + 1, ry), u_y faces (rx, ry + 1)
+        const int j = (int)(q % ry), i = (int)(q / ry);
+        const int64_t Fx = (int64_t)(rx + 1) * ry, Fy = (int64_t)rx * (ry + 1);
+        const int64_t x0 = (int64_t)i * ry + j, x1 = x0 + ry;
+        const int64_t y0 = (int64_t)i * (ry + 1) + j, y1 = y0 + 1;
+        for (int c = 0; c < 2; ++c)
+            pc[c] = a.third * (((px[c * Fx + x0] + px[c * Fx + x1]) + py[c * Fy + y0]) + py[c * Fy + y1]);
+    }
+    const int ext[3] = {rx, ry, rz};
+    float sv[3], fl[3];
+    for (int c = 0; c < DIM; ++c) {
+        const float u = pc[c] * g.inv_dx - 0.5f;
+        sv[c] = fmaxf(a.lo, fminf(u, a.hi[c]));
+        fl[c] = floorf(sv[c]);
+    }
+    // sample(): int() of the float index, clamped into the field
+    auto idx = [&](int c, int o) { return min(max((int)(fl[c] + (float)o), 0), ext[c] - 1); };
+    auto hat = [](float x) { return 1.0f - fabsf(x); };
+    if (DIM == 3) {
+        int64_t off[8];
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int k = 0; k < 2; ++k) off[(i * 2 + j) * 2 + k] = ((int64_t)idx(0, i) * ry + idx(1, j)) * rz + idx(2, k);
+        const float nx[2] = {hat(sv[0] - (fl[0] + 0.0f)), hat(sv[0] - (fl[0] + 1.0f))};
+        const float ny[2] = {hat(sv[1] - (fl[1] + 0.0f)), hat(sv[1] - (fl[1] + 1.0f))};
+        const float nz[2] = {hat(sv[2] - (fl[2] + 0.0f)), hat(sv[2] - (fl[2] + 1.0f))};
+        for (int ch = 0; ch < a.nch; ++ch) {
+            const float* f = s0 + (int64_t)ch * C;
+            const float v0 = f[off[0]];
+            if (a.mode & 1) {
+                float acc = 0.0f * v0;
+                for (int e = 0; e < 8; ++e) acc = acc + ((f[off[e]] * nx[e >> 2]) * ny[(e >> 1) & 1]) * nz[e & 1];
+                s1[(int64_t)ch * C + q] = acc;
+            }
+            if (a.mode & 2) {
+                float mn = v0, mx = v0;
+                for (int e = 0; e < 8; ++e) {
+                    mn = fminf(mn, f[off[e]]);
+                    mx = fmaxf(mx, f[off[e]]);
+                }
+                const float cur = s1[(int64_t)ch * C + q];
+                s1[(int64_t)ch * C + q] = fminf(fmaxf(cur, mn), mx);
+            }
+        }
+    } else {
+        const int64_t oa = (int64_t)idx(0, 0) * ry + idx(1, 0), ob = (int64_t)idx(0, 1) * ry + idx(1, 0);
+        const int64_t oc = (int64_t)idx(0, 0) * ry + idx(1, 1), od = (int64_t)idx(0, 1) * ry + idx(1, 1);
+        const float fu = sv[0] - fl[0], fv = sv[1] - fl[1];
+        const float nu0 = hat(fu), nu1 = hat(fu - 1.0f), nv0 = hat(fv), nv1 = hat(fv - 1.0f);
+        for (int ch = 0; ch < a.nch; ++ch) {
+            const float* f = s0 + (int64_t)ch * C;
+            s1[(int64_t)ch * C + q] =
+                (((f[oa] * nu0) * nv0 + (f[ob] * nu1) * nv0) + (f[oc] * nu0) * nv1) + (f[od] * nu1) * nv1;
+        }
+    }
+}
+
 // Point vortices: angular_vel_func + leapfrog_vel_func, 2D/init_conditions.py:5-9,29-60.
 // The vortices travel by value in the kernel arguments, kVortChunk per
 // launch; a launch after the first continues the running sums stored in vc,
@@ -331,6 +426,49 @@ __global__ void __launch_bounds__(kBlock) k_vortex_ring(GridDev g, RingArgs a, f
     vc[q] = acc[0];
     vc[C + q] = acc[1];
     vc[2 * C + q] = acc[2];
+}
+
+// The smoke part of add_vortex_ring_and_smoke (3D/init_conditions.py:31-56):
+// channel 3 accumulates curve_length * exp(-(r/delta)^3) over the ring's
+// samples (the velocity part is k_vortex_ring); k_smoke_threshold then sets
+// every cell to (color, 1) where channel 3 exceeds 0.002 and to 0 elsewhere.
+__global__ void __launch_bounds__(kBlock) k_ring_smoke(GridDev g, RingArgs a, float* __restrict__ smoke) {
+    const int64_t C = (int64_t)g.res[0] * g.res[1] * g.res[2];
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= C) return;
+    const int k = (int)(q % g.res[2]);
+    const int j = (int)((q / g.res[2]) % g.res[1]);
+    const int i = (int)(q / ((int64_t)g.res[2] * g.res[1]));
+    const float P[3] = {__fmul_rn(__fadd_rn((float)i, 0.5f), g.dx),
+                        __fmul_rn(__fadd_rn((float)j, 0.5f), g.dx),
+                        __fmul_rn(__fadd_rn((float)k, 0.5f), g.dx)};
+    float acc = smoke[3 * C + q];
+    for (int l = 0; l < a.num_samples; ++l) {
+        float theta = __fmul_rn(__fmul_rn(__fdiv_rn((float)l, (float)a.num_samples), 2.0f), a.pi);
+        float ct = cosf(theta), st = sinf(theta);
+        float pd[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float ps = __fadd_rn(__fmul_rn(a.rad, __fadd_rn(__fmul_rn(ct, a.ex[c]), __fmul_rn(st, a.ey[c]))), a.c[c]);
+            pd[c] = __fsub_rn(P[c], ps);
+        }
+        float r = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(pd[0], pd[0]), __fmul_rn(pd[1], pd[1])), __fmul_rn(pd[2], pd[2])));
+        float rd = __fdiv_rn(r, a.delta);
+        float rd3 = __fmul_rn(__fmul_rn(rd, rd), rd);
+        acc = __fadd_rn(acc, __fmul_rn(a.curve_length, expf(-rd3)));
+    }
+    smoke[3 * C + q] = acc;
+}
+
+__global__ void __launch_bounds__(kBlock) k_smoke_threshold(int64_t C, float c0, float c1, float c2,
+                                                            float* __restrict__ smoke) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= C) return;
+    const bool on = smoke[3 * C + q] > 0.002f;
+    smoke[q] = on ? c0 : 0.f;
+    smoke[C + q] = on ? c1 : 0.f;
+    smoke[2 * C + q] = on ? c2 : 0.f;
+    smoke[3 * C + q] = on ? 1.0f : 0.f;
 }
 
 // Kinetic energy (2D/run_2D.py:765-770): per-axis double sums of u^2,
@@ -566,6 +704,61 @@ int pfm_vortex_ring(const pfm_grid* gp, float w, float rad, float delta, const f
     a.num_samples = num_samples;
     const int64_t C = cell_count(g);
     hipLaunchKernelGGL(k_vortex_ring, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), g, a, vc);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_advect_smoke(const pfm_grid* gp, int channels, const float* smoke0, float* smoke1, const float* psi_x,
+                     const float* psi_y, const float* psi_z, int mode, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(channels >= 1 && channels <= 4, "channels must be 1..4");
+    PFM_CHECK_ARG(mode >= 1 && mode <= 3, "mode must be 1 (advect), 2 (clamp) or 3 (both)");
+    PFM_CHECK_ARG(g.dim == 3 || mode == 1, "clamp_smoke is 3D only");
+    PFM_CHECK_ARG(smoke0 && smoke1 && psi_x && psi_y && (g.dim == 2 || psi_z), "null pointer");
+    PFM_CHECK_ARG(smoke0 != smoke1 || mode == 2, "advect_smoke needs distinct input and output fields");
+    SmokeArgs a;
+    a.nch = channels;
+    a.mode = mode;
+    for (int c = 0; c < 3; ++c) a.hi[c] = (float)((double)g.res[c] - 1.0 - 1e-6);
+    a.lo = g.dim == 3 ? 1.0f : 0.0f;
+    a.third = g.dim == 3 ? (float)(1.0 / 6.0) : 0.25f;
+    const int64_t C = cell_count(g);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_advect_smoke<3>, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), g, a, smoke0,
+                           smoke1, psi_x, psi_y, psi_z);
+    else
+        hipLaunchKernelGGL(k_advect_smoke<2>, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), g, a, smoke0,
+                           smoke1, psi_x, psi_y, psi_z);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_vortex_ring_smoke(const pfm_grid* gp, float rad, float delta, const float* c, const float* ex,
+                          const float* ey, int num_samples, const float* color, float* smoke, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(g.dim == 3, "vortex rings are 3D");
+    PFM_CHECK_ARG(num_samples >= 1 && c && ex && ey && color && smoke, "bad arguments");
+    RingArgs a = {};
+    a.rad = rad;
+    a.delta = delta;
+    a.curve_length = (float)((2.0 * M_PI * (double)rad) / num_samples);
+    a.pi = (float)M_PI;
+    for (int q = 0; q < 3; ++q) {
+        a.c[q] = c[q];
+        a.ex[q] = ex[q];
+        a.ey[q] = ey[q];
+    }
+    a.num_samples = num_samples;
+    const int64_t C = cell_count(g);
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_ring_smoke, dim3(grid_for(C)), dim3(kBlock), 0, s, g, a, smoke);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_smoke_threshold, dim3(grid_for(C)), dim3(kBlock), 0, s, C, color[0], color[1], color[2],
+                       smoke);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

@@ -91,6 +91,8 @@ _SIGS = {
     "pfm_point_vortices": [c_vp, c_int, c_vp, c_vp, c_vp, c_f, c_f, c_vp, c_vp],
     "pfm_vortex_ring": [c_vp, c_f, c_f, c_f, c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
     "pfm_kinetic_energy": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "pfm_advect_smoke": [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp],
+    "pfm_vortex_ring_smoke": [c_vp, c_f, c_f, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp],
     "pfm_mg_create": [c_int, c_vp, c_int, c_vp, c_int, ctypes.POINTER(c_vp)],
     "pfm_mg_destroy": [c_vp],
     "pfm_mg_n_levels": [c_vp],

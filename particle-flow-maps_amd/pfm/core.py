@@ -317,6 +317,42 @@ def get_central(res, u, vc, stream=None):
     L.call("pfm_get_central", ctypes.byref(g), *_u_ptrs(u, res), L.ptr(vc, name="vc"), _stream(stream))
 
 
+def advect_smoke(res, dx, smoke0, smoke1, psi, mode=1, stream=None):
+    """advect_smoke / clamp_smoke on the grid (3D/run_3D.py:315-334,
+    2D/run_2D.py:444-450): smoke (nch, *res) channel-major, psi the d face maps
+    (d, *face_shape(a)).  mode 1: smoke1 = interp_1(smoke0, psi_c); 2: clamp
+    smoke1 in place to smoke0's range around psi_c (3D); 3: both."""
+    d = len(res)
+    nch = smoke0.shape[0]
+    for name, t in (("smoke0", smoke0), ("smoke1", smoke1)):
+        if tuple(t.shape) != (nch,) + tuple(res):
+            raise L.PfmError(f"{name} must be ({nch}, *{tuple(res)}), got {tuple(t.shape)}")
+    pp = []
+    for a in range(3):
+        if a < d:
+            if tuple(psi[a].shape) != (d,) + face_shape(res, a):
+                raise L.PfmError(f"psi[{a}] must be ({d}, *{face_shape(res, a)}), got {tuple(psi[a].shape)}")
+            pp.append(L.ptr(psi[a], name=f"psi[{a}]"))
+        else:
+            pp.append(None)
+    g = L.make_grid(res, dx)
+    L.call("pfm_advect_smoke", ctypes.byref(g), int(nch), L.ptr(smoke0, name="smoke0"),
+           L.ptr(smoke1, name="smoke1"), *pp, int(mode), _stream(stream))
+
+
+def vortex_ring_smoke(res, dx, smoke, rad, delta, c, ex, ey, color, num_samples=2000, stream=None):
+    """The smoke part of add_vortex_ring_and_smoke (3D/init_conditions.py:31-56)
+    on smoke (4, *res)."""
+    import numpy as np
+
+    if tuple(smoke.shape) != (4,) + tuple(res):
+        raise L.PfmError(f"smoke must be (4, *{tuple(res)}), got {tuple(smoke.shape)}")
+    arr = [np.ascontiguousarray(np.asarray(v, np.float32)) for v in (c, ex, ey, color)]
+    g = L.make_grid(res, dx)
+    L.call("pfm_vortex_ring_smoke", ctypes.byref(g), float(rad), float(delta), *[v.ctypes.data for v in arr[:3]],
+           int(num_samples), arr[3].ctypes.data, L.ptr(smoke, name="smoke"), _stream(stream))
+
+
 def curl(res, dx, vc, out, stream=None):
     """curl (3D/taichi_utils.py:56-78; 2D/taichi_utils.py:97-103) of the
     cell-centred vector vc (d, *res): out is (3, *res) in 3D, res in 2D."""

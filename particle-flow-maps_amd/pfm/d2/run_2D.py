@@ -78,6 +78,11 @@ def advect_u(u_x0, u_y0, u_x1, u_y1, T_x, T_y, psi_x, psi_y, dx):
     core.advect_u(res, dx, (u_x0, u_y0), (u_x1, u_y1), (T_x, T_y), (psi_x, psi_y))
 
 
+def advect_smoke(smoke0, smoke1, psi_x, psi_y, dx):
+    """2D/run_2D.py:444-450 (smoke as (3, rx, ry), psi as (2, face shape))."""
+    core.advect_smoke(tuple(smoke0.shape[1:]), dx, smoke0, smoke1, (psi_x, psi_y), mode=1)
+
+
 def update_particles_imp(particles_imp, particles_init_imp, T_x, T_y):
     """2D/run_2D.py:517-524."""
     core.update_particles_imp((T_x, T_y), particles_init_imp, particles_imp)

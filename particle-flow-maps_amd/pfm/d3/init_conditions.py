@@ -54,6 +54,17 @@ def add_vortex_ring(w, rad, delta, c, unit_x, unit_y, pf, vf, num_samples, *, dx
                      num_samples)
 
 
+def add_vortex_ring_and_smoke(w, rad, delta, c, unit_x, unit_y, pf, vf, smokef, color, num_samples, *,
+                              dx=None):
+    """3D/init_conditions.py:31-56: the ring's velocity into ``vf`` (3, *res)
+    as add_vortex_ring, its smoke into ``smokef`` (4, *res): density into
+    channel 3, then (color, 1) where it exceeds 0.002 and 0 elsewhere."""
+    res = tuple(vf.shape[1:])
+    d = core.default_dx(res) if dx is None else dx
+    core.vortex_ring(res, d, vf, w, rad, delta, c, unit_x, unit_y, num_samples)
+    core.vortex_ring_smoke(res, d, smokef, rad, delta, c, unit_x, unit_y, color, num_samples)
+
+
 def coaxial_rings(res, rings, num_samples=2000, device="cuda"):
     """Cell-centred velocity of several rings (list of dicts with w, rad,
     delta, c, unit_x, unit_y) summed as the reference's init_vorts_* do

@@ -78,6 +78,16 @@ def advect_u(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, T_x, T_y, T_z, psi_x, psi_y, ps
     core.advect_u(res, dx, (u_x0, u_y0, u_z0), (u_x1, u_y1, u_z1), (T_x, T_y, T_z), (psi_x, psi_y, psi_z))
 
 
+def advect_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx):
+    """3D/run_3D.py:315-323 (smoke as (4, rx, ry, rz), psi as (3, face shape))."""
+    core.advect_smoke(tuple(smoke0.shape[1:]), dx, smoke0, smoke1, (psi_x, psi_y, psi_z), mode=1)
+
+
+def clamp_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx):
+    """3D/run_3D.py:325-334."""
+    core.advect_smoke(tuple(smoke0.shape[1:]), dx, smoke0, smoke1, (psi_x, psi_y, psi_z), mode=2)
+
+
 def get_particles_id_in_every_cell(cell_particles_id, cell_particle_num, particles_pos, *, dx=None):
     """3D/run_3D.py:342-351, deterministic: ids sorted ascending inside a cell."""
     res = tuple(cell_particle_num.shape)

@@ -564,3 +564,98 @@ def test_mg_solid_projection(oracle):
     u0 = smooth_mac_field(res, seed=43)
     div0 = (u0[0][1:] - u0[0][:-1]) + (u0[1][:, 1:] - u0[1][:, :-1]) + (u0[2][:, :, 1:] - u0[2][:, :, :-1])
     assert np.abs(div[~solid]).max() < 1e-4 * np.abs(div0).max()
+
+
+# ---- smoke transport on the grid (advect_smoke / clamp_smoke) -------------
+def _identity_psi(res, dx, shift=0.0):
+    return [(face_coords(res, a, dx) + shift).astype(np.float32) for a in range(len(res))]
+
+
+def _cell_centres(res, dx):
+    g = np.meshgrid(*[(np.arange(n) + 0.5) * dx for n in res], indexing="ij")
+    return np.stack(g)
+
+
+@pytest.mark.parametrize("res", [(9, 8, 7), (12, 10)])
+def test_advect_smoke_identity_and_linear(oracle, res):
+    """interp_1 at the mean of the face maps: the identity map returns the
+    field (the 3D clamp s >= 1 makes the first cell of each axis read its
+    neighbour), and a uniform shift of a linear field is exact inside."""
+    d = len(res)
+    dx = 1.0 / res[1]
+    rng = np.random.default_rng(7)
+    s0 = rng.uniform(0, 1, size=(3,) + res).astype(np.float32)
+    s1 = oracle.advect_smoke(res, dx, s0, _identity_psi(res, dx))
+    lo = 1 if d == 3 else 0
+    inner = tuple(slice(lo, None) for _ in range(d))
+    np.testing.assert_allclose(s1[(slice(None),) + inner], s0[(slice(None),) + inner], rtol=0, atol=2e-6)
+    if d == 3:
+        np.testing.assert_allclose(s1[:, 0, 2:, 2:], s0[:, 1, 2:, 2:], rtol=0, atol=2e-6)
+    X = _cell_centres(res, dx)
+    coef = np.array([0.3, -0.7, 0.45][:d])
+    lin = (0.2 + np.tensordot(coef, X, axes=1)).astype(np.float32)[None]
+    delta = 0.3 * dx
+    s1 = oracle.advect_smoke(res, dx, lin, _identity_psi(res, dx, delta))
+    want = 0.2 + np.tensordot(coef, X + delta, axes=1)
+    core = tuple(slice(lo, n - 2) for n in res)
+    np.testing.assert_allclose(s1[0][core], want[core], rtol=0, atol=5e-6)
+
+
+def test_clamp_smoke_is_the_range_of_the_eight_samples(oracle):
+    res = (8, 9, 10)
+    dx = 1.0 / res[1]
+    rng = np.random.default_rng(8)
+    s0 = rng.uniform(0, 1, size=(2,) + res).astype(np.float32)
+    psi = [(p + rng.uniform(-0.6, 0.6, size=p.shape[1:]).astype(np.float32) * dx).astype(np.float32)
+           for p in _identity_psi(res, dx)]
+    s1 = rng.uniform(-1, 2, size=s0.shape).astype(np.float32)
+    out = oracle.advect_smoke(res, dx, s0, psi, smoke1=s1.copy(), mode=2)
+    # numpy: the same psi_c and floor, then min / max over the 2^3 samples
+    f32 = np.float32
+    sx = psi[0][:, :-1] + psi[0][:, 1:]
+    sy = sx + psi[1][:, :, :-1]
+    sy = sy + psi[1][:, :, 1:]
+    sz = sy + psi[2][:, :, :, :-1]
+    sz = sz + psi[2][:, :, :, 1:]
+    pc = f32(1.0 / 6.0) * sz
+    u = pc * f32(1.0 / dx) - f32(0.5)
+    hi = np.array([f32(n - 1 - 1e-6) for n in res], np.float32).reshape(3, 1, 1, 1)
+    s = np.maximum(f32(1.0), np.minimum(u, hi))
+    fl = np.floor(s).astype(np.int64)
+    mn = np.full(s0.shape, np.inf, np.float32)
+    mx = np.full(s0.shape, -np.inf, np.float32)
+    for o in np.ndindex(2, 2, 2):
+        idx = [np.clip(fl[a] + o[a], 0, res[a] - 1) for a in range(3)]
+        v = s0[:, idx[0], idx[1], idx[2]]
+        mn, mx = np.minimum(mn, v), np.maximum(mx, v)
+    np.testing.assert_array_equal(out, np.minimum(np.maximum(s1, mn), mx))
+    # advect then clamp (mode 3) equals the two passes
+    both = oracle.advect_smoke(res, dx, s0, psi, mode=3)
+    two = oracle.advect_smoke(res, dx, s0, psi)
+    two = oracle.advect_smoke(res, dx, s0, psi, smoke1=two, mode=2)
+    np.testing.assert_array_equal(both, two)
+
+
+def test_vortex_ring_smoke_density(oracle):
+    res = (10, 10, 10)
+    dx = 1.0 / 10
+    c, ex, ey, color = [0.5, 0.5, 0.4], [1.0, 0, 0], [0, 1.0, 0], [0.2, 0.5, 0.9]
+    ns, rad, delta = 64, 0.2, 0.05
+    sm = np.zeros((4,) + res, np.float32)
+    oracle.vortex_ring_smoke(res, dx, sm, rad, delta, c, ex, ey, color, ns)
+    P = np.moveaxis(_cell_centres(res, dx), 0, -1)
+    dens = np.zeros(res)
+    cl = 2 * np.pi * rad / ns
+    for l in range(ns):
+        th = l / ns * 2 * np.pi
+        ps = rad * (np.cos(th) * np.array(ex) + np.sin(th) * np.array(ey)) + c
+        r = np.linalg.norm(P - ps, axis=-1)
+        dens += cl * np.exp(-(r / delta) ** 3)
+    on = dens > 0.002
+    near = np.abs(dens - 0.002) < 1e-6
+    mismatch = (sm[3] == 1.0) != on
+    assert np.all(~mismatch | near)
+    assert 0 < on.sum() < on.size
+    for k in range(3):
+        np.testing.assert_array_equal(sm[k][sm[3] == 1.0], np.float32(color[k]))
+        assert np.all(sm[k][sm[3] == 0.0] == 0)
