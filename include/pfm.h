@@ -220,6 +220,27 @@ int pfm_advect_smoke(const pfm_grid* g, int channels, const float* smoke0, float
 int pfm_vortex_ring_smoke(const pfm_grid* g, float rad, float delta, const float* c, const float* ex,
                           const float* ey, int num_samples, const float* color, float* smoke,
                           pfm_stream_t stream);
+/* Particle smoke (3D; 4 channels).  init_particles_smoke
+ * (3D/init_conditions.py:275-279, interp_grad_2_smoke 3D/taichi_utils.py:378-421):
+ * p_smoke (4, n) and p_grad (12, n: a 3x4 matrix per particle, row-major) from
+ * the cell smoke (4, cells). */
+int pfm_g2p_smoke(const pfm_grid* g, int64_t n, const float* pos, const float* smoke, float* p_smoke,
+                  float* p_grad, pfm_stream_t stream);
+/* P2G_smoke (3D/run_3D.py:554-593) as a deterministic gather over the cell
+ * sort of pfm_bin_particles: smoke (4, cells) and the weights (cells) are
+ * overwritten; with use_apic the particle term is p_smoke + dpos
+ * (cols(T_grad_m) cols(T_init) init_grad). */
+int pfm_p2g_smoke(const pfm_grid* g, int64_t n, const float* pos, const float* p_smoke,
+                  const float* init_grad, const float* Tgm_x, const float* Tgm_y, const float* Tgm_z,
+                  const float* Tin_x, const float* Tin_y, const float* Tin_z, int use_apic,
+                  const int32_t* cell_start, const int32_t* perm, float* smoke, float* weight,
+                  pfm_stream_t stream);
+/* clamp_smoke_particle (3D/run_3D.py:595-614) with the cell lists of
+ * pfm_bin_particles (the first cell_max particles of a cell by id, as
+ * get_particles_id_in_every_cell keeps them). */
+int pfm_clamp_smoke_particle(const pfm_grid* g, int64_t n, const int32_t* cell_start,
+                             const int32_t* perm, int32_t cell_max, const float* init_pos,
+                             const float* init_smoke, float* smoke, pfm_stream_t stream);
 int pfm_kinetic_energy(const pfm_grid* g, const float* u_x, const float* u_y, const float* u_z,
                        double* out, pfm_stream_t stream);
 

@@ -78,6 +78,9 @@ def lib():
         L.oracle_vortex_ring.argtypes = [I32P, fl, fl, fl, fl, F32P, F32P, F32P, ci, vp]
         L.oracle_advect_smoke.argtypes = [ci, I32P, fl, ci, vp, vp, vp, vp, vp, ci]
         L.oracle_vortex_ring_smoke.argtypes = [I32P, fl, fl, fl, F32P, F32P, F32P, ci, F32P, vp]
+        L.oracle_g2p_smoke.argtypes = [I32P, fl, i64, vp, vp, vp, vp]
+        L.oracle_p2g_smoke.argtypes = [I32P, fl, i64, vp, vp, vp, vp, vp, ci, vp, vp]
+        L.oracle_clamp_smoke_particle.argtypes = [I32P, fl, i64, vp, vp, ci, vp, vp, vp]
         L.oracle_kinetic_energy.restype = ctypes.c_double
         L.oracle_kinetic_energy.argtypes = [ci, I32P, vp, vp, vp]
         for pre in ("oracle_mg_", "oracle_mg64_"):
@@ -300,6 +303,38 @@ def vortex_ring_smoke(res, dx, smoke, rad, delta, c, ex, ey, color, num_samples=
     lib().oracle_vortex_ring_smoke(_res(res), dx, rad, delta, np.asarray(c, np.float32),
                                    np.asarray(ex, np.float32), np.asarray(ey, np.float32), num_samples,
                                    np.asarray(color, np.float32), _p(smoke))
+
+
+def g2p_smoke(res, dx, pos, smoke):
+    """init_particles_smoke (3D/init_conditions.py:275-279): (p_smoke (4, n), p_grad (12, n))."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    n = pos.shape[1]
+    ps = np.empty((4, n), np.float32)
+    pg = np.empty((12, n), np.float32)
+    lib().oracle_g2p_smoke(_res(res), dx, n, _p(pos), _p(np.ascontiguousarray(smoke, np.float32)), _p(ps), _p(pg))
+    return ps, pg
+
+
+def p2g_smoke(res, dx, pos, ps, grad=None, Tgm=None, Tin=None, apic=True):
+    """P2G_smoke (3D/run_3D.py:554-593): (smoke (4, *res), weight res).  Tgm,
+    Tin as (3, 3, n) column stacks."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    n = pos.shape[1]
+    smoke = np.empty((4,) + tuple(res), np.float32)
+    w = np.empty(tuple(res), np.float32)
+    c = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+    lib().oracle_p2g_smoke(_res(res), dx, n, _p(pos), _p(c(ps)), _p(c(grad)), _p(c(Tgm)), _p(c(Tin)), int(apic),
+                           _p(smoke), _p(w))
+    return smoke, w
+
+
+def clamp_smoke_particle(res, dx, start, ids, cell_max, init_pos, init_smoke, smoke):
+    """clamp_smoke_particle (3D/run_3D.py:595-614) in place on smoke (4, *res)."""
+    lib().oracle_clamp_smoke_particle(_res(res), dx, np.asarray(init_pos).shape[1], _p(np.ascontiguousarray(start)),
+                                      _p(np.ascontiguousarray(ids)), int(cell_max),
+                                      _p(np.ascontiguousarray(init_pos, np.float32)),
+                                      _p(np.ascontiguousarray(init_smoke, np.float32)), _p(smoke))
+    return smoke
 
 
 def kinetic_energy(res, u):

@@ -885,6 +885,164 @@ void oracle_vortex_ring_smoke(const int* res, float dx, float rad, float delta, 
     }
 }
 
+/* ---- (f)4 particle smoke (3D, 4 channels; cell fields (4, C), particle
+ * smoke (4, n), gradient (12, n) row-major 3x4).
+ * init_particles_smoke: 3D/init_conditions.py:275-279 with
+ * interp_grad_2_smoke (3D/taichi_utils.py:378-421): s = max(1, min(u,
+ * dim-2-eps)), 64 samples in (i, j, k) order, dw = 1/dx dN N N,
+ * interped += value N N N, new_C += outer(dw, value). */
+void oracle_g2p_smoke(const int* res, float dx, int64_t n, const float* pos, const float* smoke,
+                      float* ps, float* pg) {
+    grid_t g;
+    make_grid(&g, 3, res, dx);
+    const int shp[3] = {g.res[0], g.res[1], g.res[2]};
+    const int64_t C = (int64_t)shp[0] * shp[1] * shp[2];
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; ++p) {
+        float sv[3], fl[3];
+        for (int a = 0; a < 3; ++a) {
+            float u = pos[a * n + p] * g.inv_dx - 0.5f;
+            float hi = (float)((double)shp[a] - 2.0 - ORACLE_EPS);
+            float m = u < hi ? u : hi;
+            sv[a] = m > 1.0f ? m : 1.0f;
+            fl[a] = floorf(sv[a]);
+        }
+        float val[4] = {0.f, 0.f, 0.f, 0.f}, nc[3][4];
+        memset(nc, 0, sizeof(nc));
+        for (int i = -1; i < 3; ++i)
+            for (int j = -1; j < 3; ++j)
+                for (int k = -1; k < 3; ++k) {
+                    float x = sv[0] - (fl[0] + (float)i), y = sv[1] - (fl[1] + (float)j), z = sv[2] - (fl[2] + (float)k);
+                    int ci = (int)(fl[0] + (float)i), cj = (int)(fl[1] + (float)j), ck = (int)(fl[2] + (float)k);
+                    float dw[3] = {g.inv_dx * oracle_dN2(x) * oracle_N2(y) * oracle_N2(z),
+                                   g.inv_dx * oracle_N2(x) * oracle_dN2(y) * oracle_N2(z),
+                                   g.inv_dx * oracle_N2(x) * oracle_N2(y) * oracle_dN2(z)};
+                    for (int c = 0; c < 4; ++c) {
+                        float v = sample3(smoke + c * C, shp, ci, cj, ck);
+                        val[c] = val[c] + v * oracle_N2(x) * oracle_N2(y) * oracle_N2(z);
+                        for (int r = 0; r < 3; ++r) nc[r][c] = nc[r][c] + dw[r] * v;
+                    }
+                }
+        for (int c = 0; c < 4; ++c) ps[c * n + p] = val[c];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) pg[(4 * r + c) * n + p] = nc[r][c];
+    }
+}
+
+/* P2G_smoke: 3D/run_3D.py:554-593, particles in id order, offsets -1..2 in
+ * (i, j, k) order around int(pos - 0.5), weights N_2 N_2 N_2, APIC term
+ * dpos @ (cols(T_grad_m) @ cols(T_init) @ grad); then smoke /= weight where
+ * positive and each channel clamped to [0, 1]. */
+void oracle_p2g_smoke(const int* res, float dx, int64_t n, const float* pos, const float* ps,
+                      const float* grad, const float* tgm, const float* tin, int apic, float* smoke,
+                      float* weight) {
+    grid_t g;
+    make_grid(&g, 3, res, dx);
+    const int shp[3] = {g.res[0], g.res[1], g.res[2]};
+    const int64_t C = (int64_t)shp[0] * shp[1] * shp[2];
+    const float* Tg[3] = {tgm, tgm + 3 * n, tgm + 6 * n};
+    const float* Ti[3] = {tin, tin + 3 * n, tin + 6 * n};
+    for (int64_t q = 0; q < 4 * C; ++q) smoke[q] = 0.f;
+    for (int64_t q = 0; q < C; ++q) weight[q] = 0.f;
+    for (int64_t p = 0; p < n; ++p) {
+        float P[3];
+        int base[3];
+        for (int r = 0; r < 3; ++r) {
+            P[r] = pos[r * n + p] * g.inv_dx;
+            base[r] = (int)(P[r] - 0.5f);
+        }
+        float MG[3][4];
+        if (apic) {
+            float M[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    float m = Tg[0][r * n + p] * Ti[c][0 * n + p];
+                    m = m + Tg[1][r * n + p] * Ti[c][1 * n + p];
+                    m = m + Tg[2][r * n + p] * Ti[c][2 * n + p];
+                    M[r][c] = m;
+                }
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 4; ++c) {
+                    float m = M[r][0] * grad[(0 + c) * n + p];
+                    m = m + M[r][1] * grad[(4 + c) * n + p];
+                    m = m + M[r][2] * grad[(8 + c) * n + p];
+                    MG[r][c] = m;
+                }
+        }
+        for (int oi = -1; oi < 3; ++oi)
+            for (int oj = -1; oj < 3; ++oj)
+                for (int ok = -1; ok < 3; ++ok) {
+                    int F[3] = {base[0] + oi, base[1] + oj, base[2] + ok};
+                    if (F[0] < 0 || F[0] >= shp[0] || F[1] < 0 || F[1] >= shp[1] || F[2] < 0 || F[2] >= shp[2])
+                        continue;
+                    float w = oracle_N2(P[0] - (float)F[0] - 0.5f) * oracle_N2(P[1] - (float)F[1] - 0.5f) *
+                              oracle_N2(P[2] - (float)F[2] - 0.5f);
+                    int64_t q = ((int64_t)F[0] * shp[1] + F[1]) * shp[2] + F[2];
+                    weight[q] += w;
+                    for (int c = 0; c < 4; ++c) {
+                        float delta = 0.f;
+                        if (apic) {
+                            float dpos[3];
+                            for (int r = 0; r < 3; ++r) dpos[r] = ((float)F[r] + 0.5f - P[r]) * dx;
+                            delta = dpos[0] * MG[0][c];
+                            delta = delta + dpos[1] * MG[1][c];
+                            delta = delta + dpos[2] * MG[2][c];
+                        }
+                        smoke[c * C + q] += (ps[c * n + p] + delta) * w;
+                    }
+                }
+    }
+    for (int64_t q = 0; q < C; ++q)
+        if (weight[q] > 0.f)
+            for (int c = 0; c < 4; ++c) {
+                float v = smoke[c * C + q] / weight[q];
+                if (v < 0.f) v = 0.0f;
+                if (v > 1.f) v = 1.0f;
+                smoke[c * C + q] = v;
+            }
+}
+
+/* clamp_smoke_particle: 3D/run_3D.py:595-614 over cell lists (ids ascending,
+ * at most cell_max per cell) given as start / ids arrays (CSR). */
+void oracle_clamp_smoke_particle(const int* res, float dx, int64_t n, const int32_t* start,
+                                 const int32_t* ids, int cell_max, const float* init_pos,
+                                 const float* init_smoke, float* smoke) {
+    grid_t g;
+    make_grid(&g, 3, res, dx);
+    const int shp[3] = {g.res[0], g.res[1], g.res[2]};
+    const int64_t C = (int64_t)shp[0] * shp[1] * shp[2];
+    for (int64_t q = 0; q < C; ++q) {
+        float mn[4] = {1000.f, 1000.f, 1000.f, 1000.f}, mx[4] = {-1000.f, -1000.f, -1000.f, -1000.f};
+        int valid = 0;
+        int m = start[q + 1] - start[q];
+        if (m > cell_max) m = cell_max;
+        for (int s = 0; s < m; ++s) {
+            int64_t p = ids[start[q] + s];
+            int b[3];
+            for (int r = 0; r < 3; ++r) b[r] = (int)(init_pos[r * n + p] * g.inv_dx);
+            for (int oa = -1; oa < 2; ++oa)
+                for (int ob = -1; ob < 2; ++ob)
+                    for (int oc = -1; oc < 2; ++oc) {
+                        int ca = b[0] + oa, cb = b[1] + ob, cc = b[2] + oc;
+                        if (ca < 0 || ca >= shp[0] || cb < 0 || cb >= shp[1] || cc < 0 || cc >= shp[2]) continue;
+                        int64_t nb = ((int64_t)ca * shp[1] + cb) * shp[2] + cc;
+                        for (int c = 0; c < 4; ++c) {
+                            float v = init_smoke[c * C + nb];
+                            mn[c] = v < mn[c] ? v : mn[c];
+                            mx[c] = v > mx[c] ? v : mx[c];
+                        }
+                        valid += 1;
+                    }
+        }
+        if (valid >= 1)
+            for (int c = 0; c < 4; ++c) {
+                float v = smoke[c * C + q];
+                v = v > mn[c] ? v : mn[c];
+                smoke[c * C + q] = v < mx[c] ? v : mx[c];
+            }
+    }
+}
+
 /* ---- 32. kinetic energy (2D/run_2D.py:765-770): sum u_a^2 / |F_a|, in double */
 double oracle_kinetic_energy(int dim, const int* res, const float* ux, const float* uy,
                              const float* uz) {

@@ -23,7 +23,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libpfm.so")
 ARCH = os.environ.get("PFM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["abi.cpp", "particles.hip", "grid.hip", "p2g.hip", "mgpcg.hip", "scene.cpp"]
+SOURCES = ["abi.cpp", "particles.hip", "grid.hip", "p2g.hip", "mgpcg.hip", "scene.cpp", "smoke.hip"]
 
 CFLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
@@ -42,7 +42,7 @@ def hipcc() -> str:
 # Sources whose fp32 passes must equal a sequential restatement bit for bit:
 # HIP's __fmul_rn/__fadd_rn are plain operators, so only a file-wide
 # -ffp-contract=off keeps the compiler from fusing them into FMAs.
-NO_CONTRACT = {"mgpcg.hip"}
+NO_CONTRACT = {"mgpcg.hip", "smoke.hip"}
 
 
 def _compile(src: str) -> str:

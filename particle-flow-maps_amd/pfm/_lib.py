@@ -93,6 +93,9 @@ _SIGS = {
     "pfm_kinetic_energy": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "pfm_advect_smoke": [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp],
     "pfm_vortex_ring_smoke": [c_vp, c_f, c_f, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp],
+    "pfm_g2p_smoke": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "pfm_p2g_smoke": [c_vp, c_i64, c_vp, c_vp, c_vp] + [c_vp] * 6 + [c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "pfm_clamp_smoke_particle": [c_vp, c_i64, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp],
     "pfm_mg_create": [c_int, c_vp, c_int, c_vp, c_int, ctypes.POINTER(c_vp)],
     "pfm_mg_destroy": [c_vp],
     "pfm_mg_n_levels": [c_vp],

@@ -353,6 +353,40 @@ def vortex_ring_smoke(res, dx, smoke, rad, delta, c, ex, ey, color, num_samples=
            int(num_samples), arr[3].ctypes.data, L.ptr(smoke, name="smoke"), _stream(stream))
 
 
+def g2p_smoke(res, dx, pos, smoke, p_smoke, p_grad, stream=None):
+    """init_particles_smoke (3D/init_conditions.py:275-279): p_smoke (4, n) and
+    p_grad (12, n; row-major 3x4 per particle) from the cell smoke (4, *res)."""
+    d, n = pos.shape
+    if d != 3 or tuple(smoke.shape) != (4,) + tuple(res):
+        raise L.PfmError("particle smoke is 3D with a (4, *res) smoke field")
+    g = L.make_grid(res, dx)
+    L.call("pfm_g2p_smoke", ctypes.byref(g), n, _pvec(pos, 3, n, "pos"), L.ptr(smoke, name="smoke"),
+           _pvec(p_smoke, 4, n, "p_smoke"), _pvec(p_grad, 12, n, "p_grad"), _stream(stream))
+
+
+def p2g_smoke(res, dx, pos, p_smoke, init_grad, Tgm, Tin, binning, smoke, weight, use_apic=True, stream=None):
+    """P2G_smoke (3D/run_3D.py:554-593) over the cell sort of `binning`
+    (a Binning already binned at these positions)."""
+    d, n = pos.shape
+    g = L.make_grid(res, dx)
+    Gp = _cols(Tgm, 3, n, "T_grad_m") if use_apic else [None] * 3
+    Ip = _cols(Tin, 3, n, "T_init") if use_apic else [None] * 3
+    gp = _pvec(init_grad, 12, n, "init_grad") if use_apic else None
+    L.call("pfm_p2g_smoke", ctypes.byref(g), n, _pvec(pos, 3, n, "pos"), _pvec(p_smoke, 4, n, "p_smoke"), gp,
+           *Gp, *Ip, int(bool(use_apic)), L.ptr(binning.cell_start, dtype=torch.int32),
+           L.ptr(binning.perm, dtype=torch.int32), L.ptr(smoke, name="smoke"), L.ptr(weight, name="weight"),
+           _stream(stream))
+
+
+def clamp_smoke_particle(res, dx, binning, cell_max, init_pos, init_smoke, smoke, stream=None):
+    """clamp_smoke_particle (3D/run_3D.py:595-614) over the cell lists of `binning`."""
+    n = init_pos.shape[1]
+    g = L.make_grid(res, dx)
+    L.call("pfm_clamp_smoke_particle", ctypes.byref(g), n, L.ptr(binning.cell_start, dtype=torch.int32),
+           L.ptr(binning.perm, dtype=torch.int32), int(cell_max), _pvec(init_pos, 3, n, "init_pos"),
+           L.ptr(init_smoke, name="init_smoke"), L.ptr(smoke, name="smoke"), _stream(stream))
+
+
 def curl(res, dx, vc, out, stream=None):
     """curl (3D/taichi_utils.py:56-78; 2D/taichi_utils.py:97-103) of the
     cell-centred vector vc (d, *res): out is (3, *res) in 3D, res in 2D."""

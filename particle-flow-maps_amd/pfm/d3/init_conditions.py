@@ -45,6 +45,12 @@ def init_particles_imp_grad_m(particles_imp, particles_pos, u_x, u_y, u_z, C_x, 
     _last_C = (C_x, C_y, C_z)
 
 
+def init_particles_smoke(particles_smoke, particles_grad_smoke, particles_pos, smoke, dx):
+    """3D/init_conditions.py:275-279 (smoke (4, *res); particle smoke (4, n),
+    gradient (12, n) row-major 3x4)."""
+    core.g2p_smoke(tuple(smoke.shape[1:]), dx, particles_pos, smoke, particles_smoke, particles_grad_smoke)
+
+
 def add_vortex_ring(w, rad, delta, c, unit_x, unit_y, pf, vf, num_samples, *, dx=None):
     """3D/init_conditions.py:10-22: adds one smoothed Biot-Savart ring into the
     cell-centred velocity ``vf`` (3, rx, ry, rz).  ``pf`` (cell centres) is

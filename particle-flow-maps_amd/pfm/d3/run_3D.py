@@ -88,6 +88,30 @@ def clamp_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx):
     core.advect_smoke(tuple(smoke0.shape[1:]), dx, smoke0, smoke1, (psi_x, psi_y, psi_z), mode=2)
 
 
+def P2G_smoke(particles_smoke, particles_init_grad_smoke, particles_pos, smoke, T_x_grad_m, T_y_grad_m,
+              T_z_grad_m, T_x_init, T_y_init, T_z_init, p2g_weight, *, use_APIC_smoke=True, dx=None):
+    """3D/run_3D.py:554-593 (smoke (4, *res), p2g_weight res, particle smoke
+    (4, n), gradient (12, n)); deterministic gather over the cell sort."""
+    res = tuple(p2g_weight.shape)
+    d = _dx(res, dx)
+    b = core.binning_for(res, particles_pos.shape[1], particles_pos.device)
+    b.bin(particles_pos, d)
+    core.p2g_smoke(res, d, particles_pos, particles_smoke, particles_init_grad_smoke,
+                   (T_x_grad_m, T_y_grad_m, T_z_grad_m), (T_x_init, T_y_init, T_z_init), b, smoke, p2g_weight,
+                   use_apic=use_APIC_smoke)
+
+
+def clamp_smoke_particle(cell_particles_id, cell_particle_num, particles_init_pos, smoke, init_smoke, *, dx=None):
+    """3D/run_3D.py:595-614 over the lists of the last
+    get_particles_id_in_every_cell (its cell sort, at most cell_max per cell)."""
+    res = tuple(cell_particle_num.shape)
+    b = core.binning_for(res, particles_init_pos.shape[1], particles_init_pos.device)
+    if b._key is None:
+        raise L.PfmError("clamp_smoke_particle needs get_particles_id_in_every_cell first")
+    core.clamp_smoke_particle(res, _dx(res, dx), b, cell_particles_id.shape[-1], particles_init_pos, init_smoke,
+                              smoke)
+
+
 def get_particles_id_in_every_cell(cell_particles_id, cell_particle_num, particles_pos, *, dx=None):
     """3D/run_3D.py:342-351, deterministic: ids sorted ascending inside a cell."""
     res = tuple(cell_particle_num.shape)

@@ -398,3 +398,75 @@ def test_vortex_ring_and_smoke(oracle, pfm):
     diff = np.any(N(sm) != ws, axis=0)
     assert diff.sum() <= max(2, diff.size // 1000)
     assert (ws[3] == 1).sum() > 0
+
+
+def test_particle_smoke_vs_oracle(oracle, pfm):
+    """init_particles_smoke bit-exact (the literal 64-point loop in both),
+    P2G_smoke within 1e-6 (the gather sums in cell order, the oracle in
+    particle order), clamp_smoke_particle bit-exact."""
+    from pfm import core
+
+    res = (14, 12, 10)
+    pos, dx, n = _particles(oracle, res, seed=21, order=1)
+    rng = np.random.default_rng(62)
+    smoke = rng.uniform(0, 1, size=(4,) + res).astype(np.float32)
+    ps = torch.empty((4, n), device=DEV)
+    pg = torch.empty((12, n), device=DEV)
+    core.g2p_smoke(res, dx, G(pos), G(smoke), ps, pg)
+    ops, opg = oracle.g2p_smoke(res, dx, pos, smoke)
+    np.testing.assert_array_equal(N(ps), ops)
+    np.testing.assert_array_equal(N(pg), opg)
+    # P2G_smoke with non-trivial T after moving the particles a little
+    pos2 = (pos + rng.normal(0, 0.2 * dx, size=pos.shape)).astype(np.float32)
+    Tgm = (np.eye(3)[:, :, None] + rng.normal(0, 0.05, size=(3, 3, n))).astype(np.float32)
+    Tin = (np.eye(3)[:, :, None] + rng.normal(0, 0.05, size=(3, 3, n))).astype(np.float32)
+    b = core.Binning(res, n, torch.device(DEV))
+    b.bin(G(pos2), dx)
+    for apic in (True, False):
+        sm = torch.empty((4,) + res, device=DEV)
+        w = torch.empty(res, device=DEV)
+        core.p2g_smoke(res, dx, G(pos2), G(ops), G(opg), [G(Tgm[c]) for c in range(3)],
+                       [G(Tin[c]) for c in range(3)], b, sm, w, use_apic=apic)
+        osm, ow = oracle.p2g_smoke(res, dx, pos2, ops, opg, Tgm, Tin, apic=apic)
+        assert rel_l2(N(w), ow) < 1e-6
+        assert rel_l2(N(sm), osm) < 1e-6
+    # clamp_smoke_particle with the reference's cell_max = int(1.6 * 8)
+    ipos = pos
+    cur = rng.uniform(-0.5, 1.5, size=(4,) + res).astype(np.float32)
+    t = G(cur)
+    core.clamp_smoke_particle(res, dx, b, 12, G(ipos), G(smoke), t)
+    _c, _k, start, perm = oracle.bin_particles(res, dx, pos2)
+    want = oracle.clamp_smoke_particle(res, dx, start, perm, 12, ipos, smoke, cur.copy())
+    np.testing.assert_array_equal(N(t), want)
+
+
+def test_smoke_reference_adapters(oracle, pfm):
+    """init_particles_smoke / P2G_smoke / clamp_smoke_particle under the
+    reference's names and argument order."""
+    from pfm.d3 import init_conditions as ic
+    from pfm.d3 import run_3D
+
+    res = (10, 9, 8)
+    pos, dx, n = _particles(oracle, res, seed=22)
+    rng = np.random.default_rng(63)
+    smoke = G(rng.uniform(0, 1, size=(4,) + res).astype(np.float32))
+    ps = torch.empty((4, n), device=DEV)
+    pg = torch.empty((12, n), device=DEV)
+    P = G(pos)
+    ic.init_particles_smoke(ps, pg, P, smoke, dx)
+    eye = [torch.zeros((3, n), device=DEV) for _ in range(3)]
+    for c in range(3):
+        eye[c][c] = 1.0
+    out = torch.empty_like(smoke)
+    w = torch.empty(res, device=DEV)
+    run_3D.P2G_smoke(ps, pg, P, out, *eye, *eye, w)
+    osm, ow = oracle.p2g_smoke(res, dx, pos, N(ps), N(pg), np.stack([N(t) for t in eye]),
+                               np.stack([N(t) for t in eye]))
+    assert rel_l2(N(out), osm) < 1e-6
+    ids = torch.empty(res + (12,), dtype=torch.int32, device=DEV)
+    cnt = torch.empty(res, dtype=torch.int32, device=DEV)
+    run_3D.get_particles_id_in_every_cell(ids, cnt, P)
+    before = N(out).copy()
+    run_3D.clamp_smoke_particle(ids, cnt, P, out, smoke)
+    _c, _k, start, perm = oracle.bin_particles(res, dx, pos)
+    np.testing.assert_array_equal(N(out), oracle.clamp_smoke_particle(res, dx, start, perm, 12, pos, N(smoke), before))

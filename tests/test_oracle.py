@@ -659,3 +659,87 @@ def test_vortex_ring_smoke_density(oracle):
     for k in range(3):
         np.testing.assert_array_equal(sm[k][sm[3] == 1.0], np.float32(color[k]))
         assert np.all(sm[k][sm[3] == 0.0] == 0)
+
+
+# ---- particle smoke (init_particles_smoke / P2G_smoke / clamp_smoke_particle)
+def _smoke_affine(res, dx):
+    a = np.array([0.4, 0.5, 0.45, 0.55])
+    b = np.array([[0.3, -0.2, 0.1], [-0.1, 0.25, 0.2], [0.05, 0.1, -0.3], [0.2, 0.15, 0.1]])
+    X = _cell_centres(res, dx)
+    field = (a[:, None, None, None] + np.tensordot(b, X, axes=1)).astype(np.float32)
+    return a, b, field
+
+
+def test_g2p_smoke_reproduces_affine_fields(oracle):
+    """interp_grad_2_smoke: the quadratic B-spline value and gradient of an
+    affine field are exact away from the clamp."""
+    res = (12, 12, 12)
+    dx = 1.0 / 12
+    a, b, field = _smoke_affine(res, dx)
+    rng = np.random.default_rng(9)
+    pos = rng.uniform(2.0 * dx, 10.0 * dx, size=(3, 500)).astype(np.float32)
+    ps, pg = oracle.g2p_smoke(res, dx, pos, field)
+    want = a[:, None] + b @ pos.astype(np.float64)
+    np.testing.assert_allclose(ps, want, rtol=0, atol=2e-6)
+    for r in range(3):
+        for c in range(4):
+            np.testing.assert_allclose(pg[4 * r + c], b[c, r], rtol=0, atol=2e-4)
+
+
+@pytest.mark.parametrize("apic", [True, False])
+def test_p2g_smoke_affine_reproduction(oracle, apic):
+    """P2G_smoke with APIC: particles carrying an affine smoke and its gradient
+    (T = I) splat it back exactly on cells whose 4^3 neighbourhood is covered;
+    without APIC a constant field stays constant."""
+    res = (10, 10, 10)
+    dx = 1.0 / 10
+    n = int(np.prod(res)) * 8
+    pos = oracle.seed_particles(res, 2, dx / 2, n, seed=4)
+    a, b, _ = _smoke_affine(res, dx)
+    if apic:
+        ps = (a[:, None] + b @ pos.astype(np.float64)).astype(np.float32)
+    else:
+        ps = np.repeat(a[:, None], n, axis=1).astype(np.float32)
+    grad = np.zeros((12, n), np.float32)
+    for r in range(3):
+        for c in range(4):
+            grad[4 * r + c] = b[c, r]
+    eye = np.zeros((3, 3, n), np.float32)
+    for c in range(3):
+        eye[c, c] = 1.0
+    smoke, w = oracle.p2g_smoke(res, dx, pos, ps, grad, eye, eye, apic=apic)
+    X = _cell_centres(res, dx)
+    want = a[:, None, None, None] + np.tensordot(b, X, axes=1) * (1.0 if apic else 0.0)
+    core = (slice(None),) + tuple(slice(2, n_ - 2) for n_ in res)
+    np.testing.assert_allclose(smoke[core], want[core], rtol=0, atol=3e-6)
+    assert np.all(w > 0)
+
+
+def test_clamp_smoke_particle_brute_force(oracle):
+    res = (5, 6, 4)
+    dx = 1.0 / 6
+    rng = np.random.default_rng(10)
+    n = 300
+    pos = rng.uniform(0, 1, size=(3, n)).astype(np.float32) * np.array(res, np.float32)[:, None] * np.float32(dx)
+    ipos = np.clip(pos + rng.normal(0, 0.08, size=pos.shape).astype(np.float32), 0, None).astype(np.float32)
+    init_smoke = rng.uniform(0, 1, size=(4,) + res).astype(np.float32)
+    smoke0 = rng.uniform(-0.5, 1.5, size=(4,) + res).astype(np.float32)
+    _cell, counts, start, perm = oracle.bin_particles(res, dx, pos)
+    out = oracle.clamp_smoke_particle(res, dx, start, perm, 3, ipos, init_smoke, smoke0.copy())
+    want = smoke0.copy()
+    inv = np.float32(1.0 / dx)
+    for q in range(int(np.prod(res))):
+        I = np.unravel_index(q, res)
+        ids = perm[start[q]:start[q + 1]][:3]
+        vals = []
+        for p in ids:
+            base = (ipos[:, p] * inv).astype(np.int64)
+            for o in np.ndindex(3, 3, 3):
+                nb = base + np.array(o) - 1
+                if np.all(nb >= 0) and np.all(nb < np.array(res)):
+                    vals.append(init_smoke[(slice(None),) + tuple(nb)])
+        if vals:
+            v = np.stack(vals)
+            want[(slice(None),) + I] = np.minimum(np.maximum(smoke0[(slice(None),) + I], v.min(0)), v.max(0))
+    np.testing.assert_array_equal(out, want)
+    assert np.any(counts > 3)  # the cap is exercised
