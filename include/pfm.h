@@ -163,6 +163,17 @@ int pfm_rk4_particles_p2g_record(const pfm_grid* g, int64_t n, float* pos, float
                                  float dt, const float* Tin_x, const float* Tin_y, const float* Tin_z,
                                  const float* iC_x, const float* iC_y, const float* iC_z,
                                  const float* init_imp, float* records, pfm_stream_t stream);
+/* pfm_rk4_particles_p2g_record for a 3D step whose short reinitialisation
+ * runs every step (reinit_every_grad_m = 1, 3D/hyperparameters.py:33): T_grad_m
+ * enters as the identity (not read), and T_init is replaced in place by
+ * cols(T_grad_m) cols(T_init) -- the update_T of the next step's short
+ * reinitialisation (3D/run_3D.py:767-771) -- after the record is formed, so
+ * that update_T and reset_T_to_identity(T_grad_m) can be skipped there. */
+int pfm_rk4_particles_p2g_record_fold(const pfm_grid* g, int64_t n, float* pos, const float* u_x,
+                                      const float* u_y, const float* u_z, float dt, float* Tin_x,
+                                      float* Tin_y, float* Tin_z, const float* iC_x, const float* iC_y,
+                                      const float* iC_z, const float* init_imp, float* records,
+                                      pfm_stream_t stream);
 int pfm_p2g_records(const pfm_grid* g, int64_t n, const int32_t* cell_start, const int32_t* perm,
                     float* u_x, float* u_y, float* u_z, float* W_x, float* W_y, float* W_z,
                     void* workspace, size_t ws_bytes, pfm_stream_t stream);
@@ -327,6 +338,7 @@ typedef struct {
     int32_t max_iters, tol_mode;
     double tol;
     int32_t fused_p2g;            /* 3D: pfm_rk4_particles_p2g_record + pfm_p2g_records */
+    int32_t fold_T;               /* 3D, fused, reinit_every_grad_m == 1: the _fold variant */
     int64_t n;                    /* particles */
     /* grid fields (face arrays; unused axes NULL) */
     float* u[3];

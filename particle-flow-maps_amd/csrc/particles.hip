@@ -16,9 +16,27 @@ struct RecArgs {
     const float* C[3];
     const float* imp;
     float* rec;
+    float* Tfold[3];  // FOLD: where T_init <- cols(T_grad_m) cols(T_init) goes (in place)
 };
 
-template <int DIM, bool REC = false>
+// update_T's product (3D/run_3D.py:430-439): O[m][r] = sum_k A[r][k] B[k][m]
+// with A = rows of T_grad_m (A[r][k] = T_k[r]) and B[k][m] = T_init_m[k];
+// the multiply-adds are written as FMAs so every kernel that forms it
+// (k_update_T and the folded RK4 epilogue) rounds identically.
+template <int DIM>
+__device__ __forceinline__ float t_product(const float A[3][3], const float B[3][3], int r, int m) {
+    float acc = A[r][0] * B[0][m];
+#pragma unroll
+    for (int k = 1; k < DIM; ++k) acc = __builtin_fmaf(A[r][k], B[k][m], acc);
+    return acc;
+}
+
+// FOLD (3D step with a short reinitialisation every step): T_grad_m enters as
+// the identity and is not read; the epilogue writes the record and then
+// T_init <- cols(T_grad_m) cols(T_init), the update_T that the next step's
+// short reinitialisation would apply (3D/run_3D.py:767-771), leaving
+// T_grad_m in memory as the identity that reset would write.
+template <int DIM, bool REC = false, bool FOLD = false>
 __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t n,
                                                           float* __restrict__ pos,
                                                           float* __restrict__ Tx,
@@ -35,10 +53,15 @@ __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t 
     float T0[3][3];
 #pragma unroll
     for (int c = 0; c < DIM; ++c) p0[c] = pos[c * n + i];
+    // FOLD: the identity as opaque values, so the stage arithmetic is
+    // compiled exactly as for a T_grad_m read from memory (no constant
+    // folding of the 0 / 1 entries, which could change roundings)
+    float one = 1.0f, zero = 0.0f;
+    if (FOLD) asm volatile("" : "+v"(one), "+v"(zero));
 #pragma unroll
     for (int m = 0; m < DIM; ++m)
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) T0[m][c] = Ts[m][c * n + i];
+        for (int c = 0; c < DIM; ++c) T0[m][c] = FOLD ? (m == c ? one : zero) : Ts[m][c * n + i];
 
     const float half_dt = 0.5f * dt;
     const float hstage[3] = {half_dt, half_dt, 1.0f * dt};
@@ -101,7 +124,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t 
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
             T1[m][c] = T0[m][c] - sixth_dt * dTs[m][c];
-            Ts[m][c * n + i] = T1[m][c];
+            if (!FOLD) Ts[m][c * n + i] = T1[m][c];
         }
     if (REC) {
         float A[3][3], B[3][3], C[3][3], m0[3];
@@ -116,6 +139,18 @@ __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t 
             }
         }
         write_p2g_record<DIM>(ra.rec + i * Pay<DIM>::kRec, p1, g.inv_dx, true, A, B, C, m0);
+        if (FOLD) {
+            // update_T's A is cols(T_grad_m): Au[r][k] = T_k[r] (the record's A transposed)
+            float Au[3][3];
+#pragma unroll
+            for (int r = 0; r < DIM; ++r)
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) Au[r][k] = T1[k][r];
+#pragma unroll
+            for (int m = 0; m < DIM; ++m)
+#pragma unroll
+                for (int r = 0; r < DIM; ++r) ra.Tfold[m][r * n + i] = t_product<DIM>(Au, B, r, m);
+        }
     }
 }
 
@@ -173,12 +208,7 @@ __global__ void __launch_bounds__(kBlock) k_update_T(int64_t n, const float* __r
 #pragma unroll
     for (int m = 0; m < DIM; ++m)
 #pragma unroll
-        for (int r = 0; r < DIM; ++r) {
-            float acc = A[r][0] * B[0][m];
-#pragma unroll
-            for (int k = 1; k < DIM; ++k) acc += A[r][k] * B[k][m];
-            O[m][r * n + i] = acc;
-        }
+        for (int r = 0; r < DIM; ++r) O[m][r * n + i] = t_product<DIM>(A, B, r, m);
 }
 
 // update_particles_imp: imp = cols(T) @ init_imp (2D/run_2D.py:517-524).
@@ -380,6 +410,32 @@ int pfm_rk4_particles_p2g_record(const pfm_grid* gp, int64_t n, float* pos, floa
     else
         hipLaunchKernelGGL((k_rk4_particles<2, true>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), g,
                            n, pos, T_x, T_y, T_z, u_x, u_y, u_z, dt, 1, ra);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_rk4_particles_p2g_record_fold(const pfm_grid* gp, int64_t n, float* pos, const float* u_x,
+                                      const float* u_y, const float* u_z, float dt, float* Tin_x, float* Tin_y,
+                                      float* Tin_z, const float* iC_x, const float* iC_y, const float* iC_z,
+                                      const float* init_imp, float* records, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(g.dim == 3, "the folded RK4 + record path is 3D");
+    PFM_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return PFM_OK;
+    PFM_CHECK_ARG(pos && u_x && u_y && u_z && Tin_x && Tin_y && Tin_z && iC_x && iC_y && iC_z && init_imp &&
+                      records,
+                  "null pointer");
+    PFM_CHECK_ARG(((uintptr_t)records & 15) == 0, "records must be 16-byte aligned");
+    RecArgs ra;
+    ra.Tin[0] = Tin_x; ra.Tin[1] = Tin_y; ra.Tin[2] = Tin_z;
+    ra.Tfold[0] = Tin_x; ra.Tfold[1] = Tin_y; ra.Tfold[2] = Tin_z;
+    ra.C[0] = iC_x; ra.C[1] = iC_y; ra.C[2] = iC_z;
+    ra.imp = init_imp;
+    ra.rec = records;
+    hipLaunchKernelGGL((k_rk4_particles<3, true, true>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), g,
+                       n, pos, nullptr, nullptr, nullptr, u_x, u_y, u_z, dt, 1, ra);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

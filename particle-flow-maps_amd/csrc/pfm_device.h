@@ -246,6 +246,8 @@ __device__ __forceinline__ void write_p2g_record(float* __restrict__ dst, const 
     for (int c = 0; c < DIM; ++c) rec[P::kPos + c] = pos[c] * inv_dx;
     float Co[3][3], mo[3];
     if (compose) {
+        // every multiply-add is an explicit FMA, so kernels that form the same
+        // products (update_T folded into the RK4 epilogue) round identically
         float P2[3][3], Tt[3][3];
 #pragma unroll
         for (int r = 0; r < DIM; ++r)
@@ -253,7 +255,7 @@ __device__ __forceinline__ void write_p2g_record(float* __restrict__ dst, const 
             for (int c = 0; c < DIM; ++c) {
                 float acc = C[r][0] * A[0][c];
 #pragma unroll
-                for (int k = 1; k < DIM; ++k) acc += C[r][k] * A[k][c];
+                for (int k = 1; k < DIM; ++k) acc = __builtin_fmaf(C[r][k], A[k][c], acc);
                 P2[r][c] = acc;
             }
 #pragma unroll
@@ -262,18 +264,18 @@ __device__ __forceinline__ void write_p2g_record(float* __restrict__ dst, const 
             for (int c = 0; c < DIM; ++c) {
                 float acc = A[0][r] * P2[0][c];
 #pragma unroll
-                for (int k = 1; k < DIM; ++k) acc += A[k][r] * P2[k][c];
+                for (int k = 1; k < DIM; ++k) acc = __builtin_fmaf(A[k][r], P2[k][c], acc);
                 Co[r][c] = acc;
                 float t = A[0][r] * B[0][c];
 #pragma unroll
-                for (int k = 1; k < DIM; ++k) t += A[k][r] * B[k][c];
+                for (int k = 1; k < DIM; ++k) t = __builtin_fmaf(A[k][r], B[k][c], t);
                 Tt[r][c] = t;
             }
 #pragma unroll
         for (int r = 0; r < DIM; ++r) {
             float acc = Tt[r][0] * imp[0];
 #pragma unroll
-            for (int c = 1; c < DIM; ++c) acc += Tt[r][c] * imp[c];
+            for (int c = 1; c < DIM; ++c) acc = __builtin_fmaf(Tt[r][c], imp[c], acc);
             mo[r] = acc;
         }
     } else {

@@ -120,9 +120,13 @@ int step3(Scene* sc, pfm_scene_state* st, double dt, hipStream_t s) {
     if (r.short_r) {
         PFM_STEP_TRY(pfm_g2p(g, d.n, d.pos, d.u[0], d.u[1], d.u[2], d.imp_grad_m, d.init_C[0], d.init_C[1],
                              d.init_C[2], s));
-        PFM_STEP_TRY(pfm_update_T(3, d.n, d.T_grad_m[0], d.T_grad_m[1], d.T_grad_m[2], d.T_init[0], d.T_init[1],
-                                  d.T_init[2], d.T_init[0], d.T_init[1], d.T_init[2], s));
-        PFM_STEP_TRY(pfm_reset_T_identity(3, d.n, d.T_grad_m[0], d.T_grad_m[1], d.T_grad_m[2], s));
+        // with fold_T the previous step's RK4 already applied update_T and
+        // left T_grad_m at the identity
+        if (!d.fold_T) {
+            PFM_STEP_TRY(pfm_update_T(3, d.n, d.T_grad_m[0], d.T_grad_m[1], d.T_grad_m[2], d.T_init[0],
+                                      d.T_init[1], d.T_init[2], d.T_init[0], d.T_init[1], d.T_init[2], s));
+            PFM_STEP_TRY(pfm_reset_T_identity(3, d.n, d.T_grad_m[0], d.T_grad_m[1], d.T_grad_m[2], s));
+        }
     }
     if (d.use_midpoint_vel) PFM_STEP_TRY(midpoint(sc, st, dt, s));
     if (d.fused_p2g) {
@@ -130,10 +134,15 @@ int step3(Scene* sc, pfm_scene_state* st, double dt, hipStream_t s) {
         // (3D/run_3D.py:795-812)
         float* rec = (float*)pfm_p2g_record_buffer(g, d.n, d.workspace);
         PFM_CHECK_ARG(rec != nullptr, "pfm_p2g_record_buffer failed");
-        PFM_STEP_TRY(pfm_rk4_particles_p2g_record(g, d.n, d.pos, d.T_grad_m[0], d.T_grad_m[1], d.T_grad_m[2],
-                                                  d.u[0], d.u[1], d.u[2], (float)dt, d.T_init[0], d.T_init[1],
-                                                  d.T_init[2], d.init_C[0], d.init_C[1], d.init_C[2],
-                                                  d.init_imp, rec, s));
+        if (d.fold_T)
+            PFM_STEP_TRY(pfm_rk4_particles_p2g_record_fold(g, d.n, d.pos, d.u[0], d.u[1], d.u[2], (float)dt,
+                                                           d.T_init[0], d.T_init[1], d.T_init[2], d.init_C[0],
+                                                           d.init_C[1], d.init_C[2], d.init_imp, rec, s));
+        else
+            PFM_STEP_TRY(pfm_rk4_particles_p2g_record(g, d.n, d.pos, d.T_grad_m[0], d.T_grad_m[1], d.T_grad_m[2],
+                                                      d.u[0], d.u[1], d.u[2], (float)dt, d.T_init[0], d.T_init[1],
+                                                      d.T_init[2], d.init_C[0], d.init_C[1], d.init_C[2],
+                                                      d.init_imp, rec, s));
         PFM_STEP_TRY(pfm_bin_particles(g, d.n, d.pos, d.cell_of, d.cell_start, d.perm, d.workspace, d.ws_bytes, s));
         PFM_STEP_TRY(pfm_p2g_records(g, d.n, d.cell_start, d.perm, d.u[0], d.u[1], d.u[2], d.p2g_weight[0],
                                      d.p2g_weight[1], d.p2g_weight[2], d.workspace, d.ws_bytes, s));
@@ -196,6 +205,8 @@ extern "C" int pfm_scene_create(const pfm_scene_desc* desc, void** scene) {
     PFM_CHECK_ARG(d.reinit_every >= 1 && d.reinit_every_grad_m >= 1, "reinit intervals must be >= 1");
     PFM_CHECK_ARG(d.ppc_axis >= 1 && d.n >= 1, "bad particle counts");
     PFM_CHECK_ARG(d.CFL > 0.0 && d.visualize_dt > 0.0, "CFL and visualize_dt must be positive");
+    PFM_CHECK_ARG(!d.fold_T || (d.fused_p2g && d.reinit_every_grad_m == 1),
+                  "fold_T needs the fused path and a short reinitialisation every step");
     PFM_CHECK_ARG(d.mg && d.max_speed && d.pos && d.init_imp && d.cell_start && d.perm && d.workspace,
                   "null field pointer");
     for (int a = 0; a < d.dim; ++a)
@@ -204,7 +215,7 @@ extern "C" int pfm_scene_create(const pfm_scene_desc* desc, void** scene) {
                       "null field pointer (axis %d)", a);
     if (d.dim == 2) {
         PFM_CHECK_ARG(d.imp && d.T[0] && d.T[1] && d.C[0] && d.C[1], "2D scenes need imp, T and C");
-        PFM_CHECK_ARG(!d.fused_p2g, "the fused RK4 + P2G record path is 3D only");
+        PFM_CHECK_ARG(!d.fused_p2g && !d.fold_T, "the fused RK4 + P2G record path is 3D only");
     }
     Scene* sc = new (std::nothrow) Scene();
     if (!sc) {

@@ -38,7 +38,7 @@ class PfmSceneDesc(ctypes.Structure):
         ("CFL", ctypes.c_double), ("visualize_dt", ctypes.c_double),
         ("ppc_axis", ctypes.c_int32), ("particle_order", ctypes.c_int32), ("seed", ctypes.c_int64),
         ("max_iters", ctypes.c_int32), ("tol_mode", ctypes.c_int32), ("tol", ctypes.c_double),
-        ("fused_p2g", ctypes.c_int32), ("n", ctypes.c_int64),
+        ("fused_p2g", ctypes.c_int32), ("fold_T", ctypes.c_int32), ("n", ctypes.c_int64),
         ("u", c_vp * 3), ("p2g_weight", c_vp * 3), ("psi_grid", c_vp * 3), ("T_grid", c_vp * 3),
         ("pos", c_vp), ("init_imp", c_vp), ("imp_grad_m", c_vp),
         ("T_init", c_vp * 3), ("T_grad_m", c_vp * 3), ("init_C", c_vp * 3),
@@ -83,6 +83,7 @@ _SIGS = {
     "pfm_rk4_particles_p2g_record": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_f] + [c_vp] * 8
                                     + [c_vp],
     "pfm_p2g_records": [c_vp, c_i64, c_vp, c_vp] + [c_vp] * 6 + [c_vp, c_sz, c_vp],
+    "pfm_rk4_particles_p2g_record_fold": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_f] + [c_vp] * 7 + [c_vp, c_vp],
     "pfm_p2g_record_buffer": [c_vp, c_i64, c_vp],
     "pfm_max_speed": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "pfm_split_central": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],

@@ -244,19 +244,27 @@ class Binning:
                self.workspace.data_ptr(), self.workspace.numel(), _stream(stream))
 
 
-    def rk4_record(self, dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, stream=None):
+    def rk4_record(self, dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, stream=None, fold=False):
         """RK4_T_forward (3D/run_3D.py:795-796) with the P2G particle
         composition (3D/run_3D.py:460-479) fused in: advects pos/T_grad_m in
-        place and writes one P2G record per particle into the workspace."""
+        place and writes one P2G record per particle into the workspace.
+        fold: T_grad_m is the identity on entry and stays so in memory, and
+        T_init becomes cols(T_grad_m) cols(T_init) (the next update_T)."""
         d, n = pos.shape
         g = L.make_grid(self.res, dx)
         lib = L.load()
         rec = lib.pfm_p2g_record_buffer(ctypes.byref(g), n, self.workspace.data_ptr())
         if not rec:
             raise L.PfmError("pfm_p2g_record_buffer failed")
-        L.call("pfm_rk4_particles_p2g_record", ctypes.byref(g), n, _pvec(pos, d, n, "pos"),
-               *_cols(Tgm, d, n, "T_grad_m"), *_u_ptrs(u_vel, self.res), float(dt), *_cols(Tin, d, n, "T_init"),
-               *_cols(init_C, d, n, "init_C"), _pvec(init_imp, d, n, "init_imp"), rec, _stream(stream))
+        if fold:
+            L.call("pfm_rk4_particles_p2g_record_fold", ctypes.byref(g), n, _pvec(pos, d, n, "pos"),
+                   *_u_ptrs(u_vel, self.res), float(dt), *_cols(Tin, d, n, "T_init"), *_cols(init_C, d, n, "init_C"),
+                   _pvec(init_imp, d, n, "init_imp"), rec, _stream(stream))
+        else:
+            L.call("pfm_rk4_particles_p2g_record", ctypes.byref(g), n, _pvec(pos, d, n, "pos"),
+                   *_cols(Tgm, d, n, "T_grad_m"), *_u_ptrs(u_vel, self.res), float(dt),
+                   *_cols(Tin, d, n, "T_init"), *_cols(init_C, d, n, "init_C"), _pvec(init_imp, d, n, "init_imp"),
+                   rec, _stream(stream))
         _mark_written(pos)
 
     def p2g_from_records(self, dx, n, u_out, W=None, stream=None):
@@ -269,12 +277,12 @@ class Binning:
                L.ptr(self.perm, dtype=torch.int32), *_u_ptrs(u_out, self.res), *Wp, self.workspace.data_ptr(),
                self.workspace.numel(), _stream(stream))
 
-    def rk4_then_p2g(self, dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, u_out, W=None, stream=None):
+    def rk4_then_p2g(self, dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, u_out, W=None, stream=None, fold=False):
         """RK4_T_forward followed by P2G (3D/run_3D.py:795-812) with the P2G
         composition fused into the RK4 kernel: advect pos/T_grad_m in place,
         bin the new positions, then scatter into u_out (may alias u_vel only
         after the RK4 reads are done -- the launches are ordered on one stream)."""
-        self.rk4_record(dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, stream)
+        self.rk4_record(dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, stream, fold=fold)
         self.bin(pos, dx, stream, force=True)
         self.p2g_from_records(dx, pos.shape[1], u_out, W, stream)
 

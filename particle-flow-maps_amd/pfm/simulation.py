@@ -51,6 +51,9 @@ class StepConfig:
     write_imp_grad_m: bool = True
     # MGPCG level storage: 0 fp32 (the reference), 1 fp16, 2 bf16
     mg_precision: int = 0
+    # 3D with reinit_every_grad_m == 1 and the fused RK4 + P2G record: the
+    # RK4 kernel applies the next step's update_T itself (same arithmetic)
+    fold_T: bool = True
 
 
 def reinit_decision(i: int, ik: int, cfg: StepConfig, dim: int):
@@ -198,8 +201,15 @@ class Simulation:
             self._scene = _NativeScene(self)
         return self._scene.step(self, dt)
 
+    @property
+    def folds_T(self) -> bool:
+        """Whether the RK4 kernel applies update_T for the next step (3D, fused
+        RK4 + P2G record, a short reinitialisation every step)."""
+        return bool(self.dim == 3 and self.fused_p2g and self.cfg.fold_T and self.cfg.reinit_every_grad_m == 1)
+
     def _step3(self, i, dt):
         cfg = self.cfg
+        fold = self.folds_T
         long_r, reseed, short, self.ik = reinit_decision(i, self.ik, cfg, 3)
         if long_r:
             if reseed:
@@ -210,13 +220,14 @@ class Simulation:
         if short:
             core.g2p(self.res, self.dx, self.pos, self.u,
                      self.imp_grad_m if self.imp_grad_m is not None else None, self.init_C)
-            core.update_T(self.T_grad_m, self.T_init, self.T_init)
-            core.reset_T_identity(self.T_grad_m)
+            if not fold:  # else the previous RK4 applied it and left T_grad_m = I
+                core.update_T(self.T_grad_m, self.T_init, self.T_init)
+                core.reset_T_identity(self.T_grad_m)
         if cfg.use_midpoint_vel:
             self._midpoint(dt)
         if self.fused_p2g:
             self.binning.rk4_then_p2g(self.dx, self.pos, self.T_grad_m, self.u, dt, self.T_init, self.init_C,
-                                      self.init_imp, self.u, W=self.p2g_weight)
+                                      self.init_imp, self.u, W=self.p2g_weight, fold=fold)
         else:
             core.rk4_particles(self.res, self.dx, self.pos, self.T_grad_m, self.u, dt, True)
             self.binning.bin(self.pos, self.dx, force=True)
@@ -280,6 +291,7 @@ class _NativeScene:
         D.max_iters, D.tol = int(cfg.max_iters), float(cfg.tol)
         D.tol_mode = 0 if cfg.tol_mode == "abs" else 1
         D.fused_p2g = int(sim.fused_p2g)
+        D.fold_T = int(sim.folds_T)
         D.n = sim.n
 
         def p(t):

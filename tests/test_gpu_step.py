@@ -131,3 +131,40 @@ def test_native_step_equals_python_step(oracle, pfm, res, kw, fused):
     assert py.sub_t == nat.sub_t
     for x, y in zip(py.u, nat.u):
         assert torch.equal(x, y)
+
+
+def test_fold_T_equals_separate_update(oracle, pfm):
+    """The RK4 + record kernel that applies the next update_T itself gives the
+    same step as update_T + reset_T_to_identity run at the next short
+    reinitialisation: equal velocities, positions and impulse state, and its
+    T_init equals update_T of the unfolded state, bit for bit."""
+    from pfm import core
+    from pfm.simulation import Simulation, StepConfig
+
+    res = (16, 18, 20)
+    kw = dict(particles_per_cell=8, reinit_every=4, reinit_every_grad_m=1, CFL=0.5, tol=1e-6, tol_mode="rel",
+              seed=2, particle_order=1, write_imp_grad_m=False)
+    vcc = G(oracle.get_central(res, smooth_mac_field(res, seed=34)))
+    sims = []
+    for fold in (False, True):
+        sim = Simulation(res, StepConfig(fold_T=fold, **kw))
+        sim.set_velocity_from_central(vcc.clone())
+        sims.append(sim)
+    a, b = sims
+    assert not a.folds_T and b.folds_T
+    for step in range(6):
+        assert a.step() == b.step()
+        for x, y in zip(a.u, b.u):
+            assert torch.equal(x, y), step
+        assert torch.equal(a.pos, b.pos)
+        for x, y in zip(a.init_C, b.init_C):
+            assert torch.equal(x, y)
+        want = [torch.empty_like(t) for t in a.T_init]
+        core.update_T(a.T_grad_m, a.T_init, want)
+        for x, y in zip(want, b.T_init):
+            assert torch.equal(x, y), step
+        for m in range(3):
+            eye = torch.zeros_like(b.T_grad_m[m])
+            eye[m] = 1.0
+            assert torch.equal(b.T_grad_m[m], eye)
+    assert a.poisson_iters == b.poisson_iters
