@@ -206,6 +206,46 @@ def pmc_traffic(fetch_db, write_db):
     return "\n".join(lines)
 
 
+VALU_PROFILE = os.path.join(ROOT, "profiles", "pmc_valu_step_256.md")
+# MI355X: 256 CUs x 4 SIMDs, a wave64 fp32 VALU instruction takes 4 cycles on
+# a 16-lane SIMD (no packed FP32 here), 2.4 GHz peak engine clock
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+
+
+def pmc_valu(db_path):
+    """Markdown table of VALU instructions per wave per kernel from a
+    `--pmc SQ_WAVES SQ_INSTS_VALU` run (SQ counters sample a fixed subset of
+    the waves, so the ratio per wave is what they give)."""
+    import sqlite3
+
+    db = sqlite3.connect(db_path)
+    acc = {}
+    for cname, name, val in db.execute("select counter_name, name, avg(counter_value) from pmc_events "
+                                       "group by counter_name, name"):
+        acc.setdefault(name, {})[cname] = val
+    rows = []
+    for name, d in acc.items():
+        if d.get("SQ_WAVES", 0) > 0 and "SQ_INSTS_VALU" in d:
+            rows.append((d["SQ_INSTS_VALU"] / d["SQ_WAVES"], name.split("(")[0].replace("void ", "")))
+    lines = ["| kernel | VALU instructions per wave |", "|---|---|"]
+    for v, k in sorted(rows, reverse=True)[:40]:
+        lines.append(f"| {k} | {v:.1f} |")
+    return "\n".join(lines)
+
+
+def committed_valu(kernel):
+    """VALU instructions per wave of `kernel` from the committed SQ summary."""
+    try:
+        with open(VALU_PROFILE) as fh:
+            for line in fh:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 2 and cells[0] == kernel:
+                    return float(cells[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def committed_traffic(kernel):
     """HBM bytes per dispatch of `kernel` from the committed PMC summary."""
     try:
@@ -373,6 +413,7 @@ def main():
     ap.add_argument("--summarize", metavar="DB", help="print a text summary of a rocprofv3 results db and exit")
     ap.add_argument("--by-grid", action="store_true", help="with --summarize: split rows by launch size")
     ap.add_argument("--match", default=None, help="with --summarize --by-grid: keep kernels containing this")
+    ap.add_argument("--pmc-valu", metavar="DB", help="print the VALU-instructions-per-wave table of an SQ run and exit")
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="print the per-kernel HBM traffic table of two --traffic-probe PMC runs and exit")
     args = ap.parse_args()
@@ -381,6 +422,9 @@ def main():
         return
     if args.pmc_traffic:
         print(pmc_traffic(*args.pmc_traffic))
+        return
+    if args.pmc_valu:
+        print(pmc_valu(args.pmc_valu))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the step shards by slab "
@@ -513,6 +557,14 @@ def main():
     achieved = alg_bytes / (rk4_ms * 1e-3) / 1e9
     roof_kernel = "pfm::k_rk4_particles<3, true, true>" if fold else "pfm::k_rk4_particles<3, true, false>"
     traffic = committed_traffic(roof_kernel) if args.res == 256 else None
+    # the kernel is VALU-issue bound: its instruction rate against the peak
+    vpw = committed_valu(roof_kernel) if args.res == 256 else None
+    valu = None
+    if vpw is not None:
+        waves = math.ceil(sim.n / 64)
+        rate = vpw * waves / (rk4_ms * 1e-3)
+        valu = {"insts_per_wave": vpw, "waves_per_launch": waves, "achieved_wave_insts_per_s": rate,
+                "peak_wave_insts_per_s": VALU_PEAK_WAVE_INSTS, "frac": rate / VALU_PEAK_WAVE_INSTS}
 
     out = {
         "metric": "particle-steps/s",
@@ -541,7 +593,7 @@ def main():
         "phase_ms": phases,
         "roofline": {"kernel": roof_kernel, "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_ms": rk4_ms, "alg_bytes_per_launch": alg_bytes},
+                     "kernel_ms": rk4_ms, "alg_bytes_per_launch": alg_bytes, "valu": valu},
     }
     del sim
     torch.cuda.empty_cache()
