@@ -385,6 +385,88 @@ int pfm_scene_create(const pfm_scene_desc* desc, void** scene);
 int pfm_scene_step(void* scene, pfm_scene_state* state, double dt, pfm_stream_t stream);
 int pfm_scene_destroy(void* scene);
 
+/* ---- batched 2D scenes (extension) --------------------------------------
+ * Many independent 2D scenes of different sizes advanced together: every
+ * phase of the 2D step (2D/run_2D.py:772-909) runs as one persistent launch
+ * over all scenes, the per-phase launch sequences are captured once into HIP
+ * graphs and replayed, and the persistent grid sizes (and the block size of
+ * the per-scene coarse multigrid kernel) are picked by pfm_batch2d_tune.
+ * The reference runs one scene per process; this has no reference
+ * counterpart beyond the per-scene step it repeats.
+ *
+ * Layout: every field is the concatenation of the scenes' fields in scene
+ * order.  Scene s (res_x, res_y = res[2s], res[2s+1], dx = 1/res_y) owns
+ * n_s = res_x res_y ppc_axis^2 particles, (res_x+1) res_y x-faces,
+ * res_x (res_y+1) y-faces and res_x res_y cells; particle vector fields are
+ * (2, P) and grid-attached vector fields (2, F_axis) over the totals
+ * (pfm_batch2d_sizes).  Walls are free slip (Neumann) on every side. */
+typedef struct {
+    int32_t n_scenes;
+    const int32_t* res;           /* host: res_x, res_y per scene (>= 4) */
+    const int64_t* seed;          /* host: per-scene jitter seed (< 0: the reference lattice) */
+    int32_t ppc_axis;             /* sqrt(particles_per_cell) */
+    int32_t particle_order;       /* pfm_seed_particles order */
+    int32_t reinit_every, reinit_every_grad_m;
+    int32_t use_midpoint_vel, reinit_particle_pos;
+    double CFL, visualize_dt;
+    int32_t base_level;           /* MGPCG levels: int(log2(min res)) - base_level + 1 */
+    int32_t max_iters;            /* >= 0 */
+    int32_t tol_mode;             /* 0: absolute r.r < tol, 1: relative |r| < tol |r0| */
+    double tol;
+    float* u[2];                  /* face velocities (x, y) */
+    float* p2g_weight[2];         /* P2G weights, also the copy of u before advect_u */
+    float* psi_grid[2];           /* (2, F_axis) */
+    float* T_grid[2];             /* (2, F_axis) */
+    float* pos;                   /* (2, P) */
+    float* T[2];
+    float* T_init[2];
+    float* T_grad_m[2];
+    float* C[2];
+    float* init_C[2];
+    float* imp;
+    float* init_imp;
+    float* imp_grad_m;            /* may be NULL (written, never read) */
+    int32_t* cell_of;             /* P */
+    int32_t* cell_start;          /* cells + 1 */
+    int32_t* perm;                /* P */
+} pfm_batch2d_desc;
+
+typedef struct {
+    int64_t i;                    /* step index, -1 before the first step */
+    int64_t ik;
+    int32_t n_solves;             /* projections of the last step */
+    int32_t lockstep_iters[2];    /* PCG iterations the batch ran per projection */
+} pfm_batch2d_state;
+
+/* Totals over the scenes: out = {particles, x-faces, y-faces, cells}. */
+int pfm_batch2d_sizes(int32_t n_scenes, const int32_t* res, int32_t ppc_axis, int64_t* out);
+/* Validates the descriptor, builds the scene and unit tables and allocates
+ * the multigrid levels and step workspaces (device bytes: *bytes_out). */
+int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** batch, int64_t* bytes_out);
+/* Poisson projection of every scene's u (MGPCG_2.Poisson, 2D/mgpcg.py). */
+int pfm_batch2d_project(void* batch, pfm_stream_t stream);
+/* initialize_2d_particles (2D/run_2D.py:731-738) for every scene. */
+int pfm_batch2d_init_particles(void* batch, pfm_stream_t stream);
+/* One step of every scene (2D/run_2D.py:772-909; dt per scene from its own
+ * max speed, CFL and frame clip).  Synchronises on the batch's own stream
+ * once per group of PCG iterations; ordered after prior work on `stream`. */
+int pfm_batch2d_step(void* batch, pfm_batch2d_state* state, pfm_stream_t stream);
+/* Per-scene results of the last step (host arrays of n_scenes, iters and
+ * converged of 2 per scene; any may be NULL).  Synchronises. */
+int pfm_batch2d_scene_info(void* batch, double* dt, double* sub_t, int32_t* frame, int32_t* iters,
+                           int32_t* converged);
+/* Autotuner: runs `rounds` steps per candidate launch shape, timing every
+ * kernel kind, and keeps the fastest shape per kind.  The fields are
+ * stepped, so the caller restores its state afterwards. */
+int pfm_batch2d_tune(void* batch, pfm_batch2d_state* state, pfm_stream_t stream);
+/* The chosen shapes: caps[k] = persistent grid cap of kernel kind k (0: one
+ * workgroup per unit), then the coarse kernel's block size; returns the
+ * number of values written (at most n). */
+int pfm_batch2d_tuning(void* batch, int32_t* caps, int32_t n);
+/* 0: issue every launch directly; 1 (default): capture and replay graphs. */
+int pfm_batch2d_set_graphs(void* batch, int32_t enable);
+int pfm_batch2d_destroy(void* batch);
+
 #ifdef __cplusplus
 }
 #endif

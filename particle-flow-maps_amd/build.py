@@ -23,7 +23,8 @@ BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libpfm.so")
 ARCH = os.environ.get("PFM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["abi.cpp", "particles.hip", "grid.hip", "p2g.hip", "mgpcg.hip", "scene.cpp", "smoke.hip"]
+SOURCES = ["abi.cpp", "particles.hip", "grid.hip", "p2g.hip", "mgpcg.hip", "scene.cpp", "smoke.hip",
+           "batch.cpp", "batch_mg.hip"]
 
 CFLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
@@ -41,14 +42,18 @@ def hipcc() -> str:
 
 # Sources whose fp32 passes must equal a sequential restatement bit for bit:
 # HIP's __fmul_rn/__fadd_rn are plain operators, so only a file-wide
-# -ffp-contract=off keeps the compiler from fusing them into FMAs.
-NO_CONTRACT = {"mgpcg.hip", "smoke.hip"}
+# -ffp-contract=off keeps the compiler from fusing them into FMAs.  The
+# other sources use -ffp-contract=on: a multiply-add is fused only inside one
+# source expression, the same way in every kernel that inlines it (the
+# default, fast, also fuses across statements depending on the surrounding
+# code, so two kernels sharing one device function could round differently).
+NO_CONTRACT = {"mgpcg.hip", "smoke.hip", "batch_mg.hip"}
 
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD_DIR, os.path.basename(src) + ".o")
     lang = ["-x", "hip"] if src.endswith((".hip", ".cpp")) else []
-    extra = ["-ffp-contract=off"] if src in NO_CONTRACT else []
+    extra = ["-ffp-contract=off"] if src in NO_CONTRACT else ["-ffp-contract=on"]
     cmd = [hipcc(), *lang, *CFLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

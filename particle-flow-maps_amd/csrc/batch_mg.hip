@@ -1,0 +1,581 @@
+// batch_mg.hip -- the projection of the batched 2D path: apply_bc,
+// divergence, MGPCG and subtract_grad_p for many 2D scenes at once
+// (2D/mgpcg.py, the MGPCG_2 solver of 3D/mgpcg.py:337-347 with the PCG of
+// :202-280 and the V-cycle of :182-200).
+//
+// Every scene keeps its own PCG scalars (BMg) and runs its own iteration
+// count: all scenes advance in lockstep, one persistent launch per PCG
+// phase, and a scene that has converged (or hit max_iters) is skipped by
+// every later launch.  The per-cell arithmetic is that of the single-scene
+// solver (same operation order, every operation rounded on its own: this
+// file is compiled with -ffp-contract=off); the dot products are summed per
+// unit of 256 cells and then per scene in unit order, so a scene's result
+// does not depend on which other scenes share its batch.
+//
+// Levels of a scene above kBatchCoarseCells cells run as tiled persistent
+// launches (one per red/black colour); the remaining levels, the bottom
+// solve included, run in one workgroup per scene out of LDS.
+#include "pfm_batch.h"
+
+namespace pfm {
+
+// Wall flags (BWalls): a Neumann (or, in 2D, Dirichlet) wall removes a
+// neighbour from the diagonal (2D/mgpcg.py:106-115).
+
+__device__ __forceinline__ float bnfn(const BWalls& w, int n0, int n1, int i, int j) {
+    float num = 4.0f;
+    if (i <= 0 && w.lo0) num -= 1.0f;
+    if (i >= n0 - 1 && w.hi0) num -= 1.0f;
+    if (j <= 0 && w.lo1) num -= 1.0f;
+    if (j >= n1 - 1 && w.hi1) num -= 1.0f;
+    return num;
+}
+
+// neighbor_sum (3D/mgpcg.py:92-104) at x (the cell), axis 0 then axis 1,
+// + then - (the order of the single-scene solver).
+__device__ __forceinline__ float bnsum(const float* x, int n0, int n1, int i, int j) {
+    float ret = 0.0f;
+    if (i < n0 - 1) ret = ret + x[n1];
+    if (i > 0) ret = ret + x[-n1];
+    if (j < n1 - 1) ret = ret + x[1];
+    if (j > 0) ret = ret + x[-1];
+    return ret;
+}
+
+__device__ __forceinline__ bool scene_active(const BMg* mg, int s) { return mg[s].active != 0; }
+
+// ---- field kernels -------------------------------------------------------------
+
+// apply_bc with Neumann walls everywhere: the normal faces on the domain
+// boundary are zero (2D/mgpcg.py:292-324).
+__global__ void __launch_bounds__(kBlock) kb_zero_walls(BUnits t, float* __restrict__ ux, float* __restrict__ uy) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        int64_t f = (int64_t)w.chunk * kBlock + threadIdx.x;
+        const int n0 = S.g.res[0], n1 = S.g.res[1];
+        if (f < S.nf[0]) {
+            const int i = (int)(f / n1);
+            if (i == 0 || i == n0) ux[S.f_off[0] + f] = 0.0f;
+        } else {
+            f -= S.nf[0];
+            if (f >= S.nf[1]) continue;
+            const int j = (int)(f % (n1 + 1));
+            if (j == 0 || j == n1) uy[S.f_off[1] + f] = 0.0f;
+        }
+    }
+}
+
+// divergence (3D/mgpcg.py:380-390) fused with init(r, -1) (:71-79).
+__global__ void __launch_bounds__(kBlock) kb_div_init(BUnits t, const float* __restrict__ ux,
+                                                      const float* __restrict__ uy, float* __restrict__ r,
+                                                      float* __restrict__ x, float* __restrict__ p,
+                                                      float* __restrict__ Ap) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (q >= S.nc) continue;
+        const int n1 = S.g.res[1];
+        const int i = (int)(q / n1), j = (int)(q % n1);
+        const float* vx = ux + S.f_off[0];
+        const float* vy = uy + S.f_off[1];
+        const float vl = vx[(int64_t)i * n1 + j];
+        const float vr = vx[(int64_t)(i + 1) * n1 + j];
+        const float vb = vy[(int64_t)i * (n1 + 1) + j];
+        const float vt = vy[(int64_t)i * (n1 + 1) + j + 1];
+        const float div = ((vr - vl) + vt) - vb;
+        const int64_t g = S.c_off + q;
+        r[g] = div * -1.0f;
+        x[g] = 0.0f;
+        p[g] = 0.0f;
+        Ap[g] = 0.0f;
+    }
+}
+
+// subtract_grad_p (3D/mgpcg.py:392-418), pressure outside the domain = 0.
+__global__ void __launch_bounds__(kBlock) kb_subtract_grad_p(BUnits t, const float* __restrict__ x,
+                                                             float* __restrict__ ux, float* __restrict__ uy) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        int64_t f = (int64_t)w.chunk * kBlock + threadIdx.x;
+        const int n0 = S.g.res[0], n1 = S.g.res[1];
+        const float* pr = x + S.c_off;
+        if (f < S.nf[0]) {
+            const int i = (int)(f / n1), j = (int)(f % n1);
+            const float prv = i >= n0 ? 0.0f : pr[(int64_t)i * n1 + j];
+            const float plv = i - 1 < 0 ? 0.0f : pr[(int64_t)(i - 1) * n1 + j];
+            float* v = ux + S.f_off[0] + f;
+            *v = *v - (prv - plv);
+        } else {
+            f -= S.nf[0];
+            if (f >= S.nf[1]) continue;
+            const int i = (int)(f / (n1 + 1)), j = (int)(f % (n1 + 1));
+            const float prv = j >= n1 ? 0.0f : pr[(int64_t)i * n1 + j];
+            const float plv = j - 1 < 0 ? 0.0f : pr[(int64_t)i * n1 + j - 1];
+            float* v = uy + S.f_off[1] + f;
+            *v = *v - (prv - plv);
+        }
+    }
+}
+
+// ---- level-0 PCG kernels --------------------------------------------------------
+
+// Block sum of one double over kBlock threads (valid in thread 0).
+__device__ __forceinline__ double bblock_sum(double v, double* sh) { return block_sum(v, sh); }
+
+// Per-unit partial sums of r (kBRedSum), r.r or z.r on level 0.
+__global__ void __launch_bounds__(kBlock) kb_reduce(BUnits t, int mode, const float* __restrict__ r,
+                                                    const float* __restrict__ z, const BMg* __restrict__ mg,
+                                                    double* __restrict__ part) {
+    __shared__ double sh[kBlock / 64];
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        if (!scene_active(mg, w.scene)) continue;
+        const BScene& S = t.sc[w.scene];
+        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
+        double v = 0.0;
+        if (q < S.nc) {
+            const int64_t g = S.c_off + q;
+            const double rv = (double)r[g];
+            v = mode == kBRedSum ? rv : (mode == kBRedRR ? rv * rv : (double)z[g] * rv);
+        }
+        v = bblock_sum(v, sh);
+        if (threadIdx.x == 0) part[S.part_off + w.chunk] = v;
+    }
+}
+
+// update_p (3D/mgpcg.py:147-150): p = z + beta p.
+__global__ void __launch_bounds__(kBlock) kb_update_p(BUnits t, const float* __restrict__ z, float* __restrict__ p,
+                                                      const BMg* __restrict__ mg) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        if (!scene_active(mg, w.scene)) continue;
+        const BScene& S = t.sc[w.scene];
+        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (q >= S.nc) continue;
+        const int64_t g = S.c_off + q;
+        p[g] = z[g] + mg[w.scene].beta * p[g];
+    }
+}
+
+// compute_Ap (3D/mgpcg.py:117-122) with the p.Ap partials.
+__global__ void __launch_bounds__(kBlock) kb_ap_dot(BUnits t, BWalls wl, const float* __restrict__ p,
+                                                    float* __restrict__ Ap, const BMg* __restrict__ mg,
+                                                    double* __restrict__ part) {
+    __shared__ double sh[kBlock / 64];
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        if (!scene_active(mg, w.scene)) continue;
+        const BScene& S = t.sc[w.scene];
+        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
+        double v = 0.0;
+        if (q < S.nc) {
+            const int n0 = S.g.res[0], n1 = S.g.res[1];
+            const int i = (int)(q / n1), j = (int)(q % n1);
+            const int64_t g = S.c_off + q;
+            const float a = bnfn(wl, n0, n1, i, j) * p[g] - bnsum(p + g, n0, n1, i, j);
+            Ap[g] = a;
+            v = (double)p[g] * (double)a;
+        }
+        v = bblock_sum(v, sh);
+        if (threadIdx.x == 0) part[S.part_off + w.chunk] = v;
+    }
+}
+
+// update_x + update_r (3D/mgpcg.py:137-145) with the r.r and sum(r) partials.
+__global__ void __launch_bounds__(kBlock) kb_update_xr(BUnits t, float* __restrict__ x, float* __restrict__ r,
+                                                       const float* __restrict__ p, const float* __restrict__ Ap,
+                                                       const BMg* __restrict__ mg, double* __restrict__ part,
+                                                       double* __restrict__ part2) {
+    __shared__ double sh[kBlock / 64];
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        if (!scene_active(mg, w.scene)) continue;
+        const BScene& S = t.sc[w.scene];
+        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
+        const float alpha = mg[w.scene].alpha;
+        double s2 = 0.0, s1 = 0.0;
+        if (q < S.nc) {
+            const int64_t g = S.c_off + q;
+            x[g] = x[g] + alpha * p[g];
+            const float rq = r[g] - alpha * Ap[g];
+            r[g] = rq;
+            s2 = (double)rq * (double)rq;
+            s1 = (double)rq;
+        }
+        s2 = bblock_sum(s2, sh);
+        s1 = bblock_sum(s1, sh);
+        if (threadIdx.x == 0) {
+            part[S.part_off + w.chunk] = s2;
+            part2[S.part_off + w.chunk] = s1;
+        }
+    }
+}
+
+// recenter (3D/mgpcg.py:174-180): r -= mean.
+__global__ void __launch_bounds__(kBlock) kb_sub_mean(BUnits t, float* __restrict__ r, const BMg* __restrict__ mg) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        if (!scene_active(mg, w.scene)) continue;
+        const BScene& S = t.sc[w.scene];
+        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (q >= S.nc) continue;
+        const int64_t g = S.c_off + q;
+        r[g] = r[g] - mg[w.scene].mean;
+    }
+}
+
+// The scalar recurrences of solve (3D/mgpcg.py:229-275) in double, one block
+// per scene; the scene's partials are summed in unit order.
+__global__ void __launch_bounds__(kBlock) kb_finalize(const BScene* __restrict__ sc, int mode, double eps,
+                                                      int max_iters, const double* __restrict__ part,
+                                                      const double* __restrict__ part2, BMg* __restrict__ mg) {
+    __shared__ double sh[kBlock / 64];
+    const int s = blockIdx.x;
+    if (!scene_active(mg, s)) return;
+    const BScene& S = sc[s];
+    double v = 0.0, v1 = 0.0;
+    for (int b = threadIdx.x; b < S.n_part; b += blockDim.x) v += part[S.part_off + b];
+    v = block_sum(v, sh);
+    if (mode == kBFinRTr) {
+        for (int b = threadIdx.x; b < S.n_part; b += blockDim.x) v1 += part2[S.part_off + b];
+        v1 = block_sum(v1, sh);
+    }
+    if (threadIdx.x != 0) return;
+    BMg& m = mg[s];
+    const double ncell = (double)S.nc;
+    switch (mode) {
+        case kBFinMean:
+            m.sum_r = v;
+            m.mean = (float)(v / ncell);
+            break;
+        case kBFinR0:
+            m.r0 = v;
+            m.rTr = v;
+            break;
+        case kBFinZTr:
+            m.zTr = v;
+            break;
+        case kBFinAlpha:
+            m.pAp = v;
+            m.alpha = (float)(m.zTr / (v + eps));
+            break;
+        case kBFinRTr:
+            m.rTr = v;
+            m.sum_r = v1;
+            m.mean = (float)(v1 / ncell);
+            if (v < m.thresh) {  // converged (3D/mgpcg.py:257-258)
+                m.active = 0;
+                m.converged = 1;
+                m.iters = m.it;
+            }
+            break;
+        case kBFinBeta:
+            m.beta = (float)(v / (m.zTr + eps));
+            m.zTr = v;
+            m.it += 1;
+            if (max_iters >= 0 && m.it >= max_iters) {
+                m.active = 0;
+                m.iters = m.it;
+            }
+            break;
+    }
+}
+
+// Start of a solve: fresh scalars (the reference's beta persists across
+// solves), every scene active unless max_iters is 0.
+__global__ void kb_mg_reset(int n, BMg* __restrict__ mg, int tol_mode, double tol, int max_iters) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float beta = mg[s].beta;
+    BMg m{};
+    m.beta = beta;
+    m.active = max_iters != 0;
+    m.thresh = tol_mode == 1 ? 0.0 : tol;
+    mg[s] = m;
+}
+
+// Relative tolerance: thresh = tol^2 r0 (as the host computes it).
+__global__ void kb_set_thresh(int n, BMg* __restrict__ mg, double tol) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) mg[s].thresh = tol * tol * mg[s].r0;
+}
+
+// Iterations and convergence of the solve just finished, into slot k.
+__global__ void kb_save_solve(int n, const BMg* __restrict__ mg, int k, int32_t* __restrict__ iters,
+                              int32_t* __restrict__ conv) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    iters[2 * s + k] = mg[s].converged ? mg[s].iters : mg[s].it;
+    conv[2 * s + k] = mg[s].converged;
+}
+
+__global__ void __launch_bounds__(kBlock) kb_count_active(int n, const BMg* __restrict__ mg, int32_t* out) {
+    __shared__ int32_t sh[kBlock / 64];
+    int32_t c = 0;
+    for (int s = threadIdx.x; s < n; s += blockDim.x) c += mg[s].active;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t tot = 0;
+        for (int k = 0; k < kBlock / 64; ++k) tot += sh[k];
+        *out = tot;
+    }
+}
+
+// ---- V-cycle kernels of the tiled levels ---------------------------------------
+
+// smooth (3D/mgpcg.py:165-172), one colour: a unit is kBlock red/black pairs
+// of one level (pairs along axis 1; the thread takes the member of colour
+// `phase`).  `zero`: z is known to be zero (the first sweep after the
+// level was cleared), so the neighbour sum is 0.
+__global__ void __launch_bounds__(kBlock) kb_smooth(BUnits t, int l, int phase, int zero, BWalls wl,
+                                                    float* __restrict__ z, const float* __restrict__ r,
+                                                    const BMg* __restrict__ mg) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        if (!scene_active(mg, w.scene)) continue;
+        const BScene& S = t.sc[w.scene];
+        const int n0 = S.ld[l][0], n1 = S.ld[l][1];
+        const int hp = (n1 + 1) >> 1;
+        const int64_t pr = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (pr >= (int64_t)n0 * hp) continue;
+        const int i = (int)(pr / hp);
+        const int j = 2 * (int)(pr % hp) + ((i + phase) & 1);
+        if (j >= n1) continue;
+        const int64_t g = S.l_off[l] + (int64_t)i * n1 + j;
+        const float ns = zero ? 0.0f : bnsum(z + g, n0, n1, i, j);
+        z[g] = (r[g] + ns) / bnfn(wl, n0, n1, i, j);
+    }
+}
+
+// restrict (3D/mgpcg.py:152-158) as a gather of the children of coarse cell
+// (ci, cj); the last coarse cell of an odd extent also owns the trailing
+// fine cell.  2D scale 1 / (dim - 1) = 1.
+__device__ __forceinline__ float restrict_cell(const BWalls& wl, const float* z, const float* r, int f0, int f1,
+                                               int c0, int c1, int ci, int cj) {
+    const int h0 = ci == c0 - 1 ? f0 - 1 - 2 * ci : 1;
+    const int h1 = cj == c1 - 1 ? f1 - 1 - 2 * cj : 1;
+    float acc = 0.0f;
+    for (int b = 0; b <= h0; ++b)
+        for (int c = 0; c <= h1; ++c) {
+            const int i = 2 * ci + b, j = 2 * cj + c;
+            const int64_t q = (int64_t)i * f1 + j;
+            const float res = r[q] - (bnfn(wl, f0, f1, i, j) * z[q] - bnsum(z + q, f0, f1, i, j));
+            acc = acc + (res * 1.0f) / 1.0f;
+        }
+    return acc;
+}
+
+__global__ void __launch_bounds__(kBlock) kb_restrict(BUnits t, int l, BWalls wl, const float* __restrict__ z,
+                                                      float* __restrict__ r, const BMg* __restrict__ mg) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        if (!scene_active(mg, w.scene)) continue;
+        const BScene& S = t.sc[w.scene];
+        const int c0 = S.ld[l + 1][0], c1 = S.ld[l + 1][1];
+        const int64_t qc = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (qc >= (int64_t)c0 * c1) continue;
+        const int ci = (int)(qc / c1), cj = (int)(qc % c1);
+        const int64_t fo = S.l_off[l];
+        r[S.l_off[l + 1] + qc] = restrict_cell(wl, z + fo, r + fo, S.ld[l][0], S.ld[l][1], c0, c1, ci, cj);
+    }
+}
+
+// prolongate (3D/mgpcg.py:160-163).
+__global__ void __launch_bounds__(kBlock) kb_prolong(BUnits t, int l, float* __restrict__ z,
+                                                     const BMg* __restrict__ mg) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        if (!scene_active(mg, w.scene)) continue;
+        const BScene& S = t.sc[w.scene];
+        const int n0 = S.ld[l][0], n1 = S.ld[l][1];
+        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (q >= (int64_t)n0 * n1) continue;
+        const int i = (int)(q / n1), j = (int)(q % n1);
+        const int c0 = S.ld[l + 1][0], c1 = S.ld[l + 1][1];
+        const int pi = min(i >> 1, c0 - 1), pj = min(j >> 1, c1 - 1);
+        const int64_t g = S.l_off[l] + q;
+        z[g] = z[g] + z[S.l_off[l + 1] + (int64_t)pi * c1 + pj];
+    }
+}
+
+// ---- per-scene coarse V-cycle out of LDS ------------------------------------
+
+__device__ __forceinline__ void lds_smooth(const BWalls& wl, float* z, const float* r, int n0, int n1, int phase,
+                                           bool zero) {
+    const int hp = (n1 + 1) >> 1;
+    for (int pr = threadIdx.x; pr < n0 * hp; pr += blockDim.x) {
+        const int i = pr / hp;
+        const int j = 2 * (pr % hp) + ((i + phase) & 1);
+        if (j >= n1) continue;
+        const int q = i * n1 + j;
+        const float ns = zero ? 0.0f : bnsum(z + q, n0, n1, i, j);
+        z[q] = (r[q] + ns) / bnfn(wl, n0, n1, i, j);
+    }
+    __syncthreads();
+}
+
+// Levels [coarse, n_levels) of each active scene: the rest of the V-cycle
+// (3D/mgpcg.py:182-200) from r[coarse] to z[coarse], bottom solve included.
+__global__ void kb_coarse(const BScene* __restrict__ sc, BWalls wl, float* __restrict__ zg,
+                          float* __restrict__ rg, const BMg* __restrict__ mg) {
+    extern __shared__ float lds[];
+    const int s = blockIdx.x;
+    if (!scene_active(mg, s)) return;
+    const BScene& S = sc[s];
+    const int lc = S.coarse, L = S.n_levels;
+    int zo[kBatchMaxLevels], ro[kBatchMaxLevels];
+    int off = 0;
+    for (int l = lc; l < L; ++l) {
+        const int n = S.ld[l][0] * S.ld[l][1];
+        zo[l] = off;
+        ro[l] = off + n;
+        off += 2 * n;
+    }
+    {
+        const int n = S.ld[lc][0] * S.ld[lc][1];
+        for (int q = threadIdx.x; q < n; q += blockDim.x) lds[ro[lc] + q] = rg[S.l_off[lc] + q];
+    }
+    __syncthreads();
+    for (int l = lc; l < L - 1; ++l) {
+        const int n0 = S.ld[l][0], n1 = S.ld[l][1];
+        float* z = lds + zo[l];
+        const float* r = lds + ro[l];
+        lds_smooth(wl, z, r, n0, n1, 0, true);
+        lds_smooth(wl, z, r, n0, n1, 1, false);
+        lds_smooth(wl, z, r, n0, n1, 0, false);
+        lds_smooth(wl, z, r, n0, n1, 1, false);
+        const int c0 = S.ld[l + 1][0], c1 = S.ld[l + 1][1];
+        for (int q = threadIdx.x; q < c0 * c1; q += blockDim.x)
+            lds[ro[l + 1] + q] = restrict_cell(wl, z, r, n0, n1, c0, c1, q / c1, q % c1);
+        __syncthreads();
+    }
+    {
+        const int b = L - 1;
+        const int n0 = S.ld[b][0], n1 = S.ld[b][1];
+        float* z = lds + zo[b];
+        const float* r = lds + ro[b];
+        for (int it = 0; it < 50; ++it) {
+            lds_smooth(wl, z, r, n0, n1, 0, it == 0);
+            lds_smooth(wl, z, r, n0, n1, 1, false);
+        }
+    }
+    for (int l = L - 2; l >= lc; --l) {
+        const int n0 = S.ld[l][0], n1 = S.ld[l][1];
+        const int c0 = S.ld[l + 1][0], c1 = S.ld[l + 1][1];
+        float* z = lds + zo[l];
+        const float* zc = lds + zo[l + 1];
+        for (int q = threadIdx.x; q < n0 * n1; q += blockDim.x) {
+            const int i = q / n1, j = q % n1;
+            z[q] = z[q] + zc[min(i >> 1, c0 - 1) * c1 + min(j >> 1, c1 - 1)];
+        }
+        __syncthreads();
+        const float* r = lds + ro[l];
+        lds_smooth(wl, z, r, n0, n1, 1, false);
+        lds_smooth(wl, z, r, n0, n1, 0, false);
+        lds_smooth(wl, z, r, n0, n1, 1, false);
+        lds_smooth(wl, z, r, n0, n1, 0, false);
+    }
+    {
+        const int n = S.ld[lc][0] * S.ld[lc][1];
+        for (int q = threadIdx.x; q < n; q += blockDim.x) zg[S.l_off[lc] + q] = lds[zo[lc] + q];
+    }
+}
+
+// ---- launchers ---------------------------------------------------------------
+
+#define PFM_B_LAUNCH(kern, blocks, ...)                                                   \
+    do {                                                                                  \
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, __VA_ARGS__);          \
+        PFM_LAUNCH_CHECK();                                                               \
+    } while (0)
+
+int bm_zero_walls(BUnits faces, float* ux, float* uy, unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_zero_walls, blocks, faces, ux, uy);
+    return PFM_OK;
+}
+int bm_div_init(BUnits cells, const float* ux, const float* uy, float* r, float* x, float* p, float* Ap,
+                unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_div_init, blocks, cells, ux, uy, r, x, p, Ap);
+    return PFM_OK;
+}
+int bm_subtract_grad_p(BUnits faces, const float* x, float* ux, float* uy, unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_subtract_grad_p, blocks, faces, x, ux, uy);
+    return PFM_OK;
+}
+int bm_reduce(BUnits cells, int mode, const float* r, const float* z, const BMg* mg, double* part, unsigned blocks,
+              hipStream_t s) {
+    PFM_B_LAUNCH(kb_reduce, blocks, cells, mode, r, z, mg, part);
+    return PFM_OK;
+}
+int bm_update_p(BUnits cells, const float* z, float* p, const BMg* mg, unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_update_p, blocks, cells, z, p, mg);
+    return PFM_OK;
+}
+int bm_ap_dot(BUnits cells, BWalls wl, const float* p, float* Ap, const BMg* mg, double* part, unsigned blocks,
+              hipStream_t s) {
+    PFM_B_LAUNCH(kb_ap_dot, blocks, cells, wl, p, Ap, mg, part);
+    return PFM_OK;
+}
+int bm_update_xr(BUnits cells, float* x, float* r, const float* p, const float* Ap, const BMg* mg, double* part,
+                 double* part2, unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_update_xr, blocks, cells, x, r, p, Ap, mg, part, part2);
+    return PFM_OK;
+}
+int bm_sub_mean(BUnits cells, float* r, const BMg* mg, unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_sub_mean, blocks, cells, r, mg);
+    return PFM_OK;
+}
+int bm_finalize(int n_scenes, const BScene* sc, int mode, double eps, int max_iters, const double* part,
+                const double* part2, BMg* mg, hipStream_t s) {
+    PFM_B_LAUNCH(kb_finalize, n_scenes, sc, mode, eps, max_iters, part, part2, mg);
+    return PFM_OK;
+}
+int bm_mg_reset(int n_scenes, BMg* mg, int tol_mode, double tol, int max_iters, hipStream_t s) {
+    hipLaunchKernelGGL(kb_mg_reset, dim3(grid_for(n_scenes)), dim3(kBlock), 0, s, n_scenes, mg, tol_mode, tol,
+                       max_iters);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+int bm_set_thresh(int n_scenes, BMg* mg, double tol, hipStream_t s) {
+    hipLaunchKernelGGL(kb_set_thresh, dim3(grid_for(n_scenes)), dim3(kBlock), 0, s, n_scenes, mg, tol);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+int bm_save_solve(int n_scenes, const BMg* mg, int k, int32_t* iters, int32_t* conv, hipStream_t s) {
+    hipLaunchKernelGGL(kb_save_solve, dim3(grid_for(n_scenes)), dim3(kBlock), 0, s, n_scenes, mg, k, iters, conv);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+int bm_count_active(int n_scenes, const BMg* mg, int32_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(kb_count_active, dim3(1), dim3(kBlock), 0, s, n_scenes, mg, out);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+int bm_smooth(BUnits pairs, int l, int phase, int zero, BWalls wl, float* z, const float* r, const BMg* mg,
+              unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_smooth, blocks, pairs, l, phase, zero, wl, z, r, mg);
+    return PFM_OK;
+}
+int bm_restrict(BUnits coarse_cells, int l, BWalls wl, const float* z, float* r, const BMg* mg, unsigned blocks,
+                hipStream_t s) {
+    PFM_B_LAUNCH(kb_restrict, blocks, coarse_cells, l, wl, z, r, mg);
+    return PFM_OK;
+}
+int bm_prolong(BUnits cells, int l, float* z, const BMg* mg, unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_prolong, blocks, cells, l, z, mg);
+    return PFM_OK;
+}
+int bm_coarse(int n_scenes, const BScene* sc, BWalls wl, float* z, float* r, const BMg* mg, int threads,
+              size_t lds_bytes, hipStream_t s) {
+    hipLaunchKernelGGL(kb_coarse, dim3(n_scenes), dim3(threads), lds_bytes, s, sc, wl, z, r, mg);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+}  // namespace pfm

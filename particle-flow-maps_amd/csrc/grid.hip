@@ -2,7 +2,7 @@
 // face flow map, semi-Lagrangian advection with the Jacobian, identity
 // resets, CFL max speed, centre<->face averaging, vortex initial conditions
 // and the kinetic-energy diagnostic.  One thread per face or cell.
-#include "pfm_host.h"
+#include "pfm_batch.h"
 
 namespace pfm {
 
@@ -25,15 +25,11 @@ __device__ __forceinline__ void face_ijk(const GridDev& g, int a, int64_t f, int
 }
 
 // RK4_grid: 3D/run_3D.py:144-173, 2D/run_2D.py:233-261.  dT = G @ T (rows form).
+// Face f of one axis; psi and T are (d, nf) with component stride nf.
 template <int DIM>
-__global__ void __launch_bounds__(kBlock, 4) k_rk4_grid(GridDev g, int64_t nf,
-                                                     float* __restrict__ psi,
-                                                     float* __restrict__ T,
-                                                     const float* __restrict__ ux,
-                                                     const float* __restrict__ uy,
-                                                     const float* __restrict__ uz, float dt) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
+__device__ __forceinline__ void rk4_face(const GridDev& g, int64_t f, int64_t nf, float* __restrict__ psi,
+                                         float* __restrict__ T, const float* __restrict__ ux,
+                                         const float* __restrict__ uy, const float* __restrict__ uz, float dt) {
     const float neg_dt = -1.0f * dt;
     const float hs[3] = {0.5f * neg_dt, 0.5f * neg_dt, 1.0f * neg_dt};
     const float ws[4] = {1.f, 2.f, 2.f, 1.f};
@@ -77,6 +73,18 @@ __global__ void __launch_bounds__(kBlock, 4) k_rk4_grid(GridDev g, int64_t nf,
     }
 }
 
+template <int DIM>
+__global__ void __launch_bounds__(kBlock, 4) k_rk4_grid(GridDev g, int64_t nf,
+                                                     float* __restrict__ psi,
+                                                     float* __restrict__ T,
+                                                     const float* __restrict__ ux,
+                                                     const float* __restrict__ uy,
+                                                     const float* __restrict__ uz, float dt) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    rk4_face<DIM>(g, f, nf, psi, T, ux, uy, uz, dt);
+}
+
 // advect_u: 3D/run_3D.py:296-312 (one launch covers the faces of all axes).
 struct AdvectArgs {
     const float* u0[3];
@@ -85,6 +93,22 @@ struct AdvectArgs {
     const float* psi[3];
     int64_t nf[3];
 };
+
+// Face f of axis ax: psi_ax / T_ax have component stride `stride`.
+template <int DIM>
+__device__ __forceinline__ void advect_face(const GridDev& g, const float* const u0[3], const float* __restrict__ psi,
+                                            const float* __restrict__ T, int64_t f, int64_t stride,
+                                            float* __restrict__ u1) {
+    float p[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) p[c] = psi[c * stride + f];
+    float u[3], G[3][3];
+    interp_mac<DIM, false>(u0[0], u0[1], u0[2], g, p, u, G);
+    float acc = T[f] * u[0];
+#pragma unroll
+    for (int c = 1; c < DIM; ++c) acc += T[c * stride + f] * u[c];
+    u1[f] = acc;
+}
 
 template <int DIM>
 __global__ void __launch_bounds__(kBlock, 4) k_advect_u(GridDev g, AdvectArgs a) {
@@ -95,34 +119,54 @@ __global__ void __launch_bounds__(kBlock, 4) k_advect_u(GridDev g, AdvectArgs a)
         ++ax;
     }
     if (f >= a.nf[ax]) return;
-    const int64_t nf = a.nf[ax];
-    float p[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) p[c] = a.psi[ax][c * nf + f];
-    float u[3], G[3][3];
-    interp_mac<DIM, false>(a.u0[0], a.u0[1], a.u0[2], g, p, u, G);
-    float acc = a.T[ax][f] * u[0];
-#pragma unroll
-    for (int c = 1; c < DIM; ++c) acc += a.T[ax][c * nf + f] * u[c];
-    a.u1[ax][f] = acc;
+    advect_face<DIM>(g, a.u0, a.psi[ax], a.T[ax], f, a.nf[ax], a.u1[ax]);
 }
 
 // reset_to_identity_grid with x/y/z_coords_func: 3D/run_3D.py:106-120,
 // 3D/taichi_utils.py:33-46: psi = (I + 0.5 off-axis) * dx, T = e_axis.
+// Face f of the grid's array of `axis`, stored at index `at` of psi / T
+// (component stride nf).
+template <int DIM>
+__device__ __forceinline__ void reset_identity_face(const GridDev& g, int axis, int64_t f, int64_t at, int64_t nf,
+                                                    float* __restrict__ psi, float* __restrict__ T) {
+    int I[3];
+    face_ijk<DIM>(g, axis, f, I);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        float x = c == axis ? (float)I[c] : __fadd_rn((float)I[c], 0.5f);
+        psi[c * nf + at] = __fmul_rn(x, g.dx);
+        T[c * nf + at] = c == axis ? 1.0f : 0.0f;
+    }
+}
+
 template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_reset_identity_grid(GridDev g, int axis, int64_t nf,
                                                                 float* __restrict__ psi,
                                                                 float* __restrict__ T) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nf) return;
-    int I[3];
-    face_ijk<DIM>(g, axis, f, I);
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-        float x = c == axis ? (float)I[c] : __fadd_rn((float)I[c], 0.5f);
-        psi[c * nf + f] = __fmul_rn(x, g.dx);
-        T[c * nf + f] = c == axis ? 1.0f : 0.0f;
+    reset_identity_face<DIM>(g, axis, f, f, nf, psi, T);
+}
+
+// |u| at the centre of cell (i, j, k) (calc_max_speed, 3D/run_3D.py:95-103).
+template <int DIM>
+__device__ __forceinline__ float cell_speed(const GridDev& g, const float* __restrict__ ux,
+                                            const float* __restrict__ uy, const float* __restrict__ uz, int i, int j,
+                                            int k) {
+    float s2;
+    if (DIM == 3) {
+        const int ry = g.res[1], rz = g.res[2];
+        float u = 0.5f * (ux[((int64_t)i * ry + j) * rz + k] + ux[((int64_t)(i + 1) * ry + j) * rz + k]);
+        float v = 0.5f * (uy[((int64_t)i * (ry + 1) + j) * rz + k] + uy[((int64_t)i * (ry + 1) + j + 1) * rz + k]);
+        float w = 0.5f * (uz[((int64_t)i * ry + j) * (rz + 1) + k] + uz[((int64_t)i * ry + j) * (rz + 1) + k + 1]);
+        s2 = __fadd_rn(__fadd_rn(__fmul_rn(u, u), __fmul_rn(v, v)), __fmul_rn(w, w));
+    } else {
+        const int ry = g.res[1];
+        float u = 0.5f * (ux[(int64_t)i * ry + j] + ux[(int64_t)(i + 1) * ry + j]);
+        float v = 0.5f * (uy[(int64_t)i * (ry + 1) + j] + uy[(int64_t)i * (ry + 1) + j + 1]);
+        s2 = __fadd_rn(__fmul_rn(u, u), __fmul_rn(v, v));
     }
+    return __fsqrt_rn(s2);
 }
 
 // calc_max_speed: 3D/run_3D.py:95-103.  Max over non-negative floats is exact
@@ -146,27 +190,172 @@ __global__ void __launch_bounds__(kBlock) k_max_speed(GridDev g, const float* __
             j = (int)(q % g.res[1]);
             i = (int)(q / g.res[1]);
         }
-        float s2;
-        if (DIM == 3) {
-            const int rx = g.res[0], ry = g.res[1], rz = g.res[2];
-            float u = 0.5f * (ux[((int64_t)i * ry + j) * rz + k] + ux[((int64_t)(i + 1) * ry + j) * rz + k]);
-            float v = 0.5f * (uy[((int64_t)i * (ry + 1) + j) * rz + k] + uy[((int64_t)i * (ry + 1) + j + 1) * rz + k]);
-            float w = 0.5f * (uz[((int64_t)i * ry + j) * (rz + 1) + k] + uz[((int64_t)i * ry + j) * (rz + 1) + k + 1]);
-            (void)rx;
-            s2 = __fadd_rn(__fadd_rn(__fmul_rn(u, u), __fmul_rn(v, v)), __fmul_rn(w, w));
-        } else {
-            const int ry = g.res[1];
-            float u = 0.5f * (ux[(int64_t)i * ry + j] + ux[(int64_t)(i + 1) * ry + j]);
-            float v = 0.5f * (uy[(int64_t)i * (ry + 1) + j] + uy[(int64_t)i * (ry + 1) + j + 1]);
-            s2 = __fadd_rn(__fmul_rn(u, u), __fmul_rn(v, v));
-        }
-        m = fmaxf(m, __fsqrt_rn(s2));
+        m = fmaxf(m, cell_speed<DIM>(g, ux, uy, uz, i, j, k));
     }
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
 }
 
 __global__ void k_set_u32(unsigned int* p, unsigned int v) { *p = v; }
+
+// ---- batched 2D face and cell kernels ------------------------------------------
+// A face unit covers kBlock faces of one scene, numbered over both axes
+// (x faces first, as in k_advect_u).
+struct BFace {
+    int ax;
+    int64_t f;  // face within the scene's array of axis ax
+};
+
+__device__ __forceinline__ bool face_of_unit(const BScene& S, int32_t chunk, BFace& bf) {
+    int64_t f = (int64_t)chunk * kBlock + threadIdx.x;
+    bf.ax = 0;
+    if (f >= S.nf[0]) {
+        f -= S.nf[0];
+        bf.ax = 1;
+    }
+    bf.f = f;
+    return f < S.nf[bf.ax];
+}
+
+__global__ void __launch_bounds__(kBlock) kb_max_speed(BUnits t, const float* __restrict__ ux,
+                                                       const float* __restrict__ uy, BState* __restrict__ st) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
+        float m = 0.0f;
+        if (q < S.nc) {
+            const int j = (int)(q % S.g.res[1]), i = (int)(q / S.g.res[1]);
+            m = cell_speed<2>(S.g, ux + S.f_off[0], uy + S.f_off[1], nullptr, i, j, 0);
+        }
+        m = wave_max(m);
+        if ((threadIdx.x & 63) == 0) atomicMax(&st[w.scene].ms_bits, __float_as_uint(m));
+    }
+}
+
+__global__ void kb_ms_init(int n, BState* st) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) st[s].ms_bits = __float_as_uint(1.e-3f);  // max_speed[None] = 1.e-3 (3D/run_3D.py:97)
+}
+
+// curr_dt = CFL * dx / max_speed and the frame clip (2D/run_2D.py:782-800), in
+// double as the host driver computes it.
+__global__ void kb_dt(int n, const BScene* __restrict__ sc, BState* __restrict__ st, double CFL, double vis_dt) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    BState& b = st[s];
+    const double dx = 1.0 / (double)sc[s].g.res[1];
+    double curr = CFL * dx / (double)__uint_as_float(b.ms_bits);
+    if (b.sub_t + curr >= vis_dt) {
+        curr = vis_dt - b.sub_t;
+        b.sub_t = 0.0;
+        b.frame += 1;
+    } else {
+        b.sub_t += curr;
+    }
+    b.dt = curr;
+}
+
+__global__ void __launch_bounds__(kBlock) kb_reset_identity_grid(BUnits t, int64_t F0, int64_t F1,
+                                                                 float* __restrict__ psi_x, float* __restrict__ psi_y,
+                                                                 float* __restrict__ T_x, float* __restrict__ T_y) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        BFace bf;
+        if (!face_of_unit(S, w.chunk, bf)) continue;
+        const int64_t off = S.f_off[bf.ax];
+        if (bf.ax == 0)
+            reset_identity_face<2>(S.g, 0, bf.f, off + bf.f, F0, psi_x, T_x);
+        else
+            reset_identity_face<2>(S.g, 1, bf.f, off + bf.f, F1, psi_y, T_y);
+    }
+}
+
+// The face's own index within its scene decodes (i, j); storage uses the
+// global index with the axis' total as component stride.
+__global__ void __launch_bounds__(kBlock, 4) kb_rk4_grid(BUnits t, int64_t F0, int64_t F1,
+                                                      float* __restrict__ psi_x, float* __restrict__ psi_y,
+                                                      float* __restrict__ T_x, float* __restrict__ T_y,
+                                                      const float* __restrict__ ux, const float* __restrict__ uy,
+                                                      const BState* __restrict__ st) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        BFace bf;
+        if (!face_of_unit(S, w.chunk, bf)) continue;
+        const float dt = (float)(0.5 * st[w.scene].dt);
+        const int64_t gf = S.f_off[bf.ax] + bf.f;
+        if (bf.ax == 0)
+            rk4_face<2>(S.g, gf, F0, psi_x, T_x, ux + S.f_off[0], uy + S.f_off[1], nullptr, dt);
+        else
+            rk4_face<2>(S.g, gf, F1, psi_y, T_y, ux + S.f_off[0], uy + S.f_off[1], nullptr, dt);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock, 4) kb_advect_u(BUnits t, int64_t F0, int64_t F1,
+                                                      const float* __restrict__ u0x, const float* __restrict__ u0y,
+                                                      float* __restrict__ u1x, float* __restrict__ u1y,
+                                                      const float* __restrict__ T_x, const float* __restrict__ T_y,
+                                                      const float* __restrict__ psi_x,
+                                                      const float* __restrict__ psi_y) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        BFace bf;
+        if (!face_of_unit(S, w.chunk, bf)) continue;
+        const float* u0[3] = {u0x + S.f_off[0], u0y + S.f_off[1], nullptr};
+        const int64_t gf = S.f_off[bf.ax] + bf.f;
+        if (bf.ax == 0)
+            advect_face<2>(S.g, u0, psi_x, T_x, gf, F0, u1x);
+        else
+            advect_face<2>(S.g, u0, psi_y, T_y, gf, F1, u1y);
+    }
+}
+
+int b_max_speed(BUnits cells, const float* ux, const float* uy, BState* st, unsigned blocks, hipStream_t s) {
+    hipLaunchKernelGGL(kb_max_speed, dim3(blocks), dim3(kBlock), 0, s, cells, ux, uy, st);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int b_ms_init(int n_scenes, BState* st, hipStream_t s) {
+    hipLaunchKernelGGL(kb_ms_init, dim3(grid_for(n_scenes)), dim3(kBlock), 0, s, n_scenes, st);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int b_dt(int n_scenes, const BScene* sc, BState* st, double CFL, double visualize_dt, hipStream_t s) {
+    hipLaunchKernelGGL(kb_dt, dim3(grid_for(n_scenes)), dim3(kBlock), 0, s, n_scenes, sc, st, CFL, visualize_dt);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int b_reset_identity_grid(BUnits faces, const int64_t F[2], float* psi_x, float* psi_y, float* T_x, float* T_y,
+                          unsigned blocks, hipStream_t s) {
+    hipLaunchKernelGGL(kb_reset_identity_grid, dim3(blocks), dim3(kBlock), 0, s, faces, F[0], F[1], psi_x, psi_y,
+                       T_x, T_y);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int b_rk4_grid(BUnits faces, const int64_t F[2], float* psi_x, float* psi_y, float* T_x, float* T_y,
+               const float* ux, const float* uy, const BState* st, unsigned blocks, hipStream_t s) {
+    hipLaunchKernelGGL(kb_rk4_grid, dim3(blocks), dim3(kBlock), 0, s, faces, F[0], F[1], psi_x, psi_y, T_x, T_y,
+                       ux, uy, st);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int b_advect_u(BUnits faces, const int64_t F[2], const float* u0x, const float* u0y, float* u1x, float* u1y,
+               const float* T_x, const float* T_y, const float* psi_x, const float* psi_y, unsigned blocks,
+               hipStream_t s) {
+    hipLaunchKernelGGL(kb_advect_u, dim3(blocks), dim3(kBlock), 0, s, faces, F[0], F[1], u0x, u0y, u1x, u1y, T_x,
+                       T_y, psi_x, psi_y);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
 
 // split_central_vector: 3D/taichi_utils.py:87-100 (clamped samples).
 template <int DIM>

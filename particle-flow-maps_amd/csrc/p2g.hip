@@ -14,7 +14,7 @@
 // windows of a tile in LDS in a fixed order; a combine pass adds the <= 2^d
 // tile partials of every face in tile order.  No float atomics anywhere, so
 // the result is bit-reproducible run to run.
-#include "pfm_host.h"
+#include "pfm_batch.h"
 
 namespace pfm {
 
@@ -31,25 +31,23 @@ __global__ void __launch_bounds__(kBlock) k_fill_i32(int32_t* p, int64_t n, int3
 // slot from the count atomic (reordered later).  Runs of equal cells among
 // consecutive lanes -- the common case, particles are numbered cell by cell
 // and move < 1 cell per step -- are merged into one atomic per run.
+// cell = int(pos / dx) per axis, clamped to the grid (particle i, component stride n).
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_bin_count(GridDev g, int64_t n,
-                                                      const float* __restrict__ pos,
-                                                      int32_t* __restrict__ cell_of,
-                                                      int32_t* __restrict__ counts,
-                                                      int32_t* __restrict__ slot) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    int32_t cell = -1;
-    if (valid) {
-        int ci[3] = {0, 0, 0};
+__device__ __forceinline__ int32_t particle_cell(const GridDev& g, const float* __restrict__ pos, int64_t i,
+                                                 int64_t n) {
+    int ci[3] = {0, 0, 0};
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) {
-            int v = (int)(pos[c * n + i] * g.inv_dx);
-            ci[c] = max(0, min(v, g.res[c] - 1));
-        }
-        cell = (int32_t)(((int64_t)ci[0] * g.res[1] + ci[1]) * g.res[2] + ci[2]);
-        cell_of[i] = cell;
+    for (int c = 0; c < DIM; ++c) {
+        int v = (int)(pos[c * n + i] * g.inv_dx);
+        ci[c] = max(0, min(v, g.res[c] - 1));
     }
+    return (int32_t)(((int64_t)ci[0] * g.res[1] + ci[1]) * g.res[2] + ci[2]);
+}
+
+// The count atomics of one wave: each run of equal cells among consecutive
+// valid lanes takes one atomic; every valid lane gets its slot in the cell.
+__device__ __forceinline__ void count_slots(int32_t cell, bool valid, int32_t* __restrict__ counts,
+                                            int32_t* __restrict__ slot_i) {
     const int lane = threadIdx.x & 63;
     const int32_t prev = __shfl_up(cell, 1, 64);
     const bool head = valid && (lane == 0 || prev != cell);
@@ -65,7 +63,44 @@ __global__ void __launch_bounds__(kBlock) k_bin_count(GridDev g, int64_t n,
     int32_t base = 0;
     if (head) base = atomicAdd(&counts[cell], next - lane);
     base = __shfl(base, first, 64);
-    if (valid) slot[i] = base + (lane - first);
+    if (valid) *slot_i = base + (lane - first);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_bin_count(GridDev g, int64_t n,
+                                                      const float* __restrict__ pos,
+                                                      int32_t* __restrict__ cell_of,
+                                                      int32_t* __restrict__ counts,
+                                                      int32_t* __restrict__ slot) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    int32_t cell = -1;
+    if (valid) {
+        cell = particle_cell<DIM>(g, pos, i, n);
+        cell_of[i] = cell;
+    }
+    count_slots(cell, valid, counts, slot + (valid ? i : 0));
+}
+
+// Batched: the cell index of a scene's particle is global (the scene's cell
+// offset added), so one counting sort over all scenes' cells orders every
+// scene's particles inside that scene's cell range.
+__global__ void __launch_bounds__(kBlock) kb_bin_count(BUnits t, int64_t P, const float* __restrict__ pos,
+                                                       int32_t* __restrict__ cell_of, int32_t* __restrict__ counts,
+                                                       int32_t* __restrict__ slot) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
+        const bool valid = loc < S.n;
+        const int64_t i = S.p_off + (valid ? loc : 0);
+        int32_t cell = -1;
+        if (valid) {
+            cell = (int32_t)(S.c_off + particle_cell<2>(S.g, pos, i, P));
+            cell_of[i] = cell;
+        }
+        count_slots(cell, valid, counts, slot + i);
+    }
 }
 
 // Exclusive scan, phase 1: per-tile totals.
@@ -220,11 +255,8 @@ struct PrepArgs {
 // Record pass of P2G: one thread per sorted slot q reads particle perm[q] and
 // writes record q (write_p2g_record).
 template <int DIM, bool FUSED>
-__global__ void __launch_bounds__(kBlock) k_p2g_prep(GridDev g, int64_t n, PrepArgs a,
-                                                     const int32_t* __restrict__ perm,
-                                                     float* __restrict__ pay) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
+__device__ __forceinline__ void prep_record(const GridDev& g, int64_t q, int64_t n, const PrepArgs& a,
+                                            const int32_t* __restrict__ perm, float* __restrict__ pay) {
     const int64_t i = perm[q];
     float pos[3] = {0.f, 0.f, 0.f}, A[3][3], B[3][3], C[3][3], imp[3];
 #pragma unroll
@@ -241,6 +273,28 @@ __global__ void __launch_bounds__(kBlock) k_p2g_prep(GridDev g, int64_t n, PrepA
         }
     }
     write_p2g_record<DIM>(pay + q * Pay<DIM>::kRec, pos, g.inv_dx, FUSED, A, B, C, imp);
+}
+
+template <int DIM, bool FUSED>
+__global__ void __launch_bounds__(kBlock) k_p2g_prep(GridDev g, int64_t n, PrepArgs a,
+                                                     const int32_t* __restrict__ perm,
+                                                     float* __restrict__ pay) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    prep_record<DIM, FUSED>(g, q, n, a, perm, pay);
+}
+
+// Batched: a scene's sorted slots are its own particle range (its particles
+// fill exactly its cells' part of the global sort).
+__global__ void __launch_bounds__(kBlock) kb_p2g_prep(BUnits t, int64_t P, PrepArgs a,
+                                                      const int32_t* __restrict__ perm, float* __restrict__ pay) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (loc >= S.n) continue;
+        prep_record<2, false>(S.g, S.p_off + loc, P, a, perm, pay);
+    }
 }
 
 // ---- tiled P2G --------------------------------------------------------------
@@ -381,13 +435,11 @@ __device__ __forceinline__ void tile_axis(const GridDev& g, const float* __restr
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_p2g_tile(GridDev g, const int32_t* __restrict__ start,
-                                                     const float* __restrict__ pay,
-                                                     const int32_t* __restrict__ perm, int nt1, int nt2,
-                                                     float* __restrict__ partial) {
+__device__ __forceinline__ void p2g_tile_body(const GridDev& g, const int32_t* __restrict__ start,
+                                              const float* __restrict__ pay, const int32_t* __restrict__ perm,
+                                              int tile, int nt1, int nt2, float* __restrict__ part, float* lu,
+                                              float* lw) {
     using TC = TileCfg<DIM>;
-    __shared__ float lu[kTileLds], lw[kTileLds];
-    const int tile = blockIdx.x;
     const int tb[3] = {tile / (nt1 * nt2), (tile / nt2) % nt1, tile % nt2};
     int tl[3];
     if (DIM == 3) {
@@ -413,10 +465,33 @@ __global__ void __launch_bounds__(kBlock) k_p2g_tile(GridDev g, const int32_t* _
         s = start[cell];
         e = start[cell + 1];
     }
-    float* part = partial + (int64_t)tile * TileStride<DIM>::value;
     tile_axis<DIM, 0>(g, pay, perm, s, e, cc, tl, lu, lw, part);
     tile_axis<DIM, 1>(g, pay, perm, s, e, cc, tl, lu, lw, part);
     if (DIM == 3) tile_axis<DIM, DIM == 3 ? 2 : 1>(g, pay, perm, s, e, cc, tl, lu, lw, part);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_p2g_tile(GridDev g, const int32_t* __restrict__ start,
+                                                     const float* __restrict__ pay,
+                                                     const int32_t* __restrict__ perm, int nt1, int nt2,
+                                                     float* __restrict__ partial) {
+    __shared__ float lu[kTileLds], lw[kTileLds];
+    const int tile = blockIdx.x;
+    p2g_tile_body<DIM>(g, start, pay, perm, tile, nt1, nt2, partial + (int64_t)tile * TileStride<DIM>::value, lu,
+                       lw);
+}
+
+// Batched: one unit per tile; records are in global sorted order, the
+// scene's cell_start entries start at its cell offset.
+__global__ void __launch_bounds__(kBlock) kb_p2g_tile(BUnits t, const int32_t* __restrict__ start,
+                                                      const float* __restrict__ pay, float* __restrict__ partial) {
+    __shared__ float lu[kTileLds], lw[kTileLds];
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        p2g_tile_body<2>(S.g, start + S.c_off, pay, nullptr, w.chunk, S.nt[1], 1,
+                         partial + (S.t_off + w.chunk) * (int64_t)TileStride<2>::value, lu, lw);
+    }
 }
 
 struct CombineArgs {
@@ -457,15 +532,8 @@ __device__ __forceinline__ void combine_face(const GridDev& g, const CombineArgs
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_p2g_combine(GridDev g, CombineArgs a,
-                                                        const float* __restrict__ partial) {
-    int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int A = 0;
-    while (A < DIM - 1 && f >= a.nf[A]) {
-        f -= a.nf[A];
-        ++A;
-    }
-    if (f >= a.nf[A]) return;
+__device__ __forceinline__ void combine_one(const GridDev& g, const CombineArgs& a, int A, int64_t f,
+                                            const float* __restrict__ partial) {
     int S[3], I[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) S[c] = g.res[c] + (c == A ? 1 : 0);
@@ -489,6 +557,48 @@ __global__ void __launch_bounds__(kBlock) k_p2g_combine(GridDev g, CombineArgs a
     if (sw > 0.f) su = su / sw;
     a.u[A][f] = su;
     if (a.W[A]) a.W[A][f] = sw;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_p2g_combine(GridDev g, CombineArgs a,
+                                                        const float* __restrict__ partial) {
+    int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int A = 0;
+    while (A < DIM - 1 && f >= a.nf[A]) {
+        f -= a.nf[A];
+        ++A;
+    }
+    if (f >= a.nf[A]) return;
+    combine_one<DIM>(g, a, A, f, partial);
+}
+
+__global__ void __launch_bounds__(kBlock) kb_p2g_combine(BUnits t, float* ux, float* uy, float* Wx, float* Wy,
+                                                         const float* __restrict__ partial) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        int64_t f = (int64_t)w.chunk * kBlock + threadIdx.x;
+        int A = 0;
+        if (f >= S.nf[0]) {
+            f -= S.nf[0];
+            A = 1;
+        }
+        if (f >= S.nf[A]) continue;
+        CombineArgs a;
+        a.u[0] = ux + S.f_off[0];
+        a.u[1] = uy + S.f_off[1];
+        a.u[2] = nullptr;
+        a.W[0] = Wx ? Wx + S.f_off[0] : nullptr;
+        a.W[1] = Wy ? Wy + S.f_off[1] : nullptr;
+        a.W[2] = nullptr;
+        a.nf[0] = S.nf[0];
+        a.nf[1] = S.nf[1];
+        a.nf[2] = 0;
+        a.nt[0] = S.nt[0];
+        a.nt[1] = S.nt[1];
+        a.nt[2] = 1;
+        combine_one<2>(S.g, a, A, f, partial + S.t_off * (int64_t)TileStride<2>::value);
+    }
 }
 
 struct Workspace {
@@ -716,6 +826,54 @@ static int tile_and_combine(const GridDev& g, const int32_t* cell_start, const i
         hipLaunchKernelGGL(k_p2g_combine<3>, dim3(grid_for(total)), dim3(kBlock), 0, s, g, ca, w.partial);
     else
         hipLaunchKernelGGL(k_p2g_combine<2>, dim3(grid_for(total)), dim3(kBlock), 0, s, g, ca, w.partial);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+}  // namespace pfm
+
+// ---- launchers of the batched 2D binning and P2G -------------------------------
+namespace pfm {
+
+size_t b_p2g_partial_floats(int64_t tiles) { return (size_t)tiles * TileStride<2>::value; }
+
+int b_bin(BUnits parts, int64_t P, int64_t C, const float* pos, int32_t* cell_of, int32_t* cell_start,
+          int32_t* perm, int32_t* counts, int32_t* tiles, int32_t* slot, unsigned blocks, hipStream_t s) {
+    const int64_t ntiles = (C + kScanTile - 1) / kScanTile;
+    unsigned fb = grid_for(C);
+    if (fb > 4096) fb = 4096;
+    hipLaunchKernelGGL(k_fill_i32, dim3(fb), dim3(kBlock), 0, s, counts, C, 0);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_bin_count, dim3(blocks), dim3(kBlock), 0, s, parts, P, pos, cell_of, counts, slot);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ntiles), dim3(kBlock), 0, s, counts, C, tiles);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, tiles, ntiles);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(kBlock), 0, s, counts, C, tiles, cell_start);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(P)), dim3(kBlock), 0, s, P, cell_of, slot, cell_start, perm);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(C)), dim3(kBlock), 0, s, C, cell_start, perm);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int b_p2g(BUnits parts, BUnits tiles, BUnits faces, int64_t P, const int64_t F[2], const float* pos,
+          const float* imp, const float* Cx, const float* Cy, const int32_t* cell_start, const int32_t* perm,
+          float* rec, float* partial, float* ux, float* uy, float* Wx, float* Wy, const unsigned blocks[3],
+          hipStream_t s) {
+    (void)F;
+    PrepArgs pa{};
+    pa.pos = pos;
+    pa.imp = imp;
+    pa.C[0] = Cx;
+    pa.C[1] = Cy;
+    hipLaunchKernelGGL(kb_p2g_prep, dim3(blocks[0]), dim3(kBlock), 0, s, parts, P, pa, perm, rec);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_p2g_tile, dim3(blocks[1]), dim3(kBlock), 0, s, tiles, cell_start, rec, partial);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_p2g_combine, dim3(blocks[2]), dim3(kBlock), 0, s, faces, ux, uy, Wx, Wy, partial);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

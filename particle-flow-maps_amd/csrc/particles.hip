@@ -3,7 +3,7 @@
 // composition, impulse/affine updates, identity resets and lattice seeding.
 // One thread per particle over SoA arrays (coalesced); the 4^d B-spline
 // stencil is evaluated separably from 16-byte row loads (see pfm_device.h).
-#include "pfm_host.h"
+#include "pfm_batch.h"
 
 namespace pfm {
 
@@ -36,18 +36,13 @@ __device__ __forceinline__ float t_product(const float A[3][3], const float B[3]
 // T_init <- cols(T_grad_m) cols(T_init), the update_T that the next step's
 // short reinitialisation would apply (3D/run_3D.py:767-771), leaving
 // T_grad_m in memory as the identity that reset would write.
-template <int DIM, bool REC = false, bool FOLD = false>
-__global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t n,
-                                                          float* __restrict__ pos,
-                                                          float* __restrict__ Tx,
-                                                          float* __restrict__ Ty,
-                                                          float* __restrict__ Tz,
-                                                          const float* __restrict__ ux,
-                                                          const float* __restrict__ uy,
-                                                          const float* __restrict__ uz, float dt,
-                                                          int advect_psi, RecArgs ra = RecArgs{}) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// One particle: i indexes the SoA arrays, whose component stride is n.
+template <int DIM, bool REC, bool FOLD>
+__device__ __forceinline__ void rk4_particle(const GridDev& g, int64_t i, int64_t n, float* __restrict__ pos,
+                                             float* __restrict__ Tx, float* __restrict__ Ty,
+                                             float* __restrict__ Tz, const float* __restrict__ ux,
+                                             const float* __restrict__ uy, const float* __restrict__ uz,
+                                             float dt, int advect_psi, const RecArgs& ra) {
     float* Ts[3] = {Tx, Ty, Tz};
     float p0[3] = {0.f, 0.f, 0.f};
     float T0[3][3];
@@ -154,18 +149,49 @@ __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t 
     }
 }
 
+template <int DIM, bool REC = false, bool FOLD = false>
+__global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t n,
+                                                          float* __restrict__ pos,
+                                                          float* __restrict__ Tx,
+                                                          float* __restrict__ Ty,
+                                                          float* __restrict__ Tz,
+                                                          const float* __restrict__ ux,
+                                                          const float* __restrict__ uy,
+                                                          const float* __restrict__ uz, float dt,
+                                                          int advect_psi, RecArgs ra = RecArgs{}) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rk4_particle<DIM, REC, FOLD>(g, i, n, pos, Tx, Ty, Tz, ux, uy, uz, dt, advect_psi, ra);
+}
+
+// Batched 2D RK4_T_forward: one persistent launch over the particle units of
+// every scene, each scene's particles against its own faces and dt.
+__global__ void __launch_bounds__(kBlock, 3) kb_rk4(BUnits t, int64_t P, int64_t F0, int64_t F1,
+                                                 float* __restrict__ pos, float* __restrict__ Tx,
+                                                 float* __restrict__ Ty, const float* __restrict__ ux,
+                                                 const float* __restrict__ uy, const BState* __restrict__ st) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (loc >= S.n) continue;
+        const float dt = (float)st[w.scene].dt;
+        rk4_particle<2, false, false>(S.g, S.p_off + loc, P, pos, Tx, Ty, nullptr, ux + S.f_off[0],
+                                      uy + S.f_off[1], nullptr, dt, 1, RecArgs{});
+    }
+    (void)F0;
+    (void)F1;
+}
+
 // init_particles_imp / init_particles_imp_grad_m: 3D/init_conditions.py:254-273,
 // 2D/init_conditions.py:183-210 (interp_u_MAC_grad_imp, 3D/taichi_utils.py:326-376:
 // new_C rows equal the velocity gradient rows).
 template <int DIM, bool GRAD>
-__global__ void __launch_bounds__(kBlock, 2) k_g2p(GridDev g, int64_t n, const float* __restrict__ pos,
-                                                const float* __restrict__ ux,
-                                                const float* __restrict__ uy,
-                                                const float* __restrict__ uz,
-                                                float* __restrict__ imp, float* __restrict__ Cx,
-                                                float* __restrict__ Cy, float* __restrict__ Cz) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void g2p_particle(const GridDev& g, int64_t i, int64_t n, const float* __restrict__ pos,
+                                             const float* __restrict__ ux, const float* __restrict__ uy,
+                                             const float* __restrict__ uz, float* __restrict__ imp,
+                                             float* __restrict__ Cx, float* __restrict__ Cy,
+                                             float* __restrict__ Cz) {
     float p[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int c = 0; c < DIM; ++c) p[c] = pos[c * n + i];
@@ -181,6 +207,33 @@ __global__ void __launch_bounds__(kBlock, 2) k_g2p(GridDev g, int64_t n, const f
         for (int a = 0; a < DIM; ++a)
 #pragma unroll
             for (int c = 0; c < DIM; ++c) Cs[a][c * n + i] = G[a][c];
+    }
+}
+
+template <int DIM, bool GRAD>
+__global__ void __launch_bounds__(kBlock, 2) k_g2p(GridDev g, int64_t n, const float* __restrict__ pos,
+                                                const float* __restrict__ ux,
+                                                const float* __restrict__ uy,
+                                                const float* __restrict__ uz,
+                                                float* __restrict__ imp, float* __restrict__ Cx,
+                                                float* __restrict__ Cy, float* __restrict__ Cz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    g2p_particle<DIM, GRAD>(g, i, n, pos, ux, uy, uz, imp, Cx, Cy, Cz);
+}
+
+template <bool GRAD>
+__global__ void __launch_bounds__(kBlock, 2) kb_g2p(BUnits t, int64_t P, const float* __restrict__ pos,
+                                                 const float* __restrict__ ux, const float* __restrict__ uy,
+                                                 float* __restrict__ imp, float* __restrict__ Cx,
+                                                 float* __restrict__ Cy) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (loc >= S.n) continue;
+        g2p_particle<2, GRAD>(S.g, S.p_off + loc, P, pos, ux + S.f_off[0], uy + S.f_off[1], nullptr, imp, Cx, Cy,
+                              nullptr);
     }
 }
 
@@ -315,12 +368,11 @@ __device__ __forceinline__ float jitter_u(uint32_t seed, uint64_t id, uint32_t c
 // init_particles_pos_uniform: 3D/init_conditions.py:238-252, 2D/init_conditions.py:139-166.
 // order 0: the reference numbering (lattice x fastest).  order 1: cell-major
 // (cells with the last axis fastest, the ppc_axis^d sub-lattice inside).
+// Particle i of a scene (lattice id and jitter hash use i); its position
+// goes to pos[c * stride].
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_seed(GridDev g, int ppc_axis, float dist, int64_t n,
-                                                 float* __restrict__ pos, int64_t seed,
-                                                 int order) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void seed_particle(const GridDev& g, int ppc_axis, float dist, int64_t i,
+                                              float* __restrict__ pos, int64_t stride, int64_t seed, int order) {
     int64_t id[3] = {0, 0, 0};
     if (order == 0) {
         const int64_t nx = (int64_t)ppc_axis * g.res[0];
@@ -355,8 +407,53 @@ __global__ void __launch_bounds__(kBlock) k_seed(GridDev g, int ppc_axis, float 
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
         float off = seed >= 0 ? jitter_u((uint32_t)seed, (uint64_t)i, (uint32_t)c) : 0.5f;
-        pos[c * n + i] = __fmul_rn(__fadd_rn((float)id[c], off), dist);
+        pos[c * stride] = __fmul_rn(__fadd_rn((float)id[c], off), dist);
     }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_seed(GridDev g, int ppc_axis, float dist, int64_t n,
+                                                 float* __restrict__ pos, int64_t seed,
+                                                 int order) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    seed_particle<DIM>(g, ppc_axis, dist, i, pos + i, n, seed, order);
+}
+
+__global__ void __launch_bounds__(kBlock) kb_seed(BUnits t, int ppc_axis, int order, int64_t P,
+                                                  float* __restrict__ pos) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (loc >= S.n) continue;
+        seed_particle<2>(S.g, ppc_axis, S.dist, loc, pos + S.p_off + loc, P, S.seed, order);
+    }
+}
+
+// ---- launchers of the batched kernels ----------------------------------------
+int b_seed(BUnits t, int ppc_axis, int order, int64_t P, float* pos, unsigned blocks, hipStream_t s) {
+    hipLaunchKernelGGL(kb_seed, dim3(blocks), dim3(kBlock), 0, s, t, ppc_axis, order, P, pos);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int b_g2p(BUnits t, int64_t P, const int64_t F[2], const float* pos, const float* ux, const float* uy, float* imp,
+          float* Cx, float* Cy, unsigned blocks, hipStream_t s) {
+    (void)F;
+    if (Cx)
+        hipLaunchKernelGGL(kb_g2p<true>, dim3(blocks), dim3(kBlock), 0, s, t, P, pos, ux, uy, imp, Cx, Cy);
+    else
+        hipLaunchKernelGGL(kb_g2p<false>, dim3(blocks), dim3(kBlock), 0, s, t, P, pos, ux, uy, imp, Cx, Cy);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int b_rk4(BUnits t, int64_t P, const int64_t F[2], float* pos, float* Tx, float* Ty, const float* ux,
+          const float* uy, const BState* st, unsigned blocks, hipStream_t s) {
+    hipLaunchKernelGGL(kb_rk4, dim3(blocks), dim3(kBlock), 0, s, t, P, F[0], F[1], pos, Tx, Ty, ux, uy, st);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
 }
 
 }  // namespace pfm

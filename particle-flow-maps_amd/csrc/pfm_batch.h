@@ -1,0 +1,140 @@
+// pfm_batch.h -- internal interface of the batched 2D path: many independent
+// 2D scenes of different sizes advanced together, every phase of the step as
+// one persistent launch over a table of work units.
+//
+// All scenes share concatenated arrays: scene s owns particles
+// [p_off, p_off + n) of every particle array (SoA, component stride = the
+// batch's particle total), faces [f_off[a], f_off[a] + nf[a]) of every face
+// array of axis a (component stride = that axis' face total), cells
+// [c_off, c_off + cells) of the level-0 arrays and [l_off[l], ...) of the
+// multigrid level arrays.  A work unit is (scene, chunk): a chunk of
+// kBlock items (particles, faces, cells, red/black pairs) of one scene, or
+// one P2G tile.  A persistent kernel walks the units with a grid stride, so
+// one launch covers every scene whatever its size.
+//
+// The per-item arithmetic of every batched kernel is the one of the
+// single-scene kernel of the same phase (the bodies are shared), so a scene
+// advanced in a batch reproduces its single-scene step except for the order
+// of the PCG dot-product sums.
+#pragma once
+
+#include "pfm_host.h"
+
+namespace pfm {
+
+constexpr int kBatchMaxLevels = 12;
+// Levels with at most this many cells run inside one workgroup per scene.
+constexpr int kBatchCoarseCells = 4096;
+
+struct BScene {
+    GridDev g;                 // the scene's 2D grid
+    float dist;                // particle lattice spacing dx / ppc_axis
+    int64_t seed;              // jitter seed (< 0: the reference lattice)
+    int64_t p_off, n;          // particles
+    int64_t f_off[2], nf[2];   // faces per axis
+    int64_t c_off, nc;         // level-0 cells
+    int32_t nt[2];             // P2G tiles per axis
+    int64_t t_off;             // first P2G tile partial of the scene
+    // multigrid: levels [0, coarse) run as tiled launches, [coarse,
+    // n_levels) in the per-scene coarse kernel
+    int32_t n_levels, coarse;
+    int32_t ld[kBatchMaxLevels][2];
+    int64_t l_off[kBatchMaxLevels];  // offset of level l in the level arrays
+    int32_t part_off, n_part;        // level-0 reduction partials of the scene
+};
+
+struct BUnit {
+    int32_t scene, chunk;
+};
+
+// Per-scene dynamic state of the step.
+struct BState {
+    double dt;        // this step's dt (host double arithmetic of the reference, on device)
+    double sub_t;     // time since the last frame
+    int32_t frame;    // frame index
+    uint32_t ms_bits; // max speed (float bits, exact atomic max)
+};
+
+// Per-scene scalars of the PCG (the MGScalars of the single-scene solver).
+struct BMg {
+    double zTr, pAp, rTr, r0, sum_r, thresh;
+    float alpha, beta, mean;
+    int32_t active, it, converged, iters;
+};
+
+// Kernel-side view of one unit table.
+struct BUnits {
+    const BScene* sc;
+    const BUnit* u;
+    int32_t nu;
+};
+
+// Persistent loop over the units of a launch: `UNIT` is the unit index.
+#define PFM_FOR_UNITS(T, UNIT) for (int32_t UNIT = blockIdx.x; UNIT < (T).nu; UNIT += gridDim.x)
+
+// ---- launchers (implemented beside the single-scene kernels they share) ----
+// Every launcher takes the grid size of the persistent launch (`blocks`).
+
+// particles.hip
+int b_seed(BUnits t, int ppc_axis, int order, int64_t P, float* pos, unsigned blocks, hipStream_t s);
+int b_g2p(BUnits t, int64_t P, const int64_t F[2], const float* pos, const float* ux, const float* uy,
+          float* imp, float* Cx, float* Cy, unsigned blocks, hipStream_t s);
+int b_rk4(BUnits t, int64_t P, const int64_t F[2], float* pos, float* Tx, float* Ty, const float* ux,
+          const float* uy, const BState* st, unsigned blocks, hipStream_t s);
+
+// grid.hip
+int b_max_speed(BUnits cells, const float* ux, const float* uy, BState* st, unsigned blocks, hipStream_t s);
+int b_dt(int n_scenes, const BScene* sc, BState* st, double CFL, double visualize_dt, hipStream_t s);
+int b_ms_init(int n_scenes, BState* st, hipStream_t s);
+int b_reset_identity_grid(BUnits faces, const int64_t F[2], float* psi_x, float* psi_y, float* T_x, float* T_y,
+                          unsigned blocks, hipStream_t s);
+int b_rk4_grid(BUnits faces, const int64_t F[2], float* psi_x, float* psi_y, float* T_x, float* T_y,
+               const float* ux, const float* uy, const BState* st, unsigned blocks, hipStream_t s);
+int b_advect_u(BUnits faces, const int64_t F[2], const float* u0x, const float* u0y, float* u1x, float* u1y,
+               const float* T_x, const float* T_y, const float* psi_x, const float* psi_y, unsigned blocks,
+               hipStream_t s);
+
+// p2g.hip
+int b_bin(BUnits parts, int64_t P, int64_t C, const float* pos, int32_t* cell_of, int32_t* cell_start,
+          int32_t* perm, int32_t* counts, int32_t* tiles, int32_t* slot, unsigned blocks, hipStream_t s);
+int b_p2g(BUnits parts, BUnits tiles, BUnits faces, int64_t P, const int64_t F[2], const float* pos,
+          const float* imp, const float* Cx, const float* Cy, const int32_t* cell_start, const int32_t* perm,
+          float* rec, float* partial, float* ux, float* uy, float* Wx, float* Wy, const unsigned blocks[3],
+          hipStream_t s);
+size_t b_p2g_partial_floats(int64_t tiles);
+
+// batch_mg.hip: the batched projection (apply_bc, divergence, MGPCG,
+// subtract_grad_p) of 2D scenes with Neumann walls.
+struct BWalls {
+    int lo0, hi0, lo1, hi1;  // walls that remove a neighbour from the diagonal
+};
+enum BRedMode { kBRedSum = 0, kBRedRR = 1, kBRedZR = 2 };
+enum BFinMode { kBFinMean = 0, kBFinR0 = 1, kBFinZTr = 2, kBFinAlpha = 3, kBFinRTr = 4, kBFinBeta = 5 };
+
+int bm_zero_walls(BUnits faces, float* ux, float* uy, unsigned blocks, hipStream_t s);
+int bm_div_init(BUnits cells, const float* ux, const float* uy, float* r, float* x, float* p, float* Ap,
+                unsigned blocks, hipStream_t s);
+int bm_subtract_grad_p(BUnits faces, const float* x, float* ux, float* uy, unsigned blocks, hipStream_t s);
+int bm_reduce(BUnits cells, int mode, const float* r, const float* z, const BMg* mg, double* part, unsigned blocks,
+              hipStream_t s);
+int bm_update_p(BUnits cells, const float* z, float* p, const BMg* mg, unsigned blocks, hipStream_t s);
+int bm_ap_dot(BUnits cells, BWalls wl, const float* p, float* Ap, const BMg* mg, double* part, unsigned blocks,
+              hipStream_t s);
+int bm_update_xr(BUnits cells, float* x, float* r, const float* p, const float* Ap, const BMg* mg, double* part,
+                 double* part2, unsigned blocks, hipStream_t s);
+int bm_sub_mean(BUnits cells, float* r, const BMg* mg, unsigned blocks, hipStream_t s);
+int bm_finalize(int n_scenes, const BScene* sc, int mode, double eps, int max_iters, const double* part,
+                const double* part2, BMg* mg, hipStream_t s);
+int bm_mg_reset(int n_scenes, BMg* mg, int tol_mode, double tol, int max_iters, hipStream_t s);
+int bm_set_thresh(int n_scenes, BMg* mg, double tol, hipStream_t s);
+int bm_save_solve(int n_scenes, const BMg* mg, int k, int32_t* iters, int32_t* conv, hipStream_t s);
+int bm_count_active(int n_scenes, const BMg* mg, int32_t* out, hipStream_t s);
+int bm_smooth(BUnits pairs, int l, int phase, int zero, BWalls wl, float* z, const float* r, const BMg* mg,
+              unsigned blocks, hipStream_t s);
+int bm_restrict(BUnits coarse_cells, int l, BWalls wl, const float* z, float* r, const BMg* mg, unsigned blocks,
+                hipStream_t s);
+int bm_prolong(BUnits cells, int l, float* z, const BMg* mg, unsigned blocks, hipStream_t s);
+int bm_coarse(int n_scenes, const BScene* sc, BWalls wl, float* z, float* r, const BMg* mg, int threads,
+              size_t lds_bytes, hipStream_t s);
+
+}  // namespace pfm

@@ -58,6 +58,31 @@ class PfmSceneState(ctypes.Structure):
     ]
 
 
+class PfmBatch2dDesc(ctypes.Structure):
+    """pfm_batch2d_desc of include/pfm.h."""
+
+    _fields_ = [
+        ("n_scenes", ctypes.c_int32), ("res", c_vp), ("seed", c_vp),
+        ("ppc_axis", ctypes.c_int32), ("particle_order", ctypes.c_int32),
+        ("reinit_every", ctypes.c_int32), ("reinit_every_grad_m", ctypes.c_int32),
+        ("use_midpoint_vel", ctypes.c_int32), ("reinit_particle_pos", ctypes.c_int32),
+        ("CFL", ctypes.c_double), ("visualize_dt", ctypes.c_double),
+        ("base_level", ctypes.c_int32), ("max_iters", ctypes.c_int32), ("tol_mode", ctypes.c_int32),
+        ("tol", ctypes.c_double),
+        ("u", c_vp * 2), ("p2g_weight", c_vp * 2), ("psi_grid", c_vp * 2), ("T_grid", c_vp * 2),
+        ("pos", c_vp), ("T", c_vp * 2), ("T_init", c_vp * 2), ("T_grad_m", c_vp * 2), ("C", c_vp * 2),
+        ("init_C", c_vp * 2), ("imp", c_vp), ("init_imp", c_vp), ("imp_grad_m", c_vp),
+        ("cell_of", c_vp), ("cell_start", c_vp), ("perm", c_vp),
+    ]
+
+
+class PfmBatch2dState(ctypes.Structure):
+    """pfm_batch2d_state of include/pfm.h."""
+
+    _fields_ = [("i", ctypes.c_int64), ("ik", ctypes.c_int64), ("n_solves", ctypes.c_int32),
+                ("lockstep_iters", ctypes.c_int32 * 2)]
+
+
 class PfmError(RuntimeError):
     pass
 
@@ -117,6 +142,16 @@ _SIGS = {
     "pfm_scene_create": [ctypes.POINTER(PfmSceneDesc), ctypes.POINTER(c_vp)],
     "pfm_scene_step": [c_vp, ctypes.POINTER(PfmSceneState), c_d, c_vp],
     "pfm_scene_destroy": [c_vp],
+    "pfm_batch2d_sizes": [ctypes.c_int32, c_vp, ctypes.c_int32, c_vp],
+    "pfm_batch2d_create": [ctypes.POINTER(PfmBatch2dDesc), ctypes.POINTER(c_vp), ctypes.POINTER(c_i64)],
+    "pfm_batch2d_project": [c_vp, c_vp],
+    "pfm_batch2d_init_particles": [c_vp, c_vp],
+    "pfm_batch2d_step": [c_vp, ctypes.POINTER(PfmBatch2dState), c_vp],
+    "pfm_batch2d_scene_info": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "pfm_batch2d_tune": [c_vp, ctypes.POINTER(PfmBatch2dState), c_vp],
+    "pfm_batch2d_tuning": [c_vp, c_vp, ctypes.c_int32],
+    "pfm_batch2d_set_graphs": [c_vp, ctypes.c_int32],
+    "pfm_batch2d_destroy": [c_vp],
 }
 _RESTYPES = {
     "pfm_last_error": ctypes.c_char_p,

@@ -95,7 +95,8 @@ def test_scene_structs_match_header(tmp_path):
     gcc = shutil.which("gcc")
     if gcc is None:
         pytest.skip("gcc not available")
-    structs = {"pfm_scene_desc": _lib.PfmSceneDesc, "pfm_scene_state": _lib.PfmSceneState}
+    structs = {"pfm_scene_desc": _lib.PfmSceneDesc, "pfm_scene_state": _lib.PfmSceneState,
+               "pfm_batch2d_desc": _lib.PfmBatch2dDesc, "pfm_batch2d_state": _lib.PfmBatch2dState}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
@@ -112,3 +113,21 @@ def test_scene_structs_match_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py)
         for f, _t in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_batch_sizes_query():
+    """pfm_batch2d_sizes: totals of the concatenated batch layout, host only."""
+    import numpy as np
+
+    from pfm import _lib
+
+    lib = _lib.load(require_gpu=False)
+    res = np.ascontiguousarray(np.array([64, 64, 97, 80, 4, 5], np.int32))
+    out = (ctypes.c_int64 * 4)()
+    assert lib.pfm_batch2d_sizes(3, res.ctypes.data, 2, out) == 0
+    pairs = [(64, 64), (97, 80), (4, 5)]
+    assert list(out) == [sum(a * b * 4 for a, b in pairs), sum((a + 1) * b for a, b in pairs),
+                         sum(a * (b + 1) for a, b in pairs), sum(a * b for a, b in pairs)]
+    bad = np.ascontiguousarray(np.array([64, 3], np.int32))
+    assert lib.pfm_batch2d_sizes(1, bad.ctypes.data, 2, out) == -1
+    assert "out of" in _lib.last_error()
