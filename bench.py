@@ -395,6 +395,93 @@ def serving_bench(n_jobs, rates, slots, torch):
     }
 
 
+def batch_bench(n_scenes, steps, warmup, sample, torch):
+    """Batched 2D step: n_scenes independent scenes (square grids log-uniform
+    in 64^2-512^2, 8-32 Gaussian point vortices each, seed 2, 4 particles per
+    cell, reinit every 12 steps) advanced together, one persistent launch per
+    phase replayed from HIP graphs, launch shapes autotuned; against the
+    same scenes run one at a time (single-scene Simulation), timed on a
+    size-stratified sample and scaled to all scenes, each sampled scene also
+    checked against its batched result."""
+    import numpy as np
+
+    from pfm.batch import Batch2D, random_vortex_scenes
+    from pfm.simulation import Simulation, StepConfig
+
+    cfg = StepConfig(particles_per_cell=4, reinit_every=12, reinit_every_grad_m=8, CFL=0.5, base_level=4,
+                     tol=1e-6, tol_mode="rel", write_imp_grad_m=False, visualize_dt=1e9)
+    scenes = random_vortex_scenes(n_scenes, seed=2)
+    t0 = time.perf_counter()
+    b = Batch2D([sc[0] for sc in scenes], cfg, seeds=list(range(n_scenes)))
+    for k, (res, c, w, rad) in enumerate(scenes):
+        b.set_point_vortices(k, c, w, rad)
+    b.project()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    shapes = b.tune()
+    order = np.argsort([sc[0][0] for sc in scenes], kind="stable")
+    picked = [int(order[i]) for i in np.linspace(0, n_scenes - 1, sample).round().astype(int)]
+    ic = {k: [b.u_view(k, a).clone() for a in range(2)] for k in picked}
+    b.init_particles()
+    for _ in range(warmup):
+        b.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lock = []
+    for _ in range(steps):
+        lock.append(b.step())
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    info = b.scene_info()
+    batch_ms = el / steps * 1e3
+    n_part = sum(b.n)
+    print(f"# batch {n_scenes} scenes: {batch_ms:.1f} ms/step ({n_part} particles), lockstep iters {lock}",
+          file=sys.stderr, flush=True)
+    # one at a time: the sampled scenes from the same initial velocity and seed
+    single_ms, rel = [], []
+    for k in picked:
+        sim = Simulation(scenes[k][0], StepConfig(**{**cfg.__dict__, "seed": k}))
+        for a in range(2):
+            sim.u[a].copy_(ic[k][a])
+        sim.initialize_2d_particles()
+        for _ in range(warmup):
+            sim.step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            sim.step()
+        torch.cuda.synchronize()
+        single_ms.append((time.perf_counter() - t1) / steps * 1e3)
+        num = sum(float(torch.linalg.vector_norm(b.u_view(k, a) - sim.u[a]) ** 2) for a in range(2))
+        den = sum(float(torch.linalg.vector_norm(sim.u[a]) ** 2) for a in range(2))
+        rel.append((num / max(den, 1e-30)) ** 0.5)
+        del sim
+    one_ms = float(np.sum(single_ms)) * n_scenes / len(picked)
+    sizes = [sc[0][0] for sc in scenes]
+    return {
+        "workload": f"{n_scenes} independent 2D scenes, square grids log-uniform 64^2-512^2 (seed 2), 8-32 "
+                    "Gaussian point vortices each at uniform positions in [0.2,0.8]^2, circulations N(0,1), "
+                    "core radius 1.5 dx, 4 particles/cell, reinit every 12 (grad_m 8), CFL 0.5, MGPCG 1e-6 "
+                    "relative, fp32",
+        "cells": int(sum(s * s for s in sizes)), "particles": int(n_part),
+        "size_min": int(min(sizes)), "size_max": int(max(sizes)),
+        "setup_s": t_setup, "steps": steps, "warmup": warmup,
+        "ms_per_step": batch_ms,
+        "particle_steps_per_s": n_part * steps / el,
+        "scene_steps_per_s": n_scenes * steps / el,
+        "lockstep_pcg_iters": lock,
+        "scene_pcg_iters_last_step": {"min": int(info["iters"].min()), "max": int(info["iters"].max()),
+                                      "mean": float(info["iters"].mean())},
+        "launch_shapes": shapes,
+        "one_at_a_time": {"sample_scenes": len(picked), "sample": "size-stratified (every "
+                          f"{n_scenes // len(picked)}th scene in size order), Simulation per scene, same IC "
+                          "and seed", "ms_per_scene_step_sum_sample": float(np.sum(single_ms)),
+                          "est_ms_per_step_all_scenes": one_ms},
+        "speedup_vs_one_at_a_time": one_ms / batch_ms,
+        "check_vs_single_u_rel_l2_max": float(max(rel)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,6 +494,11 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-only", action="store_true", help="run only the served job stream and print it")
     ap.add_argument("--profile-steps", type=int, default=2, help="instrumented steps after the timed ones")
+    ap.add_argument("--batch-scenes", type=int, default=512, help="scenes of the batched 2D step")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-sample", type=int, default=32, help="scenes timed one at a time")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D step")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched 2D step and print it")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: one plain 1 GiB copy (the counter scale reference) "
                          "then one launch of each step kernel after two steps; prints no JSON")
@@ -437,6 +529,9 @@ def main():
 
     if args.serve_only:
         print(json.dumps(serving_bench(args.serve_jobs, (200.0,), args.serve_slots, torch)))
+        return
+    if args.batch_only:
+        print(json.dumps(batch_bench(args.batch_scenes, args.batch_steps, 1, args.batch_sample, torch)))
         return
 
     res = (args.res,) * 3
@@ -599,6 +694,9 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_serve:
         out["serving"] = serving_bench(args.serve_jobs, (10.0, 50.0, 200.0), args.serve_slots, torch)
+    if not args.no_batch:
+        out["batch2d"] = batch_bench(args.batch_scenes, args.batch_steps, 1, args.batch_sample, torch)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline()
