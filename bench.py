@@ -409,7 +409,7 @@ def batch_bench(n_scenes, steps, warmup, sample, torch):
     from pfm.simulation import Simulation, StepConfig
 
     cfg = StepConfig(particles_per_cell=4, reinit_every=12, reinit_every_grad_m=8, CFL=0.5, base_level=4,
-                     tol=1e-6, tol_mode="rel", write_imp_grad_m=False, visualize_dt=1e9)
+                     tol=1e-6, tol_mode="rel", write_imp_grad_m=False, visualize_dt=1e9, particle_order=1)
     scenes = random_vortex_scenes(n_scenes, seed=2)
     t0 = time.perf_counter()
     b = Batch2D([sc[0] for sc in scenes], cfg, seeds=list(range(n_scenes)))
@@ -461,8 +461,8 @@ def batch_bench(n_scenes, steps, warmup, sample, torch):
     return {
         "workload": f"{n_scenes} independent 2D scenes, square grids log-uniform 64^2-512^2 (seed 2), 8-32 "
                     "Gaussian point vortices each at uniform positions in [0.2,0.8]^2, circulations N(0,1), "
-                    "core radius 1.5 dx, 4 particles/cell, reinit every 12 (grad_m 8), CFL 0.5, MGPCG 1e-6 "
-                    "relative, fp32",
+                    "core radius 1.5 dx, 4 particles/cell (cell-major numbering), reinit every 12 (grad_m 8), "
+                    "CFL 0.5, MGPCG 1e-6 relative, fp32",
         "cells": int(sum(s * s for s in sizes)), "particles": int(n_part),
         "size_min": int(min(sizes)), "size_max": int(max(sizes)),
         "setup_s": t_setup, "steps": steps, "warmup": warmup,
