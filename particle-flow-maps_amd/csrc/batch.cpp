@@ -20,8 +20,7 @@ namespace pfm {
 namespace {
 
 enum Kind {
-    kKRk4 = 0, kKG2p, kKSeed, kKFaceRk4, kKAdvect, kKFace, kKPrep, kKTile, kKBin, kKCell, kKSmooth, kKRestrict,
-    kKProlong, kKCount
+    kKRk4 = 0, kKG2p, kKSeed, kKFaceRk4, kKAdvect, kKFace, kKPrep, kKTile, kKBin, kKCell, kKDown, kKUp, kKCount
 };
 constexpr int kCapCand[] = {0, 4096, 2048, 1024, 512};
 constexpr int kNCap = sizeof(kCapCand) / sizeof(kCapCand[0]);
@@ -42,7 +41,7 @@ struct Batch {
     BScene* sc = nullptr;
     BUnit* units = nullptr;
     List parts, faces, tiles, cells0;
-    std::vector<List> pairs, pcells, rcells;  // per tiled level
+    std::vector<List> mgtiles;                // per tiled level: 32 x 32 tiles
     int Lt = 0;                               // tiled levels (max over scenes)
     BWalls wl{1, 1, 1, 1};
     BState* st = nullptr;
@@ -113,33 +112,24 @@ int copy_f(Batch* b, float* dst, const float* src, int64_t n) {
 
 // ---- multigrid ----------------------------------------------------------------
 
+// apply_preconditioner (3D/mgpcg.py:182-200): per tiled level one launch for
+// the pre-smoother + restriction and one for the prolongation +
+// post-smoother; the coarse levels in one workgroup per scene.
 int vcycle(Batch* b) {
     hipStream_t s = b->bs;
     for (int l = 0; l < b->Lt; ++l) {
-        const BUnits pu = view(b, b->pairs[l]);
-        const unsigned pb = blocks_for(b, kKSmooth, b->pairs[l]);
-        for (int k = 0; k < 2; ++k) {
-            PFM_B_TRY(timed_launch(b, kKSmooth, [&] { return bm_smooth(pu, l, 0, k == 0, b->wl, b->z, b->r, b->mg, pb, s); }));
-            PFM_B_TRY(timed_launch(b, kKSmooth, [&] { return bm_smooth(pu, l, 1, 0, b->wl, b->z, b->r, b->mg, pb, s); }));
-        }
-        PFM_B_TRY(timed_launch(b, kKRestrict, [&] {
-            return bm_restrict(view(b, b->rcells[l]), l, b->wl, b->z, b->r, b->mg,
-                               blocks_for(b, kKRestrict, b->rcells[l]), s);
+        PFM_B_TRY(timed_launch(b, kKDown, [&] {
+            return bm_down(view(b, b->mgtiles[l]), l, b->wl, b->z, b->r, b->mg, blocks_for(b, kKDown, b->mgtiles[l]),
+                           s);
         }));
     }
     PFM_B_TRY(timed_launch(b, kKCount, [&] {
         return bm_coarse(b->S, b->sc, b->wl, b->z, b->r, b->mg, b->coarse_threads, b->coarse_lds, s);
     }));
     for (int l = b->Lt - 1; l >= 0; --l) {
-        PFM_B_TRY(timed_launch(b, kKProlong, [&] {
-            return bm_prolong(view(b, b->pcells[l]), l, b->z, b->mg, blocks_for(b, kKProlong, b->pcells[l]), s);
+        PFM_B_TRY(timed_launch(b, kKUp, [&] {
+            return bm_up(view(b, b->mgtiles[l]), l, b->wl, b->z, b->r, b->mg, blocks_for(b, kKUp, b->mgtiles[l]), s);
         }));
-        const BUnits pu = view(b, b->pairs[l]);
-        const unsigned pb = blocks_for(b, kKSmooth, b->pairs[l]);
-        for (int k = 0; k < 2; ++k) {
-            PFM_B_TRY(timed_launch(b, kKSmooth, [&] { return bm_smooth(pu, l, 1, 0, b->wl, b->z, b->r, b->mg, pb, s); }));
-            PFM_B_TRY(timed_launch(b, kKSmooth, [&] { return bm_smooth(pu, l, 0, 0, b->wl, b->z, b->r, b->mg, pb, s); }));
-        }
     }
     return PFM_OK;
 }
@@ -577,24 +567,15 @@ int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** handle, int64_t* byt
     for (int s = 0; s < b->S; ++s) add(b->tiles, s, (int64_t)b->hs[s].nt[0] * b->hs[s].nt[1]);
     b->cells0.off = (int32_t)U.size();
     for (int s = 0; s < b->S; ++s) add(b->cells0, s, b->hs[s].n_part);
-    b->pairs.resize(b->Lt);
-    b->pcells.resize(b->Lt);
-    b->rcells.resize(b->Lt);
+    b->mgtiles.resize(b->Lt);
     for (int l = 0; l < b->Lt; ++l) {
-        b->pairs[l].off = (int32_t)U.size();
+        b->mgtiles[l].off = (int32_t)U.size();
         for (int s = 0; s < b->S; ++s)
             if (l < b->hs[s].coarse) {
-                const int64_t hp = (b->hs[s].ld[l][1] + 1) / 2;
-                add(b->pairs[l], s, ((int64_t)b->hs[s].ld[l][0] * hp + kBlock - 1) / kBlock);
+                const int64_t t0 = (b->hs[s].ld[l][0] + kBatchTile - 1) / kBatchTile;
+                const int64_t t1 = (b->hs[s].ld[l][1] + kBatchTile - 1) / kBatchTile;
+                add(b->mgtiles[l], s, t0 * t1);
             }
-        b->pcells[l].off = (int32_t)U.size();
-        for (int s = 0; s < b->S; ++s)
-            if (l < b->hs[s].coarse)
-                add(b->pcells[l], s, ((int64_t)b->hs[s].ld[l][0] * b->hs[s].ld[l][1] + kBlock - 1) / kBlock);
-        b->rcells[l].off = (int32_t)U.size();
-        for (int s = 0; s < b->S; ++s)
-            if (l < b->hs[s].coarse)
-                add(b->rcells[l], s, ((int64_t)b->hs[s].ld[l + 1][0] * b->hs[s].ld[l + 1][1] + kBlock - 1) / kBlock);
     }
     // device memory
     int64_t bytes = 0;

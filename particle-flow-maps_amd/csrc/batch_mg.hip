@@ -327,31 +327,7 @@ __global__ void __launch_bounds__(kBlock) kb_count_active(int n, const BMg* __re
     }
 }
 
-// ---- V-cycle kernels of the tiled levels ---------------------------------------
-
-// smooth (3D/mgpcg.py:165-172), one colour: a unit is kBlock red/black pairs
-// of one level (pairs along axis 1; the thread takes the member of colour
-// `phase`).  `zero`: z is known to be zero (the first sweep after the
-// level was cleared), so the neighbour sum is 0.
-__global__ void __launch_bounds__(kBlock) kb_smooth(BUnits t, int l, int phase, int zero, BWalls wl,
-                                                    float* __restrict__ z, const float* __restrict__ r,
-                                                    const BMg* __restrict__ mg) {
-    PFM_FOR_UNITS(t, u) {
-        const BUnit w = t.u[u];
-        if (!scene_active(mg, w.scene)) continue;
-        const BScene& S = t.sc[w.scene];
-        const int n0 = S.ld[l][0], n1 = S.ld[l][1];
-        const int hp = (n1 + 1) >> 1;
-        const int64_t pr = (int64_t)w.chunk * kBlock + threadIdx.x;
-        if (pr >= (int64_t)n0 * hp) continue;
-        const int i = (int)(pr / hp);
-        const int j = 2 * (int)(pr % hp) + ((i + phase) & 1);
-        if (j >= n1) continue;
-        const int64_t g = S.l_off[l] + (int64_t)i * n1 + j;
-        const float ns = zero ? 0.0f : bnsum(z + g, n0, n1, i, j);
-        z[g] = (r[g] + ns) / bnfn(wl, n0, n1, i, j);
-    }
-}
+// ---- shared restriction gather ----------------------------------------------
 
 // restrict (3D/mgpcg.py:152-158) as a gather of the children of coarse cell
 // (ci, cj); the last coarse cell of an odd extent also owns the trailing
@@ -371,36 +347,145 @@ __device__ __forceinline__ float restrict_cell(const BWalls& wl, const float* z,
     return acc;
 }
 
-__global__ void __launch_bounds__(kBlock) kb_restrict(BUnits t, int l, BWalls wl, const float* __restrict__ z,
-                                                      float* __restrict__ r, const BMg* __restrict__ mg) {
-    PFM_FOR_UNITS(t, u) {
-        const BUnit w = t.u[u];
-        if (!scene_active(mg, w.scene)) continue;
-        const BScene& S = t.sc[w.scene];
-        const int c0 = S.ld[l + 1][0], c1 = S.ld[l + 1][1];
-        const int64_t qc = (int64_t)w.chunk * kBlock + threadIdx.x;
-        if (qc >= (int64_t)c0 * c1) continue;
-        const int ci = (int)(qc / c1), cj = (int)(qc % c1);
-        const int64_t fo = S.l_off[l];
-        r[S.l_off[l + 1] + qc] = restrict_cell(wl, z + fo, r + fo, S.ld[l][0], S.ld[l][1], c0, c1, ci, cj);
-    }
+// ---- fused tiled V-cycle passes ---------------------------------------------
+// A unit is a kBT x kBT tile of one scene's level.  The tile is loaded with
+// a halo of H cells; each red/black colour pass then updates the region one
+// cell narrower than the last, so after k passes every cell within H - k of
+// the tile holds exactly the value of k sequential sweeps over the whole
+// level (a cell's update reads only its four neighbours, all of which were
+// exact after the previous pass).  Cells outside the level are never loaded
+// as neighbours (the wall tests of bnsum) and never updated.
+constexpr int kBT = 32;
+
+__device__ __forceinline__ float bnsum_s(const float* x, int stride, int n0, int n1, int i, int j) {
+    float ret = 0.0f;
+    if (i < n0 - 1) ret = ret + x[stride];
+    if (i > 0) ret = ret + x[-stride];
+    if (j < n1 - 1) ret = ret + x[1];
+    if (j > 0) ret = ret + x[-1];
+    return ret;
 }
 
-// prolongate (3D/mgpcg.py:160-163).
-__global__ void __launch_bounds__(kBlock) kb_prolong(BUnits t, int l, float* __restrict__ z,
-                                                     const BMg* __restrict__ mg) {
+// One colour over the cells within `ring` of the tile (LDS arrays of width W,
+// tile origin (i0, j0) at local (H, H)).
+template <int H>
+__device__ __forceinline__ void tile_sweep(const BWalls& wl, float* z, const float* r, int n0, int n1, int i0,
+                                           int j0, int ring, int phase) {
+    constexpr int W = kBT + 2 * H;
+    const int lo = H - ring, R = kBT + 2 * ring;
+    for (int t = threadIdx.x; t < R * R; t += blockDim.x) {
+        const int li = lo + t / R, lj = lo + t % R;
+        const int i = i0 - H + li, j = j0 - H + lj;
+        if (i < 0 || j < 0 || i >= n0 || j >= n1 || ((i + j) & 1) != phase) continue;
+        const int q = li * W + lj;
+        z[q] = (r[q] + bnsum_s(z + q, W, n0, n1, i, j)) / bnfn(wl, n0, n1, i, j);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void tile_of_unit(const BScene& S, int l, int32_t chunk, int& i0, int& j0) {
+    const int nt1 = (S.ld[l][1] + kBT - 1) / kBT;
+    i0 = (chunk / nt1) * kBT;
+    j0 = (chunk % nt1) * kBT;
+}
+
+// Pre-smoothing (z = 0, then colours 0, 1, 0, 1) and restriction of one tile
+// of level l (3D/mgpcg.py:182-192): writes z[l] on the tile and r[l + 1] on the
+// coarse cells whose first child lies in the tile.  The halo of 6 keeps z
+// exact two cells around the tile after the four passes, which the residual
+// of the trailing child of an odd extent (one cell past the tile) needs.
+__global__ void __launch_bounds__(kBlock) kb_down(BUnits t, int l, BWalls wl, float* __restrict__ zg,
+                                                  float* __restrict__ rg, const BMg* __restrict__ mg) {
+    constexpr int H = 6, W = kBT + 2 * H;
+    __shared__ float z[W * W], r[W * W];
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
         if (!scene_active(mg, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int n0 = S.ld[l][0], n1 = S.ld[l][1];
-        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
-        if (q >= (int64_t)n0 * n1) continue;
-        const int i = (int)(q / n1), j = (int)(q % n1);
+        int i0, j0;
+        tile_of_unit(S, l, w.chunk, i0, j0);
+        const float* rl = rg + S.l_off[l];
+        for (int q = threadIdx.x; q < W * W; q += blockDim.x) {
+            const int i = i0 - H + q / W, j = j0 - H + q % W;
+            const bool in = i >= 0 && j >= 0 && i < n0 && j < n1;
+            r[q] = in ? rl[(int64_t)i * n1 + j] : 0.0f;
+            z[q] = 0.0f;
+        }
+        __syncthreads();
+        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 1, 0);
+        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 2, 1);
+        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 3, 0);
+        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 4, 1);
+        float* zl = zg + S.l_off[l];
+        for (int q = threadIdx.x; q < kBT * kBT; q += blockDim.x) {
+            const int i = i0 + q / kBT, j = j0 + q % kBT;
+            if (i < n0 && j < n1) zl[(int64_t)i * n1 + j] = z[(q / kBT + H) * W + q % kBT + H];
+        }
+        // restriction (3D/mgpcg.py:152-158) of the coarse cells owned by the tile
         const int c0 = S.ld[l + 1][0], c1 = S.ld[l + 1][1];
-        const int pi = min(i >> 1, c0 - 1), pj = min(j >> 1, c1 - 1);
-        const int64_t g = S.l_off[l] + q;
-        z[g] = z[g] + z[S.l_off[l + 1] + (int64_t)pi * c1 + pj];
+        float* rc = rg + S.l_off[l + 1];
+        constexpr int CT = kBT / 2;
+        for (int q = threadIdx.x; q < CT * CT; q += blockDim.x) {
+            const int ci = i0 / 2 + q / CT, cj = j0 / 2 + q % CT;
+            if (ci >= c0 || cj >= c1) continue;
+            const int h0 = ci == c0 - 1 ? n0 - 1 - 2 * ci : 1;
+            const int h1 = cj == c1 - 1 ? n1 - 1 - 2 * cj : 1;
+            float acc = 0.0f;
+            for (int b = 0; b <= h0; ++b)
+                for (int c = 0; c <= h1; ++c) {
+                    const int i = 2 * ci + b, j = 2 * cj + c;
+                    const int lq = (i - i0 + H) * W + (j - j0 + H);
+                    const float res = r[lq] - (bnfn(wl, n0, n1, i, j) * z[lq] - bnsum_s(z + lq, W, n0, n1, i, j));
+                    acc = acc + (res * 1.0f) / 1.0f;
+                }
+            rc[(int64_t)ci * c1 + cj] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// Prolongation and post-smoothing (colours 1, 0, 1, 0) of one tile of level l
+// (3D/mgpcg.py:194-200): z[l] += z[l + 1][parent] on the tile and a halo of
+// 4, then the four passes; writes z[l] on the tile.
+__global__ void __launch_bounds__(kBlock) kb_up(BUnits t, int l, BWalls wl, float* __restrict__ zg,
+                                                const float* __restrict__ rg, const BMg* __restrict__ mg) {
+    constexpr int H = 4, W = kBT + 2 * H;
+    __shared__ float z[W * W], r[W * W];
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        if (!scene_active(mg, w.scene)) continue;
+        const BScene& S = t.sc[w.scene];
+        const int n0 = S.ld[l][0], n1 = S.ld[l][1];
+        const int c0 = S.ld[l + 1][0], c1 = S.ld[l + 1][1];
+        int i0, j0;
+        tile_of_unit(S, l, w.chunk, i0, j0);
+        const float* rl = rg + S.l_off[l];
+        const float* zl = zg + S.l_off[l];
+        const float* zc = zg + S.l_off[l + 1];
+        for (int q = threadIdx.x; q < W * W; q += blockDim.x) {
+            const int i = i0 - H + q / W, j = j0 - H + q % W;
+            const bool in = i >= 0 && j >= 0 && i < n0 && j < n1;
+            float zv = 0.0f, rv = 0.0f;
+            if (in) {
+                const int64_t g = (int64_t)i * n1 + j;
+                zv = zl[g] + zc[(int64_t)min(i >> 1, c0 - 1) * c1 + min(j >> 1, c1 - 1)];
+                rv = rl[g];
+            }
+            z[q] = zv;
+            r[q] = rv;
+        }
+        __syncthreads();
+        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 1, 1);
+        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 2, 0);
+        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 3, 1);
+        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 4, 0);
+        float* zo = zg + S.l_off[l];
+        for (int q = threadIdx.x; q < kBT * kBT; q += blockDim.x) {
+            const int i = i0 + q / kBT, j = j0 + q % kBT;
+            if (i < n0 && j < n1) zo[(int64_t)i * n1 + j] = z[(q / kBT + H) * W + q % kBT + H];
+        }
+        __syncthreads();
     }
 }
 
@@ -557,18 +642,12 @@ int bm_count_active(int n_scenes, const BMg* mg, int32_t* out, hipStream_t s) {
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
-int bm_smooth(BUnits pairs, int l, int phase, int zero, BWalls wl, float* z, const float* r, const BMg* mg,
-              unsigned blocks, hipStream_t s) {
-    PFM_B_LAUNCH(kb_smooth, blocks, pairs, l, phase, zero, wl, z, r, mg);
+int bm_down(BUnits tiles, int l, BWalls wl, float* z, float* r, const BMg* mg, unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_down, blocks, tiles, l, wl, z, r, mg);
     return PFM_OK;
 }
-int bm_restrict(BUnits coarse_cells, int l, BWalls wl, const float* z, float* r, const BMg* mg, unsigned blocks,
-                hipStream_t s) {
-    PFM_B_LAUNCH(kb_restrict, blocks, coarse_cells, l, wl, z, r, mg);
-    return PFM_OK;
-}
-int bm_prolong(BUnits cells, int l, float* z, const BMg* mg, unsigned blocks, hipStream_t s) {
-    PFM_B_LAUNCH(kb_prolong, blocks, cells, l, z, mg);
+int bm_up(BUnits tiles, int l, BWalls wl, float* z, const float* r, const BMg* mg, unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_up, blocks, tiles, l, wl, z, r, mg);
     return PFM_OK;
 }
 int bm_coarse(int n_scenes, const BScene* sc, BWalls wl, float* z, float* r, const BMg* mg, int threads,

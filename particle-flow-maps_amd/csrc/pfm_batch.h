@@ -129,11 +129,10 @@ int bm_mg_reset(int n_scenes, BMg* mg, int tol_mode, double tol, int max_iters, 
 int bm_set_thresh(int n_scenes, BMg* mg, double tol, hipStream_t s);
 int bm_save_solve(int n_scenes, const BMg* mg, int k, int32_t* iters, int32_t* conv, hipStream_t s);
 int bm_count_active(int n_scenes, const BMg* mg, int32_t* out, hipStream_t s);
-int bm_smooth(BUnits pairs, int l, int phase, int zero, BWalls wl, float* z, const float* r, const BMg* mg,
-              unsigned blocks, hipStream_t s);
-int bm_restrict(BUnits coarse_cells, int l, BWalls wl, const float* z, float* r, const BMg* mg, unsigned blocks,
-                hipStream_t s);
-int bm_prolong(BUnits cells, int l, float* z, const BMg* mg, unsigned blocks, hipStream_t s);
+// fused tiled V-cycle levels: units are 32 x 32 tiles of one scene's level
+constexpr int kBatchTile = 32;
+int bm_down(BUnits tiles, int l, BWalls wl, float* z, float* r, const BMg* mg, unsigned blocks, hipStream_t s);
+int bm_up(BUnits tiles, int l, BWalls wl, float* z, const float* r, const BMg* mg, unsigned blocks, hipStream_t s);
 int bm_coarse(int n_scenes, const BScene* sc, BWalls wl, float* z, float* r, const BMg* mg, int threads,
               size_t lds_bytes, hipStream_t s);
 

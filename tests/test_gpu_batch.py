@@ -184,3 +184,26 @@ def test_random_vortex_workload_steps():
     assert (info["converged"] == 1).all()
     for a in range(2):
         assert torch.isfinite(b.u[a]).all()
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3])
+def test_batch_pcg_iterations_match_single(iters):
+    """A fixed number of PCG iterations: the V-cycle of every level (tiled
+    down/up launches and the per-scene coarse kernel) reproduces the
+    single-scene solver's, so the velocities agree far below the solver
+    tolerance (only alpha/beta may round differently)."""
+    cfg = _cfg(max_iters=iters, tol=1e-30)
+    b = Batch2D(RES, cfg, seeds=list(range(len(RES))))
+    sims = []
+    for s, res in enumerate(RES):
+        c, w = _vortices(s)
+        b.set_point_vortices(s, c, w, rad=1.5 / res[1])
+        sim = Simulation(res, cfg)
+        for a in range(2):
+            sim.u[a].copy_(b.u_view(s, a))
+        sims.append(sim)
+    b.project()
+    for s, sim in enumerate(sims):
+        sim.project()
+        for a in range(2):
+            assert _rel(b.u_view(s, a), sim.u[a]) <= 1e-6, (s, a, _rel(b.u_view(s, a), sim.u[a]))
