@@ -42,6 +42,13 @@ struct Batch {
     BUnit* units = nullptr;
     List parts, faces, tiles, cells0;
     std::vector<List> mgtiles;                // per tiled level: 32 x 32 tiles
+    // compacted copies of cells0 (list 0) and mgtiles[l] (list 1 + l): the
+    // units of the scenes whose PCG is still iterating
+    int n_clists = 0;
+    std::vector<List> clist;
+    std::vector<BUnit*> cdst;
+    int32_t* cbeg = nullptr;   // n_clists x (S + 1) scene offsets within each list
+    int32_t* ccount = nullptr; // n_clists
     int Lt = 0;                               // tiled levels (max over scenes)
     BWalls wl{1, 1, 1, 1};
     BState* st = nullptr;
@@ -50,7 +57,9 @@ struct Batch {
     int32_t* conv = nullptr;
     int32_t* count = nullptr;
     int32_t* host_count = nullptr;  // pinned
-    float *r = nullptr, *z = nullptr, *x = nullptr, *p = nullptr, *Ap = nullptr;
+    // level arrays: r, the pre-smoothed z of each tiled level (z) and the
+    // V-cycle output of every level (zu; level 0 is the preconditioned r)
+    float *r = nullptr, *z = nullptr, *zu = nullptr, *x = nullptr, *p = nullptr, *Ap = nullptr;
     double *part = nullptr, *part2 = nullptr;
     int32_t *counts = nullptr, *scan_tiles = nullptr, *slot = nullptr;
     float *rec = nullptr, *p2g_partial = nullptr;
@@ -71,7 +80,20 @@ struct Batch {
     std::vector<std::pair<int, size_t>> timed;  // (kind, first event index)
 };
 
-BUnits view(const Batch* b, const List& l) { return BUnits{b->sc, b->units + l.off, l.n}; }
+BUnits view(const Batch* b, const List& l) { return BUnits{b->sc, b->units + l.off, l.n, nullptr}; }
+
+// Compacted list k (0: level-0 cells, 1 + l: tiles of level l).
+BUnits cview(const Batch* b, int k) { return BUnits{b->sc, b->cdst[k], b->clist[k].n, b->ccount + k}; }
+
+int compact(Batch* b) {
+    std::vector<const BUnit*> src(b->n_clists);
+    std::vector<int32_t> len(b->n_clists);
+    for (int k = 0; k < b->n_clists; ++k) {
+        src[k] = b->units + b->clist[k].off;
+        len[k] = b->clist[k].n;
+    }
+    return bm_compact(b->S, b->n_clists, b->mg, src.data(), len.data(), b->cbeg, b->cdst.data(), b->ccount, b->bs);
+}
 
 unsigned blocks_for(const Batch* b, int kind, const List& l) {
     const int c = b->cap[kind];
@@ -119,23 +141,26 @@ int vcycle(Batch* b) {
     hipStream_t s = b->bs;
     for (int l = 0; l < b->Lt; ++l) {
         PFM_B_TRY(timed_launch(b, kKDown, [&] {
-            return bm_down(view(b, b->mgtiles[l]), l, b->wl, b->z, b->r, b->mg, blocks_for(b, kKDown, b->mgtiles[l]),
-                           s);
+            return bm_down(cview(b, 1 + l), l, b->wl, b->z, b->r, b->mg, blocks_for(b, kKDown, b->mgtiles[l]), s);
         }));
     }
     PFM_B_TRY(timed_launch(b, kKCount, [&] {
-        return bm_coarse(b->S, b->sc, b->wl, b->z, b->r, b->mg, b->coarse_threads, b->coarse_lds, s);
+        return bm_coarse(b->S, b->sc, b->wl, b->zu, b->r, b->mg, b->coarse_threads, b->coarse_lds, s);
     }));
     for (int l = b->Lt - 1; l >= 0; --l) {
         PFM_B_TRY(timed_launch(b, kKUp, [&] {
-            return bm_up(view(b, b->mgtiles[l]), l, b->wl, b->z, b->r, b->mg, blocks_for(b, kKUp, b->mgtiles[l]), s);
+            return bm_up(cview(b, 1 + l), l, b->wl, b->z, b->zu, b->r, b->mg, blocks_for(b, kKUp, b->mgtiles[l]), s);
         }));
     }
     return PFM_OK;
 }
 
-int cells_launch(Batch* b, int (*fn)(Batch*, BUnits, unsigned)) {
-    return timed_launch(b, kKCell, [&] { return fn(b, view(b, b->cells0), blocks_for(b, kKCell, b->cells0)); });
+// A level-0 cell launch over every scene, or (active) over the compacted
+// list of the scenes still iterating.
+int cells_launch(Batch* b, int (*fn)(Batch*, BUnits, unsigned), bool active = true) {
+    return timed_launch(b, kKCell, [&] {
+        return fn(b, active ? cview(b, 0) : view(b, b->cells0), blocks_for(b, kKCell, b->cells0));
+    });
 }
 
 int finalize(Batch* b, int mode) {
@@ -151,8 +176,9 @@ int project_setup(Batch* b) {
     }));
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
         return bm_div_init(t, b->d.u[0], b->d.u[1], b->r, b->x, b->p, b->Ap, nb, b->bs);
-    }));
+    }, false));
     PFM_B_TRY(bm_mg_reset(b->S, b->mg, b->d.tol_mode, b->d.tol, b->d.max_iters, s));
+    PFM_B_TRY(compact(b));
     // all walls Neumann: recentre r (3D/mgpcg.py:215)
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
         return bm_reduce(t, kBRedSum, b->r, nullptr, b->mg, b->part, nb, b->bs);
@@ -168,14 +194,14 @@ int project_setup(Batch* b) {
     }
     PFM_B_TRY(vcycle(b));
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
-        return bm_reduce(t, kBRedZR, b->r, b->z, b->mg, b->part, nb, b->bs);
+        return bm_reduce(t, kBRedZR, b->r, b->zu, b->mg, b->part, nb, b->bs);
     }));
     return finalize(b, kBFinZTr);
 }
 
 // One PCG iteration of every active scene (3D/mgpcg.py:229-275).
 int pcg_iteration(Batch* b) {
-    PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) { return bm_update_p(t, b->z, b->p, b->mg, nb, b->bs); }));
+    PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) { return bm_update_p(t, b->zu, b->p, b->mg, nb, b->bs); }));
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
         return bm_ap_dot(t, b->wl, b->p, b->Ap, b->mg, b->part, nb, b->bs);
     }));
@@ -184,12 +210,14 @@ int pcg_iteration(Batch* b) {
         return bm_update_xr(t, b->x, b->r, b->p, b->Ap, b->mg, b->part, b->part2, nb, b->bs);
     }));
     PFM_B_TRY(finalize(b, kBFinRTr));
+    PFM_B_TRY(compact(b));
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) { return bm_sub_mean(t, b->r, b->mg, nb, b->bs); }));
     PFM_B_TRY(vcycle(b));
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
-        return bm_reduce(t, kBRedZR, b->r, b->z, b->mg, b->part, nb, b->bs);
+        return bm_reduce(t, kBRedZR, b->r, b->zu, b->mg, b->part, nb, b->bs);
     }));
-    return finalize(b, kBFinBeta);
+    PFM_B_TRY(finalize(b, kBFinBeta));
+    return compact(b);
 }
 
 // get_result + subtract_grad_p + apply_bc; records the solve in slot k.
@@ -412,9 +440,11 @@ void free_batch(Batch* b) {
     if (!b) return;
     drop_graphs(b);
     for (hipEvent_t e : b->evpool) (void)hipEventDestroy(e);
-    void* dev[] = {b->sc, b->units, b->st, b->mg, b->iters, b->conv, b->count, b->r, b->z, b->x, b->p, b->Ap,
-                   b->part, b->part2, b->counts, b->scan_tiles, b->slot, b->rec, b->p2g_partial};
+    void* dev[] = {b->sc, b->units, b->st, b->mg, b->iters, b->conv, b->count, b->r, b->z, b->zu, b->x, b->p, b->Ap,
+                   b->part, b->part2, b->counts, b->scan_tiles, b->slot, b->rec, b->p2g_partial, b->cbeg, b->ccount};
     for (void* q : dev)
+        if (q) (void)hipFree(q);
+    for (BUnit* q : b->cdst)
         if (q) (void)hipFree(q);
     if (b->host_count) (void)hipHostFree(b->host_count);
     if (b->ev_in) (void)hipEventDestroy(b->ev_in);
@@ -467,6 +497,7 @@ int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** handle, int64_t* byt
     int64_t tot[4];
     PFM_B_TRY(pfm_batch2d_sizes(d.n_scenes, d.res, d.ppc_axis, tot));
     PFM_CHECK_ARG(tot[0] < (int64_t)INT32_MAX && tot[3] < (int64_t)INT32_MAX, "batch too large for int32 ids");
+    PFM_CHECK_ARG(d.n_scenes <= kBatchCompactMaxScenes, "at most %d scenes per batch", kBatchCompactMaxScenes);
     Batch* b = new (std::nothrow) Batch();
     if (!b) {
         set_error("host allocation failed");
@@ -567,16 +598,25 @@ int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** handle, int64_t* byt
     for (int s = 0; s < b->S; ++s) add(b->tiles, s, (int64_t)b->hs[s].nt[0] * b->hs[s].nt[1]);
     b->cells0.off = (int32_t)U.size();
     for (int s = 0; s < b->S; ++s) add(b->cells0, s, b->hs[s].n_part);
+    b->n_clists = 1 + b->Lt;
+    std::vector<int32_t> hbeg((size_t)b->n_clists * (b->S + 1), 0);
+    for (int s = 0; s <= b->S; ++s) hbeg[s] = s < b->S ? b->hs[s].part_off : (int32_t)b->NP;
     b->mgtiles.resize(b->Lt);
     for (int l = 0; l < b->Lt; ++l) {
         b->mgtiles[l].off = (int32_t)U.size();
-        for (int s = 0; s < b->S; ++s)
+        int32_t* bl = hbeg.data() + (size_t)(1 + l) * (b->S + 1);
+        for (int s = 0; s < b->S; ++s) {
+            bl[s] = b->mgtiles[l].n;
             if (l < b->hs[s].coarse) {
                 const int64_t t0 = (b->hs[s].ld[l][0] + kBatchTile - 1) / kBatchTile;
                 const int64_t t1 = (b->hs[s].ld[l][1] + kBatchTile - 1) / kBatchTile;
                 add(b->mgtiles[l], s, t0 * t1);
             }
+        }
+        bl[b->S] = b->mgtiles[l].n;
     }
+    b->clist.push_back(b->cells0);
+    for (int l = 0; l < b->Lt; ++l) b->clist.push_back(b->mgtiles[l]);
     // device memory
     int64_t bytes = 0;
     auto alloc = [&](void** ptr, size_t n) {
@@ -590,13 +630,18 @@ int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** handle, int64_t* byt
               alloc((void**)&b->iters, sizeof(int32_t) * 2 * b->S) &&
               alloc((void**)&b->conv, sizeof(int32_t) * 2 * b->S) && alloc((void**)&b->count, sizeof(int32_t)) &&
               alloc((void**)&b->r, sizeof(float) * b->Lcells) && alloc((void**)&b->z, sizeof(float) * b->Lcells) &&
+              alloc((void**)&b->zu, sizeof(float) * b->Lcells) &&
               alloc((void**)&b->x, sizeof(float) * b->C) && alloc((void**)&b->p, sizeof(float) * b->C) &&
               alloc((void**)&b->Ap, sizeof(float) * b->C) && alloc((void**)&b->part, sizeof(double) * b->NP) &&
               alloc((void**)&b->part2, sizeof(double) * b->NP) &&
               alloc((void**)&b->counts, sizeof(int32_t) * (b->C + 1)) &&
               alloc((void**)&b->scan_tiles, sizeof(int32_t) * (ntiles + 1)) &&
               alloc((void**)&b->slot, sizeof(int32_t) * b->P) && alloc((void**)&b->rec, sizeof(float) * 8 * b->P) &&
-              alloc((void**)&b->p2g_partial, sizeof(float) * b_p2g_partial_floats(b->T));
+              alloc((void**)&b->p2g_partial, sizeof(float) * b_p2g_partial_floats(b->T)) &&
+              alloc((void**)&b->cbeg, sizeof(int32_t) * hbeg.size()) &&
+              alloc((void**)&b->ccount, sizeof(int32_t) * b->n_clists);
+    b->cdst.assign(b->n_clists, nullptr);
+    for (int k = 0; ok && k < b->n_clists; ++k) ok = alloc((void**)&b->cdst[k], sizeof(BUnit) * b->clist[k].n);
     if (!ok) {
         free_batch(b);
         set_error("hipMalloc failed for the batch tables and workspaces");
@@ -621,6 +666,9 @@ int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** handle, int64_t* byt
         return fail_hip(e);
     if ((e = hipMemcpy(b->units, U.data(), sizeof(BUnit) * U.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return fail_hip(e);
+    if ((e = hipMemcpy(b->cbeg, hbeg.data(), sizeof(int32_t) * hbeg.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail_hip(e);
+    if ((e = hipMemset(b->ccount, 0, sizeof(int32_t) * b->n_clists)) != hipSuccess) return fail_hip(e);
     if ((e = hipMemset(b->st, 0, sizeof(BState) * b->S)) != hipSuccess) return fail_hip(e);
     if ((e = hipMemset(b->mg, 0, sizeof(BMg) * b->S)) != hipSuccess) return fail_hip(e);
     if ((e = hipMemset(b->iters, 0, sizeof(int32_t) * 2 * b->S)) != hipSuccess) return fail_hip(e);

@@ -447,9 +447,12 @@ __global__ void __launch_bounds__(kBlock) kb_down(BUnits t, int l, BWalls wl, fl
 
 // Prolongation and post-smoothing (colours 1, 0, 1, 0) of one tile of level l
 // (3D/mgpcg.py:194-200): z[l] += z[l + 1][parent] on the tile and a halo of
-// 4, then the four passes; writes z[l] on the tile.
-__global__ void __launch_bounds__(kBlock) kb_up(BUnits t, int l, BWalls wl, float* __restrict__ zg,
-                                                const float* __restrict__ rg, const BMg* __restrict__ mg) {
+// 4, then the four passes.  The pre-smoothed z comes from zpre, the coarse
+// correction from zpost[l + 1], and the result goes to zpost[l]: tiles read
+// their halo from an array no block of the launch writes.
+__global__ void __launch_bounds__(kBlock) kb_up(BUnits t, int l, BWalls wl, const float* __restrict__ zpre,
+                                                float* __restrict__ zpost, const float* __restrict__ rg,
+                                                const BMg* __restrict__ mg) {
     constexpr int H = 4, W = kBT + 2 * H;
     __shared__ float z[W * W], r[W * W];
     PFM_FOR_UNITS(t, u) {
@@ -461,8 +464,8 @@ __global__ void __launch_bounds__(kBlock) kb_up(BUnits t, int l, BWalls wl, floa
         int i0, j0;
         tile_of_unit(S, l, w.chunk, i0, j0);
         const float* rl = rg + S.l_off[l];
-        const float* zl = zg + S.l_off[l];
-        const float* zc = zg + S.l_off[l + 1];
+        const float* zl = zpre + S.l_off[l];
+        const float* zc = zpost + S.l_off[l + 1];
         for (int q = threadIdx.x; q < W * W; q += blockDim.x) {
             const int i = i0 - H + q / W, j = j0 - H + q % W;
             const bool in = i >= 0 && j >= 0 && i < n0 && j < n1;
@@ -480,7 +483,7 @@ __global__ void __launch_bounds__(kBlock) kb_up(BUnits t, int l, BWalls wl, floa
         tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 2, 0);
         tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 3, 1);
         tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 4, 0);
-        float* zo = zg + S.l_off[l];
+        float* zo = zpost + S.l_off[l];
         for (int q = threadIdx.x; q < kBT * kBT; q += blockDim.x) {
             const int i = i0 + q / kBT, j = j0 + q % kBT;
             if (i < n0 && j < n1) zo[(int64_t)i * n1 + j] = z[(q / kBT + H) * W + q % kBT + H];
@@ -642,12 +645,82 @@ int bm_count_active(int n_scenes, const BMg* mg, int32_t* out, hipStream_t s) {
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
+// Compaction of unit tables to the scenes still iterating.  Every block
+// forms the exclusive scan over scenes of the active scenes' unit counts of
+// its list (blockIdx.y) in LDS, then moves its share of the list's units:
+// unit u of scene s goes to off[s] + (u - beg[s]), so scene order and the
+// order inside a scene are kept.
+constexpr int kMaxCompactLists = kBatchMaxLevels + 1;
+constexpr int kCompactThreads = 256;
+struct CompactArgs {
+    const BUnit* src[kMaxCompactLists];
+    BUnit* dst[kMaxCompactLists];
+    int32_t len[kMaxCompactLists];
+};
+
+__global__ void __launch_bounds__(kCompactThreads) kb_compact(int S, const BMg* __restrict__ mg, CompactArgs a,
+                                                              const int32_t* __restrict__ beg,
+                                                              int32_t* __restrict__ count) {
+    __shared__ int32_t off[kBatchCompactMaxScenes + 1];
+    __shared__ int32_t wsum[kCompactThreads / 64];
+    const int k = blockIdx.y;
+    const int32_t* bk = beg + (int64_t)k * (S + 1);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int32_t carry = 0;
+    for (int base = 0; base < S; base += blockDim.x) {
+        const int s = base + threadIdx.x;
+        const int32_t n = (s < S && mg[s].active) ? bk[s + 1] - bk[s] : 0;
+        int32_t x = n;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        int32_t woff = 0, tot = 0;
+        for (int w2 = 0; w2 < kCompactThreads / 64; ++w2) {
+            if (w2 < wid) woff += wsum[w2];
+            tot += wsum[w2];
+        }
+        if (s < S) off[s] = carry + woff + x - n;
+        carry += tot;
+        __syncthreads();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) count[k] = carry;
+    for (int32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < a.len[k]; u += gridDim.x * blockDim.x) {
+        const BUnit w = a.src[k][u];
+        if (mg[w.scene].active) a.dst[k][off[w.scene] + (u - bk[w.scene])] = w;
+    }
+}
+
+int bm_compact(int n_scenes, int n_lists, const BMg* mg, const BUnit* const* src, const int32_t* len,
+               const int32_t* beg, BUnit* const* dst, int32_t* count, hipStream_t s) {
+    PFM_CHECK_ARG(n_lists >= 1 && n_lists <= kMaxCompactLists, "bad list count %d", n_lists);
+    PFM_CHECK_ARG(n_scenes <= kBatchCompactMaxScenes, "too many scenes for the compaction (%d)", n_scenes);
+    CompactArgs a{};
+    int32_t mx = 1;
+    for (int k = 0; k < n_lists; ++k) {
+        a.src[k] = src[k];
+        a.dst[k] = dst[k];
+        a.len[k] = len[k];
+        mx = len[k] > mx ? len[k] : mx;
+    }
+    unsigned gx = (unsigned)((mx + kCompactThreads - 1) / kCompactThreads);
+    if (gx > 256) gx = 256;
+    hipLaunchKernelGGL(kb_compact, dim3(gx, (unsigned)n_lists), dim3(kCompactThreads), 0, s, n_scenes, mg, a, beg,
+                       count);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
 int bm_down(BUnits tiles, int l, BWalls wl, float* z, float* r, const BMg* mg, unsigned blocks, hipStream_t s) {
     PFM_B_LAUNCH(kb_down, blocks, tiles, l, wl, z, r, mg);
     return PFM_OK;
 }
-int bm_up(BUnits tiles, int l, BWalls wl, float* z, const float* r, const BMg* mg, unsigned blocks, hipStream_t s) {
-    PFM_B_LAUNCH(kb_up, blocks, tiles, l, wl, z, r, mg);
+int bm_up(BUnits tiles, int l, BWalls wl, const float* zpre, float* zpost, const float* r, const BMg* mg,
+          unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_up, blocks, tiles, l, wl, zpre, zpost, r, mg);
     return PFM_OK;
 }
 int bm_coarse(int n_scenes, const BScene* sc, BWalls wl, float* z, float* r, const BMg* mg, int threads,

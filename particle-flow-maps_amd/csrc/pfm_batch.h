@@ -62,15 +62,20 @@ struct BMg {
     int32_t active, it, converged, iters;
 };
 
-// Kernel-side view of one unit table.
+// Kernel-side view of one unit table.  A compacted table (the units of the
+// scenes whose PCG is still running) keeps its length in device memory
+// (nu_dev), written by the compaction kernel that ran before the launch.
 struct BUnits {
     const BScene* sc;
     const BUnit* u;
     int32_t nu;
+    const int32_t* nu_dev;
 };
 
 // Persistent loop over the units of a launch: `UNIT` is the unit index.
-#define PFM_FOR_UNITS(T, UNIT) for (int32_t UNIT = blockIdx.x; UNIT < (T).nu; UNIT += gridDim.x)
+#define PFM_FOR_UNITS(T, UNIT)                                                              \
+    for (int32_t UNIT = blockIdx.x, UNIT##_n = (T).nu_dev ? *(T).nu_dev : (T).nu; UNIT < UNIT##_n; \
+         UNIT += gridDim.x)
 
 // ---- launchers (implemented beside the single-scene kernels they share) ----
 // Every launcher takes the grid size of the persistent launch (`blocks`).
@@ -132,7 +137,15 @@ int bm_count_active(int n_scenes, const BMg* mg, int32_t* out, hipStream_t s);
 // fused tiled V-cycle levels: units are 32 x 32 tiles of one scene's level
 constexpr int kBatchTile = 32;
 int bm_down(BUnits tiles, int l, BWalls wl, float* z, float* r, const BMg* mg, unsigned blocks, hipStream_t s);
-int bm_up(BUnits tiles, int l, BWalls wl, float* z, const float* r, const BMg* mg, unsigned blocks, hipStream_t s);
+int bm_up(BUnits tiles, int l, BWalls wl, const float* zpre, float* zpost, const float* r, const BMg* mg,
+          unsigned blocks, hipStream_t s);
+// Compaction of unit tables to the active scenes: list k (len[k] units,
+// grouped by scene in scene order) holds the units of scene s at
+// [beg[k * (S + 1) + s], beg[k * (S + 1) + s + 1]); the active scenes' units
+// go to dst[k] (same capacity) and their count to count[k].
+constexpr int kBatchCompactMaxScenes = 8192;
+int bm_compact(int n_scenes, int n_lists, const BMg* mg, const BUnit* const* src, const int32_t* len,
+               const int32_t* beg, BUnit* const* dst, int32_t* count, hipStream_t s);
 int bm_coarse(int n_scenes, const BScene* sc, BWalls wl, float* z, float* r, const BMg* mg, int threads,
               size_t lds_bytes, hipStream_t s);
 
