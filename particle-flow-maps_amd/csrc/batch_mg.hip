@@ -366,17 +366,20 @@ __device__ __forceinline__ float bnsum_s(const float* x, int stride, int n0, int
     return ret;
 }
 
-// One colour over the cells within `ring` of the tile (LDS arrays of width W,
-// tile origin (i0, j0) at local (H, H)).
-template <int H>
+// One colour over the cells within RING of the tile (LDS arrays of width W,
+// tile origin (i0, j0) at local (H, H)): thread t takes row t / HP and the
+// member of pair t % HP of that row whose colour is `phase`.
+template <int H, int RING>
 __device__ __forceinline__ void tile_sweep(const BWalls& wl, float* z, const float* r, int n0, int n1, int i0,
-                                           int j0, int ring, int phase) {
-    constexpr int W = kBT + 2 * H;
-    const int lo = H - ring, R = kBT + 2 * ring;
-    for (int t = threadIdx.x; t < R * R; t += blockDim.x) {
-        const int li = lo + t / R, lj = lo + t % R;
-        const int i = i0 - H + li, j = j0 - H + lj;
-        if (i < 0 || j < 0 || i >= n0 || j >= n1 || ((i + j) & 1) != phase) continue;
+                                           int j0, int phase) {
+    constexpr int W = kBT + 2 * H, lo = H - RING, R = kBT + 2 * RING, HP = (R + 1) / 2;
+    const int jstart = j0 - H + lo;
+    for (int t = threadIdx.x; t < R * HP; t += blockDim.x) {
+        const int li = lo + t / HP;
+        const int i = i0 - H + li;
+        const int lj = lo + 2 * (t % HP) + ((i + jstart + phase) & 1);
+        const int j = j0 - H + lj;
+        if (lj >= lo + R || i < 0 || j < 0 || i >= n0 || j >= n1) continue;
         const int q = li * W + lj;
         z[q] = (r[q] + bnsum_s(z + q, W, n0, n1, i, j)) / bnfn(wl, n0, n1, i, j);
     }
@@ -413,10 +416,10 @@ __global__ void __launch_bounds__(kBlock) kb_down(BUnits t, int l, BWalls wl, fl
             z[q] = 0.0f;
         }
         __syncthreads();
-        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 1, 0);
-        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 2, 1);
-        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 3, 0);
-        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 4, 1);
+        tile_sweep<H, H - 1>(wl, z, r, n0, n1, i0, j0, 0);
+        tile_sweep<H, H - 2>(wl, z, r, n0, n1, i0, j0, 1);
+        tile_sweep<H, H - 3>(wl, z, r, n0, n1, i0, j0, 0);
+        tile_sweep<H, H - 4>(wl, z, r, n0, n1, i0, j0, 1);
         float* zl = zg + S.l_off[l];
         for (int q = threadIdx.x; q < kBT * kBT; q += blockDim.x) {
             const int i = i0 + q / kBT, j = j0 + q % kBT;
@@ -479,10 +482,10 @@ __global__ void __launch_bounds__(kBlock) kb_up(BUnits t, int l, BWalls wl, cons
             r[q] = rv;
         }
         __syncthreads();
-        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 1, 1);
-        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 2, 0);
-        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 3, 1);
-        tile_sweep<H>(wl, z, r, n0, n1, i0, j0, H - 4, 0);
+        tile_sweep<H, H - 1>(wl, z, r, n0, n1, i0, j0, 1);
+        tile_sweep<H, H - 2>(wl, z, r, n0, n1, i0, j0, 0);
+        tile_sweep<H, H - 3>(wl, z, r, n0, n1, i0, j0, 1);
+        tile_sweep<H, H - 4>(wl, z, r, n0, n1, i0, j0, 0);
         float* zo = zpost + S.l_off[l];
         for (int q = threadIdx.x; q < kBT * kBT; q += blockDim.x) {
             const int i = i0 + q / kBT, j = j0 + q % kBT;
