@@ -25,7 +25,7 @@ enum Kind {
 constexpr int kCapCand[] = {0, 4096, 2048, 1024, 512};
 constexpr int kNCap = sizeof(kCapCand) / sizeof(kCapCand[0]);
 constexpr int kCoarseThreadCand[] = {256, 512, 1024};
-constexpr int kItersPerReplay = 4;
+constexpr int kItersPerReplay = 4;  // even: the p ping-pong is back on pp[0] after a replay
 
 struct List {
     int32_t off = 0, n = 0;
@@ -59,8 +59,11 @@ struct Batch {
     int32_t* host_count = nullptr;  // pinned
     // level arrays: r, the pre-smoothed z of each tiled level (z) and the
     // V-cycle output of every level (zu; level 0 is the preconditioned r)
-    float *r = nullptr, *z = nullptr, *zu = nullptr, *x = nullptr, *p = nullptr, *Ap = nullptr;
+    float *r = nullptr, *z = nullptr, *zu = nullptr, *x = nullptr, *Ap = nullptr;
+    float* pp[2] = {nullptr, nullptr};  // search direction, ping-pong (iteration k reads pp[k & 1])
     double *part = nullptr, *part2 = nullptr;
+    double* zr = nullptr;               // z.r partials (BScene::zr_off)
+    int64_t NZR = 0;
     int32_t *counts = nullptr, *scan_tiles = nullptr, *slot = nullptr;
     float *rec = nullptr, *p2g_partial = nullptr;
     size_t coarse_lds = 0;
@@ -145,11 +148,12 @@ int vcycle(Batch* b) {
         }));
     }
     PFM_B_TRY(timed_launch(b, kKCount, [&] {
-        return bm_coarse(b->S, b->sc, b->wl, b->zu, b->r, b->mg, b->coarse_threads, b->coarse_lds, s);
+        return bm_coarse(b->S, b->sc, b->wl, b->zu, b->r, b->mg, b->zr, b->coarse_threads, b->coarse_lds, s);
     }));
     for (int l = b->Lt - 1; l >= 0; --l) {
         PFM_B_TRY(timed_launch(b, kKUp, [&] {
-            return bm_up(cview(b, 1 + l), l, b->wl, b->z, b->zu, b->r, b->mg, blocks_for(b, kKUp, b->mgtiles[l]), s);
+            return bm_up(cview(b, 1 + l), l, b->wl, b->z, b->zu, b->r, b->mg, b->zr,
+                         blocks_for(b, kKUp, b->mgtiles[l]), s);
         }));
     }
     return PFM_OK;
@@ -164,7 +168,7 @@ int cells_launch(Batch* b, int (*fn)(Batch*, BUnits, unsigned), bool active = tr
 }
 
 int finalize(Batch* b, int mode) {
-    return bm_finalize(b->S, b->sc, mode, 1e-12, b->d.max_iters, b->part, b->part2, b->mg, b->bs);
+    return bm_finalize(b->S, b->sc, mode, 1e-12, b->d.max_iters, b->part, b->part2, b->zr, b->mg, b->bs);
 }
 
 // apply_bc, divergence, init and the PCG set-up through the first
@@ -175,7 +179,7 @@ int project_setup(Batch* b) {
         return bm_zero_walls(view(b, b->faces), b->d.u[0], b->d.u[1], blocks_for(b, kKFace, b->faces), s);
     }));
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
-        return bm_div_init(t, b->d.u[0], b->d.u[1], b->r, b->x, b->p, b->Ap, nb, b->bs);
+        return bm_div_init(t, b->d.u[0], b->d.u[1], b->r, b->x, b->pp[0], b->Ap, nb, b->bs);
     }, false));
     PFM_B_TRY(bm_mg_reset(b->S, b->mg, b->d.tol_mode, b->d.tol, b->d.max_iters, s));
     PFM_B_TRY(compact(b));
@@ -192,30 +196,28 @@ int project_setup(Batch* b) {
         PFM_B_TRY(finalize(b, kBFinR0));
         PFM_B_TRY(bm_set_thresh(b->S, b->mg, b->d.tol, s));
     }
-    PFM_B_TRY(vcycle(b));
-    PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
-        return bm_reduce(t, kBRedZR, b->r, b->zu, b->mg, b->part, nb, b->bs);
-    }));
+    PFM_B_TRY(vcycle(b));  // leaves the z.r partials
     return finalize(b, kBFinZTr);
 }
 
-// One PCG iteration of every active scene (3D/mgpcg.py:229-275).
-int pcg_iteration(Batch* b) {
-    PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) { return bm_update_p(t, b->zu, b->p, b->mg, nb, b->bs); }));
-    PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
-        return bm_ap_dot(t, b->wl, b->p, b->Ap, b->mg, b->part, nb, b->bs);
+// PCG iteration k of every active scene (3D/mgpcg.py:229-275): update_p and
+// Ap in one launch from pp[k & 1] into pp[(k + 1) & 1].
+int pcg_iteration(Batch* b, int k) {
+    float* pin = b->pp[k & 1];
+    float* pout = b->pp[(k + 1) & 1];
+    PFM_B_TRY(timed_launch(b, kKCell, [&] {
+        return bm_ap_dot(cview(b, 0), b->wl, b->zu, pin, pout, b->Ap, b->mg, b->part,
+                         blocks_for(b, kKCell, b->cells0), b->bs);
     }));
     PFM_B_TRY(finalize(b, kBFinAlpha));
-    PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
-        return bm_update_xr(t, b->x, b->r, b->p, b->Ap, b->mg, b->part, b->part2, nb, b->bs);
+    PFM_B_TRY(timed_launch(b, kKCell, [&] {
+        return bm_update_xr(cview(b, 0), b->x, b->r, pout, b->Ap, b->mg, b->part, b->part2,
+                            blocks_for(b, kKCell, b->cells0), b->bs);
     }));
     PFM_B_TRY(finalize(b, kBFinRTr));
     PFM_B_TRY(compact(b));
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) { return bm_sub_mean(t, b->r, b->mg, nb, b->bs); }));
-    PFM_B_TRY(vcycle(b));
-    PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
-        return bm_reduce(t, kBRedZR, b->r, b->zu, b->mg, b->part, nb, b->bs);
-    }));
+    PFM_B_TRY(vcycle(b));  // leaves the z.r partials
     PFM_B_TRY(finalize(b, kBFinBeta));
     return compact(b);
 }
@@ -278,7 +280,7 @@ int solve_loop(Batch* b, int* lockstep) {
     const int limit = b->d.max_iters + kItersPerReplay;
     while (true) {
         PFM_B_TRY(run_segment(b, kSegIter, [&] {
-            for (int k = 0; k < kItersPerReplay; ++k) PFM_B_TRY(pcg_iteration(b));
+            for (int k = 0; k < kItersPerReplay; ++k) PFM_B_TRY(pcg_iteration(b, k));
             return bm_count_active(b->S, b->mg, b->count, b->bs);
         }));
         n += kItersPerReplay;
@@ -440,8 +442,9 @@ void free_batch(Batch* b) {
     if (!b) return;
     drop_graphs(b);
     for (hipEvent_t e : b->evpool) (void)hipEventDestroy(e);
-    void* dev[] = {b->sc, b->units, b->st, b->mg, b->iters, b->conv, b->count, b->r, b->z, b->zu, b->x, b->p, b->Ap,
-                   b->part, b->part2, b->counts, b->scan_tiles, b->slot, b->rec, b->p2g_partial, b->cbeg, b->ccount};
+    void* dev[] = {b->sc, b->units, b->st, b->mg, b->iters, b->conv, b->count, b->r, b->z, b->zu, b->x, b->pp[0],
+                   b->pp[1], b->Ap, b->part, b->part2, b->zr, b->counts, b->scan_tiles, b->slot, b->rec,
+                   b->p2g_partial, b->cbeg, b->ccount};
     for (void* q : dev)
         if (q) (void)hipFree(q);
     for (BUnit* q : b->cdst)
@@ -579,6 +582,10 @@ int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** handle, int64_t* byt
         S.part_off = part_off;
         S.n_part = (int32_t)((S.nc + kBlock - 1) / kBlock);
         part_off += S.n_part;
+        S.zr_off = (int32_t)b->NZR;
+        S.zr_n = S.coarse == 0 ? 1
+                               : (int32_t)(((n0 + kBatchTile - 1) / kBatchTile) * ((n1 + kBatchTile - 1) / kBatchTile));
+        b->NZR += S.zr_n;
         p_off += S.n;
         f_off0 += S.nf[0];
         f_off1 += S.nf[1];
@@ -631,7 +638,8 @@ int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** handle, int64_t* byt
               alloc((void**)&b->conv, sizeof(int32_t) * 2 * b->S) && alloc((void**)&b->count, sizeof(int32_t)) &&
               alloc((void**)&b->r, sizeof(float) * b->Lcells) && alloc((void**)&b->z, sizeof(float) * b->Lcells) &&
               alloc((void**)&b->zu, sizeof(float) * b->Lcells) &&
-              alloc((void**)&b->x, sizeof(float) * b->C) && alloc((void**)&b->p, sizeof(float) * b->C) &&
+              alloc((void**)&b->x, sizeof(float) * b->C) && alloc((void**)&b->pp[0], sizeof(float) * b->C) &&
+              alloc((void**)&b->pp[1], sizeof(float) * b->C) && alloc((void**)&b->zr, sizeof(double) * b->NZR) &&
               alloc((void**)&b->Ap, sizeof(float) * b->C) && alloc((void**)&b->part, sizeof(double) * b->NP) &&
               alloc((void**)&b->part2, sizeof(double) * b->NP) &&
               alloc((void**)&b->counts, sizeof(int32_t) * (b->C + 1)) &&

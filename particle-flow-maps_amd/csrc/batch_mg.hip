@@ -146,22 +146,12 @@ __global__ void __launch_bounds__(kBlock) kb_reduce(BUnits t, int mode, const fl
     }
 }
 
-// update_p (3D/mgpcg.py:147-150): p = z + beta p.
-__global__ void __launch_bounds__(kBlock) kb_update_p(BUnits t, const float* __restrict__ z, float* __restrict__ p,
-                                                      const BMg* __restrict__ mg) {
-    PFM_FOR_UNITS(t, u) {
-        const BUnit w = t.u[u];
-        if (!scene_active(mg, w.scene)) continue;
-        const BScene& S = t.sc[w.scene];
-        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
-        if (q >= S.nc) continue;
-        const int64_t g = S.c_off + q;
-        p[g] = z[g] + mg[w.scene].beta * p[g];
-    }
-}
-
-// compute_Ap (3D/mgpcg.py:117-122) with the p.Ap partials.
-__global__ void __launch_bounds__(kBlock) kb_ap_dot(BUnits t, BWalls wl, const float* __restrict__ p,
+// update_p (3D/mgpcg.py:147-150) fused with compute_Ap (:117-122) and the
+// p.Ap partials: p_new = z + beta p_old is formed for the cell and its four
+// neighbours from p_old (pin) and written to pout, so no block reads a p that
+// another block of the launch has already replaced.
+__global__ void __launch_bounds__(kBlock) kb_ap_dot(BUnits t, BWalls wl, const float* __restrict__ z,
+                                                    const float* __restrict__ pin, float* __restrict__ pout,
                                                     float* __restrict__ Ap, const BMg* __restrict__ mg,
                                                     double* __restrict__ part) {
     __shared__ double sh[kBlock / 64];
@@ -170,14 +160,23 @@ __global__ void __launch_bounds__(kBlock) kb_ap_dot(BUnits t, BWalls wl, const f
         if (!scene_active(mg, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
+        const float beta = mg[w.scene].beta;
         double v = 0.0;
         if (q < S.nc) {
             const int n0 = S.g.res[0], n1 = S.g.res[1];
             const int i = (int)(q / n1), j = (int)(q % n1);
             const int64_t g = S.c_off + q;
-            const float a = bnfn(wl, n0, n1, i, j) * p[g] - bnsum(p + g, n0, n1, i, j);
+            const float pc = z[g] + beta * pin[g];
+            // neighbor_sum of p_new, axis 0 then axis 1, + then - (as bnsum)
+            float ns = 0.0f;
+            if (i < n0 - 1) ns = ns + (z[g + n1] + beta * pin[g + n1]);
+            if (i > 0) ns = ns + (z[g - n1] + beta * pin[g - n1]);
+            if (j < n1 - 1) ns = ns + (z[g + 1] + beta * pin[g + 1]);
+            if (j > 0) ns = ns + (z[g - 1] + beta * pin[g - 1]);
+            const float a = bnfn(wl, n0, n1, i, j) * pc - ns;
+            pout[g] = pc;
             Ap[g] = a;
-            v = (double)p[g] * (double)a;
+            v = (double)pc * (double)a;
         }
         v = bblock_sum(v, sh);
         if (threadIdx.x == 0) part[S.part_off + w.chunk] = v;
@@ -231,13 +230,19 @@ __global__ void __launch_bounds__(kBlock) kb_sub_mean(BUnits t, float* __restric
 // per scene; the scene's partials are summed in unit order.
 __global__ void __launch_bounds__(kBlock) kb_finalize(const BScene* __restrict__ sc, int mode, double eps,
                                                       int max_iters, const double* __restrict__ part,
-                                                      const double* __restrict__ part2, BMg* __restrict__ mg) {
+                                                      const double* __restrict__ part2,
+                                                      const double* __restrict__ zr, BMg* __restrict__ mg) {
     __shared__ double sh[kBlock / 64];
     const int s = blockIdx.x;
     if (!scene_active(mg, s)) return;
     const BScene& S = sc[s];
     double v = 0.0, v1 = 0.0;
-    for (int b = threadIdx.x; b < S.n_part; b += blockDim.x) v += part[S.part_off + b];
+    if (mode == kBFinZTr || mode == kBFinBeta) {
+        // z.r from the V-cycle's last launch (level-0 tiles, or the coarse kernel)
+        for (int b = threadIdx.x; b < S.zr_n; b += blockDim.x) v += zr[S.zr_off + b];
+    } else {
+        for (int b = threadIdx.x; b < S.n_part; b += blockDim.x) v += part[S.part_off + b];
+    }
     v = block_sum(v, sh);
     if (mode == kBFinRTr) {
         for (int b = threadIdx.x; b < S.n_part; b += blockDim.x) v1 += part2[S.part_off + b];
@@ -453,11 +458,15 @@ __global__ void __launch_bounds__(kBlock) kb_down(BUnits t, int l, BWalls wl, fl
 // 4, then the four passes.  The pre-smoothed z comes from zpre, the coarse
 // correction from zpost[l + 1], and the result goes to zpost[l]: tiles read
 // their halo from an array no block of the launch writes.
+// DOT (level 0): also the z.r partial of the tile (the PCG's z.r after the
+// preconditioner, 3D/mgpcg.py:231-235 and :264-268), into zr[S.zr_off + tile].
+template <bool DOT>
 __global__ void __launch_bounds__(kBlock) kb_up(BUnits t, int l, BWalls wl, const float* __restrict__ zpre,
                                                 float* __restrict__ zpost, const float* __restrict__ rg,
-                                                const BMg* __restrict__ mg) {
+                                                const BMg* __restrict__ mg, double* __restrict__ zr) {
     constexpr int H = 4, W = kBT + 2 * H;
     __shared__ float z[W * W], r[W * W];
+    __shared__ double sh[kBlock / 64];
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
         if (!scene_active(mg, w.scene)) continue;
@@ -487,9 +496,18 @@ __global__ void __launch_bounds__(kBlock) kb_up(BUnits t, int l, BWalls wl, cons
         tile_sweep<H, H - 3>(wl, z, r, n0, n1, i0, j0, 1);
         tile_sweep<H, H - 4>(wl, z, r, n0, n1, i0, j0, 0);
         float* zo = zpost + S.l_off[l];
+        double v = 0.0;
         for (int q = threadIdx.x; q < kBT * kBT; q += blockDim.x) {
             const int i = i0 + q / kBT, j = j0 + q % kBT;
-            if (i < n0 && j < n1) zo[(int64_t)i * n1 + j] = z[(q / kBT + H) * W + q % kBT + H];
+            const int lq = (q / kBT + H) * W + q % kBT + H;
+            if (i < n0 && j < n1) {
+                zo[(int64_t)i * n1 + j] = z[lq];
+                if (DOT) v += (double)z[lq] * (double)r[lq];
+            }
+        }
+        if (DOT) {
+            v = bblock_sum(v, sh);
+            if (threadIdx.x == 0) zr[S.zr_off + w.chunk] = v;
         }
         __syncthreads();
     }
@@ -514,8 +532,9 @@ __device__ __forceinline__ void lds_smooth(const BWalls& wl, float* z, const flo
 // Levels [coarse, n_levels) of each active scene: the rest of the V-cycle
 // (3D/mgpcg.py:182-200) from r[coarse] to z[coarse], bottom solve included.
 __global__ void kb_coarse(const BScene* __restrict__ sc, BWalls wl, float* __restrict__ zg,
-                          float* __restrict__ rg, const BMg* __restrict__ mg) {
+                          float* __restrict__ rg, const BMg* __restrict__ mg, double* __restrict__ zr) {
     extern __shared__ float lds[];
+    __shared__ double wsum[1024 / 64];
     const int s = blockIdx.x;
     if (!scene_active(mg, s)) return;
     const BScene& S = sc[s];
@@ -574,7 +593,22 @@ __global__ void kb_coarse(const BScene* __restrict__ sc, BWalls wl, float* __res
     }
     {
         const int n = S.ld[lc][0] * S.ld[lc][1];
-        for (int q = threadIdx.x; q < n; q += blockDim.x) zg[S.l_off[lc] + q] = lds[zo[lc] + q];
+        double v = 0.0;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            zg[S.l_off[lc] + q] = lds[zo[lc] + q];
+            if (lc == 0) v += (double)lds[zo[lc] + q] * (double)lds[ro[lc] + q];
+        }
+        if (lc == 0) {
+            // the scene's z.r: wave sums, then the waves in order
+            v = wave_sum(v);
+            if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double tot = 0.0;
+                for (int k = 0; k < (int)(blockDim.x >> 6); ++k) tot += wsum[k];
+                zr[S.zr_off] = tot;
+            }
+        }
     }
 }
 
@@ -604,13 +638,9 @@ int bm_reduce(BUnits cells, int mode, const float* r, const float* z, const BMg*
     PFM_B_LAUNCH(kb_reduce, blocks, cells, mode, r, z, mg, part);
     return PFM_OK;
 }
-int bm_update_p(BUnits cells, const float* z, float* p, const BMg* mg, unsigned blocks, hipStream_t s) {
-    PFM_B_LAUNCH(kb_update_p, blocks, cells, z, p, mg);
-    return PFM_OK;
-}
-int bm_ap_dot(BUnits cells, BWalls wl, const float* p, float* Ap, const BMg* mg, double* part, unsigned blocks,
-              hipStream_t s) {
-    PFM_B_LAUNCH(kb_ap_dot, blocks, cells, wl, p, Ap, mg, part);
+int bm_ap_dot(BUnits cells, BWalls wl, const float* z, const float* pin, float* pout, float* Ap, const BMg* mg,
+              double* part, unsigned blocks, hipStream_t s) {
+    PFM_B_LAUNCH(kb_ap_dot, blocks, cells, wl, z, pin, pout, Ap, mg, part);
     return PFM_OK;
 }
 int bm_update_xr(BUnits cells, float* x, float* r, const float* p, const float* Ap, const BMg* mg, double* part,
@@ -623,8 +653,8 @@ int bm_sub_mean(BUnits cells, float* r, const BMg* mg, unsigned blocks, hipStrea
     return PFM_OK;
 }
 int bm_finalize(int n_scenes, const BScene* sc, int mode, double eps, int max_iters, const double* part,
-                const double* part2, BMg* mg, hipStream_t s) {
-    PFM_B_LAUNCH(kb_finalize, n_scenes, sc, mode, eps, max_iters, part, part2, mg);
+                const double* part2, const double* zr, BMg* mg, hipStream_t s) {
+    PFM_B_LAUNCH(kb_finalize, n_scenes, sc, mode, eps, max_iters, part, part2, zr, mg);
     return PFM_OK;
 }
 int bm_mg_reset(int n_scenes, BMg* mg, int tol_mode, double tol, int max_iters, hipStream_t s) {
@@ -722,13 +752,16 @@ int bm_down(BUnits tiles, int l, BWalls wl, float* z, float* r, const BMg* mg, u
     return PFM_OK;
 }
 int bm_up(BUnits tiles, int l, BWalls wl, const float* zpre, float* zpost, const float* r, const BMg* mg,
-          unsigned blocks, hipStream_t s) {
-    PFM_B_LAUNCH(kb_up, blocks, tiles, l, wl, zpre, zpost, r, mg);
+          double* zr, unsigned blocks, hipStream_t s) {
+    if (l == 0)
+        PFM_B_LAUNCH(kb_up<true>, blocks, tiles, l, wl, zpre, zpost, r, mg, zr);
+    else
+        PFM_B_LAUNCH(kb_up<false>, blocks, tiles, l, wl, zpre, zpost, r, mg, zr);
     return PFM_OK;
 }
-int bm_coarse(int n_scenes, const BScene* sc, BWalls wl, float* z, float* r, const BMg* mg, int threads,
+int bm_coarse(int n_scenes, const BScene* sc, BWalls wl, float* z, float* r, const BMg* mg, double* zr, int threads,
               size_t lds_bytes, hipStream_t s) {
-    hipLaunchKernelGGL(kb_coarse, dim3(n_scenes), dim3(threads), lds_bytes, s, sc, wl, z, r, mg);
+    hipLaunchKernelGGL(kb_coarse, dim3(n_scenes), dim3(threads), lds_bytes, s, sc, wl, z, r, mg, zr);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

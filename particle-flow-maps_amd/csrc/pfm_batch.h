@@ -41,6 +41,7 @@ struct BScene {
     int32_t ld[kBatchMaxLevels][2];
     int64_t l_off[kBatchMaxLevels];  // offset of level l in the level arrays
     int32_t part_off, n_part;        // level-0 reduction partials of the scene
+    int32_t zr_off, zr_n;            // its z.r partials (level-0 V-cycle tiles, or 1 from the coarse kernel)
 };
 
 struct BUnit {
@@ -122,14 +123,13 @@ int bm_div_init(BUnits cells, const float* ux, const float* uy, float* r, float*
 int bm_subtract_grad_p(BUnits faces, const float* x, float* ux, float* uy, unsigned blocks, hipStream_t s);
 int bm_reduce(BUnits cells, int mode, const float* r, const float* z, const BMg* mg, double* part, unsigned blocks,
               hipStream_t s);
-int bm_update_p(BUnits cells, const float* z, float* p, const BMg* mg, unsigned blocks, hipStream_t s);
-int bm_ap_dot(BUnits cells, BWalls wl, const float* p, float* Ap, const BMg* mg, double* part, unsigned blocks,
-              hipStream_t s);
+int bm_ap_dot(BUnits cells, BWalls wl, const float* z, const float* pin, float* pout, float* Ap, const BMg* mg,
+              double* part, unsigned blocks, hipStream_t s);
 int bm_update_xr(BUnits cells, float* x, float* r, const float* p, const float* Ap, const BMg* mg, double* part,
                  double* part2, unsigned blocks, hipStream_t s);
 int bm_sub_mean(BUnits cells, float* r, const BMg* mg, unsigned blocks, hipStream_t s);
 int bm_finalize(int n_scenes, const BScene* sc, int mode, double eps, int max_iters, const double* part,
-                const double* part2, BMg* mg, hipStream_t s);
+                const double* part2, const double* zr, BMg* mg, hipStream_t s);
 int bm_mg_reset(int n_scenes, BMg* mg, int tol_mode, double tol, int max_iters, hipStream_t s);
 int bm_set_thresh(int n_scenes, BMg* mg, double tol, hipStream_t s);
 int bm_save_solve(int n_scenes, const BMg* mg, int k, int32_t* iters, int32_t* conv, hipStream_t s);
@@ -138,7 +138,7 @@ int bm_count_active(int n_scenes, const BMg* mg, int32_t* out, hipStream_t s);
 constexpr int kBatchTile = 32;
 int bm_down(BUnits tiles, int l, BWalls wl, float* z, float* r, const BMg* mg, unsigned blocks, hipStream_t s);
 int bm_up(BUnits tiles, int l, BWalls wl, const float* zpre, float* zpost, const float* r, const BMg* mg,
-          unsigned blocks, hipStream_t s);
+          double* zr, unsigned blocks, hipStream_t s);
 // Compaction of unit tables to the active scenes: list k (len[k] units,
 // grouped by scene in scene order) holds the units of scene s at
 // [beg[k * (S + 1) + s], beg[k * (S + 1) + s + 1]); the active scenes' units
@@ -146,7 +146,7 @@ int bm_up(BUnits tiles, int l, BWalls wl, const float* zpre, float* zpost, const
 constexpr int kBatchCompactMaxScenes = 8192;
 int bm_compact(int n_scenes, int n_lists, const BMg* mg, const BUnit* const* src, const int32_t* len,
                const int32_t* beg, BUnit* const* dst, int32_t* count, hipStream_t s);
-int bm_coarse(int n_scenes, const BScene* sc, BWalls wl, float* z, float* r, const BMg* mg, int threads,
+int bm_coarse(int n_scenes, const BScene* sc, BWalls wl, float* z, float* r, const BMg* mg, double* zr, int threads,
               size_t lds_bytes, hipStream_t s);
 
 }  // namespace pfm
