@@ -640,8 +640,8 @@ int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** handle, int64_t* byt
               alloc((void**)&b->zu, sizeof(float) * b->Lcells) &&
               alloc((void**)&b->x, sizeof(float) * b->C) && alloc((void**)&b->pp[0], sizeof(float) * b->C) &&
               alloc((void**)&b->pp[1], sizeof(float) * b->C) && alloc((void**)&b->zr, sizeof(double) * b->NZR) &&
-              alloc((void**)&b->Ap, sizeof(float) * b->C) && alloc((void**)&b->part, sizeof(double) * b->NP) &&
-              alloc((void**)&b->part2, sizeof(double) * b->NP) &&
+              alloc((void**)&b->Ap, sizeof(float) * b->C) && alloc((void**)&b->part, sizeof(double) * b->NP * kBatchWaves) &&
+              alloc((void**)&b->part2, sizeof(double) * b->NP * kBatchWaves) &&
               alloc((void**)&b->counts, sizeof(int32_t) * (b->C + 1)) &&
               alloc((void**)&b->scan_tiles, sizeof(int32_t) * (ntiles + 1)) &&
               alloc((void**)&b->slot, sizeof(int32_t) * b->P) && alloc((void**)&b->rec, sizeof(float) * 8 * b->P) &&

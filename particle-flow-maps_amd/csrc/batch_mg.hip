@@ -125,11 +125,18 @@ __global__ void __launch_bounds__(kBlock) kb_subtract_grad_p(BUnits t, const flo
 // Block sum of one double over kBlock threads (valid in thread 0).
 __device__ __forceinline__ double bblock_sum(double v, double* sh) { return block_sum(v, sh); }
 
+// Level-0 partial sums are kept per wave (kBatchWaves per unit of kBlock
+// cells): a wave sum needs no block barrier, and the per-scene sum over
+// (unit, wave) in order is still fixed.
+__device__ __forceinline__ void wave_partial(double v, double* __restrict__ part, int32_t unit) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) part[(int64_t)unit * kBatchWaves + (threadIdx.x >> 6)] = v;
+}
+
 // Per-unit partial sums of r (kBRedSum), r.r or z.r on level 0.
 __global__ void __launch_bounds__(kBlock) kb_reduce(BUnits t, int mode, const float* __restrict__ r,
                                                     const float* __restrict__ z, const BMg* __restrict__ mg,
                                                     double* __restrict__ part) {
-    __shared__ double sh[kBlock / 64];
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
         if (!scene_active(mg, w.scene)) continue;
@@ -141,8 +148,7 @@ __global__ void __launch_bounds__(kBlock) kb_reduce(BUnits t, int mode, const fl
             const double rv = (double)r[g];
             v = mode == kBRedSum ? rv : (mode == kBRedRR ? rv * rv : (double)z[g] * rv);
         }
-        v = bblock_sum(v, sh);
-        if (threadIdx.x == 0) part[S.part_off + w.chunk] = v;
+        wave_partial(v, part, S.part_off + w.chunk);
     }
 }
 
@@ -154,7 +160,6 @@ __global__ void __launch_bounds__(kBlock) kb_ap_dot(BUnits t, BWalls wl, const f
                                                     const float* __restrict__ pin, float* __restrict__ pout,
                                                     float* __restrict__ Ap, const BMg* __restrict__ mg,
                                                     double* __restrict__ part) {
-    __shared__ double sh[kBlock / 64];
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
         if (!scene_active(mg, w.scene)) continue;
@@ -178,8 +183,7 @@ __global__ void __launch_bounds__(kBlock) kb_ap_dot(BUnits t, BWalls wl, const f
             Ap[g] = a;
             v = (double)pc * (double)a;
         }
-        v = bblock_sum(v, sh);
-        if (threadIdx.x == 0) part[S.part_off + w.chunk] = v;
+        wave_partial(v, part, S.part_off + w.chunk);
     }
 }
 
@@ -188,7 +192,6 @@ __global__ void __launch_bounds__(kBlock) kb_update_xr(BUnits t, float* __restri
                                                        const float* __restrict__ p, const float* __restrict__ Ap,
                                                        const BMg* __restrict__ mg, double* __restrict__ part,
                                                        double* __restrict__ part2) {
-    __shared__ double sh[kBlock / 64];
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
         if (!scene_active(mg, w.scene)) continue;
@@ -204,12 +207,8 @@ __global__ void __launch_bounds__(kBlock) kb_update_xr(BUnits t, float* __restri
             s2 = (double)rq * (double)rq;
             s1 = (double)rq;
         }
-        s2 = bblock_sum(s2, sh);
-        s1 = bblock_sum(s1, sh);
-        if (threadIdx.x == 0) {
-            part[S.part_off + w.chunk] = s2;
-            part2[S.part_off + w.chunk] = s1;
-        }
+        wave_partial(s2, part, S.part_off + w.chunk);
+        wave_partial(s1, part2, S.part_off + w.chunk);
     }
 }
 
@@ -241,11 +240,13 @@ __global__ void __launch_bounds__(kBlock) kb_finalize(const BScene* __restrict__
         // z.r from the V-cycle's last launch (level-0 tiles, or the coarse kernel)
         for (int b = threadIdx.x; b < S.zr_n; b += blockDim.x) v += zr[S.zr_off + b];
     } else {
-        for (int b = threadIdx.x; b < S.n_part; b += blockDim.x) v += part[S.part_off + b];
+        const int64_t o = (int64_t)S.part_off * kBatchWaves;
+        for (int b = threadIdx.x; b < S.n_part * kBatchWaves; b += blockDim.x) v += part[o + b];
     }
     v = block_sum(v, sh);
     if (mode == kBFinRTr) {
-        for (int b = threadIdx.x; b < S.n_part; b += blockDim.x) v1 += part2[S.part_off + b];
+        const int64_t o = (int64_t)S.part_off * kBatchWaves;
+        for (int b = threadIdx.x; b < S.n_part * kBatchWaves; b += blockDim.x) v1 += part2[o + b];
         v1 = block_sum(v1, sh);
     }
     if (threadIdx.x != 0) return;

@@ -25,6 +25,8 @@ namespace pfm {
 constexpr int kBatchMaxLevels = 12;
 // Levels with at most this many cells run inside one workgroup per scene.
 constexpr int kBatchCoarseCells = 4096;
+// waves of a kBlock workgroup: level-0 reduction partials per unit
+constexpr int kBatchWaves = kBlock / 64;
 
 struct BScene {
     GridDev g;                 // the scene's 2D grid
