@@ -20,7 +20,7 @@ namespace pfm {
 namespace {
 
 enum Kind {
-    kKRk4 = 0, kKG2p, kKSeed, kKFaceRk4, kKAdvect, kKFace, kKPrep, kKTile, kKBin, kKCell, kKDown, kKUp, kKCount
+    kKRk4 = 0, kKG2p, kKSeed, kKFaceRk4, kKAdvect, kKFace, kKTile, kKBin, kKCell, kKDown, kKUp, kKCount
 };
 constexpr int kCapCand[] = {0, 4096, 2048, 1024, 512};
 constexpr int kNCap = sizeof(kCapCand) / sizeof(kCapCand[0]);
@@ -355,30 +355,25 @@ int midpoint_pre(Batch* b) {
     });
 }
 
-// RK4_T_forward, the T / C / impulse updates, binning and P2G
-// (2D/run_2D.py:874-903).
+// RK4_T_forward with the T / C / impulse updates and the P2G record in its
+// epilogue, binning, P2G through the cell permutation (2D/run_2D.py:874-903).
 int particles_to_grid(Batch* b) {
     const pfm_batch2d_desc& d = b->d;
     hipStream_t s = b->bs;
     PFM_B_TRY(timed_launch(b, kKRk4, [&] {
-        return b_rk4(view(b, b->parts), b->P, b->F, d.pos, d.T_grad_m[0], d.T_grad_m[1], d.u[0], d.u[1], b->st,
-                     blocks_for(b, kKRk4, b->parts), s);
+        const float* Tin[2] = {d.T_init[0], d.T_init[1]};
+        const float* iC[2] = {d.init_C[0], d.init_C[1]};
+        return b_rk4_update(view(b, b->parts), b->P, d.pos, d.T_grad_m[0], d.T_grad_m[1], d.u[0], d.u[1], b->st, Tin,
+                            iC, d.init_imp, d.C, d.T, d.imp, b->rec, blocks_for(b, kKRk4, b->parts), s);
     }));
-    PFM_B_TRY(pfm_update_particles_grad_m(2, b->P, d.C[0], d.C[1], nullptr, d.init_C[0], d.init_C[1], nullptr,
-                                          d.T_grad_m[0], d.T_grad_m[1], nullptr, s));
-    PFM_B_TRY(pfm_update_T(2, b->P, d.T_grad_m[0], d.T_grad_m[1], nullptr, d.T_init[0], d.T_init[1], nullptr, d.T[0],
-                           d.T[1], nullptr, s));
-    PFM_B_TRY(pfm_update_particles_imp(2, b->P, d.T[0], d.T[1], nullptr, d.init_imp, d.imp, s));
     PFM_B_TRY(timed_launch(b, kKBin, [&] {
         return b_bin(view(b, b->parts), b->P, b->C, d.pos, d.cell_of, d.cell_start, d.perm, b->counts, b->scan_tiles,
                      b->slot, blocks_for(b, kKBin, b->parts), s);
     }));
-    const unsigned nb[3] = {blocks_for(b, kKPrep, b->parts), blocks_for(b, kKTile, b->tiles),
-                            blocks_for(b, kKFace, b->faces)};
+    const unsigned nb[2] = {blocks_for(b, kKTile, b->tiles), blocks_for(b, kKFace, b->faces)};
     return timed_launch(b, kKTile, [&] {
-        return b_p2g(view(b, b->parts), view(b, b->tiles), view(b, b->faces), b->P, b->F, d.pos, d.imp, d.C[0], d.C[1],
-                     d.cell_start, d.perm, b->rec, b->p2g_partial, d.u[0], d.u[1], d.p2g_weight[0], d.p2g_weight[1], nb,
-                     s);
+        return b_p2g_records(view(b, b->tiles), view(b, b->faces), d.cell_start, d.perm, b->rec, b->p2g_partial,
+                             d.u[0], d.u[1], d.p2g_weight[0], d.p2g_weight[1], nb, s);
     });
 }
 

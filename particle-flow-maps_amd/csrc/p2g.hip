@@ -284,18 +284,6 @@ __global__ void __launch_bounds__(kBlock) k_p2g_prep(GridDev g, int64_t n, PrepA
     prep_record<DIM, FUSED>(g, q, n, a, perm, pay);
 }
 
-// Batched: a scene's sorted slots are its own particle range (its particles
-// fill exactly its cells' part of the global sort).
-__global__ void __launch_bounds__(kBlock) kb_p2g_prep(BUnits t, int64_t P, PrepArgs a,
-                                                      const int32_t* __restrict__ perm, float* __restrict__ pay) {
-    PFM_FOR_UNITS(t, u) {
-        const BUnit w = t.u[u];
-        const BScene& S = t.sc[w.scene];
-        const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
-        if (loc >= S.n) continue;
-        prep_record<2, false>(S.g, S.p_off + loc, P, a, perm, pay);
-    }
-}
 
 // ---- tiled P2G --------------------------------------------------------------
 // A block owns a tile of cells (one thread per cell).  For each face axis A,
@@ -484,12 +472,13 @@ __global__ void __launch_bounds__(kBlock) k_p2g_tile(GridDev g, const int32_t* _
 // Batched: one unit per tile; records are in global sorted order, the
 // scene's cell_start entries start at its cell offset.
 __global__ void __launch_bounds__(kBlock) kb_p2g_tile(BUnits t, const int32_t* __restrict__ start,
-                                                      const float* __restrict__ pay, float* __restrict__ partial) {
+                                                      const float* __restrict__ pay, const int32_t* __restrict__ perm,
+                                                      float* __restrict__ partial) {
     __shared__ float lu[kTileLds], lw[kTileLds];
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
         const BScene& S = t.sc[w.scene];
-        p2g_tile_body<2>(S.g, start + S.c_off, pay, nullptr, w.chunk, S.nt[1], 1,
+        p2g_tile_body<2>(S.g, start + S.c_off, pay, perm, w.chunk, S.nt[1], 1,
                          partial + (S.t_off + w.chunk) * (int64_t)TileStride<2>::value, lu, lw);
     }
 }
@@ -859,23 +848,14 @@ int b_bin(BUnits parts, int64_t P, int64_t C, const float* pos, int32_t* cell_of
     return PFM_OK;
 }
 
-int b_p2g(BUnits parts, BUnits tiles, BUnits faces, int64_t P, const int64_t F[2], const float* pos,
-          const float* imp, const float* Cx, const float* Cy, const int32_t* cell_start, const int32_t* perm,
-          float* rec, float* partial, float* ux, float* uy, float* Wx, float* Wy, const unsigned blocks[3],
-          hipStream_t s) {
-    (void)F;
-    PrepArgs pa{};
-    pa.pos = pos;
-    pa.imp = imp;
-    pa.C[0] = Cx;
-    pa.C[1] = Cy;
-    hipLaunchKernelGGL(kb_p2g_prep, dim3(blocks[0]), dim3(kBlock), 0, s, parts, P, pa, perm, rec);
+int b_p2g_records(BUnits tiles, BUnits faces, const int32_t* cell_start, const int32_t* perm, const float* rec,
+                  float* partial, float* ux, float* uy, float* Wx, float* Wy, const unsigned blocks[2], hipStream_t s) {
+    hipLaunchKernelGGL(kb_p2g_tile, dim3(blocks[0]), dim3(kBlock), 0, s, tiles, cell_start, rec, perm, partial);
     PFM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(kb_p2g_tile, dim3(blocks[1]), dim3(kBlock), 0, s, tiles, cell_start, rec, partial);
-    PFM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(kb_p2g_combine, dim3(blocks[2]), dim3(kBlock), 0, s, faces, ux, uy, Wx, Wy, partial);
+    hipLaunchKernelGGL(kb_p2g_combine, dim3(blocks[1]), dim3(kBlock), 0, s, faces, ux, uy, Wx, Wy, partial);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
+
 
 }  // namespace pfm

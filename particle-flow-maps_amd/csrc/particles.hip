@@ -164,24 +164,6 @@ __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t 
     rk4_particle<DIM, REC, FOLD>(g, i, n, pos, Tx, Ty, Tz, ux, uy, uz, dt, advect_psi, ra);
 }
 
-// Batched 2D RK4_T_forward: one persistent launch over the particle units of
-// every scene, each scene's particles against its own faces and dt.
-__global__ void __launch_bounds__(kBlock, 3) kb_rk4(BUnits t, int64_t P, int64_t F0, int64_t F1,
-                                                 float* __restrict__ pos, float* __restrict__ Tx,
-                                                 float* __restrict__ Ty, const float* __restrict__ ux,
-                                                 const float* __restrict__ uy, const BState* __restrict__ st) {
-    PFM_FOR_UNITS(t, u) {
-        const BUnit w = t.u[u];
-        const BScene& S = t.sc[w.scene];
-        const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
-        if (loc >= S.n) continue;
-        const float dt = (float)st[w.scene].dt;
-        rk4_particle<2, false, false>(S.g, S.p_off + loc, P, pos, Tx, Ty, nullptr, ux + S.f_off[0],
-                                      uy + S.f_off[1], nullptr, dt, 1, RecArgs{});
-    }
-    (void)F0;
-    (void)F1;
-}
 
 // init_particles_imp / init_particles_imp_grad_m: 3D/init_conditions.py:254-273,
 // 2D/init_conditions.py:183-210 (interp_u_MAC_grad_imp, 3D/taichi_utils.py:326-376:
@@ -264,6 +246,15 @@ __global__ void __launch_bounds__(kBlock) k_update_T(int64_t n, const float* __r
         for (int r = 0; r < DIM; ++r) O[m][r * n + i] = t_product<DIM>(A, B, r, m);
 }
 
+// Row r of cols(T) @ m0 with Tc[c][r] = component r of T_c.
+template <int DIM>
+__device__ __forceinline__ float imp_row(const float Tc[3][3], const float m0[3], int r) {
+    float acc = Tc[0][r] * m0[0];
+#pragma unroll
+    for (int c = 1; c < DIM; ++c) acc += Tc[c][r] * m0[c];
+    return acc;
+}
+
 // update_particles_imp: imp = cols(T) @ init_imp (2D/run_2D.py:517-524).
 template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_update_imp(int64_t n, const float* __restrict__ tx,
@@ -274,16 +265,15 @@ __global__ void __launch_bounds__(kBlock) k_update_imp(int64_t n, const float* _
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* T[3] = {tx, ty, tz};
-    float m0[3];
+    float m0[3], Tc[3][3];
 #pragma unroll
     for (int c = 0; c < DIM; ++c) m0[c] = init_imp[c * n + i];
 #pragma unroll
-    for (int r = 0; r < DIM; ++r) {
-        float acc = T[0][r * n + i] * m0[0];
+    for (int c = 0; c < DIM; ++c)
 #pragma unroll
-        for (int c = 1; c < DIM; ++c) acc += T[c][r * n + i] * m0[c];
-        imp[r * n + i] = acc;
-    }
+        for (int r = 0; r < DIM; ++r) Tc[c][r] = T[c][r * n + i];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) imp[r * n + i] = imp_row<DIM>(Tc, m0, r);
 }
 
 // T_init_C_T = cols(T_gm) @ (rows(init_C) @ rows(T_gm)) (2D/run_2D.py:526-536,
@@ -431,6 +421,73 @@ __global__ void __launch_bounds__(kBlock) kb_seed(BUnits t, int ppc_axis, int or
     }
 }
 
+// Batched 2D RK4_T_forward followed, per particle, by the three updates of
+// the 2D step (2D/run_2D.py:874-879: update_particles_grad_m, update_T,
+// update_particles_imp, with the arithmetic of k_update_grad_m, k_update_T
+// and k_update_imp) and the particle's P2G record in particle order (the
+// record of k_p2g_prep<2, false>), so the P2G tile pass reads records through
+// the cell permutation and no separate update or record pass runs.
+struct Upd2Args {
+    const float* Tin[2];   // T_init columns
+    const float* iC[2];    // init_C rows
+    const float* init_imp;
+    float* C[2];           // out: C rows
+    float* T[2];           // out: T columns
+    float* imp;            // out
+    float* rec;            // out: P2G records, particle order
+};
+
+__global__ void __launch_bounds__(kBlock, 3) kb_rk4_update(BUnits t, int64_t P, float* __restrict__ pos,
+                                                        float* __restrict__ Tx, float* __restrict__ Ty,
+                                                        const float* __restrict__ ux, const float* __restrict__ uy,
+                                                        const BState* __restrict__ st, Upd2Args a) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        const BScene& S = t.sc[w.scene];
+        const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
+        if (loc >= S.n) continue;
+        const float dt = (float)st[w.scene].dt;
+        const int64_t i = S.p_off + loc;
+        rk4_particle<2, false, false>(S.g, i, P, pos, Tx, Ty, nullptr, ux + S.f_off[0], uy + S.f_off[1], nullptr,
+                                      dt, 1, RecArgs{});
+        const float* G[2] = {Tx, Ty};
+        float A[3][3] = {}, Au[3][3] = {}, B[3][3] = {}, Ci[3][3] = {}, Q[3][3] = {}, Tn[3][3] = {};
+        float p1[3] = {0.f, 0.f, 0.f}, m0[3] = {0.f, 0.f, 0.f}, mo[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            p1[r] = pos[r * P + i];
+            m0[r] = a.init_imp[r * P + i];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                A[r][c] = G[r][c * P + i];      // rows of T_grad_m: A[r][c] = T_r[c]
+                Ci[r][c] = a.iC[r][c * P + i];  // rows of init_C
+                B[r][c] = a.Tin[c][r * P + i];  // B[r][m] = T_init_m[r]
+            }
+        }
+        compose_C<2>(A, Ci, Q);  // C = cols(T_gm) (rows(init_C) rows(T_gm))
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int m = 0; m < 2; ++m) Au[r][m] = A[m][r];
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int r = 0; r < 2; ++r) Tn[m][r] = t_product<2>(Au, B, r, m);  // T = cols(T_gm) cols(T_init)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) mo[r] = imp_row<2>(Tn, m0, r);           // imp = cols(T) init_imp
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            a.imp[r * P + i] = mo[r];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                a.C[r][c * P + i] = Q[r][c];
+                a.T[r][c * P + i] = Tn[r][c];
+            }
+        }
+        write_p2g_record<2>(a.rec + i * Pay<2>::kRec, p1, S.g.inv_dx, false, A, B, Q, mo);
+    }
+}
+
 // ---- launchers of the batched kernels ----------------------------------------
 int b_seed(BUnits t, int ppc_axis, int order, int64_t P, float* pos, unsigned blocks, hipStream_t s) {
     hipLaunchKernelGGL(kb_seed, dim3(blocks), dim3(kBlock), 0, s, t, ppc_axis, order, P, pos);
@@ -449,12 +506,24 @@ int b_g2p(BUnits t, int64_t P, const int64_t F[2], const float* pos, const float
     return PFM_OK;
 }
 
-int b_rk4(BUnits t, int64_t P, const int64_t F[2], float* pos, float* Tx, float* Ty, const float* ux,
-          const float* uy, const BState* st, unsigned blocks, hipStream_t s) {
-    hipLaunchKernelGGL(kb_rk4, dim3(blocks), dim3(kBlock), 0, s, t, P, F[0], F[1], pos, Tx, Ty, ux, uy, st);
+int b_rk4_update(BUnits t, int64_t P, float* pos, float* Tx, float* Ty, const float* ux, const float* uy,
+                 const BState* st, const float* const* Tin, const float* const* iC, const float* init_imp, float* const* C,
+                 float* const* T, float* imp, float* rec, unsigned blocks, hipStream_t s) {
+    Upd2Args a;
+    for (int k = 0; k < 2; ++k) {
+        a.Tin[k] = Tin[k];
+        a.iC[k] = iC[k];
+        a.C[k] = C[k];
+        a.T[k] = T[k];
+    }
+    a.init_imp = init_imp;
+    a.imp = imp;
+    a.rec = rec;
+    hipLaunchKernelGGL(kb_rk4_update, dim3(blocks), dim3(kBlock), 0, s, t, P, pos, Tx, Ty, ux, uy, st, a);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
+
 
 }  // namespace pfm
 

@@ -87,8 +87,10 @@ struct BUnits {
 int b_seed(BUnits t, int ppc_axis, int order, int64_t P, float* pos, unsigned blocks, hipStream_t s);
 int b_g2p(BUnits t, int64_t P, const int64_t F[2], const float* pos, const float* ux, const float* uy,
           float* imp, float* Cx, float* Cy, unsigned blocks, hipStream_t s);
-int b_rk4(BUnits t, int64_t P, const int64_t F[2], float* pos, float* Tx, float* Ty, const float* ux,
-          const float* uy, const BState* st, unsigned blocks, hipStream_t s);
+
+int b_rk4_update(BUnits t, int64_t P, float* pos, float* Tx, float* Ty, const float* ux, const float* uy,
+                 const BState* st, const float* const* Tin, const float* const* iC, const float* init_imp, float* const* C,
+                 float* const* T, float* imp, float* rec, unsigned blocks, hipStream_t s);
 
 // grid.hip
 int b_max_speed(BUnits cells, const float* ux, const float* uy, BState* st, unsigned blocks, hipStream_t s);
@@ -105,10 +107,9 @@ int b_advect_u(BUnits faces, const int64_t F[2], const float* u0x, const float* 
 // p2g.hip
 int b_bin(BUnits parts, int64_t P, int64_t C, const float* pos, int32_t* cell_of, int32_t* cell_start,
           int32_t* perm, int32_t* counts, int32_t* tiles, int32_t* slot, unsigned blocks, hipStream_t s);
-int b_p2g(BUnits parts, BUnits tiles, BUnits faces, int64_t P, const int64_t F[2], const float* pos,
-          const float* imp, const float* Cx, const float* Cy, const int32_t* cell_start, const int32_t* perm,
-          float* rec, float* partial, float* ux, float* uy, float* Wx, float* Wy, const unsigned blocks[3],
-          hipStream_t s);
+// P2G from records in particle order (b_rk4_update), read through perm.
+int b_p2g_records(BUnits tiles, BUnits faces, const int32_t* cell_start, const int32_t* perm, const float* rec,
+                  float* partial, float* ux, float* uy, float* Wx, float* Wy, const unsigned blocks[2], hipStream_t s);
 size_t b_p2g_partial_floats(int64_t tiles);
 
 // batch_mg.hip: the batched projection (apply_bc, divergence, MGPCG,

@@ -22,8 +22,7 @@ from . import core
 from .simulation import StepConfig
 
 # kernel kinds of pfm_batch2d_tuning, in order (then the coarse kernel's block size)
-KINDS = ("rk4", "g2p", "seed", "rk4_grid", "advect_u", "faces", "p2g_prep", "p2g_tile", "bin", "cells", "mg_down",
-         "mg_up")
+KINDS = ("rk4", "g2p", "seed", "rk4_grid", "advect_u", "faces", "p2g_tile", "bin", "cells", "mg_down", "mg_up")
 
 
 class Batch2D:
