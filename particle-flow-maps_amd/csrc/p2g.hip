@@ -382,20 +382,36 @@ __device__ __forceinline__ void tile_axis(const GridDev& g, const float* __restr
                 dd[c][w] = crow[c] * (o * g.dx);
             }
         }
+        if (DIM == 3) {
+            // w (imp_A + C_A . dpos) with w = w0 w1 w2 and the affine term
+            // separable per axis: (imp_A + d0 + d1) w0 w1 times w2, plus
+            // w0 w1 times (w2 d2) -- three FMAs per window face
+            float x2[W2];
 #pragma unroll
-        for (int i = 0; i < W0; ++i)
+            for (int k = 0; k < W2; ++k) x2[k] = wt[2][k] * dd[2][k];
 #pragma unroll
-            for (int j = 0; j < W1; ++j) {
-                const float wij = wt[0][i] * wt[1][j];
-                const float dij = dd[0][i] + dd[1][j];
+            for (int i = 0; i < W0; ++i)
 #pragma unroll
-                for (int k = 0; k < W2; ++k) {
-                    const float w = DIM == 3 ? wij * wt[2][k] : wij;
-                    const float delta = DIM == 3 ? dij + dd[2][k] : dij;
-                    au[i][j][k] += (impA + delta) * w;
-                    aw[i][j][k] += w;
+                for (int j = 0; j < W1; ++j) {
+                    const float wij = wt[0][i] * wt[1][j];
+                    const float vij = wij * ((impA + dd[0][i]) + dd[1][j]);
+#pragma unroll
+                    for (int k = 0; k < W2; ++k) {
+                        au[i][j][k] = __builtin_fmaf(vij, wt[2][k], __builtin_fmaf(wij, x2[k], au[i][j][k]));
+                        aw[i][j][k] = __builtin_fmaf(wij, wt[2][k], aw[i][j][k]);
+                    }
                 }
-            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < W0; ++i)
+#pragma unroll
+                for (int j = 0; j < W1; ++j) {
+                    const float wij = wt[0][i] * wt[1][j];
+                    const float dij = dd[0][i] + dd[1][j];
+                    au[i][j][0] += (impA + dij) * wij;
+                    aw[i][j][0] += wij;
+                }
+        }
     }
     // zero the box, then the offset phases
     for (int t = threadIdx.x; t < BX::size; t += blockDim.x) {
