@@ -131,3 +131,29 @@ def test_batch_sizes_query():
     bad = np.ascontiguousarray(np.array([64, 3], np.int32))
     assert lib.pfm_batch2d_sizes(1, bad.ctypes.data, 2, out) == -1
     assert "out of" in _lib.last_error()
+
+
+def test_batch_create_rejects_bad_descriptors():
+    """pfm_batch2d_create validates the descriptor before touching the GPU."""
+    import numpy as np
+
+    from pfm import _lib
+
+    lib = _lib.load(require_gpu=False)
+    res = np.ascontiguousarray(np.array([64, 64, 80, 72], np.int32))
+    seeds = np.ascontiguousarray(np.array([1, 2], np.int64))
+    d = _lib.PfmBatch2dDesc()
+    d.n_scenes, d.res, d.seed = 2, res.ctypes.data, seeds.ctypes.data
+    d.ppc_axis, d.reinit_every, d.reinit_every_grad_m = 2, 12, 8
+    d.CFL, d.visualize_dt, d.base_level, d.max_iters, d.tol = 0.5, 0.1, 4, 400, 1e-6
+    h = ctypes.c_void_p()
+    nbytes = ctypes.c_int64()
+    # every device field is still NULL
+    assert lib.pfm_batch2d_create(ctypes.byref(d), ctypes.byref(h), ctypes.byref(nbytes)) == -1
+    assert "null field pointer" in _lib.last_error()
+    d.max_iters = -1
+    assert lib.pfm_batch2d_create(ctypes.byref(d), ctypes.byref(h), ctypes.byref(nbytes)) == -1
+    assert "max_iters" in _lib.last_error()
+    d.max_iters, d.reinit_every = 400, 0
+    assert lib.pfm_batch2d_create(ctypes.byref(d), ctypes.byref(h), ctypes.byref(nbytes)) == -1
+    assert not h.value
