@@ -384,6 +384,14 @@ def serving_bench(n_jobs, rates, slots, torch):
         served[f"rate_{r:g}_jobs_per_s"] = s
         print(f"# served at {r:g} jobs/s: {s['jobs_per_s']:.2f} jobs/s, p50 {s['step_ms_p50']:.2f} ms "
               f"p99 {s['step_ms_p99']:.2f} ms", file=sys.stderr, flush=True)
+    batched = {}
+    for r in rates[-1:]:
+        s = serving.Server(slots=slots, mg_precision=1, batch_2d=True).run(
+            serving.job_stream(n_jobs, r, seed=3), realtime=True).summary()
+        s["speedup_jobs_per_s_vs_one_at_a_time"] = s["jobs_per_s"] / base["jobs_per_s"]
+        batched[f"rate_{r:g}_jobs_per_s"] = s
+        print(f"# served (batched 2D lane) at {r:g} jobs/s: {s['jobs_per_s']:.2f} jobs/s, p50 "
+              f"{s['step_ms_p50']:.2f} ms p99 {s['step_ms_p99']:.2f} ms", file=sys.stderr, flush=True)
     sizes = [j.res for j in jobs]
     return {
         "workload": f"{n_jobs} jobs, Poisson arrivals, 70% 2D 128^2-1024^2 (4 ppc) / 30% 3D 64^3-192^3 (8 ppc), "
@@ -392,6 +400,7 @@ def serving_bench(n_jobs, rates, slots, torch):
         "total_steps": sum(j.steps for j in jobs),
         "one_at_a_time_fp32": base,
         "served_fp16": served,
+        "served_fp16_3d_batched_2d": batched,
     }
 
 

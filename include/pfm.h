@@ -463,6 +463,10 @@ int pfm_batch2d_tune(void* batch, pfm_batch2d_state* state, pfm_stream_t stream)
  * workgroup per unit), then the coarse kernel's block size; returns the
  * number of values written (at most n). */
 int pfm_batch2d_tuning(void* batch, int32_t* caps, int32_t n);
+/* Scene `scene` takes part in the following steps (1, the default) or is
+ * left exactly as it is (0: a finished served job; its units are skipped by
+ * every launch of the step except the particle binning).  Synchronises. */
+int pfm_batch2d_set_enabled(void* batch, int32_t scene, int32_t enabled);
 /* 0: issue every launch directly; 1 (default): capture and replay graphs. */
 int pfm_batch2d_set_graphs(void* batch, int32_t enable);
 int pfm_batch2d_destroy(void* batch);

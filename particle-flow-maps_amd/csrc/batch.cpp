@@ -83,10 +83,10 @@ struct Batch {
     std::vector<std::pair<int, size_t>> timed;  // (kind, first event index)
 };
 
-BUnits view(const Batch* b, const List& l) { return BUnits{b->sc, b->units + l.off, l.n, nullptr}; }
+BUnits view(const Batch* b, const List& l) { return BUnits{b->sc, b->units + l.off, l.n, nullptr, b->st}; }
 
 // Compacted list k (0: level-0 cells, 1 + l: tiles of level l).
-BUnits cview(const Batch* b, int k) { return BUnits{b->sc, b->cdst[k], b->clist[k].n, b->ccount + k}; }
+BUnits cview(const Batch* b, int k) { return BUnits{b->sc, b->cdst[k], b->clist[k].n, b->ccount + k, b->st}; }
 
 int compact(Batch* b) {
     std::vector<const BUnit*> src(b->n_clists);
@@ -181,7 +181,7 @@ int project_setup(Batch* b) {
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
         return bm_div_init(t, b->d.u[0], b->d.u[1], b->r, b->x, b->pp[0], b->Ap, nb, b->bs);
     }, false));
-    PFM_B_TRY(bm_mg_reset(b->S, b->mg, b->d.tol_mode, b->d.tol, b->d.max_iters, s));
+    PFM_B_TRY(bm_mg_reset(b->S, b->mg, b->st, b->d.tol_mode, b->d.tol, b->d.max_iters, s));
     PFM_B_TRY(compact(b));
     // all walls Neumann: recentre r (3D/mgpcg.py:215)
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
@@ -672,11 +672,18 @@ int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** handle, int64_t* byt
     if ((e = hipMemcpy(b->cbeg, hbeg.data(), sizeof(int32_t) * hbeg.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return fail_hip(e);
     if ((e = hipMemset(b->ccount, 0, sizeof(int32_t) * b->n_clists)) != hipSuccess) return fail_hip(e);
-    if ((e = hipMemset(b->st, 0, sizeof(BState) * b->S)) != hipSuccess) return fail_hip(e);
+    {
+        std::vector<BState> st0(b->S);
+        for (auto& x : st0) {
+            x = BState{};
+            x.enabled = 1;
+        }
+        if ((e = hipMemcpy(b->st, st0.data(), sizeof(BState) * b->S, hipMemcpyHostToDevice)) != hipSuccess)
+            return fail_hip(e);
+    }
     if ((e = hipMemset(b->mg, 0, sizeof(BMg) * b->S)) != hipSuccess) return fail_hip(e);
     if ((e = hipMemset(b->iters, 0, sizeof(int32_t) * 2 * b->S)) != hipSuccess) return fail_hip(e);
     if ((e = hipMemset(b->conv, 0, sizeof(int32_t) * 2 * b->S)) != hipSuccess) return fail_hip(e);
-    if ((e = hipDeviceSynchronize()) != hipSuccess) return fail_hip(e);
     if (bytes_out) *bytes_out = bytes;
     *handle = b;
     return PFM_OK;
@@ -781,7 +788,18 @@ int pfm_batch2d_tune(void* h, pfm_batch2d_state* st, pfm_stream_t stream) {
     b->coarse_threads = kCoarseThreadCand[pick[kKCount] % kNT];
     if (rc != PFM_OK) return rc;
     // fresh per-scene clocks and PCG scalars for the run that follows
-    PFM_HIP_TRY(hipMemsetAsync(b->st, 0, sizeof(BState) * b->S, b->bs));
+    {
+        std::vector<BState> st0(b->S);
+        PFM_HIP_TRY(hipMemcpyAsync(st0.data(), b->st, sizeof(BState) * b->S, hipMemcpyDeviceToHost, b->bs));
+        PFM_HIP_TRY(hipStreamSynchronize(b->bs));
+        for (auto& x : st0) {
+            const int32_t en = x.enabled;
+            x = BState{};
+            x.enabled = en;
+        }
+        PFM_HIP_TRY(hipMemcpyAsync(b->st, st0.data(), sizeof(BState) * b->S, hipMemcpyHostToDevice, b->bs));
+        PFM_HIP_TRY(hipStreamSynchronize(b->bs));
+    }
     PFM_HIP_TRY(hipMemsetAsync(b->mg, 0, sizeof(BMg) * b->S, b->bs));
     PFM_HIP_TRY(hipMemsetAsync(b->iters, 0, sizeof(int32_t) * 2 * b->S, b->bs));
     PFM_HIP_TRY(hipMemsetAsync(b->conv, 0, sizeof(int32_t) * 2 * b->S, b->bs));
@@ -795,6 +813,16 @@ int pfm_batch2d_tuning(void* h, int32_t* caps, int32_t n) {
     for (int k = 0; k < kKCount && w < n; ++k) caps[w++] = b->cap[k];
     if (w < n) caps[w++] = b->coarse_threads;
     return w;
+}
+
+int pfm_batch2d_set_enabled(void* h, int32_t scene, int32_t enabled) {
+    PFM_CHECK_ARG(h, "null batch");
+    Batch* b = (Batch*)h;
+    PFM_CHECK_ARG(scene >= 0 && scene < b->S, "scene %d out of [0, %d)", scene, b->S);
+    const int32_t v = enabled ? 1 : 0;
+    PFM_HIP_TRY(hipStreamSynchronize(b->bs));
+    PFM_HIP_TRY(hipMemcpy(&b->st[scene].enabled, &v, sizeof(int32_t), hipMemcpyHostToDevice));
+    return PFM_OK;
 }
 
 int pfm_batch2d_set_graphs(void* h, int32_t enable) {

@@ -51,6 +51,7 @@ __device__ __forceinline__ bool scene_active(const BMg* mg, int s) { return mg[s
 __global__ void __launch_bounds__(kBlock) kb_zero_walls(BUnits t, float* __restrict__ ux, float* __restrict__ uy) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         int64_t f = (int64_t)w.chunk * kBlock + threadIdx.x;
         const int n0 = S.g.res[0], n1 = S.g.res[1];
@@ -73,6 +74,7 @@ __global__ void __launch_bounds__(kBlock) kb_div_init(BUnits t, const float* __r
                                                       float* __restrict__ Ap) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
         if (q >= S.nc) continue;
@@ -98,6 +100,7 @@ __global__ void __launch_bounds__(kBlock) kb_subtract_grad_p(BUnits t, const flo
                                                              float* __restrict__ ux, float* __restrict__ uy) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         int64_t f = (int64_t)w.chunk * kBlock + threadIdx.x;
         const int n0 = S.g.res[0], n1 = S.g.res[1];
@@ -139,6 +142,7 @@ __global__ void __launch_bounds__(kBlock) kb_reduce(BUnits t, int mode, const fl
                                                     double* __restrict__ part) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         if (!scene_active(mg, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
@@ -162,6 +166,7 @@ __global__ void __launch_bounds__(kBlock) kb_ap_dot(BUnits t, BWalls wl, const f
                                                     double* __restrict__ part) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         if (!scene_active(mg, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
@@ -194,6 +199,7 @@ __global__ void __launch_bounds__(kBlock) kb_update_xr(BUnits t, float* __restri
                                                        double* __restrict__ part2) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         if (!scene_active(mg, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
@@ -216,6 +222,7 @@ __global__ void __launch_bounds__(kBlock) kb_update_xr(BUnits t, float* __restri
 __global__ void __launch_bounds__(kBlock) kb_sub_mean(BUnits t, float* __restrict__ r, const BMg* __restrict__ mg) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         if (!scene_active(mg, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
@@ -292,13 +299,14 @@ __global__ void __launch_bounds__(kBlock) kb_finalize(const BScene* __restrict__
 
 // Start of a solve: fresh scalars (the reference's beta persists across
 // solves), every scene active unless max_iters is 0.
-__global__ void kb_mg_reset(int n, BMg* __restrict__ mg, int tol_mode, double tol, int max_iters) {
+__global__ void kb_mg_reset(int n, BMg* __restrict__ mg, const BState* __restrict__ st, int tol_mode, double tol,
+                            int max_iters) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const float beta = mg[s].beta;
     BMg m{};
     m.beta = beta;
-    m.active = max_iters != 0;
+    m.active = max_iters != 0 && st[s].enabled;
     m.thresh = tol_mode == 1 ? 0.0 : tol;
     mg[s] = m;
 }
@@ -409,6 +417,7 @@ __global__ void __launch_bounds__(kBlock) kb_down(BUnits t, int l, BWalls wl, fl
     __shared__ float z[W * W], r[W * W];
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         if (!scene_active(mg, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int n0 = S.ld[l][0], n1 = S.ld[l][1];
@@ -470,6 +479,7 @@ __global__ void __launch_bounds__(kBlock) kb_up(BUnits t, int l, BWalls wl, cons
     __shared__ double sh[kBlock / 64];
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         if (!scene_active(mg, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int n0 = S.ld[l][0], n1 = S.ld[l][1];
@@ -658,8 +668,8 @@ int bm_finalize(int n_scenes, const BScene* sc, int mode, double eps, int max_it
     PFM_B_LAUNCH(kb_finalize, n_scenes, sc, mode, eps, max_iters, part, part2, zr, mg);
     return PFM_OK;
 }
-int bm_mg_reset(int n_scenes, BMg* mg, int tol_mode, double tol, int max_iters, hipStream_t s) {
-    hipLaunchKernelGGL(kb_mg_reset, dim3(grid_for(n_scenes)), dim3(kBlock), 0, s, n_scenes, mg, tol_mode, tol,
+int bm_mg_reset(int n_scenes, BMg* mg, const BState* st, int tol_mode, double tol, int max_iters, hipStream_t s) {
+    hipLaunchKernelGGL(kb_mg_reset, dim3(grid_for(n_scenes)), dim3(kBlock), 0, s, n_scenes, mg, st, tol_mode, tol,
                        max_iters);
     PFM_LAUNCH_CHECK();
     return PFM_OK;

@@ -221,6 +221,7 @@ __global__ void __launch_bounds__(kBlock) kb_max_speed(BUnits t, const float* __
                                                        const float* __restrict__ uy, BState* __restrict__ st) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
         float m = 0.0f;
@@ -242,7 +243,7 @@ __global__ void kb_ms_init(int n, BState* st) {
 // double as the host driver computes it.
 __global__ void kb_dt(int n, const BScene* __restrict__ sc, BState* __restrict__ st, double CFL, double vis_dt) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+    if (s >= n || !st[s].enabled) return;
     BState& b = st[s];
     const double dx = 1.0 / (double)sc[s].g.res[1];
     double curr = CFL * dx / (double)__uint_as_float(b.ms_bits);
@@ -261,6 +262,7 @@ __global__ void __launch_bounds__(kBlock) kb_reset_identity_grid(BUnits t, int64
                                                                  float* __restrict__ T_x, float* __restrict__ T_y) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         BFace bf;
         if (!face_of_unit(S, w.chunk, bf)) continue;
@@ -281,6 +283,7 @@ __global__ void __launch_bounds__(kBlock, 4) kb_rk4_grid(BUnits t, int64_t F0, i
                                                       const BState* __restrict__ st) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         BFace bf;
         if (!face_of_unit(S, w.chunk, bf)) continue;
@@ -301,6 +304,7 @@ __global__ void __launch_bounds__(kBlock, 4) kb_advect_u(BUnits t, int64_t F0, i
                                                       const float* __restrict__ psi_y) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         BFace bf;
         if (!face_of_unit(S, w.chunk, bf)) continue;

@@ -88,6 +88,8 @@ __global__ void __launch_bounds__(kBlock) k_bin_count(GridDev g, int64_t n,
 __global__ void __launch_bounds__(kBlock) kb_bin_count(BUnits t, int64_t P, const float* __restrict__ pos,
                                                        int32_t* __restrict__ cell_of, int32_t* __restrict__ counts,
                                                        int32_t* __restrict__ slot) {
+    // every scene's particles are binned, enabled or not: the counting sort
+    // places all particles of the batch
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
         const BScene& S = t.sc[w.scene];
@@ -493,6 +495,7 @@ __global__ void __launch_bounds__(kBlock) kb_p2g_tile(BUnits t, const int32_t* _
     __shared__ float lu[kTileLds], lw[kTileLds];
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         p2g_tile_body<2>(S.g, start + S.c_off, pay, perm, w.chunk, S.nt[1], 1,
                          partial + (S.t_off + w.chunk) * (int64_t)TileStride<2>::value, lu, lw);
@@ -581,6 +584,7 @@ __global__ void __launch_bounds__(kBlock) kb_p2g_combine(BUnits t, float* ux, fl
                                                          const float* __restrict__ partial) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         int64_t f = (int64_t)w.chunk * kBlock + threadIdx.x;
         int A = 0;

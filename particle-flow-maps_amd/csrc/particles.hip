@@ -211,6 +211,7 @@ __global__ void __launch_bounds__(kBlock, 2) kb_g2p(BUnits t, int64_t P, const f
                                                  float* __restrict__ Cy) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
         if (loc >= S.n) continue;
@@ -414,6 +415,7 @@ __global__ void __launch_bounds__(kBlock) kb_seed(BUnits t, int ppc_axis, int or
                                                   float* __restrict__ pos) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
         if (loc >= S.n) continue;
@@ -443,6 +445,7 @@ __global__ void __launch_bounds__(kBlock, 3) kb_rk4_update(BUnits t, int64_t P, 
                                                         const BState* __restrict__ st, Upd2Args a) {
     PFM_FOR_UNITS(t, u) {
         const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
         const BScene& S = t.sc[w.scene];
         const int64_t loc = (int64_t)w.chunk * kBlock + threadIdx.x;
         if (loc >= S.n) continue;

@@ -56,6 +56,8 @@ struct BState {
     double sub_t;     // time since the last frame
     int32_t frame;    // frame index
     uint32_t ms_bits; // max speed (float bits, exact atomic max)
+    int32_t enabled;  // 0: the scene is left as it is (a served job that has finished)
+    int32_t pad;
 };
 
 // Per-scene scalars of the PCG (the MGScalars of the single-scene solver).
@@ -73,7 +75,11 @@ struct BUnits {
     const BUnit* u;
     int32_t nu;
     const int32_t* nu_dev;
+    const BState* st;  // per-scene enable flags (nullptr: every scene)
 };
+
+// Whether the unit's scene is stepped (a disabled scene keeps its fields).
+__device__ __forceinline__ bool unit_on(const BUnits& t, int32_t scene) { return !t.st || t.st[scene].enabled; }
 
 // Persistent loop over the units of a launch: `UNIT` is the unit index.
 #define PFM_FOR_UNITS(T, UNIT)                                                              \
@@ -133,7 +139,7 @@ int bm_update_xr(BUnits cells, float* x, float* r, const float* p, const float* 
 int bm_sub_mean(BUnits cells, float* r, const BMg* mg, unsigned blocks, hipStream_t s);
 int bm_finalize(int n_scenes, const BScene* sc, int mode, double eps, int max_iters, const double* part,
                 const double* part2, const double* zr, BMg* mg, hipStream_t s);
-int bm_mg_reset(int n_scenes, BMg* mg, int tol_mode, double tol, int max_iters, hipStream_t s);
+int bm_mg_reset(int n_scenes, BMg* mg, const BState* st, int tol_mode, double tol, int max_iters, hipStream_t s);
 int bm_set_thresh(int n_scenes, BMg* mg, double tol, hipStream_t s);
 int bm_save_solve(int n_scenes, const BMg* mg, int k, int32_t* iters, int32_t* conv, hipStream_t s);
 int bm_count_active(int n_scenes, const BMg* mg, int32_t* out, hipStream_t s);

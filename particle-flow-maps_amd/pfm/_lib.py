@@ -150,6 +150,7 @@ _SIGS = {
     "pfm_batch2d_scene_info": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "pfm_batch2d_tune": [c_vp, ctypes.POINTER(PfmBatch2dState), c_vp],
     "pfm_batch2d_tuning": [c_vp, c_vp, ctypes.c_int32],
+    "pfm_batch2d_set_enabled": [c_vp, ctypes.c_int32, ctypes.c_int32],
     "pfm_batch2d_set_graphs": [c_vp, ctypes.c_int32],
     "pfm_batch2d_destroy": [c_vp],
 }

@@ -188,6 +188,10 @@ class Batch2D:
             out["coarse_threads"] = int(vals[len(KINDS)])
         return out
 
+    def set_enabled(self, s: int, enabled: bool):
+        """Step scene s (True) or leave its fields as they are (False)."""
+        L.check(self.lib.pfm_batch2d_set_enabled(self._h, int(s), int(bool(enabled))), "pfm_batch2d_set_enabled")
+
     def set_graphs(self, enable: bool):
         L.check(self.lib.pfm_batch2d_set_graphs(self._h, int(bool(enable))), "pfm_batch2d_set_graphs")
 

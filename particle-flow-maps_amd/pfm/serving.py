@@ -22,6 +22,15 @@ device memory until it finishes:
 * Multigrid hierarchies allocated by the library are released when the
   server is idle or stops (``hipFree`` synchronises the whole device, which
   would stall the other slots mid-step).
+* Batched 2D lane (``batch_2d=True``).  2D jobs are not given a slot of their
+  own: the admitted 2D jobs waiting for a lane form one batch
+  (``pfm.batch.Batch2D``) that advances all of them together, one persistent
+  launch per phase of the step, while other lanes and the 3D slots keep
+  running.  A job that has done its steps is disabled in its batch (its
+  fields stay as they are) and reported; the batch's memory counts against
+  the budget until the whole batch has finished.  A batch's result for a job
+  does not depend on the other jobs of the batch, so it is the same alone and
+  in the mix; it uses the fp32 V-cycle.
 
 Per-step latency is the host wall time from issuing a step to its completion
 on the job's stream; ``ServeReport`` gives p50/p99 over every step of every
@@ -157,14 +166,18 @@ class Admission:
     part of its footprint allocated until the server frees it (``retained``);
     that part still counts against the budget."""
 
-    def __init__(self, budget_bytes: int, slots: int, footprint=footprint_bytes):
+    def __init__(self, budget_bytes: int, slots: int, footprint=footprint_bytes, takes_slot=None):
         if slots < 1:
             raise ValueError("need at least one slot")
         self.budget = int(budget_bytes)
         self.slots = int(slots)
         self.footprint = footprint
+        # jobs for which takes_slot(job) is False (the batched 2D lane) are
+        # limited by memory only
+        self.takes_slot = takes_slot or (lambda job: True)
         self.used = 0
         self.retained = 0
+        self.slots_used = 0
         self.running: Dict[int, int] = {}
 
     def admit_from(self, pending: "collections.deque[Job]") -> Tuple[List[Job], List[Job]]:
@@ -173,8 +186,9 @@ class Admission:
         run and is rejected."""
         admitted, rejected = [], []
         for job in list(pending):
-            if len(self.running) >= self.slots:
-                break
+            slot = self.takes_slot(job)
+            if slot and self.slots_used >= self.slots:
+                continue
             fp = self.footprint(job)
             if fp > self.budget:
                 pending.remove(job)
@@ -183,18 +197,21 @@ class Admission:
                 pending.remove(job)
                 self.running[job.id] = fp
                 self.used += fp
+                self.slots_used += 1 if slot else 0
                 admitted.append(job)
         return admitted, rejected
 
     def blocked_by_retained(self, pending) -> bool:
         """Whether a pending job would be admitted if the retained bytes were freed."""
-        if self.retained == 0 or len(self.running) >= self.slots:
+        if self.retained == 0:
             return False
-        return any(self.used + self.footprint(j) <= self.budget for j in pending)
+        return any(self.used + self.footprint(j) <= self.budget and
+                   (not self.takes_slot(j) or self.slots_used < self.slots) for j in pending)
 
     def release(self, job: Job, retain: int = 0) -> None:
         fp = self.running.pop(job.id)
         self.used -= fp
+        self.slots_used -= 1 if self.takes_slot(job) else 0
         self.retained += min(int(retain), fp)
 
     def free_retained(self) -> None:
@@ -210,15 +227,13 @@ def build_scene(job: Job, mg_precision: int = 1, device="cuda") -> Simulation:
     3D: one or two vortex rings (add_vortex_ring, 3D/init_conditions.py:10-22)
     of major radius in [0.1, 0.2], core 0.02, unit circulation, centred in
     [0.35, 0.65]^2 x [0.3, 0.5], normal along z.  Both are projected."""
-    rng = np.random.default_rng(job.seed)
     sim = Simulation(job.res, step_config(job, mg_precision), device=device)
     if job.dim == 2:
-        nv = int(rng.integers(8, 33))
-        centers = rng.uniform(0.2, 0.8, size=(nv, 2))
-        weights = rng.normal(0.0, 1.0, size=nv)
-        sim.init_leapfrog_2d(centers, weights, rad=1.5 * sim.dx)
+        centers, weights, rad = point_vortex_ic(job)
+        sim.init_leapfrog_2d(centers, weights, rad=rad)
         sim.initialize_2d_particles()
     else:
+        rng = np.random.default_rng(job.seed)
         rings = []
         for _ in range(int(rng.integers(1, 3))):
             c = [float(rng.uniform(0.35, 0.65)), float(rng.uniform(0.35, 0.65)), float(rng.uniform(0.3, 0.5))]
@@ -247,6 +262,7 @@ class ServeReport:
     slots: int
     mg_precision: int
     wall_s: float
+    batch_2d: bool = False
 
     def summary(self) -> dict:
         ok = [r for r in self.results if r.error is None]
@@ -266,14 +282,33 @@ class ServeReport:
             if self.results else None,
             "slots": self.slots,
             "mg_precision": {0: "fp32", 1: "fp16", 2: "bf16"}[self.mg_precision],
+            "batch_2d": self.batch_2d,
         }
 
 
+def batch_step_config() -> StepConfig:
+    """The knobs of the batched 2D lane: those of a served 2D job (the seed is
+    per scene) with the fp32 V-cycle."""
+    return StepConfig(particles_per_cell=4, reinit_every=12, reinit_every_grad_m=12, CFL=0.5, tol=1e-6,
+                      tol_mode="rel", particle_order=1, write_imp_grad_m=False)
+
+
+def point_vortex_ic(job: Job):
+    """The seeded 2D initial condition of build_scene: centres, weights, core radius."""
+    rng = np.random.default_rng(job.seed)
+    nv = int(rng.integers(8, 33))
+    centers = rng.uniform(0.2, 0.8, size=(nv, 2))
+    weights = rng.normal(0.0, 1.0, size=nv)
+    return centers, weights, 1.5 / job.res[1]
+
+
 class Server:
-    """Runs job streams on one GPU with `slots` concurrently resident jobs."""
+    """Runs job streams on one GPU with `slots` concurrently resident jobs
+    (and, with batch_2d, `batch_lanes` batches of 2D jobs)."""
 
     def __init__(self, slots: int = 8, mem_budget: Optional[int] = None, mg_precision: int = 1, device="cuda",
-                 keep_fields: bool = False):
+                 keep_fields: bool = False, batch_2d: bool = False, batch_lanes: int = 2, max_batch: int = 64,
+                 batch_window_s: float = 0.02):
         L.load()  # fails loudly without the HIP library or a GPU
         self.device = torch.device(device)
         if mem_budget is None:
@@ -282,8 +317,60 @@ class Server:
         self.slots = int(slots)
         self.mg_precision = int(mg_precision)
         self.keep_fields = keep_fields
+        self.batch_2d = bool(batch_2d)
+        self.batch_lanes = int(batch_lanes)
+        self.max_batch = int(max_batch)
+        self.batch_window_s = float(batch_window_s)
+        # 2D jobs of the batched lane do not take a slot
         self.admission = Admission(mem_budget, self.slots,
-                                   footprint=lambda j: footprint_bytes(j, self.mg_precision))
+                                   footprint=lambda j: footprint_bytes(j, 0 if self._batched(j) else self.mg_precision),
+                                   takes_slot=lambda j: not self._batched(j))
+
+    def _batched(self, job: Job) -> bool:
+        return self.batch_2d and job.dim == 2
+
+    def _run_batch(self, group, t0):
+        """One batch of 2D jobs from initial condition to the last job's last
+        step; returns [(JobResult, ...)] and the batch (its memory is freed by
+        the caller when the device is idle)."""
+        from .batch import Batch2D
+
+        jobs = [j for j, _ in group]
+        b = Batch2D([j.res for j in jobs], batch_step_config(), seeds=[j.seed % (2**31) for j in jobs],
+                    device=self.device)
+        for k, j in enumerate(jobs):
+            c, w, rad = point_vortex_ic(j)
+            b.set_point_vortices(k, c, w, rad)
+        b.project()
+        b.init_particles()
+        stream = torch.cuda.current_stream(self.device)
+        stream.synchronize()
+        started = time.perf_counter() - t0
+        left = [j.steps for j in jobs]
+        step_ms = [[] for _ in jobs]
+        iters = [[] for _ in jobs]
+        out = [None] * len(jobs)
+        while any(n > 0 for n in left):
+            a = time.perf_counter()
+            b.step()
+            stream.synchronize()
+            ms = 1e3 * (time.perf_counter() - a)
+            info = b.scene_info()
+            ns = b.state.n_solves
+            for k, j in enumerate(jobs):
+                if left[k] <= 0:
+                    continue
+                step_ms[k].append(ms)
+                iters[k].extend(int(x) for x in info["iters"][k][:ns])
+                left[k] -= 1
+                if left[k] == 0:
+                    u = [b.u_view(k, a_) for a_ in range(2)]
+                    ke = float(core.kinetic_energy(j.res, u).item())
+                    fields = [x.clone() for x in u] if self.keep_fields else None
+                    b.set_enabled(k, False)
+                    out[k] = JobResult(j, group[k][1], started, time.perf_counter() - t0, step_ms[k], iters[k], ke,
+                                       fields)
+        return out, b
 
     def _run_job(self, job: Job, admitted: float, t0: float):
         started = time.perf_counter() - t0
@@ -326,7 +413,39 @@ class Server:
                         now = time.perf_counter() - t0
                         done.put((JobResult(job, admitted, now, now, [], [], float("nan"), None, repr(e)), None))
 
+        bwork: "queue.Queue" = queue.Queue()
+
+        def batch_worker():
+            s = torch.cuda.Stream(device=self.device)
+            with torch.cuda.stream(s):
+                while True:
+                    item = bwork.get()
+                    if item is None:
+                        return
+                    group = [item]
+                    # the 2D jobs admitted within the collection window join this batch
+                    deadline = time.perf_counter() + self.batch_window_s
+                    while len(group) < self.max_batch:
+                        try:
+                            nxt = bwork.get(timeout=max(0.0, deadline - time.perf_counter()))
+                        except queue.Empty:
+                            break
+                        if nxt is None:
+                            bwork.put(None)
+                            break
+                        group.append(nxt)
+                    try:
+                        out, b = self._run_batch(group, t0)
+                        for k, r in enumerate(out):
+                            done.put((r, b if k == len(out) - 1 else None))
+                    except Exception as e:  # reported per job; the server keeps going
+                        now = time.perf_counter() - t0
+                        for job, admitted in group:
+                            done.put((JobResult(job, admitted, now, now, [], [], float("nan"), None, repr(e)), None))
+
         threads = [threading.Thread(target=worker, daemon=True) for _ in range(self.slots)]
+        if self.batch_2d:
+            threads += [threading.Thread(target=batch_worker, daemon=True) for _ in range(self.batch_lanes)]
         for th in threads:
             th.start()
         arrivals = collections.deque(jobs)
@@ -343,7 +462,7 @@ class Server:
                     self.admission.free_retained()
                 admitted, rejected = self.admission.admit_from(pending)
                 for job in admitted:
-                    work.put((job, time.perf_counter() - t0))
+                    (bwork if self._batched(job) else work).put((job, time.perf_counter() - t0))
                 for job in rejected:
                     results.append(JobResult(job, now, now, now, [], [], float("nan"), None,
                                              "rejected: footprint exceeds the memory budget"))
@@ -354,8 +473,12 @@ class Server:
                     r, solver = done.get(timeout=timeout)
                 except queue.Empty:
                     continue
-                keep = mg_bytes(r.job.res, 3, self.mg_precision) if solver is not None else 0
-                self.admission.release(r.job, retain=keep)
+                if self._batched(r.job):
+                    # the batch's memory stays until the batch object is freed
+                    self.admission.release(r.job, retain=self.admission.running.get(r.job.id, 0))
+                else:
+                    keep = mg_bytes(r.job.res, 3, self.mg_precision) if solver is not None else 0
+                    self.admission.release(r.job, retain=keep)
                 if solver is not None:
                     retired.append(solver)
                 del solver
@@ -363,11 +486,13 @@ class Server:
                 if progress is not None:
                     progress(r)
         finally:
-            for _ in threads:
+            for _ in range(self.slots):
                 work.put(None)
+            for _ in range(self.batch_lanes if self.batch_2d else 0):
+                bwork.put(None)
             for th in threads:
                 th.join()
             retired.clear()
             self.admission.free_retained()
         return ServeReport(sorted(results, key=lambda r: r.job.id), self.slots, self.mg_precision,
-                           time.perf_counter() - t0)
+                           time.perf_counter() - t0, self.batch_2d)

@@ -77,3 +77,41 @@ def test_footprint_bounds_torch_allocations(job):
     fp = serving.footprint_bytes(job, 1) - serving.mg_bytes(job.res, 3, 1)
     assert 0 < used <= fp
     del sim
+
+
+def test_batched_lane_same_result_alone_and_in_the_mix():
+    """2D jobs served by the batched lane: each job's fields are the same when
+    it is served alone and together with the others (a batch's result for a
+    scene does not depend on the other scenes; finished scenes are frozen)."""
+    mixed = _serve(2, batch_2d=True)
+    assert mixed.summary()["batch_2d"] is True
+    for job in MIX:
+        alone = _serve(1, jobs=[job], batch_2d=True)
+        a = alone.results[0]
+        m = next(r for r in mixed.results if r.job == job)
+        assert a.poisson_iters == m.poisson_iters, job
+        for ua, um in zip(a.fields, m.fields):
+            assert torch.equal(ua, um), f"job {job.id} differs between alone and mixed"
+        assert a.kinetic_energy == m.kinetic_energy
+
+
+def test_batched_lane_matches_plain_simulation():
+    """The batched lane's result for a 2D job agrees with a single-scene
+    Simulation of the same job and knobs (only the PCG dot-product order
+    differs)."""
+    import dataclasses
+
+    from pfm.simulation import Simulation
+
+    job = MIX[2]
+    rep = _serve(1, jobs=[job, MIX[4]], batch_2d=True)
+    got = next(r for r in rep.results if r.job == job)
+    cfg = dataclasses.replace(serving.batch_step_config(), seed=job.seed % (2**31))
+    sim = Simulation(job.res, cfg)
+    c, w, rad = serving.point_vortex_ic(job)
+    sim.init_leapfrog_2d(c, w, rad=rad)
+    sim.initialize_2d_particles()
+    for _ in range(job.steps):
+        sim.step()
+    for ua, ub in zip(got.fields, sim.u):
+        assert float(torch.linalg.vector_norm(ua - ub) / torch.linalg.vector_norm(ub)) <= 1e-5
