@@ -110,11 +110,18 @@ def instrumented_step(sim, timer):
             core.advect_u(sim.res, sim.dx, sim.p2g_weight, sim.u, sim.T_grid, sim.psi_grid)
         with timer.region("poisson"):
             sim.project()
-    with timer.region("rk4_record"):
-        sim.binning.rk4_record(sim.dx, sim.pos, sim.T_grad_m, sim.u, dt, sim.T_init, sim.init_C, sim.init_imp,
-                               fold=fold)
-    with timer.region("binning"):
-        sim.binning.bin(sim.pos, sim.dx, force=True)
+    if fold:
+        # the binning's count pass runs in the RK4 kernel's epilogue
+        with timer.region("rk4_record"):
+            sim.binning.rk4_record_count(sim.dx, sim.pos, sim.u, dt, sim.T_init, sim.init_C, sim.init_imp)
+        with timer.region("binning"):
+            sim.binning.bin_finish(sim.pos, sim.dx)
+    else:
+        with timer.region("rk4_record"):
+            sim.binning.rk4_record(sim.dx, sim.pos, sim.T_grad_m, sim.u, dt, sim.T_init, sim.init_C, sim.init_imp,
+                                   fold=False)
+        with timer.region("binning"):
+            sim.binning.bin(sim.pos, sim.dx, force=True)
     with timer.region("p2g"):
         sim.binning.p2g_from_records(sim.dx, sim.n, sim.u, W=sim.p2g_weight)
     with timer.region("poisson"):
@@ -640,11 +647,15 @@ def main():
     del u_f, u_bc, u64, p64
 
     # roofline of the step's dominant kernel, RK4 + P2G record with the
-    # update_T fold (k_rk4_particles<3, true, true>): per particle it reads pos
-    # 12 B, T_init 36 B, init_C 36 B, init_imp 12 B and writes pos 12 B, the
-    # folded T_init 36 B and a 64 B record: 208 B/particle (T_grad_m enters as
-    # the identity).  Without the fold (k_rk4_particles<3, true>) T_grad_m is
-    # also read and written instead of T_init being written: 244 B/particle.
+    # update_T fold and the binning count pass (k_rk4_particles<3, true, true,
+    # true>): per particle it reads pos 12 B, T_init 36 B, init_C 36 B,
+    # init_imp 12 B and writes pos 12 B, the folded T_init 36 B, a 64 B record,
+    # its cell 4 B and its slot 4 B: 216 B/particle (T_grad_m enters as the
+    # identity); per cell the count is zero-filled (4 B) and read and written by
+    # the atomics (8 B).  The timed launch is the entry: the fill and the
+    # kernel.  Without the fold (k_rk4_particles<3, true>) T_grad_m is also
+    # read and written instead of T_init being written, and nothing is binned:
+    # 244 B/particle.
     fold = sim.folds_T
     ev_a = torch.cuda.Event(enable_timing=True)
     ev_b = torch.cuda.Event(enable_timing=True)
@@ -652,14 +663,18 @@ def main():
     # the measurement launches advance state; the step benchmark is done
     ev_a.record(torch.cuda.current_stream())
     for _ in range(nrep):
-        sim.binning.rk4_record(sim.dx, sim.pos, sim.T_grad_m, sim.u, 1e-4, sim.T_init, sim.init_C, sim.init_imp,
-                               fold=fold)
+        if fold:
+            sim.binning.rk4_record_count(sim.dx, sim.pos, sim.u, 1e-4, sim.T_init, sim.init_C, sim.init_imp)
+        else:
+            sim.binning.rk4_record(sim.dx, sim.pos, sim.T_grad_m, sim.u, 1e-4, sim.T_init, sim.init_C,
+                                   sim.init_imp, fold=False)
     ev_b.record(torch.cuda.current_stream())
     torch.cuda.synchronize()
     rk4_ms = ev_a.elapsed_time(ev_b) / nrep
-    alg_bytes = (208.0 if fold else 244.0) * sim.n
+    alg_bytes = 216.0 * sim.n + 12.0 * cells if fold else 244.0 * sim.n
     achieved = alg_bytes / (rk4_ms * 1e-3) / 1e9
-    roof_kernel = "pfm::k_rk4_particles<3, true, true>" if fold else "pfm::k_rk4_particles<3, true, false>"
+    roof_kernel = ("pfm::k_rk4_particles<3, true, true, true>" if fold
+                   else "pfm::k_rk4_particles<3, true, false, false>")
     traffic = committed_traffic(roof_kernel) if args.res == 256 else None
     # the kernel is VALU-issue bound: its instruction rate against the peak
     vpw = committed_valu(roof_kernel) if args.res == 256 else None

@@ -174,6 +174,22 @@ int pfm_rk4_particles_p2g_record_fold(const pfm_grid* g, int64_t n, float* pos, 
                                       float* Tin_y, float* Tin_z, const float* iC_x, const float* iC_y,
                                       const float* iC_z, const float* init_imp, float* records,
                                       pfm_stream_t stream);
+/* pfm_rk4_particles_p2g_record_fold with the count pass of the binning of
+ * the advected positions (the cell of each particle and its slot from the
+ * per-cell count atomics, get_particles_id_in_every_cell 3D/run_3D.py:342-351)
+ * done in the RK4 kernel's epilogue; the records, counts and slots go to the
+ * workspace, the cells to cell_of (NULL: the workspace).  pfm_bin_finish then
+ * gives the cell_start and perm of pfm_bin_particles on the new positions. */
+int pfm_rk4_particles_p2g_record_fold_count(const pfm_grid* g, int64_t n, float* pos, const float* u_x,
+                                            const float* u_y, const float* u_z, float dt, float* Tin_x,
+                                            float* Tin_y, float* Tin_z, const float* iC_x, const float* iC_y,
+                                            const float* iC_z, const float* init_imp, int32_t* cell_of,
+                                            void* workspace, size_t ws_bytes, pfm_stream_t stream);
+/* The rest of pfm_bin_particles after a fused count pass: exclusive scan of
+ * the counts into cell_start (C + 1), scatter by (cell, slot) into perm and the
+ * per-cell sort to ascending ids.  cell_of as given to the count pass. */
+int pfm_bin_finish(const pfm_grid* g, int64_t n, const int32_t* cell_of, int32_t* cell_start, int32_t* perm,
+                   void* workspace, size_t ws_bytes, pfm_stream_t stream);
 int pfm_p2g_records(const pfm_grid* g, int64_t n, const int32_t* cell_start, const int32_t* perm,
                     float* u_x, float* u_y, float* u_z, float* W_x, float* W_y, float* W_z,
                     void* workspace, size_t ws_bytes, pfm_stream_t stream);

@@ -18,6 +18,11 @@
 
 namespace pfm {
 
+// particles.hip
+int launch_rk4_fold_bin(const GridDev& g, int64_t n, float* pos, const float* ux, const float* uy, const float* uz,
+                        float dt, float* const Tin[3], const float* const iC[3], const float* init_imp, float* rec,
+                        int32_t* cell_of, int32_t* counts, int32_t* slot, hipStream_t s);
+
 constexpr int kScanItems = 16;                       // per thread
 constexpr int kScanTile = kBlock * kScanItems;       // per block
 
@@ -31,41 +36,6 @@ __global__ void __launch_bounds__(kBlock) k_fill_i32(int32_t* p, int64_t n, int3
 // slot from the count atomic (reordered later).  Runs of equal cells among
 // consecutive lanes -- the common case, particles are numbered cell by cell
 // and move < 1 cell per step -- are merged into one atomic per run.
-// cell = int(pos / dx) per axis, clamped to the grid (particle i, component stride n).
-template <int DIM>
-__device__ __forceinline__ int32_t particle_cell(const GridDev& g, const float* __restrict__ pos, int64_t i,
-                                                 int64_t n) {
-    int ci[3] = {0, 0, 0};
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-        int v = (int)(pos[c * n + i] * g.inv_dx);
-        ci[c] = max(0, min(v, g.res[c] - 1));
-    }
-    return (int32_t)(((int64_t)ci[0] * g.res[1] + ci[1]) * g.res[2] + ci[2]);
-}
-
-// The count atomics of one wave: each run of equal cells among consecutive
-// valid lanes takes one atomic; every valid lane gets its slot in the cell.
-__device__ __forceinline__ void count_slots(int32_t cell, bool valid, int32_t* __restrict__ counts,
-                                            int32_t* __restrict__ slot_i) {
-    const int lane = threadIdx.x & 63;
-    const int32_t prev = __shfl_up(cell, 1, 64);
-    const bool head = valid && (lane == 0 || prev != cell);
-    const uint64_t heads = __ballot(head);
-    // run start: highest head at or below this lane; run end: next head
-    const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
-    const int first = 63 - __clzll(upto | 1ull);
-    const uint64_t above = lane == 63 ? 0ull : (heads & ~((2ull << lane) - 1));
-    const uint64_t validm = __ballot(valid);
-    int next = above ? __ffsll((long long)above) - 1 : 64;
-    const int last_valid = 63 - __clzll(validm | 1ull);
-    if (next > last_valid + 1) next = last_valid + 1;
-    int32_t base = 0;
-    if (head) base = atomicAdd(&counts[cell], next - lane);
-    base = __shfl(base, first, 64);
-    if (valid) *slot_i = base + (lane - first);
-}
-
 template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_bin_count(GridDev g, int64_t n,
                                                       const float* __restrict__ pos,
@@ -659,6 +629,29 @@ static int tile_and_combine(const GridDev& g, const int32_t* cell_start, const i
                             const Workspace& w, float* u_x, float* u_y, float* u_z, float* W_x, float* W_y,
                             float* W_z, hipStream_t s);
 
+// Exclusive scan of the per-cell counts into cell_start, scatter of the
+// particles by (cell, slot), and the per-cell sort that restores ascending ids.
+static int finish_binning(const GridDev& g, int64_t n, const Workspace& w, const int32_t* cell_of,
+                          int32_t* cell_start, int32_t* perm, hipStream_t s) {
+    const int64_t C = cell_count(g);
+    const int64_t ntiles = (C + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ntiles), dim3(kBlock), 0, s, w.counts, C, w.tiles);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, w.tiles, ntiles);
+    PFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(kBlock), 0, s, w.counts, C, w.tiles,
+                       cell_start);
+    PFM_LAUNCH_CHECK();
+    if (n > 0) {
+        hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, n, cell_of, w.slot,
+                           cell_start, perm);
+        PFM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(C)), dim3(kBlock), 0, s, C, cell_start, perm);
+        PFM_LAUNCH_CHECK();
+    }
+    return PFM_OK;
+}
+
 }  // namespace pfm
 
 using namespace pfm;
@@ -701,21 +694,50 @@ int pfm_bin_particles(const pfm_grid* gp, int64_t n, const float* pos, int32_t* 
                                w.counts, w.slot);
         PFM_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ntiles), dim3(kBlock), 0, s, w.counts, C, w.tiles);
+    return finish_binning(g, n, w, cell_of, cell_start, perm, s);
+}
+
+int pfm_rk4_particles_p2g_record_fold_count(const pfm_grid* gp, int64_t n, float* pos, const float* u_x,
+                                            const float* u_y, const float* u_z, float dt, float* Tin_x, float* Tin_y,
+                                            float* Tin_z, const float* iC_x, const float* iC_y, const float* iC_z,
+                                            const float* init_imp, int32_t* cell_of, void* workspace, size_t ws_bytes,
+                                            pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(g.dim == 3, "the folded RK4 + record path is 3D");
+    PFM_CHECK_ARG(n >= 1 && n < (int64_t)INT32_MAX, "n out of range for int32 ids");
+    PFM_CHECK_ARG(pos && u_x && u_y && u_z && Tin_x && Tin_y && Tin_z && iC_x && iC_y && iC_z && init_imp &&
+                      workspace,
+                  "null pointer");
+    const size_t need = ws_layout(g, n, nullptr, nullptr);
+    PFM_CHECK_ARG(ws_bytes >= need, "workspace too small: %zu < %zu bytes", ws_bytes, need);
+    Workspace w;
+    ws_layout(g, n, (char*)workspace, &w);
+    if (!cell_of) cell_of = w.cell_tmp;
+    hipStream_t s = as_stream(stream);
+    const int64_t C = cell_count(g);
+    unsigned fb = grid_for(C);
+    if (fb > 4096) fb = 4096;
+    hipLaunchKernelGGL(k_fill_i32, dim3(fb), dim3(kBlock), 0, s, w.counts, C, 0);
     PFM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, w.tiles, ntiles);
-    PFM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(kBlock), 0, s, w.counts, C, w.tiles,
-                       cell_start);
-    PFM_LAUNCH_CHECK();
-    if (n > 0) {
-        hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, n, cell_of, w.slot,
-                           cell_start, perm);
-        PFM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(C)), dim3(kBlock), 0, s, C, cell_start, perm);
-        PFM_LAUNCH_CHECK();
-    }
-    return PFM_OK;
+    float* const Tin[3] = {Tin_x, Tin_y, Tin_z};
+    const float* const iC[3] = {iC_x, iC_y, iC_z};
+    return launch_rk4_fold_bin(g, n, pos, u_x, u_y, u_z, dt, Tin, iC, init_imp, w.pay, cell_of, w.counts, w.slot, s);
+}
+
+int pfm_bin_finish(const pfm_grid* gp, int64_t n, const int32_t* cell_of, int32_t* cell_start, int32_t* perm,
+                   void* workspace, size_t ws_bytes, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(n >= 0 && n < (int64_t)INT32_MAX, "n out of range for int32 ids");
+    PFM_CHECK_ARG(cell_start && perm && workspace, "null pointer");
+    const size_t need = ws_layout(g, n, nullptr, nullptr);
+    PFM_CHECK_ARG(ws_bytes >= need, "workspace too small: %zu < %zu bytes", ws_bytes, need);
+    Workspace w;
+    ws_layout(g, n, (char*)workspace, &w);
+    return finish_binning(g, n, w, cell_of ? cell_of : w.cell_tmp, cell_start, perm, as_stream(stream));
 }
 
 int pfm_cell_lists(const pfm_grid* gp, const int32_t* cell_start, const int32_t* perm,

@@ -17,6 +17,10 @@ struct RecArgs {
     const float* imp;
     float* rec;
     float* Tfold[3];  // FOLD: where T_init <- cols(T_grad_m) cols(T_init) goes (in place)
+    // BIN: the count pass of the particle binning of the advected positions
+    int32_t* cell_of;
+    int32_t* counts;
+    int32_t* slot;
 };
 
 // update_T's product (3D/run_3D.py:430-439): O[m][r] = sum_k A[r][k] B[k][m]
@@ -37,7 +41,7 @@ __device__ __forceinline__ float t_product(const float A[3][3], const float B[3]
 // short reinitialisation would apply (3D/run_3D.py:767-771), leaving
 // T_grad_m in memory as the identity that reset would write.
 // One particle: i indexes the SoA arrays, whose component stride is n.
-template <int DIM, bool REC, bool FOLD>
+template <int DIM, bool REC, bool FOLD, bool BIN = false>
 __device__ __forceinline__ void rk4_particle(const GridDev& g, int64_t i, int64_t n, float* __restrict__ pos,
                                              float* __restrict__ Tx, float* __restrict__ Ty,
                                              float* __restrict__ Tz, const float* __restrict__ ux,
@@ -147,9 +151,17 @@ __device__ __forceinline__ void rk4_particle(const GridDev& g, int64_t i, int64_
                 for (int r = 0; r < DIM; ++r) ra.Tfold[m][r * n + i] = t_product<DIM>(Au, B, r, m);
         }
     }
+    if (BIN) {
+        // get_particles_id_in_every_cell on the advected position (the count
+        // pass of pfm_bin_particles: cell, then a slot from the run-merged
+        // count atomics; inactive lanes of the last wave are its top lanes)
+        const int32_t cell = particle_cell<DIM>(g, pos, i, n);
+        ra.cell_of[i] = cell;
+        count_slots(cell, true, ra.counts, ra.slot + i);
+    }
 }
 
-template <int DIM, bool REC = false, bool FOLD = false>
+template <int DIM, bool REC = false, bool FOLD = false, bool BIN = false>
 __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t n,
                                                           float* __restrict__ pos,
                                                           float* __restrict__ Tx,
@@ -161,7 +173,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t 
                                                           int advect_psi, RecArgs ra = RecArgs{}) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    rk4_particle<DIM, REC, FOLD>(g, i, n, pos, Tx, Ty, Tz, ux, uy, uz, dt, advect_psi, ra);
+    rk4_particle<DIM, REC, FOLD, BIN>(g, i, n, pos, Tx, Ty, Tz, ux, uy, uz, dt, advect_psi, ra);
 }
 
 
@@ -489,6 +501,28 @@ __global__ void __launch_bounds__(kBlock, 3) kb_rk4_update(BUnits t, int64_t P, 
         }
         write_p2g_record<2>(a.rec + i * Pay<2>::kRec, p1, S.g.inv_dx, false, A, B, Q, mo);
     }
+}
+
+// The folded RK4 + P2G record kernel with the count pass of the binning of the
+// advected positions in its epilogue (3D).
+int launch_rk4_fold_bin(const GridDev& g, int64_t n, float* pos, const float* ux, const float* uy, const float* uz,
+                        float dt, float* const Tin[3], const float* const iC[3], const float* init_imp, float* rec,
+                        int32_t* cell_of, int32_t* counts, int32_t* slot, hipStream_t s) {
+    RecArgs ra{};
+    for (int c = 0; c < 3; ++c) {
+        ra.Tin[c] = Tin[c];
+        ra.Tfold[c] = Tin[c];
+        ra.C[c] = iC[c];
+    }
+    ra.imp = init_imp;
+    ra.rec = rec;
+    ra.cell_of = cell_of;
+    ra.counts = counts;
+    ra.slot = slot;
+    hipLaunchKernelGGL((k_rk4_particles<3, true, true, true>), dim3(grid_for(n)), dim3(kBlock), 0, s, g, n, pos,
+                       nullptr, nullptr, nullptr, ux, uy, uz, dt, 1, ra);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
 }
 
 // ---- launchers of the batched kernels ----------------------------------------

@@ -298,6 +298,42 @@ __device__ __forceinline__ void write_p2g_record(float* __restrict__ dst, const 
         d4[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
 }
 
+// ---- binning (get_particles_id_in_every_cell, 3D/run_3D.py:342-351) --------
+// cell = int(pos / dx) per axis, clamped to the grid (particle i, component stride n).
+template <int DIM>
+__device__ __forceinline__ int32_t particle_cell(const GridDev& g, const float* __restrict__ pos, int64_t i,
+                                                 int64_t n) {
+    int ci[3] = {0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        int v = (int)(pos[c * n + i] * g.inv_dx);
+        ci[c] = max(0, min(v, g.res[c] - 1));
+    }
+    return (int32_t)(((int64_t)ci[0] * g.res[1] + ci[1]) * g.res[2] + ci[2]);
+}
+
+// The count atomics of one wave: each run of equal cells among consecutive
+// valid lanes takes one atomic; every valid lane gets its slot in the cell.
+__device__ __forceinline__ void count_slots(int32_t cell, bool valid, int32_t* __restrict__ counts,
+                                            int32_t* __restrict__ slot_i) {
+    const int lane = threadIdx.x & 63;
+    const int32_t prev = __shfl_up(cell, 1, 64);
+    const bool head = valid && (lane == 0 || prev != cell);
+    const uint64_t heads = __ballot(head);
+    // run start: highest head at or below this lane; run end: next head
+    const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+    const int first = 63 - __clzll(upto | 1ull);
+    const uint64_t above = lane == 63 ? 0ull : (heads & ~((2ull << lane) - 1));
+    const uint64_t validm = __ballot(valid);
+    int next = above ? __ffsll((long long)above) - 1 : 64;
+    const int last_valid = 63 - __clzll(validm | 1ull);
+    if (next > last_valid + 1) next = last_valid + 1;
+    int32_t base = 0;
+    if (head) base = atomicAdd(&counts[cell], next - lane);
+    base = __shfl(base, first, 64);
+    if (valid) *slot_i = base + (lane - first);
+}
+
 // ---- reductions ----------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

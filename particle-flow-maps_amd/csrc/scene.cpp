@@ -134,16 +134,21 @@ int step3(Scene* sc, pfm_scene_state* st, double dt, hipStream_t s) {
         // (3D/run_3D.py:795-812)
         float* rec = (float*)pfm_p2g_record_buffer(g, d.n, d.workspace);
         PFM_CHECK_ARG(rec != nullptr, "pfm_p2g_record_buffer failed");
-        if (d.fold_T)
-            PFM_STEP_TRY(pfm_rk4_particles_p2g_record_fold(g, d.n, d.pos, d.u[0], d.u[1], d.u[2], (float)dt,
-                                                           d.T_init[0], d.T_init[1], d.T_init[2], d.init_C[0],
-                                                           d.init_C[1], d.init_C[2], d.init_imp, rec, s));
-        else
+        if (d.fold_T) {
+            // the binning's count pass rides in the RK4 kernel
+            PFM_STEP_TRY(pfm_rk4_particles_p2g_record_fold_count(g, d.n, d.pos, d.u[0], d.u[1], d.u[2], (float)dt,
+                                                                 d.T_init[0], d.T_init[1], d.T_init[2], d.init_C[0],
+                                                                 d.init_C[1], d.init_C[2], d.init_imp, d.cell_of,
+                                                                 d.workspace, d.ws_bytes, s));
+            PFM_STEP_TRY(pfm_bin_finish(g, d.n, d.cell_of, d.cell_start, d.perm, d.workspace, d.ws_bytes, s));
+        } else {
             PFM_STEP_TRY(pfm_rk4_particles_p2g_record(g, d.n, d.pos, d.T_grad_m[0], d.T_grad_m[1], d.T_grad_m[2],
                                                       d.u[0], d.u[1], d.u[2], (float)dt, d.T_init[0], d.T_init[1],
                                                       d.T_init[2], d.init_C[0], d.init_C[1], d.init_C[2],
                                                       d.init_imp, rec, s));
-        PFM_STEP_TRY(pfm_bin_particles(g, d.n, d.pos, d.cell_of, d.cell_start, d.perm, d.workspace, d.ws_bytes, s));
+            PFM_STEP_TRY(pfm_bin_particles(g, d.n, d.pos, d.cell_of, d.cell_start, d.perm, d.workspace, d.ws_bytes,
+                                           s));
+        }
         PFM_STEP_TRY(pfm_p2g_records(g, d.n, d.cell_start, d.perm, d.u[0], d.u[1], d.u[2], d.p2g_weight[0],
                                      d.p2g_weight[1], d.p2g_weight[2], d.workspace, d.ws_bytes, s));
     } else {

@@ -109,6 +109,9 @@ _SIGS = {
                                     + [c_vp],
     "pfm_p2g_records": [c_vp, c_i64, c_vp, c_vp] + [c_vp] * 6 + [c_vp, c_sz, c_vp],
     "pfm_rk4_particles_p2g_record_fold": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_f] + [c_vp] * 7 + [c_vp, c_vp],
+    "pfm_rk4_particles_p2g_record_fold_count": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_f] + [c_vp] * 7
+                                               + [c_vp, c_vp, c_sz, c_vp],
+    "pfm_bin_finish": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp],
     "pfm_p2g_record_buffer": [c_vp, c_i64, c_vp],
     "pfm_max_speed": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "pfm_split_central": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],

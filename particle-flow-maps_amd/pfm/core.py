@@ -267,6 +267,25 @@ class Binning:
                    rec, _stream(stream))
         _mark_written(pos)
 
+    def rk4_record_count(self, dx, pos, u_vel, dt, Tin, init_C, init_imp, stream=None):
+        """rk4_record(fold=True) with the count pass of the binning of the
+        advected positions in the same kernel (finish with bin_finish)."""
+        d, n = pos.shape
+        g = L.make_grid(self.res, dx)
+        L.call("pfm_rk4_particles_p2g_record_fold_count", ctypes.byref(g), n, _pvec(pos, d, n, "pos"),
+               *_u_ptrs(u_vel, self.res), float(dt), *_cols(Tin, d, n, "T_init"), *_cols(init_C, d, n, "init_C"),
+               _pvec(init_imp, d, n, "init_imp"), L.ptr(self.cell_of, dtype=torch.int32), self.workspace.data_ptr(),
+               self.workspace.numel(), _stream(stream))
+        _mark_written(pos)
+
+    def bin_finish(self, pos, dx, stream=None):
+        """The rest of bin() after rk4_record_count."""
+        g = L.make_grid(self.res, dx)
+        L.call("pfm_bin_finish", ctypes.byref(g), pos.shape[1], L.ptr(self.cell_of, dtype=torch.int32),
+               L.ptr(self.cell_start, dtype=torch.int32), L.ptr(self.perm, dtype=torch.int32),
+               self.workspace.data_ptr(), self.workspace.numel(), _stream(stream))
+        self._key = self.key(pos)
+
     def p2g_from_records(self, dx, n, u_out, W=None, stream=None):
         """P2G (3D/run_3D.py:812, 3D/taichi_utils.py P2G) from the records of
         rk4_record, using the current binning of the advected positions."""
@@ -282,8 +301,13 @@ class Binning:
         composition fused into the RK4 kernel: advect pos/T_grad_m in place,
         bin the new positions, then scatter into u_out (may alias u_vel only
         after the RK4 reads are done -- the launches are ordered on one stream)."""
-        self.rk4_record(dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, stream, fold=fold)
-        self.bin(pos, dx, stream, force=True)
+        if fold:
+            # the binning's count pass rides in the RK4 kernel
+            self.rk4_record_count(dx, pos, u_vel, dt, Tin, init_C, init_imp, stream)
+            self.bin_finish(pos, dx, stream)
+        else:
+            self.rk4_record(dx, pos, Tgm, u_vel, dt, Tin, init_C, init_imp, stream, fold=fold)
+            self.bin(pos, dx, stream, force=True)
         self.p2g_from_records(dx, pos.shape[1], u_out, W, stream)
 
 
