@@ -341,6 +341,41 @@ def test_fused_rk4_p2g_matches_separate(oracle, pfm):
         assert rel_l2(N(u2[a]), N(u1[a])) < 1e-6
 
 
+@pytest.mark.parametrize("res,n_drop", [((16, 16, 16), 0), ((20, 17, 23), 37)])
+def test_fused_count_pass_matches_record_then_bin(oracle, pfm, res, n_drop):
+    """pfm_rk4_particles_p2g_record_fold_count + pfm_bin_finish give the same
+    positions, folded T_init, cell_start, perm and P2G result as
+    pfm_rk4_particles_p2g_record_fold followed by pfm_bin_particles, bit for
+    bit (a ragged particle count included)."""
+    from pfm import core
+
+    pos, dx, n = _particles(oracle, res, seed=23, order=1)
+    n -= n_drop
+    pos = np.ascontiguousarray(pos[:, :n])
+    u = [G(x) for x in smooth_mac_field(res, seed=24)]
+    rng = np.random.default_rng(25)
+    Tin = (np.eye(3)[:, :, None] + 0.05 * rng.normal(size=(3, 3, n))).astype(np.float32)
+    C0 = rng.normal(size=(3, 3, n)).astype(np.float32)
+    m0 = rng.normal(size=(3, n)).astype(np.float32)
+    eye = [G(np.repeat(np.eye(3, dtype=np.float32)[m][:, None], n, axis=1)) for m in range(3)]
+    outs = []
+    for fused in (False, True):
+        p, Ti = G(pos), [G(Tin[m]) for m in range(3)]
+        b = core.Binning(res, n)
+        uo = [torch.empty(core.face_shape(res, a), device=DEV) for a in range(3)]
+        if fused:
+            b.rk4_record_count(dx, p, u, 0.02, Ti, [G(C0[m]) for m in range(3)], G(m0))
+            b.bin_finish(p, dx)
+        else:
+            b.rk4_record(dx, p, eye, u, 0.02, Ti, [G(C0[m]) for m in range(3)], G(m0), fold=True)
+            b.bin(p, dx, force=True)
+        b.p2g_from_records(dx, n, uo)
+        torch.cuda.synchronize()
+        outs.append([N(p)] + [N(t) for t in Ti] + [N(b.cell_start), N(b.perm)] + [N(x) for x in uo])
+    for a, b_ in zip(*outs):
+        np.testing.assert_array_equal(a, b_)
+
+
 @pytest.mark.parametrize("res", [(20, 17, 23), (40, 33)])
 def test_advect_and_clamp_smoke_bit_exact(oracle, pfm, res):
     """advect_smoke / clamp_smoke (3D/run_3D.py:315-334, 2D/run_2D.py:444-450)
