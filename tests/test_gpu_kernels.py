@@ -107,6 +107,49 @@ def test_rk4_particles(oracle, pfm, res):
         assert rel_l2(N(Tg[m]), T[m]) < 1e-5
 
 
+@pytest.mark.parametrize("dt_cells", [0.5, 3.0])
+def test_particle_results_do_not_depend_on_particle_order(oracle, pfm, dt_cells):
+    """A particle's RK4 and G2P results must not depend on the particles that
+    share its block: the same particles in cell order, in a random order and
+    in cell order with one in 17 swapped far away give bit-identical results
+    (at a CFL-sized step and at a step of three cells)."""
+    from pfm import core
+
+    res = (24, 20, 36)
+    pos, dx, n = _particles(oracle, res, seed=31, order=1)
+    u = smooth_mac_field(res, seed=32, amp=0.5)
+    umax = max(float(np.abs(x).max()) for x in u)
+    dt = dt_cells * dx / umax
+    rng = np.random.default_rng(33)
+    T = (np.eye(3)[:, :, None] + 0.1 * rng.normal(size=(3, 3, n))).astype(np.float32)
+    idx = np.arange(0, n, 17)
+    mixed = np.arange(n)
+    for a_, b_ in zip(idx, rng.permutation(n)[: len(idx)]):
+        mixed[a_], mixed[b_] = mixed[b_], mixed[a_]
+    orders = [np.arange(n), rng.permutation(n), mixed]
+    ug = [G(x) for x in u]
+    outs = []
+    for order in orders:
+        inv = np.argsort(order)
+        pg = G(np.ascontiguousarray(pos[:, order]))
+        Tg = [G(np.ascontiguousarray(T[m][:, order])) for m in range(3)]
+        core.rk4_particles(res, dx, pg, Tg, ug, dt)
+        ig = torch.empty((3, n), device=DEV)
+        Cg = [torch.empty((3, n), device=DEV) for _ in range(3)]
+        core.g2p(res, dx, pg, ug, ig, Cg)
+        ig0 = torch.empty((3, n), device=DEV)
+        core.g2p(res, dx, pg, ug, ig0, None)
+        torch.cuda.synchronize()
+        res_ = [N(pg)] + [N(t) for t in Tg] + [N(ig), N(ig0)] + [N(c) for c in Cg]
+        outs.append([x[:, inv] for x in res_])
+    for other in outs[1:]:
+        for a_, b_ in zip(outs[0], other):
+            np.testing.assert_array_equal(a_, b_)
+    # and the result is the RK4 of the oracle
+    oracle.rk4_particles(res, dx, pos, T, u, dt)
+    assert np.max(np.abs(outs[0][0] - pos)) < 1e-5
+
+
 def test_rk4_particles_2d_no_psi(oracle, pfm):
     from pfm import core
 
