@@ -41,12 +41,15 @@ __device__ __forceinline__ float t_product(const float A[3][3], const float B[3]
 // short reinitialisation would apply (3D/run_3D.py:767-771), leaving
 // T_grad_m in memory as the identity that reset would write.
 // One particle: i indexes the SoA arrays, whose component stride is n.
+// A non-null `lds_rec` receives the record (kRec floats at lds_rec, stride 1)
+// instead of global memory; the kernel then copies its block's records out.
 template <int DIM, bool REC, bool FOLD, bool BIN = false>
 __device__ __forceinline__ void rk4_particle(const GridDev& g, int64_t i, int64_t n, float* __restrict__ pos,
                                              float* __restrict__ Tx, float* __restrict__ Ty,
                                              float* __restrict__ Tz, const float* __restrict__ ux,
                                              const float* __restrict__ uy, const float* __restrict__ uz,
-                                             float dt, int advect_psi, const RecArgs& ra) {
+                                             float dt, int advect_psi, const RecArgs& ra,
+                                             float* lds_rec = nullptr) {
     float* Ts[3] = {Tx, Ty, Tz};
     float p0[3] = {0.f, 0.f, 0.f};
     float T0[3][3];
@@ -137,7 +140,14 @@ __device__ __forceinline__ void rk4_particle(const GridDev& g, int64_t i, int64_
                 C[r][c] = ra.C[r][c * n + i];     // rows(init_C)
             }
         }
-        write_p2g_record<DIM>(ra.rec + i * Pay<DIM>::kRec, p1, g.inv_dx, true, A, B, C, m0);
+        if (lds_rec) {
+            float rec[Pay<DIM>::kRec];
+            form_p2g_record<DIM>(rec, p1, g.inv_dx, true, A, B, C, m0);
+#pragma unroll
+            for (int k = 0; k < Pay<DIM>::kRec; ++k) lds_rec[k] = rec[k];
+        } else {
+            write_p2g_record<DIM>(ra.rec + i * Pay<DIM>::kRec, p1, g.inv_dx, true, A, B, C, m0);
+        }
         if (FOLD) {
             // update_T's A is cols(T_grad_m): Au[r][k] = T_k[r] (the record's A transposed)
             float Au[3][3];
@@ -172,8 +182,27 @@ __global__ void __launch_bounds__(kBlock, 3) k_rk4_particles(GridDev g, int64_t 
                                                           const float* __restrict__ uz, float dt,
                                                           int advect_psi, RecArgs ra = RecArgs{}) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    rk4_particle<DIM, REC, FOLD, BIN>(g, i, n, pos, Tx, Ty, Tz, ux, uy, uz, dt, advect_psi, ra);
+    if constexpr (REC && DIM == 3) {
+        // the block's records pass through LDS (stride kRec + 1 floats: the
+        // per-lane writes hit distinct banks) so that they leave as
+        // contiguous 16-byte chunks, one 1 KB span per wave store
+        constexpr int kR = Pay<3>::kRec, kS = kR + 1;
+        __shared__ float recs[kBlock * kS];
+        if (i < n)
+            rk4_particle<DIM, REC, FOLD, BIN>(g, i, n, pos, Tx, Ty, Tz, ux, uy, uz, dt, advect_psi, ra,
+                                              recs + threadIdx.x * kS);
+        __syncthreads();
+        const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+        const int nact = (int)min((int64_t)blockDim.x, n - b0);
+        float4* dst = reinterpret_cast<float4*>(ra.rec + b0 * kR);
+        for (int c = threadIdx.x; c < nact * (kR / 4); c += blockDim.x) {
+            const float* q = recs + (c / (kR / 4)) * kS + (c % (kR / 4)) * 4;
+            dst[c] = make_float4(q[0], q[1], q[2], q[3]);
+        }
+    } else {
+        if (i >= n) return;
+        rk4_particle<DIM, REC, FOLD, BIN>(g, i, n, pos, Tx, Ty, Tz, ux, uy, uz, dt, advect_psi, ra);
+    }
 }
 
 

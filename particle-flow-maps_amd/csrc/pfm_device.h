@@ -235,11 +235,10 @@ struct Pay {
 // Without `compose`, C and imp are written as given (the 2D driver forms them
 // in update_particles_grad_m / update_particles_imp).
 template <int DIM>
-__device__ __forceinline__ void write_p2g_record(float* __restrict__ dst, const float pos[3], float inv_dx,
-                                                 bool compose, const float A[3][3], const float B[3][3],
-                                                 const float C[3][3], const float imp[3]) {
+__device__ __forceinline__ void form_p2g_record(float rec[Pay<DIM>::kRec], const float pos[3], float inv_dx,
+                                                bool compose, const float A[3][3], const float B[3][3],
+                                                const float C[3][3], const float imp[3]) {
     using P = Pay<DIM>;
-    float rec[P::kRec];
 #pragma unroll
     for (int k = 0; k < P::kRec; ++k) rec[k] = 0.f;
 #pragma unroll
@@ -292,6 +291,15 @@ __device__ __forceinline__ void write_p2g_record(float* __restrict__ dst, const 
 #pragma unroll
         for (int c = 0; c < DIM; ++c) rec[P::kC + r * DIM + c] = Co[r][c];
     }
+}
+
+template <int DIM>
+__device__ __forceinline__ void write_p2g_record(float* __restrict__ dst, const float pos[3], float inv_dx,
+                                                 bool compose, const float A[3][3], const float B[3][3],
+                                                 const float C[3][3], const float imp[3]) {
+    using P = Pay<DIM>;
+    float rec[P::kRec];
+    form_p2g_record<DIM>(rec, pos, inv_dx, compose, A, B, C, imp);
     float4* d4 = reinterpret_cast<float4*>(dst);
 #pragma unroll
     for (int k = 0; k < P::kRec / 4; ++k)
