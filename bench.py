@@ -102,12 +102,8 @@ def instrumented_step(sim, timer):
     if cfg.use_midpoint_vel:
         with timer.region("midpoint_advect"):
             for a in range(sim.dim):
-                core.reset_identity_grid(sim.res, sim.dx, sim.psi_grid[a], sim.T_grid[a])
-            for a in range(sim.dim):
-                core.rk4_grid(sim.res, sim.dx, sim.psi_grid[a], sim.T_grid[a], sim.u, 0.5 * dt)
-            for a in range(sim.dim):
                 sim.p2g_weight[a].copy_(sim.u[a])
-            core.advect_u(sim.res, sim.dx, sim.p2g_weight, sim.u, sim.T_grid, sim.psi_grid)
+            core.midpoint_advect(sim.res, sim.dx, sim.p2g_weight, sim.u, sim.psi_grid, sim.T_grid, 0.5 * dt)
         with timer.region("poisson"):
             sim.project()
     if fold:

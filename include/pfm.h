@@ -72,6 +72,17 @@ int pfm_advect_u(const pfm_grid* g, const float* u0_x, const float* u0_y, const 
                  const float* T_z, const float* psi_x, const float* psi_y, const float* psi_z,
                  pfm_stream_t stream);
 
+/* The grid part of the midpoint velocity, 3D/run_3D.py:778-784 (2D:
+ * 2D/run_2D.py:865-870) in one launch: for every axis a,
+ * reset_to_identity_grid(psi_a, T_a), RK4_grid(psi_a, T_a, u0, dt) and
+ * advect_u(u0 -> u1_a), per face in registers.  u0 is the copy of the
+ * velocity that the reference's RK4_grid samples as u and advect_u as u0,
+ * so u1 (not aliasing u0) can be the live velocity; psi and T end as the
+ * three separate calls leave them. */
+int pfm_midpoint_advect(const pfm_grid* g, const float* u0_x, const float* u0_y, const float* u0_z,
+                        float* u1_x, float* u1_y, float* u1_z, float* psi_x, float* psi_y, float* psi_z,
+                        float* T_x, float* T_y, float* T_z, float dt, pfm_stream_t stream);
+
 /* reset_to_identity_grid: 3D/run_3D.py:106-120 with the face coordinates of
  * x/y/z_coords_func (3D/taichi_utils.py:33-46); 2D/run_2D.py:120-129. */
 int pfm_reset_identity_grid(const pfm_grid* g, int axis, float* psi, float* T,

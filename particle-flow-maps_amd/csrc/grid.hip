@@ -26,19 +26,14 @@ __device__ __forceinline__ void face_ijk(const GridDev& g, int a, int64_t f, int
 
 // RK4_grid: 3D/run_3D.py:144-173, 2D/run_2D.py:233-261.  dT = G @ T (rows form).
 // Face f of one axis; psi and T are (d, nf) with component stride nf.
+// rk4_face_reg: the same on values (p0, T0 in; p1, T1 out).
 template <int DIM>
-__device__ __forceinline__ void rk4_face(const GridDev& g, int64_t f, int64_t nf, float* __restrict__ psi,
-                                         float* __restrict__ T, const float* __restrict__ ux,
-                                         const float* __restrict__ uy, const float* __restrict__ uz, float dt) {
+__device__ __forceinline__ void rk4_face_reg(const GridDev& g, const float p0[3], const float T0[3],
+                                             const float* __restrict__ ux, const float* __restrict__ uy,
+                                             const float* __restrict__ uz, float dt, float p1[3], float T1[3]) {
     const float neg_dt = -1.0f * dt;
     const float hs[3] = {0.5f * neg_dt, 0.5f * neg_dt, 1.0f * neg_dt};
     const float ws[4] = {1.f, 2.f, 2.f, 1.f};
-    float p0[3] = {0.f, 0.f, 0.f}, T0[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-        p0[c] = psi[c * nf + f];
-        T0[c] = T[c * nf + f];
-    }
     float pk[3] = {p0[0], p0[1], p0[2]}, Tk[3] = {T0[0], T0[1], T0[2]};
     float us[3] = {0.f, 0.f, 0.f}, dTs[3] = {0.f, 0.f, 0.f};
 #pragma unroll 1
@@ -68,8 +63,26 @@ __device__ __forceinline__ void rk4_face(const GridDev& g, int64_t f, int64_t nf
     const float sixth = neg_dt * 1.0f / 6.0f;
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
-        psi[c * nf + f] = p0[c] + sixth * us[c];
-        T[c * nf + f] = T0[c] + sixth * dTs[c];
+        p1[c] = p0[c] + sixth * us[c];
+        T1[c] = T0[c] + sixth * dTs[c];
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void rk4_face(const GridDev& g, int64_t f, int64_t nf, float* __restrict__ psi,
+                                         float* __restrict__ T, const float* __restrict__ ux,
+                                         const float* __restrict__ uy, const float* __restrict__ uz, float dt) {
+    float p0[3] = {0.f, 0.f, 0.f}, T0[3] = {0.f, 0.f, 0.f}, p1[3] = {0.f, 0.f, 0.f}, T1[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        p0[c] = psi[c * nf + f];
+        T0[c] = T[c * nf + f];
+    }
+    rk4_face_reg<DIM>(g, p0, T0, ux, uy, uz, dt, p1, T1);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        psi[c * nf + f] = p1[c];
+        T[c * nf + f] = T1[c];
     }
 }
 
@@ -96,18 +109,27 @@ struct AdvectArgs {
 
 // Face f of axis ax: psi_ax / T_ax have component stride `stride`.
 template <int DIM>
+__device__ __forceinline__ float advect_face_reg(const GridDev& g, const float* const u0[3], const float p[3],
+                                                 const float Tv[3]) {
+    float u[3], G[3][3];
+    interp_mac<DIM, false>(u0[0], u0[1], u0[2], g, p, u, G);
+    float acc = Tv[0] * u[0];
+#pragma unroll
+    for (int c = 1; c < DIM; ++c) acc += Tv[c] * u[c];
+    return acc;
+}
+
+template <int DIM>
 __device__ __forceinline__ void advect_face(const GridDev& g, const float* const u0[3], const float* __restrict__ psi,
                                             const float* __restrict__ T, int64_t f, int64_t stride,
                                             float* __restrict__ u1) {
-    float p[3] = {0.f, 0.f, 0.f};
+    float p[3] = {0.f, 0.f, 0.f}, Tv[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) p[c] = psi[c * stride + f];
-    float u[3], G[3][3];
-    interp_mac<DIM, false>(u0[0], u0[1], u0[2], g, p, u, G);
-    float acc = T[f] * u[0];
-#pragma unroll
-    for (int c = 1; c < DIM; ++c) acc += T[c * stride + f] * u[c];
-    u1[f] = acc;
+    for (int c = 0; c < DIM; ++c) {
+        p[c] = psi[c * stride + f];
+        Tv[c] = T[c * stride + f];
+    }
+    u1[f] = advect_face_reg<DIM>(g, u0, p, Tv);
 }
 
 template <int DIM>
@@ -127,16 +149,57 @@ __global__ void __launch_bounds__(kBlock, 4) k_advect_u(GridDev g, AdvectArgs a)
 // Face f of the grid's array of `axis`, stored at index `at` of psi / T
 // (component stride nf).
 template <int DIM>
-__device__ __forceinline__ void reset_identity_face(const GridDev& g, int axis, int64_t f, int64_t at, int64_t nf,
-                                                    float* __restrict__ psi, float* __restrict__ T) {
+__device__ __forceinline__ void identity_face(const GridDev& g, int axis, int64_t f, float p[3], float Tv[3]) {
     int I[3];
     face_ijk<DIM>(g, axis, f, I);
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
         float x = c == axis ? (float)I[c] : __fadd_rn((float)I[c], 0.5f);
-        psi[c * nf + at] = __fmul_rn(x, g.dx);
-        T[c * nf + at] = c == axis ? 1.0f : 0.0f;
+        p[c] = __fmul_rn(x, g.dx);
+        Tv[c] = c == axis ? 1.0f : 0.0f;
     }
+}
+
+template <int DIM>
+__device__ __forceinline__ void reset_identity_face(const GridDev& g, int axis, int64_t f, int64_t at, int64_t nf,
+                                                    float* __restrict__ psi, float* __restrict__ T) {
+    float p[3] = {0.f, 0.f, 0.f}, Tv[3] = {0.f, 0.f, 0.f};
+    identity_face<DIM>(g, axis, f, p, Tv);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        psi[c * nf + at] = p[c];
+        T[c * nf + at] = Tv[c];
+    }
+}
+
+// The midpoint velocity's grid part (3D/run_3D.py:778-784): per face,
+// reset_to_identity_grid, RK4_grid over `dt` and advect_u, in registers.
+// Every face samples u0 (the copy of u), so u1 can be the live u.  psi and
+// T are still written, as the separate kernels leave them.
+struct MidOut {
+    float* psi[3];
+    float* T[3];
+};
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock, 4) k_midpoint_advect(GridDev g, AdvectArgs a, MidOut o, float dt) {
+    int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int ax = 0;
+    while (ax < DIM - 1 && f >= a.nf[ax]) {
+        f -= a.nf[ax];
+        ++ax;
+    }
+    if (f >= a.nf[ax]) return;
+    float p0[3] = {0.f, 0.f, 0.f}, T0[3] = {0.f, 0.f, 0.f}, p1[3] = {0.f, 0.f, 0.f}, T1[3] = {0.f, 0.f, 0.f};
+    identity_face<DIM>(g, ax, f, p0, T0);
+    rk4_face_reg<DIM>(g, p0, T0, a.u0[0], a.u0[1], a.u0[2], dt, p1, T1);
+    const int64_t nf = a.nf[ax];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        o.psi[ax][c * nf + f] = p1[c];
+        o.T[ax][c * nf + f] = T1[c];
+    }
+    a.u1[ax][f] = advect_face_reg<DIM>(g, a.u0, p1, T1);
 }
 
 template <int DIM>
@@ -749,6 +812,44 @@ int pfm_advect_u(const pfm_grid* gp, const float* u0_x, const float* u0_y, const
         hipLaunchKernelGGL(k_advect_u<3>, dim3(grid_for(total)), dim3(kBlock), 0, as_stream(stream), g, a);
     else
         hipLaunchKernelGGL(k_advect_u<2>, dim3(grid_for(total)), dim3(kBlock), 0, as_stream(stream), g, a);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+int pfm_midpoint_advect(const pfm_grid* gp, const float* u0_x, const float* u0_y, const float* u0_z,
+                        float* u1_x, float* u1_y, float* u1_z, float* psi_x, float* psi_y, float* psi_z,
+                        float* T_x, float* T_y, float* T_z, float dt, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    AdvectArgs a;
+    MidOut o;
+    const float* u0[3] = {u0_x, u0_y, u0_z};
+    float* u1[3] = {u1_x, u1_y, u1_z};
+    float* psi[3] = {psi_x, psi_y, psi_z};
+    float* T[3] = {T_x, T_y, T_z};
+    int64_t total = 0;
+    for (int c = 0; c < 3; ++c) {
+        a.u0[c] = u0[c];
+        a.u1[c] = u1[c];
+        a.T[c] = T[c];
+        a.psi[c] = psi[c];
+        o.psi[c] = psi[c];
+        o.T[c] = T[c];
+        a.nf[c] = c < g.dim ? face_count_h(g, c) : 0;
+        if (c < g.dim) {
+            PFM_CHECK_ARG(u0[c] && u1[c] && T[c] && psi[c], "null pointer for axis %d", c);
+            for (int d = 0; d < g.dim; ++d)
+                PFM_CHECK_ARG(u1[c] != u0[d], "u1 must not alias u0");
+        }
+        total += a.nf[c];
+    }
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_midpoint_advect<3>, dim3(grid_for(total)), dim3(kBlock), 0, as_stream(stream), g, a, o,
+                           dt);
+    else
+        hipLaunchKernelGGL(k_midpoint_advect<2>, dim3(grid_for(total)), dim3(kBlock), 0, as_stream(stream), g, a, o,
+                           dt);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }

@@ -95,15 +95,12 @@ int seed(Scene* sc, hipStream_t s) {
 // through it (the p2g weight arrays hold the copy of u), projection.
 int midpoint(Scene* sc, pfm_scene_state* st, double dt, hipStream_t s) {
     const pfm_scene_desc& d = sc->d;
-    for (int a = 0; a < d.dim; ++a)
-        PFM_STEP_TRY(pfm_reset_identity_grid(&sc->grid, a, d.psi_grid[a], d.T_grid[a], s));
-    for (int a = 0; a < d.dim; ++a)
-        PFM_STEP_TRY(pfm_rk4_grid(&sc->grid, a, d.psi_grid[a], d.T_grid[a], d.u[0], d.u[1], d.u[2],
-                                  (float)(0.5 * dt), s));
+    // the copy of u first: the fused launch samples it for RK4_grid and
+    // advect_u alike and writes the advected velocity into u
     for (int a = 0; a < d.dim; ++a) PFM_STEP_TRY(copy(d.p2g_weight[a], d.u[a], sc->faces[a], s));
-    PFM_STEP_TRY(pfm_advect_u(&sc->grid, d.p2g_weight[0], d.p2g_weight[1], d.p2g_weight[2], d.u[0], d.u[1],
-                              d.u[2], d.T_grid[0], d.T_grid[1], d.T_grid[2], d.psi_grid[0], d.psi_grid[1],
-                              d.psi_grid[2], s));
+    PFM_STEP_TRY(pfm_midpoint_advect(&sc->grid, d.p2g_weight[0], d.p2g_weight[1], d.p2g_weight[2], d.u[0], d.u[1],
+                                     d.u[2], d.psi_grid[0], d.psi_grid[1], d.psi_grid[2], d.T_grid[0], d.T_grid[1],
+                                     d.T_grid[2], (float)(0.5 * dt), s));
     return project(sc, st, s);
 }
 

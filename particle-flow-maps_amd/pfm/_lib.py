@@ -94,6 +94,7 @@ _SIGS = {
     "pfm_rk4_particles": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_f, c_int, c_vp],
     "pfm_rk4_grid": [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_f, c_vp],
     "pfm_advect_u": [c_vp] + [c_vp] * 12 + [c_vp],
+    "pfm_midpoint_advect": [c_vp] + [c_vp] * 12 + [c_f, c_vp],
     "pfm_reset_identity_grid": [c_vp, c_int, c_vp, c_vp, c_vp],
     "pfm_reset_T_identity": [c_int, c_i64, c_vp, c_vp, c_vp, c_vp],
     "pfm_seed_particles": [c_vp, c_int, c_f, c_i64, c_vp, c_i64, c_int, c_vp],

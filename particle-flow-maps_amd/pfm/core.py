@@ -133,6 +133,26 @@ def advect_u(res, dx, u0, u1, T, psi, stream=None):
            _stream(stream))
 
 
+def midpoint_advect(res, dx, u0, u1, psi, T, dt, stream=None):
+    """reset_identity_grid + rk4_grid(dt) on every axis, then advect_u(u0 -> u1),
+    in one launch (3D/run_3D.py:778-784).  u0 is the copy of the velocity that
+    both sample; u1 must not alias it."""
+    d = len(res)
+    g = L.make_grid(res, dx)
+    Pp, Tp = [], []
+    for a in range(3):
+        if a < d:
+            if face_axis(res, psi[a]) != a or tuple(T[a].shape) != tuple(psi[a].shape) or psi[a].shape[0] != d:
+                raise L.PfmError(f"psi/T of axis {a} must both be (d, *face_shape(axis {a}))")
+            Pp.append(L.ptr(psi[a], name=f"psi[{a}]"))
+            Tp.append(L.ptr(T[a], name=f"T[{a}]"))
+        else:
+            Pp.append(None)
+            Tp.append(None)
+    L.call("pfm_midpoint_advect", ctypes.byref(g), *_u_ptrs(u0, res), *_u_ptrs(u1, res), *Pp, *Tp, float(dt),
+           _stream(stream))
+
+
 def reset_identity_grid(res, dx, psi, T, stream=None):
     g = L.make_grid(res, dx)
     a = face_axis(res, psi)

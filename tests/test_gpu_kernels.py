@@ -165,6 +165,38 @@ def test_rk4_particles_2d_no_psi(oracle, pfm):
         assert rel_l2(N(Tg[m]), T[m]) < 1e-5
 
 
+@pytest.mark.parametrize("res", [(40, 33), (20, 17, 23)])
+def test_midpoint_advect_equals_separate_kernels(oracle, pfm, res):
+    """pfm_midpoint_advect = reset_identity_grid, rk4_grid and advect_u on
+    every axis (3D/run_3D.py:778-784), bit for bit: the advected velocity and
+    the psi / T grids it leaves."""
+    from pfm import core
+
+    d = len(res)
+    dx = 1.0 / res[1]
+    u = [G(x) for x in smooth_mac_field(res, seed=41)]
+    dt = 0.37 * dx
+    outs = []
+    for fused in (False, True):
+        u0 = [x.clone() for x in u]
+        u1 = [torch.full_like(x, 7.0) for x in u]
+        psi = [torch.empty((d,) + core.face_shape(res, a), device=DEV) for a in range(d)]
+        T = [torch.empty_like(p) for p in psi]
+        if fused:
+            core.midpoint_advect(res, dx, u0, u1, psi, T, dt)
+        else:
+            for a in range(d):
+                core.reset_identity_grid(res, dx, psi[a], T[a])
+                core.rk4_grid(res, dx, psi[a], T[a], u0, dt)
+            core.advect_u(res, dx, u0, u1, T, psi)
+        torch.cuda.synchronize()
+        outs.append([N(x) for x in u1 + psi + T])
+    for a_, b_ in zip(*outs):
+        np.testing.assert_array_equal(a_, b_)
+    with pytest.raises(Exception):
+        core.midpoint_advect(res, dx, u, u, psi, T, dt)
+
+
 @pytest.mark.parametrize("res", [(32, 32), (16, 16, 16)])
 def test_rk4_grid_and_advect(oracle, pfm, res):
     from pfm import core
