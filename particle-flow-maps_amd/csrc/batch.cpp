@@ -19,6 +19,8 @@
 namespace pfm {
 namespace {
 
+// kKFaceRk4 is the fused midpoint launch; kKAdvect is no longer launched and
+// keeps its slot so that pfm_batch2d_tuning's index layout stays as it was.
 enum Kind {
     kKRk4 = 0, kKG2p, kKSeed, kKFaceRk4, kKAdvect, kKFace, kKTile, kKBin, kKCell, kKDown, kKUp, kKCount
 };
@@ -340,18 +342,13 @@ int reinit(Batch* b, bool long_r, bool reseed, bool short_r) {
 int midpoint_pre(Batch* b) {
     const pfm_batch2d_desc& d = b->d;
     const BUnits fu = view(b, b->faces);
-    PFM_B_TRY(timed_launch(b, kKFace, [&] {
-        return b_reset_identity_grid(fu, b->F, d.psi_grid[0], d.psi_grid[1], d.T_grid[0], d.T_grid[1],
-                                     blocks_for(b, kKFace, b->faces), b->bs);
-    }));
-    PFM_B_TRY(timed_launch(b, kKFaceRk4, [&] {
-        return b_rk4_grid(fu, b->F, d.psi_grid[0], d.psi_grid[1], d.T_grid[0], d.T_grid[1], d.u[0], d.u[1], b->st,
-                          blocks_for(b, kKFaceRk4, b->faces), b->bs);
-    }));
+    // the copy of u first: one launch then resets, runs RK4_grid and advects
+    // every face, sampling the copy for both (the values RK4_grid reads as u)
     for (int a = 0; a < 2; ++a) PFM_B_TRY(copy_f(b, d.p2g_weight[a], d.u[a], b->F[a]));
-    return timed_launch(b, kKAdvect, [&] {
-        return b_advect_u(fu, b->F, d.p2g_weight[0], d.p2g_weight[1], d.u[0], d.u[1], d.T_grid[0], d.T_grid[1],
-                          d.psi_grid[0], d.psi_grid[1], blocks_for(b, kKAdvect, b->faces), b->bs);
+    return timed_launch(b, kKFaceRk4, [&] {
+        return b_midpoint_advect(fu, b->F, d.p2g_weight[0], d.p2g_weight[1], d.u[0], d.u[1], d.psi_grid[0],
+                                 d.psi_grid[1], d.T_grid[0], d.T_grid[1], b->st, blocks_for(b, kKFaceRk4, b->faces),
+                                 b->bs);
     });
 }
 

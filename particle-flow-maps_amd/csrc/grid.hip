@@ -320,66 +320,6 @@ __global__ void kb_dt(int n, const BScene* __restrict__ sc, BState* __restrict__
     b.dt = curr;
 }
 
-__global__ void __launch_bounds__(kBlock) kb_reset_identity_grid(BUnits t, int64_t F0, int64_t F1,
-                                                                 float* __restrict__ psi_x, float* __restrict__ psi_y,
-                                                                 float* __restrict__ T_x, float* __restrict__ T_y) {
-    PFM_FOR_UNITS(t, u) {
-        const BUnit w = t.u[u];
-        if (!unit_on(t, w.scene)) continue;
-        const BScene& S = t.sc[w.scene];
-        BFace bf;
-        if (!face_of_unit(S, w.chunk, bf)) continue;
-        const int64_t off = S.f_off[bf.ax];
-        if (bf.ax == 0)
-            reset_identity_face<2>(S.g, 0, bf.f, off + bf.f, F0, psi_x, T_x);
-        else
-            reset_identity_face<2>(S.g, 1, bf.f, off + bf.f, F1, psi_y, T_y);
-    }
-}
-
-// The face's own index within its scene decodes (i, j); storage uses the
-// global index with the axis' total as component stride.
-__global__ void __launch_bounds__(kBlock, 4) kb_rk4_grid(BUnits t, int64_t F0, int64_t F1,
-                                                      float* __restrict__ psi_x, float* __restrict__ psi_y,
-                                                      float* __restrict__ T_x, float* __restrict__ T_y,
-                                                      const float* __restrict__ ux, const float* __restrict__ uy,
-                                                      const BState* __restrict__ st) {
-    PFM_FOR_UNITS(t, u) {
-        const BUnit w = t.u[u];
-        if (!unit_on(t, w.scene)) continue;
-        const BScene& S = t.sc[w.scene];
-        BFace bf;
-        if (!face_of_unit(S, w.chunk, bf)) continue;
-        const float dt = (float)(0.5 * st[w.scene].dt);
-        const int64_t gf = S.f_off[bf.ax] + bf.f;
-        if (bf.ax == 0)
-            rk4_face<2>(S.g, gf, F0, psi_x, T_x, ux + S.f_off[0], uy + S.f_off[1], nullptr, dt);
-        else
-            rk4_face<2>(S.g, gf, F1, psi_y, T_y, ux + S.f_off[0], uy + S.f_off[1], nullptr, dt);
-    }
-}
-
-__global__ void __launch_bounds__(kBlock, 4) kb_advect_u(BUnits t, int64_t F0, int64_t F1,
-                                                      const float* __restrict__ u0x, const float* __restrict__ u0y,
-                                                      float* __restrict__ u1x, float* __restrict__ u1y,
-                                                      const float* __restrict__ T_x, const float* __restrict__ T_y,
-                                                      const float* __restrict__ psi_x,
-                                                      const float* __restrict__ psi_y) {
-    PFM_FOR_UNITS(t, u) {
-        const BUnit w = t.u[u];
-        if (!unit_on(t, w.scene)) continue;
-        const BScene& S = t.sc[w.scene];
-        BFace bf;
-        if (!face_of_unit(S, w.chunk, bf)) continue;
-        const float* u0[3] = {u0x + S.f_off[0], u0y + S.f_off[1], nullptr};
-        const int64_t gf = S.f_off[bf.ax] + bf.f;
-        if (bf.ax == 0)
-            advect_face<2>(S.g, u0, psi_x, T_x, gf, F0, u1x);
-        else
-            advect_face<2>(S.g, u0, psi_y, T_y, gf, F1, u1y);
-    }
-}
-
 int b_max_speed(BUnits cells, const float* ux, const float* uy, BState* st, unsigned blocks, hipStream_t s) {
     hipLaunchKernelGGL(kb_max_speed, dim3(blocks), dim3(kBlock), 0, s, cells, ux, uy, st);
     PFM_LAUNCH_CHECK();
@@ -398,31 +338,46 @@ int b_dt(int n_scenes, const BScene* sc, BState* st, double CFL, double visualiz
     return PFM_OK;
 }
 
-int b_reset_identity_grid(BUnits faces, const int64_t F[2], float* psi_x, float* psi_y, float* T_x, float* T_y,
-                          unsigned blocks, hipStream_t s) {
-    hipLaunchKernelGGL(kb_reset_identity_grid, dim3(blocks), dim3(kBlock), 0, s, faces, F[0], F[1], psi_x, psi_y,
-                       T_x, T_y);
+// Batched k_midpoint_advect: every face samples u0 (the copy of u) for
+// RK4_grid over half its scene's dt and for advect_u, writes psi / T and the
+// advected velocity into u1.
+__global__ void __launch_bounds__(kBlock, 4) kb_midpoint_advect(BUnits t, int64_t F0, int64_t F1,
+                                                             const float* __restrict__ u0x,
+                                                             const float* __restrict__ u0y, float* __restrict__ u1x,
+                                                             float* __restrict__ u1y, float* __restrict__ psi_x,
+                                                             float* __restrict__ psi_y, float* __restrict__ T_x,
+                                                             float* __restrict__ T_y, const BState* __restrict__ st) {
+    PFM_FOR_UNITS(t, u) {
+        const BUnit w = t.u[u];
+        if (!unit_on(t, w.scene)) continue;
+        const BScene& S = t.sc[w.scene];
+        BFace bf;
+        if (!face_of_unit(S, w.chunk, bf)) continue;
+        const float dt = (float)(0.5 * st[w.scene].dt);
+        const float* u0[3] = {u0x + S.f_off[0], u0y + S.f_off[1], nullptr};
+        const int64_t gf = S.f_off[bf.ax] + bf.f, nf = bf.ax == 0 ? F0 : F1;
+        float p0[3] = {0.f, 0.f, 0.f}, T0[3] = {0.f, 0.f, 0.f}, p1[3] = {0.f, 0.f, 0.f}, T1[3] = {0.f, 0.f, 0.f};
+        identity_face<2>(S.g, bf.ax, bf.f, p0, T0);
+        rk4_face_reg<2>(S.g, p0, T0, u0[0], u0[1], nullptr, dt, p1, T1);
+        float* psi = bf.ax == 0 ? psi_x : psi_y;
+        float* T = bf.ax == 0 ? T_x : T_y;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            psi[c * nf + gf] = p1[c];
+            T[c * nf + gf] = T1[c];
+        }
+        (bf.ax == 0 ? u1x : u1y)[gf] = advect_face_reg<2>(S.g, u0, p1, T1);
+    }
+}
+
+int b_midpoint_advect(BUnits faces, const int64_t F[2], const float* u0x, const float* u0y, float* u1x, float* u1y,
+                      float* psi_x, float* psi_y, float* T_x, float* T_y, const BState* st, unsigned blocks,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(kb_midpoint_advect, dim3(blocks), dim3(kBlock), 0, s, faces, F[0], F[1], u0x, u0y, u1x, u1y,
+                       psi_x, psi_y, T_x, T_y, st);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
-
-int b_rk4_grid(BUnits faces, const int64_t F[2], float* psi_x, float* psi_y, float* T_x, float* T_y,
-               const float* ux, const float* uy, const BState* st, unsigned blocks, hipStream_t s) {
-    hipLaunchKernelGGL(kb_rk4_grid, dim3(blocks), dim3(kBlock), 0, s, faces, F[0], F[1], psi_x, psi_y, T_x, T_y,
-                       ux, uy, st);
-    PFM_LAUNCH_CHECK();
-    return PFM_OK;
-}
-
-int b_advect_u(BUnits faces, const int64_t F[2], const float* u0x, const float* u0y, float* u1x, float* u1y,
-               const float* T_x, const float* T_y, const float* psi_x, const float* psi_y, unsigned blocks,
-               hipStream_t s) {
-    hipLaunchKernelGGL(kb_advect_u, dim3(blocks), dim3(kBlock), 0, s, faces, F[0], F[1], u0x, u0y, u1x, u1y, T_x,
-                       T_y, psi_x, psi_y);
-    PFM_LAUNCH_CHECK();
-    return PFM_OK;
-}
-
 
 // split_central_vector: 3D/taichi_utils.py:87-100 (clamped samples).
 template <int DIM>

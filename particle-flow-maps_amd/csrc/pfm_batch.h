@@ -102,13 +102,9 @@ int b_rk4_update(BUnits t, int64_t P, float* pos, float* Tx, float* Ty, const fl
 int b_max_speed(BUnits cells, const float* ux, const float* uy, BState* st, unsigned blocks, hipStream_t s);
 int b_dt(int n_scenes, const BScene* sc, BState* st, double CFL, double visualize_dt, hipStream_t s);
 int b_ms_init(int n_scenes, BState* st, hipStream_t s);
-int b_reset_identity_grid(BUnits faces, const int64_t F[2], float* psi_x, float* psi_y, float* T_x, float* T_y,
-                          unsigned blocks, hipStream_t s);
-int b_rk4_grid(BUnits faces, const int64_t F[2], float* psi_x, float* psi_y, float* T_x, float* T_y,
-               const float* ux, const float* uy, const BState* st, unsigned blocks, hipStream_t s);
-int b_advect_u(BUnits faces, const int64_t F[2], const float* u0x, const float* u0y, float* u1x, float* u1y,
-               const float* T_x, const float* T_y, const float* psi_x, const float* psi_y, unsigned blocks,
-               hipStream_t s);
+int b_midpoint_advect(BUnits faces, const int64_t F[2], const float* u0x, const float* u0y, float* u1x, float* u1y,
+                      float* psi_x, float* psi_y, float* T_x, float* T_y, const BState* st, unsigned blocks,
+                      hipStream_t s);
 
 // p2g.hip
 int b_bin(BUnits parts, int64_t P, int64_t C, const float* pos, int32_t* cell_of, int32_t* cell_start,

@@ -21,7 +21,9 @@ from . import _lib as L
 from . import core
 from .simulation import StepConfig
 
-# kernel kinds of pfm_batch2d_tuning, in order (then the coarse kernel's block size)
+# kernel kinds of pfm_batch2d_tuning, in order (then the coarse kernel's block size); "rk4_grid" is the
+# fused midpoint launch (identity, RK4_grid and advect_u per face) and "advect_u" is no longer launched
+# (its slot keeps the index layout)
 KINDS = ("rk4", "g2p", "seed", "rk4_grid", "advect_u", "faces", "p2g_tile", "bin", "cells", "mg_down", "mg_up")
 
 
