@@ -157,3 +157,32 @@ def test_batch_create_rejects_bad_descriptors():
     d.max_iters, d.reinit_every = 400, 0
     assert lib.pfm_batch2d_create(ctypes.byref(d), ctypes.byref(h), ctypes.byref(nbytes)) == -1
     assert not h.value
+
+
+def test_fused_entry_points_validate_arguments():
+    """The fused step entry points check their arguments on the host (no GPU
+    call is made before the checks)."""
+    from pfm import _lib
+
+    lib = _lib.load(require_gpu=False)
+    g = _lib.make_grid((16, 16, 16), 1 / 16)
+    f = ctypes.c_void_p(0x1000)  # never dereferenced: validation fails first
+    u0 = [ctypes.c_void_p(0x1000 + 0x100 * k) for k in range(3)]
+    u1 = [ctypes.c_void_p(0x2000 + 0x100 * k) for k in range(3)]
+    # the midpoint launch needs every face pointer of every axis
+    rc = lib.pfm_midpoint_advect(ctypes.byref(g), *u0, *u1, f, None, f, f, f, f, ctypes.c_float(0.1), None)
+    assert rc == -1 and "null pointer" in _lib.last_error()
+    # ... and u1 must not alias u0
+    rc = lib.pfm_midpoint_advect(ctypes.byref(g), f, f, f, f, f, f, f, f, f, f, f, f, ctypes.c_float(0.1), None)
+    assert rc == -1 and "alias" in _lib.last_error()
+    # the fused RK4 + record + count pass is 3D and needs n >= 1
+    g2 = _lib.make_grid((16, 16), 1 / 16)
+    args = [f] * 3 + [ctypes.c_float(0.1)] + [f] * 7 + [None, f, ctypes.c_size_t(1 << 30), None]
+    rc = lib.pfm_rk4_particles_p2g_record_fold_count(ctypes.byref(g2), 8, f, *args)
+    assert rc == -1 and "3D" in _lib.last_error()
+    rc = lib.pfm_rk4_particles_p2g_record_fold_count(ctypes.byref(g), 0, f, *args)
+    assert rc == -1 and "n out of range" in _lib.last_error()
+    # the rest of the binning rejects a workspace smaller than the layout
+    need = lib.pfm_p2g_workspace_size(ctypes.byref(g), 4096)
+    rc = lib.pfm_bin_finish(ctypes.byref(g), 4096, f, f, f, f, ctypes.c_size_t(need - 1), None)
+    assert rc == -1 and "workspace too small" in _lib.last_error()
