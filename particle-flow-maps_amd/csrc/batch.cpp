@@ -189,8 +189,7 @@ int project_setup(Batch* b) {
     PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
         return bm_reduce(t, kBRedSum, b->r, nullptr, b->mg, b->part, nb, b->bs);
     }));
-    PFM_B_TRY(finalize(b, kBFinMean));
-    PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) { return bm_sub_mean(t, b->r, b->mg, nb, b->bs); }));
+    PFM_B_TRY(finalize(b, kBFinMean));  // the level-0 readers subtract the mean as they load r
     if (b->d.tol_mode == 1) {
         PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) {
             return bm_reduce(t, kBRedRR, b->r, nullptr, b->mg, b->part, nb, b->bs);
@@ -218,8 +217,7 @@ int pcg_iteration(Batch* b, int k) {
     }));
     PFM_B_TRY(finalize(b, kBFinRTr));
     PFM_B_TRY(compact(b));
-    PFM_B_TRY(cells_launch(b, [](Batch* b, BUnits t, unsigned nb) { return bm_sub_mean(t, b->r, b->mg, nb, b->bs); }));
-    PFM_B_TRY(vcycle(b));  // leaves the z.r partials
+    PFM_B_TRY(vcycle(b));  // recentres r as it loads it; leaves the z.r partials
     PFM_B_TRY(finalize(b, kBFinBeta));
     return compact(b);
 }

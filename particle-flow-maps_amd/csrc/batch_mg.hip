@@ -136,7 +136,10 @@ __device__ __forceinline__ void wave_partial(double v, double* __restrict__ part
     if ((threadIdx.x & 63) == 0) part[(int64_t)unit * kBatchWaves + (threadIdx.x >> 6)] = v;
 }
 
-// Per-unit partial sums of r (kBRedSum), r.r or z.r on level 0.
+// Per-unit partial sums of r (kBRedSum), r.r or z.r on level 0.  Level-0 r is
+// kept as update_xr leaves it; every reader of the recentred r (3D/mgpcg.py:
+// 174-180, 215) forms r - mean as it loads (the same rounding as storing it),
+// so no separate recentring pass runs.  The sum itself reads r as stored.
 __global__ void __launch_bounds__(kBlock) kb_reduce(BUnits t, int mode, const float* __restrict__ r,
                                                     const float* __restrict__ z, const BMg* __restrict__ mg,
                                                     double* __restrict__ part) {
@@ -149,7 +152,7 @@ __global__ void __launch_bounds__(kBlock) kb_reduce(BUnits t, int mode, const fl
         double v = 0.0;
         if (q < S.nc) {
             const int64_t g = S.c_off + q;
-            const double rv = (double)r[g];
+            const double rv = mode == kBRedSum ? (double)r[g] : (double)(r[g] - mg[w.scene].mean);
             v = mode == kBRedSum ? rv : (mode == kBRedRR ? rv * rv : (double)z[g] * rv);
         }
         wave_partial(v, part, S.part_off + w.chunk);
@@ -208,27 +211,14 @@ __global__ void __launch_bounds__(kBlock) kb_update_xr(BUnits t, float* __restri
         if (q < S.nc) {
             const int64_t g = S.c_off + q;
             x[g] = x[g] + alpha * p[g];
-            const float rq = r[g] - alpha * Ap[g];
+            const float rr = r[g] - mg[w.scene].mean;  // the recentred r
+            const float rq = rr - alpha * Ap[g];
             r[g] = rq;
             s2 = (double)rq * (double)rq;
             s1 = (double)rq;
         }
         wave_partial(s2, part, S.part_off + w.chunk);
         wave_partial(s1, part2, S.part_off + w.chunk);
-    }
-}
-
-// recenter (3D/mgpcg.py:174-180): r -= mean.
-__global__ void __launch_bounds__(kBlock) kb_sub_mean(BUnits t, float* __restrict__ r, const BMg* __restrict__ mg) {
-    PFM_FOR_UNITS(t, u) {
-        const BUnit w = t.u[u];
-        if (!unit_on(t, w.scene)) continue;
-        if (!scene_active(mg, w.scene)) continue;
-        const BScene& S = t.sc[w.scene];
-        const int64_t q = (int64_t)w.chunk * kBlock + threadIdx.x;
-        if (q >= S.nc) continue;
-        const int64_t g = S.c_off + q;
-        r[g] = r[g] - mg[w.scene].mean;
     }
 }
 
@@ -424,10 +414,16 @@ __global__ void __launch_bounds__(kBlock) kb_down(BUnits t, int l, BWalls wl, fl
         int i0, j0;
         tile_of_unit(S, l, w.chunk, i0, j0);
         const float* rl = rg + S.l_off[l];
+        const float mean = mg[w.scene].mean;
         for (int q = threadIdx.x; q < W * W; q += blockDim.x) {
             const int i = i0 - H + q / W, j = j0 - H + q % W;
             const bool in = i >= 0 && j >= 0 && i < n0 && j < n1;
-            r[q] = in ? rl[(int64_t)i * n1 + j] : 0.0f;
+            float rv = 0.0f;
+            if (in) {
+                rv = rl[(int64_t)i * n1 + j];
+                if (l == 0) rv = rv - mean;  // level 0: the recentred r
+            }
+            r[q] = rv;
             z[q] = 0.0f;
         }
         __syncthreads();
@@ -497,6 +493,7 @@ __global__ void __launch_bounds__(kBlock) kb_up(BUnits t, int l, BWalls wl, cons
                 const int64_t g = (int64_t)i * n1 + j;
                 zv = zl[g] + zc[(int64_t)min(i >> 1, c0 - 1) * c1 + min(j >> 1, c1 - 1)];
                 rv = rl[g];
+                if (l == 0) rv = rv - mg[w.scene].mean;  // level 0: the recentred r
             }
             z[q] = zv;
             r[q] = rv;
@@ -560,7 +557,13 @@ __global__ void kb_coarse(const BScene* __restrict__ sc, BWalls wl, float* __res
     }
     {
         const int n = S.ld[lc][0] * S.ld[lc][1];
-        for (int q = threadIdx.x; q < n; q += blockDim.x) lds[ro[lc] + q] = rg[S.l_off[lc] + q];
+        // a scene whose level 0 is coarse reads the recentred r
+        const float mean = mg[s].mean;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            float rv = rg[S.l_off[lc] + q];
+            if (lc == 0) rv = rv - mean;
+            lds[ro[lc] + q] = rv;
+        }
     }
     __syncthreads();
     for (int l = lc; l < L - 1; ++l) {
@@ -657,10 +660,6 @@ int bm_ap_dot(BUnits cells, BWalls wl, const float* z, const float* pin, float* 
 int bm_update_xr(BUnits cells, float* x, float* r, const float* p, const float* Ap, const BMg* mg, double* part,
                  double* part2, unsigned blocks, hipStream_t s) {
     PFM_B_LAUNCH(kb_update_xr, blocks, cells, x, r, p, Ap, mg, part, part2);
-    return PFM_OK;
-}
-int bm_sub_mean(BUnits cells, float* r, const BMg* mg, unsigned blocks, hipStream_t s) {
-    PFM_B_LAUNCH(kb_sub_mean, blocks, cells, r, mg);
     return PFM_OK;
 }
 int bm_finalize(int n_scenes, const BScene* sc, int mode, double eps, int max_iters, const double* part,

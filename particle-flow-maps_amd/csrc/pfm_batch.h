@@ -132,7 +132,6 @@ int bm_ap_dot(BUnits cells, BWalls wl, const float* z, const float* pin, float* 
               double* part, unsigned blocks, hipStream_t s);
 int bm_update_xr(BUnits cells, float* x, float* r, const float* p, const float* Ap, const BMg* mg, double* part,
                  double* part2, unsigned blocks, hipStream_t s);
-int bm_sub_mean(BUnits cells, float* r, const BMg* mg, unsigned blocks, hipStream_t s);
 int bm_finalize(int n_scenes, const BScene* sc, int mode, double eps, int max_iters, const double* part,
                 const double* part2, const double* zr, BMg* mg, hipStream_t s);
 int bm_mg_reset(int n_scenes, BMg* mg, const BState* st, int tol_mode, double tol, int max_iters, hipStream_t s);
