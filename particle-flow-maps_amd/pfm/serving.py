@@ -336,8 +336,12 @@ class Server:
         from .batch import Batch2D
 
         jobs = [j for j, _ in group]
+        # no HIP graphs here: capturing one while the server's other threads
+        # make synchronous HIP calls (a job's solver set-up, the allocator)
+        # fails both sides on ROCm, so a served batch replays its launches
+        # directly (the same kernels, the same results)
         b = Batch2D([j.res for j in jobs], batch_step_config(), seeds=[j.seed % (2**31) for j in jobs],
-                    device=self.device)
+                    device=self.device, graphs=False)
         for k, j in enumerate(jobs):
             c, w, rad = point_vortex_ic(j)
             b.set_point_vortices(k, c, w, rad)
