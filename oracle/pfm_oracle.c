@@ -27,6 +27,7 @@
  * particle p at [c*n + p].  A 2D run uses dim=2 and res[2]=1.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -116,18 +117,31 @@ static void interp_grad(const grid_t* g, const float* vf, const int shp[3],
     float iu = floorf(s), iv = floorf(t), iw = floorf(l);
     float px = 0.0f, py = 0.0f, pz = 0.0f, interped = 0.0f;
     int k0 = g->dim == 3 ? -1 : 0, k1 = g->dim == 3 ? 2 : 0;
+    /* The spline factors of offset o along an axis depend only on that axis,
+     * so they are evaluated once per axis (the same values the point loop of
+     * the reference computes for every point, so the products are unchanged). */
+    float Nxa[4], dNxa[4], Nya[4], dNya[4], Nza[4], dNza[4];
+    int ia[4], ja[4], ka[4];
+    for (int o = -1; o <= 2; ++o) {
+        float xp = s - (iu + (float)o);
+        float yp = t - (iv + (float)o);
+        float zp = l - (iw + (float)o);
+        Nxa[o + 1] = oracle_N2(xp); dNxa[o + 1] = oracle_dN2(xp);
+        Nya[o + 1] = oracle_N2(yp); dNya[o + 1] = oracle_dN2(yp);
+        Nza[o + 1] = g->dim == 3 ? oracle_N2(zp) : 1.0f;
+        dNza[o + 1] = g->dim == 3 ? oracle_dN2(zp) : 0.0f;
+        ia[o + 1] = (int)(iu + (float)o);
+        ja[o + 1] = (int)(iv + (float)o);
+        ka[o + 1] = (int)(iw + (float)o);
+    }
     for (int i = -1; i <= 2; ++i)
         for (int j = -1; j <= 2; ++j)
             for (int k = k0; k <= k1; ++k) {
-                float xp = s - (iu + (float)i);
-                float yp = t - (iv + (float)j);
-                float zp = l - (iw + (float)k);
-                float value = sample3(vf, shp, (int)(iu + (float)i), (int)(iv + (float)j),
-                                      (int)(iw + (float)k));
-                float Nx = oracle_N2(xp), Ny = oracle_N2(yp);
-                float dNx = oracle_dN2(xp), dNy = oracle_dN2(yp);
-                float Nz = g->dim == 3 ? oracle_N2(zp) : 1.0f;
-                float dNz = g->dim == 3 ? oracle_dN2(zp) : 0.0f;
+                float value = sample3(vf, shp, ia[i + 1], ja[j + 1], ka[k + 1]);
+                float Nx = Nxa[i + 1], Ny = Nya[j + 1];
+                float dNx = dNxa[i + 1], dNy = dNya[j + 1];
+                float Nz = Nza[k + 1];
+                float dNz = dNza[k + 1];
                 if (imp_form) {
                     float dwx = g->inv_dx * dNx * Ny * Nz;
                     float dwy = g->inv_dx * Nx * dNy * Nz;
@@ -538,17 +552,31 @@ void oracle_p2g(int dim, const int* res, float dx, int64_t n, const float* pos, 
         memset(uf[a], 0, sizeof(float) * (size_t)nf[a]);
         memset(Wf[a], 0, sizeof(float) * (size_t)nf[a]);
     }
+    /* Each thread owns a contiguous range of first face indices of every axis
+     * and walks all particles in index order, adding only to its own faces:
+     * every face still receives its contributions in particle order, so the
+     * sums equal the sequential scatter bit for bit. */
+#pragma omp parallel
+    {
+        int nt = omp_get_num_threads(), tid = omp_get_thread_num();
+        int lo[3], hi[3];
+        for (int a = 0; a < dim; ++a) {
+            lo[a] = (int)((int64_t)shp[a][0] * tid / nt);
+            hi[a] = (int)((int64_t)shp[a][0] * (tid + 1) / nt);
+        }
     for (int64_t i = 0; i < n; ++i) {
         float pos_g[3] = {0, 0, 0};
         for (int c = 0; c < dim; ++c) pos_g[c] = pos[c * n + i] * g.inv_dx;
         for (int a = 0; a < dim; ++a) {
             int base[3] = {0, 0, 0};
             for (int c = 0; c < dim; ++c) base[c] = (int)(c == a ? pos_g[c] : pos_g[c] - 0.5f);
+            if (base[0] + 2 < lo[a] || base[0] - 1 >= hi[a]) continue;
             int k0 = dim == 3 ? -1 : 0, k1 = dim == 3 ? 2 : 0;
             for (int oi = -1; oi <= 2; ++oi)
                 for (int oj = -1; oj <= 2; ++oj)
                     for (int ok = k0; ok <= k1; ++ok) {
                         int f[3] = {base[0] + oi, base[1] + oj, base[2] + ok};
+                        if (f[0] < lo[a] || f[0] >= hi[a]) continue;
                         int inb = 1;
                         for (int c = 0; c < dim; ++c)
                             if (f[c] < 0 || f[c] > shp[a][c] - 1) inb = 0;
@@ -570,7 +598,9 @@ void oracle_p2g(int dim, const int* res, float dx, int64_t n, const float* pos, 
                     }
         }
     }
+    }
     for (int a = 0; a < dim; ++a) {
+#pragma omp parallel for schedule(static)
         for (int64_t f = 0; f < nf[a]; ++f)
             if (Wf[a][f] > 0.0f) uf[a][f] /= Wf[a][f];
         free(Wtmp[a]);

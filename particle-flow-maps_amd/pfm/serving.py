@@ -214,6 +214,13 @@ class Admission:
         self.slots_used -= 1 if self.takes_slot(job) else 0
         self.retained += min(int(retain), fp)
 
+    def release_batch(self, jobs: Sequence[Job]) -> None:
+        """Release every job of a finished batch at once, keeping all their
+        footprints as retained: the batch's device memory stays allocated as
+        one object until the server drops it."""
+        for job in jobs:
+            self.release(job, retain=self.running.get(job.id, 0))
+
     def free_retained(self) -> None:
         self.retained = 0
 
@@ -438,14 +445,17 @@ class Server:
                             bwork.put(None)
                             break
                         group.append(nxt)
+                    # one item per batch: the main thread releases the batch's jobs and
+                    # takes the batch object together, so the batch's memory is counted
+                    # until the server itself drops it
                     try:
                         out, b = self._run_batch(group, t0)
-                        for k, r in enumerate(out):
-                            done.put((r, b if k == len(out) - 1 else None))
+                        done.put((out, b))
+                        del out, b  # this thread keeps no reference to the batch
                     except Exception as e:  # reported per job; the server keeps going
                         now = time.perf_counter() - t0
-                        for job, admitted in group:
-                            done.put((JobResult(job, admitted, now, now, [], [], float("nan"), None, repr(e)), None))
+                        done.put(([JobResult(job, admitted, now, now, [], [], float("nan"), None, repr(e))
+                                   for job, admitted in group], None))
 
         threads = [threading.Thread(target=worker, daemon=True) for _ in range(self.slots)]
         if self.batch_2d:
@@ -477,18 +487,21 @@ class Server:
                     r, solver = done.get(timeout=timeout)
                 except queue.Empty:
                     continue
-                if self._batched(r.job):
-                    # the batch's memory stays until the batch object is freed
-                    self.admission.release(r.job, retain=self.admission.running.get(r.job.id, 0))
+                if isinstance(r, list):
+                    # a whole batch: its memory stays counted until the batch object is freed
+                    self.admission.release_batch([x.job for x in r])
+                    finished = r
                 else:
                     keep = mg_bytes(r.job.res, 3, self.mg_precision) if solver is not None else 0
                     self.admission.release(r.job, retain=keep)
+                    finished = [r]
                 if solver is not None:
                     retired.append(solver)
                 del solver
-                results.append(r)
-                if progress is not None:
-                    progress(r)
+                for x in finished:
+                    results.append(x)
+                    if progress is not None:
+                        progress(x)
         finally:
             for _ in range(self.slots):
                 work.put(None)

@@ -193,6 +193,52 @@ def test_p2g_prep_composition(oracle):
         assert np.max(np.abs(Tout[:, :, p].T - G @ Bm)) < 1e-4
 
 
+def test_update_imp_columns_are_images_of_the_basis(oracle):
+    """2D update_particles_imp (2D/run_2D.py:517-524): T = cols(T_x, T_y), so
+    T_x and T_y are the images of e_x and e_y and imp = a T_x + b T_y for
+    init_imp = (a, b).  A 90-degree rotation and a shear (both non-symmetric,
+    small integers so every product is exact) catch a transposed T."""
+    # particle 0: rotation by +90 degrees (e_x -> e_y, e_y -> -e_x)
+    # particle 1: shear x += 3 y (e_x -> e_x, e_y -> (3, 1))
+    # particle 2: rotation times shear, T_x = (0, 1), T_y = (-1, 3)
+    T = np.zeros((2, 2, 3), np.float32)
+    T[0, :, 0], T[1, :, 0] = (0, 1), (-1, 0)
+    T[0, :, 1], T[1, :, 1] = (1, 0), (3, 1)
+    T[0, :, 2], T[1, :, 2] = (0, 1), (-1, 3)
+    m = np.array([[1, 2, 2], [0, 5, -1]], np.float32)  # init_imp columns per particle
+    imp = oracle.update_imp(T, m)
+    # rotated e_x is e_y; shear of (2, 5) is (17, 5); T @ (2, -1) = 2 (0,1) - (-1, 3) = (1, -1)
+    want = np.array([[0, 17, 1], [1, 5, -1]], np.float32)
+    assert np.array_equal(imp, want)
+    # the transposed product differs on every particle, so the check has teeth
+    wrong = np.einsum("mcp,cp->mp", T, m)
+    assert all(not np.array_equal(wrong[:, p], want[:, p]) for p in range(3))
+
+
+def test_cfl_dt_frame_clip_hand_computed(oracle):
+    """dt logic of 3D/run_3D.py:725-743: dt = CFL dx / max_speed (max_speed a
+    float32 read into a double), and the step that would reach visualize_dt is
+    clipped to end the frame exactly, resetting sub_t."""
+    dx = 1.0 / 128
+    # max speed 0.5: dt = 0.5 / 128 / 0.5 = 1/128 exactly; 12 steps reach 12/128 =
+    # 0.09375 and the 13th (0.09375 + 1/128 >= 0.1) is clipped to 0.1 - 0.09375
+    sub_t, seen = 0.0, []
+    for _ in range(14):
+        dt, sub_t, end = oracle.cfl_dt(0.5, dx, 0.5, sub_t, 0.1)
+        seen.append((dt, sub_t, end))
+    for k in range(12):
+        assert seen[k] == (1 / 128, (k + 1) / 128, False)
+    assert seen[12][2] and seen[12][1] == 0.0 and abs(seen[12][0] - 0.00625) < 1e-15
+    assert seen[13] == (1 / 128, 1 / 128, False)
+    # ">=": with visualize_dt = 4/128 the 4th step lands exactly on it and ends the frame
+    sub_t = 3 / 128
+    dt, sub_t, end = oracle.cfl_dt(0.5, dx, 0.5, sub_t, 4 / 128)
+    assert end and dt == 1 / 128 and sub_t == 0.0
+    # the max speed is the float32 value of the field, not a double
+    dt, _, _ = oracle.cfl_dt(np.float32(0.3), dx, 0.5, 0.0, 1.0)
+    assert dt == 0.5 * dx / 0.30000001192092896 and dt != 0.5 * dx / 0.3
+
+
 # ---- row 15: binning = stable argsort by truncated cell ------------------
 @pytest.mark.parametrize("res", [(8, 8), (6, 8, 5)])
 def test_binning_stable_argsort(oracle, res):

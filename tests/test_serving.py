@@ -88,6 +88,29 @@ def test_admission_retained_bytes_count_until_freed():
         Admission(1, slots=0)
 
 
+def test_admission_batch_memory_counted_until_the_batch_is_dropped():
+    """A batch of two 2D jobs (no slot) finishes while a slot job runs.  Its
+    jobs are released together with their whole footprints retained, so a job
+    that needs that memory waits until the server frees the batch."""
+    jobs, fp = _jobs([30, 30, 30, 60])
+    batched = {0, 1}
+    adm = Admission(100, slots=1, footprint=lambda j: fp[j.id], takes_slot=lambda j: j.id not in batched)
+    pending = collections.deque(jobs)
+    got, _ = adm.admit_from(pending)
+    assert [j.id for j in got] == [0, 1, 2] and adm.used == 90 and adm.slots_used == 1
+    adm.release_batch(got[:2])
+    assert adm.used == 30 and adm.retained == 60
+    # job 3 (60) fits in the budget only once the batch's 60 retained bytes are freed
+    assert adm.admit_from(pending) == ([], [])
+    assert not adm.blocked_by_retained(pending)  # the slot is still taken by job 2
+    adm.release(got[2], retain=0)
+    assert adm.blocked_by_retained(pending)
+    assert adm.admit_from(pending) == ([], [])
+    adm.free_retained()
+    got, _ = adm.admit_from(pending)
+    assert [j.id for j in got] == [3]
+
+
 def test_footprint_bounds_the_allocations():
     for res, prec in (((64, 64), 1), ((128, 128), 0), ((32, 32, 32), 1), ((48, 48, 48), 0)):
         job = Job(id=0, res=res, steps=1, seed=0)
