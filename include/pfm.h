@@ -221,6 +221,12 @@ int pfm_split_central(const pfm_grid* g, const float* vc, float* u_x, float* u_y
 int pfm_get_central(const pfm_grid* g, const float* u_x, const float* u_y, const float* u_z,
                     float* vc, pfm_stream_t stream);
 
+/* add_fields: 3D/taichi_utils.py:18-21 (2D/taichi_utils.py has the same
+ * kernel): dest[i] = f1[i] + multiplier * f2[i] over n floats (any field,
+ * e.g. the 4-channel smoke; dest may alias f1 or f2). */
+int pfm_add_fields(int64_t n, const float* f1, const float* f2, float* dest, float multiplier,
+                   pfm_stream_t stream);
+
 /* curl: 3D/taichi_utils.py:56-78 (vc (3, cells) -> out (3, cells)) and
  * 2D/taichi_utils.py:97-103 (vc (2, cells) -> out (cells)); central
  * differences with sample() clamping at the walls. */

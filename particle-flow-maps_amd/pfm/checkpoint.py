@@ -5,6 +5,9 @@
   3D/run_3D.py:699-701,846-849 and read back at :676-680 (2D/run_2D.py:726-728,
   753-755, 956-958).  A face tensor here has exactly the reference's numpy
   shape and order, so the files are interchangeable.
+* Reference-style restart (``resume_from_frame``): velocity (and smoke) from a
+  frame's files, flow maps restarted, as the reference does with
+  ``from_frame > 0``.
 * Full state (absent in the reference, which can only restart from the
   velocity): every array a step reads -- velocities, particle positions, the
   flow maps, impulses and C -- one ``.npy`` each, plus ``state.json`` with the
@@ -46,6 +49,47 @@ def load_velocity(ckptdir: str, frame: int, u: Sequence[torch.Tensor]) -> None:
         if a.dtype != np.float32 or tuple(a.shape) != tuple(t.shape):
             raise ValueError(f"{path}: expected float32 {tuple(t.shape)}, got {a.dtype} {a.shape}")
         t.copy_(torch.from_numpy(a))
+
+
+def smoke_file(ckptdir: str, frame: int) -> str:
+    return os.path.join(ckptdir, f"smoke_numpy_{frame}.npy")
+
+
+def save_smoke(ckptdir: str, frame: int, smoke: torch.Tensor) -> None:
+    """np.save of the cell smoke in the reference's layout: a 4-vector field's
+    to_numpy() is (*res, 4); the device field here is (4, *res)."""
+    os.makedirs(ckptdir, exist_ok=True)
+    s = smoke.detach().to("cpu", torch.float32)
+    np.save(smoke_file(ckptdir, frame), np.ascontiguousarray(np.moveaxis(s.numpy(), 0, -1)))
+
+
+def load_smoke(ckptdir: str, frame: int, smoke: torch.Tensor) -> None:
+    """smoke.from_numpy(np.load(... "smoke_numpy_" + str(frame) + ".npy"))
+    (3D/run_3D.py:680, 2D/run_2D.py:728) into a (4, *res) field, in place."""
+    path = smoke_file(ckptdir, frame)
+    a = np.load(path, allow_pickle=False)
+    want = tuple(smoke.shape[1:]) + (smoke.shape[0],)
+    if a.dtype != np.float32 or tuple(a.shape) != want:
+        raise ValueError(f"{path}: expected float32 {want}, got {a.dtype} {a.shape}")
+    smoke.copy_(torch.from_numpy(np.ascontiguousarray(np.moveaxis(a, -1, 0))))
+
+
+def resume_from_frame(sim, ckptdir: str, from_frame: int, smoke=None) -> None:
+    """The reference's restart with ``from_frame > 0`` (3D/run_3D.py:676-680,
+    708-713; 2D/run_2D.py:726-738): the face velocities (and, when given, the
+    smoke) are read from the frame's files, not projected again, and the flow
+    maps restart: the schedule counters start over (i = -1, ik = 0, sub_t =
+    0, frame_idx = from_frame), so the first step reseeds the particles and
+    reinitialises every map; a 2D scene also runs its particle set-up
+    (initialize_2d_particles) as the 2D driver does before its loop."""
+    if from_frame <= 0:
+        raise ValueError("from_frame must be positive (frame 0 starts from the initial condition)")
+    load_velocity(ckptdir, from_frame, sim.u)
+    if smoke is not None:
+        load_smoke(ckptdir, from_frame, smoke)
+    sim.i, sim.ik, sim.sub_t, sim.frame_idx, sim.last_dt = -1, 0, 0.0, int(from_frame), 0.0
+    if sim.dim == 2:
+        sim.initialize_2d_particles()
 
 
 def _state_arrays(sim):

@@ -475,6 +475,15 @@ def vortex_ring(res, dx, vc, w, rad, delta, c, ex, ey, num_samples=2000, stream=
            xx.ctypes.data, yy.ctypes.data, int(num_samples), L.ptr(vc, name="vc"), _stream(stream))
 
 
+def add_fields(f1, f2, dest, multiplier, stream=None):
+    """add_fields (3D/taichi_utils.py:18-21, the same kernel in 2D):
+    dest = f1 + multiplier * f2 elementwise over same-shaped float32 fields."""
+    if not (f1.shape == f2.shape == dest.shape):
+        raise L.PfmError(f"add_fields shapes differ: {tuple(f1.shape)}, {tuple(f2.shape)}, {tuple(dest.shape)}")
+    L.call("pfm_add_fields", int(dest.numel()), L.ptr(f1, name="f1"), L.ptr(f2, name="f2"),
+           L.ptr(dest, name="dest"), float(multiplier), _stream(stream))
+
+
 _KE_SCRATCH = {}
 
 

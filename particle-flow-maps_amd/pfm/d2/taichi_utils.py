@@ -22,3 +22,9 @@ def get_central_vector(horizontal, vertical, central):
 def curl(vf, cf, dx):
     """2D/taichi_utils.py:97-103: vf is cell-centred (2, rx, ry), cf (rx, ry)."""
     core.curl(tuple(vf.shape[1:]), dx, vf, cf)
+
+
+def add_fields(f1, f2, dest, multiplier):
+    """2D/taichi_utils.py:13-16: dest = f1 + multiplier * f2 (e.g. the smoke fields of the
+    initial conditions)."""
+    core.add_fields(f1, f2, dest, multiplier)

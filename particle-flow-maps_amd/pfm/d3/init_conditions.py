@@ -79,3 +79,68 @@ def coaxial_rings(res, rings, num_samples=2000, device="cuda"):
     for r in rings:
         add_vortex_ring(r["w"], r["rad"], r["delta"], r["c"], r["unit_x"], r["unit_y"], None, vc, num_samples)
     return vc
+
+
+def _unit(v):
+    n = math.sqrt(sum(x * x for x in v))
+    return [x / n for x in v]
+
+
+def _ring_and_smoke(u, smoke, w, rad, delta, c, unit_x, unit_y, color, num_samples):
+    add_vortex_ring_and_smoke(w=w, rad=rad, delta=delta, c=c, unit_x=unit_x, unit_y=unit_y, pf=None, vf=u,
+                              smokef=smoke, color=color, num_samples=num_samples)
+
+
+def init_vorts_leapfrog(X, u, smoke1, smoke2):
+    """3D/init_conditions.py:81-114 (the "front and back revised" pair, case 0):
+    two coaxial rings of radius 0.21 travelling along +x, core 0.08 r,
+    strength 0.1 r, centred at x = 0.16 and 0.16 + 0.625 r (y = z = 0.5), each
+    with its own smoke colour (red, blue); smoke1 ends as the sum of both.
+    u (3, *res) is the cell-centred velocity the rings are added to, smoke1/2
+    (4, *res); X (the cell centres) is implied by the grid.  The reference's
+    smoke2 is a fresh (zero) field."""
+    radius = 0.21
+    x_gap = 0.625 * radius
+    x_start = 0.16
+    delta = 0.08 * radius
+    w = radius * 0.1
+    ex, ey = [0.0, 0.0, -1.0], [0.0, 1.0, 0.0]
+    _ring_and_smoke(u, smoke1, w, radius, delta, [x_start, 0.5, 0.5], ex, ey, [1.0, 0.0, 0.0], 2000)
+    _ring_and_smoke(u, smoke2, w, radius, delta, [x_start + x_gap, 0.5, 0.5], ex, ey, [0.0, 0.0, 1.0], 2000)
+    core.add_fields(smoke1, smoke2, smoke1, 1.0)
+
+
+def init_vorts_eight(X, u, smoke1, smoke2):
+    """3D/init_conditions.py:119-180 (case 1): eight rings of radius 0.08, core
+    0.024, strength +-0.02 (500 samples each) at the corners of the cube
+    0.5 +- 0.16, each in the plane normal to its corner's diagonal, with one
+    smoke colour per ring; smoke1 accumulates every ring's smoke, smoke2 is
+    the per-ring scratch (it ends holding the last ring's smoke)."""
+    smoke1.zero_()
+    smoke2.zero_()
+    size = 0.08
+    cos45 = math.cos(math.pi / 4)
+    tan_theta = math.sqrt(2) / 2
+    cot_theta = math.sqrt(2)
+    off = (0.16, 0.16, 0.16 / cos45 * tan_theta)
+    c45 = cos45
+    # (w sign, corner signs (x, y, z), unit_x, unit_y, colour) per ring, in the reference's order
+    rings = [
+        (+1, (-1, -1, -1), (-c45, c45, 0.0), (-c45, -c45, cot_theta), (1, 0, 0)),
+        (-1, (+1, -1, -1), (c45, c45, 0.0), (c45, -c45, cot_theta), (0, 1, 0)),
+        (+1, (-1, +1, -1), (c45, c45, 0.0), (-c45, c45, cot_theta), (0, 0, 1)),
+        (-1, (+1, +1, -1), (-c45, c45, 0.0), (c45, c45, cot_theta), (0, 0, 0)),
+        (+1, (-1, -1, +1), (-c45, c45, 0.0), (c45, c45, cot_theta), (0, 1, 1)),
+        (-1, (+1, -1, +1), (c45, c45, 0.0), (-c45, c45, cot_theta), (1, 0, 1)),
+        (+1, (-1, +1, +1), (c45, c45, 0.0), (c45, -c45, cot_theta), (1, 1, 0)),
+        (-1, (+1, +1, +1), (-c45, c45, 0.0), (-c45, -c45, cot_theta), (1, 1, 1)),
+    ]
+    for k, (sgn, corner, ex, ey, color) in enumerate(rings):
+        c = [0.5 + s * o for s, o in zip(corner, off)]
+        target = smoke1 if k == 0 else smoke2
+        _ring_and_smoke(u, target, sgn * 2.0e-2, size, 0.024, c, _unit(ex), _unit(ey), [float(x) for x in color],
+                        500)
+        if k > 0:
+            core.add_fields(smoke1, smoke2, smoke1, 1.0)
+            if k < len(rings) - 1:  # the reference leaves the last ring's smoke in smoke2
+                smoke2.zero_()

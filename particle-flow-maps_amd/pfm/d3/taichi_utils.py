@@ -22,3 +22,9 @@ def get_central_vector(vx, vy, vz, vc):
 def curl(vf, cf, dx):
     """3D/taichi_utils.py:56-78: vf, cf are cell-centred (3, rx, ry, rz)."""
     core.curl(tuple(vf.shape[1:]), dx, vf, cf)
+
+
+def add_fields(f1, f2, dest, multiplier):
+    """3D/taichi_utils.py:18-21: dest = f1 + multiplier * f2 (e.g. the smoke fields of the
+    initial conditions)."""
+    core.add_fields(f1, f2, dest, multiplier)
