@@ -287,6 +287,15 @@ int pfm_clamp_smoke_particle(const pfm_grid* g, int64_t n, const int32_t* cell_s
                              const float* init_smoke, float* smoke, pfm_stream_t stream);
 int pfm_kinetic_energy(const pfm_grid* g, const float* u_x, const float* u_y, const float* u_z,
                        double* out, pfm_stream_t stream);
+/* pfm_kinetic_energy without a caller workspace and without any allocation:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, per-step serving loops).  `out` is a
+ * device double[1]; the per-block partial sums go to a fixed 24 KiB device
+ * array of the library (3 axes x 1024 doubles, one per GPU, kept until the
+ * process ends), so any grid size is accepted.  Calls that overlap on one GPU
+ * (different streams) race on that array and must be serialised. */
+int pfm_kinetic_energy_noalloc(const pfm_grid* g, const float* u_x, const float* u_y, const float* u_z,
+                               double* out, pfm_stream_t stream);
 
 /* ---- MGPCG pressure solver --------------------------------------------- */
 

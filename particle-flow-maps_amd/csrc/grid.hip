@@ -4,6 +4,14 @@
 // and the kinetic-energy diagnostic.  One thread per face or cell.
 #include "pfm_batch.h"
 
+#include <mutex>
+
+// Device scratch of pfm_kinetic_energy_noalloc: the FP64 per-block partial
+// sums of the three face axes (3 x 1024 doubles = 24 KiB), held by every GPU
+// the library runs on until the process ends.  Host code never passes this
+// name to a kernel: it uses the per-device address from hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[3 * 1024];
+
 namespace pfm {
 
 // Decode a flat face index of axis `a` into (i, j, k) (last axis fastest).
@@ -1059,3 +1067,55 @@ int pfm_kinetic_energy(const pfm_grid* gp, const float* u_x, const float* u_y, c
 
 }  // extern "C"
 
+namespace {
+
+// Device address of `scratch` on each GPU, looked up once per device with that
+// device current (every GPU holds its own copy of the array).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(double** out) {
+    int dev = -1;
+    PFM_HIP_TRY(hipGetDevice(&dev));
+    PFM_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d devices the scratch table holds", dev,
+                  kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        PFM_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PFM_CHECK_ARG(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return PFM_OK;
+}
+
+}  // namespace
+
+extern "C" int pfm_kinetic_energy_noalloc(const pfm_grid* gp, const float* u_x, const float* u_y,
+                                          const float* u_z, double* out, pfm_stream_t stream) {
+    GridDev g;
+    int rc = make_grid(gp, &g);
+    if (rc) return rc;
+    PFM_CHECK_ARG(u_x && u_y && (g.dim == 2 || u_z) && out, "null pointer");
+    // The partials of every axis fill all kRedBlocks slots of their row
+    // (one per block of k_sumsq) before k_ke_finish reads them, so nothing
+    // left by an earlier call is read and nothing needs clearing.
+    static_assert(sizeof(scratch) / sizeof(double) >= 3 * kRedBlocks, "scratch holds 3 x kRedBlocks partials");
+    double* partial = nullptr;
+    rc = scratch_for_current_device(&partial);
+    if (rc) return rc;
+    const float* u[3] = {u_x, u_y, u_z};
+    double nf[3] = {1, 1, 1};
+    hipStream_t s = as_stream(stream);
+    for (int a = 0; a < g.dim; ++a) {
+        nf[a] = (double)face_count_h(g, a);
+        hipLaunchKernelGGL(k_sumsq, dim3(kRedBlocks), dim3(kBlock), 0, s, u[a], face_count_h(g, a),
+                           partial + a * kRedBlocks);
+        PFM_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_ke_finish, dim3(1), dim3(kBlock), 0, s, partial, g.dim, nf[0], nf[1], nf[2], out);
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}

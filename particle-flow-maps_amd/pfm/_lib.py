@@ -121,6 +121,7 @@ _SIGS = {
     "pfm_point_vortices": [c_vp, c_int, c_vp, c_vp, c_vp, c_f, c_f, c_vp, c_vp],
     "pfm_vortex_ring": [c_vp, c_f, c_f, c_f, c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
     "pfm_kinetic_energy": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "pfm_kinetic_energy_noalloc": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "pfm_add_fields": [c_i64, c_vp, c_vp, c_vp, c_f, c_vp],
     "pfm_advect_smoke": [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp],
     "pfm_vortex_ring_smoke": [c_vp, c_f, c_f, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp],
