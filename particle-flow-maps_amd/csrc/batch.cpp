@@ -660,25 +660,28 @@ int pfm_batch2d_create(const pfm_batch2d_desc* desc, void** handle, int64_t* byt
     if ((e = hipStreamCreateWithFlags(&b->bs, hipStreamNonBlocking)) != hipSuccess) return fail_hip(e);
     if ((e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming)) != hipSuccess) return fail_hip(e);
     if ((e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming)) != hipSuccess) return fail_hip(e);
-    if ((e = hipMemcpy(b->sc, b->hs.data(), sizeof(BScene) * b->S, hipMemcpyHostToDevice)) != hipSuccess)
+    // set-up copies on the batch's own non-blocking stream: the legacy stream
+    // may not be used while another thread captures a graph
+    hipStream_t bs = b->bs;
+    if ((e = hipMemcpyAsync(b->sc, b->hs.data(), sizeof(BScene) * b->S, hipMemcpyHostToDevice, bs)) != hipSuccess)
         return fail_hip(e);
-    if ((e = hipMemcpy(b->units, U.data(), sizeof(BUnit) * U.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpyAsync(b->units, U.data(), sizeof(BUnit) * U.size(), hipMemcpyHostToDevice, bs)) != hipSuccess)
         return fail_hip(e);
-    if ((e = hipMemcpy(b->cbeg, hbeg.data(), sizeof(int32_t) * hbeg.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpyAsync(b->cbeg, hbeg.data(), sizeof(int32_t) * hbeg.size(), hipMemcpyHostToDevice, bs)) !=
+        hipSuccess)
         return fail_hip(e);
-    if ((e = hipMemset(b->ccount, 0, sizeof(int32_t) * b->n_clists)) != hipSuccess) return fail_hip(e);
-    {
-        std::vector<BState> st0(b->S);
-        for (auto& x : st0) {
-            x = BState{};
-            x.enabled = 1;
-        }
-        if ((e = hipMemcpy(b->st, st0.data(), sizeof(BState) * b->S, hipMemcpyHostToDevice)) != hipSuccess)
-            return fail_hip(e);
+    if ((e = hipMemsetAsync(b->ccount, 0, sizeof(int32_t) * b->n_clists, bs)) != hipSuccess) return fail_hip(e);
+    std::vector<BState> st0(b->S);
+    for (auto& x : st0) {
+        x = BState{};
+        x.enabled = 1;
     }
-    if ((e = hipMemset(b->mg, 0, sizeof(BMg) * b->S)) != hipSuccess) return fail_hip(e);
-    if ((e = hipMemset(b->iters, 0, sizeof(int32_t) * 2 * b->S)) != hipSuccess) return fail_hip(e);
-    if ((e = hipMemset(b->conv, 0, sizeof(int32_t) * 2 * b->S)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMemcpyAsync(b->st, st0.data(), sizeof(BState) * b->S, hipMemcpyHostToDevice, bs)) != hipSuccess)
+        return fail_hip(e);
+    if ((e = hipMemsetAsync(b->mg, 0, sizeof(BMg) * b->S, bs)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMemsetAsync(b->iters, 0, sizeof(int32_t) * 2 * b->S, bs)) != hipSuccess) return fail_hip(e);
+    if ((e = hipMemsetAsync(b->conv, 0, sizeof(int32_t) * 2 * b->S, bs)) != hipSuccess) return fail_hip(e);
+    if ((e = hipStreamSynchronize(bs)) != hipSuccess) return fail_hip(e);
     if (bytes_out) *bytes_out = bytes;
     *handle = b;
     return PFM_OK;
@@ -723,16 +726,17 @@ int pfm_batch2d_step(void* h, pfm_batch2d_state* st, pfm_stream_t stream) {
 int pfm_batch2d_scene_info(void* h, double* dt, double* sub_t, int32_t* frame, int32_t* iters, int32_t* converged) {
     PFM_CHECK_ARG(h, "null batch");
     Batch* b = (Batch*)h;
-    PFM_HIP_TRY(hipStreamSynchronize(b->bs));
     std::vector<BState> st(b->S);
-    PFM_HIP_TRY(hipMemcpy(st.data(), b->st, sizeof(BState) * b->S, hipMemcpyDeviceToHost));
+    PFM_HIP_TRY(hipMemcpyAsync(st.data(), b->st, sizeof(BState) * b->S, hipMemcpyDeviceToHost, b->bs));
+    if (iters) PFM_HIP_TRY(hipMemcpyAsync(iters, b->iters, sizeof(int32_t) * 2 * b->S, hipMemcpyDeviceToHost, b->bs));
+    if (converged)
+        PFM_HIP_TRY(hipMemcpyAsync(converged, b->conv, sizeof(int32_t) * 2 * b->S, hipMemcpyDeviceToHost, b->bs));
+    PFM_HIP_TRY(hipStreamSynchronize(b->bs));
     for (int s = 0; s < b->S; ++s) {
         if (dt) dt[s] = st[s].dt;
         if (sub_t) sub_t[s] = st[s].sub_t;
         if (frame) frame[s] = st[s].frame;
     }
-    if (iters) PFM_HIP_TRY(hipMemcpy(iters, b->iters, sizeof(int32_t) * 2 * b->S, hipMemcpyDeviceToHost));
-    if (converged) PFM_HIP_TRY(hipMemcpy(converged, b->conv, sizeof(int32_t) * 2 * b->S, hipMemcpyDeviceToHost));
     return PFM_OK;
 }
 
@@ -815,8 +819,8 @@ int pfm_batch2d_set_enabled(void* h, int32_t scene, int32_t enabled) {
     Batch* b = (Batch*)h;
     PFM_CHECK_ARG(scene >= 0 && scene < b->S, "scene %d out of [0, %d)", scene, b->S);
     const int32_t v = enabled ? 1 : 0;
+    PFM_HIP_TRY(hipMemcpyAsync(&b->st[scene].enabled, &v, sizeof(int32_t), hipMemcpyHostToDevice, b->bs));
     PFM_HIP_TRY(hipStreamSynchronize(b->bs));
-    PFM_HIP_TRY(hipMemcpy(&b->st[scene].enabled, &v, sizeof(int32_t), hipMemcpyHostToDevice));
     return PFM_OK;
 }
 

@@ -22,6 +22,7 @@
 //     per iteration is one 8-byte read of r.r for the convergence test;
 //   - the coarsest level's 50 red/black sweeps run in one workgroup out of LDS.
 #include <algorithm>
+#include <cstddef>
 #include <new>
 #include <type_traits>
 
@@ -49,7 +50,17 @@ struct MGScalars {
     // vs * r (vs a power of two near 1/rms(r), exact to apply and undo) and
     // its output is read back times vinv = 1/vs; keeps values in fp16 range
     float vs, vinv;
+    // Convergence on the device (solve): the threshold on r.r (tol, or
+    // tol^2 r0 for a relative tolerance), the iteration counter and the stop
+    // flags.  Once `done` is set every kernel of the iterations still
+    // enqueued returns at once, so the host can enqueue several iterations
+    // (or replay them from a graph) and read the outcome once.
+    double tol, thresh;
+    int tol_mode, max_it, it, done, conv;
 };
+
+// Kernels of a PCG iteration return immediately once the solve has stopped.
+__device__ __forceinline__ bool pcg_stopped(const MGScalars* sc) { return sc != nullptr && sc->done != 0; }
 
 struct Level {
     int d[3];      // (1, n0, n1) for a 2D level
@@ -85,7 +96,30 @@ struct MGHandle {
     uint8_t* solid[kMaxLevels];
     uint8_t* dg[kMaxLevels];
     int64_t n_fluid;        // cells that carry pressure (the recentring mean)
+    // PCG iterations on the device: the first group of a solve enqueues as
+    // many iterations as the previous solve took, then one per group, with
+    // one host read per group.  Each iteration can be replayed from a HIP
+    // graph (one per p-buffer parity and replacement step), captured on a
+    // private non-blocking stream.
+    int last_iters;
+    int use_graphs;
+    float* p_base;          // the allocation first held by p (graph parity key)
+    hipStream_t cap_stream;
+    hipGraphExec_t iter_graph[2][2];
+    int graph_failed;
+    double graph_eps;       // the eps baked into the captured launches
 };
+
+// Captured iterations hold the level descriptors (obstacle masks), the fluid
+// cell count and eps by value: drop them when any of those changes.
+static void drop_graphs(MGHandle* h) {
+    for (auto& row : h->iter_graph)
+        for (auto& ex : row)
+            if (ex) {
+                (void)hipGraphExecDestroy(ex);
+                ex = nullptr;
+            }
+}
 
 // Stencil description of one level, always in 3D index form.
 struct LvlDev {
@@ -284,6 +318,7 @@ template <typename ZT, typename RT>
 __global__ void __launch_bounds__(kBlock) k_restrict_any(LvlDev Lf, LvlDev Lc, float inv_scale,
                                                          const ZT* __restrict__ z, const RT* __restrict__ r,
                                                          ZT* __restrict__ rc, const MGScalars* __restrict__ sc) {
+    if (pcg_stopped(sc)) return;
     int C[3];
     if (!cell_of_thread(Lc, 0, false, C)) return;
     const int64_t qc = cell_index(Lc, C);
@@ -437,6 +472,7 @@ __global__ void __launch_bounds__(kSweepThreads, 3) k_rbgs(LvlDev L, LvlDev Lc, 
                                                         const RT* __restrict__ r, ZT* __restrict__ zout, int c0,
                                                         const MGScalars* __restrict__ sc,
                                                         double* __restrict__ partial) {
+    if (pcg_stopped(sc)) return;
     using T = MgTile<DIM>;
     constexpr int NT = kSweepThreads;
     constexpr int H = 2, HI = DIM == 3 ? H : 0, A0 = DIM == 3 ? 1 : 0;
@@ -685,6 +721,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc,
                                                            const ZT* __restrict__ z, RT* __restrict__ r,
                                                            ZT* __restrict__ rc,
                                                            const MGScalars* __restrict__ sc) {
+    if (pcg_stopped(sc)) return;
     using C = RsTile<DIM>;
     constexpr int FI = DIM == 3 ? 2 * C::I : 1, FJ = 2 * C::J, FK = 2 * C::K, NF = FI * FJ * FK;
     constexpr int HI = DIM == 3 ? 1 : 0;
@@ -774,6 +811,7 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
                                                      const float* __restrict__ p_old, float* __restrict__ p_new,
                                                      float* __restrict__ Ap, const MGScalars* __restrict__ sc,
                                                      double* __restrict__ partial) {
+    if (pcg_stopped(sc)) return;
     using T = MgTile<DIM>;
     constexpr int HI = DIM == 3 ? 1 : 0;
     constexpr int SI = T::I + 2 * HI, SJ = T::J + 2, SK = T::K + 2, SP = SJ * SK, NS = SI * SP;
@@ -953,6 +991,7 @@ __device__ void coarse_prolong(const LvlDev& Lf, const LvlDev& Lc, float* zs, co
 template <int DIM, typename ZT, typename R0T, typename BT>
 __global__ void __launch_bounds__(kCoarseThreads) k_coarse(CoarseArgs a, int sub_mean, int dot,
                                                    const MGScalars* __restrict__ sc, double* __restrict__ partial) {
+    if (pcg_stopped(sc)) return;
     __shared__ float sm[kCoarseLds];
     const int tid = threadIdx.x;
     for (int l = 0; l < a.nl; ++l)
@@ -1161,6 +1200,7 @@ __global__ void __launch_bounds__(kBlock) k_update_xr(int64_t n, float* __restri
                                                       const float* __restrict__ Ap,
                                                       const MGScalars* __restrict__ sc,
                                                       double* __restrict__ partial) {
+    if (pcg_stopped(sc)) return;
     __shared__ double sh[kBlock / 64];
     const float alpha = sc->alpha;
     double s2 = 0.0, s1 = 0.0, s3 = 0.0;
@@ -1189,6 +1229,7 @@ template <typename ZT>
 __global__ void __launch_bounds__(kBlock) k_update_p(int64_t n, float* __restrict__ p,
                                                      const ZT* __restrict__ z,
                                                      const MGScalars* __restrict__ sc) {
+    if (pcg_stopped(sc)) return;
     const float beta = sc->beta;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x)
@@ -1198,6 +1239,7 @@ __global__ void __launch_bounds__(kBlock) k_update_p(int64_t n, float* __restric
 __global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t n, float* __restrict__ r,
                                                      const MGScalars* __restrict__ sc,
                                                      const uint8_t* __restrict__ dg) {
+    if (pcg_stopped(sc)) return;
     const float m = sc->mean;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x)
@@ -1221,8 +1263,12 @@ enum FinalizeMode { kFinMean = 0, kFinR0 = 1, kFinZTr = 2, kFinAlpha = 3, kFinRT
 
 // Fixed-order reduction of the per-block partials and the scalar recurrences
 // of solve() (3D/mgpcg.py:229-275), all in double.
+// test: after r.r, stop the solve when it is below the threshold (the
+// convergence test of 3D/mgpcg.py:255); kFinBeta counts the iteration and
+// stops the solve at max_it.
 __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ partial, int nparts, int mode,
-                                                     double eps, double ncell, int vshift, MGScalars* sc) {
+                                                     double eps, double ncell, int vshift, int test, MGScalars* sc) {
+    if (pcg_stopped(sc)) return;
     __shared__ double sh[kBlock / 64];
     double s = 0.0;
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += partial[b];
@@ -1245,6 +1291,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ 
         case kFinR0:
             sc->r0 = s;
             sc->rTr = s;
+            if (sc->tol_mode == 1) sc->thresh = sc->tol * sc->tol * s;
             set_vscale(sc, s, ncell, vshift);
             break;
         case kFinZTr:
@@ -1262,22 +1309,35 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ 
             sc->sum_r = s1;
             sc->mean = (float)(s1 / ncell);
             set_vscale(sc, s, ncell, vshift);
+            if (test && s < sc->thresh) {
+                sc->done = 1;
+                sc->conv = 1;
+            }
             break;
         case kFinBeta: {
             sc->new_zTr = s;
             sc->beta = (float)(s / (sc->zTr + eps));
             sc->zTr = s;
+            sc->it += 1;
+            if (sc->max_it >= 0 && sc->it >= sc->max_it) sc->done = 1;
             break;
         }
     }
 }
 
-__global__ void k_reset_scalars(MGScalars* sc) {
+__global__ void k_reset_scalars(MGScalars* sc, double tol, int tol_mode, int max_it) {
     float beta = sc->beta;  // the reference's beta field persists across solves
     *sc = MGScalars{};
     sc->beta = beta;
     sc->vs = sc->vinv = 1.0f;
+    sc->tol = tol;
+    sc->thresh = tol;  // tol_mode 1: tol^2 r0, set with r0
+    sc->tol_mode = tol_mode;
+    sc->max_it = max_it;
 }
+
+// The solve has ended: kernels run normally again.
+__global__ void k_clear_stop(MGScalars* sc) { sc->done = 0; }
 
 
 // ---- field kernels (velocity <-> level 0) ---------------------------------
@@ -1364,9 +1424,12 @@ __global__ void __launch_bounds__(kBlock) k_zero_plane(int S0, int S1, int S2, i
     u[((int64_t)I[0] * S[1] + I[1]) * S[2] + I[2]] = 0.0f;
 }
 
-// r = b - A x: the true-residual replacement of the mixed-precision PCG.
+// r = b - A x: the true-residual replacement of the mixed-precision PCG,
+// made only while r.r exceeds floor2 x.x (see kFloorMargin).
 __global__ void __launch_bounds__(kBlock) k_true_residual(LvlDev L, const float* __restrict__ b,
-                                                          const float* __restrict__ x, float* __restrict__ r) {
+                                                          const float* __restrict__ x, float* __restrict__ r,
+                                                          const MGScalars* __restrict__ sc, double floor2) {
+    if (pcg_stopped(sc) || !(sc->rTr > floor2 * sc->xTx)) return;
     int I[3];
     if (!cell_of_thread(L, 0, false, I)) return;
     const int64_t q = cell_index(L, I);
@@ -1777,9 +1840,9 @@ static int vcycle(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
     }
 }
 
-static int finalize(MGHandle* h, int mode, double eps, hipStream_t s, int nparts = kMgRedBlocks) {
+static int finalize(MGHandle* h, int mode, double eps, hipStream_t s, int nparts = kMgRedBlocks, int test = 0) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, h->partial, nparts, mode, eps,
-                       (double)h->n_fluid, h->vshift, h->sc);
+                       (double)h->n_fluid, h->vshift, test, h->sc);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -1883,11 +1946,112 @@ constexpr double kFloorMargin = 16.0;
 // launch (same arithmetic, same beta), or explicitly when the loop ends
 // without converging.  At precision 1/2 (16-bit V-cycle storage) the PCG
 // residual is replaced by the true residual b - A x every kReplaceEvery
-// iterations.
-static int read_rtr(MGHandle* h, hipStream_t s) {
+// iterations.  Convergence is tested on the device (k_finalize): the host
+// reads the outcome once per group of iterations.
+
+// Host copy of the device outcome of a solve: r.r, x.x, iterations, flags.
+struct SolveOutcome {
+    double rtr, xtx;
+    int it, done, conv;
+};
+
+static int read_outcome(MGHandle* h, hipStream_t s, SolveOutcome* o) {
+    static_assert(offsetof(MGScalars, xTx) == offsetof(MGScalars, rTr) + sizeof(double), "rTr, xTx adjacent");
     PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr, &h->sc->rTr, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr + 2, &h->sc->it, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     PFM_HIP_TRY(hipStreamSynchronize(s));
+    const int* iv = (const int*)(h->host_rtr + 2);
+    *o = SolveOutcome{h->host_rtr[0], h->host_rtr[1], iv[0], iv[1], iv[2]};
     return PFM_OK;
+}
+
+// One PCG iteration (3D/mgpcg.py:236-275) as launches only: Ap and alpha,
+// the x / r update and the convergence test on the device, every
+// kReplaceEvery-th iteration of the mixed-precision solver the true-residual
+// replacement (taken on the device when r.r is above the representation
+// floor), then the V-cycle, z.r and beta.  `k` is the iteration's index.
+static int pcg_iteration(MGHandle* h, int k, double eps, bool neumann, hipStream_t s) {
+    const int64_t n = h->lv[0].n;
+    float* r0 = (float*)h->r[0];
+    const bool replace = h->precision != 0;
+    int parts = 0;
+    PFM_TRY(ap_dot_p(h, s, &parts));
+    PFM_TRY(finalize(h, kFinAlpha, eps, s, parts));
+    if (replace)
+        hipLaunchKernelGGL(k_update_xr<true>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, r0, h->p, h->Ap,
+                           h->sc, h->partial);
+    else
+        hipLaunchKernelGGL(k_update_xr<false>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, r0, h->p, h->Ap,
+                           h->sc, h->partial);
+    PFM_LAUNCH_CHECK();
+    const bool due = replace && (k + 1) % kReplaceEvery == 0;
+    PFM_TRY(finalize(h, replace ? kFinRTrX : kFinRTr, eps, s, kMgRedBlocks, due ? 0 : 1));
+    if (due) {
+        const LvlDev L0 = lvl_dev(h, 0);
+        const double floor_c = kFloorMargin * 2.0 * h->dim * 0x1p-24;
+        hipLaunchKernelGGL(k_true_residual, cell_grid(L0.d, false), kCellBlock, 0, s, L0, h->b, h->x, r0, h->sc,
+                           floor_c * floor_c);
+        PFM_LAUNCH_CHECK();
+        PFM_TRY(dot_t(h, r0, r0, s, h->partial));
+        PFM_TRY(dot_t(h, r0, (const float*)nullptr, s, h->partial + kMgRedBlocks));
+        PFM_TRY(finalize(h, kFinRTr, eps, s, kMgRedBlocks, 1));
+    }
+    PFM_TRY(precondition_dot(h, neumann, s, &parts));
+    PFM_TRY(finalize(h, kFinBeta, eps, s, parts));
+    return PFM_OK;
+}
+
+// The iteration from a HIP graph: captured once per (p-buffer parity,
+// replacement step) on the handle's own non-blocking stream in thread-local
+// capture mode (other threads' HIP calls do not disturb it), then replayed on
+// `s`.  The launches hold no memset, so replays match eager runs bit for bit.
+// Returns false (and the caller runs the iteration eagerly) when capture is
+// not possible.
+static bool pcg_iteration_graph(MGHandle* h, int k, double eps, bool neumann, hipStream_t s) {
+    if (!h->use_graphs || h->graph_failed) return false;
+    if (h->graph_eps != eps) {
+        drop_graphs(h);
+        h->graph_eps = eps;
+    }
+    const int par = h->p == h->p_base ? 0 : 1;
+    const int due = h->precision != 0 && (k + 1) % kReplaceEvery == 0;
+    hipGraphExec_t& ex = h->iter_graph[par][due];
+    if (!ex) {
+        if (!h->cap_stream && hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+            h->cap_stream = nullptr;
+            h->graph_failed = 1;
+            return false;
+        }
+        float* p0 = h->p;
+        float* p1 = h->p2;
+        hipGraph_t g = nullptr;
+        bool ok = hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            const int rc = pcg_iteration(h, k, eps, neumann, h->cap_stream);
+            const bool ended = hipStreamEndCapture(h->cap_stream, &g) == hipSuccess;
+            ok = rc == PFM_OK && ended && g != nullptr;
+        }
+        // capturing launched nothing: the buffers are as before
+        h->p = p0;
+        h->p2 = p1;
+        if (ok && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+            ex = nullptr;
+            ok = false;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        if (!ok) {
+            h->graph_failed = 1;
+            return false;
+        }
+    }
+    if (hipGraphLaunch(ex, s) != hipSuccess) {
+        (void)hipGetLastError();
+        h->graph_failed = 1;
+        return false;
+    }
+    std::swap(h->p, h->p2);  // the iteration's ap_dot_p wrote the partner buffer
+    return true;
 }
 
 static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mode, int* iters_out,
@@ -1895,7 +2059,7 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
     const int64_t n = h->lv[0].n;
     float* r0 = (float*)h->r[0];
     const bool neumann = all_neumann(h);
-    hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(1), 0, s, h->sc);
+    hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(1), 0, s, h->sc, tol, tol_mode, max_iters);
     PFM_LAUNCH_CHECK();
     if (neumann) {
         PFM_TRY(dot(h, r0, nullptr, s));
@@ -1905,59 +2069,37 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
     }
     const bool replace = h->precision != 0;
     if (replace) PFM_HIP_TRY(hipMemcpyAsync(h->b, r0, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    double thresh = tol;
-    if (tol_mode == 1 || replace) {  // r0 (and the V-cycle scale of 16-bit storage)
+    if (tol_mode == 1 || replace) {  // r0 (the relative threshold, and the V-cycle scale of 16-bit storage)
         PFM_TRY(dot(h, r0, r0, s));
         PFM_TRY(finalize(h, kFinR0, eps, s));
     }
-    if (tol_mode == 1) {
-        PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr, &h->sc->r0, sizeof(double), hipMemcpyDeviceToHost, s));
-        PFM_HIP_TRY(hipStreamSynchronize(s));
-        thresh = tol * tol * h->host_rtr[0];
-    }
-    const LvlDev L0 = lvl_dev(h, 0);
     int parts = 0;
     PFM_TRY(precondition_dot(h, false, s, &parts));
     PFM_TRY(finalize(h, kFinZTr, eps, s, parts));
-    int it = 0;
-    double rtr = 0.0;
-    bool converged = false;
-    const double floor_c = kFloorMargin * 2.0 * h->dim * 0x1p-24;
-    while (max_iters == -1 || it < max_iters) {
-        PFM_TRY(ap_dot_p(h, s, &parts));
-        PFM_TRY(finalize(h, kFinAlpha, eps, s, parts));
-        if (replace)
-            hipLaunchKernelGGL(k_update_xr<true>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, r0, h->p, h->Ap,
-                               h->sc, h->partial);
-        else
-            hipLaunchKernelGGL(k_update_xr<false>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, r0, h->p, h->Ap,
-                               h->sc, h->partial);
-        PFM_LAUNCH_CHECK();
-        PFM_TRY(finalize(h, replace ? kFinRTrX : kFinRTr, eps, s));
-        PFM_TRY(read_rtr(h, s));
-        rtr = h->host_rtr[0];
-        const double xtx = h->host_rtr[1];
-        if (replace && (it + 1) % kReplaceEvery == 0 && rtr > floor_c * floor_c * xtx) {
-            hipLaunchKernelGGL(k_true_residual, cell_grid(L0.d, false), kCellBlock, 0, s, L0, h->b, h->x, r0);
-            PFM_LAUNCH_CHECK();
-            PFM_TRY(dot_t(h, r0, r0, s, h->partial));
-            PFM_TRY(dot_t(h, r0, (const float*)nullptr, s, h->partial + kMgRedBlocks));
-            PFM_TRY(finalize(h, kFinRTr, eps, s));
-            PFM_TRY(read_rtr(h, s));
-            rtr = h->host_rtr[0];
-        }
-        if (rtr < thresh) {
-            converged = true;
-            break;
-        }
-        PFM_TRY(precondition_dot(h, neumann, s, &parts));
-        PFM_TRY(finalize(h, kFinBeta, eps, s, parts));
-        ++it;
+    // Iterations in groups: the first as long as the previous solve, then one
+    // at a time; each group ends with one read of the outcome.
+    int enq = 0;
+    SolveOutcome o{0.0, 0.0, 0, 0, 0};
+    while (max_iters == -1 || enq < max_iters) {
+        int g = enq == 0 ? std::max(1, h->last_iters) : 1;
+        if (max_iters != -1) g = std::min(g, max_iters - enq);
+        for (int j = 0; j < g; ++j, ++enq)
+            if (!pcg_iteration_graph(h, enq, eps, neumann, s)) PFM_TRY(pcg_iteration(h, enq, eps, neumann, s));
+        PFM_TRY(read_outcome(h, s, &o));
+        if (o.done) break;
     }
-    if (!converged) PFM_TRY(update_p(h, s));
-    if (iters_out) *iters_out = it;
-    if (rtr_out) *rtr_out = rtr;
-    return converged ? PFM_OK : PFM_WARN_NOT_CONVERGED;
+    if (enq == 0) PFM_TRY(read_outcome(h, s, &o));
+    // Every enqueued iteration swapped p and its partner on the host; the
+    // device ran ap_dot_p only until the solve stopped.
+    const int ran = o.conv ? o.it + 1 : o.it;
+    if ((enq - ran) & 1) std::swap(h->p, h->p2);
+    h->last_iters = ran;
+    hipLaunchKernelGGL(k_clear_stop, dim3(1), dim3(1), 0, s, h->sc);
+    PFM_LAUNCH_CHECK();
+    if (!o.conv) PFM_TRY(update_p(h, s));
+    if (iters_out) *iters_out = o.it;
+    if (rtr_out) *rtr_out = o.rtr;
+    return o.conv ? PFM_OK : PFM_WARN_NOT_CONVERGED;
 }
 
 static int apply_bc(MGHandle* h, float* const* u, hipStream_t s) {
@@ -2119,6 +2261,8 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     const size_t b0 = sizeof(float) * h->lv[0].n;
     if (hipMalloc(&h->x, b0) != hipSuccess) return fail("x");
     if (hipMalloc(&h->p, b0) != hipSuccess) return fail("p");
+    h->p_base = h->p;
+    h->use_graphs = 1;
     if (hipMalloc(&h->p2, b0) != hipSuccess) return fail("p scratch");
     if (hipMalloc(&h->Ap, b0) != hipSuccess) return fail("Ap");
     h->n_partial = std::max(3 * kMgRedBlocks, sweep_tiles(h, 0).total);
@@ -2126,18 +2270,26 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     if (hipMalloc(&h->sc, sizeof(MGScalars)) != hipSuccess) return fail("scalars");
     if (hipHostMalloc(&h->host_rtr, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess)
         return fail("pinned scalars");
-    PFM_HIP_TRY(hipMemset(h->sc, 0, sizeof(MGScalars)));
+    // The handle's own non-blocking stream (also the graph-capture stream)
+    // clears the arrays: nothing here touches the legacy stream, which other
+    // threads' graph captures forbid.
+    if (hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        h->cap_stream = nullptr;
+        return fail("stream");
+    }
+    hipStream_t cs = h->cap_stream;
+    PFM_HIP_TRY(hipMemsetAsync(h->sc, 0, sizeof(MGScalars), cs));
     for (int l = 0; l < L; ++l) {
         const size_t b = esz * h->lv[l].n;
-        PFM_HIP_TRY(hipMemset(h->r[l], 0, l == 0 ? sizeof(float) * h->lv[l].n : b));
-        PFM_HIP_TRY(hipMemset(h->z[l], 0, b));
-        PFM_HIP_TRY(hipMemset(h->z2[l], 0, b));
+        PFM_HIP_TRY(hipMemsetAsync(h->r[l], 0, l == 0 ? sizeof(float) * h->lv[l].n : b, cs));
+        PFM_HIP_TRY(hipMemsetAsync(h->z[l], 0, b, cs));
+        PFM_HIP_TRY(hipMemsetAsync(h->z2[l], 0, b, cs));
     }
-    PFM_HIP_TRY(hipMemset(h->x, 0, b0));
-    PFM_HIP_TRY(hipMemset(h->p, 0, b0));
-    PFM_HIP_TRY(hipMemset(h->p2, 0, b0));
-    PFM_HIP_TRY(hipMemset(h->Ap, 0, b0));
-    PFM_HIP_TRY(hipDeviceSynchronize());
+    PFM_HIP_TRY(hipMemsetAsync(h->x, 0, b0, cs));
+    PFM_HIP_TRY(hipMemsetAsync(h->p, 0, b0, cs));
+    PFM_HIP_TRY(hipMemsetAsync(h->p2, 0, b0, cs));
+    PFM_HIP_TRY(hipMemsetAsync(h->Ap, 0, b0, cs));
+    PFM_HIP_TRY(hipStreamSynchronize(cs));
     *handle = h;
     return PFM_OK;
 }
@@ -2145,6 +2297,8 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
 int pfm_mg_destroy(void* hv) {
     MGHandle* h = (MGHandle*)hv;
     if (!h) return PFM_OK;
+    drop_graphs(h);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     for (int l = 0; l < kMaxLevels; ++l) {
         if (h->r[l]) (void)hipFree(h->r[l]);
         if (h->z[l]) (void)hipFree(h->z[l]);
@@ -2166,10 +2320,20 @@ int pfm_mg_destroy(void* hv) {
 
 int pfm_mg_n_levels(void* hv) { return hv ? ((MGHandle*)hv)->n_levels : 0; }
 
+int pfm_mg_set_graphs(void* hv, int enable) {
+    MGHandle* h = (MGHandle*)hv;
+    PFM_CHECK_ARG(h, "null solver handle");
+    h->use_graphs = enable != 0;
+    h->graph_failed = 0;
+    if (!enable) drop_graphs(h);
+    return PFM_OK;
+}
+
 int pfm_mg_set_solid(void* hv, const uint8_t* solid, pfm_stream_t stream) {
     MGHandle* h = (MGHandle*)hv;
     PFM_CHECK_ARG(h, "null solver handle");
     hipStream_t s = as_stream(stream);
+    drop_graphs(h);  // the captured launches hold the old masks and fluid count
     if (!solid) {
         PFM_HIP_TRY(hipStreamSynchronize(s));
         for (int l = 0; l < kMaxLevels; ++l) {

@@ -130,6 +130,7 @@ _SIGS = {
     "pfm_clamp_smoke_particle": [c_vp, c_i64, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp],
     "pfm_mg_create": [c_int, c_vp, c_int, c_vp, c_int, ctypes.POINTER(c_vp)],
     "pfm_mg_destroy": [c_vp],
+    "pfm_mg_set_graphs": [c_vp, c_int],
     "pfm_mg_n_levels": [c_vp],
     "pfm_mg_field": [c_vp, c_int, c_int],
     "pfm_mg_apply_bc": [c_vp, c_vp, c_vp, c_vp, c_vp],

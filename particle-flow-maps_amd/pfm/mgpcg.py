@@ -83,6 +83,11 @@ class MGPCG:
             pass
 
     # ---- solid obstacle (extension) ----
+    def set_graphs(self, enable: bool):
+        """Replay each PCG iteration from a HIP graph (default) or issue its
+        launches directly; the results are the same."""
+        L.call("pfm_mg_set_graphs", self._h, int(bool(enable)))
+
     def set_solid(self, solid, stream=None):
         """Mark solid cells of the finest level (bool/uint8 CUDA tensor of shape N,
         nonzero = solid; None removes the obstacle).  Solid cells carry no

@@ -288,7 +288,7 @@ def test_stream_jobs_same_alone_and_in_the_mix(pfm):
     print(f"stream jobs {[j.res for j in jobs]}", flush=True)
     for batch_2d in (False, True):
         mixed = Server(slots=8, keep_fields=True, batch_2d=batch_2d).run(jobs, realtime=False)
-        assert mixed.summary()["failed"] == 0
+        assert mixed.summary()["failed"] == 0, [(r.job.id, r.error) for r in mixed.results if r.error]
         for job in jobs:
             print(f"  batch_2d={batch_2d} job {job.id} {job.res} alone", flush=True)
             alone = Server(slots=1, keep_fields=True, batch_2d=batch_2d).run([job], realtime=False)

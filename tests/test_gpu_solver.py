@@ -59,6 +59,44 @@ def test_mg_passes_bit_exact(oracle, pfm, N_):
     np.testing.assert_array_equal(N(mg_g.z[0]), mg_o.vcycle(r))
 
 
+@pytest.mark.parametrize("N_,prec,with_solid", [((32, 32, 32), 0, False), ((48, 32, 32), 1, True),
+                                                ((96, 80), 0, False), ((128, 128), 1, False)])
+def test_graph_replayed_iterations_equal_eager(pfm, N_, prec, with_solid):
+    """PCG iterations replayed from HIP graphs against the same launches
+    issued directly: equal iteration counts and equal bits, over several
+    solves with eager work and synchronisation between the replays (the
+    p buffers alternate, the first group follows the previous solve's length,
+    a mid-group convergence leaves the remaining launches idle), including a
+    solve stopped by max_iters and a mixed-precision true-residual step."""
+    from helpers import box_solid
+
+    d = len(N_)
+    solid = box_solid(N_, 0.4, 0.6) if with_solid else None
+    mgs = []
+    for graphs in (True, False):
+        mg = pfm.MGPCG([[2, 2]] * d, list(N_), dim=d, base_level=3, precision=prec)
+        mg.set_graphs(graphs)
+        if solid is not None:
+            mg.set_solid(G(solid))
+        mgs.append(mg)
+    for rep, (seed, tol, max_iters) in enumerate([(60, 1e-6, 400), (61, 1e-9, 400), (62, 1e-6, 3), (63, 1e-7, 400),
+                                                  (64, 1e-6, 400)]):
+        u0 = smooth_mac_field(N_, seed=seed)
+        out = []
+        for mg in mgs:
+            ug = [G(x) for x in u0]
+            it = mg.Poisson(*ug, tol=tol, tol_mode="rel", max_iters=max_iters)
+            torch.cuda.synchronize()
+            junk = torch.randn(1 << 16, device=DEV).sum()  # eager work between replays
+            out.append((it, mg.converged, [N(t) for t in ug], N(mg.p).copy(), float(junk)))
+        (ia, ca, ua, pa, _), (ib, cb, ub, pb, _) = out
+        assert ia == ib and ca == cb, (rep, ia, ib)
+        assert ia <= max_iters
+        for a in range(d):
+            np.testing.assert_array_equal(ua[a], ub[a])
+        np.testing.assert_array_equal(pa, pb)
+
+
 @pytest.mark.parametrize("N_,tol_mode,tol", [((64, 64), "abs", 1e-12), ((32, 32, 32), "rel", 1e-6),
                                              ((16, 16, 16), "abs", 1e-12), ((64, 32, 32), "rel", 1e-7)])
 def test_poisson_matches_oracle(oracle, pfm, N_, tol_mode, tol):
