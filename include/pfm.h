@@ -307,8 +307,8 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
                   void** handle);
 int pfm_mg_destroy(void* handle);
 int pfm_mg_n_levels(void* handle);
-/* 1 (default): replay each PCG iteration of pfm_mg_solve / pfm_mg_poisson
- * from a HIP graph captured once per handle; 0: issue the launches directly.
+/* 1: replay each PCG iteration of pfm_mg_solve / pfm_mg_poisson from a HIP
+ * graph captured once per handle; 0 (default): issue the launches directly.
  * Both run the same kernels in the same order (equal results).  No reference
  * counterpart. */
 int pfm_mg_set_graphs(void* handle, int enable);
@@ -343,8 +343,8 @@ int pfm_mg_init(void* handle, const float* rhs, float k, pfm_stream_t stream);
  * (the first as long as the handle's previous solve, then one at a time; the
  * kernels of iterations past convergence return at once) and the stream is
  * synchronised once per group (the reference does three host reads per
- * iteration).  Each iteration is replayed from a HIP graph captured on the
- * handle's own stream (pfm_mg_set_graphs).  Returns PFM_OK or
+ * iteration).  With pfm_mg_set_graphs(h, 1) each iteration is replayed from a
+ * HIP graph captured on the handle's own stream.  Returns PFM_OK or
  * PFM_WARN_NOT_CONVERGED; iters_out / rTr_out may be NULL. */
 int pfm_mg_solve(void* handle, int max_iters, double eps, double tol, int tol_mode,
                  int* iters_out, double* rTr_out, pfm_stream_t stream);

@@ -40,8 +40,11 @@ constexpr int kMgRedBlocks = 1024;
 struct MGScalars {
     double zTr;     // old z.r
     double pAp;
+    // read by the host in one copy: r.r, x.x (the fp32 floor of the true
+    // residual), and the iteration counter and stop flags of solve()
     double rTr;
-    double xTx;     // x.x (read with rTr: the fp32 floor of the true residual)
+    double xTx;
+    int it, done, conv;
     double r0;      // r.r of the recentred initial residual
     double sum_r;
     double new_zTr;
@@ -56,7 +59,7 @@ struct MGScalars {
     // enqueued returns at once, so the host can enqueue several iterations
     // (or replay them from a graph) and read the outcome once.
     double tol, thresh;
-    int tol_mode, max_it, it, done, conv;
+    int tol_mode, max_it;
 };
 
 // Kernels of a PCG iteration return immediately once the solve has stopped.
@@ -97,11 +100,12 @@ struct MGHandle {
     uint8_t* dg[kMaxLevels];
     int64_t n_fluid;        // cells that carry pressure (the recentring mean)
     // PCG iterations on the device: the first group of a solve enqueues as
-    // many iterations as the previous solve took, then one per group, with
-    // one host read per group.  Each iteration can be replayed from a HIP
+    // many iterations as the shorter of the previous two solves took (a step
+    // alternates two projections of different lengths), then one per group,
+    // with one host read per group.  Each iteration can be replayed from a HIP
     // graph (one per p-buffer parity and replacement step), captured on a
     // private non-blocking stream.
-    int last_iters;
+    int last_iters[2];      // iterations of the previous two solves
     int use_graphs;
     float* p_base;          // the allocation first held by p (graph parity key)
     hipStream_t cap_stream;
@@ -1956,9 +1960,12 @@ struct SolveOutcome {
 };
 
 static int read_outcome(MGHandle* h, hipStream_t s, SolveOutcome* o) {
-    static_assert(offsetof(MGScalars, xTx) == offsetof(MGScalars, rTr) + sizeof(double), "rTr, xTx adjacent");
-    PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr, &h->sc->rTr, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr + 2, &h->sc->it, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+    static_assert(offsetof(MGScalars, xTx) == offsetof(MGScalars, rTr) + sizeof(double) &&
+                      offsetof(MGScalars, it) == offsetof(MGScalars, rTr) + 2 * sizeof(double) &&
+                      offsetof(MGScalars, conv) == offsetof(MGScalars, rTr) + 2 * sizeof(double) + 2 * sizeof(int),
+                  "the outcome fields are contiguous");
+    PFM_HIP_TRY(hipMemcpyAsync(h->host_rtr, &h->sc->rTr, 2 * sizeof(double) + 3 * sizeof(int), hipMemcpyDeviceToHost,
+                               s));
     PFM_HIP_TRY(hipStreamSynchronize(s));
     const int* iv = (const int*)(h->host_rtr + 2);
     *o = SolveOutcome{h->host_rtr[0], h->host_rtr[1], iv[0], iv[1], iv[2]};
@@ -2076,12 +2083,14 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
     int parts = 0;
     PFM_TRY(precondition_dot(h, false, s, &parts));
     PFM_TRY(finalize(h, kFinZTr, eps, s, parts));
-    // Iterations in groups: the first as long as the previous solve, then one
-    // at a time; each group ends with one read of the outcome.
+    // Iterations in groups: the first as long as the shorter of the previous
+    // two solves (so it rarely runs past convergence), then one at a time;
+    // each group ends with one read of the outcome.
+    const int predicted = std::max(1, std::min(h->last_iters[0], h->last_iters[1]));
     int enq = 0;
     SolveOutcome o{0.0, 0.0, 0, 0, 0};
     while (max_iters == -1 || enq < max_iters) {
-        int g = enq == 0 ? std::max(1, h->last_iters) : 1;
+        int g = enq == 0 ? predicted : 1;
         if (max_iters != -1) g = std::min(g, max_iters - enq);
         for (int j = 0; j < g; ++j, ++enq)
             if (!pcg_iteration_graph(h, enq, eps, neumann, s)) PFM_TRY(pcg_iteration(h, enq, eps, neumann, s));
@@ -2093,7 +2102,8 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
     // device ran ap_dot_p only until the solve stopped.
     const int ran = o.conv ? o.it + 1 : o.it;
     if ((enq - ran) & 1) std::swap(h->p, h->p2);
-    h->last_iters = ran;
+    h->last_iters[1] = h->last_iters[0];
+    h->last_iters[0] = ran;
     hipLaunchKernelGGL(k_clear_stop, dim3(1), dim3(1), 0, s, h->sc);
     PFM_LAUNCH_CHECK();
     if (!o.conv) PFM_TRY(update_p(h, s));
@@ -2262,7 +2272,9 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     if (hipMalloc(&h->x, b0) != hipSuccess) return fail("x");
     if (hipMalloc(&h->p, b0) != hipSuccess) return fail("p");
     h->p_base = h->p;
-    h->use_graphs = 1;
+    // graph replay is opt-in: measured on the served job stream, capturing and
+    // instantiating per job cost more than the launches it saves
+    h->use_graphs = 0;
     if (hipMalloc(&h->p2, b0) != hipSuccess) return fail("p scratch");
     if (hipMalloc(&h->Ap, b0) != hipSuccess) return fail("Ap");
     h->n_partial = std::max(3 * kMgRedBlocks, sweep_tiles(h, 0).total);

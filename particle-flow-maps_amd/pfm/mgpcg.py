@@ -84,8 +84,8 @@ class MGPCG:
 
     # ---- solid obstacle (extension) ----
     def set_graphs(self, enable: bool):
-        """Replay each PCG iteration from a HIP graph (default) or issue its
-        launches directly; the results are the same."""
+        """Replay each PCG iteration from a HIP graph, or issue its launches
+        directly (the default); the results are the same."""
         L.call("pfm_mg_set_graphs", self._h, int(bool(enable)))
 
     def set_solid(self, solid, stream=None):
