@@ -125,35 +125,52 @@ def instrumented_step(sim, timer):
     return dt
 
 
-def cpu_baseline(seconds_budget=15.0):
-    """The CPU oracle (C/OpenMP fp32 restatement) on a bounded sample of the
-    same workload: full 3D steps at 32^3 x 8 ppc with the same schedule."""
+def cpu_baseline(torch):
+    """The CPU oracle (C/OpenMP fp32 restatement of the reference step) on the
+    box's host cores, on bounded samples of the benchmark's workloads: one full
+    step of the 3D 128^3 x 8 particles/cell ring scene (16.8 M particles; the
+    long and short reinitialisation, the midpoint projection, RK4, P2G and the
+    final projection) -- `value` -- and the 2D 128^2 leapfrog configuration in
+    full (65,536 particles, 100 steps).  The ring velocity is computed on the
+    GPU and copied over (initial conditions are not timed)."""
     import numpy as np
 
     sys.path.insert(0, ROOT)
     cores = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    import oracle
-
-    res = (32, 32, 32)
-    osim = oracle.OracleSim(res, reinit_every=20, reinit_every_grad_m=1, particles_per_cell=8, tol=1e-6,
-                            tol_mode=1, seed=1)
-    vc = np.zeros((3,) + res, np.float32)
-    for r in rings_3d():
-        oracle.vortex_ring(res, 1 / 32, vc, r["w"], r["rad"], r["delta"], r["c"], r["unit_x"], r["unit_y"], 2000)
-    osim.set_velocity_from_central(vc)
-    osim.step()  # warm
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        osim.step()
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > seconds_budget or steps >= 20:
-            break
     omp = int(os.environ.get("OMP_NUM_THREADS", cores))
-    return {"value": osim.n * steps / el, "unit": "particle-steps/s", "cores": omp, "kind": "port",
-            "sample": f"oracle (C/OpenMP fp32 restatement of the reference step) on 3D 32^3 x 8 ppc "
-                      f"({osim.n} particles), {steps} full steps after 1 warm-up step, {el:.1f} s"}
+    import oracle
+    from pfm import core
+
+    # 2D 128^2: four Gaussian-core vortices, 4 ppc, reinit 12 / 8, 100 steps
+    res2 = (128, 128)
+    o2 = oracle.OracleSim(res2, reinit_every=12, reinit_every_grad_m=8, CFL=0.5, particles_per_cell=4, base_level=4,
+                          tol=1e-6, tol_mode=1, seed=0, visualize_dt=1e9)
+    o2.set_velocity_from_central(oracle.point_vortices(res2, 1 / 128, [(0.25, 0.45), (0.25, 0.55), (0.35, 0.45),
+                                                                       (0.35, 0.55)], [1.0, -1.0, 1.0, -1.0]))
+    o2.initialize_2d_particles()
+    t0 = time.perf_counter()
+    for _ in range(100):
+        o2.step()
+    t2 = time.perf_counter() - t0
+    # 3D 128^3 rings, one full step
+    res3 = (128, 128, 128)
+    vc = torch.zeros((3,) + res3, dtype=torch.float32, device="cuda")
+    for r in rings_3d():
+        core.vortex_ring(res3, 1 / 128, vc, r["w"], r["rad"], r["delta"], r["c"], r["unit_x"], r["unit_y"], 2000)
+    o3 = oracle.OracleSim(res3, reinit_every=20, reinit_every_grad_m=1, particles_per_cell=8, tol=1e-6, tol_mode=1,
+                          seed=1, particle_order=1, visualize_dt=1e9)
+    o3.set_velocity_from_central(vc.cpu().numpy())
+    del vc
+    t0 = time.perf_counter()
+    o3.step()
+    t3 = time.perf_counter() - t0
+    return {"value": o3.n / t3, "unit": "particle-steps/s", "cores": omp, "kind": "port",
+            "sample": f"oracle (C/OpenMP fp32 restatement of the reference step) on {omp} threads: one full step of "
+                      f"3D 128^3 x 8 ppc two rings ({o3.n} particles) in {t3:.1f} s; the 2D 128^2 leapfrog "
+                      f"configuration in full (100 steps, {o2.n} particles) in {t2:.1f} s",
+            "config1_2d_128_100_steps_s": t2, "config1_particle_steps_per_s": o2.n * 100 / t2,
+            "step_3d_128_s": t3}
 
 
 def traffic_probe(sim, torch):
@@ -320,7 +337,8 @@ def fp64_reference(solver, u_bc, div, solid, torch, max_refine=6):
     correction comes from the fp32 MGPCG `solver` at a 1e-6 relative
     residual; stops at a 1e-12 relative FP64 residual.  Returns the pressure
     (zero mean over fluid cells), the projected FP64 face velocities and the
-    final relative residual."""
+    final relative residual, and the estimated bound of the pressure's relative
+    error that residual implies."""
     f64 = torch.float64
     fl = (solid == 0).to(f64)
     nb = sum(_shift(fl, a, s) for a in range(3) for s in (1, -1)) * fl
@@ -349,6 +367,9 @@ def fp64_reference(solver, u_bc, div, solid, torch, max_refine=6):
         solver.solve(max_iters=100, tol=1e-6, tol_mode="rel")
         x = x + solver.x.to(f64) * fl
     rel, x = best_rel, center(best_x)
+    # certificate: |x - x*| <= |b - A x| / lambda_1, relative to |x|
+    lam1 = 4.0 * math.sin(math.pi / (2 * max(x.shape))) ** 2
+    cert = rel * nb_ / (lam1 * max(float(torch.linalg.vector_norm(x)), 1e-300))
     u64 = []
     for a in range(3):
         ua = u_bc[a].to(f64).clone()
@@ -361,7 +382,55 @@ def fp64_reference(solver, u_bc, div, solid, torch, max_refine=6):
         ua.narrow(a, 0, 1).zero_()
         ua.narrow(a, n, 1).zero_()
         u64.append(ua)
-    return x, u64, rel
+    return x, u64, rel, cert
+
+
+def kernel_rooflines(sim, torch, committed=True):
+    """G2P (short reinitialisation), the fused midpoint grid pass and P2G
+    (tile + combine over the last step's records) of the 3D step, each timed
+    alone with HIP events on the launch stream (the current torch stream),
+    against HBM (their algorithmic bytes) and VALU issue (the committed
+    per-wave instruction counts).  Algorithmic bytes: G2P reads pos 12 and
+    writes C 36 per particle; the midpoint pass reads the velocity samples
+    (counted once, 4 B per face) and writes psi 12, T 12 and u 4 per face of
+    every axis (32 B / face); P2G reads each 64 B record and its permutation
+    entry (68 B / particle) and writes u and W per face (8 B / face)."""
+    from pfm import core
+
+    res, n, dx = sim.res, sim.n, sim.dx
+    faces = sum(math.prod(core.face_shape(res, a)) for a in range(3))
+    cells = math.prod(res)
+
+    def timed(fn, reps=3):
+        fn()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(torch.cuda.current_stream())
+        for _ in range(reps):
+            fn()
+        b.record(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    out = {}
+    specs = [
+        ("g2p", "pfm::k_g2p<3, true>", 48.0 * n, n,
+         lambda: core.g2p(res, dx, sim.pos, sim.u, None, sim.init_C)),
+        ("midpoint_advect", "pfm::k_midpoint_advect<3>", 32.0 * faces, faces,
+         lambda: core.midpoint_advect(res, dx, sim.p2g_weight, sim.u, sim.psi_grid, sim.T_grid, 1e-4)),
+        ("p2g", "pfm::k_p2g_tile<3>", 68.0 * n + 8.0 * faces, cells,
+         lambda: sim.binning.p2g_from_records(dx, n, sim.u, W=sim.p2g_weight)),
+    ]
+    for name, kname, alg, lanes, fn in specs:
+        ms = timed(fn)
+        gbs = alg / (ms * 1e-3) / 1e9
+        row = {"kernel": kname, "ms": ms, "alg_bytes": alg, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+        vpw = committed_valu(kname) if committed else None
+        if vpw is not None:
+            rate = vpw * math.ceil(lanes / 64) / (ms * 1e-3)
+            row.update({"valu_insts_per_wave": vpw, "valu_frac": rate / VALU_PEAK_WAVE_INSTS})
+        out[name] = row
+    return out
 
 
 def serving_bench(n_jobs, rates, slots, torch):
@@ -575,6 +644,22 @@ def main():
     its = sim.poisson_iters[iters_before:]
     print(f"# timed {args.steps} steps: {ms_per_step:.1f} ms/step, poisson iters {its}", file=sys.stderr,
           flush=True)
+    # sanity of the timed run's end state: kinetic energy, the discrete
+    # divergence left by the last projection (relative to the face-difference
+    # scale), particles in the domain
+    sanity = {"kinetic_energy": sim.kinetic_energy()}
+    with torch.no_grad():
+        div = None
+        for a in range(3):
+            x = sim.u[a].double()
+            t = x.narrow(a, 1, x.shape[a] - 1) - x.narrow(a, 0, x.shape[a] - 1)
+            div = t if div is None else div + t
+        scale = sum(float(torch.linalg.vector_norm(x.double())) for x in sim.u)
+        sanity["div_rel_l2"] = float(torch.linalg.vector_norm(div)) / scale
+        sanity["max_abs_div"] = float(div.abs().max())
+        sanity["finite"] = bool(all(torch.isfinite(x).all() for x in sim.u))
+        sanity["pos_min"], sanity["pos_max"] = float(sim.pos.min()), float(sim.pos.max())
+        del div
 
     # per-phase breakdown (instrumented steps, not part of `value`)
     timer = EventTimer(torch)
@@ -602,7 +687,7 @@ def main():
     ref_solver.apply_bc(*u_bc)
     ref_solver.divergence(*u_bc)
     div = ref_solver.u_div.clone()
-    p64, u64, ref_rel = fp64_reference(ref_solver, u_bc, div, solid, torch)
+    p64, u64, ref_rel, ref_cert = fp64_reference(ref_solver, u_bc, div, solid, torch)
     del ref_solver, div
     fl = solid == 0
     n_u64 = math.sqrt(sum(float((t * t).sum()) for t in u64))
@@ -642,6 +727,10 @@ def main():
     pits = mg_runs["fp32"]["iters"]
     del u_f, u_bc, u64, p64
 
+    # the other particle / face kernels of the step, timed alone (HIP events on
+    # the launch stream) with their algorithmic bytes and VALU issue
+    kernels = kernel_rooflines(sim, torch, args.res == 256)
+
     # roofline of the step's dominant kernel, RK4 + P2G record with the
     # update_T fold and the binning count pass (k_rk4_particles<3, true, true,
     # true>): per particle it reads pos 12 B, T_init 36 B, init_C 36 B,
@@ -673,13 +762,20 @@ def main():
                    else "pfm::k_rk4_particles<3, true, false, false>")
     traffic = committed_traffic(roof_kernel) if args.res == 256 else None
     # the kernel is VALU-issue bound: its instruction rate against the peak
+    # (the binding resource); its HBM rate beside it
     vpw = committed_valu(roof_kernel) if args.res == 256 else None
-    valu = None
+    hbm = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+           "alg_bytes_per_launch": alg_bytes}
     if vpw is not None:
         waves = math.ceil(sim.n / 64)
         rate = vpw * waves / (rk4_ms * 1e-3)
-        valu = {"insts_per_wave": vpw, "waves_per_launch": waves, "achieved_wave_insts_per_s": rate,
-                "peak_wave_insts_per_s": VALU_PEAK_WAVE_INSTS, "frac": rate / VALU_PEAK_WAVE_INSTS}
+        roofline = {"kernel": roof_kernel, "bound": "valu", "achieved": rate, "peak": VALU_PEAK_WAVE_INSTS,
+                    "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK_WAVE_INSTS, "traffic": traffic,
+                    "kernel_ms": rk4_ms, "insts_per_wave": vpw, "waves_per_launch": waves, "hbm": hbm}
+    else:
+        roofline = {"kernel": roof_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms": rk4_ms,
+                    "alg_bytes_per_launch": alg_bytes}
 
     out = {
         "metric": "particle-steps/s",
@@ -702,13 +798,19 @@ def main():
         "mgpcg_iters": pits,
         "mgpcg": {"workload": f"{args.res}^3 Poisson, free-slip walls + solid box [0.4,0.6]^3, "
                               "RHS = div of 64 Fourier modes (seed 0), 1e-6 relative residual",
-                  "fp64_reference": "iterative refinement, float64 residuals, to a "
-                                    f"{ref_rel:.1e} relative residual",
+                  "fp64_reference": {
+                      "method": "iterative refinement: float64 residuals (torch, the 7-point operator over fluid "
+                                "neighbours), fp32 MGPCG corrections",
+                      "relative_residual": ref_rel,
+                      # |e| <= |r| / lambda_1 with lambda_1 = 4 sin^2(pi / 2N) the smallest nonzero
+                      # eigenvalue of the Neumann Laplacian of the box (the obstacle domain's is of
+                      # the same order: an estimate, not a proof)
+                      "pressure_error_bound_estimate": ref_cert},
                   **mg_runs},
         "phase_ms": phases,
-        "roofline": {"kernel": roof_kernel, "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_ms": rk4_ms, "alg_bytes_per_launch": alg_bytes, "valu": valu},
+        "sanity": sanity,
+        "roofline": roofline,
+        "kernel_rooflines": kernels,
     }
     del sim
     torch.cuda.empty_cache()
@@ -719,7 +821,7 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline()
+            out["cpu_baseline"] = cpu_baseline(torch)
         except Exception as e:  # reported, never fatal for the GPU number
             out["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(out))

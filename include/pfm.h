@@ -308,9 +308,10 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
 int pfm_mg_destroy(void* handle);
 int pfm_mg_n_levels(void* handle);
 /* 1: replay each PCG iteration of pfm_mg_solve / pfm_mg_poisson from a HIP
- * graph captured once per handle; 0 (default): issue the launches directly.
- * Both run the same kernels in the same order (equal results).  No reference
- * counterpart. */
+ * graph captured once per handle, enqueuing the iterations the previous
+ * solves needed before the first host read; 0 (default): issue the launches
+ * directly, one host read per iteration.  Both run the same kernels in the
+ * same order (equal results).  No reference counterpart. */
 int pfm_mg_set_graphs(void* handle, int enable);
 
 /* Device pointer of an internal field (0 r, 1 z, 2 x, 3 p, 4 Ap, 5 u_div,
@@ -339,12 +340,12 @@ int pfm_mg_init(void* handle, const float* rhs, float k, pfm_stream_t stream);
 
 /* solve (3D/mgpcg.py:202-280).  tol_mode 0: stop when rTr < tol (reference,
  * absolute); 1: stop when sqrt(rTr/r0Tr0) < tol (relative extension).
- * The convergence test runs on the device: iterations are enqueued in groups
- * (the first as long as the handle's previous solve, then one at a time; the
- * kernels of iterations past convergence return at once) and the stream is
- * synchronised once per group (the reference does three host reads per
- * iteration).  With pfm_mg_set_graphs(h, 1) each iteration is replayed from a
- * HIP graph captured on the handle's own stream.  Returns PFM_OK or
+ * The convergence test runs on the device and the stream is synchronised
+ * once per group of iterations (the reference does three host reads per
+ * iteration).  A group is one iteration, or with pfm_mg_set_graphs(h, 1) the
+ * first group is as long as the shorter of the handle's previous two solves
+ * (iterations past convergence change nothing) and each iteration is replayed
+ * from a HIP graph captured on the handle's own stream.  Returns PFM_OK or
  * PFM_WARN_NOT_CONVERGED; iters_out / rTr_out may be NULL. */
 int pfm_mg_solve(void* handle, int max_iters, double eps, double tol, int tol_mode,
                  int* iters_out, double* rTr_out, pfm_stream_t stream);

@@ -54,15 +54,16 @@ struct MGScalars {
     // its output is read back times vinv = 1/vs; keeps values in fp16 range
     float vs, vinv;
     // Convergence on the device (solve): the threshold on r.r (tol, or
-    // tol^2 r0 for a relative tolerance), the iteration counter and the stop
-    // flags.  Once `done` is set every kernel of the iterations still
-    // enqueued returns at once, so the host can enqueue several iterations
-    // (or replay them from a graph) and read the outcome once.
+    // tol^2 r0 for a relative tolerance).  Once `done` is set, the kernels of
+    // the iterations still enqueued change nothing the solve returns: k_finalize
+    // leaves every scalar alone and k_update_xr leaves x and r alone (the
+    // V-cycle and Ap launches only write scratch), so the host can enqueue
+    // several iterations (or replay them from a graph) and read the outcome once.
     double tol, thresh;
     int tol_mode, max_it;
 };
 
-// Kernels of a PCG iteration return immediately once the solve has stopped.
+// Whether the solve has stopped (checked by the kernels that write its state).
 __device__ __forceinline__ bool pcg_stopped(const MGScalars* sc) { return sc != nullptr && sc->done != 0; }
 
 struct Level {
@@ -322,7 +323,6 @@ template <typename ZT, typename RT>
 __global__ void __launch_bounds__(kBlock) k_restrict_any(LvlDev Lf, LvlDev Lc, float inv_scale,
                                                          const ZT* __restrict__ z, const RT* __restrict__ r,
                                                          ZT* __restrict__ rc, const MGScalars* __restrict__ sc) {
-    if (pcg_stopped(sc)) return;
     int C[3];
     if (!cell_of_thread(Lc, 0, false, C)) return;
     const int64_t qc = cell_index(Lc, C);
@@ -476,7 +476,6 @@ __global__ void __launch_bounds__(kSweepThreads, 3) k_rbgs(LvlDev L, LvlDev Lc, 
                                                         const RT* __restrict__ r, ZT* __restrict__ zout, int c0,
                                                         const MGScalars* __restrict__ sc,
                                                         double* __restrict__ partial) {
-    if (pcg_stopped(sc)) return;
     using T = MgTile<DIM>;
     constexpr int NT = kSweepThreads;
     constexpr int H = 2, HI = DIM == 3 ? H : 0, A0 = DIM == 3 ? 1 : 0;
@@ -725,7 +724,6 @@ __global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc,
                                                            const ZT* __restrict__ z, RT* __restrict__ r,
                                                            ZT* __restrict__ rc,
                                                            const MGScalars* __restrict__ sc) {
-    if (pcg_stopped(sc)) return;
     using C = RsTile<DIM>;
     constexpr int FI = DIM == 3 ? 2 * C::I : 1, FJ = 2 * C::J, FK = 2 * C::K, NF = FI * FJ * FK;
     constexpr int HI = DIM == 3 ? 1 : 0;
@@ -815,7 +813,6 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
                                                      const float* __restrict__ p_old, float* __restrict__ p_new,
                                                      float* __restrict__ Ap, const MGScalars* __restrict__ sc,
                                                      double* __restrict__ partial) {
-    if (pcg_stopped(sc)) return;
     using T = MgTile<DIM>;
     constexpr int HI = DIM == 3 ? 1 : 0;
     constexpr int SI = T::I + 2 * HI, SJ = T::J + 2, SK = T::K + 2, SP = SJ * SK, NS = SI * SP;
@@ -842,6 +839,9 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
     }
     const float beta = sc->beta;
     const float zi = sizeof(ZT) == 2 ? sc->vinv : 1.0f;  // 16-bit z is stored scaled
+    // past convergence the search direction stays as it is: a second idle
+    // iteration would otherwise write its partner buffer, the current p
+    const bool keep = pcg_stopped(sc);
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
         const int e = tid + k * kBlock;
@@ -869,8 +869,10 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
         const float a = fluid ? __fsub_rn(__fmul_rn(nf, pq), nsum_lds<DIM>(ps + le, SP, SK)) : 0.f;
         if (inside(L, g0, g1, g2)) {
             const int q = (g0 * L.d[1] + g1) * L.d[2] + g2;
-            p_new[q] = pq;
-            Ap[q] = a;
+            if (!keep) {
+                p_new[q] = pq;
+                Ap[q] = a;
+            }
             s += (double)pq * (double)a;
         }
     }
@@ -995,7 +997,6 @@ __device__ void coarse_prolong(const LvlDev& Lf, const LvlDev& Lc, float* zs, co
 template <int DIM, typename ZT, typename R0T, typename BT>
 __global__ void __launch_bounds__(kCoarseThreads) k_coarse(CoarseArgs a, int sub_mean, int dot,
                                                    const MGScalars* __restrict__ sc, double* __restrict__ partial) {
-    if (pcg_stopped(sc)) return;
     __shared__ float sm[kCoarseLds];
     const int tid = threadIdx.x;
     for (int l = 0; l < a.nl; ++l)
@@ -1204,16 +1205,19 @@ __global__ void __launch_bounds__(kBlock) k_update_xr(int64_t n, float* __restri
                                                       const float* __restrict__ Ap,
                                                       const MGScalars* __restrict__ sc,
                                                       double* __restrict__ partial) {
-    if (pcg_stopped(sc)) return;
     __shared__ double sh[kBlock / 64];
     const float alpha = sc->alpha;
+    // past convergence x and r stay as they are (the partials go unused)
+    const bool keep = pcg_stopped(sc);
     double s2 = 0.0, s1 = 0.0, s3 = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x) {
         const float xq = __fadd_rn(x[q], __fmul_rn(alpha, p[q]));
-        x[q] = xq;
         const float rq = __fsub_rn(r[q], __fmul_rn(alpha, Ap[q]));
-        r[q] = rq;
+        if (!keep) {
+            x[q] = xq;
+            r[q] = rq;
+        }
         s2 += (double)rq * (double)rq;
         s1 += (double)rq;
         if (XX) s3 += (double)xq * (double)xq;
@@ -1243,7 +1247,6 @@ __global__ void __launch_bounds__(kBlock) k_update_p(int64_t n, float* __restric
 __global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t n, float* __restrict__ r,
                                                      const MGScalars* __restrict__ sc,
                                                      const uint8_t* __restrict__ dg) {
-    if (pcg_stopped(sc)) return;
     const float m = sc->mean;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x)
@@ -1972,12 +1975,12 @@ static int read_outcome(MGHandle* h, hipStream_t s, SolveOutcome* o) {
     return PFM_OK;
 }
 
-// One PCG iteration (3D/mgpcg.py:236-275) as launches only: Ap and alpha,
-// the x / r update and the convergence test on the device, every
+// One PCG iteration (3D/mgpcg.py:236-275) as launches only.  Head: Ap and
+// alpha, the x / r update and the convergence test on the device, every
 // kReplaceEvery-th iteration of the mixed-precision solver the true-residual
 // replacement (taken on the device when r.r is above the representation
-// floor), then the V-cycle, z.r and beta.  `k` is the iteration's index.
-static int pcg_iteration(MGHandle* h, int k, double eps, bool neumann, hipStream_t s) {
+// floor).  Tail: the V-cycle, z.r and beta.  `k` is the iteration's index.
+static int pcg_head(MGHandle* h, int k, double eps, hipStream_t s) {
     const int64_t n = h->lv[0].n;
     float* r0 = (float*)h->r[0];
     const bool replace = h->precision != 0;
@@ -2003,9 +2006,19 @@ static int pcg_iteration(MGHandle* h, int k, double eps, bool neumann, hipStream
         PFM_TRY(dot_t(h, r0, (const float*)nullptr, s, h->partial + kMgRedBlocks));
         PFM_TRY(finalize(h, kFinRTr, eps, s, kMgRedBlocks, 1));
     }
+    return PFM_OK;
+}
+
+static int pcg_tail(MGHandle* h, double eps, bool neumann, hipStream_t s) {
+    int parts = 0;
     PFM_TRY(precondition_dot(h, neumann, s, &parts));
     PFM_TRY(finalize(h, kFinBeta, eps, s, parts));
     return PFM_OK;
+}
+
+static int pcg_iteration(MGHandle* h, int k, double eps, bool neumann, hipStream_t s) {
+    PFM_TRY(pcg_head(h, k, eps, s));
+    return pcg_tail(h, eps, neumann, s);
 }
 
 // The iteration from a HIP graph: captured once per (p-buffer parity,
@@ -2083,21 +2096,37 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
     int parts = 0;
     PFM_TRY(precondition_dot(h, false, s, &parts));
     PFM_TRY(finalize(h, kFinZTr, eps, s, parts));
-    // Iterations in groups: the first as long as the shorter of the previous
-    // two solves (so it rarely runs past convergence), then one at a time;
-    // each group ends with one read of the outcome.
-    const int predicted = std::max(1, std::min(h->last_iters[0], h->last_iters[1]));
     int enq = 0;
     SolveOutcome o{0.0, 0.0, 0, 0, 0};
-    while (max_iters == -1 || enq < max_iters) {
-        int g = enq == 0 ? predicted : 1;
-        if (max_iters != -1) g = std::min(g, max_iters - enq);
-        for (int j = 0; j < g; ++j, ++enq)
-            if (!pcg_iteration_graph(h, enq, eps, neumann, s)) PFM_TRY(pcg_iteration(h, enq, eps, neumann, s));
-        PFM_TRY(read_outcome(h, s, &o));
-        if (o.done) break;
+    if (!h->use_graphs || h->graph_failed) {
+        // The reference's loop shape: the host reads the convergence test of
+        // each iteration before issuing its V-cycle.
+        for (; max_iters == -1 || enq < max_iters; ++enq) {
+            PFM_TRY(pcg_head(h, enq, eps, s));
+            PFM_TRY(read_outcome(h, s, &o));
+            if (o.done) {
+                ++enq;
+                break;
+            }
+            PFM_TRY(pcg_tail(h, eps, neumann, s));
+        }
+    } else {
+        // Whole iterations replayed from graphs in groups, each ending with one
+        // read of the outcome: the first group as long as the shorter of the
+        // previous two solves (so it rarely runs past convergence), then one at
+        // a time.  Enqueued iterations past convergence change nothing the
+        // solve returns.
+        const int predicted = std::max(1, std::min(h->last_iters[0], h->last_iters[1]));
+        while (max_iters == -1 || enq < max_iters) {
+            int g = enq == 0 ? predicted : 1;
+            if (max_iters != -1) g = std::min(g, max_iters - enq);
+            for (int j = 0; j < g; ++j, ++enq)
+                if (!pcg_iteration_graph(h, enq, eps, neumann, s)) PFM_TRY(pcg_iteration(h, enq, eps, neumann, s));
+            PFM_TRY(read_outcome(h, s, &o));
+            if (o.done) break;
+        }
     }
-    if (enq == 0) PFM_TRY(read_outcome(h, s, &o));
+    if (enq == 0 || !o.done) PFM_TRY(read_outcome(h, s, &o));
     // Every enqueued iteration swapped p and its partner on the host; the
     // device ran ap_dot_p only until the solve stopped.
     const int ran = o.conv ? o.it + 1 : o.it;
