@@ -265,7 +265,12 @@ __global__ void __launch_bounds__(kBlock) k_p2g_prep(GridDev g, int64_t n, PrepA
 // are then summed into an LDS box in 4*3*3 fixed "offset phases": in phase
 // (i,j,k) every thread adds its window entry (i,j,k) to the face at its cell
 // + (i,j,k) - 1, so no two threads touch the same face within a phase and the
-// order of additions is fixed.  Each tile writes its whole box (faces of its
+// order of additions is fixed.  Only the phases of different i need a block
+// barrier between them: a wave holds whole rows of the tile along axis 0
+// (3D: one row, 2D: four), so for one i the waves add to disjoint faces, and
+// one wave's phases (i, j, k) run in program order (a wave's LDS operations
+// are performed in order; a wavefront-scope fence keeps the compiler from
+// reordering them).  Each tile writes its whole box (faces of its
 // cells plus the halo its particles reach) as a partial; k_p2g_combine sums
 // the partials of the <= 2^d tiles that reach a face in tile order and
 // normalises by the weight.  Every step is deterministic.
@@ -311,6 +316,12 @@ struct TileStride {
 
 constexpr int kTileLds = 704;  // >= the largest box (3D x-faces: 7*10*10)
 
+// Orders one wave's LDS accesses before and after it in program order (the
+// hardware performs them in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
 template <int DIM, int A>
 __device__ __forceinline__ void tile_axis(const GridDev& g, const float* __restrict__ pay,
                                           const int32_t* __restrict__ perm, int32_t s, int32_t e,
@@ -334,11 +345,11 @@ __device__ __forceinline__ void tile_axis(const GridDev& g, const float* __restr
         // records in sorted order, or in particle order through the permutation
         const float* rec = pay + (int64_t)(perm ? perm[q] : q) * P::kRec;
         float p[3], crow[3];
-        const float impA = rec[P::kImp + A];
+        const float impA = rec[P::imp(A)];
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
             p[c] = rec[P::kPos + c];
-            crow[c] = rec[P::kC + A * DIM + c];
+            crow[c] = rec[P::C(A, c)];
         }
         float wt[3][4], dd[3][4];
 #pragma unroll
@@ -400,7 +411,10 @@ __device__ __forceinline__ void tile_axis(const GridDev& g, const float* __restr
                 const int slot = ((tl[0] + i) * BX::B1 + (tl[1] + j)) * BX::B2 + (DIM == 3 ? tl[2] + k : 0);
                 lu[slot] += au[i][j][k];
                 lw[slot] += aw[i][j][k];
-                __syncthreads();
+                if (j + 1 < W1 || k + 1 < W2)
+                    wave_lds_order();
+                else
+                    __syncthreads();
             }
     float* dst = part + BoxOff<DIM, A>::value;
     for (int t = threadIdx.x; t < BX::size; t += blockDim.x) {
@@ -470,6 +484,224 @@ __global__ void __launch_bounds__(kBlock) kb_p2g_tile(BUnits t, const int32_t* _
         p2g_tile_body<2>(S.g, start + S.c_off, pay, perm, w.chunk, S.nt[1], 1,
                          partial + (S.t_off + w.chunk) * (int64_t)TileStride<2>::value, lu, lw);
     }
+}
+
+// ---- 3D tiled P2G, one block per (tile, face axis) ---------------------------
+// Same tiles, windows, offset phases and partial boxes as tile_axis, with
+//  - the three face axes of a tile in three blocks, numbered 24 g + 8 A + x
+//    for tile 8 g + x: workgroups are dealt to the 8 XCDs round-robin by id,
+//    so the three blocks of a tile run on one XCD at about the same time and
+//    the second and third reads of the tile's records hit that XCD's L2
+//    (one block walking the three axes in turn re-fetches them from HBM);
+//  - the spline weights and offsets of a particle formed from its offset in
+//    its cell, t = p - cc (exact: cc is floor(p), or p's clamped cell), as
+//    the closed-form pieces of N_2 on [0, 1), with the general N_2 kept for
+//    a particle outside its clamped cell (t outside [0, 1));
+//  - the next record loaded while the current one is scattered.
+// The face values and their summation order are those of tile_axis: every
+// d = t - (w - 1) and o = (w - 1) - t here is the same real number, rounded
+// once, as p - f and f - p there.
+constexpr int kRec3Fields = 7;  // p[3], imp_A, row A of C
+
+template <int A>
+__device__ __forceinline__ void load_rec3(const float* __restrict__ pay, int32_t id, float r[kRec3Fields]) {
+    using P = Pay<3>;
+    static_assert(P::kPos == 0 && P::imp(A) == 4 + 4 * A && P::C(A, 0) == P::imp(A) + 1, "record quads");
+    const float4* rec = reinterpret_cast<const float4*>(pay + (int64_t)id * P::kRec);
+    const float4 a = rec[0], b = rec[1 + A];
+    r[0] = a.x;
+    r[1] = a.y;
+    r[2] = a.z;
+    r[3] = b.x;
+    r[4] = b.y;
+    r[5] = b.z;
+    r[6] = b.w;
+}
+
+// N_2 at the four faces cc-1 .. cc+2 along the face axis (d = t+1, t, t-1,
+// t-2) and at the three cell centres cc-1 .. cc+1 across it (d = t+1/2,
+// t-1/2, t-3/2), for t in [0, 1).
+__device__ __forceinline__ void face_weights01(float t, float w[4]) {
+    const float h = t - 0.5f;
+    const float q = 0.5f * (h * h);  // 0.5 (1.5 - |t+1|)^2 and 0.5 (1.5 - |t-2|)^2
+    const float a = 1.5f - t;
+    const float b = 1.0f - t;
+    const float c = 0.5f + t;
+    const bool lo = t < 0.5f, hi = t > 0.5f;
+    w[0] = lo ? q : 0.f;
+    w[1] = lo ? 0.75f - t * t : 0.5f * (a * a);
+    w[2] = hi ? 0.75f - b * b : 0.5f * (c * c);
+    w[3] = hi ? q : 0.f;
+}
+
+__device__ __forceinline__ void centre_weights01(float t, float w[3]) {
+    const float b = 1.0f - t;
+    const float h = t - 0.5f;
+    w[0] = 0.5f * (b * b);
+    w[1] = 0.75f - h * h;
+    w[2] = 0.5f * (t * t);
+}
+
+// Scatter of one record (p, imp_A, row A of C) of a particle of cell cc
+// into the face window of axis A.
+template <int A>
+__device__ __forceinline__ void scatter3(const float r[kRec3Fields], const float fc[3], float dx,
+                                         float (&au)[Win<3, A, 0>::value][Win<3, A, 1>::value][Win<3, A, 2>::value],
+                                         float (&aw)[Win<3, A, 0>::value][Win<3, A, 1>::value][Win<3, A, 2>::value]) {
+    constexpr int W0 = Win<3, A, 0>::value, W1 = Win<3, A, 1>::value, W2 = Win<3, A, 2>::value;
+    constexpr int Wn[3] = {W0, W1, W2};
+    float t[3];
+    bool inside = true;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        t[c] = r[c] - fc[c];
+        // 0 <= t < 1 as one unsigned compare of the bits (t is never -0)
+        inside = inside && __float_as_uint(t[c]) < 0x3f800000u;
+    }
+    float wt[3][4];
+    if (inside) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c == A)
+                face_weights01(t[c], wt[c]);
+            else
+                centre_weights01(t[c], wt[c]);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int w = 0; w < Wn[c]; ++w) {
+                const float d = t[c] - (float)(w - 1);
+                wt[c][w] = N2(c == A ? d : d - 0.5f);
+            }
+    }
+    // imp_A + C_A . dpos with dpos_c = (face (+1/2) - p)_c dx (3D/run_3D.py:
+    // 486-489) is affine in the window offsets: with o_c = w - 1 along A and
+    // w - 1/2 across, it is b + sum_c (C_A,c dx) o_c, b = imp_A - sum_c
+    // (C_A,c dx) t_c.  e(i, j, k) = e01(i, j) + g2(k), and the face gets
+    // w0 w1 w2 e = w0 w1 (w2 e01 + w2 g2): three FMAs per window face.
+    float Cd[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Cd[c] = r[4 + c] * dx;
+    float b = r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b = __builtin_fmaf(-Cd[c], t[c], b);
+    auto off = [](int c, int w) { return c == A ? (float)(w - 1) : (float)w - 0.5f; };
+    float e0[W0], x2[W2];
+#pragma unroll
+    for (int i = 0; i < W0; ++i) e0[i] = __builtin_fmaf(off(0, i), Cd[0], b);
+#pragma unroll
+    for (int k = 0; k < W2; ++k) x2[k] = wt[2][k] * (off(2, k) * Cd[2]);
+#pragma unroll
+    for (int i = 0; i < W0; ++i)
+#pragma unroll
+        for (int j = 0; j < W1; ++j) {
+            const float wij = wt[0][i] * wt[1][j];
+            const float eij = __builtin_fmaf(off(1, j), Cd[1], e0[i]);
+#pragma unroll
+            for (int k = 0; k < W2; ++k) {
+                au[i][j][k] = __builtin_fmaf(wij, __builtin_fmaf(wt[2][k], eij, x2[k]), au[i][j][k]);
+                aw[i][j][k] = __builtin_fmaf(wij, wt[2][k], aw[i][j][k]);
+            }
+        }
+}
+
+template <int A>
+__device__ __forceinline__ void p2g_axis3(const GridDev& g, const float* __restrict__ pay,
+                                          const int32_t* __restrict__ perm, int32_t s, int32_t e, const int cc[3],
+                                          const int tl[3], float* lu, float* lw, float* __restrict__ dst) {
+    constexpr int W0 = Win<3, A, 0>::value, W1 = Win<3, A, 1>::value, W2 = Win<3, A, 2>::value;
+    using BX = Box<3, A>;
+    float au[W0][W1][W2], aw[W0][W1][W2];
+#pragma unroll
+    for (int i = 0; i < W0; ++i)
+#pragma unroll
+        for (int j = 0; j < W1; ++j)
+#pragma unroll
+            for (int k = 0; k < W2; ++k) {
+                au[i][j][k] = 0.f;
+                aw[i][j][k] = 0.f;
+            }
+    const float fc[3] = {(float)cc[0], (float)cc[1], (float)cc[2]};
+    auto id_of = [&](int32_t q) { return perm ? perm[q] : q; };
+    // record q+1 and the id of q+2 are in flight while q is scattered; two
+    // record buffers alternate, so no copies
+    float ra[kRec3Fields], rb[kRec3Fields];
+    int32_t id2 = 0;
+    if (s < e) load_rec3<A>(pay, id_of(s), ra);
+    if (s + 1 < e) id2 = id_of(s + 1);
+    int32_t q = s;
+    while (q < e) {
+        if (q + 1 < e) load_rec3<A>(pay, id2, rb);
+        if (q + 2 < e) id2 = id_of(q + 2);
+        scatter3<A>(ra, fc, g.dx, au, aw);
+        if (++q >= e) break;
+        if (q + 1 < e) load_rec3<A>(pay, id2, ra);
+        if (q + 2 < e) id2 = id_of(q + 2);
+        scatter3<A>(rb, fc, g.dx, au, aw);
+        ++q;
+    }
+    for (int t = threadIdx.x; t < BX::size; t += blockDim.x) {
+        lu[t] = 0.f;
+        lw[t] = 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < W0; ++i)
+#pragma unroll
+        for (int j = 0; j < W1; ++j)
+#pragma unroll
+            for (int k = 0; k < W2; ++k) {
+                const int slot = ((tl[0] + i) * BX::B1 + (tl[1] + j)) * BX::B2 + tl[2] + k;
+                lu[slot] += au[i][j][k];
+                lw[slot] += aw[i][j][k];
+                if (j + 1 < W1 || k + 1 < W2)
+                    wave_lds_order();
+                else
+                    __syncthreads();
+            }
+    float* box = dst + BoxOff<3, A>::value;
+    for (int t = threadIdx.x; t < BX::size; t += blockDim.x) {
+        box[t] = lu[t];
+        box[BX::size + t] = lw[t];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_p2g_tile3(GridDev g, const int32_t* __restrict__ start,
+                                                      const float* __restrict__ pay,
+                                                      const int32_t* __restrict__ perm, int nt1, int nt2,
+                                                      int64_t ntiles, float* __restrict__ partial) {
+    using TC = TileCfg<3>;
+    __shared__ float lu[kTileLds], lw[kTileLds];
+    const int64_t b = blockIdx.x;
+    const int A = (int)((b % 24) / 8);
+    const int64_t tile = (b / 24) * 8 + (b % 8);
+    if (tile >= ntiles) return;  // the grid is rounded up to whole groups of 24
+    const int tb[3] = {(int)(tile / ((int64_t)nt1 * nt2)), (int)((tile / nt2) % nt1), (int)(tile % nt2)};
+    const int tl[3] = {(int)threadIdx.x / (TC::T1 * TC::T2), ((int)threadIdx.x / TC::T2) % TC::T1,
+                       (int)threadIdx.x % TC::T2};
+    const int T[3] = {TC::T0, TC::T1, TC::T2};
+    int cc[3];
+    bool valid = true;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        cc[c] = tb[c] * T[c] + tl[c];
+        if (cc[c] >= g.res[c]) valid = false;
+    }
+    int32_t s = 0, e = 0;
+    if (valid) {
+        const int64_t cell = ((int64_t)cc[0] * g.res[1] + cc[1]) * g.res[2] + cc[2];
+        s = start[cell];
+        e = start[cell + 1];
+    }
+    float* dst = partial + tile * TileStride<3>::value;
+    if (A == 0)
+        p2g_axis3<0>(g, pay, perm, s, e, cc, tl, lu, lw, dst);
+    else if (A == 1)
+        p2g_axis3<1>(g, pay, perm, s, e, cc, tl, lu, lw, dst);
+    else
+        p2g_axis3<2>(g, pay, perm, s, e, cc, tl, lu, lw, dst);
 }
 
 struct CombineArgs {
@@ -836,8 +1068,8 @@ static int tile_and_combine(const GridDev& g, const int32_t* cell_start, const i
     tile_counts(g, nt);
     const unsigned ptiles = (unsigned)((int64_t)nt[0] * nt[1] * nt[2]);
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_p2g_tile<3>, dim3(ptiles), dim3(kBlock), 0, s, g, cell_start, w.pay, perm, nt[1],
-                           nt[2], w.partial);
+        hipLaunchKernelGGL(k_p2g_tile3, dim3((ptiles + 7) / 8 * 24), dim3(kBlock), 0, s, g, cell_start, w.pay,
+                           perm, nt[1], nt[2], (int64_t)ptiles, w.partial);
     else
         hipLaunchKernelGGL(k_p2g_tile<2>, dim3(ptiles), dim3(kBlock), 0, s, g, cell_start, w.pay, perm, nt[1],
                            nt[2], w.partial);

@@ -219,14 +219,15 @@ __device__ __forceinline__ void interp_mac(const float* __restrict__ ux,
 
 // ---- P2G particle record ---------------------------------------------------
 // Record of one particle for the tiled P2G, kRec floats:
-//   3D: [0..2] pos/dx, [3..5] imp, [6..14] C rows (row r at 6 + 3r), [15] pad
+//   3D: [0..2] pos/dx, [3] 0, then a quad per face axis A at 4 + 4A holding
+//       imp_A and row A of C (the tile pass of axis A reads two aligned quads)
 //   2D: [0..1] pos/dx, [2..3] imp, [4..7] C rows (row r at 4 + 2r)
 template <int DIM>
 struct Pay {
     static constexpr int kPos = 0;
-    static constexpr int kImp = DIM;
-    static constexpr int kC = 2 * DIM;
     static constexpr int kRec = DIM == 3 ? 16 : 8;
+    static constexpr int imp(int r) { return DIM == 3 ? 4 + 4 * r : DIM + r; }
+    static constexpr int C(int r, int c) { return DIM == 3 ? 5 + 4 * r + c : 2 * DIM + r * DIM + c; }
 };
 
 // P2G per-particle composition (3D/run_3D.py:460-479) and record write:
@@ -287,9 +288,9 @@ __device__ __forceinline__ void form_p2g_record(float rec[Pay<DIM>::kRec], const
     }
 #pragma unroll
     for (int r = 0; r < DIM; ++r) {
-        rec[P::kImp + r] = mo[r];
+        rec[P::imp(r)] = mo[r];
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) rec[P::kC + r * DIM + c] = Co[r][c];
+        for (int c = 0; c < DIM; ++c) rec[P::C(r, c)] = Co[r][c];
     }
 }
 
