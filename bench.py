@@ -226,10 +226,14 @@ def pmc_traffic(fetch_db, write_db):
     return "\n".join(lines)
 
 
-VALU_PROFILE = os.path.join(ROOT, "profiles", "pmc_valu_step_256.md")
+ISSUE_PROFILE = os.path.join(ROOT, "profiles", "pmc_issue_step_256.md")
 # MI355X: 256 CUs x 4 SIMDs, a wave64 fp32 VALU instruction takes 4 cycles on
 # a 16-lane SIMD (no packed FP32 here), 2.4 GHz peak engine clock
 VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+N_CU = 256
+# one vector-memory address unit (TA) per CU, busy at most every cycle of the
+# 2.4 GHz peak engine clock
+TA_PEAK_CYCLES_PER_S = 2.4e9
 
 
 def pmc_valu(db_path):
@@ -253,17 +257,64 @@ def pmc_valu(db_path):
     return "\n".join(lines)
 
 
-def committed_valu(kernel):
-    """VALU instructions per wave of `kernel` from the committed SQ summary."""
+def pmc_issue(sq_db, ta_db):
+    """Markdown table of instructions issued per wave by kind (VALU, vector
+    memory, LDS, scalar) per kernel from a `--pmc SQ_WAVES SQ_INSTS_VALU
+    SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_INSTS_LDS SQ_INSTS_SALU` run, and the
+    busy fraction of the vector-memory address unit (TA) from a `--pmc
+    TA_TA_BUSY_sum GRBM_GUI_ACTIVE` run: TA busy cycles summed over the CUs
+    over CUs x GPU-active cycles."""
+    import sqlite3
+
+    def per_kernel(path):
+        db = sqlite3.connect(path)
+        acc = {}
+        for cname, name, val in db.execute("select counter_name, name, avg(counter_value) from pmc_events "
+                                           "group by counter_name, name"):
+            acc.setdefault(name.split("(")[0].replace("void ", ""), {})[cname] = val
+        return acc
+
+    sq, ta = per_kernel(sq_db), per_kernel(ta_db)
+    lines = ["| kernel | VALU/wave | VMEM/wave (rd + wr) | LDS/wave | SALU/wave | TA busy cycles/CU | TA busy | "
+             "GPU-active cycles |", "|---|---|---|---|---|---|---|---|"]
+    rows = []
+    for k, d in sq.items():
+        w = d.get("SQ_WAVES", 0)
+        if w <= 0:
+            continue
+        t = ta.get(k, {})
+        cu_busy = t["TA_TA_BUSY_sum"] / N_CU if "TA_TA_BUSY_sum" in t else None
+        busy = cu_busy / t["GRBM_GUI_ACTIVE"] if cu_busy is not None and t.get("GRBM_GUI_ACTIVE") else None
+        rows.append((t.get("GRBM_GUI_ACTIVE", 0), k, d.get("SQ_INSTS_VALU", 0) / w,
+                     (d.get("SQ_INSTS_VMEM_RD", 0) + d.get("SQ_INSTS_VMEM_WR", 0)) / w, d.get("SQ_INSTS_LDS", 0) / w,
+                     d.get("SQ_INSTS_SALU", 0) / w, cu_busy, busy))
+    for cyc, k, va, vm, ld, sa, cub, busy in sorted(rows, reverse=True)[:30]:
+        b = f"{busy:.2f}" if busy is not None else "-"
+        c = f"{cub:.4g}" if cub is not None else "-"
+        lines.append(f"| {k} | {va:.1f} | {vm:.1f} | {ld:.1f} | {sa:.1f} | {c} | {b} | {cyc:.4g} |")
+    return "\n".join(lines)
+
+
+def committed_issue(kernel):
+    """Per-wave instruction mix and TA busy cycles of `kernel` from the
+    committed instruction-issue summary (None if absent)."""
     try:
-        with open(VALU_PROFILE) as fh:
+        with open(ISSUE_PROFILE) as fh:
             for line in fh:
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cells) == 2 and cells[0] == kernel:
-                    return float(cells[1])
+                if len(cells) == 8 and cells[0] == kernel:
+                    return {"valu": float(cells[1]), "vmem": float(cells[2]), "lds": float(cells[3]),
+                            "salu": float(cells[4]), "ta_busy_cycles_per_cu": float(cells[5]),
+                            "ta_busy": float(cells[6])}
     except (OSError, ValueError):
         pass
     return None
+
+
+def committed_valu(kernel):
+    """VALU instructions per wave of `kernel` from the committed summary."""
+    d = committed_issue(kernel)
+    return d["valu"] if d else None
 
 
 def committed_traffic(kernel):
@@ -389,8 +440,9 @@ def kernel_rooflines(sim, torch, committed=True):
     """G2P (short reinitialisation), the fused midpoint grid pass and P2G
     (tile + combine over the last step's records) of the 3D step, each timed
     alone with HIP events on the launch stream (the current torch stream),
-    against HBM (their algorithmic bytes) and VALU issue (the committed
-    per-wave instruction counts).  Algorithmic bytes: G2P reads pos 12 and
+    against HBM (their algorithmic bytes), VALU issue and the vector-memory
+    address unit (the committed per-wave instruction counts and TA busy
+    cycles; the P2G tile kernel runs one wave per 64 cells and face axis).  Algorithmic bytes: G2P reads pos 12 and
     writes C 36 per particle; the midpoint pass reads the velocity samples
     (counted once, 4 B per face) and writes psi 12, T 12 and u 4 per face of
     every axis (32 B / face); P2G reads each 64 B record and its permutation
@@ -418,17 +470,19 @@ def kernel_rooflines(sim, torch, committed=True):
          lambda: core.g2p(res, dx, sim.pos, sim.u, None, sim.init_C)),
         ("midpoint_advect", "pfm::k_midpoint_advect<3>", 32.0 * faces, faces,
          lambda: core.midpoint_advect(res, dx, sim.p2g_weight, sim.u, sim.psi_grid, sim.T_grid, 1e-4)),
-        ("p2g", "pfm::k_p2g_tile<3>", 68.0 * n + 8.0 * faces, cells,
+        ("p2g", "pfm::k_p2g_tile3", 68.0 * n + 8.0 * faces, 3 * cells,
          lambda: sim.binning.p2g_from_records(dx, n, sim.u, W=sim.p2g_weight)),
     ]
     for name, kname, alg, lanes, fn in specs:
         ms = timed(fn)
         gbs = alg / (ms * 1e-3) / 1e9
         row = {"kernel": kname, "ms": ms, "alg_bytes": alg, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
-        vpw = committed_valu(kname) if committed else None
-        if vpw is not None:
-            rate = vpw * math.ceil(lanes / 64) / (ms * 1e-3)
-            row.update({"valu_insts_per_wave": vpw, "valu_frac": rate / VALU_PEAK_WAVE_INSTS})
+        iss = committed_issue(kname) if committed else None
+        if iss is not None:
+            rate = iss["valu"] * math.ceil(lanes / 64) / (ms * 1e-3)
+            row.update({"valu_insts_per_wave": iss["valu"], "valu_frac": rate / VALU_PEAK_WAVE_INSTS,
+                        "vmem_insts_per_wave": iss["vmem"],
+                        "ta_frac": iss["ta_busy_cycles_per_cu"] / (ms * 1e-3) / TA_PEAK_CYCLES_PER_S})
         out[name] = row
     return out
 
@@ -587,6 +641,8 @@ def main():
     ap.add_argument("--by-grid", action="store_true", help="with --summarize: split rows by launch size")
     ap.add_argument("--match", default=None, help="with --summarize --by-grid: keep kernels containing this")
     ap.add_argument("--pmc-valu", metavar="DB", help="print the VALU-instructions-per-wave table of an SQ run and exit")
+    ap.add_argument("--pmc-issue", nargs=2, metavar=("SQ_DB", "TA_DB"),
+                    help="print the per-wave instruction mix and TA busy table of an SQ run and a TA run and exit")
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="print the per-kernel HBM traffic table of two --traffic-probe PMC runs and exit")
     args = ap.parse_args()
@@ -598,6 +654,9 @@ def main():
         return
     if args.pmc_valu:
         print(pmc_valu(args.pmc_valu))
+        return
+    if args.pmc_issue:
+        print(pmc_issue(*args.pmc_issue))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the step shards by slab "
@@ -761,17 +820,26 @@ def main():
     roof_kernel = ("pfm::k_rk4_particles<3, true, true, true>" if fold
                    else "pfm::k_rk4_particles<3, true, false, false>")
     traffic = committed_traffic(roof_kernel) if args.res == 256 else None
-    # the kernel is VALU-issue bound: its instruction rate against the peak
-    # (the binding resource); its HBM rate beside it
-    vpw = committed_valu(roof_kernel) if args.res == 256 else None
+    # The kernel is bound by the vector-memory address unit (TA): its 155
+    # vector-memory instructions per wave (108 of them the 12-byte row
+    # gathers of the four velocity samples) keep the TA busy 0.95 of the GPU's
+    # active cycles in the committed profile.  Its TA busy cycles per CU per
+    # launch over the live launch time, against one TA cycle per engine cycle
+    # at the 2.4 GHz peak clock; VALU issue and HBM beside it.
+    iss = committed_issue(roof_kernel) if args.res == 256 else None
     hbm = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
            "alg_bytes_per_launch": alg_bytes}
-    if vpw is not None:
+    if iss is not None:
         waves = math.ceil(sim.n / 64)
-        rate = vpw * waves / (rk4_ms * 1e-3)
-        roofline = {"kernel": roof_kernel, "bound": "valu", "achieved": rate, "peak": VALU_PEAK_WAVE_INSTS,
-                    "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK_WAVE_INSTS, "traffic": traffic,
-                    "kernel_ms": rk4_ms, "insts_per_wave": vpw, "waves_per_launch": waves, "hbm": hbm}
+        rate = iss["valu"] * waves / (rk4_ms * 1e-3)
+        ta_rate = iss["ta_busy_cycles_per_cu"] / (rk4_ms * 1e-3)
+        valu = {"achieved": rate, "peak": VALU_PEAK_WAVE_INSTS, "unit": "wave64 VALU instructions/s",
+                "frac": rate / VALU_PEAK_WAVE_INSTS, "insts_per_wave": iss["valu"]}
+        roofline = {"kernel": roof_kernel, "bound": "vmem", "achieved": ta_rate, "peak": TA_PEAK_CYCLES_PER_S,
+                    "unit": "TA (vector-memory address unit) busy cycles/s per CU",
+                    "frac": ta_rate / TA_PEAK_CYCLES_PER_S, "traffic": traffic, "kernel_ms": rk4_ms,
+                    "vmem_insts_per_wave": iss["vmem"], "ta_busy_of_active_cycles": iss["ta_busy"],
+                    "waves_per_launch": waves, "valu": valu, "hbm": hbm}
     else:
         roofline = {"kernel": roof_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms": rk4_ms,

@@ -821,6 +821,10 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
     static_assert(NT % kBlock == 0, "whole tile per thread sweep");
     __shared__ float ps[NS];
     __shared__ double sh[kBlock / 64];
+    // past convergence the search direction stays as it is (a second idle
+    // iteration would otherwise write its partner buffer, the current p) and
+    // nothing reads Ap or the partials
+    if (pcg_stopped(sc)) return;
     int t[3];
     tile_of_block(tm, t);
     const int o0 = t[0] * T::I, o1 = t[1] * T::J, o2 = t[2] * T::K;
@@ -839,9 +843,6 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
     }
     const float beta = sc->beta;
     const float zi = sizeof(ZT) == 2 ? sc->vinv : 1.0f;  // 16-bit z is stored scaled
-    // past convergence the search direction stays as it is: a second idle
-    // iteration would otherwise write its partner buffer, the current p
-    const bool keep = pcg_stopped(sc);
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
         const int e = tid + k * kBlock;
@@ -869,10 +870,8 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
         const float a = fluid ? __fsub_rn(__fmul_rn(nf, pq), nsum_lds<DIM>(ps + le, SP, SK)) : 0.f;
         if (inside(L, g0, g1, g2)) {
             const int q = (g0 * L.d[1] + g1) * L.d[2] + g2;
-            if (!keep) {
-                p_new[q] = pq;
-                Ap[q] = a;
-            }
+            p_new[q] = pq;
+            Ap[q] = a;
             s += (double)pq * (double)a;
         }
     }
@@ -1206,18 +1205,17 @@ __global__ void __launch_bounds__(kBlock) k_update_xr(int64_t n, float* __restri
                                                       const MGScalars* __restrict__ sc,
                                                       double* __restrict__ partial) {
     __shared__ double sh[kBlock / 64];
+    // past convergence x and r stay as they are (and nothing reads the
+    // partials: k_finalize leaves the scalars alone)
+    if (pcg_stopped(sc)) return;
     const float alpha = sc->alpha;
-    // past convergence x and r stay as they are (the partials go unused)
-    const bool keep = pcg_stopped(sc);
     double s2 = 0.0, s1 = 0.0, s3 = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x) {
         const float xq = __fadd_rn(x[q], __fmul_rn(alpha, p[q]));
         const float rq = __fsub_rn(r[q], __fmul_rn(alpha, Ap[q]));
-        if (!keep) {
-            x[q] = xq;
-            r[q] = rq;
-        }
+        x[q] = xq;
+        r[q] = rq;
         s2 += (double)rq * (double)rq;
         s1 += (double)rq;
         if (XX) s3 += (double)xq * (double)xq;
