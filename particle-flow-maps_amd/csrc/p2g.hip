@@ -170,6 +170,13 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(int64_t n, const int32_t
 }
 
 // Ascending ids inside each cell (restores the stable order the atomics lost).
+// A cell of up to kSortReg particles is held in registers under static
+// indices: if its ids already ascend (the usual case: a run of equal cells
+// in one wave gets its slots in lane order) nothing is written; otherwise
+// each id goes to its rank among the cell's ids (distinct, so the ranks are
+// a permutation).  Larger cells sort in place in global memory.
+constexpr int kSortReg = 16;
+
 __global__ void __launch_bounds__(kBlock) k_sort_cells(int64_t C, const int32_t* __restrict__ start,
                                                        int32_t* __restrict__ perm) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -177,20 +184,25 @@ __global__ void __launch_bounds__(kBlock) k_sort_cells(int64_t C, const int32_t*
     const int32_t s = start[c], e = start[c + 1];
     const int m = e - s;
     if (m <= 1) return;
-    constexpr int kReg = 32;
-    if (m <= kReg) {
-        int32_t v[kReg];
-        for (int q = 0; q < m; ++q) v[q] = perm[s + q];
-        for (int q = 1; q < m; ++q) {
-            int32_t key = v[q];
-            int r = q - 1;
-            while (r >= 0 && v[r] > key) {
-                v[r + 1] = v[r];
-                --r;
-            }
-            v[r + 1] = key;
+    if (m <= kSortReg) {
+        int32_t v[kSortReg];
+#pragma unroll
+        for (int q = 0; q < kSortReg; ++q) v[q] = q < m ? perm[s + q] : 0x7fffffff;
+        bool sorted = true;
+#pragma unroll
+        for (int q = 1; q < kSortReg; ++q) sorted = sorted && (q >= m || v[q - 1] < v[q]);
+        if (sorted) return;
+        int rank[kSortReg];
+#pragma unroll
+        for (int q = 0; q < kSortReg; ++q) {
+            int k = 0;
+#pragma unroll
+            for (int r = 0; r < kSortReg; ++r) k += v[r] < v[q] ? 1 : 0;
+            rank[q] = k;
         }
-        for (int q = 0; q < m; ++q) perm[s + q] = v[q];
+#pragma unroll
+        for (int q = 0; q < kSortReg; ++q)
+            if (q < m) perm[s + rank[q]] = v[q];
     } else {
         for (int q = 1; q < m; ++q) {
             int32_t key = perm[s + q];
