@@ -234,7 +234,7 @@ __device__ __forceinline__ void g2p_particle(const GridDev& g, int64_t i, int64_
 }
 
 template <int DIM, bool GRAD>
-__global__ void __launch_bounds__(kBlock, 2) k_g2p(GridDev g, int64_t n, const float* __restrict__ pos,
+__global__ void __launch_bounds__(kBlock, 6) k_g2p(GridDev g, int64_t n, const float* __restrict__ pos,
                                                 const float* __restrict__ ux,
                                                 const float* __restrict__ uy,
                                                 const float* __restrict__ uz,
@@ -242,7 +242,27 @@ __global__ void __launch_bounds__(kBlock, 2) k_g2p(GridDev g, int64_t n, const f
                                                 float* __restrict__ Cy, float* __restrict__ Cz) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    g2p_particle<DIM, GRAD>(g, i, n, pos, ux, uy, uz, imp, Cx, Cy, Cz);
+    if constexpr (DIM == 3) {
+        // one component per trip of a rolled loop: the registers of one
+        // component's stencil and rows are live at a time
+        float p[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = pos[c * n + i];
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a) {
+            const float* fld = a == 0 ? ux : (a == 1 ? uy : uz);
+            float val, grad[3];
+            sample_component_rt<GRAD>(fld, g, a, p, val, grad);
+            if (imp != nullptr) imp[a * n + i] = val;
+            if (GRAD) {
+                float* Ca = a == 0 ? Cx : (a == 1 ? Cy : Cz);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) Ca[c * n + i] = grad[c];
+            }
+        }
+    } else {
+        g2p_particle<DIM, GRAD>(g, i, n, pos, ux, uy, uz, imp, Cx, Cy, Cz);
+    }
 }
 
 template <bool GRAD>

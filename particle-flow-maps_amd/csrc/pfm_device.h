@@ -194,6 +194,56 @@ __device__ __forceinline__ void combine_rows(const GridDev& g, const MacStencil<
     }
 }
 
+// One MAC component `a` (chosen at run time) of a 3D face field at p, value
+// and (if GRAD) gradient: the face stencil along a, the cell stencils across,
+// the rows and their separable contraction exactly as load_rows and
+// combine_rows form them for that component, so the result is bit-identical
+// to interp_mac's; only one component's registers are live at a time.
+template <bool GRAD>
+__device__ __forceinline__ void sample_component_rt(const float* __restrict__ fld, const GridDev& g, int a,
+                                                    const float p[3], float& val, float grad[3]) {
+    AxisStencil s[3];
+    int S[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const bool face = c == a;
+        axis_stencil<GRAD>(p[c], g.inv_dx, face ? 0.0f : 0.5f, face ? g.hi_f[c] : g.hi_c[c], s[c]);
+        S[c] = g.res[c] + (face ? 1 : 0);
+    }
+    const uint32_t rowb = (uint32_t)S[2] * 4u;
+    const uint32_t plane = (uint32_t)S[1] * rowb;
+    const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, plane * (uint32_t)S[0]);
+    const uint32_t base = (uint32_t)s[0].j0 * plane + (uint32_t)s[1].j0 * rowb + (uint32_t)s[2].j0 * 4u;
+    float v0 = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float c_v = 0.f, c_dy = 0.f, c_dz = 0.f;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const v3f q = __builtin_amdgcn_raw_buffer_load_b96(rs, base + i * plane + j * rowb, 0, 0);
+            const float rz = s[2].w[0] * q.x + s[2].w[1] * q.y + s[2].w[2] * q.z;
+            c_v += s[1].w[j] * rz;
+            if (GRAD) {
+                const float rdz = s[2].dw[0] * q.x + s[2].dw[1] * q.y + s[2].dw[2] * q.z;
+                c_dy += s[1].dw[j] * rz;
+                c_dz += s[1].w[j] * rdz;
+            }
+        }
+        v0 += s[0].w[i] * c_v;
+        if (GRAD) {
+            gx += s[0].dw[i] * c_v;
+            gy += s[0].w[i] * c_dy;
+            gz += s[0].w[i] * c_dz;
+        }
+    }
+    val = v0;
+    if (GRAD) {
+        grad[0] = gx * g.inv_dx;
+        grad[1] = gy * g.inv_dx;
+        grad[2] = gz * g.inv_dx;
+    }
+}
+
 // interp_u_MAC_grad (3D/taichi_utils.py:312-317): u[a] and G[a][c] = du_a/dx_c.
 template <int DIM, bool GRAD>
 __device__ __forceinline__ void interp_mac(const float* __restrict__ ux,
