@@ -88,6 +88,31 @@ def test_binning_empty_and_crowded_cells(oracle, pfm):
     np.testing.assert_array_equal(N(b.cell_start), start)
 
 
+def test_binning_cells_around_the_register_sort_size(oracle, pfm):
+    """Cells of 15, 16 and 17 particles (the per-cell sort keeps up to 16 in
+    registers and places them by rank; more sort in memory), with the
+    particle ids of every cell shuffled across the array so that the count
+    atomics of different waves interleave, plus single and empty cells."""
+    from pfm import core
+
+    res = (6, 5, 7)
+    dx = 1 / 5
+    C = int(np.prod(res))
+    rng = np.random.default_rng(11)
+    sizes = rng.choice([0, 1, 15, 16, 17], size=C)
+    cells = np.repeat(np.arange(C), sizes)
+    rng.shuffle(cells)
+    n = cells.size
+    ijk = np.stack(np.unravel_index(cells, res))
+    pos = ((ijk + rng.uniform(0.05, 0.95, size=ijk.shape)) * dx).astype(np.float32)
+    cell, counts, start, perm = oracle.bin_particles(res, dx, pos)
+    np.testing.assert_array_equal(counts, np.bincount(cells, minlength=C))
+    b = core.Binning(res, n)
+    b.bin(G(pos), dx)
+    np.testing.assert_array_equal(N(b.cell_start), start)
+    np.testing.assert_array_equal(N(b.perm), perm)
+
+
 # ---------------------------------------------------------------- RK4
 @pytest.mark.parametrize("res", [(32, 32), (16, 16, 16), (12, 16, 10)])
 def test_rk4_particles(oracle, pfm, res):
