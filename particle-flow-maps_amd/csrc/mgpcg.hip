@@ -890,7 +890,7 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
 constexpr int kCoarseMaxLevels = 4;
 constexpr int kCoarseLds = 14336;  // floats (56 KiB)
 constexpr int kCoarseSlots = 4;    // bottom-level cells per colour per thread
-constexpr int kCoarseThreads = 512;
+constexpr int kCoarseThreads = 1024;
 
 constexpr int kCoarseBottom64 = 4096;  // padded bottom cells of the FP64 bottom solve
 
