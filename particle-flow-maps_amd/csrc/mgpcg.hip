@@ -1271,10 +1271,14 @@ enum FinalizeMode { kFinMean = 0, kFinR0 = 1, kFinZTr = 2, kFinAlpha = 3, kFinRT
 // test: after r.r, stop the solve when it is below the threshold (the
 // convergence test of 3D/mgpcg.py:255); kFinBeta counts the iteration and
 // stops the solve at max_it.
-__global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ partial, int nparts, int mode,
+// One block of kFinThreads: the partial sums (up to one per level-0 tile)
+// spread over more lanes shorten the serial part of this one-block launch.
+constexpr int kFinThreads = 1024;
+
+__global__ void __launch_bounds__(kFinThreads) k_finalize(const double* __restrict__ partial, int nparts, int mode,
                                                      double eps, double ncell, int vshift, int test, MGScalars* sc) {
     if (pcg_stopped(sc)) return;
-    __shared__ double sh[kBlock / 64];
+    __shared__ double sh[kFinThreads / 64];
     double s = 0.0;
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += partial[b];
     s = block_sum(s, sh);
@@ -1846,7 +1850,7 @@ static int vcycle(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
 }
 
 static int finalize(MGHandle* h, int mode, double eps, hipStream_t s, int nparts = kMgRedBlocks, int test = 0) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, h->partial, nparts, mode, eps,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, s, h->partial, nparts, mode, eps,
                        (double)h->n_fluid, h->vshift, test, h->sc);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
