@@ -356,8 +356,10 @@ __global__ void __launch_bounds__(kBlock) k_restrict_any(LvlDev Lf, LvlDev Lc, f
 // Buffer view of a level array: the tiled kernels load through 32-bit byte
 // offsets, and an offset past the end (kOOB) reads 0, which is the value the
 // LDS tiles hold for cells outside the level.  Loads are issued without
-// branches, so all of a thread's loads can be in flight at once.
-constexpr uint32_t kOOB = 0xFFFFFFF0u;
+// branches, so all of a thread's loads can be in flight at once.  kOOB plus
+// the widest load (16 bytes) stays below 2^32, so the range check cannot wrap,
+// and it exceeds every level array's size (pfm_mg_create caps the grid).
+constexpr uint32_t kOOB = 0xFFFFFFC0u;
 
 // Level arrays are stored as V = float, or (mixed precision) __half or
 // __hip_bfloat16; every kernel computes in fp32 and rounds to nearest on store.
@@ -703,6 +705,365 @@ __global__ void __launch_bounds__(kSweepThreads, 3) k_rbgs(LvlDev L, LvlDev Lc, 
                 if (g0 < d0) zout[g0 * plane + base] = from_f<ZT>(zs[((ti + HI) * SJ + tj + H) * SK + tk + H]);
             }
         }
+    }
+    if (DOT) {
+        __shared__ double sh[NT / 64];
+        s = block_sum(s, sh);
+        if (tid == 0) partial[blockIdx.x] = kScale ? s * (double)sc->vinv : s;  // z was stored scaled
+    }
+}
+
+// ---- 3D sweeps on groups of 4 cells ---------------------------------------
+// Four consecutive cells along axis 2 starting at a multiple of 4 (a "group")
+// move as one 16-byte load or store (8 bytes at 16-bit storage).  On a level
+// whose axis-2 extent is a multiple of 4 a group lies wholly inside or wholly
+// outside the level, so one bounds test covers its four cells.
+//
+// The 16- and 8-byte loads call the LLVM buffer-load intrinsics by name: this
+// compiler lowers __builtin_amdgcn_raw_buffer_load_b64 / _b128 to a single
+// dword load.
+typedef float pfm_v4f __attribute__((ext_vector_type(4)));
+typedef float pfm_v2f __attribute__((ext_vector_type(2)));
+__device__ pfm_v4f pfm_buffer_load_v4f32(__amdgpu_buffer_rsrc_t rs, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+__device__ pfm_v2f pfm_buffer_load_v2f32(__amdgpu_buffer_rsrc_t rs, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+
+__device__ __forceinline__ unsigned short lo16(float w) { return (unsigned short)(__builtin_bit_cast(uint32_t, w)); }
+__device__ __forceinline__ unsigned short hi16(float w) {
+    return (unsigned short)(__builtin_bit_cast(uint32_t, w) >> 16);
+}
+
+template <typename V>
+__device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t rs, uint32_t boff) {
+    float4 v;
+    if constexpr (sizeof(V) == 4) {
+        const pfm_v4f q = pfm_buffer_load_v4f32(rs, (int)boff, 0, 0);
+        v.x = q[0];
+        v.y = q[1];
+        v.z = q[2];
+        v.w = q[3];
+    } else {
+        const pfm_v2f q = pfm_buffer_load_v2f32(rs, (int)boff, 0, 0);
+        v.x = to_f(__builtin_bit_cast(V, lo16(q[0])));
+        v.y = to_f(__builtin_bit_cast(V, hi16(q[0])));
+        v.z = to_f(__builtin_bit_cast(V, lo16(q[1])));
+        v.w = to_f(__builtin_bit_cast(V, hi16(q[1])));
+    }
+    return v;
+}
+
+// Two consecutive values (the coarse parents of a group).
+template <typename V>
+__device__ __forceinline__ float2 ld2(__amdgpu_buffer_rsrc_t rs, uint32_t boff) {
+    float2 v;
+    if constexpr (sizeof(V) == 4) {
+        const pfm_v2f q = pfm_buffer_load_v2f32(rs, (int)boff, 0, 0);
+        v.x = q[0];
+        v.y = q[1];
+    } else {
+        const uint32_t q = __builtin_amdgcn_raw_buffer_load_b32(rs, boff, 0, 0);
+        v.x = to_f(__builtin_bit_cast(V, (unsigned short)(q & 0xffffu)));
+        v.y = to_f(__builtin_bit_cast(V, (unsigned short)(q >> 16)));
+    }
+    return v;
+}
+
+template <typename V>
+__device__ __forceinline__ void st4(V* p, uint32_t idx, float4 v) {
+    if constexpr (sizeof(V) == 4) {
+        *reinterpret_cast<float4*>(p + idx) = v;
+    } else {
+        uint2 w;
+        w.x = (uint32_t)__builtin_bit_cast(unsigned short, from_f<V>(v.x)) |
+              ((uint32_t)__builtin_bit_cast(unsigned short, from_f<V>(v.y)) << 16);
+        w.y = (uint32_t)__builtin_bit_cast(unsigned short, from_f<V>(v.z)) |
+              ((uint32_t)__builtin_bit_cast(unsigned short, from_f<V>(v.w)) << 16);
+        *reinterpret_cast<uint2*>(p + idx) = w;
+    }
+}
+
+// One colour pass over one group held in an LDS tile: the group's cells of
+// colour c are the two at offsets s and s + 2 (s = (c + g0 + g1) & 1, the
+// group starting at an even cell).  Their axis-0 / axis-1 neighbours come
+// from the neighbouring groups (16-byte LDS reads), the axis-2 neighbours
+// from the group itself and the single cells just outside it.
+struct GroupPass {
+    float4 v;        // the group after the pass
+    float za, zb;    // the new values of the two cells
+    float ra, rb;    // their r as used (recentred, unscaled)
+    bool ua, ub;     // whether each was updated
+};
+
+constexpr int kSw3Threads = 256;
+
+// Items of the first pass of k_sweep3: the tile's groups first, numbered as
+// the second pass numbers them (so a thread's r of its first four items is
+// also the r of its second-pass groups), then the ring along axes 0 / 1 at
+// the tile's axis-2 groups, then the edge groups along axis 2.  (ri, rj, lg)
+// are the item's offsets from the tile origin minus one cell on axes 0 / 1
+// and minus one group on axis 2.
+template <int TI, int TJ, int TG>
+__device__ __forceinline__ void sweep3_item(int e, int& ri, int& rj, int& lg) {
+    constexpr int NTILE = TI * TJ * TG, RW = 2 * (TJ + 2) + 2 * TI, NRING = RW * TG;
+    if (e < NTILE) {
+        lg = 1 + e % TG;
+        const int q = e / TG;
+        rj = 1 + q % TJ;
+        ri = 1 + q / TJ;
+    } else if (e < NTILE + NRING) {
+        const int u = e - NTILE;
+        lg = 1 + u % TG;
+        const int w = u / TG;  // position on the ring of the (TI + 2) x (TJ + 2) rectangle
+        const int wa = w - (TJ + 2), wb = w - 2 * (TJ + 2), wc = wb - TI;
+        ri = w < TJ + 2 ? 0 : (wa < TJ + 2 ? TI + 1 : 1 + (wb < TI ? wb : wc));
+        rj = w < TJ + 2 ? w : (wa < TJ + 2 ? wa : (wb < TI ? 0 : TJ + 1));
+    } else {
+        const int u = e - NTILE - NRING;
+        lg = (u & 1) ? TG + 1 : 0;
+        const int pos = u >> 1;
+        rj = pos % (TJ + 2);
+        ri = pos / (TJ + 2);
+    }
+}
+
+// The two colour passes of k_rbgs for 3D levels with an axis-2 extent that is
+// a multiple of 4, on groups.  The LDS tile holds z on the 8 x 16 x 32 tile
+// plus 2 planes and 2 rows on axes 0 and 1 and one group (4 cells) on each
+// side along axis 2.  The first pass covers the tile plus its one-cell ring
+// along axes 0 and 1 and one group each side along axis 2; of those edge
+// groups only the cell next to the tile is needed (the others are computed
+// from the edge of the LDS tile and never read).  Each group of the second
+// pass (the tile) is written to z_out from registers by the thread that
+// updated it.  With a zero input the first pass reads nothing from LDS (its
+// neighbours are all +0) and writes every group the second pass reads, so
+// the tile is not filled first.  Per cell the arithmetic is exactly k_rbgs's
+// (and so the reference's smooth, 3D/mgpcg.py:165-172): both give the same bits.
+template <typename ZT, typename RT, bool ZERO_IN, bool PROLONG, bool SUB_MEAN, bool DOT>
+__global__ void __launch_bounds__(kSw3Threads, 4) k_sweep3(LvlDev L, LvlDev Lc, TileMap tm,
+                                                       const ZT* __restrict__ zin, const ZT* __restrict__ zc,
+                                                       const RT* __restrict__ r, ZT* __restrict__ zout, int c0,
+                                                       const MGScalars* __restrict__ sc,
+                                                       double* __restrict__ partial) {
+    using T = MgTile<3>;
+    constexpr int NT = kSw3Threads;
+    constexpr int TG = T::K / 4;                            // groups per tile row
+    constexpr int GK = TG + 2;                              // groups per LDS row
+    constexpr int SI = T::I + 4, SJ = T::J + 4, SK = 4 * GK, SP = SJ * SK;
+    constexpr int NG = SI * SJ * GK, ZR = (NG + NT - 1) / NT;               // z tile groups, rounds
+    constexpr int P0G = (T::I + 2) * (T::J + 2) * GK, P0R = (P0G + NT - 1) / NT;  // first pass
+    constexpr int P1G = T::I * T::J * TG, P1R = P1G / NT;  // second pass: the tile
+    static_assert(P1G % NT == 0 && P1R <= P0R, "the tile fills whole rounds of the first pass");
+    // 4 floats of slack each side (the LDS tile's outer groups are never first-pass items, so
+    // every LDS read stays inside the tile; the slack keeps that true for any tile change)
+    __shared__ __align__(16) float zsm[SI * SP + 8];
+    float* const zs = zsm + 4;
+    int t[3];
+    tile_of_block(tm, t);
+    const int o0 = t[0] * T::I, o1 = t[1] * T::J, o2 = t[2] * T::K;
+    const int b0 = o0 - 2, b1 = o1 - 2, bk = o2 - 4;  // global cell of LDS (0, 0, 0)
+    const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+    const int tid = threadIdx.x;
+    const int c1 = c0 ^ 1;
+    const bool solid = L.dg != nullptr;
+    const __amdgpu_buffer_rsrc_t rr = lvl_rsrc(r, L);
+    const __amdgpu_buffer_rsrc_t rd = solid ? dg_rsrc(L) : rr;
+    auto in3 = [&](int g0, int g1, int gk) {
+        return (unsigned)g0 < (unsigned)d0 && (unsigned)g1 < (unsigned)d1 && (unsigned)gk < (unsigned)d2;
+    };
+    auto cell = [&](int g0, int g1, int gk) { return (uint32_t)((g0 * d1 + g1) * d2 + gk); };
+    // ---- z tile
+    float4 zv[ZERO_IN ? 1 : ZR];
+    if (!ZERO_IN) {
+        const __amdgpu_buffer_rsrc_t rz = lvl_rsrc(zin, L);
+        const __amdgpu_buffer_rsrc_t rc = lvl_rsrc(zc, Lc);
+#pragma unroll
+        for (int m = 0; m < ZR; ++m) {
+            const int e = tid + m * NT;
+            const int lg = e % GK, q = e / GK, lj = q % SJ, li = q / SJ;
+            const int g0 = b0 + li, g1 = b1 + lj, gk = bk + 4 * lg;
+            const bool in = e < NG && in3(g0, g1, gk);
+            const uint32_t off = cell(g0, g1, gk);
+            zv[m] = ld4<ZT>(rz, in ? off * (uint32_t)sizeof(ZT) : kOOB);
+            if (PROLONG) {
+                // prolongate (3D/mgpcg.py:160-163): cells 4q..4q+3 have the
+                // coarse parents 2q, 2q, 2q+1, 2q+1 along axis 2
+                const int p0 = min(g0 >> 1, Lc.d[0] - 1), p1 = min(g1 >> 1, Lc.d[1] - 1);
+                const uint32_t co = (uint32_t)((p0 * Lc.d[1] + p1) * Lc.d[2] + (gk >> 1));
+                const float2 zp = ld2<ZT>(rc, in ? co * (uint32_t)sizeof(ZT) : kOOB);
+                zv[m].x = __fadd_rn(zv[m].x, zp.x);
+                zv[m].y = __fadd_rn(zv[m].y, zp.x);
+                zv[m].z = __fadd_rn(zv[m].z, zp.y);
+                zv[m].w = __fadd_rn(zv[m].w, zp.y);
+            }
+        }
+        if (PROLONG && solid) {
+            // solid cells take no prolongation (they carry no pressure)
+#pragma unroll
+            for (int m = 0; m < ZR; ++m) {
+                const int e = tid + m * NT;
+                const int lg = e % GK, q = e / GK, lj = q % SJ, li = q / SJ;
+                const int g0 = b0 + li, g1 = b1 + lj, gk = bk + 4 * lg;
+                const bool in = e < NG && in3(g0, g1, gk);
+                const uint32_t dw = __builtin_amdgcn_raw_buffer_load_b32(rd, in ? cell(g0, g1, gk) : kOOB, 0, 0);
+                zv[m].x = (dw & 0xffu) == 0 ? 0.f : zv[m].x;
+                zv[m].y = (dw & 0xff00u) == 0 ? 0.f : zv[m].y;
+                zv[m].z = (dw & 0xff0000u) == 0 ? 0.f : zv[m].z;
+                zv[m].w = (dw & 0xff000000u) == 0 ? 0.f : zv[m].w;
+            }
+        }
+    }
+    // ---- r (and the obstacle's diagonals) of the first pass; items < P1G are the tile.
+    // The r of the ring items is requested only once the z tile is in LDS, so the
+    // z and r registers in flight stay within four waves per SIMD.
+    float4 rv[P0R];
+    uint32_t dgv[P0R];
+    auto load_r = [&](int m) {
+        const int e = tid + m * NT;
+        int ri, rj, lg;
+        sweep3_item<T::I, T::J, TG>(e, ri, rj, lg);
+        const int g0 = o0 - 1 + ri, g1 = o1 - 1 + rj, gk = bk + 4 * lg;
+        const bool in = e < P0G && in3(g0, g1, gk);
+        rv[m] = ld4<RT>(rr, in ? cell(g0, g1, gk) * (uint32_t)sizeof(RT) : kOOB);
+        dgv[m] = 0u;
+    };
+#pragma unroll
+    for (int m = 0; m < P1R; ++m) load_r(m);
+    if (ZERO_IN) {
+#pragma unroll
+        for (int m = P1R; m < P0R; ++m) load_r(m);
+    }
+    if (!ZERO_IN) {
+#pragma unroll
+        for (int m = 0; m < ZR; ++m) {
+            const int e = tid + m * NT;
+            if (e < NG) {
+                const int lg = e % GK, q = e / GK, lj = q % SJ, li = q / SJ;
+                *reinterpret_cast<float4*>(zs + li * SP + lj * SK + 4 * lg) = zv[m];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int m = P1R; m < P0R; ++m) load_r(m);
+    }
+    if (solid) {
+#pragma unroll
+        for (int m = 0; m < P0R; ++m) {
+            const int e = tid + m * NT;
+            int ri, rj, lg;
+            sweep3_item<T::I, T::J, TG>(e, ri, rj, lg);
+            const int g0 = o0 - 1 + ri, g1 = o1 - 1 + rj, gk = bk + 4 * lg;
+            const bool in = e < P0G && in3(g0, g1, gk);
+            dgv[m] = __builtin_amdgcn_raw_buffer_load_b32(rd, in ? cell(g0, g1, gk) : kOOB, 0, 0);
+        }
+    }
+    const float mean = SUB_MEAN ? sc->mean : 0.f;
+    // 16-bit z: level 0 works on vs * r (coarser r arrive scaled)
+    constexpr bool kScale = sizeof(ZT) == 2 && sizeof(RT) == 4;
+    const float vs = kScale ? sc->vs : 1.0f;
+    const bool interior0 = !solid && box_interior(L, o0 - 1, o1 - 1, o2 - 1, o0 + T::I + 1, o1 + T::J + 1,
+                                                  o2 + T::K + 1, true);
+    const bool interior1 = !solid && box_interior(L, o0, o1, o2, o0 + T::I, o1 + T::J, o2 + T::K, true);
+    const float n_int = (float)L.nb2, y_int = pfm_rcp_small(L.nb2);
+    // `zero`: every z is +0 (the first pass of a zero input), nothing is read from LDS
+    auto pass = [&](int base, int g0, int g1, int gk, int c, bool in, bool intr, float4 rg, uint32_t dw,
+                    bool zero) -> GroupPass {
+        const bool s1 = ((c + g0 + g1) & 1) != 0;
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 own = zero ? z4 : *reinterpret_cast<const float4*>(zs + base);
+        const float4 ip = zero ? z4 : *reinterpret_cast<const float4*>(zs + base + SP);
+        const float4 im = zero ? z4 : *reinterpret_cast<const float4*>(zs + base - SP);
+        const float4 jp = zero ? z4 : *reinterpret_cast<const float4*>(zs + base + SK);
+        const float4 jm = zero ? z4 : *reinterpret_cast<const float4*>(zs + base - SK);
+        const float lf = zero ? 0.f : zs[base - 1], rt = zero ? 0.f : zs[base + 4];
+        const float kmid = s1 ? own.z : own.y;  // axis-2 +1 of cell a and -1 of cell b
+        // neighbor_sum order (3D/mgpcg.py:92-104): +e then -e per axis
+        float na = 0.0f, nb = 0.0f;
+        na = __fadd_rn(na, s1 ? ip.y : ip.x);
+        nb = __fadd_rn(nb, s1 ? ip.w : ip.z);
+        na = __fadd_rn(na, s1 ? im.y : im.x);
+        nb = __fadd_rn(nb, s1 ? im.w : im.z);
+        na = __fadd_rn(na, s1 ? jp.y : jp.x);
+        nb = __fadd_rn(nb, s1 ? jp.w : jp.z);
+        na = __fadd_rn(na, s1 ? jm.y : jm.x);
+        nb = __fadd_rn(nb, s1 ? jm.w : jm.z);
+        na = __fadd_rn(na, kmid);
+        nb = __fadd_rn(nb, s1 ? rt : own.w);
+        na = __fadd_rn(na, s1 ? own.x : lf);
+        nb = __fadd_rn(nb, kmid);
+        GroupPass o;
+        o.ra = s1 ? rg.y : rg.x;
+        o.rb = s1 ? rg.w : rg.z;
+        if (SUB_MEAN) {
+            o.ra = __fsub_rn(o.ra, mean);
+            o.rb = __fsub_rn(o.rb, mean);
+        }
+        const float va = kScale ? o.ra * vs : o.ra, vb = kScale ? o.rb * vs : o.rb;
+        const int ka = gk + (s1 ? 1 : 0), kb = ka + 2;
+        float nfa = n_int, ya = y_int, nfb = n_int, yb = y_int;
+        bool fa = true, fb = true;
+        if (solid) {
+            const int sa = s1 ? 8 : 0;
+            const int na_ = (int)((dw >> sa) & 0xffu), nb_ = (int)((dw >> (sa + 16)) & 0xffu);
+            nfa = (float)na_;
+            ya = pfm_rcp_small(na_);
+            fa = na_ != 0;
+            nfb = (float)nb_;
+            yb = pfm_rcp_small(nb_);
+            fb = nb_ != 0;
+        } else if (!intr) {
+            const int wa = L.nb2 - walls(L, g0, g1, ka), wb = L.nb2 - walls(L, g0, g1, kb);
+            nfa = (float)wa;
+            ya = pfm_rcp_small(wa);
+            nfb = (float)wb;
+            yb = pfm_rcp_small(wb);
+        }
+        o.za = pfm_div_small(__fadd_rn(va, na), nfa, ya);
+        o.zb = pfm_div_small(__fadd_rn(vb, nb), nfb, yb);
+        o.ua = in && fa;
+        o.ub = in && fb;
+        const float xa = o.ua ? o.za : (s1 ? own.y : own.x);
+        const float xb = o.ub ? o.zb : (s1 ? own.w : own.z);
+        o.v.x = s1 ? own.x : xa;
+        o.v.y = s1 ? xa : own.y;
+        o.v.z = s1 ? own.z : xb;
+        o.v.w = s1 ? xb : own.w;
+        return o;
+    };
+    double s = 0.0;
+    auto dot = [&](const GroupPass& o) {
+        if (o.ua) s += (double)to_f(from_f<ZT>(o.za)) * (double)o.ra;
+        if (o.ub) s += (double)to_f(from_f<ZT>(o.zb)) * (double)o.rb;
+    };
+    if (!ZERO_IN) __syncthreads();
+    // ---- pass 0 (colour c0): the tile, its ring along axes 0 / 1 and the edge groups
+#pragma unroll
+    for (int m = 0; m < P0R; ++m) {
+        const int e = tid + m * NT;
+        if (e < P0G) {
+            int ri, rj, lg;
+            sweep3_item<T::I, T::J, TG>(e, ri, rj, lg);
+            const int g0 = o0 - 1 + ri, g1 = o1 - 1 + rj, gk = bk + 4 * lg;
+            const int base = (ri + 1) * SP + (rj + 1) * SK + 4 * lg;
+            const GroupPass o = pass(base, g0, g1, gk, c0, in3(g0, g1, gk), interior0, rv[m], dgv[m], ZERO_IN);
+            *reinterpret_cast<float4*>(zs + base) = o.v;
+            // cells of colour c0 inside the tile are final after this pass
+            if (DOT && m < P1R) dot(o);
+        }
+    }
+    __syncthreads();
+    // ---- pass 1 (colour c1) on the tile, stored from registers
+#pragma unroll
+    for (int m = 0; m < P1R; ++m) {
+        const int e = tid + m * NT;
+        int ri, rj, lg;
+        sweep3_item<T::I, T::J, TG>(e, ri, rj, lg);
+        const int g0 = o0 - 1 + ri, g1 = o1 - 1 + rj, gk = bk + 4 * lg;
+        const int base = (ri + 1) * SP + (rj + 1) * SK + 4 * lg;
+        const bool in = in3(g0, g1, gk);
+        const GroupPass o = pass(base, g0, g1, gk, c1, in, interior1, rv[m], dgv[m], false);
+        if (DOT) dot(o);
+        if (in) st4<ZT>(zout, cell(g0, g1, gk), o.v);
     }
     if (DOT) {
         __shared__ double sh[NT / 64];
@@ -1648,7 +2009,31 @@ static void launch_rbgs(int kind, bool sub_mean, bool dot, dim3 g, hipStream_t s
 #undef PFM_RBGS
 }
 
+template <typename ZT, typename RT>
+static void launch_sweep3(int kind, bool sub_mean, bool dot, dim3 g, hipStream_t s, LvlDev L, LvlDev Lc, TileMap tm,
+                          const ZT* zin, const ZT* zc, const RT* r, ZT* zout, int c0, const MGScalars* sc,
+                          double* partial) {
+#define PFM_SW3(Z, P, M, D)                                                                                   \
+    hipLaunchKernelGGL((k_sweep3<ZT, RT, Z, P, M, D>), g, dim3(kSw3Threads), 0, s, L, Lc, tm, zin, zc, r, zout, \
+                       c0, sc, partial)
+    if (kind == kSweepZero) {
+        if (sub_mean) PFM_SW3(true, false, true, false);
+        else PFM_SW3(true, false, false, false);
+    } else if (kind == kSweepProlong) {
+        PFM_SW3(false, true, false, false);
+    } else if (sub_mean) {
+        PFM_SW3(false, false, true, false);
+    } else if (dot) {
+        PFM_SW3(false, false, false, true);
+    } else {
+        PFM_SW3(false, false, false, false);
+    }
+#undef PFM_SW3
+}
+
 // Two colour passes (c0, c0 ^ 1) on level l, z_in -> z_out (ZT arrays).
+// 3D levels whose axis-2 extent is a multiple of 4 run the group kernel
+// k_sweep3, the others k_rbgs (same tiles, same bits).
 template <typename ZT>
 static int rbgs(MGHandle* h, int l, int kind, int c0, const ZT* zin, ZT* zout, bool sub_mean, bool dot,
                 hipStream_t s) {
@@ -1658,15 +2043,20 @@ static int rbgs(MGHandle* h, int l, int kind, int c0, const ZT* zin, ZT* zout, b
     const TileMap tm = sweep_tiles(h, l);
     const dim3 g((unsigned)tm.total);
     // r is fp32 on level 0 (the PCG residual) and ZT below
+    const bool groups = h->dim == 3 && L.d[2] % 4 == 0;
     if (l == 0 || std::is_same<ZT, float>::value) {
         const float* r = (const float*)h->r[l];
-        if (h->dim == 3)
+        if (groups)
+            launch_sweep3<ZT, float>(kind, sub_mean, dot, g, s, L, Lc, tm, zin, zc, r, zout, c0, h->sc, h->partial);
+        else if (h->dim == 3)
             launch_rbgs<3, ZT, float>(kind, sub_mean, dot, g, s, L, Lc, tm, zin, zc, r, zout, c0, h->sc, h->partial);
         else
             launch_rbgs<2, ZT, float>(kind, sub_mean, dot, g, s, L, Lc, tm, zin, zc, r, zout, c0, h->sc, h->partial);
     } else {
         const ZT* r = (const ZT*)h->r[l];
-        if (h->dim == 3)
+        if (groups)
+            launch_sweep3<ZT, ZT>(kind, false, false, g, s, L, Lc, tm, zin, zc, r, zout, c0, h->sc, h->partial);
+        else if (h->dim == 3)
             launch_rbgs<3, ZT, ZT>(kind, false, false, g, s, L, Lc, tm, zin, zc, r, zout, c0, h->sc, h->partial);
         else
             launch_rbgs<2, ZT, ZT>(kind, false, false, g, s, L, Lc, tm, zin, zc, r, zout, c0, h->sc, h->partial);
@@ -2240,8 +2630,9 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     }
     int64_t cells = 1;
     for (int a = 0; a < dim; ++a) cells *= N[a];
-    // level arrays are addressed with 32-bit byte offsets
-    PFM_CHECK_ARG(cells < (int64_t)1 << 30, "grid too large for one solver (%lld cells, limit 2^30)",
+    // level arrays are addressed with 32-bit byte offsets, and an fp32 level
+    // stays below the out-of-range offset of the tiled loads
+    PFM_CHECK_ARG(cells <= ((int64_t)1 << 30) - 64, "grid too large for one solver (%lld cells, limit 2^30 - 64)",
                   (long long)cells);
     MGHandle* h = new (std::nothrow) MGHandle();
     if (!h) {

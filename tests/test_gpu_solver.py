@@ -20,7 +20,7 @@ def N(t):
 
 
 @pytest.mark.parametrize("N_", [(32, 32), (16, 16, 16), (32, 16, 16), (64, 64), (50, 34, 34), (100, 68),
-                                (40, 72, 24)])
+                                (40, 72, 24), (64, 40, 48), (64, 64, 64)])
 def test_mg_passes_bit_exact(oracle, pfm, N_):
     d = len(N_)
     mg_o = oracle.MG(N_, base_level=3)
@@ -161,7 +161,7 @@ def test_solver_api_pieces(oracle, pfm):
 
 
 # ---- solid obstacle (extension) ---------------------------------------------
-@pytest.mark.parametrize("N_", [(16, 16, 16), (32, 32, 32), (64, 64), (50, 34, 34), (40, 72, 24)])
+@pytest.mark.parametrize("N_", [(16, 16, 16), (32, 32, 32), (64, 64), (50, 34, 34), (40, 72, 24), (64, 40, 48)])
 def test_mg_solid_passes_bit_exact(oracle, pfm, N_):
     from helpers import box_solid
 
