@@ -294,7 +294,22 @@ struct TileMap {
 // tiles are numbered with axis 0 fastest: the tiles that share halo cells
 // with a tile ran just before it on the same XCD, so the halo is still in
 // that XCD's L2.
+//
+// When the axis-0 tile count is a multiple of 8, XCD x instead takes the slab
+// of axis-0 tiles [x q, x q + q) (q = n0 / 8) and walks it with axis 0
+// fastest, then axis 2, then axis 1, so a tile's six neighbours are at most
+// q n2 dispatches apart on the same XCD and their shared halo lines are still
+// in its L2 (with axis 1 at n0 n2 dispatches apart, its halos were fetched
+// twice from memory); only the slab faces are fetched by two XCDs.
 __device__ __forceinline__ void tile_of_block(const TileMap& tm, int t[3]) {
+    if ((tm.n0 & 7) == 0) {
+        const int q = tm.n0 >> 3, x = blockIdx.x & 7, k = blockIdx.x >> 3;
+        const int rest = k / q;
+        t[0] = x * q + (k - rest * q);
+        t[2] = rest % tm.n2;
+        t[1] = rest / tm.n2;
+        return;
+    }
     int b = blockIdx.x;
     if ((tm.total & 7) == 0) b = (b & 7) * (tm.total >> 3) + (b >> 3);
     t[0] = b % tm.n0;
@@ -803,14 +818,27 @@ constexpr int kSw3Threads = 256;
 // the tile's axis-2 groups, then the edge groups along axis 2.  (ri, rj, lg)
 // are the item's offsets from the tile origin minus one cell on axes 0 / 1
 // and minus one group on axis 2.
+//
+// Tile item e (block of 256 threads: round e >> 8, wave (e >> 6) & 3, lane
+// e & 63) is group 1 + (lane & 7) of row 1 + 2 (lane >> 3) + (wave & 1) of
+// plane 1 + (wave >> 1) + 2 round: the rows of one wave have one parity, so
+// every tile group of a wave has the same (axis 0 + axis 1) parity and picks
+// the same two cells of a colour.
+//
+// Tile item e (block of 256 threads: round e >> 8, wave (e >> 6) & 3, lane
+// e & 63) is group 1 + (lane & 7) of row 1 + 2 (lane >> 3) + (wave & 1) of
+// plane 1 + (wave >> 1) + 2 round: the rows of one wave have one parity, so
+// every tile group of a wave has the same (axis 0 + axis 1) parity and picks
+// the same two cells of a colour.
 template <int TI, int TJ, int TG>
 __device__ __forceinline__ void sweep3_item(int e, int& ri, int& rj, int& lg) {
+    static_assert(TI == 8 && TJ == 16 && TG == 8, "tile item layout assumes an 8 x 16 x 32 tile and 256 threads");
     constexpr int NTILE = TI * TJ * TG, RW = 2 * (TJ + 2) + 2 * TI, NRING = RW * TG;
     if (e < NTILE) {
-        lg = 1 + e % TG;
-        const int q = e / TG;
-        rj = 1 + q % TJ;
-        ri = 1 + q / TJ;
+        const int w = (e >> 6) & 3, l = e & 63;
+        lg = 1 + (l & 7);
+        rj = 1 + 2 * (l >> 3) + (w & 1);
+        ri = 1 + (w >> 1) + 2 * (e >> 8);
     } else if (e < NTILE + NRING) {
         const int u = e - NTILE;
         lg = 1 + u % TG;
@@ -965,10 +993,15 @@ __global__ void __launch_bounds__(kSw3Threads, 4) k_sweep3(LvlDev L, LvlDev Lc, 
                                                   o2 + T::K + 1, true);
     const bool interior1 = !solid && box_interior(L, o0, o1, o2, o0 + T::I, o1 + T::J, o2 + T::K, true);
     const float n_int = (float)L.nb2, y_int = pfm_rcp_small(L.nb2);
-    // `zero`: every z is +0 (the first pass of a zero input), nothing is read from LDS
-    auto pass = [&](int base, int g0, int g1, int gk, int c, bool in, bool intr, float4 rg, uint32_t dw,
+    // `zero`: every z is +0 (the first pass of a zero input), nothing is read from LDS.
+    // S: the offset of the group's first cell of colour c, either fixed at
+    // compile time (an std::integral_constant, for waves whose groups share it)
+    // or an int taken from the group.
+    auto pass = [&](auto S, int base, int g0, int g1, int gk, int c, bool in, bool intr, float4 rg, uint32_t dw,
                     bool zero) -> GroupPass {
-        const bool s1 = ((c + g0 + g1) & 1) != 0;
+        bool s1;
+        if constexpr (std::is_same<decltype(S), int>::value) s1 = ((c + g0 + g1) & 1) != 0;
+        else s1 = decltype(S)::value != 0;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 own = zero ? z4 : *reinterpret_cast<const float4*>(zs + base);
         const float4 ip = zero ? z4 : *reinterpret_cast<const float4*>(zs + base + SP);
@@ -1012,7 +1045,11 @@ __global__ void __launch_bounds__(kSw3Threads, 4) k_sweep3(LvlDev L, LvlDev Lc, 
             yb = pfm_rcp_small(nb_);
             fb = nb_ != 0;
         } else if (!intr) {
-            const int wa = L.nb2 - walls(L, g0, g1, ka), wb = L.nb2 - walls(L, g0, g1, kb);
+            // walls (num_fluid_neighbors): axes 0 / 1 once per group, axis 2 per cell
+            const int wij = (g0 <= 0 ? L.bc_lo[0] : 0) + (g0 >= d0 - 1 ? L.bc_hi[0] : 0) +
+                            (g1 <= 0 ? L.bc_lo[1] : 0) + (g1 >= d1 - 1 ? L.bc_hi[1] : 0);
+            const int wa = L.nb2 - wij - (ka <= 0 ? L.bc_lo[2] : 0) - (ka >= d2 - 1 ? L.bc_hi[2] : 0);
+            const int wb = L.nb2 - wij - (kb <= 0 ? L.bc_lo[2] : 0) - (kb >= d2 - 1 ? L.bc_hi[2] : 0);
             nfa = (float)wa;
             ya = pfm_rcp_small(wa);
             nfb = (float)wb;
@@ -1045,7 +1082,19 @@ __global__ void __launch_bounds__(kSw3Threads, 4) k_sweep3(LvlDev L, LvlDev Lc, 
             sweep3_item<T::I, T::J, TG>(e, ri, rj, lg);
             const int g0 = o0 - 1 + ri, g1 = o1 - 1 + rj, gk = bk + 4 * lg;
             const int base = (ri + 1) * SP + (rj + 1) * SK + 4 * lg;
-            const GroupPass o = pass(base, g0, g1, gk, c0, in3(g0, g1, gk), interior0, rv[m], dgv[m], ZERO_IN);
+            const bool in = in3(g0, g1, gk);
+            GroupPass o;
+            if (m < P1R) {
+                // tile items: the wave's groups share the colour offset
+                if (__builtin_amdgcn_readfirstlane((c0 + g0 + g1) & 1))
+                    o = pass(std::integral_constant<int, 1>{}, base, g0, g1, gk, c0, in, interior0, rv[m], dgv[m],
+                             ZERO_IN);
+                else
+                    o = pass(std::integral_constant<int, 0>{}, base, g0, g1, gk, c0, in, interior0, rv[m], dgv[m],
+                             ZERO_IN);
+            } else {
+                o = pass(0, base, g0, g1, gk, c0, in, interior0, rv[m], dgv[m], ZERO_IN);
+            }
             *reinterpret_cast<float4*>(zs + base) = o.v;
             // cells of colour c0 inside the tile are final after this pass
             if (DOT && m < P1R) dot(o);
@@ -1061,7 +1110,11 @@ __global__ void __launch_bounds__(kSw3Threads, 4) k_sweep3(LvlDev L, LvlDev Lc, 
         const int g0 = o0 - 1 + ri, g1 = o1 - 1 + rj, gk = bk + 4 * lg;
         const int base = (ri + 1) * SP + (rj + 1) * SK + 4 * lg;
         const bool in = in3(g0, g1, gk);
-        const GroupPass o = pass(base, g0, g1, gk, c1, in, interior1, rv[m], dgv[m], false);
+        GroupPass o;
+        if (__builtin_amdgcn_readfirstlane((c1 + g0 + g1) & 1))
+            o = pass(std::integral_constant<int, 1>{}, base, g0, g1, gk, c1, in, interior1, rv[m], dgv[m], false);
+        else
+            o = pass(std::integral_constant<int, 0>{}, base, g0, g1, gk, c1, in, interior1, rv[m], dgv[m], false);
         if (DOT) dot(o);
         if (in) st4<ZT>(zout, cell(g0, g1, gk), o.v);
     }
@@ -1234,6 +1287,138 @@ __global__ void __launch_bounds__(kBlock) k_ap_dot_p(LvlDev L, TileMap tm, const
             p_new[q] = pq;
             Ap[q] = a;
             s += (double)pq * (double)a;
+        }
+    }
+    s = block_sum(s, sh);
+    if (tid == 0) partial[blockIdx.x] = s;
+}
+
+// k_ap_dot_p for 3D levels whose axis-2 extent is a multiple of 4, on groups
+// of 4 cells (see k_sweep3): p_new = z + beta p_old on the tile plus a
+// one-cell ring along axes 0 / 1 and one group each side along axis 2 goes to
+// LDS (the items and their order are k_sweep3's first pass, the tile first),
+// then each tile group forms Ap from its own p_new (kept in registers) and
+// its neighbours in LDS.  Per cell the arithmetic is k_ap_dot_p's.
+template <typename ZT>
+__global__ void __launch_bounds__(kSw3Threads, 4) k_ap_dot_p3(LvlDev L, TileMap tm, const ZT* __restrict__ z,
+                                                          const float* __restrict__ p_old,
+                                                          float* __restrict__ p_new, float* __restrict__ Ap,
+                                                          const MGScalars* __restrict__ sc,
+                                                          double* __restrict__ partial) {
+    using T = MgTile<3>;
+    constexpr int NT = kSw3Threads;
+    constexpr int TG = T::K / 4, GK = TG + 2;
+    constexpr int SI = T::I + 2, SJ = T::J + 2, SK = 4 * GK, SP = SJ * SK;
+    constexpr int NI = SI * SJ * GK, NR = (NI + NT - 1) / NT;  // p_new items (the tile first), rounds
+    constexpr int P1G = T::I * T::J * TG, P1R = P1G / NT;     // the tile's groups
+    static_assert(P1G % NT == 0 && P1R <= NR, "the tile fills whole rounds");
+    __shared__ __align__(16) float ps[SI * SP];
+    __shared__ double sh[NT / 64];
+    // past convergence the search direction stays as it is (a second idle
+    // iteration would otherwise write its partner buffer, the current p) and
+    // nothing reads Ap or the partials
+    if (pcg_stopped(sc)) return;
+    int t[3];
+    tile_of_block(tm, t);
+    const int o0 = t[0] * T::I, o1 = t[1] * T::J, o2 = t[2] * T::K, bk = o2 - 4;
+    const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+    const int tid = threadIdx.x;
+    const bool solid = L.dg != nullptr;
+    const __amdgpu_buffer_rsrc_t rz = lvl_rsrc(z, L), rp = lvl_rsrc(p_old, L);
+    auto in3 = [&](int g0, int g1, int gk) {
+        return (unsigned)g0 < (unsigned)d0 && (unsigned)g1 < (unsigned)d1 && (unsigned)gk < (unsigned)d2;
+    };
+    auto cell = [&](int g0, int g1, int gk) { return (uint32_t)((g0 * d1 + g1) * d2 + gk); };
+    const float beta = sc->beta;
+    const float zi = sizeof(ZT) == 2 ? sc->vinv : 1.0f;  // 16-bit z is stored scaled
+    float4 pn[P1R];
+    auto item = [&](int m) {
+        const int e = tid + m * NT;
+        int ri, rj, lg;
+        sweep3_item<T::I, T::J, TG>(e, ri, rj, lg);
+        const int g0 = o0 - 1 + ri, g1 = o1 - 1 + rj, gk = bk + 4 * lg;
+        const bool in = e < NI && in3(g0, g1, gk);
+        const uint32_t q = cell(g0, g1, gk);
+        float4 zv = ld4<ZT>(rz, in ? q * (uint32_t)sizeof(ZT) : kOOB);
+        const float4 pv = ld4<float>(rp, in ? q * 4u : kOOB);
+        if (sizeof(ZT) == 2) {
+            zv.x = zv.x * zi;
+            zv.y = zv.y * zi;
+            zv.z = zv.z * zi;
+            zv.w = zv.w * zi;
+        }
+        float4 v;
+        v.x = in ? __fadd_rn(zv.x, __fmul_rn(beta, pv.x)) : 0.f;
+        v.y = in ? __fadd_rn(zv.y, __fmul_rn(beta, pv.y)) : 0.f;
+        v.z = in ? __fadd_rn(zv.z, __fmul_rn(beta, pv.z)) : 0.f;
+        v.w = in ? __fadd_rn(zv.w, __fmul_rn(beta, pv.w)) : 0.f;
+        if (e < NI) *reinterpret_cast<float4*>(ps + ri * SP + rj * SK + 4 * lg) = v;
+        return v;
+    };
+#pragma unroll
+    for (int m = 0; m < P1R; ++m) pn[m] = item(m);
+#pragma unroll
+    for (int m = P1R; m < NR; ++m) (void)item(m);
+    uint32_t dgv[P1R];
+#pragma unroll
+    for (int m = 0; m < P1R; ++m) dgv[m] = 0u;
+    if (solid) {
+        const __amdgpu_buffer_rsrc_t rd = dg_rsrc(L);
+#pragma unroll
+        for (int m = 0; m < P1R; ++m) {
+            int ri, rj, lg;
+            sweep3_item<T::I, T::J, TG>(tid + m * NT, ri, rj, lg);
+            const int g0 = o0 - 1 + ri, g1 = o1 - 1 + rj, gk = bk + 4 * lg;
+            dgv[m] = __builtin_amdgcn_raw_buffer_load_b32(rd, in3(g0, g1, gk) ? cell(g0, g1, gk) : kOOB, 0, 0);
+        }
+    }
+    const bool interior = !solid && box_interior(L, o0, o1, o2, o0 + T::I, o1 + T::J, o2 + T::K, true);
+    const float n_int = (float)L.nb2;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < P1R; ++m) {
+        int ri, rj, lg;
+        sweep3_item<T::I, T::J, TG>(tid + m * NT, ri, rj, lg);
+        const int g0 = o0 - 1 + ri, g1 = o1 - 1 + rj, gk = bk + 4 * lg;
+        const int base = ri * SP + rj * SK + 4 * lg;
+        const float4 own = pn[m];
+        const float4 ip = *reinterpret_cast<const float4*>(ps + base + SP);
+        const float4 im = *reinterpret_cast<const float4*>(ps + base - SP);
+        const float4 jp = *reinterpret_cast<const float4*>(ps + base + SK);
+        const float4 jm = *reinterpret_cast<const float4*>(ps + base - SK);
+        const float lf = ps[base - 1], rt = ps[base + 4];
+        const float pc[4] = {own.x, own.y, own.z, own.w};
+        const float ipc[4] = {ip.x, ip.y, ip.z, ip.w}, imc[4] = {im.x, im.y, im.z, im.w};
+        const float jpc[4] = {jp.x, jp.y, jp.z, jp.w}, jmc[4] = {jm.x, jm.y, jm.z, jm.w};
+        float a[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            // neighbor_sum order (3D/mgpcg.py:92-104): +e then -e per axis
+            float ns = 0.0f;
+            ns = __fadd_rn(ns, ipc[c]);
+            ns = __fadd_rn(ns, imc[c]);
+            ns = __fadd_rn(ns, jpc[c]);
+            ns = __fadd_rn(ns, jmc[c]);
+            ns = __fadd_rn(ns, c < 3 ? pc[c + 1] : rt);
+            ns = __fadd_rn(ns, c > 0 ? pc[c - 1] : lf);
+            float nf = n_int;
+            bool fluid = true;
+            if (solid) {
+                const int n = (int)((dgv[m] >> (8 * c)) & 0xffu);
+                nf = (float)n;
+                fluid = n != 0;
+            } else if (!interior) {
+                nf = (float)(L.nb2 - walls(L, g0, g1, gk + c));
+            }
+            a[c] = fluid ? __fsub_rn(__fmul_rn(nf, pc[c]), ns) : 0.f;
+        }
+        if (in3(g0, g1, gk)) {
+            const uint32_t q = cell(g0, g1, gk);
+            *reinterpret_cast<float4*>(p_new + q) = own;
+            *reinterpret_cast<float4*>(Ap + q) = make_float4(a[0], a[1], a[2], a[3]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s += (double)pc[c] * (double)a[c];
         }
     }
     s = block_sum(s, sh);
@@ -2271,8 +2456,15 @@ static int ap_dot_p(MGHandle* h, hipStream_t s, int* parts) {
     const LvlDev L0 = lvl_dev(h, 0);
     const TileMap tm = sweep_tiles(h, 0);
     const dim3 g((unsigned)tm.total);
-#define PFM_AP(D, ZT) \
-    hipLaunchKernelGGL((k_ap_dot_p<D, ZT>), g, dim3(kBlock), 0, s, L0, tm, (const ZT*)h->z[0], h->p, h->p2, h->Ap, h->sc, h->partial)
+    // 3D levels with an axis-2 extent that is a multiple of 4 run the group kernel
+    const bool groups = h->dim == 3 && L0.d[2] % 4 == 0;
+#define PFM_AP(D, ZT)                                                                                           \
+    if (D == 3 && groups)                                                                                       \
+        hipLaunchKernelGGL((k_ap_dot_p3<ZT>), g, dim3(kSw3Threads), 0, s, L0, tm, (const ZT*)h->z[0], h->p, h->p2, \
+                           h->Ap, h->sc, h->partial);                                                           \
+    else                                                                                                        \
+        hipLaunchKernelGGL((k_ap_dot_p<D, ZT>), g, dim3(kBlock), 0, s, L0, tm, (const ZT*)h->z[0], h->p, h->p2,  \
+                           h->Ap, h->sc, h->partial)
     switch (h->precision) {
         case 1:
             if (h->dim == 3) PFM_AP(3, __half);
