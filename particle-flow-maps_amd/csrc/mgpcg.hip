@@ -1036,24 +1036,31 @@ __global__ void __launch_bounds__(kSw3Threads, 4) k_sweep3(LvlDev L, LvlDev Lc, 
         float nfa = n_int, ya = y_int, nfb = n_int, yb = y_int;
         bool fa = true, fb = true;
         if (solid) {
-            const int sa = s1 ? 8 : 0;
-            const int na_ = (int)((dw >> sa) & 0xffu), nb_ = (int)((dw >> (sa + 16)) & 0xffu);
-            nfa = (float)na_;
-            ya = pfm_rcp_small(na_);
-            fa = na_ != 0;
-            nfb = (float)nb_;
-            yb = pfm_rcp_small(nb_);
-            fb = nb_ != 0;
+            // a group of four fluid cells with the interior diagonal keeps n_int / y_int
+            if (dw != (uint32_t)L.nb2 * 0x01010101u) {
+                const int sa = s1 ? 8 : 0;
+                const int na_ = (int)((dw >> sa) & 0xffu), nb_ = (int)((dw >> (sa + 16)) & 0xffu);
+                nfa = (float)na_;
+                ya = pfm_rcp_small(na_);
+                fa = na_ != 0;
+                nfb = (float)nb_;
+                yb = pfm_rcp_small(nb_);
+                fb = nb_ != 0;
+            }
         } else if (!intr) {
-            // walls (num_fluid_neighbors): axes 0 / 1 once per group, axis 2 per cell
-            const int wij = (g0 <= 0 ? L.bc_lo[0] : 0) + (g0 >= d0 - 1 ? L.bc_hi[0] : 0) +
-                            (g1 <= 0 ? L.bc_lo[1] : 0) + (g1 >= d1 - 1 ? L.bc_hi[1] : 0);
-            const int wa = L.nb2 - wij - (ka <= 0 ? L.bc_lo[2] : 0) - (ka >= d2 - 1 ? L.bc_hi[2] : 0);
-            const int wb = L.nb2 - wij - (kb <= 0 ? L.bc_lo[2] : 0) - (kb >= d2 - 1 ? L.bc_hi[2] : 0);
-            nfa = (float)wa;
-            ya = pfm_rcp_small(wa);
-            nfb = (float)wb;
-            yb = pfm_rcp_small(wb);
+            // walls (num_fluid_neighbors): axes 0 / 1 once per group, axis 2 per cell;
+            // only the groups on a wall layer leave the interior diagonal
+            const bool edge = g0 <= 0 || g0 >= d0 - 1 || g1 <= 0 || g1 >= d1 - 1 || gk <= 0 || gk + 4 >= d2;
+            if (edge) {
+                const int wij = (g0 <= 0 ? L.bc_lo[0] : 0) + (g0 >= d0 - 1 ? L.bc_hi[0] : 0) +
+                                (g1 <= 0 ? L.bc_lo[1] : 0) + (g1 >= d1 - 1 ? L.bc_hi[1] : 0);
+                const int wa = L.nb2 - wij - (ka <= 0 ? L.bc_lo[2] : 0) - (ka >= d2 - 1 ? L.bc_hi[2] : 0);
+                const int wb = L.nb2 - wij - (kb <= 0 ? L.bc_lo[2] : 0) - (kb >= d2 - 1 ? L.bc_hi[2] : 0);
+                nfa = (float)wa;
+                ya = pfm_rcp_small(wa);
+                nfb = (float)wb;
+                yb = pfm_rcp_small(wb);
+            }
         }
         o.za = pfm_div_small(__fadd_rn(va, na), nfa, ya);
         o.zb = pfm_div_small(__fadd_rn(vb, nb), nfb, yb);
