@@ -275,6 +275,36 @@ def test_g2p(oracle, pfm, res):
     np.testing.assert_array_equal(N(ig2), N(ig))
 
 
+@pytest.mark.parametrize("res", [(48, 40, 36), (20, 17, 23)])
+def test_g2p_independent_of_particle_order(oracle, pfm, res):
+    """3D G2P in cell-major particle order (a block's particles close
+    together) and in a shuffled order (every block spread over the domain):
+    the same particle gets the same bits either way, and both match the
+    oracle."""
+    from pfm import core
+
+    pos, dx, n = _particles(oracle, res, seed=12, order=1)
+    u = smooth_mac_field(res, seed=13)
+    perm = np.random.default_rng(14).permutation(n)
+    outs = []
+    for order in (np.arange(n), perm):
+        ig = torch.empty((3, n), device=DEV)
+        Cg = [torch.empty((3, n), device=DEV) for _ in range(3)]
+        core.g2p(res, dx, G(np.ascontiguousarray(pos[:, order])), [G(x) for x in u], ig, Cg)
+        inv = np.empty(n, np.int64)
+        inv[order] = np.arange(n)
+        outs.append((N(ig)[:, inv], [N(c)[:, inv] for c in Cg]))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    for a in range(3):
+        np.testing.assert_array_equal(outs[0][1][a], outs[1][1][a])
+    imp = np.empty((3, n), np.float32)
+    C = np.empty((3, 3, n), np.float32)
+    oracle.g2p(res, dx, pos, u, imp, C)
+    assert rel_l2(outs[0][0], imp) < 1e-6
+    for a in range(3):
+        assert rel_l2(outs[0][1][a], C[a]) < 1e-5
+
+
 @pytest.mark.parametrize("d", [2, 3])
 def test_update_T_imp_gradm(oracle, pfm, d):
     from pfm import core
