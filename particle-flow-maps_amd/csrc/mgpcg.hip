@@ -1442,16 +1442,27 @@ __global__ void __launch_bounds__(kSw3Threads, 4) k_ap_dot_p3(LvlDev L, TileMap 
 // sums need no tests.
 constexpr int kCoarseMaxLevels = 4;
 constexpr int kCoarseLds = 14336;  // floats (56 KiB)
-constexpr int kCoarseSlots = 4;    // bottom-level cells per colour per thread
+constexpr int kCoarseSlots = 4;    // cells per colour per thread on every level
 constexpr int kCoarseThreads = 1024;
+constexpr int kCoarseNd = kCoarseLds / 2;  // diagonal bytes: the levels hold at most this many cells
 
 constexpr int kCoarseBottom64 = 4096;  // padded bottom cells of the FP64 bottom solve
+
+// floor(x / d) for 0 <= x, d < 2^16 as a multiply-high by m = ceil(2^32 / d):
+// x m / 2^32 = x / d + x (m - 2^32 / d) / 2^32, and the second term is below
+// x / 2^32 < 1 / d, too small to carry x / d past the next integer.  m = 0
+// stands for d = 1.
+__host__ __device__ __forceinline__ uint32_t magic16(int d) { return d > 1 ? 0xFFFFFFFFu / (uint32_t)d + 1u : 0u; }
+__device__ __forceinline__ int div16(int x, uint32_t m) { return m ? (int)__umulhi((uint32_t)x, m) : x; }
 
 struct CoarseArgs {
     int nl;         // levels handled; the last one is the bottom
     int sweeps;     // bottom red/black iterations
     float inv_scale;
     int zoff[kCoarseMaxLevels], roff[kCoarseMaxLevels];
+    int noff[kCoarseMaxLevels];  // offset of each level's diagonal bytes
+    // division magics of each level: by d[1], by d[2] and by (d[2] + 1) / 2
+    uint32_t m1[kCoarseMaxLevels], m2[kCoarseMaxLevels], mh[kCoarseMaxLevels];
     LvlDev L[kCoarseMaxLevels];
     void* z[kCoarseMaxLevels];  // ZT arrays
     void* r[kCoarseMaxLevels];  // R0T on the first level, ZT below
@@ -1474,38 +1485,64 @@ __host__ __device__ __forceinline__ int padded_size(const int d[3]) {
     return (DIM == 3 ? d[0] + 2 : 1) * (d[1] + 2) * (d[2] + 2);
 }
 
-// smooth(l, c) (3D/mgpcg.py:165-172) on one padded LDS level; the cells of
-// colour c are spread over all threads (a 2D level has a single axis-0 row
-// plane, so per-row work would leave most threads idle).
+// The cells of one level dealt to this thread, per colour: slot m holds
+// entry tid + m * kCoarseThreads of the colour's (row, k / 2) enumeration,
+// decoded once per level visit (its LDS offset, r and diagonal kept in
+// registers for the visit's passes; zq = -1 for no cell or a solid one).
+struct CoarseCells {
+    int zq[2][kCoarseSlots];
+    float r[2][kCoarseSlots], n[2][kCoarseSlots], y[2][kCoarseSlots];
+};
+
 template <int DIM>
-__device__ void coarse_pass(const LvlDev& L, float* zs, const float* rs, int c) {
+__device__ __forceinline__ void coarse_cells(const LvlDev& L, uint32_t m1, uint32_t mh, const float* rs,
+                                             const uint8_t* nd, CoarseCells& C) {
     const Padded<DIM> P(L);
-    const int h2 = (L.d[2] + 1) / 2;
-    const int ne = L.d[0] * L.d[1] * h2;
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-        const int row = e / h2, pk = e - row * h2;
-        const int i = row / L.d[1], j = row - i * L.d[1];
-        const int k = 2 * pk + ((c + i + j) & 1);
-        if (k >= L.d[2]) continue;
-        const int q = row * L.d[2] + k;
-        const int n = L.dg ? (int)L.dg[q] : L.nb2 - walls(L, i, j, k);
-        if (n == 0) continue;  // solid cell
-        const int zq = P.at(i, j, k);
-        zs[zq] = pfm_div_small(__fadd_rn(rs[q], nsum_lds<DIM>(zs + zq, P.s0, P.s1)), (float)n, pfm_rcp_small(n));
-    }
+    const int h2 = (L.d[2] + 1) / 2, nslot = L.d[0] * L.d[1] * h2;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int m = 0; m < kCoarseSlots; ++m) {
+            const int s = (int)threadIdx.x + m * kCoarseThreads;
+            C.zq[c][m] = -1;
+            C.r[c][m] = C.n[c][m] = C.y[c][m] = 0.f;
+            if (s < nslot) {
+                const int row = div16(s, mh), pk = s - row * h2;
+                const int i = div16(row, m1), j = row - i * L.d[1];
+                const int k = 2 * pk + ((c + i + j) & 1);
+                const int q = row * L.d[2] + k;
+                const int n = k < L.d[2] ? (int)nd[q] : 0;
+                if (n != 0) {
+                    C.zq[c][m] = P.at(i, j, k);
+                    C.r[c][m] = rs[q];
+                    C.n[c][m] = (float)n;
+                    C.y[c][m] = pfm_rcp_small(n);
+                }
+            }
+        }
+}
+
+// smooth(l, c) (3D/mgpcg.py:165-172) over the decoded cells of colour c.
+template <int DIM, int c>
+__device__ __forceinline__ void coarse_pass(float* zs, const Padded<DIM>& P, const CoarseCells& C) {
+#pragma unroll
+    for (int m = 0; m < kCoarseSlots; ++m)
+        if (C.zq[c][m] >= 0)
+            zs[C.zq[c][m]] = pfm_div_small(__fadd_rn(C.r[c][m], nsum_lds<DIM>(zs + C.zq[c][m], P.s0, P.s1)),
+                                           C.n[c][m], C.y[c][m]);
 }
 
 // Residual and restriction (3D/mgpcg.py:152-158) from LDS level f to LDS
 // level c, one coarse cell per thread iteration, children summed in fixed
 // order, odd extents as in k_restrict.
 template <int DIM>
-__device__ void coarse_restrict(const LvlDev& Lf, const LvlDev& Lc, const float* zs, const float* rs, float* rcs,
-                                float inv_scale) {
+__device__ void coarse_restrict(const LvlDev& Lf, const LvlDev& Lc, uint32_t mc1, uint32_t mc2, const float* zs,
+                                const float* rs, const uint8_t* ndf, float* rcs, float inv_scale) {
     const Padded<DIM> P(Lf);
     const int nc = Lc.d[0] * Lc.d[1] * Lc.d[2];
     for (int e = threadIdx.x; e < nc; e += blockDim.x) {
-        const int row = e / Lc.d[2], C2 = e - row * Lc.d[2];
-        const int C0 = row / Lc.d[1], C1 = row - C0 * Lc.d[1];
+        const int row = div16(e, mc2), C2 = e - row * Lc.d[2];
+        const int C0 = div16(row, mc1), C1 = row - C0 * Lc.d[1];
         const int h0 = C0 == Lc.d[0] - 1 ? Lf.d[0] - 1 - 2 * C0 : 1;
         const int h1 = C1 == Lc.d[1] - 1 ? Lf.d[1] - 1 - 2 * C1 : 1;
         const int h2 = C2 == Lc.d[2] - 1 ? Lf.d[2] - 1 - 2 * C2 : 1;
@@ -1516,7 +1553,7 @@ __device__ void coarse_restrict(const LvlDev& Lf, const LvlDev& Lc, const float*
                     const int i = 2 * C0 + a, j = 2 * C1 + b, k = 2 * C2 + c;
                     const int zq = P.at(i, j, k);
                     const int fq = (i * Lf.d[1] + j) * Lf.d[2] + k;
-                    const int n = Lf.dg ? (int)Lf.dg[fq] : Lf.nb2 - walls(Lf, i, j, k);
+                    const int n = ndf[fq];
                     if (n == 0) continue;  // solid children add nothing
                     const float nf = (float)n;
                     const float res =
@@ -1530,13 +1567,14 @@ __device__ void coarse_restrict(const LvlDev& Lf, const LvlDev& Lc, const float*
 // prolongate (3D/mgpcg.py:160-163) between padded LDS levels, one fine cell
 // per thread iteration.
 template <int DIM>
-__device__ void coarse_prolong(const LvlDev& Lf, const LvlDev& Lc, float* zs, const float* zcs) {
+__device__ void coarse_prolong(const LvlDev& Lf, const LvlDev& Lc, uint32_t mf1, uint32_t mf2, float* zs,
+                               const float* zcs, const uint8_t* ndf) {
     const Padded<DIM> P(Lf), Pc(Lc);
     const int nf = Lf.d[0] * Lf.d[1] * Lf.d[2];
     for (int e = threadIdx.x; e < nf; e += blockDim.x) {
-        if (Lf.dg && Lf.dg[e] == 0) continue;  // solid cells carry no pressure
-        const int row = e / Lf.d[2], k = e - row * Lf.d[2];
-        const int i = row / Lf.d[1], j = row - i * Lf.d[1];
+        if (Lf.dg && ndf[e] == 0) continue;  // solid cells carry no pressure
+        const int row = div16(e, mf2), k = e - row * Lf.d[2];
+        const int i = div16(row, mf1), j = row - i * Lf.d[1];
         const int c0 = min(i >> 1, Lc.d[0] - 1), c1 = min(j >> 1, Lc.d[1] - 1), c2 = min(k >> 1, Lc.d[2] - 1);
         const int q = P.at(i, j, k);
         zs[q] = __fadd_rn(zs[q], zcs[Pc.at(c0, c1, c2)]);
@@ -1548,11 +1586,27 @@ __device__ void coarse_prolong(const LvlDev& Lf, const LvlDev& Lc, float* zs, co
 // for the mixed-precision solver).
 template <int DIM, typename ZT, typename R0T, typename BT>
 __global__ void __launch_bounds__(kCoarseThreads) k_coarse(CoarseArgs a, int sub_mean, int dot,
-                                                   const MGScalars* __restrict__ sc, double* __restrict__ partial) {
+                                                           const MGScalars* __restrict__ sc, double* __restrict__ partial) {
     __shared__ float sm[kCoarseLds];
+    __shared__ uint8_t nd[kCoarseNd];
     const int tid = threadIdx.x;
-    for (int l = 0; l < a.nl; ++l)
-        for (int e = tid; e < padded_size<DIM>(a.L[l].d); e += kCoarseThreads) sm[a.zoff[l] + e] = 0.f;
+    for (int l = 0; l < a.nl; ++l) {
+        const LvlDev& L = a.L[l];
+        for (int e = tid; e < padded_size<DIM>(L.d); e += kCoarseThreads) sm[a.zoff[l] + e] = 0.f;
+        // the diagonal of every cell (num_fluid_neighbors, or the obstacle's)
+        const int n = L.d[0] * L.d[1] * L.d[2];
+        for (int q = tid; q < n; q += kCoarseThreads) {
+            int v;
+            if (L.dg) {
+                v = L.dg[q];
+            } else {
+                const int row = div16(q, a.m2[l]), k = q - row * L.d[2];
+                const int i = div16(row, a.m1[l]), j = row - i * L.d[1];
+                v = L.nb2 - walls(L, i, j, k);
+            }
+            nd[a.noff[l] + q] = (uint8_t)v;
+        }
+    }
     {
         const LvlDev& L0 = a.L[0];
         const int n0 = L0.d[0] * L0.d[1] * L0.d[2];
@@ -1570,56 +1624,35 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse(CoarseArgs a, int sub
         }
     }
     __syncthreads();
+    CoarseCells C;
     for (int l = 0; l < a.nl - 1; ++l) {
-        for (int it = 0; it < 2; ++it)
-            for (int c = 0; c < 2; ++c) {
-                coarse_pass<DIM>(a.L[l], sm + a.zoff[l], sm + a.roff[l], c);
-                __syncthreads();
-            }
-        coarse_restrict<DIM>(a.L[l], a.L[l + 1], sm + a.zoff[l], sm + a.roff[l], sm + a.roff[l + 1], a.inv_scale);
+        const Padded<DIM> P(a.L[l]);
+        float* zs = sm + a.zoff[l];
+        coarse_cells<DIM>(a.L[l], a.m1[l], a.mh[l], sm + a.roff[l], nd + a.noff[l], C);
+        for (int it = 0; it < 2; ++it) {
+            coarse_pass<DIM, 0>(zs, P, C);
+            __syncthreads();
+            coarse_pass<DIM, 1>(zs, P, C);
+            __syncthreads();
+        }
+        coarse_restrict<DIM>(a.L[l], a.L[l + 1], a.m1[l + 1], a.m2[l + 1], zs, sm + a.roff[l], nd + a.noff[l],
+                             sm + a.roff[l + 1], a.inv_scale);
         __syncthreads();
     }
-    // bottom: each thread keeps its cells of each colour decoded in registers
+    // bottom: 50 red/black iterations from z = 0
     {
         const int b = a.nl - 1;
         const LvlDev& L = a.L[b];
         const Padded<DIM> P(L);
         float* zs = sm + a.zoff[b];
-        const float* rs = sm + a.roff[b];
-        const int h2 = (L.d[2] + 1) / 2, nslot = L.d[0] * L.d[1] * h2;
-        int bl[2][kCoarseSlots];
-        float br[2][kCoarseSlots], bn[2][kCoarseSlots], by[2][kCoarseSlots];
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-            for (int m = 0; m < kCoarseSlots; ++m) {
-                const int s = tid + m * kCoarseThreads;
-                bl[c][m] = -1;
-                br[c][m] = bn[c][m] = by[c][m] = 0.f;
-                if (s < nslot) {
-                    const int row = s / h2, pk = s - row * h2;
-                    const int i = row / L.d[1], j = row - i * L.d[1];
-                    const int k = 2 * pk + ((c + i + j) & 1);
-                    const int n = k >= L.d[2] ? 0 : (L.dg ? (int)L.dg[row * L.d[2] + k] : L.nb2 - walls(L, i, j, k));
-                    if (n != 0) {
-                        bl[c][m] = P.at(i, j, k);
-                        br[c][m] = rs[row * L.d[2] + k];
-                        bn[c][m] = (float)n;
-                        by[c][m] = pfm_rcp_small(n);
-                    }
-                }
-            }
+        coarse_cells<DIM>(L, a.m1[b], a.mh[b], sm + a.roff[b], nd + a.noff[b], C);
         if constexpr (sizeof(BT) == 4) {
-            for (int it = 0; it < a.sweeps; ++it)
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-#pragma unroll
-                    for (int m = 0; m < kCoarseSlots; ++m)
-                        if (bl[c][m] >= 0)
-                            zs[bl[c][m]] = pfm_div_small(
-                                __fadd_rn(br[c][m], nsum_lds<DIM>(zs + bl[c][m], P.s0, P.s1)), bn[c][m], by[c][m]);
-                    __syncthreads();
-                }
+            for (int it = 0; it < a.sweeps; ++it) {
+                coarse_pass<DIM, 0>(zs, P, C);
+                __syncthreads();
+                coarse_pass<DIM, 1>(zs, P, C);
+                __syncthreads();
+            }
         } else {
             // the coarsest-level solve in double, rounded to fp32 once
             __shared__ double bz[kCoarseBottom64];
@@ -1631,12 +1664,12 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse(CoarseArgs a, int sub
                 for (int c = 0; c < 2; ++c) {
 #pragma unroll
                     for (int m = 0; m < kCoarseSlots; ++m)
-                        if (bl[c][m] >= 0) {
-                            const double* x = bz + bl[c][m];
+                        if (C.zq[c][m] >= 0) {
+                            const double* x = bz + C.zq[c][m];
                             double ns = 0.0;
                             if (DIM == 3) ns = (ns + x[P.s0]) + x[-P.s0];
                             ns = (((ns + x[P.s1]) + x[-P.s1]) + x[1]) + x[-1];
-                            bz[bl[c][m]] = ((double)br[c][m] + ns) / (double)bn[c][m];
+                            bz[C.zq[c][m]] = ((double)C.r[c][m] + ns) / (double)C.n[c][m];
                         }
                     __syncthreads();
                 }
@@ -1645,13 +1678,17 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse(CoarseArgs a, int sub
         }
     }
     for (int l = a.nl - 2; l >= 0; --l) {
-        coarse_prolong<DIM>(a.L[l], a.L[l + 1], sm + a.zoff[l], sm + a.zoff[l + 1]);
+        const Padded<DIM> P(a.L[l]);
+        float* zs = sm + a.zoff[l];
+        coarse_prolong<DIM>(a.L[l], a.L[l + 1], a.m1[l], a.m2[l], zs, sm + a.zoff[l + 1], nd + a.noff[l]);
+        coarse_cells<DIM>(a.L[l], a.m1[l], a.mh[l], sm + a.roff[l], nd + a.noff[l], C);
         __syncthreads();
-        for (int it = 0; it < 2; ++it)
-            for (int c = 1; c >= 0; --c) {
-                coarse_pass<DIM>(a.L[l], sm + a.zoff[l], sm + a.roff[l], c);
-                __syncthreads();
-            }
+        for (int it = 0; it < 2; ++it) {
+            coarse_pass<DIM, 1>(zs, P, C);
+            __syncthreads();
+            coarse_pass<DIM, 0>(zs, P, C);
+            __syncthreads();
+        }
     }
     // write every level back, as the level-by-level V-cycle leaves them
     double s = 0.0;
@@ -1660,8 +1697,8 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse(CoarseArgs a, int sub
         const Padded<DIM> P(L);
         const int n = L.d[0] * L.d[1] * L.d[2];
         for (int q = tid; q < n; q += kCoarseThreads) {
-            const int row = q / L.d[2], k = q - row * L.d[2];
-            const int i = row / L.d[1], j = row - i * L.d[1];
+            const int row = div16(q, a.m2[l]), k = q - row * L.d[2];
+            const int i = div16(row, a.m1[l]), j = row - i * L.d[1];
             const ZT zq = from_f<ZT>(sm[a.zoff[l] + P.at(i, j, k)]);
             ((ZT*)a.z[l])[q] = zq;
             if (l > 0) ((ZT*)a.r[l])[q] = from_f<ZT>(sm[a.roff[l] + q]);
@@ -2327,17 +2364,18 @@ static int bottom(MGHandle* h, int b, hipStream_t s) {
 }
 
 // Level from which k_coarse can run the rest of the V-cycle (all those
-// levels fit its LDS and the bottom fits its register slots), or -1.
+// levels fit its LDS and the register slots of its cell decode), or -1.
 static int pick_coarse_level(const MGHandle* h) {
     const int L = h->n_levels;
     const int* db = h->lv[L - 1].d;
-    if ((int64_t)db[0] * db[1] * ((db[2] + 1) / 2) > (int64_t)kCoarseSlots * kCoarseThreads) return -1;
     const int pb = h->dim == 3 ? padded_size<3>(db) : padded_size<2>(db);
     if (h->precision != 0 && pb > kCoarseBottom64) return -1;
     int best = -1;
     int64_t used = 0;
     for (int l = L - 1; l >= 0 && L - l <= kCoarseMaxLevels; --l) {
-        used += (h->dim == 3 ? padded_size<3>(h->lv[l].d) : padded_size<2>(h->lv[l].d)) + h->lv[l].n;
+        const int* d = h->lv[l].d;
+        if ((int64_t)d[0] * d[1] * ((d[2] + 1) / 2) > (int64_t)kCoarseSlots * kCoarseThreads) break;
+        used += (h->dim == 3 ? padded_size<3>(d) : padded_size<2>(d)) + h->lv[l].n;
         if (used > kCoarseLds) break;
         best = l;
     }
@@ -2351,17 +2389,26 @@ static int coarse(MGHandle* h, bool sub_mean, bool dot, hipStream_t s) {
     a.nl = h->n_levels - lc;
     a.sweeps = 50;
     a.inv_scale = restrict_inv_scale(h);
-    int off = 0;
+    int off = 0, noff = 0;
     for (int i = 0; i < a.nl; ++i) {
         const int l = lc + i;
+        const int* d = h->lv[l].d;
         a.L[i] = lvl_dev(h, l);
         a.zoff[i] = off;
-        off += h->dim == 3 ? padded_size<3>(h->lv[l].d) : padded_size<2>(h->lv[l].d);
+        off += h->dim == 3 ? padded_size<3>(d) : padded_size<2>(d);
         a.roff[i] = off;
         off += (int)h->lv[l].n;
+        a.noff[i] = noff;
+        noff += (int)h->lv[l].n;
+        a.m1[i] = magic16(d[1]);
+        a.m2[i] = magic16(d[2]);
+        a.mh[i] = magic16((d[2] + 1) / 2);
         a.z[i] = h->z[l];
         a.r[i] = h->r[l];
     }
+    // every level's padded z holds at least its cells, so the LDS bound also
+    // bounds the diagonal bytes and keeps indices and extents below 2^16
+    if (off > kCoarseLds || noff > kCoarseNd) return PFM_ERR_ARG;
 #define PFM_CO(D, R0T, BT) \
     hipLaunchKernelGGL((k_coarse<D, ZT, R0T, BT>), dim3(1), dim3(kCoarseThreads), 0, s, a, (int)sub_mean, (int)dot, h->sc, h->partial)
     if constexpr (std::is_same<ZT, float>::value) {
