@@ -1225,6 +1225,304 @@ __global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc,
     }
 }
 
+// ---- fused visits of the coarser tiled levels --------------------------------
+// On levels whose launches are latency bound, each half of a V-cycle visit
+// runs as one launch: down = the zero-input sweep, the plain sweep and the
+// restriction (3D/mgpcg.py:184-191); up = the prolongation sweep and the plain
+// sweep (:196-200).  A block loads its tile with a halo of H cells; each
+// red/black pass updates the region one cell narrower than the last, so after
+// p passes every cell within H - p of the tile holds exactly the value of p
+// sequential passes over the whole level (an update reads only its face
+// neighbours, which were exact after the previous pass).  z is rounded to its
+// storage type after the second and the fourth pass, where the separate
+// launches store it, so the results are those of the separate launches bit
+// for bit.  A cell's diagonal byte is kNoUpd outside the level and on solid
+// cells, which are never updated and hold 0.
+template <int DIM> struct FsTile;
+template <> struct FsTile<3> { static constexpr int I = 8, J = 8, K = 16; };
+template <> struct FsTile<2> { static constexpr int I = 1, J = 32, K = 32; };
+constexpr int kFuseThreads = 256;
+constexpr uint8_t kNoUpd = 0xFF;
+
+template <int DIM, int H>
+struct FsGeom {
+    using T = FsTile<DIM>;
+    static constexpr int HI = DIM == 3 ? H : 0;
+    static constexpr int SI = T::I + 2 * HI, SJ = T::J + 2 * H, SK = T::K + 2 * H;
+    static constexpr int SP = SJ * SK, NS = SI * SP;
+};
+
+// Round every LDS value to ZT (a no-op for fp32 storage).
+template <typename ZT, int NS>
+__device__ __forceinline__ void fs_round(float* zs) {
+    if constexpr (sizeof(ZT) == 2) {
+        for (int e = threadIdx.x; e < NS; e += kFuseThreads) zs[e] = to_f(from_f<ZT>(zs[e]));
+    }
+}
+
+// One colour pass over the tile grown by RING cells.  Tile origin (o0, o1,
+// o2) sits at LDS (HI, H, H); cells come in pairs along axis 2, of which the
+// member of colour c (global (g0 + g1 + g2) & 1 == c) is updated.  A thread
+// takes SEG consecutive pairs of a row (the member's parity is the same along
+// a row): it gathers their neighbour sums first, so the LDS reads overlap, and
+// stores after (a pass reads only the other colour).  ny[n] = (n, RN(1/n)).
+template <int DIM, int H, int RING>
+__device__ __forceinline__ void fs_pass(float* zs, const float* rs, const uint8_t* nds, const float2* ny, int o0,
+                                        int o1, int o2, int c) {
+    using G = FsGeom<DIM, H>;
+    using T = FsTile<DIM>;
+    constexpr int RI = DIM == 3 ? RING : 0;
+    constexpr int NI = T::I + 2 * RI, NJ = T::J + 2 * RING, NK = T::K + 2 * RING, HP = NK / 2;
+    static_assert(NK % 2 == 0, "whole pairs along axis 2");
+    constexpr int SEG = 4, NSEG = (HP + SEG - 1) / SEG, NU = NI * NJ * NSEG;
+    for (int u = threadIdx.x; u < NU; u += kFuseThreads) {
+        const int row = u / NSEG, sg = u - row * NSEG;
+        const int ri = row / NJ, rj = row - ri * NJ;
+        const int li = G::HI - RI + ri, lj = H - RING + rj, lk0 = H - RING + 2 * SEG * sg;
+        const int g0 = o0 - G::HI + li, g1 = o1 - H + lj, gk0 = o2 - H + lk0;
+        const int base = (li * G::SJ + lj) * G::SK + lk0 + ((g0 + g1 + gk0 + c) & 1);
+        int nn[SEG];
+        float ns[SEG], rv[SEG];
+#pragma unroll
+        for (int m = 0; m < SEG; ++m) {
+            const bool ok = HP % SEG == 0 || SEG * sg + m < HP;
+            const int le = ok ? base + 2 * m : base;
+            nn[m] = ok ? (int)nds[le] : (int)kNoUpd;
+            ns[m] = nsum_lds<DIM>(zs + le, G::SP, G::SK);
+            rv[m] = rs[le];
+        }
+#pragma unroll
+        for (int m = 0; m < SEG; ++m) {
+            if (nn[m] == kNoUpd) continue;
+            const float2 t = ny[nn[m]];
+            zs[base + 2 * m] = pfm_div_small(__fadd_rn(rv[m], ns[m]), t.x, t.y);
+        }
+    }
+    __syncthreads();
+}
+
+// Diagonal byte of global cell g (dv: the obstacle's byte, when there is
+// one): kNoUpd outside the level or on a solid cell.
+__device__ __forceinline__ uint8_t fs_diag(const LvlDev& L, int dv, int g0, int g1, int g2) {
+    if (!inside(L, g0, g1, g2)) return kNoUpd;
+    if (L.dg) return dv == 0 ? kNoUpd : (uint8_t)dv;
+    return (uint8_t)(L.nb2 - walls(L, g0, g1, g2));
+}
+
+// The LDS box is loaded by columns: a thread owns positions (lj, lk) of the
+// box face and walks down axis 0 with incremental offsets, all loads issued
+// before any is used (kOOB reads 0 for cells outside the level).
+template <int DIM, int H>
+struct FsCols {
+    using G = FsGeom<DIM, H>;
+    static constexpr int ZC = G::SJ * G::SK, ZR = (ZC + kFuseThreads - 1) / kFuseThreads;
+};
+
+// The obstacle bytes of a thread's columns (0 without an obstacle).
+template <int DIM, int H>
+__device__ __forceinline__ void fs_load_dg(const LvlDev& L, int o0, int o1, int o2,
+                                           int (&dv)[FsCols<DIM, H>::ZR][FsGeom<DIM, H>::SI]) {
+    using G = FsGeom<DIM, H>;
+    using C = FsCols<DIM, H>;
+#pragma unroll
+    for (int m = 0; m < C::ZR; ++m)
+#pragma unroll
+        for (int li = 0; li < G::SI; ++li) dv[m][li] = 0;
+    if (!L.dg) return;
+    const __amdgpu_buffer_rsrc_t rd = dg_rsrc(L);
+    const int d1 = L.d[1], d2 = L.d[2], plane = d1 * d2;
+#pragma unroll
+    for (int m = 0; m < C::ZR; ++m) {
+        const int p = threadIdx.x + m * kFuseThreads;
+        const int lj = p / G::SK, lk = p - lj * G::SK;
+        const int g1 = o1 - H + lj, g2 = o2 - H + lk;
+        const bool col = p < C::ZC && (unsigned)g1 < (unsigned)d1 && (unsigned)g2 < (unsigned)d2;
+        const int base = g1 * d2 + g2;
+#pragma unroll
+        for (int li = 0; li < G::SI; ++li) {
+            const int g0 = o0 - G::HI + li;
+            const bool in = col && (unsigned)g0 < (unsigned)L.d[0];
+            dv[m][li] = (int)__builtin_amdgcn_raw_buffer_load_b8(rd, in ? (uint32_t)(g0 * plane + base) : kOOB, 0, 0);
+        }
+    }
+}
+
+// Down half of a visit of level l >= 1 (r and z stored as ZT): z = 0, colours
+// 0, 1, 0, 1; writes the tile's z to zout and r[l + 1] of the coarse cells
+// whose first child lies in the tile.  The halo of 5 keeps z exact one cell
+// around the tile after the four passes: the trailing child of an odd extent
+// lies one cell past the tile, and its outer neighbour is outside the level.
+template <int DIM, typename ZT>
+__global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc, TileMap tm, float inv_scale,
+                                                            const ZT* __restrict__ r, ZT* __restrict__ zout,
+                                                            ZT* __restrict__ rc) {
+    constexpr int H = 5;
+    using G = FsGeom<DIM, H>;
+    using T = FsTile<DIM>;
+    __shared__ float zs[G::NS], rs[G::NS];
+    __shared__ uint8_t nds[G::NS];
+    __shared__ float2 ny[8];
+    if (threadIdx.x < 8) ny[threadIdx.x] = make_float2((float)threadIdx.x, pfm_rcp_small(threadIdx.x));
+    int t[3];
+    tile_of_block(tm, t);
+    const int o0 = t[0] * T::I, o1 = t[1] * T::J, o2 = t[2] * T::K;
+    const __amdgpu_buffer_rsrc_t rr = lvl_rsrc(r, L);
+    using C = FsCols<DIM, H>;
+    const int d1 = L.d[1], d2 = L.d[2], plane = d1 * d2;
+    float rv[C::ZR][G::SI];
+    int dv[C::ZR][G::SI];
+#pragma unroll
+    for (int m = 0; m < C::ZR; ++m) {
+        const int p = threadIdx.x + m * kFuseThreads;
+        const int lj = p / G::SK, lk = p - lj * G::SK;
+        const int g1 = o1 - H + lj, g2 = o2 - H + lk;
+        const bool col = p < C::ZC && (unsigned)g1 < (unsigned)d1 && (unsigned)g2 < (unsigned)d2;
+        const int base = g1 * d2 + g2;
+#pragma unroll
+        for (int li = 0; li < G::SI; ++li) {
+            const int g0 = o0 - G::HI + li;
+            const bool in = col && (unsigned)g0 < (unsigned)L.d[0];
+            rv[m][li] = ldv<ZT>(rr, in ? (uint32_t)(g0 * plane + base) * (uint32_t)sizeof(ZT) : kOOB);
+        }
+    }
+    fs_load_dg<DIM, H>(L, o0, o1, o2, dv);
+#pragma unroll
+    for (int m = 0; m < C::ZR; ++m) {
+        const int p = threadIdx.x + m * kFuseThreads;
+        if (p >= C::ZC) continue;
+        const int lj = p / G::SK, lk = p - lj * G::SK;
+#pragma unroll
+        for (int li = 0; li < G::SI; ++li) {
+            const int e = li * G::SP + p;
+            rs[e] = rv[m][li];
+            zs[e] = 0.f;
+            nds[e] = fs_diag(L, dv[m][li], o0 - G::HI + li, o1 - H + lj, o2 - H + lk);
+        }
+    }
+    __syncthreads();
+    fs_pass<DIM, H, H - 1>(zs, rs, nds, ny, o0, o1, o2, 0);
+    fs_pass<DIM, H, H - 2>(zs, rs, nds, ny, o0, o1, o2, 1);
+    fs_round<ZT, G::NS>(zs);
+    __syncthreads();
+    fs_pass<DIM, H, H - 3>(zs, rs, nds, ny, o0, o1, o2, 0);
+    fs_pass<DIM, H, H - 4>(zs, rs, nds, ny, o0, o1, o2, 1);
+    fs_round<ZT, G::NS>(zs);
+    __syncthreads();
+    for (int e = threadIdx.x; e < T::I * T::J * T::K; e += kFuseThreads) {
+        const int ti = e / (T::J * T::K), rem = e - ti * (T::J * T::K), tj = rem / T::K, tk = rem - tj * T::K;
+        const int g0 = o0 + ti, g1 = o1 + tj, g2 = o2 + tk;
+        if (!inside(L, g0, g1, g2)) continue;
+        zout[((int64_t)g0 * L.d[1] + g1) * L.d[2] + g2] = from_f<ZT>(zs[((ti + G::HI) * G::SJ + tj + H) * G::SK + tk + H]);
+    }
+    // restriction (3D/mgpcg.py:152-158) of the coarse cells owned by the tile
+    constexpr int CI = DIM == 3 ? T::I / 2 : 1, CJ = T::J / 2, CK = T::K / 2;
+    const int c0 = DIM == 3 ? o0 / 2 : 0, c1 = o1 / 2, c2 = o2 / 2;
+    for (int e = threadIdx.x; e < CI * CJ * CK; e += kFuseThreads) {
+        const int ci = e / (CJ * CK), rem = e - ci * (CJ * CK), cj = rem / CK, ck = rem - cj * CK;
+        const int C0 = c0 + ci, C1 = c1 + cj, C2 = c2 + ck;
+        if (!inside(Lc, C0, C1, C2)) continue;
+        const int h0 = C0 == Lc.d[0] - 1 ? L.d[0] - 1 - 2 * C0 : 1;
+        const int h1 = C1 == Lc.d[1] - 1 ? L.d[1] - 1 - 2 * C1 : 1;
+        const int h2 = C2 == Lc.d[2] - 1 ? L.d[2] - 1 - 2 * C2 : 1;
+        float acc = 0.0f;
+        for (int a = 0; a <= h0; ++a)
+            for (int b = 0; b <= h1; ++b)
+                for (int c = 0; c <= h2; ++c) {
+                    const int li = 2 * C0 + a - (o0 - G::HI), lj = 2 * C1 + b - (o1 - H), lk = 2 * C2 + c - (o2 - H);
+                    const int le = (li * G::SJ + lj) * G::SK + lk;
+                    const int n = nds[le];
+                    if (n == kNoUpd) continue;  // solid children add nothing
+                    const float res = __fsub_rn(
+                        rs[le], __fsub_rn(__fmul_rn((float)n, zs[le]), nsum_lds<DIM>(zs + le, G::SP, G::SK)));
+                    acc = __fadd_rn(acc, __fmul_rn(__fmul_rn(res, 1.0f), inv_scale));
+                }
+        rc[((int64_t)C0 * Lc.d[1] + C1) * Lc.d[2] + C2] = from_f<ZT>(acc);
+    }
+}
+
+// Up half of a visit of level l >= 1: z = zin + zc[parent] (solid cells keep
+// 0), colours 1, 0, 1, 0; writes the tile's z to zout.  Tiles read their halo
+// from zin, which no block of the launch writes.
+template <int DIM, typename ZT>
+__global__ void __launch_bounds__(kFuseThreads) k_fuse_up(LvlDev L, LvlDev Lc, TileMap tm,
+                                                          const ZT* __restrict__ zin, const ZT* __restrict__ zc,
+                                                          const ZT* __restrict__ r, ZT* __restrict__ zout) {
+    constexpr int H = 4;
+    using G = FsGeom<DIM, H>;
+    using T = FsTile<DIM>;
+    __shared__ float zs[G::NS], rs[G::NS];
+    __shared__ uint8_t nds[G::NS];
+    __shared__ float2 ny[8];
+    if (threadIdx.x < 8) ny[threadIdx.x] = make_float2((float)threadIdx.x, pfm_rcp_small(threadIdx.x));
+    int t[3];
+    tile_of_block(tm, t);
+    const int o0 = t[0] * T::I, o1 = t[1] * T::J, o2 = t[2] * T::K;
+    const __amdgpu_buffer_rsrc_t rr = lvl_rsrc(r, L), rz = lvl_rsrc(zin, L), rc = lvl_rsrc(zc, Lc);
+    using C = FsCols<DIM, H>;
+    static_assert(G::HI % 2 == 0 && FsTile<DIM>::I % 2 == 0 || DIM == 2, "even axis-0 box origin");
+    constexpr int SH = (G::SI + 1) / 2;  // axis-0 pairs sharing a coarse parent
+    const int d1 = L.d[1], d2 = L.d[2], plane = d1 * d2;
+    const int e1 = Lc.d[1], e2 = Lc.d[2], pplane = e1 * e2;
+    float rv[C::ZR][G::SI], zv[C::ZR][G::SI], zp[C::ZR][SH];
+    int dv[C::ZR][G::SI];
+#pragma unroll
+    for (int m = 0; m < C::ZR; ++m) {
+        const int p = threadIdx.x + m * kFuseThreads;
+        const int lj = p / G::SK, lk = p - lj * G::SK;
+        const int g1 = o1 - H + lj, g2 = o2 - H + lk;
+        const bool col = p < C::ZC && (unsigned)g1 < (unsigned)d1 && (unsigned)g2 < (unsigned)d2;
+        const int base = g1 * d2 + g2;
+        const int pcol = min(g1 >> 1, e1 - 1) * e2 + min(g2 >> 1, e2 - 1);
+#pragma unroll
+        for (int li = 0; li < G::SI; ++li) {
+            const int g0 = o0 - G::HI + li;
+            const bool in = col && (unsigned)g0 < (unsigned)L.d[0];
+            const uint32_t off = in ? (uint32_t)(g0 * plane + base) * (uint32_t)sizeof(ZT) : kOOB;
+            rv[m][li] = ldv<ZT>(rr, off);
+            zv[m][li] = ldv<ZT>(rz, off);
+        }
+#pragma unroll
+        for (int h = 0; h < SH; ++h) {
+            const int g0 = o0 - G::HI + 2 * h;
+            const bool in = col && (unsigned)g0 < (unsigned)L.d[0];
+            const int pc = min(g0 >> 1, Lc.d[0] - 1) * pplane + pcol;
+            zp[m][h] = ldv<ZT>(rc, in ? (uint32_t)pc * (uint32_t)sizeof(ZT) : kOOB);
+        }
+    }
+    fs_load_dg<DIM, H>(L, o0, o1, o2, dv);
+#pragma unroll
+    for (int m = 0; m < C::ZR; ++m) {
+        const int p = threadIdx.x + m * kFuseThreads;
+        if (p >= C::ZC) continue;
+        const int lj = p / G::SK, lk = p - lj * G::SK;
+        const int g1 = o1 - H + lj, g2 = o2 - H + lk;
+#pragma unroll
+        for (int li = 0; li < G::SI; ++li) {
+            const int e = li * G::SP + p, g0 = o0 - G::HI + li;
+            const uint8_t n = fs_diag(L, dv[m][li], g0, g1, g2);
+            const bool in = inside(L, g0, g1, g2);
+            rs[e] = rv[m][li];
+            nds[e] = n;
+            // solid cells take no prolongation (they carry no pressure); outside
+            // the level both loads read 0 and 0 + 0 stays +0
+            zs[e] = (in && n == kNoUpd) ? 0.f : __fadd_rn(zv[m][li], in ? zp[m][li >> 1] : 0.f);
+        }
+    }
+    __syncthreads();
+    fs_pass<DIM, H, H - 1>(zs, rs, nds, ny, o0, o1, o2, 1);
+    fs_pass<DIM, H, H - 2>(zs, rs, nds, ny, o0, o1, o2, 0);
+    fs_round<ZT, G::NS>(zs);
+    __syncthreads();
+    fs_pass<DIM, H, H - 3>(zs, rs, nds, ny, o0, o1, o2, 1);
+    fs_pass<DIM, H, H - 4>(zs, rs, nds, ny, o0, o1, o2, 0);
+    for (int e = threadIdx.x; e < T::I * T::J * T::K; e += kFuseThreads) {
+        const int ti = e / (T::J * T::K), rem = e - ti * (T::J * T::K), tj = rem / T::K, tk = rem - tj * T::K;
+        const int g0 = o0 + ti, g1 = o1 + tj, g2 = o2 + tk;
+        if (!inside(L, g0, g1, g2)) continue;
+        zout[((int64_t)g0 * L.d[1] + g1) * L.d[2] + g2] = from_f<ZT>(zs[((ti + G::HI) * G::SJ + tj + H) * G::SK + tk + H]);
+    }
+}
+
 // update_p (3D/mgpcg.py:147-150) fused with compute_Ap (:117-122) and the
 // p.Ap reduction: p_new = z + beta p_old on the tile and a one-cell halo,
 // then Ap from LDS.  p_new goes to a separate buffer because neighbouring
@@ -2429,6 +2727,60 @@ static int coarse(MGHandle* h, bool sub_mean, bool dot, hipStream_t s) {
     return PFM_OK;
 }
 
+// Levels whose visits run as one fused down and one fused up launch: below
+// level 0 (whose launches are bandwidth bound and carry the recentring and the
+// z.r partials) and small enough that the launches' fixed latency dominates.
+constexpr int64_t kFuseMaxCells = (int64_t)1 << 18;
+
+static bool fuse_level(const MGHandle* h, int l) { return l >= 1 && h->lv[l].n <= kFuseMaxCells; }
+
+template <int D>
+static TileMap fuse_tiles(const MGHandle* h, int l) {
+    return tile_map(h->lv[l].d, FsTile<D>::I, FsTile<D>::J, FsTile<D>::K);
+}
+
+// z2[l] <- pre-smoothed z of level l, r[l + 1] <- its restricted residual.
+template <typename ZT>
+static int fuse_down(MGHandle* h, int l, hipStream_t s) {
+    const LvlDev L = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
+    const float isc = restrict_inv_scale(h);
+    const ZT* r = (const ZT*)h->r[l];
+    ZT* zo = (ZT*)h->z2[l];
+    ZT* rc = (ZT*)h->r[l + 1];
+    if (h->dim == 3) {
+        const TileMap tm = fuse_tiles<3>(h, l);
+        hipLaunchKernelGGL((k_fuse_down<3, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm, isc,
+                           r, zo, rc);
+    } else {
+        const TileMap tm = fuse_tiles<2>(h, l);
+        hipLaunchKernelGGL((k_fuse_down<2, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm, isc,
+                           r, zo, rc);
+    }
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
+// z[l] <- post-smoothed z2[l] + z[l + 1][parent].
+template <typename ZT>
+static int fuse_up(MGHandle* h, int l, hipStream_t s) {
+    const LvlDev L = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
+    const ZT* zi = (const ZT*)h->z2[l];
+    const ZT* zc = (const ZT*)h->z[l + 1];
+    const ZT* r = (const ZT*)h->r[l];
+    ZT* zo = (ZT*)h->z[l];
+    if (h->dim == 3) {
+        const TileMap tm = fuse_tiles<3>(h, l);
+        hipLaunchKernelGGL((k_fuse_up<3, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm, zi,
+                           zc, r, zo);
+    } else {
+        const TileMap tm = fuse_tiles<2>(h, l);
+        hipLaunchKernelGGL((k_fuse_up<2, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm, zi,
+                           zc, r, zo);
+    }
+    PFM_LAUNCH_CHECK();
+    return PFM_OK;
+}
+
 // apply_preconditioner: 3D/mgpcg.py:182-200.  z[l] holds each level's final
 // z afterwards (z2[l] is scratch).  Levels above coarse_level run as tiled
 // launches, the rest in one k_coarse launch.  With sub_mean, r[0] is
@@ -2450,6 +2802,10 @@ static int vcycle_t(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
         PFM_LAUNCH_CHECK();
     }
     for (int l = 0; l < top; ++l) {
+        if (fuse_level(h, l)) {
+            PFM_TRY(fuse_down<ZT>(h, l, s));
+            continue;
+        }
         const bool sm = tiled_mean && l == 0;
         PFM_TRY(rbgs<ZT>(h, l, kSweepZero, 0, nullptr, (ZT*)h->z2[l], sm, false, s));
         PFM_TRY(rbgs<ZT>(h, l, kSweepPlain, 0, (const ZT*)h->z2[l], (ZT*)h->z[l], sm, false, s));
@@ -2462,6 +2818,10 @@ static int vcycle_t(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
         PFM_TRY(bottom<ZT>(h, L - 1, s));
     }
     for (int l = top - 1; l >= 0; --l) {
+        if (fuse_level(h, l)) {
+            PFM_TRY(fuse_up<ZT>(h, l, s));
+            continue;
+        }
         const bool dot = want_dot && l == 0;
         PFM_TRY(rbgs<ZT>(h, l, kSweepProlong, 1, (const ZT*)h->z[l], (ZT*)h->z2[l], false, false, s));
         PFM_TRY(rbgs<ZT>(h, l, kSweepPlain, 1, (const ZT*)h->z2[l], (ZT*)h->z[l], false, dot, s));
