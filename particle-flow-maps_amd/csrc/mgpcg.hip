@@ -1238,15 +1238,18 @@ __global__ void __launch_bounds__(kBlock) k_restrict_tiled(LvlDev Lf, LvlDev Lc,
 // launches store it, so the results are those of the separate launches bit
 // for bit.  A cell's diagonal byte is kNoUpd outside the level and on solid
 // cells, which are never updated and hold 0.
-template <int DIM> struct FsTile;
-template <> struct FsTile<3> { static constexpr int I = 8, J = 8, K = 16; };
-template <> struct FsTile<2> { static constexpr int I = 1, J = 32, K = 32; };
+template <int D, int TI, int TJ, int TK>
+struct FsTile {
+    static constexpr int DIM = D, I = TI, J = TJ, K = TK;
+};
+using FsTile3 = FsTile<3, 8, 8, 16>;  // 2 blocks per CU
+using FsTile2 = FsTile<2, 1, 32, 32>;
 constexpr int kFuseThreads = 256;
 constexpr uint8_t kNoUpd = 0xFF;
 
-template <int DIM, int H>
+template <typename T, int H>
 struct FsGeom {
-    using T = FsTile<DIM>;
+    static constexpr int DIM = T::DIM;
     static constexpr int HI = DIM == 3 ? H : 0;
     static constexpr int SI = T::I + 2 * HI, SJ = T::J + 2 * H, SK = T::K + 2 * H;
     static constexpr int SP = SJ * SK, NS = SI * SP;
@@ -1266,11 +1269,11 @@ __device__ __forceinline__ void fs_round(float* zs) {
 // takes SEG consecutive pairs of a row (the member's parity is the same along
 // a row): it gathers their neighbour sums first, so the LDS reads overlap, and
 // stores after (a pass reads only the other colour).  ny[n] = (n, RN(1/n)).
-template <int DIM, int H, int RING>
+template <typename T, int H, int RING>
 __device__ __forceinline__ void fs_pass(float* zs, const float* rs, const uint8_t* nds, const float2* ny, int o0,
                                         int o1, int o2, int c) {
-    using G = FsGeom<DIM, H>;
-    using T = FsTile<DIM>;
+    using G = FsGeom<T, H>;
+    constexpr int DIM = T::DIM;
     constexpr int RI = DIM == 3 ? RING : 0;
     constexpr int NI = T::I + 2 * RI, NJ = T::J + 2 * RING, NK = T::K + 2 * RING, HP = NK / 2;
     static_assert(NK % 2 == 0, "whole pairs along axis 2");
@@ -1312,18 +1315,18 @@ __device__ __forceinline__ uint8_t fs_diag(const LvlDev& L, int dv, int g0, int 
 // The LDS box is loaded by columns: a thread owns positions (lj, lk) of the
 // box face and walks down axis 0 with incremental offsets, all loads issued
 // before any is used (kOOB reads 0 for cells outside the level).
-template <int DIM, int H>
+template <typename T, int H>
 struct FsCols {
-    using G = FsGeom<DIM, H>;
+    using G = FsGeom<T, H>;
     static constexpr int ZC = G::SJ * G::SK, ZR = (ZC + kFuseThreads - 1) / kFuseThreads;
 };
 
 // The obstacle bytes of a thread's columns (0 without an obstacle).
-template <int DIM, int H>
+template <typename T, int H>
 __device__ __forceinline__ void fs_load_dg(const LvlDev& L, int o0, int o1, int o2,
-                                           int (&dv)[FsCols<DIM, H>::ZR][FsGeom<DIM, H>::SI]) {
-    using G = FsGeom<DIM, H>;
-    using C = FsCols<DIM, H>;
+                                           int (&dv)[FsCols<T, H>::ZR][FsGeom<T, H>::SI]) {
+    using G = FsGeom<T, H>;
+    using C = FsCols<T, H>;
 #pragma unroll
     for (int m = 0; m < C::ZR; ++m)
 #pragma unroll
@@ -1352,13 +1355,13 @@ __device__ __forceinline__ void fs_load_dg(const LvlDev& L, int o0, int o1, int 
 // whose first child lies in the tile.  The halo of 5 keeps z exact one cell
 // around the tile after the four passes: the trailing child of an odd extent
 // lies one cell past the tile, and its outer neighbour is outside the level.
-template <int DIM, typename ZT>
+template <typename T, typename ZT>
 __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc, TileMap tm, float inv_scale,
                                                             const ZT* __restrict__ r, ZT* __restrict__ zout,
                                                             ZT* __restrict__ rc) {
     constexpr int H = 5;
-    using G = FsGeom<DIM, H>;
-    using T = FsTile<DIM>;
+    using G = FsGeom<T, H>;
+    constexpr int DIM = T::DIM;
     __shared__ float zs[G::NS], rs[G::NS];
     __shared__ uint8_t nds[G::NS];
     __shared__ float2 ny[8];
@@ -1367,7 +1370,7 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc,
     tile_of_block(tm, t);
     const int o0 = t[0] * T::I, o1 = t[1] * T::J, o2 = t[2] * T::K;
     const __amdgpu_buffer_rsrc_t rr = lvl_rsrc(r, L);
-    using C = FsCols<DIM, H>;
+    using C = FsCols<T, H>;
     const int d1 = L.d[1], d2 = L.d[2], plane = d1 * d2;
     float rv[C::ZR][G::SI];
     int dv[C::ZR][G::SI];
@@ -1385,7 +1388,7 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc,
             rv[m][li] = ldv<ZT>(rr, in ? (uint32_t)(g0 * plane + base) * (uint32_t)sizeof(ZT) : kOOB);
         }
     }
-    fs_load_dg<DIM, H>(L, o0, o1, o2, dv);
+    fs_load_dg<T, H>(L, o0, o1, o2, dv);
 #pragma unroll
     for (int m = 0; m < C::ZR; ++m) {
         const int p = threadIdx.x + m * kFuseThreads;
@@ -1400,12 +1403,12 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc,
         }
     }
     __syncthreads();
-    fs_pass<DIM, H, H - 1>(zs, rs, nds, ny, o0, o1, o2, 0);
-    fs_pass<DIM, H, H - 2>(zs, rs, nds, ny, o0, o1, o2, 1);
+    fs_pass<T, H, H - 1>(zs, rs, nds, ny, o0, o1, o2, 0);
+    fs_pass<T, H, H - 2>(zs, rs, nds, ny, o0, o1, o2, 1);
     fs_round<ZT, G::NS>(zs);
     __syncthreads();
-    fs_pass<DIM, H, H - 3>(zs, rs, nds, ny, o0, o1, o2, 0);
-    fs_pass<DIM, H, H - 4>(zs, rs, nds, ny, o0, o1, o2, 1);
+    fs_pass<T, H, H - 3>(zs, rs, nds, ny, o0, o1, o2, 0);
+    fs_pass<T, H, H - 4>(zs, rs, nds, ny, o0, o1, o2, 1);
     fs_round<ZT, G::NS>(zs);
     __syncthreads();
     for (int e = threadIdx.x; e < T::I * T::J * T::K; e += kFuseThreads) {
@@ -1443,13 +1446,13 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc,
 // Up half of a visit of level l >= 1: z = zin + zc[parent] (solid cells keep
 // 0), colours 1, 0, 1, 0; writes the tile's z to zout.  Tiles read their halo
 // from zin, which no block of the launch writes.
-template <int DIM, typename ZT>
+template <typename T, typename ZT>
 __global__ void __launch_bounds__(kFuseThreads) k_fuse_up(LvlDev L, LvlDev Lc, TileMap tm,
                                                           const ZT* __restrict__ zin, const ZT* __restrict__ zc,
                                                           const ZT* __restrict__ r, ZT* __restrict__ zout) {
     constexpr int H = 4;
-    using G = FsGeom<DIM, H>;
-    using T = FsTile<DIM>;
+    using G = FsGeom<T, H>;
+    constexpr int DIM = T::DIM;
     __shared__ float zs[G::NS], rs[G::NS];
     __shared__ uint8_t nds[G::NS];
     __shared__ float2 ny[8];
@@ -1458,8 +1461,8 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_up(LvlDev L, LvlDev Lc, T
     tile_of_block(tm, t);
     const int o0 = t[0] * T::I, o1 = t[1] * T::J, o2 = t[2] * T::K;
     const __amdgpu_buffer_rsrc_t rr = lvl_rsrc(r, L), rz = lvl_rsrc(zin, L), rc = lvl_rsrc(zc, Lc);
-    using C = FsCols<DIM, H>;
-    static_assert(G::HI % 2 == 0 && FsTile<DIM>::I % 2 == 0 || DIM == 2, "even axis-0 box origin");
+    using C = FsCols<T, H>;
+    static_assert((G::HI % 2 == 0 && T::I % 2 == 0) || DIM == 2, "even axis-0 box origin");
     constexpr int SH = (G::SI + 1) / 2;  // axis-0 pairs sharing a coarse parent
     const int d1 = L.d[1], d2 = L.d[2], plane = d1 * d2;
     const int e1 = Lc.d[1], e2 = Lc.d[2], pplane = e1 * e2;
@@ -1489,7 +1492,7 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_up(LvlDev L, LvlDev Lc, T
             zp[m][h] = ldv<ZT>(rc, in ? (uint32_t)pc * (uint32_t)sizeof(ZT) : kOOB);
         }
     }
-    fs_load_dg<DIM, H>(L, o0, o1, o2, dv);
+    fs_load_dg<T, H>(L, o0, o1, o2, dv);
 #pragma unroll
     for (int m = 0; m < C::ZR; ++m) {
         const int p = threadIdx.x + m * kFuseThreads;
@@ -1509,12 +1512,12 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_up(LvlDev L, LvlDev Lc, T
         }
     }
     __syncthreads();
-    fs_pass<DIM, H, H - 1>(zs, rs, nds, ny, o0, o1, o2, 1);
-    fs_pass<DIM, H, H - 2>(zs, rs, nds, ny, o0, o1, o2, 0);
+    fs_pass<T, H, H - 1>(zs, rs, nds, ny, o0, o1, o2, 1);
+    fs_pass<T, H, H - 2>(zs, rs, nds, ny, o0, o1, o2, 0);
     fs_round<ZT, G::NS>(zs);
     __syncthreads();
-    fs_pass<DIM, H, H - 3>(zs, rs, nds, ny, o0, o1, o2, 1);
-    fs_pass<DIM, H, H - 4>(zs, rs, nds, ny, o0, o1, o2, 0);
+    fs_pass<T, H, H - 3>(zs, rs, nds, ny, o0, o1, o2, 1);
+    fs_pass<T, H, H - 4>(zs, rs, nds, ny, o0, o1, o2, 0);
     for (int e = threadIdx.x; e < T::I * T::J * T::K; e += kFuseThreads) {
         const int ti = e / (T::J * T::K), rem = e - ti * (T::J * T::K), tj = rem / T::K, tk = rem - tj * T::K;
         const int g0 = o0 + ti, g1 = o1 + tj, g2 = o2 + tk;
@@ -2734,28 +2737,32 @@ constexpr int64_t kFuseMaxCells = (int64_t)1 << 18;
 
 static bool fuse_level(const MGHandle* h, int l) { return l >= 1 && h->lv[l].n <= kFuseMaxCells; }
 
-template <int D>
+template <typename T>
 static TileMap fuse_tiles(const MGHandle* h, int l) {
-    return tile_map(h->lv[l].d, FsTile<D>::I, FsTile<D>::J, FsTile<D>::K);
+    return tile_map(h->lv[l].d, T::I, T::J, T::K);
+}
+
+template <typename T, typename ZT>
+static void fuse_down_t(MGHandle* h, int l, hipStream_t s) {
+    const LvlDev L = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
+    const TileMap tm = fuse_tiles<T>(h, l);
+    hipLaunchKernelGGL((k_fuse_down<T, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm,
+                       restrict_inv_scale(h), (const ZT*)h->r[l], (ZT*)h->z2[l], (ZT*)h->r[l + 1]);
+}
+
+template <typename T, typename ZT>
+static void fuse_up_t(MGHandle* h, int l, hipStream_t s) {
+    const LvlDev L = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
+    const TileMap tm = fuse_tiles<T>(h, l);
+    hipLaunchKernelGGL((k_fuse_up<T, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm,
+                       (const ZT*)h->z2[l], (const ZT*)h->z[l + 1], (const ZT*)h->r[l], (ZT*)h->z[l]);
 }
 
 // z2[l] <- pre-smoothed z of level l, r[l + 1] <- its restricted residual.
 template <typename ZT>
 static int fuse_down(MGHandle* h, int l, hipStream_t s) {
-    const LvlDev L = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
-    const float isc = restrict_inv_scale(h);
-    const ZT* r = (const ZT*)h->r[l];
-    ZT* zo = (ZT*)h->z2[l];
-    ZT* rc = (ZT*)h->r[l + 1];
-    if (h->dim == 3) {
-        const TileMap tm = fuse_tiles<3>(h, l);
-        hipLaunchKernelGGL((k_fuse_down<3, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm, isc,
-                           r, zo, rc);
-    } else {
-        const TileMap tm = fuse_tiles<2>(h, l);
-        hipLaunchKernelGGL((k_fuse_down<2, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm, isc,
-                           r, zo, rc);
-    }
+    if (h->dim == 2) fuse_down_t<FsTile2, ZT>(h, l, s);
+    else fuse_down_t<FsTile3, ZT>(h, l, s);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -2763,20 +2770,8 @@ static int fuse_down(MGHandle* h, int l, hipStream_t s) {
 // z[l] <- post-smoothed z2[l] + z[l + 1][parent].
 template <typename ZT>
 static int fuse_up(MGHandle* h, int l, hipStream_t s) {
-    const LvlDev L = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
-    const ZT* zi = (const ZT*)h->z2[l];
-    const ZT* zc = (const ZT*)h->z[l + 1];
-    const ZT* r = (const ZT*)h->r[l];
-    ZT* zo = (ZT*)h->z[l];
-    if (h->dim == 3) {
-        const TileMap tm = fuse_tiles<3>(h, l);
-        hipLaunchKernelGGL((k_fuse_up<3, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm, zi,
-                           zc, r, zo);
-    } else {
-        const TileMap tm = fuse_tiles<2>(h, l);
-        hipLaunchKernelGGL((k_fuse_up<2, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm, zi,
-                           zc, r, zo);
-    }
+    if (h->dim == 2) fuse_up_t<FsTile2, ZT>(h, l, s);
+    else fuse_up_t<FsTile3, ZT>(h, l, s);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
