@@ -1355,10 +1355,10 @@ __device__ __forceinline__ void fs_load_dg(const LvlDev& L, int o0, int o1, int 
 // whose first child lies in the tile.  The halo of 5 keeps z exact one cell
 // around the tile after the four passes: the trailing child of an odd extent
 // lies one cell past the tile, and its outer neighbour is outside the level.
-template <typename T, typename ZT>
+template <typename T, typename ZT, typename RT>
 __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc, TileMap tm, float inv_scale,
-                                                            const ZT* __restrict__ r, ZT* __restrict__ zout,
-                                                            ZT* __restrict__ rc) {
+                                                            const RT* __restrict__ r, ZT* __restrict__ zout,
+                                                            ZT* __restrict__ rc, const MGScalars* __restrict__ sc) {
     constexpr int H = 5;
     using G = FsGeom<T, H>;
     constexpr int DIM = T::DIM;
@@ -1385,10 +1385,13 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc,
         for (int li = 0; li < G::SI; ++li) {
             const int g0 = o0 - G::HI + li;
             const bool in = col && (unsigned)g0 < (unsigned)L.d[0];
-            rv[m][li] = ldv<ZT>(rr, in ? (uint32_t)(g0 * plane + base) * (uint32_t)sizeof(ZT) : kOOB);
+            rv[m][li] = ldv<RT>(rr, in ? (uint32_t)(g0 * plane + base) * (uint32_t)sizeof(RT) : kOOB);
         }
     }
     fs_load_dg<T, H>(L, o0, o1, o2, dv);
+    // 16-bit z with an fp32 r (level 0): work on vs * r, as the sweeps do
+    constexpr bool kScale = sizeof(ZT) == 2 && sizeof(RT) == 4;
+    const float vs = kScale ? sc->vs : 1.0f;
 #pragma unroll
     for (int m = 0; m < C::ZR; ++m) {
         const int p = threadIdx.x + m * kFuseThreads;
@@ -1397,7 +1400,7 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc,
 #pragma unroll
         for (int li = 0; li < G::SI; ++li) {
             const int e = li * G::SP + p;
-            rs[e] = rv[m][li];
+            rs[e] = kScale ? rv[m][li] * vs : rv[m][li];
             zs[e] = 0.f;
             nds[e] = fs_diag(L, dv[m][li], o0 - G::HI + li, o1 - H + lj, o2 - H + lk);
         }
@@ -1446,10 +1449,14 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc,
 // Up half of a visit of level l >= 1: z = zin + zc[parent] (solid cells keep
 // 0), colours 1, 0, 1, 0; writes the tile's z to zout.  Tiles read their halo
 // from zin, which no block of the launch writes.
-template <typename T, typename ZT>
+// DOT (level 0): also the z.r partial of the tile (z as stored, r unscaled),
+// the PCG's z.r after the preconditioner (3D/mgpcg.py:231-235, 264-268).
+template <typename T, typename ZT, typename RT, bool DOT>
 __global__ void __launch_bounds__(kFuseThreads) k_fuse_up(LvlDev L, LvlDev Lc, TileMap tm,
                                                           const ZT* __restrict__ zin, const ZT* __restrict__ zc,
-                                                          const ZT* __restrict__ r, ZT* __restrict__ zout) {
+                                                          const RT* __restrict__ r, ZT* __restrict__ zout,
+                                                          const MGScalars* __restrict__ sc,
+                                                          double* __restrict__ partial) {
     constexpr int H = 4;
     using G = FsGeom<T, H>;
     constexpr int DIM = T::DIM;
@@ -1481,7 +1488,7 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_up(LvlDev L, LvlDev Lc, T
             const int g0 = o0 - G::HI + li;
             const bool in = col && (unsigned)g0 < (unsigned)L.d[0];
             const uint32_t off = in ? (uint32_t)(g0 * plane + base) * (uint32_t)sizeof(ZT) : kOOB;
-            rv[m][li] = ldv<ZT>(rr, off);
+            rv[m][li] = ldv<RT>(rr, in ? (uint32_t)(g0 * plane + base) * (uint32_t)sizeof(RT) : kOOB);
             zv[m][li] = ldv<ZT>(rz, off);
         }
 #pragma unroll
@@ -1493,6 +1500,8 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_up(LvlDev L, LvlDev Lc, T
         }
     }
     fs_load_dg<T, H>(L, o0, o1, o2, dv);
+    constexpr bool kScale = sizeof(ZT) == 2 && sizeof(RT) == 4;
+    const float vs = kScale ? sc->vs : 1.0f;
 #pragma unroll
     for (int m = 0; m < C::ZR; ++m) {
         const int p = threadIdx.x + m * kFuseThreads;
@@ -1504,7 +1513,7 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_up(LvlDev L, LvlDev Lc, T
             const int e = li * G::SP + p, g0 = o0 - G::HI + li;
             const uint8_t n = fs_diag(L, dv[m][li], g0, g1, g2);
             const bool in = inside(L, g0, g1, g2);
-            rs[e] = rv[m][li];
+            rs[e] = kScale ? rv[m][li] * vs : rv[m][li];
             nds[e] = n;
             // solid cells take no prolongation (they carry no pressure); outside
             // the level both loads read 0 and 0 + 0 stays +0
@@ -1518,11 +1527,22 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_up(LvlDev L, LvlDev Lc, T
     __syncthreads();
     fs_pass<T, H, H - 3>(zs, rs, nds, ny, o0, o1, o2, 1);
     fs_pass<T, H, H - 4>(zs, rs, nds, ny, o0, o1, o2, 0);
+    double dsum = 0.0;
     for (int e = threadIdx.x; e < T::I * T::J * T::K; e += kFuseThreads) {
         const int ti = e / (T::J * T::K), rem = e - ti * (T::J * T::K), tj = rem / T::K, tk = rem - tj * T::K;
         const int g0 = o0 + ti, g1 = o1 + tj, g2 = o2 + tk;
         if (!inside(L, g0, g1, g2)) continue;
-        zout[((int64_t)g0 * L.d[1] + g1) * L.d[2] + g2] = from_f<ZT>(zs[((ti + G::HI) * G::SJ + tj + H) * G::SK + tk + H]);
+        const int le = ((ti + G::HI) * G::SJ + tj + H) * G::SK + tk + H;
+        const int64_t q = ((int64_t)g0 * L.d[1] + g1) * L.d[2] + g2;
+        const ZT zq = from_f<ZT>(zs[le]);
+        zout[q] = zq;
+        if (DOT && nds[le] != kNoUpd) dsum += (double)to_f(zq) * (double)to_f(r[q]);
+    }
+    if (DOT) {
+        __shared__ double sh[kFuseThreads / 64];
+        dsum = block_sum(dsum, sh);
+        // z was stored scaled when the first level is
+        if (threadIdx.x == 0) partial[blockIdx.x] = kScale ? dsum * (double)sc->vinv : dsum;
     }
 }
 
@@ -2730,12 +2750,17 @@ static int coarse(MGHandle* h, bool sub_mean, bool dot, hipStream_t s) {
     return PFM_OK;
 }
 
-// Levels whose visits run as one fused down and one fused up launch: below
-// level 0 (whose launches are bandwidth bound and carry the recentring and the
-// z.r partials) and small enough that the launches' fixed latency dominates.
+// Levels whose visits run as one fused down and one fused up launch: the
+// levels below level 0 up to 2^18 cells, where the launches' fixed latency
+// dominates, and level 0 of a 2D grid (whose tiles need a small halo box).
+// 3D level 0 keeps the two-pass group sweeps, which are bandwidth bound.
 constexpr int64_t kFuseMaxCells = (int64_t)1 << 18;
+constexpr int64_t kFuseMaxCells2D0 = (int64_t)1 << 22;
 
-static bool fuse_level(const MGHandle* h, int l) { return l >= 1 && h->lv[l].n <= kFuseMaxCells; }
+static bool fuse_level(const MGHandle* h, int l) {
+    if (l == 0) return h->dim == 2 && h->lv[0].n <= kFuseMaxCells2D0;
+    return h->lv[l].n <= kFuseMaxCells;
+}
 
 template <typename T>
 static TileMap fuse_tiles(const MGHandle* h, int l) {
@@ -2746,16 +2771,37 @@ template <typename T, typename ZT>
 static void fuse_down_t(MGHandle* h, int l, hipStream_t s) {
     const LvlDev L = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
     const TileMap tm = fuse_tiles<T>(h, l);
-    hipLaunchKernelGGL((k_fuse_down<T, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm,
-                       restrict_inv_scale(h), (const ZT*)h->r[l], (ZT*)h->z2[l], (ZT*)h->r[l + 1]);
+    const dim3 g((unsigned)tm.total);
+    const float isc = restrict_inv_scale(h);
+    // r is fp32 on level 0 (the PCG residual) and ZT below
+    if (l == 0)
+        hipLaunchKernelGGL((k_fuse_down<T, ZT, float>), g, dim3(kFuseThreads), 0, s, L, Lc, tm, isc,
+                           (const float*)h->r[0], (ZT*)h->z2[0], (ZT*)h->r[1], h->sc);
+    else
+        hipLaunchKernelGGL((k_fuse_down<T, ZT, ZT>), g, dim3(kFuseThreads), 0, s, L, Lc, tm, isc,
+                           (const ZT*)h->r[l], (ZT*)h->z2[l], (ZT*)h->r[l + 1], h->sc);
 }
 
 template <typename T, typename ZT>
-static void fuse_up_t(MGHandle* h, int l, hipStream_t s) {
+static int fuse_up_t(MGHandle* h, int l, bool dot, hipStream_t s) {
     const LvlDev L = lvl_dev(h, l), Lc = lvl_dev(h, l + 1);
     const TileMap tm = fuse_tiles<T>(h, l);
-    hipLaunchKernelGGL((k_fuse_up<T, ZT>), dim3((unsigned)tm.total), dim3(kFuseThreads), 0, s, L, Lc, tm,
-                       (const ZT*)h->z2[l], (const ZT*)h->z[l + 1], (const ZT*)h->r[l], (ZT*)h->z[l]);
+    const dim3 g((unsigned)tm.total);
+    const ZT* zi = (const ZT*)h->z2[l];
+    const ZT* zc = (const ZT*)h->z[l + 1];
+    ZT* zo = (ZT*)h->z[l];
+    if (l == 0 && dot) {
+        if (tm.total > h->n_partial) return PFM_ERR_ARG;  // sized at create
+        hipLaunchKernelGGL((k_fuse_up<T, ZT, float, true>), g, dim3(kFuseThreads), 0, s, L, Lc, tm, zi, zc,
+                           (const float*)h->r[0], zo, h->sc, h->partial);
+    } else if (l == 0) {
+        hipLaunchKernelGGL((k_fuse_up<T, ZT, float, false>), g, dim3(kFuseThreads), 0, s, L, Lc, tm, zi, zc,
+                           (const float*)h->r[0], zo, h->sc, h->partial);
+    } else {
+        hipLaunchKernelGGL((k_fuse_up<T, ZT, ZT, false>), g, dim3(kFuseThreads), 0, s, L, Lc, tm, zi, zc,
+                           (const ZT*)h->r[l], zo, h->sc, h->partial);
+    }
+    return PFM_OK;
 }
 
 // z2[l] <- pre-smoothed z of level l, r[l + 1] <- its restricted residual.
@@ -2767,11 +2813,17 @@ static int fuse_down(MGHandle* h, int l, hipStream_t s) {
     return PFM_OK;
 }
 
-// z[l] <- post-smoothed z2[l] + z[l + 1][parent].
+// z[l] <- post-smoothed z2[l] + z[l + 1][parent]; with dot (level 0) the z.r
+// partials go to `partial`, their count to *nparts.
 template <typename ZT>
-static int fuse_up(MGHandle* h, int l, hipStream_t s) {
-    if (h->dim == 2) fuse_up_t<FsTile2, ZT>(h, l, s);
-    else fuse_up_t<FsTile3, ZT>(h, l, s);
+static int fuse_up(MGHandle* h, int l, bool dot, int* nparts, hipStream_t s) {
+    if (h->dim == 2) {
+        PFM_TRY((fuse_up_t<FsTile2, ZT>(h, l, dot, s)));
+        if (nparts) *nparts = fuse_tiles<FsTile2>(h, l).total;
+    } else {
+        PFM_TRY((fuse_up_t<FsTile3, ZT>(h, l, dot, s)));
+        if (nparts) *nparts = fuse_tiles<FsTile3>(h, l).total;
+    }
     PFM_LAUNCH_CHECK();
     return PFM_OK;
 }
@@ -2789,7 +2841,8 @@ static int vcycle_t(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
     const int top = lc >= 0 ? lc : L - 1;  // levels [0, top) run as tiled launches
     const bool want_dot = dot_parts != nullptr;
     if (dot_parts) *dot_parts = 0;
-    const bool tiled_mean = sub_mean && top > 0 && restrict_tiled_ok(h, 0);
+    // a fused level 0 reads r with a halo, so r is recentred by its own launch first
+    const bool tiled_mean = sub_mean && top > 0 && restrict_tiled_ok(h, 0) && !fuse_level(h, 0);
     const bool coarse_mean = sub_mean && lc == 0;
     if (sub_mean && !tiled_mean && !coarse_mean) {
         hipLaunchKernelGGL(k_sub_mean, dim3(kMgRedBlocks), dim3(kBlock), 0, s, h->lv[0].n, (float*)h->r[0], h->sc,
@@ -2813,11 +2866,11 @@ static int vcycle_t(MGHandle* h, hipStream_t s, bool sub_mean, int* dot_parts) {
         PFM_TRY(bottom<ZT>(h, L - 1, s));
     }
     for (int l = top - 1; l >= 0; --l) {
+        const bool dot = want_dot && l == 0;
         if (fuse_level(h, l)) {
-            PFM_TRY(fuse_up<ZT>(h, l, s));
+            PFM_TRY(fuse_up<ZT>(h, l, dot, dot ? dot_parts : nullptr, s));
             continue;
         }
-        const bool dot = want_dot && l == 0;
         PFM_TRY(rbgs<ZT>(h, l, kSweepProlong, 1, (const ZT*)h->z[l], (ZT*)h->z2[l], false, false, s));
         PFM_TRY(rbgs<ZT>(h, l, kSweepPlain, 1, (const ZT*)h->z2[l], (ZT*)h->z[l], false, dot, s));
         if (dot) *dot_parts = sweep_tiles(h, 0).total;
@@ -3301,6 +3354,7 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     if (hipMalloc(&h->p2, b0) != hipSuccess) return fail("p scratch");
     if (hipMalloc(&h->Ap, b0) != hipSuccess) return fail("Ap");
     h->n_partial = std::max(3 * kMgRedBlocks, sweep_tiles(h, 0).total);
+    if (fuse_level(h, 0)) h->n_partial = std::max(h->n_partial, fuse_tiles<FsTile2>(h, 0).total);
     if (hipMalloc(&h->partial, sizeof(double) * h->n_partial) != hipSuccess) return fail("partial");
     if (hipMalloc(&h->sc, sizeof(MGScalars)) != hipSuccess) return fail("scalars");
     if (hipHostMalloc(&h->host_rtr, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess)
