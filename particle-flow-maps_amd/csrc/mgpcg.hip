@@ -1968,11 +1968,26 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse(CoarseArgs a, int sub
         float* zs = sm + a.zoff[b];
         coarse_cells<DIM>(L, a.m1[b], a.mh[b], sm + a.roff[b], nd + a.noff[b], C);
         if constexpr (sizeof(BT) == 4) {
-            for (int it = 0; it < a.sweeps; ++it) {
-                coarse_pass<DIM, 0>(zs, P, C);
+            if (L.d[0] * L.d[1] * ((L.d[2] + 1) / 2) <= 64) {
+                // every bottom cell is wave 0's (one per lane and colour): a wave's
+                // LDS operations take effect in program order, so its passes need
+                // no block barrier, only a fence against compiler reordering
+                if (tid < 64) {
+                    for (int it = 0; it < a.sweeps; ++it) {
+                        coarse_pass<DIM, 0>(zs, P, C);
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                        coarse_pass<DIM, 1>(zs, P, C);
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                    }
+                }
                 __syncthreads();
-                coarse_pass<DIM, 1>(zs, P, C);
-                __syncthreads();
+            } else {
+                for (int it = 0; it < a.sweeps; ++it) {
+                    coarse_pass<DIM, 0>(zs, P, C);
+                    __syncthreads();
+                    coarse_pass<DIM, 1>(zs, P, C);
+                    __syncthreads();
+                }
             }
         } else {
             // the coarsest-level solve in double, rounded to fp32 once
