@@ -2342,21 +2342,34 @@ __global__ void __launch_bounds__(kBlock) k_subtract_grad_p(LvlDev L, LvlDev F, 
     u[f] = __fsub_rn(u[f], __fsub_rn(prv, plv));
 }
 
-// apply_bc: zero one boundary plane (face index `idx` along `plane_axis`) of
-// the faces of `axis` (3D/mgpcg.py:359-378, 2D/mgpcg.py:292-324).
-__global__ void __launch_bounds__(kBlock) k_zero_plane(int S0, int S1, int S2, int plane_axis,
-                                                       int idx, float* __restrict__ u) {
-    int S[3] = {S0, S1, S2};
-    int a1 = plane_axis == 0 ? 1 : 0;
-    int a2 = plane_axis == 2 ? 1 : 2;
-    const int64_t m = (int64_t)S[a1] * S[a2];
+// apply_bc (3D/mgpcg.py:359-378, 2D/mgpcg.py:292-324): zero boundary planes
+// of the face arrays, every plane of one call in one launch.  Plane k is the
+// face index idx[k] along plane_axis[k] of the array u[k] of extents S[k];
+// thread t takes entry t - off[k] of the plane with off[k] <= t < off[k + 1]
+// (zeroing is idempotent, so planes that share an edge need no order).
+constexpr int kMaxBcPlanes = 12;
+struct BcPlanes {
+    int n;
+    float* u[kMaxBcPlanes];
+    int S[kMaxBcPlanes][3];
+    int plane_axis[kMaxBcPlanes], idx[kMaxBcPlanes];
+    int64_t off[kMaxBcPlanes + 1];
+};
+
+__global__ void __launch_bounds__(kBlock) k_zero_planes(BcPlanes bp) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
+    if (t >= bp.off[bp.n]) return;
+    int k = 0;
+    while (k + 1 < bp.n && t >= bp.off[k + 1]) ++k;
+    const int* S = bp.S[k];
+    const int pa = bp.plane_axis[k];
+    const int a1 = pa == 0 ? 1 : 0, a2 = pa == 2 ? 1 : 2;
+    const int64_t e = t - bp.off[k];
     int I[3];
-    I[plane_axis] = idx;
-    I[a1] = (int)(t / S[a2]);
-    I[a2] = (int)(t % S[a2]);
-    u[((int64_t)I[0] * S[1] + I[1]) * S[2] + I[2]] = 0.0f;
+    I[pa] = bp.idx[k];
+    I[a1] = (int)(e / S[a2]);
+    I[a2] = (int)(e % S[a2]);
+    bp.u[k][((int64_t)I[0] * S[1] + I[1]) * S[2] + I[2]] = 0.0f;
 }
 
 // r = b - A x: the true-residual replacement of the mixed-precision PCG,
@@ -3203,6 +3216,16 @@ static int solve(MGHandle* h, int max_iters, double eps, double tol, int tol_mod
 }
 
 static int apply_bc(MGHandle* h, float* const* u, hipStream_t s) {
+    BcPlanes bp{};
+    auto add = [&](float* f, const int S[3], int pa, int idx) {
+        const int k = bp.n++;
+        bp.u[k] = f;
+        for (int c = 0; c < 3; ++c) bp.S[k][c] = S[c];
+        bp.plane_axis[k] = pa;
+        bp.idx[k] = idx;
+        const int a1 = pa == 0 ? 1 : 0, a2 = pa == 2 ? 1 : 2;
+        bp.off[k + 1] = bp.off[k] + (int64_t)S[a1] * S[a2];
+    };
     for (int a = 0; a < h->dim; ++a) {
         int S[3] = {h->N[0], h->N[1], h->N[2]};
         S[a] += 1;
@@ -3210,28 +3233,20 @@ static int apply_bc(MGHandle* h, float* const* u, hipStream_t s) {
         for (int side = 0; side < 2; ++side) {
             int bt = h->bc[a][side];
             bool zero = bt == 2 || (h->dim == 2 && bt == 1);
-            if (!zero) continue;
-            int idx = side == 0 ? 0 : S[a] - 1;
-            int a1 = a == 0 ? 1 : 0, a2 = a == 2 ? 1 : 2;
-            int64_t m = (int64_t)S[a1] * S[a2];
-            hipLaunchKernelGGL(k_zero_plane, dim3(grid_for(m)), dim3(kBlock), 0, s, S[0], S[1], S[2], a, idx, u[a]);
-            PFM_LAUNCH_CHECK();
+            if (zero) add(u[a], S, a, side == 0 ? 0 : S[a] - 1);
         }
         // 2D Dirichlet walls also zero the tangential faces on them
         if (h->dim == 2) {
             for (int b = 0; b < 2; ++b) {
                 if (b == a) continue;
-                for (int side = 0; side < 2; ++side) {
-                    if (h->bc[b][side] != 1) continue;
-                    int idx = side == 0 ? 0 : S[b] - 1;
-                    int a1 = b == 0 ? 1 : 0, a2 = b == 2 ? 1 : 2;
-                    int64_t m = (int64_t)S[a1] * S[a2];
-                    hipLaunchKernelGGL(k_zero_plane, dim3(grid_for(m)), dim3(kBlock), 0, s, S[0], S[1], S[2], b,
-                                       idx, u[a]);
-                    PFM_LAUNCH_CHECK();
-                }
+                for (int side = 0; side < 2; ++side)
+                    if (h->bc[b][side] == 1) add(u[a], S, b, side == 0 ? 0 : S[b] - 1);
             }
         }
+    }
+    if (bp.n > 0) {
+        hipLaunchKernelGGL(k_zero_planes, dim3(grid_for(bp.off[bp.n])), dim3(kBlock), 0, s, bp);
+        PFM_LAUNCH_CHECK();
     }
     // free slip at the obstacle (an extension: the reference has no solids)
     if (h->dg[0]) {
