@@ -1350,11 +1350,13 @@ __device__ __forceinline__ void fs_load_dg(const LvlDev& L, int o0, int o1, int 
     }
 }
 
-// Down half of a visit of level l >= 1 (r and z stored as ZT): z = 0, colours
-// 0, 1, 0, 1; writes the tile's z to zout and r[l + 1] of the coarse cells
-// whose first child lies in the tile.  The halo of 5 keeps z exact one cell
-// around the tile after the four passes: the trailing child of an odd extent
-// lies one cell past the tile, and its outer neighbour is outside the level.
+// Down half of a level visit: z = 0, colours 0, 1, 0, 1; writes the tile's z
+// to zout and r[l + 1] of the coarse cells whose first child lies in the tile.
+// z is stored as ZT, r as RT (ZT below level 0; fp32 on level 0, where it
+// enters as vs * r with 16-bit z, and has been recentred by its own launch).
+// The halo of 5 keeps z exact one cell around the tile after the four passes:
+// the trailing child of an odd extent lies one cell past the tile, and its
+// outer neighbour is outside the level.
 template <typename T, typename ZT, typename RT>
 __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc, TileMap tm, float inv_scale,
                                                             const RT* __restrict__ r, ZT* __restrict__ zout,
@@ -1446,9 +1448,9 @@ __global__ void __launch_bounds__(kFuseThreads) k_fuse_down(LvlDev L, LvlDev Lc,
     }
 }
 
-// Up half of a visit of level l >= 1: z = zin + zc[parent] (solid cells keep
-// 0), colours 1, 0, 1, 0; writes the tile's z to zout.  Tiles read their halo
-// from zin, which no block of the launch writes.
+// Up half of a level visit: z = zin + zc[parent] (solid cells keep 0),
+// colours 1, 0, 1, 0; writes the tile's z to zout.  Tiles read their halo from
+// zin, which no block of the launch writes.  Storage types as k_fuse_down.
 // DOT (level 0): also the z.r partial of the tile (z as stored, r unscaled),
 // the PCG's z.r after the preconditioner (3D/mgpcg.py:231-235, 264-268).
 template <typename T, typename ZT, typename RT, bool DOT>
