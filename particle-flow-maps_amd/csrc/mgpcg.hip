@@ -2123,20 +2123,55 @@ __global__ void __launch_bounds__(kBlock) k_dot(int64_t n, const AT* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = ascale ? s * (double)*ascale : s;
 }
 
+// The fixed-order sum of k_finalize (kFinThreads = 1024 threads: thread t adds
+// partials t, t + 1024, ... in turn, each wave sums its lanes, thread 0 adds
+// the 16 wave sums in order) formed by a 256-thread block: physical wave w
+// holds the lanes of the virtual waves w, w + 4, w + 8 and w + 12, so every
+// block gets the bits k_finalize would.  Every thread returns the sum.
+__device__ __forceinline__ double fin_sum(const double* __restrict__ part, int nparts, double* sh16) {
+    static_assert(kBlock * 4 == 1024 && kBlock % 64 == 0, "four virtual finalize threads per thread");
+    double v[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        double a = 0.0;
+        for (int b = (int)threadIdx.x + m * kBlock; b < nparts; b += 1024) a += part[b];
+        v[m] = wave_sum(a);
+    }
+    const int pw = (int)threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) sh16[pw + 4 * m] = v[m];
+    }
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) r += sh16[i];
+    __syncthreads();
+    return r;
+}
+
 // update_x + update_r (3D/mgpcg.py:137-145) fused with the r.r and sum(r)
-// reductions (convergence test :249-255 and recenter :174-180).
+// reductions (convergence test :249-255 and recenter :174-180), and with
+// alpha = zTr / (p.Ap + eps) (:239-243) formed in every block from the Ap
+// launch's partials as k_finalize forms it (block 0 also stores it).
 // XX: also the partials of x.x (third block of kMgRedBlocks partials).
 template <bool XX>
 __global__ void __launch_bounds__(kBlock) k_update_xr(int64_t n, float* __restrict__ x,
                                                       float* __restrict__ r, const float* __restrict__ p,
-                                                      const float* __restrict__ Ap,
-                                                      const MGScalars* __restrict__ sc,
+                                                      const float* __restrict__ Ap, MGScalars* __restrict__ sc,
+                                                      const double* __restrict__ ap_part, int ap_n, double eps,
                                                       double* __restrict__ partial) {
     __shared__ double sh[kBlock / 64];
+    __shared__ double sh16[16];
     // past convergence x and r stay as they are (and nothing reads the
     // partials: k_finalize leaves the scalars alone)
     if (pcg_stopped(sc)) return;
-    const float alpha = sc->alpha;
+    const double pAp = fin_sum(ap_part, ap_n, sh16);
+    const float alpha = (float)(sc->zTr / (pAp + eps));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->pAp = pAp;
+        sc->alpha = alpha;
+    }
     double s2 = 0.0, s1 = 0.0, s3 = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x) {
@@ -2944,6 +2979,10 @@ static int dot_zr(MGHandle* h, hipStream_t s) {
 
 // p <- z + beta p (into the partner buffer), Ap and p.Ap partials; returns
 // the partial count through *parts.
+// The Ap launch's partials follow the three blocks of k_update_xr's, which
+// reads them while writing its own.
+constexpr int kApPartOff = 3 * kMgRedBlocks;
+
 static int ap_dot_p(MGHandle* h, hipStream_t s, int* parts) {
     const LvlDev L0 = lvl_dev(h, 0);
     const TileMap tm = sweep_tiles(h, 0);
@@ -2953,10 +2992,10 @@ static int ap_dot_p(MGHandle* h, hipStream_t s, int* parts) {
 #define PFM_AP(D, ZT)                                                                                           \
     if (D == 3 && groups)                                                                                       \
         hipLaunchKernelGGL((k_ap_dot_p3<ZT>), g, dim3(kSw3Threads), 0, s, L0, tm, (const ZT*)h->z[0], h->p, h->p2, \
-                           h->Ap, h->sc, h->partial);                                                           \
+                           h->Ap, h->sc, h->partial + kApPartOff);                                              \
     else                                                                                                        \
         hipLaunchKernelGGL((k_ap_dot_p<D, ZT>), g, dim3(kBlock), 0, s, L0, tm, (const ZT*)h->z[0], h->p, h->p2,  \
-                           h->Ap, h->sc, h->partial)
+                           h->Ap, h->sc, h->partial + kApPartOff)
     switch (h->precision) {
         case 1:
             if (h->dim == 3) PFM_AP(3, __half);
@@ -3062,13 +3101,13 @@ static int pcg_head(MGHandle* h, int k, double eps, hipStream_t s) {
     const bool replace = h->precision != 0;
     int parts = 0;
     PFM_TRY(ap_dot_p(h, s, &parts));
-    PFM_TRY(finalize(h, kFinAlpha, eps, s, parts));
+    const double* ap_part = h->partial + kApPartOff;
     if (replace)
         hipLaunchKernelGGL(k_update_xr<true>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, r0, h->p, h->Ap,
-                           h->sc, h->partial);
+                           h->sc, ap_part, parts, eps, h->partial);
     else
         hipLaunchKernelGGL(k_update_xr<false>, dim3(kMgRedBlocks), dim3(kBlock), 0, s, n, h->x, r0, h->p, h->Ap,
-                           h->sc, h->partial);
+                           h->sc, ap_part, parts, eps, h->partial);
     PFM_LAUNCH_CHECK();
     const bool due = replace && (k + 1) % kReplaceEvery == 0;
     PFM_TRY(finalize(h, replace ? kFinRTrX : kFinRTr, eps, s, kMgRedBlocks, due ? 0 : 1));
@@ -3386,6 +3425,7 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     if (hipMalloc(&h->p2, b0) != hipSuccess) return fail("p scratch");
     if (hipMalloc(&h->Ap, b0) != hipSuccess) return fail("Ap");
     h->n_partial = std::max(3 * kMgRedBlocks, sweep_tiles(h, 0).total);
+    h->n_partial = std::max(h->n_partial, kApPartOff + sweep_tiles(h, 0).total);
     if (fuse_level(h, 0)) h->n_partial = std::max(h->n_partial, fuse_tiles<FsTile2>(h, 0).total);
     if (hipMalloc(&h->partial, sizeof(double) * h->n_partial) != hipSuccess) return fail("partial");
     if (hipMalloc(&h->sc, sizeof(MGScalars)) != hipSuccess) return fail("scalars");
