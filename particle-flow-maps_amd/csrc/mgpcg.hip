@@ -2208,6 +2208,7 @@ __global__ void __launch_bounds__(kBlock) k_update_p(int64_t n, float* __restric
 __global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t n, float* __restrict__ r,
                                                      const MGScalars* __restrict__ sc,
                                                      const uint8_t* __restrict__ dg) {
+    if (pcg_stopped(sc)) return;  // past convergence r stays as it is
     const float m = sc->mean;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * blockDim.x)
