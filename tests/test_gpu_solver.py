@@ -19,8 +19,10 @@ def N(t):
     return t.detach().cpu().numpy()
 
 
+# (256, 192) and (170, 130): 2D grids large enough that level 0 and the next
+# level run as fused visits before the coarse kernel (the latter with odd extents)
 @pytest.mark.parametrize("N_", [(32, 32), (16, 16, 16), (32, 16, 16), (64, 64), (50, 34, 34), (100, 68),
-                                (40, 72, 24), (64, 40, 48), (64, 64, 64)])
+                                (40, 72, 24), (64, 40, 48), (64, 64, 64), (256, 192), (170, 130)])
 def test_mg_passes_bit_exact(oracle, pfm, N_):
     d = len(N_)
     mg_o = oracle.MG(N_, base_level=3)
@@ -98,7 +100,8 @@ def test_graph_replayed_iterations_equal_eager(pfm, N_, prec, with_solid):
 
 
 @pytest.mark.parametrize("N_,tol_mode,tol", [((64, 64), "abs", 1e-12), ((32, 32, 32), "rel", 1e-6),
-                                             ((16, 16, 16), "abs", 1e-12), ((64, 32, 32), "rel", 1e-7)])
+                                             ((16, 16, 16), "abs", 1e-12), ((64, 32, 32), "rel", 1e-7),
+                                             ((256, 192), "rel", 1e-7)])
 def test_poisson_matches_oracle(oracle, pfm, N_, tol_mode, tol):
     d = len(N_)
     u = smooth_mac_field(N_, seed=21)
@@ -161,7 +164,8 @@ def test_solver_api_pieces(oracle, pfm):
 
 
 # ---- solid obstacle (extension) ---------------------------------------------
-@pytest.mark.parametrize("N_", [(16, 16, 16), (32, 32, 32), (64, 64), (50, 34, 34), (40, 72, 24), (64, 40, 48)])
+@pytest.mark.parametrize("N_", [(16, 16, 16), (32, 32, 32), (64, 64), (50, 34, 34), (40, 72, 24), (64, 40, 48),
+                                (200, 150)])
 def test_mg_solid_passes_bit_exact(oracle, pfm, N_):
     from helpers import box_solid
 
@@ -203,7 +207,7 @@ def test_mg_solid_passes_bit_exact(oracle, pfm, N_):
     assert np.all(zo[solid] == 0)
 
 
-@pytest.mark.parametrize("N_", [(32, 32, 32), (64, 48)])
+@pytest.mark.parametrize("N_", [(32, 32, 32), (64, 48), (200, 150)])
 def test_poisson_solid_vs_oracles(oracle, pfm, N_):
     from helpers import box_solid
 
