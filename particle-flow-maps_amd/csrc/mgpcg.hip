@@ -1242,8 +1242,14 @@ template <int D, int TI, int TJ, int TK>
 struct FsTile {
     static constexpr int DIM = D, I = TI, J = TJ, K = TK;
 };
-using FsTile3 = FsTile<3, 8, 8, 16>;  // 2 blocks per CU
+// 3D: small tiles, so each block's chain (box load, four passes, stores) is
+// short; the launches of these levels are latency bound (measured: 32^3 down
+// and up 10.2 / 8.3 us with 4x8x8 against 15.4 / 13.4 us with 8x8x16; 64^3
+// 15.5 / 12.8 us with 4x8x16)
+using FsTile3 = FsTile<3, 4, 8, 16>;
+using FsTile3S = FsTile<3, 4, 8, 8>;  // levels up to kFuseSmallCells
 using FsTile2 = FsTile<2, 1, 32, 32>;
+constexpr int64_t kFuseSmallCells = (int64_t)1 << 15;
 constexpr int kFuseThreads = 256;
 constexpr uint8_t kNoUpd = 0xFF;
 
@@ -2874,6 +2880,7 @@ static int fuse_up_t(MGHandle* h, int l, bool dot, hipStream_t s) {
 template <typename ZT>
 static int fuse_down(MGHandle* h, int l, hipStream_t s) {
     if (h->dim == 2) fuse_down_t<FsTile2, ZT>(h, l, s);
+    else if (h->lv[l].n <= kFuseSmallCells) fuse_down_t<FsTile3S, ZT>(h, l, s);
     else fuse_down_t<FsTile3, ZT>(h, l, s);
     PFM_LAUNCH_CHECK();
     return PFM_OK;
@@ -2886,6 +2893,9 @@ static int fuse_up(MGHandle* h, int l, bool dot, int* nparts, hipStream_t s) {
     if (h->dim == 2) {
         PFM_TRY((fuse_up_t<FsTile2, ZT>(h, l, dot, s)));
         if (nparts) *nparts = fuse_tiles<FsTile2>(h, l).total;
+    } else if (h->lv[l].n <= kFuseSmallCells) {
+        PFM_TRY((fuse_up_t<FsTile3S, ZT>(h, l, dot, s)));
+        if (nparts) *nparts = fuse_tiles<FsTile3S>(h, l).total;
     } else {
         PFM_TRY((fuse_up_t<FsTile3, ZT>(h, l, dot, s)));
         if (nparts) *nparts = fuse_tiles<FsTile3>(h, l).total;
