@@ -759,14 +759,18 @@ __device__ __forceinline__ void combine_one(const GridDev& g, const CombineArgs&
     int S[3], I[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) S[c] = g.res[c] + (c == A ? 1 : 0);
+    // make_grid rejects face fields of 2 GiB or more, so f < 2^29 and the
+    // decode is 32-bit
+    const uint32_t fu = (uint32_t)f;
     if (DIM == 3) {
-        I[2] = (int)(f % S[2]);
-        I[1] = (int)((f / S[2]) % S[1]);
-        I[0] = (int)(f / ((int64_t)S[2] * S[1]));
+        const uint32_t r = fu / (uint32_t)S[2];
+        I[2] = (int)(fu - r * (uint32_t)S[2]);
+        I[1] = (int)(r % (uint32_t)S[1]);
+        I[0] = (int)(r / (uint32_t)S[1]);
     } else {
         I[2] = 0;
-        I[1] = (int)(f % S[1]);
-        I[0] = (int)(f / S[1]);
+        I[1] = (int)(fu % (uint32_t)S[1]);
+        I[0] = (int)(fu / (uint32_t)S[1]);
     }
     float su, sw;
     if (A == 0)
@@ -781,10 +785,20 @@ __device__ __forceinline__ void combine_one(const GridDev& g, const CombineArgs&
     if (a.W[A]) a.W[A][f] = sw;
 }
 
+// Workgroups are dealt to the 8 XCDs round robin by id; block b is remapped
+// so that XCD b % 8 walks a contiguous range of face rows.  A 128-byte line of
+// a tile's partial box holds about three rows of it, read by consecutive
+// blocks: with the plain order three XCDs fetched it from memory, with this
+// order one does (the map is a bijection onto [0, gridDim.x) for any count).
+__device__ __forceinline__ int64_t xcd_contiguous_block() {
+    const int nb = gridDim.x, x = blockIdx.x & 7, k = blockIdx.x >> 3;
+    return (int64_t)x * (nb >> 3) + min(x, nb & 7) + k;
+}
+
 template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_p2g_combine(GridDev g, CombineArgs a,
                                                         const float* __restrict__ partial) {
-    int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t f = xcd_contiguous_block() * blockDim.x + threadIdx.x;
     int A = 0;
     while (A < DIM - 1 && f >= a.nf[A]) {
         f -= a.nf[A];
