@@ -785,16 +785,8 @@ __device__ __forceinline__ void combine_one(const GridDev& g, const CombineArgs&
     if (a.W[A]) a.W[A][f] = sw;
 }
 
-// Workgroups are dealt to the 8 XCDs round robin by id; block b is remapped
-// so that XCD b % 8 walks a contiguous range of face rows.  A 128-byte line of
-// a tile's partial box holds about three rows of it, read by consecutive
-// blocks: with the plain order three XCDs fetched it from memory, with this
-// order one does (the map is a bijection onto [0, gridDim.x) for any count).
-__device__ __forceinline__ int64_t xcd_contiguous_block() {
-    const int nb = gridDim.x, x = blockIdx.x & 7, k = blockIdx.x >> 3;
-    return (int64_t)x * (nb >> 3) + min(x, nb & 7) + k;
-}
-
+// Blocks in XCD-contiguous order: a 128-byte line of a tile's partial box
+// holds about three face rows of it, read by consecutive blocks.
 template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_p2g_combine(GridDev g, CombineArgs a,
                                                         const float* __restrict__ partial) {
