@@ -1248,8 +1248,12 @@ struct FsTile {
 // 15.5 / 12.8 us with 4x8x16)
 using FsTile3 = FsTile<3, 4, 8, 16>;
 using FsTile3S = FsTile<3, 4, 8, 8>;  // levels up to kFuseSmallCells
-using FsTile2 = FsTile<2, 1, 32, 32>;
 constexpr int64_t kFuseSmallCells = (int64_t)1 << 15;
+// 2D likewise (2D Poisson 512^2: 0.707 ms with 8x32 up to 2^18 cells, 0.778
+// with 32x32; at 1024^2 16x32 tiles on level 0 beat 8x32, 0.873 / 0.916 ms)
+using FsTile2 = FsTile<2, 1, 16, 32>;
+using FsTile2S = FsTile<2, 1, 8, 32>;  // levels up to kFuseSmallCells2D
+constexpr int64_t kFuseSmallCells2D = (int64_t)1 << 18;
 constexpr int kFuseThreads = 256;
 constexpr uint8_t kNoUpd = 0xFF;
 
@@ -2879,7 +2883,8 @@ static int fuse_up_t(MGHandle* h, int l, bool dot, hipStream_t s) {
 // z2[l] <- pre-smoothed z of level l, r[l + 1] <- its restricted residual.
 template <typename ZT>
 static int fuse_down(MGHandle* h, int l, hipStream_t s) {
-    if (h->dim == 2) fuse_down_t<FsTile2, ZT>(h, l, s);
+    if (h->dim == 2 && h->lv[l].n <= kFuseSmallCells2D) fuse_down_t<FsTile2S, ZT>(h, l, s);
+    else if (h->dim == 2) fuse_down_t<FsTile2, ZT>(h, l, s);
     else if (h->lv[l].n <= kFuseSmallCells) fuse_down_t<FsTile3S, ZT>(h, l, s);
     else fuse_down_t<FsTile3, ZT>(h, l, s);
     PFM_LAUNCH_CHECK();
@@ -2890,7 +2895,10 @@ static int fuse_down(MGHandle* h, int l, hipStream_t s) {
 // partials go to `partial`, their count to *nparts.
 template <typename ZT>
 static int fuse_up(MGHandle* h, int l, bool dot, int* nparts, hipStream_t s) {
-    if (h->dim == 2) {
+    if (h->dim == 2 && h->lv[l].n <= kFuseSmallCells2D) {
+        PFM_TRY((fuse_up_t<FsTile2S, ZT>(h, l, dot, s)));
+        if (nparts) *nparts = fuse_tiles<FsTile2S>(h, l).total;
+    } else if (h->dim == 2) {
         PFM_TRY((fuse_up_t<FsTile2, ZT>(h, l, dot, s)));
         if (nparts) *nparts = fuse_tiles<FsTile2>(h, l).total;
     } else if (h->lv[l].n <= kFuseSmallCells) {
@@ -3437,7 +3445,9 @@ int pfm_mg_create(int dim, const int32_t* N, int base_level, const int32_t* bc, 
     if (hipMalloc(&h->Ap, b0) != hipSuccess) return fail("Ap");
     h->n_partial = std::max(3 * kMgRedBlocks, sweep_tiles(h, 0).total);
     h->n_partial = std::max(h->n_partial, kApPartOff + sweep_tiles(h, 0).total);
-    if (fuse_level(h, 0)) h->n_partial = std::max(h->n_partial, fuse_tiles<FsTile2>(h, 0).total);
+    if (fuse_level(h, 0))
+        h->n_partial = std::max(h->n_partial, h->lv[0].n <= kFuseSmallCells2D ? fuse_tiles<FsTile2S>(h, 0).total
+                                                                               : fuse_tiles<FsTile2>(h, 0).total);
     if (hipMalloc(&h->partial, sizeof(double) * h->n_partial) != hipSuccess) return fail("partial");
     if (hipMalloc(&h->sc, sizeof(MGScalars)) != hipSuccess) return fail("scalars");
     if (hipHostMalloc(&h->host_rtr, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess)
