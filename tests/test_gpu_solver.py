@@ -20,9 +20,11 @@ def N(t):
 
 
 # (256, 192) and (170, 130): 2D grids large enough that level 0 and the next
-# level run as fused visits before the coarse kernel (the latter with odd extents)
+# level run as fused visits before the coarse kernel (the latter with odd
+# extents); (640, 480): a 2D level 0 above 2^18 cells, which the fused visits
+# tile with the larger of their two 2D tiles
 @pytest.mark.parametrize("N_", [(32, 32), (16, 16, 16), (32, 16, 16), (64, 64), (50, 34, 34), (100, 68),
-                                (40, 72, 24), (64, 40, 48), (64, 64, 64), (256, 192), (170, 130)])
+                                (40, 72, 24), (64, 40, 48), (64, 64, 64), (256, 192), (170, 130), (640, 480)])
 def test_mg_passes_bit_exact(oracle, pfm, N_):
     d = len(N_)
     mg_o = oracle.MG(N_, base_level=3)
@@ -101,7 +103,7 @@ def test_graph_replayed_iterations_equal_eager(pfm, N_, prec, with_solid):
 
 @pytest.mark.parametrize("N_,tol_mode,tol", [((64, 64), "abs", 1e-12), ((32, 32, 32), "rel", 1e-6),
                                              ((16, 16, 16), "abs", 1e-12), ((64, 32, 32), "rel", 1e-7),
-                                             ((256, 192), "rel", 1e-7)])
+                                             ((256, 192), "rel", 1e-7), ((640, 480), "rel", 1e-7)])
 def test_poisson_matches_oracle(oracle, pfm, N_, tol_mode, tol):
     d = len(N_)
     u = smooth_mac_field(N_, seed=21)
