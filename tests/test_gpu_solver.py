@@ -22,9 +22,11 @@ def N(t):
 # (256, 192) and (170, 130): 2D grids large enough that level 0 and the next
 # level run as fused visits before the coarse kernel (the latter with odd
 # extents); (640, 480): a 2D level 0 above 2^18 cells, which the fused visits
-# tile with the larger of their two 2D tiles
+# tile with the larger of their two 2D tiles; (96, 80, 64): a 3D level 1 of
+# 48 x 40 x 32 cells, above 2^15, which takes the larger 3D tile
 @pytest.mark.parametrize("N_", [(32, 32), (16, 16, 16), (32, 16, 16), (64, 64), (50, 34, 34), (100, 68),
-                                (40, 72, 24), (64, 40, 48), (64, 64, 64), (256, 192), (170, 130), (640, 480)])
+                                (40, 72, 24), (64, 40, 48), (64, 64, 64), (256, 192), (170, 130), (640, 480),
+                                (96, 80, 64)])
 def test_mg_passes_bit_exact(oracle, pfm, N_):
     d = len(N_)
     mg_o = oracle.MG(N_, base_level=3)
