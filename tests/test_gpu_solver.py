@@ -168,8 +168,10 @@ def test_solver_api_pieces(oracle, pfm):
 
 
 # ---- solid obstacle (extension) ---------------------------------------------
+# (96, 80, 64) and (640, 480): the larger fused tiles in 3D and 2D (see
+# test_mg_passes_bit_exact) with the obstacle masks
 @pytest.mark.parametrize("N_", [(16, 16, 16), (32, 32, 32), (64, 64), (50, 34, 34), (40, 72, 24), (64, 40, 48),
-                                (200, 150)])
+                                (200, 150), (96, 80, 64), (640, 480)])
 def test_mg_solid_passes_bit_exact(oracle, pfm, N_):
     from helpers import box_solid
 
@@ -267,11 +269,16 @@ def _fp64_pressure(oracle, N_, u, solid, require_converged=True):
 @pytest.mark.parametrize("precision", [1, 2])
 @pytest.mark.parametrize("N_,with_solid", [((32, 32, 32), False), ((48, 32, 32), True), ((64, 64), False),
                                            ((64, 48), True), ((51, 34, 34), False), ((170, 170), False),
-                                           ((90, 34, 34), True)])
+                                           ((90, 34, 34), True), ((96, 80, 64), True)])
 def test_mixed_precision_poisson_vs_fp64(oracle, pfm, precision, N_, with_solid):
     """(51,34,34), (170,170) and (90,34,34) have odd extents on levels that
     run as separate launches (51: on the finest level), so their restriction
-    is the typed per-cell gather."""
+    is the typed per-cell gather; (96,80,64) has a fused level above 2^15
+    cells (the larger 3D tile).  The 1e-5 bound on the pressure holds for the
+    fp32 solve on these sizes only: at a 1e-6 relative residual the error
+    grows with the condition number (N^2), and the 2D grids above 2^18 cells
+    are covered by test_mixed_precision_large_2d_point_vortices, which bounds
+    the mixed solve by the fp32 solve's own error."""
     from helpers import box_solid
 
     d = len(N_)
