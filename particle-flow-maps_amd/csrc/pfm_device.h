@@ -278,6 +278,63 @@ __device__ __forceinline__ void interp_mac(const float* __restrict__ ux,
     }
 }
 
+// The rows of the three MAC components of one point kept across calls (the
+// stages of a particle's RK4), each with the byte offset of its first row.
+// A later point whose stencil starts at the same offset reads the same rows,
+// so they are not gathered again: with exec masks only the lanes whose
+// stencil moved issue the row loads, and the vector-memory address unit's
+// work falls with the active lanes.  The values are those the loads would
+// return, so the results do not change.
+template <int DIM>
+struct RowCache {
+    Rows<DIM> r[DIM];
+    uint32_t key[DIM];
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) key[a] = 0xFFFFFFFFu;  // no row starts there (offsets < 2^31)
+    }
+};
+
+template <int DIM, int A, bool GRAD>
+__device__ __forceinline__ void load_rows_cached(const float* __restrict__ fld, const GridDev& g,
+                                                 const MacStencil<DIM, GRAD>& st, Rows<DIM>& r, uint32_t& key) {
+    static_assert(DIM == 3, "3D rows");
+    const AxisStencil& s0 = A == 0 ? st.F[0] : st.C[0];
+    const AxisStencil& s1 = A == 1 ? st.F[1] : st.C[1];
+    const AxisStencil& s2 = A == 2 ? st.F[2] : st.C[2];
+    const int S0 = g.res[0] + (A == 0 ? 1 : 0);
+    const int S1 = g.res[1] + (A == 1 ? 1 : 0);
+    const int S2 = g.res[2] + (A == 2 ? 1 : 0);
+    const uint32_t rowb = (uint32_t)S2 * 4u;
+    const uint32_t plane = (uint32_t)S1 * rowb;
+    const uint32_t base = (uint32_t)s0.j0 * plane + (uint32_t)s1.j0 * rowb + (uint32_t)s2.j0 * 4u;
+    if (base != key) {
+        key = base;
+        const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, plane * (uint32_t)S0);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                r.q[3 * a + b] = __builtin_amdgcn_raw_buffer_load_b96(rs, base + a * plane + b * rowb, 0, 0);
+    }
+}
+
+// interp_mac (3D) with the rows taken from `cache` where the stencil did not
+// move; the contraction is combine_rows's, so the result is bit-identical.
+template <bool GRAD>
+__device__ __forceinline__ void interp_mac_cached(const float* __restrict__ ux, const float* __restrict__ uy,
+                                                  const float* __restrict__ uz, const GridDev& g, const float p[3],
+                                                  float u[3], float G[3][3], RowCache<3>& cache) {
+    MacStencil<3, GRAD> st;
+    st.build(g, p);
+    load_rows_cached<3, 0, GRAD>(ux, g, st, cache.r[0], cache.key[0]);
+    load_rows_cached<3, 1, GRAD>(uy, g, st, cache.r[1], cache.key[1]);
+    load_rows_cached<3, 2, GRAD>(uz, g, st, cache.r[2], cache.key[2]);
+    combine_rows<3, 0, GRAD>(g, st, cache.r[0], u[0], G[0]);
+    combine_rows<3, 1, GRAD>(g, st, cache.r[1], u[1], G[1]);
+    combine_rows<3, 2, GRAD>(g, st, cache.r[2], u[2], G[2]);
+}
+
 // ---- P2G particle record ---------------------------------------------------
 // Record of one particle for the tiled P2G, kRec floats:
 //   3D: [0..2] pos/dx, [3] 0, then a quad per face axis A at 4 + 4A holding
