@@ -39,22 +39,19 @@ template <int DIM>
 __device__ __forceinline__ void rk4_face_reg(const GridDev& g, const float p0[3], const float T0[3],
                                              const float* __restrict__ ux, const float* __restrict__ uy,
                                              const float* __restrict__ uz, float dt, float p1[3], float T1[3],
-                                             RowCache<3>& rows) {
+                                             RowCache<DIM>& rows) {
     const float neg_dt = -1.0f * dt;
     const float hs[3] = {0.5f * neg_dt, 0.5f * neg_dt, 1.0f * neg_dt};
     const float ws[4] = {1.f, 2.f, 2.f, 1.f};
     float pk[3] = {p0[0], p0[1], p0[2]}, Tk[3] = {T0[0], T0[1], T0[2]};
     float us[3] = {0.f, 0.f, 0.f}, dTs[3] = {0.f, 0.f, 0.f};
-    // 3D: the stages reuse a component's rows where its stencil did not move
+    // the stages reuse a component's rows where its stencil did not move
     // (interp_mac_cached; bit-identical to interp_mac); `rows` arrives cleared
     // and leaves holding the last stage's rows
 #pragma unroll 1
     for (int st = 0; st < 4; ++st) {
         float u[3], G[3][3], dT[3];
-        if constexpr (DIM == 3)
-            interp_mac_cached<true>(ux, uy, uz, g, pk, u, G, rows);
-        else
-            interp_mac<DIM, true>(ux, uy, uz, g, pk, u, G);
+        interp_mac_cached<DIM, true>(ux, uy, uz, g, pk, u, G, rows);
 #pragma unroll
         for (int r = 0; r < DIM; ++r) {
             float acc = G[r][0] * Tk[0];
@@ -93,7 +90,7 @@ __device__ __forceinline__ void rk4_face(const GridDev& g, int64_t f, int64_t nf
         p0[c] = psi[c * nf + f];
         T0[c] = T[c * nf + f];
     }
-    RowCache<3> rows;
+    RowCache<DIM> rows;
     rows.clear();
     rk4_face_reg<DIM>(g, p0, T0, ux, uy, uz, dt, p1, T1, rows);
 #pragma unroll
@@ -141,7 +138,7 @@ __device__ __forceinline__ float advect_face_reg(const GridDev& g, const float* 
 __device__ __forceinline__ float advect_face_reg3_cached(const GridDev& g, const float* const u0[3], const float p[3],
                                                          const float Tv[3], RowCache<3>& rows) {
     float u[3], G[3][3];
-    interp_mac_cached<false>(u0[0], u0[1], u0[2], g, p, u, G, rows);
+    interp_mac_cached<3, false>(u0[0], u0[1], u0[2], g, p, u, G, rows);
     float acc = Tv[0] * u[0];
 #pragma unroll
     for (int c = 1; c < 3; ++c) acc += Tv[c] * u[c];
@@ -221,7 +218,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_midpoint_advect(GridDev g, Advect
     if (f >= a.nf[ax]) return;
     float p0[3] = {0.f, 0.f, 0.f}, T0[3] = {0.f, 0.f, 0.f}, p1[3] = {0.f, 0.f, 0.f}, T1[3] = {0.f, 0.f, 0.f};
     identity_face<DIM>(g, ax, f, p0, T0);
-    RowCache<3> rows;
+    RowCache<DIM> rows;
     rows.clear();
     rk4_face_reg<DIM>(g, p0, T0, a.u0[0], a.u0[1], a.u0[2], dt, p1, T1, rows);
     const int64_t nf = a.nf[ax];
@@ -392,7 +389,7 @@ __global__ void __launch_bounds__(kBlock, 4) kb_midpoint_advect(BUnits t, int64_
         const int64_t gf = S.f_off[bf.ax] + bf.f, nf = bf.ax == 0 ? F0 : F1;
         float p0[3] = {0.f, 0.f, 0.f}, T0[3] = {0.f, 0.f, 0.f}, p1[3] = {0.f, 0.f, 0.f}, T1[3] = {0.f, 0.f, 0.f};
         identity_face<2>(S.g, bf.ax, bf.f, p0, T0);
-        RowCache<3> rows;  // unused in 2D
+        RowCache<2> rows;
         rows.clear();
         rk4_face_reg<2>(S.g, p0, T0, u0[0], u0[1], nullptr, dt, p1, T1, rows);
         float* psi = bf.ax == 0 ? psi_x : psi_y;

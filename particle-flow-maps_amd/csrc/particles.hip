@@ -80,15 +80,12 @@ __device__ __forceinline__ void rk4_particle(const GridDev& g, int64_t i, int64_
             dTs[m][c] = 0.f;
         }
 
-    RowCache<3> rows;
+    RowCache<DIM> rows;
     rows.clear();
 #pragma unroll 1
     for (int st = 0; st < 4; ++st) {
         float u[3], G[3][3];
-        if constexpr (DIM == 3)
-            interp_mac_cached<true>(ux, uy, uz, g, pk, u, G, rows);
-        else
-            interp_mac<DIM, true>(ux, uy, uz, g, pk, u, G);
+        interp_mac_cached<DIM, true>(ux, uy, uz, g, pk, u, G, rows);
         float dT[3][3];
 #pragma unroll
         for (int m = 0; m < DIM; ++m)

@@ -298,41 +298,56 @@ struct RowCache {
 template <int DIM, int A, bool GRAD>
 __device__ __forceinline__ void load_rows_cached(const float* __restrict__ fld, const GridDev& g,
                                                  const MacStencil<DIM, GRAD>& st, Rows<DIM>& r, uint32_t& key) {
-    static_assert(DIM == 3, "3D rows");
     const AxisStencil& s0 = A == 0 ? st.F[0] : st.C[0];
     const AxisStencil& s1 = A == 1 ? st.F[1] : st.C[1];
-    const AxisStencil& s2 = A == 2 ? st.F[2] : st.C[2];
     const int S0 = g.res[0] + (A == 0 ? 1 : 0);
     const int S1 = g.res[1] + (A == 1 ? 1 : 0);
-    const int S2 = g.res[2] + (A == 2 ? 1 : 0);
-    const uint32_t rowb = (uint32_t)S2 * 4u;
-    const uint32_t plane = (uint32_t)S1 * rowb;
-    const uint32_t base = (uint32_t)s0.j0 * plane + (uint32_t)s1.j0 * rowb + (uint32_t)s2.j0 * 4u;
-    if (base != key) {
-        key = base;
-        const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, plane * (uint32_t)S0);
+    if constexpr (DIM == 3) {
+        const AxisStencil& s2 = A == 2 ? st.F[2] : st.C[2];
+        const int S2 = g.res[2] + (A == 2 ? 1 : 0);
+        const uint32_t rowb = (uint32_t)S2 * 4u;
+        const uint32_t plane = (uint32_t)S1 * rowb;
+        const uint32_t base = (uint32_t)s0.j0 * plane + (uint32_t)s1.j0 * rowb + (uint32_t)s2.j0 * 4u;
+        if (base != key) {
+            key = base;
+            const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, plane * (uint32_t)S0);
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 3; ++b)
-                r.q[3 * a + b] = __builtin_amdgcn_raw_buffer_load_b96(rs, base + a * plane + b * rowb, 0, 0);
+                for (int b = 0; b < 3; ++b)
+                    r.q[3 * a + b] = __builtin_amdgcn_raw_buffer_load_b96(rs, base + a * plane + b * rowb, 0, 0);
+        }
+    } else {
+        const uint32_t rowb = (uint32_t)S1 * 4u;
+        const uint32_t base = (uint32_t)s0.j0 * rowb + (uint32_t)s1.j0 * 4u;
+        if (base != key) {
+            key = base;
+            const __amdgpu_buffer_rsrc_t rs = face_rsrc(fld, rowb * (uint32_t)S0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) r.q[a] = __builtin_amdgcn_raw_buffer_load_b96(rs, base + a * rowb, 0, 0);
+        }
     }
 }
 
-// interp_mac (3D) with the rows taken from `cache` where the stencil did not
+// interp_mac with the rows taken from `cache` where the stencil did not
 // move; the contraction is combine_rows's, so the result is bit-identical.
-template <bool GRAD>
+template <int DIM, bool GRAD>
 __device__ __forceinline__ void interp_mac_cached(const float* __restrict__ ux, const float* __restrict__ uy,
                                                   const float* __restrict__ uz, const GridDev& g, const float p[3],
-                                                  float u[3], float G[3][3], RowCache<3>& cache) {
-    MacStencil<3, GRAD> st;
+                                                  float u[3], float G[3][3], RowCache<DIM>& cache) {
+    MacStencil<DIM, GRAD> st;
     st.build(g, p);
-    load_rows_cached<3, 0, GRAD>(ux, g, st, cache.r[0], cache.key[0]);
-    load_rows_cached<3, 1, GRAD>(uy, g, st, cache.r[1], cache.key[1]);
-    load_rows_cached<3, 2, GRAD>(uz, g, st, cache.r[2], cache.key[2]);
-    combine_rows<3, 0, GRAD>(g, st, cache.r[0], u[0], G[0]);
-    combine_rows<3, 1, GRAD>(g, st, cache.r[1], u[1], G[1]);
-    combine_rows<3, 2, GRAD>(g, st, cache.r[2], u[2], G[2]);
+    load_rows_cached<DIM, 0, GRAD>(ux, g, st, cache.r[0], cache.key[0]);
+    load_rows_cached<DIM, 1, GRAD>(uy, g, st, cache.r[1], cache.key[1]);
+    if constexpr (DIM == 3) load_rows_cached<3, 2, GRAD>(uz, g, st, cache.r[2], cache.key[2]);
+    combine_rows<DIM, 0, GRAD>(g, st, cache.r[0], u[0], G[0]);
+    combine_rows<DIM, 1, GRAD>(g, st, cache.r[1], u[1], G[1]);
+    if constexpr (DIM == 3) {
+        combine_rows<3, 2, GRAD>(g, st, cache.r[2], u[2], G[2]);
+    } else {
+        u[2] = 0.f;
+        G[2][0] = G[2][1] = G[2][2] = 0.f;
+    }
 }
 
 // ---- P2G particle record ---------------------------------------------------
